@@ -1,0 +1,912 @@
+"""FP64 CPU oracle for the SCF evaluation path -- TEST INFRASTRUCTURE ONLY.
+
+This module is the checker, never the product: only tests/, the package's
+smoke() and bench.py's cpu_baseline leg may import it.  It restates the
+reference's Julia algorithm with numpy.fft (pocketfft; the reference uses
+FFTW through FFTW.jl -- same unnormalised forward / 1/N backward
+convention), following the reference files line by line, quirks included:
+
+  objective_dft           src/iDFT/evaluation/objective/objective_dft.jl:17-30
+  get_onebody_potential!  src/iDFT/evaluation/objective/functions/eval_spatial_density.jl:138-149
+  child propagators       src/iDFT/coordinate_system/Cartesian/species_model/freely_jointed/child_propagators.jl:1-193
+  parent propagators      .../freely_jointed/parent_propagators.jl:1-153
+  handle_fixed            .../freely_jointed/handle_fixed.jl:1-120
+  accumulate_densities!   .../freely_jointed/accumulate_densities.jl:95-184
+  update_spatial_structures!  src/iDFT/evaluation/objective/functions/update_spatial_structures.jl:1-34
+  rho_hat                 src/iDFT/coordinate_system/Cartesian/fourier_transforms/calc_rho_hat.jl:17-97
+  weights                 .../fourier_transforms/{delta_func,step_func,vectored_delta_func,coul_func_fft,process_propagators}.jl
+  filter                  src/iDFT/coordinate_system/Cartesian/utilities/apply_filter.jl:15-41
+  mfmt / TPT1mfmt         src/iDFT/functionals/excluded_volume/{mfmt,TPT1mfmt}.jl,
+                          src/iLST/free_energy/excluded_volume/{mfmt,TPT1mfmt}.jl
+  msa / TPT1msa           src/iDFT/functionals/electrostaticcorrelations/{msa,TPT1msa}.jl,
+                          src/iLST/free_energy/electrostaticcorrelations/{msa,TPT1msa}.jl
+  eval_mu_functionals     src/iDFT/evaluation/objective/functions/eval_mu_functional.jl:1-52
+  Poisson + neutrality    src/iDFT/coulomb/models/coulomb.jl:84-125, src/iDFT/coulomb/neutrality_shift.jl:1-99
+  Anderson mixing         src/NonlinearSolvers/fixed_point/AndersonMixing.jl:56-169
+  grand potential         src/iDFT/evaluation/analysis/grand_potential.jl:4-28, num_species.jl:1-38
+  ionization fractions    src/iDFT/evaluation/analysis/degree_of_dissociation.jl:1-78
+  contact value theorem   src/iDFT/evaluation/analysis/contact_value_theorem.jl:1-20,
+                          .../external_fields/{hard_walls.jl:49-79, charged_walls.jl:60-81}
+
+Deliberate differences (documented, not silent):
+  * the Anderson least squares uses numpy lstsq (exact minimum-norm LS)
+    where the reference calls IterativeSolvers.lsqr! warm-started from the
+    previous lambda; only iteration paths differ, not the fixed point;
+  * the White-Bear cubic term uses its n3 -> 0 limit (0) when n3 == 0
+    exactly, where the reference would evaluate 0*0*Inf = NaN.
+
+Parity status of every function is listed in the test-suite header
+(tests/test_oracle_pins.py); all functions here are pinned by closed forms,
+invariants or brute force there.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import List, Optional
+
+import numpy as np
+
+PI = math.pi
+
+
+# ---------------------------------------------------------------- weights
+
+def kgrid(L: int, dz: float) -> np.ndarray:
+    """Signed wavenumbers of the reference's weight builders: m <= L//2 -> +, else m - L."""
+    m = np.arange(L)
+    n = np.where(m <= L // 2, m, m - L)
+    return 2.0 * PI * n / (L * dz)
+
+
+def w_delta(k: np.ndarray, b: float) -> np.ndarray:
+    a = np.abs(k)
+    out = np.empty_like(k)
+    nz = a != 0
+    out[nz] = 4.0 * PI * b * np.sin(b * a[nz]) / a[nz]
+    out[~nz] = 4.0 * PI * b ** 2
+    return out
+
+
+def w_step(k: np.ndarray, b: float) -> np.ndarray:
+    a = np.abs(k)
+    out = np.empty_like(k)
+    nz = a != 0
+    out[nz] = 4.0 * PI * (np.sin(b * a[nz]) - b * a[nz] * np.cos(b * a[nz])) / a[nz] ** 3
+    out[~nz] = 4.0 / 3.0 * PI * b ** 3
+    return out
+
+
+def w_vector(k: np.ndarray, b: float) -> np.ndarray:
+    a = np.abs(k)
+    out = np.zeros_like(k)
+    nz = a != 0
+    out[nz] = -4.0 * PI * (np.sin(b * a[nz]) - b * a[nz] * np.cos(b * a[nz])) / a[nz] ** 3 * k[nz]
+    return out
+
+
+def w_coulomb(k: np.ndarray, dz: float, lB: float) -> np.ndarray:
+    out = np.zeros_like(k)
+    nz = np.arange(len(k)) != 0
+    out[nz] = 2.0 * PI * lB / ((1.0 - np.cos(k[nz] * dz)) / dz ** 2)
+    return out
+
+
+def spectral_filter(k: np.ndarray, dz: float) -> np.ndarray:
+    kc = 0.5 * (PI / dz)
+    return np.exp(-(np.abs(k) / kc) ** 4)
+
+
+def mirror(f: np.ndarray, L: int) -> np.ndarray:
+    """Even extension onto the calculation domain (apply_mirroring.jl:1-23)."""
+    NP = f.shape[0]
+    out = np.zeros((L,) + f.shape[1:], dtype=np.result_type(f, np.float64))
+    out[:NP] = f
+    out[L - 1 - np.arange(NP)] = f
+    return out
+
+
+# ---------------------------------------------------------------- pointwise
+
+def gamma_msa_vec(rho: np.ndarray, sig: np.ndarray, Z: np.ndarray, lB: float):
+    """Gamma_msa per grid point (src/iLST/free_energy/electrostaticcorrelations/msa.jl:181-230).
+    rho: [P, NB].  Same start, damping, tolerance and iteration cap per point."""
+    tol = 1e-10
+    P = rho.shape[0]
+    s_rz = (rho * Z * Z).sum(1)
+    kap = np.sqrt(4.0 * PI * lB * s_rz)
+    G = np.zeros(P)
+    eta = np.zeros(P)
+    H = np.full(P, 2.0 / PI)
+    small = kap < tol
+    G[small] = kap[small] / 2.0
+    act = ~small
+    if act.any():
+        r = rho[act]
+        seff = (r * sig).sum(1) / r.sum(1)
+        x = seff * np.sqrt(4.0 * PI * lB * (r * Z * Z).sum(1))
+        g = (np.sqrt(1.0 + 2.0 * x) - 1.0) / (2.0 * seff)
+        term2 = 2.0 / PI * (1.0 - PI / 6.0 * (r * sig ** 3).sum(1))
+        gf = g.copy()
+        ef = np.zeros_like(g)
+        hf = np.zeros_like(g)
+        live = np.ones(len(g), dtype=bool)
+        it = 0
+        while live.any():
+            rl = r[live]
+            gl = g[live]
+            opg = 1.0 + gl[:, None] * sig[None, :]
+            Hl = (rl * sig ** 3 / opg).sum(1) + term2[live]
+            el = (rl * Z * sig / opg).sum(1) / Hl
+            q = Z[None, :] - el[:, None] * sig[None, :] ** 2
+            g1 = np.sqrt(PI * lB * (rl * q * q / opg / opg).sum(1))
+            gn = 0.9 * gl + 0.1 * g1
+            it += 1
+            stop = (np.abs(g1 - gl) < tol) | (it >= 1000)
+            idx = np.nonzero(live)[0]
+            g[idx] = gn
+            ef[idx] = el
+            hf[idx] = Hl
+            gf[idx] = gn
+            live[idx[stop]] = False
+        G[act] = gf
+        eta[act] = ef
+        H[act] = hf
+    return G, eta, H
+
+
+def msa_deriv_vec(rho, sig, Z, lB, GEH=None):
+    G, eta, H = gamma_msa_vec(rho, sig, Z, lB) if GEH is None else GEH
+    opg = 1.0 + G[:, None] * sig[None, :]
+    ustar = -PI / 6.0 * (rho * sig * sig * ((Z - eta[:, None] * sig * sig) / opg + Z)).sum(1)
+    e = eta[:, None]
+    return -lB * (G[:, None] * Z ** 2 / opg + e * sig * ((2.0 * Z - e * sig ** 2) / opg + e * sig ** 2 / 3.0)
+                  + Z * ustar[:, None])
+
+
+def tpt1msa_deriv_vec(rho_bonds, bond_types, rho, sig, Z, lB):
+    """compute_TPT1msa_deriv! per point; rho_bonds [P, 2, MB]."""
+    GEH = gamma_msa_vec(rho, sig, Z, lB)
+    dphi = msa_deriv_vec(rho, sig, Z, lB, GEH)
+    G, eta, H = GEH
+    P, MB = rho.shape[0], len(bond_types)
+    lng = np.zeros((P, MB))
+    zero = G == 0.0
+    for u, (i, j) in enumerate(bond_types):
+        sij = (sig[i] + sig[j]) / 2.0
+        Xi_ = (Z[i] - eta * sig[i] ** 2) / (1.0 + G * sig[i])
+        Xj_ = (Z[j] - eta * sig[j] ** 2) / (1.0 + G * sig[j])
+        lng[:, u] = np.where(zero, -lB / sij * Z[i] * Z[j], -lB / sij * Xi_ * Xj_)
+    nz = ~zero
+    if nz.any():
+        r, g, e, h = rho[nz], G[nz], eta[nz], H[nz]
+        opg = 1.0 + g[:, None] * sig[None, :]
+        q = Z[None, :] - e[:, None] * sig[None, :] ** 2
+        dfG = -2.0 * PI * lB * (r * sig * q * q / opg ** 3).sum(1)
+        dfe = -2.0 * PI * lB * (r * sig * sig * q / opg ** 2).sum(1)
+        deG = -1.0 / h * (r * Z * sig * sig / opg ** 2).sum(1)
+        deH = -e / h
+        dHG = -(r * sig ** 4 / opg ** 2).sum(1)
+        detaG = deG + deH * dHG
+        denom = 1.0 / (2.0 * g - dfe * detaG - dfG)
+        rb = rho_bonds[nz]
+        add = np.zeros_like(r)
+        for w in range(rho.shape[1]):
+            ig = 1.0 / (1.0 + g * sig[w])
+            Xw = (Z[w] - e * sig[w] ** 2) * ig
+            dfr = PI * lB * Xw ** 2
+            der = sig[w] / h * Z[w] * ig
+            dHr = sig[w] ** 3 * (ig - 1.0 / 3.0)
+            dGr = (dfr + dfe * (der + dfe * deH * dHr)) * denom
+            detr = detaG * dGr
+            for u, (i, j) in enumerate(bond_types):
+                sij = (sig[i] + sig[j]) / 2.0
+                igi = 1.0 / (1.0 + g * sig[i])
+                Xi_ = (Z[i] - e * sig[i] ** 2) * igi
+                dXi = -(sig[i] ** 2 * igi * detr + sig[i] * Xi_ * igi * dGr)
+                igj = 1.0 / (1.0 + g * sig[j])
+                Xj_ = (Z[j] - e * sig[j] ** 2) * igj
+                dXj = -(sig[j] ** 2 * igj * detr + sig[j] * Xj_ * igj * dGr)
+                add[:, w] += (rb[:, 0, u] + rb[:, 1, u]) / 2.0 * lB / sij * (dXi * Xj_ + Xi_ * dXj)
+        dphi[nz] += add
+    return dphi, lng, GEH
+
+
+def free_energy_msa_vec(rho, sig, Z, lB, GEH=None):
+    G, eta, _ = gamma_msa_vec(rho, sig, Z, lB) if GEH is None else GEH
+    opg = 1.0 + G[:, None] * sig[None, :]
+    return G ** 3 / (3.0 * PI) - lB * (rho * Z * (Z * G[:, None] + eta[:, None] * sig) / opg).sum(1)
+
+
+def free_energy_tpt1msa_vec(rho_bonds, bond_types, rho, sig, Z, lB):
+    GEH = gamma_msa_vec(rho, sig, Z, lB)
+    f = free_energy_msa_vec(rho, sig, Z, lB, GEH)
+    G, eta, _ = GEH
+    for u, (i, j) in enumerate(bond_types):
+        sij = (sig[i] + sig[j]) / 2.0
+        Xi_ = (Z[i] - eta * sig[i] ** 2) / (1.0 + G * sig[i])
+        Xj_ = (Z[j] - eta * sig[j] ** 2) / (1.0 + G * sig[j])
+        f = f - 0.5 * (rho_bonds[:, 0, u] + rho_bonds[:, 1, u]) * (-lB / sij * Xi_ * Xj_)
+    return f
+
+
+def fmt_phi_vec(n0, n1, n2, n3, nV1, nV2):
+    """free_energy_mfmt (inhomogeneous) per point; cubic term at n3 == 0 takes its limit 0."""
+    om = 1.0 - n3
+    lg = np.log(om)
+    dot12 = nV1 * nV2
+    dot22 = nV2 * nV2
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inv_den = 1.0 / (36.0 * PI * n3 ** 2 * om ** 2)
+        cubic = (n2 ** 3 - 3.0 * n2 * dot22) * (n3 + om ** 2 * lg) * inv_den
+    cubic = np.where(n3 == 0.0, 0.0, cubic)
+    return -n0 * lg + (n1 * n2 - dot12) / om + cubic
+
+
+def fmt_deriv_vec(n0, n1, n2, n3, nV1, nV2):
+    """compute_mfmt_deriv! per point, 1-D (one vector component); returns rows 1-4, 5, 8."""
+    om = 1.0 - n3
+    lg = np.log(om)
+    i1 = 1.0 / om
+    i3 = 1.0 / n3
+    c = 1.0 / (36.0 * PI)
+    dot12 = nV1 * nV2
+    dot22 = nV2 * nV2
+    d1 = -lg
+    d2 = n2 * i1
+    d3 = n1 * i1 + (n2 ** 2 - dot22) * 3.0 * c * i3 * (lg * i3 + i1 ** 2)
+    d4 = (n0 * i1 + (n1 * n2 - dot12) * i1 ** 2
+          - (lg * i3 ** 3 * 2.0 * c + i3 ** 2 * i1 * c + (1.0 - 3.0 * n3) * i3 ** 2 * c * i1 ** 3)
+          * (n2 ** 3 - 3.0 * n2 * dot22))
+    d5 = -nV2 * i1
+    d8 = -nV1 * i1 - (lg * i3 + i1 ** 2) * n2 * nV2 * i3 * 6.0 * c
+    return d1, d2, d3, d4, d5, d8
+
+
+# ---------------------------------------------------------------- oracle
+
+@dataclass
+class ObjectiveResult:
+    mu_ex_K: np.ndarray
+    lng_K: np.ndarray
+    Psi: np.ndarray
+    PsiC: float
+
+
+class OracleDFT:
+    """Holds one state point (softdft_amd.problem.StatePoint) and the fields of the last evaluation."""
+
+    def __init__(self, sp):
+        self.sp = sp
+        ms = sp.ms
+        self.ms = ms
+        self.NP = sp.NP
+        self.L = 2 * sp.NP
+        self.dz = sp.dz
+        k = kgrid(self.L, self.dz)
+        self.k = k
+        sig = ms.diameters
+        NB, MB = ms.NB, ms.MB
+        self.w2 = np.stack([w_delta(k, s / 2.0) for s in sig], 1)          # [L, NB]
+        self.w3 = np.stack([w_step(k, s / 2.0) for s in sig], 1)
+        self.wV = np.stack([w_vector(k, s / 2.0) for s in sig], 1)
+        self.wmsa = np.stack([w_delta(k, s / 2.0) / (4.0 * PI * (s / 2.0) ** 2) for s in sig], 1)
+        filt = spectral_filter(k, self.dz)
+        self.wbond = np.zeros((self.L, MB))
+        self.wbond_filt = np.zeros((self.L, MB))
+        for u, (i, j) in enumerate(ms.bond_types):
+            sij = (sig[i] + sig[j]) / 2.0
+            self.wbond[:, u] = w_delta(k, sij) / (4.0 * PI * sij ** 2)
+            self.wbond_filt[:, u] = self.wbond[:, u] * filt
+        self.wcoul = w_coulomb(k, self.dz, ms.bjerrum)
+        self.surf_hat = np.fft.fft(sp.surf)
+        self.mu_species = sp.mu_species.copy()
+        self.fix_lambda = [1.0] * len(ms.species)
+        self.rho_seg = None
+        self.rho_bonds = None
+        self.rho_beads = None
+        self.rho_beads_hat = None
+        self.rho_bonds_hat = None
+        self.n_objective = 0
+
+    # -------------------------------------------------------- convolution
+    def _prop_conv(self, f: np.ndarray, bond: int) -> np.ndarray:
+        """mirror -> FFT -> filter -> x w_bond -> IFFT -> |Re| on the primary half."""
+        g = np.fft.ifft(np.fft.fft(mirror(f, self.L)) * self.wbond_filt[:, bond])
+        return np.abs(g.real[:self.NP])
+
+    # -------------------------------------------------------- densities
+    def _species_density(self, u, lam, lng, rho_seg_u, rho_bonds):
+        ms, sp = self.ms, self.ms.species[u]
+        NP, dz = self.NP, self.dz
+        fam = sp.state_family
+        N = sp.n_seg
+        S = max(len(f) for f in fam)
+        fx = self.sp.fixed[u]
+        dmu = ms.delta_muH
+        normative = 1.0
+        for level in reversed(sp.topology.levels):
+            for p in level:
+                normative *= sum(math.exp(-dmu[s]) for s in fam[p]) ** (1.0 / N)
+        trapez = self.sp.trapez
+        Kidx = np.arange(NP)
+
+        if sp.model == "monomerbead":
+            if fx.any:
+                norm = fx.density
+                self.mu_species[u] = 0.0
+                p = int(np.nonzero(fx.segments)[0][0])
+                if p != 0:
+                    raise IndexError("fixed monomerbead coordinate lookup (coordinates[1][parent])")
+                Kf = int(fx.coords[0])
+                # monomerbead/handle_fixed.jl:38-42: exp(-mu_ex - Ext - delta_muH), no Psi term
+                lamb = sum(math.exp(-(self._muex[Kf, s] + self.sp.Ext[Kf, s] + dmu[s])) for s in fam[p])
+            else:
+                lamb = 1.0
+                norm = 1.0
+            self.fix_lambda[u] = lamb
+            if not math.isfinite(lamb) or lamb <= 0:
+                raise FloatingPointError(f"Normalization error for species {u}")
+            norm /= lamb
+            for p in range(N):
+                for a, si in enumerate(fam[p]):
+                    rho_seg_u[:, a, p] += norm * np.exp(self.mu_species[u] - lam[:, si])
+            return
+
+        ch, pa, lv = sp.topology.children, sp.topology.parents, sp.topology.levels
+        C = max(max((len(c) for c in ch), default=0), 1)
+        sig = ms.diameters
+
+        def coord(seg):
+            return int(fx.coords[seg])
+
+        def dist2(K1, K):
+            return ((K1 - K) * dz) ** 2
+
+        # ---- child propagators
+        gC = np.zeros((NP, S, C, S, N))
+        for level in reversed(lv):
+            for p in level:
+                for a, si in enumerate(fam[p]):
+                    if fx.segments[p]:
+                        gC[coord(p), 0, 0, a, p] = 1.0
+                    else:
+                        gC[:, 0, 0, a, p] = 1.0
+                    for ci, c in enumerate(ch[p]):
+                        for b, sj in enumerate(fam[c]):
+                            bond, pi, pj = ms.bond_index(si, sj)
+                            Dij = (sig[si] + sig[sj]) / 2.0
+                            renorm = 2.0 * Dij
+                            Dij2 = Dij ** 2
+                            f = np.exp(-lam[:, sj] + lng[:, pj, bond] / 2.0)
+                            for gci, _ in enumerate(ch[c]):
+                                f = f * gC[:, :, gci, b, c].sum(1)
+                            if fx.segments[p]:
+                                K = coord(p)
+                                if fx.segments[c]:
+                                    Kc = coord(c)
+                                    gC[K, b, ci, a, p] = (abs(f[Kc]) * math.exp(lng[K, pi, bond] / 2.0)
+                                                          / renorm / dz / normative)
+                                else:
+                                    acc = 0.0
+                                    d2 = dist2(Kidx, K)
+                                    for K1 in np.nonzero(d2 <= Dij2)[0]:
+                                        t = abs(f[K1]) * math.exp(lng[K, pi, bond] / 2.0) * dz * trapez[K1, sj] / renorm
+                                        if abs(d2[K1] - Dij2) <= 1e-6:
+                                            t /= 2.0
+                                        acc += t / normative
+                                    gC[K, b, ci, a, p] = acc
+                            elif fx.segments[c]:
+                                Kc = coord(c)
+                                val = abs(f[Kc])
+                                d2 = dist2(Kidx, Kc)
+                                m = d2 <= Dij2
+                                g = val * np.exp(lng[:, pi, bond] / 2.0) / renorm / normative
+                                g = np.where(np.abs(d2 - Dij2) <= 1e-6, g / 2.0, g)
+                                gC[m, b, ci, a, p] = g[m]
+                            else:
+                                g = self._prop_conv(f * trapez[:, sj], bond)
+                                gC[:, b, ci, a, p] = g * np.exp(lng[:, pi, bond] / 2.0) / normative
+                for seg in np.nonzero(fx.segments)[0]:
+                    md2 = sp.max_distance[p, seg] ** 2
+                    far = dist2(Kidx, coord(seg)) > md2
+                    gC[far, :, :, :, p] = 0.0
+
+        # ---- parent propagators
+        gP = np.zeros((NP, S, S, N))
+        for level in lv:
+            for c in level:
+                for a, si in enumerate(fam[c]):
+                    if fx.segments[c]:
+                        gP[coord(c), 0, a, c] = 1.0
+                    else:
+                        gP[:, 0, a, c] = 1.0
+                    for p in pa[c]:
+                        for b, sj in enumerate(fam[p]):
+                            bond, pi, pj = ms.bond_index(si, sj)
+                            Dij = (sig[si] + sig[sj]) / 2.0
+                            Dij2 = Dij ** 2
+                            renorm = 2.0 * Dij
+                            f = np.exp(-lam[:, sj] + lng[:, pj, bond] / 2.0) * gP[:, :, b, p].sum(1)
+                            for oci, oc in enumerate(ch[p]):
+                                if oc != c:
+                                    # index order as written at parent_propagators.jl:150
+                                    f = f * gC[:, :, b, oci, p].sum(1)
+                            if fx.segments[p]:
+                                Kp = coord(p)
+                                val = abs(f[Kp])
+                                d2 = dist2(Kidx, Kp)
+                                m = d2 <= Dij2
+                                g = val * np.exp(lng[:, pi, bond] / 2.0) / renorm / normative
+                                g = np.where(np.abs(d2 - Dij2) <= 1e-6, g / 2.0, g)
+                                gP[m, b, a, c] = g[m]
+                            else:
+                                g = self._prop_conv(f * trapez[:, sj], bond)
+                                gP[:, b, a, c] = g * np.exp(lng[:, pi, bond] / 2.0) / normative
+                for seg in np.nonzero(fx.segments)[0]:
+                    md2 = sp.max_distance[c, seg] ** 2
+                    far = dist2(Kidx, coord(seg)) > md2
+                    gP[far, :, :, c] = 0.0
+
+        # ---- normalisation (handle_fixed)
+        if fx.any:
+            norm = fx.density
+            self.mu_species[u] = 0.0
+            lamb = 0.0
+            nf = np.nonzero(~fx.segments)[0]
+            assert len(nf), "No non-fixed segment available for lambda normalization."
+            p = int(nf[0])
+            for a, si in enumerate(fam[p]):
+                contr = np.exp(-lam[:, si]) * gP[:, :, a, p].sum(1)
+                if not ch[p]:
+                    if a >= C:
+                        raise IndexError("handle_fixed leaf index gC[K, :, idx_i, 1, parent] out of bounds")
+                    temp = gC[:, :, a, 0, p].sum(1)
+                else:
+                    temp = np.zeros(NP)
+                    for ci, c in enumerate(ch[p]):
+                        t1 = np.zeros(NP)
+                        for b, _ in enumerate(fam[c]):
+                            t2 = gC[:, b, ci, a, p].copy()
+                            for oci, oc in enumerate(ch[p]):
+                                if oc != c:
+                                    t2 = t2 * gC[:, :, oci, a, p].sum(1)
+                            t1 = t1 + t2
+                        temp = t1
+                lamb += float(np.sum(contr * temp * trapez[:, si] * dz))
+        else:
+            lamb = 1.0
+            norm = normative ** (N - 1)
+        if not math.isfinite(lamb) or lamb <= 0:
+            raise FloatingPointError(f"Normalization error: lambda = {lamb} for species {u}")
+        norm *= 1.0 / lamb
+        lamb /= normative ** (N - 1)
+        self.fix_lambda[u] = lamb
+
+        # ---- densities
+        mu_u = self.mu_species[u]
+        for p in range(N):
+            for a, si in enumerate(fam[p]):
+                contr = norm * np.exp(mu_u - lam[:, si]) * gP[:, :, a, p].sum(1)
+                if not ch[p]:
+                    temp = gC[:, :, 0, a, p].sum(1)
+                else:
+                    temp = np.zeros(NP)
+                    for ci, c in enumerate(ch[p]):
+                        t1 = np.zeros(NP)
+                        for b, sj in enumerate(fam[c]):
+                            bond, _, _ = ms.bond_index(si, sj)
+                            t2 = gC[:, b, ci, a, p].copy()
+                            for oci, oc in enumerate(ch[p]):
+                                if oc != c:
+                                    t2 = t2 * gC[:, :, oci, a, p].sum(1)
+                            rho_bonds[:, 0, bond] += contr * t2
+                            t1 = t1 + t2
+                        temp = t1
+                rho_seg_u[:, a, p] += contr * temp
+        for c in range(N):
+            if not pa[c]:
+                continue
+            for a, si in enumerate(fam[c]):
+                contr = norm * np.exp(mu_u - lam[:, si])
+                for gci, _ in enumerate(ch[c]):
+                    contr = contr * gC[:, :, gci, a, c].sum(1)
+                for p in pa[c]:
+                    for b, sj in enumerate(fam[p]):
+                        bond, _, _ = ms.bond_index(si, sj)
+                        rho_bonds[:, 1, bond] += contr * gP[:, b, a, c]
+        self._gC = gC
+        self._gP = gP
+
+    # -------------------------------------------------------- functionals
+    def _wd_beads(self, w, absval=True):
+        out = np.fft.ifft(self.rho_beads_hat * w, axis=0).real
+        return np.abs(out) if absval else out
+
+    def _fmt_weighted(self):
+        ms = self.ms
+        sig = ms.diameters
+        n2j = self._wd_beads(self.w2)
+        n3j = self._wd_beads(self.w3)
+        nVj = np.fft.ifft(self.rho_beads_hat * (1j * self.wV), axis=0).real
+        return n2j, n3j, nVj
+
+    def _bond_weighted(self, w_by_bead):
+        ms = self.ms
+        out = np.zeros((self.L, 2, ms.MB))
+        for u, (i, j) in enumerate(ms.bond_types):
+            for idx, bt in ((0, i), (1, j)):
+                out[:, idx, u] = np.abs(np.fft.ifft(self.rho_bonds_hat[:, idx, u] * w_by_bead[:, bt]).real)
+        return out
+
+    def _tpt1mfmt_point(self, n2j, n3j, nVj, wtb, want_fe=False):
+        """Per point (all rows of the inputs) derivative or free energy of TPT1mfmt."""
+        ms = self.ms
+        sig = ms.diameters
+        n0i = n2j / (PI * sig ** 2)
+        n1i = n2j / (2.0 * PI * sig)
+        n2i = n2j.copy()
+        n3i = n3j.copy()
+        nV1 = (nVj / (2.0 * PI * sig)).sum(1)
+        nV2 = nVj.sum(1)
+        s3 = n3i.sum(1)
+        fac = np.where(s3 > 0.95, 0.95 / np.where(s3 > 0, s3, 1.0), 1.0)
+        n0i *= fac[:, None]
+        n1i *= fac[:, None]
+        n2i *= fac[:, None]
+        n3i *= fac[:, None]
+        nV1 = nV1 * fac
+        nV2 = nV2 * fac
+        n0, n1, n2, n3 = n0i.sum(1), n1i.sum(1), n2i.sum(1), n3i.sum(1)
+        if want_fe:
+            f = fmt_phi_vec(n0, n1, n2, n3, nV1, nV2)
+            with np.errstate(divide="ignore", invalid="ignore"):
+                Xi = 1.0 - nV2 * nV2 / n2 ** 2
+                i1 = 1.0 / (1.0 - n3)
+                t2 = n2 * Xi / 4.0 * i1 ** 2
+                t3 = n2 ** 2 * Xi / 72.0 * i1 ** 3
+                for u, (i, j) in enumerate(ms.bond_types):
+                    sij = (sig[i] + sig[j]) / 2
+                    tau = sig[i] * sig[j] / sij
+                    y = i1 + t2 * tau + t3 * tau ** 2
+                    f = f - 0.5 * (wtb[:, 0, u] + wtb[:, 1, u]) * np.log(y)
+            return f
+        live = s3 > 1e-10
+        P = n2j.shape[0]
+        d = np.zeros((P, 6))
+        lng = np.zeros((P, ms.MB))
+        if live.any():
+            n0l, n1l, n2l, n3l, v1, v2 = n0[live], n1[live], n2[live], n3[live], nV1[live], nV2[live]
+            d1, d2, d3, d4, d5, d8 = fmt_deriv_vec(n0l, n1l, n2l, n3l, v1, v2)
+            Xi = np.maximum(1.0 - v2 * v2 / n2l ** 2, 0.0)
+            i1 = 1.0 / (1.0 - n3l)
+            t2 = n2l * Xi / 4.0 * i1 ** 2
+            t3 = n2l ** 2 * Xi / 72.0 * i1 ** 3
+            wl = wtb[live]
+            for u, (i, j) in enumerate(ms.bond_types):
+                sij = (sig[i] + sig[j]) / 2
+                tau = sig[i] * sig[j] / sij
+                y = i1 + t2 * tau + t3 * tau ** 2
+                dn2 = (tau / 4.0 * i1 ** 2 + n2l * tau ** 2 * i1 ** 3 / 36.0) / y
+                dn3 = (i1 ** 2 + n2l * Xi / 2.0 * i1 ** 3 * tau + n2l ** 2 * Xi / 24.0 * i1 ** 4 * tau ** 2) / y
+                rb = wl[:, 0, u] + wl[:, 1, u]
+                d3 = d3 - 0.5 * rb * dn2
+                d4 = d4 - 0.5 * rb * dn3
+                dv = -v2 * (tau / (2.0 * n2l) * i1 ** 2 + tau ** 2 / 36.0 * i1 ** 3) / y
+                d8 = d8 - 0.5 * rb * dv
+                lng[live, u] = np.log(y)
+            d[live] = np.stack([d1, d2, d3, d4, d5, d8], 1)
+        return d, lng
+
+    def _mfmt_point(self, n2j, n3j, nVj, want_fe=False):
+        sig = self.ms.diameters
+        n0 = (n2j / (PI * sig ** 2)).sum(1)
+        n1 = (n2j / (2.0 * PI * sig)).sum(1)
+        n2 = n2j.sum(1)
+        n3 = n3j.sum(1)
+        nV1 = (nVj / (2.0 * PI * sig)).sum(1)
+        nV2 = nVj.sum(1)
+        fac = np.where(n3 > 0.95, 0.95 / np.where(n3 > 0, n3, 1.0), 1.0)
+        n0, n1, n2, n3, nV1, nV2 = n0 * fac, n1 * fac, n2 * fac, n3 * fac, nV1 * fac, nV2 * fac
+        if want_fe:
+            return fmt_phi_vec(n0, n1, n2, n3, nV1, nV2)
+        live = n3 > 1e-10
+        d = np.zeros((len(n0), 6))
+        if live.any():
+            d[live] = np.stack(fmt_deriv_vec(n0[live], n1[live], n2[live], n3[live], nV1[live], nV2[live]), 1)
+        return d
+
+    def _add_fmt_back(self, d, mu_ex_hat):
+        sig = self.ms.diameters
+        F = np.fft.fft
+        for j in range(self.ms.NB):
+            s = sig[j]
+            ch0 = d[:, 0] / (PI * s ** 2) + d[:, 1] / (2.0 * PI * s) + d[:, 2]
+            ch1 = d[:, 3]
+            chv = d[:, 4] / (2.0 * PI * s) + d[:, 5]
+            mu_ex_hat[:, j] += F(ch0) * self.w2[:, j]
+            mu_ex_hat[:, j] += F(ch1) * self.w3[:, j]
+            mu_ex_hat[:, j] += F(chv) * (-1j * self.wV[:, j])
+
+    def _add_lng_back(self, lngP, w_by_bead, lng_hat):
+        for u, (i, j) in enumerate(self.ms.bond_types):
+            for idx, bt in ((0, i), (1, j)):
+                lng_hat[:, idx, u] += np.fft.fft(lngP[:, u]) * w_by_bead[:, bt]
+
+    def _functionals(self):
+        ms = self.ms
+        NB, MB, L = ms.NB, ms.MB, self.L
+        mu_ex_hat = np.zeros((L, NB), complex)
+        lng_hat = np.zeros((L, 2, MB), complex)
+        sig, Z, lB = ms.diameters, ms.valences, ms.bjerrum
+        wn2b = self.w2 / sig[None, :] ** 2 / PI
+        for fn in self.sp.functionals:
+            if fn in ("mfmt", "TPT1mfmt"):
+                n2j, n3j, nVj = self._fmt_weighted()
+                if fn == "mfmt":
+                    d = self._mfmt_point(n2j, n3j, nVj)
+                    self._add_fmt_back(d, mu_ex_hat)
+                else:
+                    wtb = self._bond_weighted(wn2b)
+                    d, lngP = self._tpt1mfmt_point(n2j, n3j, nVj, wtb)
+                    self._add_fmt_back(d, mu_ex_hat)
+                    self._add_lng_back(lngP, wn2b, lng_hat)
+            elif fn in ("msa", "TPT1msa"):
+                rb = self._wd_beads(self.wmsa)
+                if fn == "msa":
+                    d = msa_deriv_vec(rb, sig, Z, lB)
+                else:
+                    wtb = self._bond_weighted(self.wmsa)
+                    d, lngP, _ = tpt1msa_deriv_vec(wtb, ms.bond_types, rb, sig, Z, lB)
+                    self._add_lng_back(lngP, self.wmsa, lng_hat)
+                for j in range(NB):
+                    mu_ex_hat[:, j] += np.fft.fft(d[:, j]) * self.wmsa[:, j]
+        mu = np.fft.ifft(mu_ex_hat, axis=0).real[:self.NP]
+        lng = np.fft.ifft(lng_hat, axis=0).real[:self.NP]
+        return mu, lng
+
+    # -------------------------------------------------------- Coulomb
+    def _coulomb(self, PsiC_old):
+        ms = self.ms
+        f = self.surf_hat.copy()
+        for j in range(ms.NB):
+            f = f + self.rho_beads_hat[:, j] * ms.valences[j]
+        Psi = np.fft.ifft(f * self.wcoul).real[:self.NP]
+        # neutrality shift
+        Qtot = self.sp.total_charge
+        nsp = len(ms.species)
+        Num = np.zeros(nsp)
+        qs = np.zeros(nsp)
+        for u, sp in enumerate(ms.species):
+            fx = self.sp.fixed[u]
+            for seg in range(sp.n_seg):
+                for a, si in enumerate(sp.state_family[seg]):
+                    if fx.segments[seg]:
+                        if seg != 0:
+                            raise IndexError("fixed coordinate lookup coordinates[1][seg] out of bounds")
+                        num = self.rho_seg[u][int(fx.coords[0]), a, seg]
+                    else:
+                        num = float(np.sum(self.rho_seg[u][:, a, seg] * self.sp.trapez[:, si] * self.dz))
+                    Num[u] += num
+                    qs[u] += num * ms.valences[si]
+            Num[u] /= sp.n_seg
+            qs[u] = 0.0 if Num[u] <= 0.0 else qs[u] / Num[u]
+        for u, sp in enumerate(ms.species):
+            multi = any(len(f) > 1 for f in sp.state_family)
+            if multi or self.sp.fixed[u].any or sp.evaluation == "simulation":
+                Qtot += Num[u] * qs[u]
+                qs[u] = 0.0
+        err = abs(Qtot + float(np.sum(Num * qs)))
+        psi1 = 0.0
+        cst = math.exp(-psi1)
+        it = 1
+        while err > 1e-10:
+            fx = Qtot
+            fx1 = 0.0
+            for i in range(nsp):
+                t = (cst ** qs[i]) * Num[i] * qs[i]
+                fx += t
+                fx1 -= t * qs[i]
+            err = abs(fx)
+            if abs(fx1) < 1e-10:
+                break
+            dl = max(1.0, abs(fx / fx1))
+            psi1 -= fx / (fx1 * dl)
+            cst = math.exp(-psi1)
+            it += 1
+            if it > 1000:
+                break
+        return Psi, PsiC_old + psi1
+
+    # -------------------------------------------------------- objective
+    def objective(self, x: np.ndarray) -> np.ndarray:
+        sp, ms = self.sp, self.ms
+        NP, NB, MB, L = self.NP, ms.NB, ms.MB, self.L
+        a = NP * NB
+        b = a + NP * 2 * MB
+        mu_ex = x[:a].reshape((NP, NB), order="F")
+        lng = x[a:b].reshape((NP, 2, MB), order="F")
+        Psi = x[b:b + NP]
+        PsiC = float(x[b + NP])
+        self._muex = mu_ex
+        lam = mu_ex + (Psi + PsiC)[:, None] * ms.valences[None, :] + sp.Ext + ms.delta_muH[None, :]
+        self.lambda_K = lam
+        rho_bonds = np.zeros((NP, 2, MB))
+        rho_seg = []
+        for u, s in enumerate(ms.species):
+            S = max(len(f) for f in s.state_family)
+            rs = np.zeros((NP, S, s.n_seg))
+            if s.evaluation != "simulation":
+                self._species_density(u, lam, lng, rs, rho_bonds)
+            rho_seg.append(rs)
+        # update_spatial_structures!
+        beads = np.zeros((NP, NB))
+        for u, s in enumerate(ms.species):
+            fx = sp.fixed[u]
+            for seg in range(s.n_seg):
+                for a_, si in enumerate(s.state_family[seg]):
+                    if fx.segments[seg]:
+                        if seg != 0:
+                            raise IndexError("fixed coordinate lookup coordinates[1][seg] out of bounds")
+                        K = int(fx.coords[0])
+                        beads[K, si] += rho_seg[u][K, a_, seg] / self.dz / sp.trapez[K, si]
+                    else:
+                        beads[:, si] += rho_seg[u][:, a_, seg]
+        self.rho_seg, self.rho_bonds, self.rho_beads = rho_seg, rho_bonds, beads
+        # rho_hat
+        self.rho_beads_hat = np.fft.fft(mirror(beads * sp.trapez, L), axis=0)
+        rbh = np.zeros((L, 2, MB), complex)
+        for u, (i, j) in enumerate(ms.bond_types):
+            for idx, bt in ((0, i), (1, j)):
+                rbh[:, idx, u] = np.fft.fft(mirror(rho_bonds[:, idx, u] * sp.trapez[:, bt], L))
+        self.rho_bonds_hat = rbh
+        mu_new, lng_new = self._functionals()
+        Psi_new, PsiC_new = self._coulomb(PsiC)
+        self.n_objective += 1
+        return np.concatenate([mu_new.ravel(order="F"), lng_new.ravel(order="F"), Psi_new, [PsiC_new]])
+
+    # -------------------------------------------------------- observables
+    def grand_potential(self, x: np.ndarray) -> float:
+        """Omega with rho from the last objective call and mu_ex/lng from x (grand_potential.jl:4-28)."""
+        sp, ms = self.sp, self.ms
+        NP, NB, MB = self.NP, ms.NB, ms.MB
+        a = NP * NB
+        b = a + NP * 2 * MB
+        mu_ex = x[:a].reshape((NP, NB), order="F")
+        lng = x[a:b].reshape((NP, 2, MB), order="F")
+        sig, Z, lB = ms.diameters, ms.valences, ms.bjerrum
+        wn2b = self.w2 / sig[None, :] ** 2 / PI
+        om = 0.0
+        for fn in sp.functionals:
+            if fn in ("mfmt", "TPT1mfmt"):
+                n2j, n3j, nVj = self._fmt_weighted()
+                sl = slice(0, NP)
+                if fn == "mfmt":
+                    f = self._mfmt_point(n2j[sl], n3j[sl], nVj[sl], want_fe=True)
+                else:
+                    wtb = self._bond_weighted(wn2b)
+                    f = self._tpt1mfmt_point(n2j[sl], n3j[sl], nVj[sl], wtb[sl], want_fe=True)
+                om += float(np.sum(f))
+            elif fn in ("msa", "TPT1msa"):
+                rb = self._wd_beads(self.wmsa)[:NP]
+                if fn == "msa":
+                    f = free_energy_msa_vec(rb, sig, Z, lB)
+                else:
+                    wtb = self._bond_weighted(self.wmsa)[:NP]
+                    f = free_energy_tpt1msa_vec(wtb, ms.bond_types, rb, sig, Z, lB)
+                om += float(np.sum(f))
+        om -= float(np.sum(self.rho_beads * mu_ex * sp.trapez))
+        om += float(np.sum(self.rho_bonds * lng)) / 2.0
+        om *= self.dz
+        om -= float(np.sum(self.num_species()))
+        return om
+
+    def num_species(self) -> np.ndarray:
+        sp, ms = self.sp, self.ms
+        out = np.zeros(len(ms.species))
+        for u, s in enumerate(ms.species):
+            fx = sp.fixed[u]
+            for seg in range(s.n_seg):
+                for a, si in enumerate(s.state_family[seg]):
+                    if fx.segments[seg]:
+                        num = self.rho_seg[u][int(fx.coords[0]), a, seg]
+                    else:
+                        num = float(np.sum(self.rho_seg[u][:, a, seg] * sp.trapez[:, si] * self.dz))
+                    out[u] += num / s.n_seg
+        return out
+
+    def state_fractions(self) -> List[Optional[np.ndarray]]:
+        """Per species: average state fractions over multistate segments (raw grid sums)."""
+        out = []
+        for u, s in enumerate(self.ms.species):
+            if not any(len(f) > 1 for f in s.state_family):
+                out.append(None)
+                continue
+            S = max(len(f) for f in s.state_family)
+            acc = np.zeros(S)
+            cnt = np.zeros(S)
+            for seg in range(s.n_seg):
+                ns = len(s.state_family[seg])
+                if ns == 1:
+                    continue
+                v = self.rho_seg[u][:, :ns, seg].sum(0)
+                tot = v.sum()
+                nv = v / tot if tot > 0 else np.zeros(ns)
+                acc[:ns] += nv
+                cnt[:ns] += 1
+            out.append(np.where(cnt > 0, acc / np.maximum(cnt, 1), np.nan))
+        return out
+
+    def contact_pressure(self) -> np.ndarray:
+        """[left, right] contact-value-theorem contributions summed over wall fields."""
+        sp, ms = self.sp, self.ms
+        c = np.zeros(2)
+        if sp.hard_walls:
+            for j in range(ms.NB):
+                c[0] += self.rho_beads[sp.wall_lo[j], j]
+                c[1] += self.rho_beads[sp.wall_hi[j], j]
+        if sp.wall_charges is not None:
+            c[0] += -2.0 * PI * ms.bjerrum * sp.wall_charges[0] ** 2
+            c[1] += -2.0 * PI * ms.bjerrum * sp.wall_charges[1] ** 2
+        return c
+
+
+# ---------------------------------------------------------------- Anderson
+
+@dataclass
+class SolveResult:
+    x: np.ndarray
+    iters: int
+    err: float
+    history: List[float]
+
+
+def anderson_solve(F, x0: np.ndarray, numerics: dict, max_iters: Optional[int] = None,
+                   callback=None) -> SolveResult:
+    """solver_function(::AndersonMixing) (AndersonMixing.jl:56-169); F maps x -> F(x)."""
+    n = len(x0)
+    mi = int(numerics.get("max_iters", 100000)) if max_iters is None else max_iters
+    damping = float(numerics.get("damping", 0.5))
+    mix_max = float(numerics.get("mixing_max", 1.0))
+    switch = float(numerics.get("switch_tole", 1e-3))
+    tole = float(numerics.get("tole", 1e-6))
+    m = int(numerics.get("anderm", 5))
+    X = np.zeros((n, m))
+    G = np.zeros((n, m))
+    x = x0.copy()
+    use_picard = True
+    err = 1.0
+    it = 0
+    hist = []
+    while it < mi:
+        it += 1
+        Fx = F(x)
+        col = (it - 1) % m
+        g = Fx - x
+        X[:, col] = x
+        G[:, col] = g
+        err = max(float(np.max(np.abs(g))), 1e-20)
+        hist.append(err)
+        amix = min(mix_max, damping / err)
+        if it < m or (err > switch and use_picard):
+            x = (1.0 - amix) * x + amix * Fx
+        else:
+            use_picard = False
+            cols = [j for j in range(m) if j != col]
+            dX = X[:, cols] - x[:, None]
+            dG = G[:, cols] - g[:, None]
+            lam, *_ = np.linalg.lstsq(dG, -g, rcond=None)
+            x = x + dX @ lam + amix * (g + dG @ lam)
+        if callback is not None:
+            callback(it, err)
+        if err < tole:
+            break
+    return SolveResult(x=x, iters=it, err=err, history=hist)
+
+
+def solve_state_point(sp, max_iters: Optional[int] = None, numerics: Optional[dict] = None):
+    """iDFT solve of one state point; returns (oracle, SolveResult, Omega)."""
+    o = OracleDFT(sp)
+    res = anderson_solve(o.objective, sp.x0(), numerics or sp.numerics, max_iters=max_iters)
+    omega = o.grand_potential(res.x)
+    return o, res, omega
