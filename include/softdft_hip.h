@@ -1,0 +1,167 @@
+/*
+ * libsoftdft_hip -- MI355X (gfx950) implementation of the SoftDFT iDFT
+ * self-consistent-field (SCF) evaluation behind a plain C ABI.
+ *
+ * A context holds a batch of independent state points that share one
+ * molecular system layout (bead types, bond types, species/segment topology,
+ * functional list) but may differ in grid length, delta_muH, mu_species,
+ * external fields and wall charges (pH / salt / slit-width sweeps).
+ *
+ * Conventions
+ *  - every entry point returns 0 on success or a negative SDFT_E* code; the
+ *    message of the last failure is available from sdft_last_error();
+ *  - the caller owns every host buffer; the library owns device memory;
+ *  - a context is bound to one HIP stream and is not re-entrant;
+ *  - the solution vector of one state point uses the reference's flatten
+ *    order (Julia column-major, first index fastest):
+ *        mu_ex_K (NP, NB) ++ lng_K (NP, 2, MB) ++ Psi (NP) ++ PsiC (1)
+ *    so n_p = NP*(NB + 2*MB + 1) + 1; batched buffers concatenate points.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *  - sdft_objective  <- objective_dft(problem_data) -> (mu_ex_K, lng_K, Psi, PsiC)
+ *                       src/iDFT/evaluation/objective/objective_dft.jl:17-30
+ *  - sdft_solve      <- solver_dft(dft_system) + solver_function(::AndersonMixing, ...)
+ *                       src/iDFT/evaluation/objective/solver_dft.jl:5-25,
+ *                       src/NonlinearSolvers/fixed_point/AndersonMixing.jl:56-169
+ *  - sdft_observables <- grand_potential (src/iDFT/evaluation/analysis/grand_potential.jl:4-28),
+ *                       write_total_state_fractions (degree_of_dissociation.jl:1-78),
+ *                       contact_value_theorem (contact_value_theorem.jl:1-20)
+ *  - sdft_get_field  <- reads of fields.rho_K / fields.excess after a solve
+ *                       (src/iDFT/structures/{DensityFields,ExcessFields}.jl)
+ *  - sdft_create     <- setup_iDFT (src/iDFT/setup/setup_iDFT.jl:9-30): takes the
+ *                       host-built arrays (Ext, trapez, wall charge, grafting)
+ *                       and precomputes the k-space weights of
+ *                       src/iDFT/coordinate_system/Cartesian/fourier_transforms/ (delta, step, vector, coulomb weights)
+ */
+#ifndef SOFTDFT_HIP_H
+#define SOFTDFT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDFT_ABI_VERSION 1
+
+/* error codes */
+#define SDFT_OK 0
+#define SDFT_E_ARG (-1)          /* bad descriptor / argument */
+#define SDFT_E_HIP (-2)          /* HIP runtime error */
+#define SDFT_E_NORMALIZATION (-3)/* non-finite or non-positive chain normalisation (handle_fixed.jl:110) */
+#define SDFT_E_NOCONV (-4)       /* max_iters reached for at least one state point */
+#define SDFT_E_UNSUPPORTED (-5)  /* topology / option outside this build */
+#define SDFT_E_STATE (-6)        /* call order violated (e.g. observables before solve) */
+
+/* functional flags (src/iDFT/functionals/Abstract_Types/functional_registry.jl:54-71) */
+#define SDFT_F_MFMT 1
+#define SDFT_F_TPT1MFMT 2
+#define SDFT_F_MSA 4
+#define SDFT_F_TPT1MSA 8
+
+/* species models */
+#define SDFT_MODEL_FREELY_JOINTED 0
+#define SDFT_MODEL_MONOMERBEAD 1
+
+/* precision of the objective evaluation */
+#define SDFT_PREC_FP64 64
+#define SDFT_PREC_FP32 32
+#define SDFT_PREC_MIXED 3264   /* FP32 until ||F(x)-x||_inf < switch, then FP64 */
+
+/* field ids for sdft_get_field */
+#define SDFT_FIELD_RHO_BEADS 1   /* (NP, NB) */
+#define SDFT_FIELD_RHO_BONDS 2   /* (NP, 2, MB) */
+#define SDFT_FIELD_X 3           /* current solution vector (n_p) */
+#define SDFT_FIELD_FX 4          /* last F(x) (n_p) */
+#define SDFT_FIELD_SEGMENT_SUMS 5 /* per species/segment/state: sum_K rho_seg (raw), then sum_K rho_seg*trapez*dz */
+
+typedef struct sdft_ctx sdft_ctx_t;
+
+/*
+ * Batch descriptor.  Per-point arrays are concatenated over points in point
+ * order; grid arrays use the point's own NP.  All indices are 0-based.
+ */
+typedef struct sdft_desc {
+    int32_t abi_version;       /* must equal SDFT_ABI_VERSION */
+    int32_t n_points;
+    int32_t NB;                /* bead (monomer state) types */
+    int32_t MB;                /* bond types */
+    int32_t n_species;
+    int32_t functionals;       /* OR of SDFT_F_* */
+    double bjerrum;            /* l_B in units of l_sc */
+    const double* diameters;   /* [NB] */
+    const double* valences;    /* [NB] */
+    const int32_t* bond_types; /* [MB][2], i <= j */
+
+    /* species: linear chains, segment 0 is the root (parse_topology.jl) */
+    const int32_t* species_nseg;    /* [n_species] */
+    const int32_t* species_model;   /* [n_species] SDFT_MODEL_* */
+    const int32_t* species_nstates; /* [n_species] max states per segment S_u */
+    const int32_t* seg_states;      /* concat over species of [nseg_u][S_u] bead types, -1 = unused */
+    const int32_t* species_fixed_seg; /* [n_species] grafted segment (0) or -1 */
+    const double* fixed_density;    /* [n_species] grafting density (used when fixed) */
+    const double* max_distance;     /* concat over species of [nseg_u] max distance to the grafted segment */
+
+    /* per point scalars */
+    const int32_t* NP;             /* [P] */
+    const double* dz;              /* [P] */
+    const double* delta_muH;       /* [P][NB] */
+    const double* mu_species;      /* [P][n_species] (grafted species already 0) */
+    const double* total_charge;    /* [P] wall charge sum */
+    const int32_t* fixed_coord;    /* [P][n_species] grid index of the grafted segment, -1 if none */
+
+    /* per point grids */
+    const double* Ext;             /* concat [P] of (NP, NB) column-major */
+    const double* trapez;          /* concat [P] of (NP, NB) */
+    const double* surf;            /* concat [P] of (NP): wall charge / dz on the primary half */
+
+    /* walls (for the contact-value theorem); per point */
+    int32_t hard_walls;            /* 0/1 */
+    const int32_t* wall_lo;        /* [P][NB] contact index of the left wall */
+    const int32_t* wall_hi;        /* [P][NB] */
+    const double* wall_charge;     /* [P][2] or NULL */
+} sdft_desc_t;
+
+typedef struct sdft_solver_opts {
+    double tole;          /* ||F(x)-x||_inf convergence tolerance */
+    double damping;
+    double mixing_max;
+    double switch_tole;   /* Picard -> Anderson switch */
+    int32_t anderm;       /* Anderson depth (<= 8) */
+    int32_t max_iters;
+    int32_t precision;    /* SDFT_PREC_* */
+    double fp32_switch;   /* mixed: switch to FP64 once ||F(x)-x||_inf < fp32_switch */
+    int32_t band_fp32;    /* 1: FP32 propagator convolutions use the banded bond kernel */
+    int32_t check_every;  /* host polls convergence every this many iterations */
+} sdft_solver_opts_t;
+
+int32_t sdft_abi_version(void);
+const char* sdft_last_error(const sdft_ctx_t* ctx);   /* ctx may be NULL: global last error */
+
+int32_t sdft_create(const sdft_desc_t* desc, sdft_ctx_t** out);
+void sdft_destroy(sdft_ctx_t* ctx);
+
+/* total length of the batched solution vector and per-point offsets (n_points+1 entries) */
+int64_t sdft_solution_length(const sdft_ctx_t* ctx);
+int32_t sdft_solution_offsets(const sdft_ctx_t* ctx, int64_t* offsets);
+
+/* one evaluation of F(x) for every point; x and fx are host buffers of sdft_solution_length */
+int32_t sdft_objective(sdft_ctx_t* ctx, const double* x, double* fx, int32_t precision);
+
+/* Anderson solve of every point from x_inout (host, batched); iters/err/prec_switch_iter per point */
+int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* opts, double* x_inout,
+                   int32_t* iters, double* err);
+
+/* after sdft_solve: Omega[P], alpha[P][n_species*S_max] (NaN where undefined), p_cvt[P][2] */
+int32_t sdft_observables(sdft_ctx_t* ctx, double* omega, double* alpha, double* p_cvt);
+
+/* copy one field of one point to host (layouts above) */
+int32_t sdft_get_field(sdft_ctx_t* ctx, int32_t field_id, int32_t point, double* host_out, int64_t capacity);
+
+/* timing of the last sdft_solve: device milliseconds of the whole solve and of the objective only */
+int32_t sdft_last_timing(const sdft_ctx_t* ctx, double* solve_ms, double* iterations_total);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,733 @@
+// C ABI of libsoftdft_hip: context creation, objective, Anderson solve,
+// observables.  Host orchestration only; all grid work runs in the kernels of
+// sdft_kernels.hip on one HIP stream per context.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <tuple>
+
+#include "sdft_launch.h"
+
+using namespace sdft;
+
+struct sdft_ctx {
+    Ctx c;
+    Pipeline<float> p32;
+    Pipeline<double> p64;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    std::vector<DevPoint> pts;
+    std::vector<void*> allocs;
+    std::string err;
+    bool solved = false;
+    bool have_eval = false;
+    double last_solve_ms = 0.0;
+    double last_iters_total = 0.0;
+    int nsets = 0;
+    int S_max = 1;
+    std::vector<int> nseg;
+    std::vector<int> S;
+    std::vector<int> seg_states_h;
+    std::vector<int> state_off;
+    std::vector<int> multistate;
+    std::vector<int> hard_lo, hard_hi;
+    std::vector<double> wall_q;
+    int hard_walls = 0, charged_walls = 0;
+    std::vector<double> valences;
+    double lB = 0.0;
+};
+
+static thread_local std::string g_last_error;
+
+static void set_error(sdft_ctx* ctx, int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    (void)code;
+    g_last_error = buf;
+    if (ctx) ctx->err = buf;
+}
+
+template <typename T>
+static int dalloc(sdft_ctx* ctx, T** p, int64_t count) {
+    if (count <= 0) count = 1;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, (size_t)count * sizeof(T));
+    if (e != hipSuccess) {
+        set_error(ctx, SDFT_E_HIP, "hipMalloc(%lld bytes) failed: %s", (long long)(count * sizeof(T)), hipGetErrorString(e));
+        return SDFT_E_HIP;
+    }
+    e = hipMemsetAsync(q, 0, (size_t)count * sizeof(T), ctx->stream);
+    if (e != hipSuccess) {
+        set_error(ctx, SDFT_E_HIP, "hipMemsetAsync failed: %s", hipGetErrorString(e));
+        return SDFT_E_HIP;
+    }
+    ctx->allocs.push_back(q);
+    *p = (T*)q;
+    return SDFT_OK;
+}
+
+#define TRY(expr)                      \
+    do {                               \
+        int _r = (expr);               \
+        if (_r != SDFT_OK) return _r;  \
+    } while (0)
+
+template <typename T>
+static int upload(sdft_ctx* ctx, T** dst, const T* src, int64_t count) {
+    TRY(dalloc(ctx, dst, count));
+    if (count > 0) HIP_CHECK_RET(hipMemcpyAsync(*dst, src, (size_t)count * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    return SDFT_OK;
+}
+
+extern "C" int32_t sdft_abi_version(void) { return SDFT_ABI_VERSION; }
+
+extern "C" const char* sdft_last_error(const sdft_ctx_t* ctx) {
+    if (ctx && !ctx->err.empty()) return ctx->err.c_str();
+    return g_last_error.c_str();
+}
+
+extern "C" void sdft_destroy(sdft_ctx_t* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void* p : ctx->allocs) (void)hipFree(p);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+static int build_jobs(const sdft_desc_t* d, std::vector<ConvJob>& fwd, std::vector<ConvJob>& back) {
+    const int NB = d->NB, MB = d->MB;
+    const int F = d->functionals;
+    const bool fmt = F & (SDFT_F_MFMT | SDFT_F_TPT1MFMT);
+    const bool msa = F & (SDFT_F_MSA | SDFT_F_TPT1MSA);
+    const bool tpt1 = F & (SDFT_F_TPT1MFMT | SDFT_F_TPT1MSA);
+    auto job = [](int outkind, int outfield, int absval, double scale) {
+        ConvJob j;
+        memset(&j, 0, sizeof(j));
+        j.outkind = outkind; j.outfield = outfield; j.absval = absval; j.scale = scale;
+        return j;
+    };
+    auto add = [](ConvJob& j, int kern, int odd, int infam, int infield) {
+        ConvTerm& t = j.t[j.nterms++];
+        t.kern = kern; t.odd = odd; t.infam = infam; t.infield = infield;
+    };
+    for (int j = 0; j < NB; ++j) {
+        if (fmt || msa) { ConvJob a = job(0, j, 1, 1.0); add(a, 3 * j + 0, 0, 0, j); fwd.push_back(a); }
+        if (fmt) {
+            ConvJob b = job(0, NB + j, 1, 1.0); add(b, 3 * j + 1, 0, 0, j); fwd.push_back(b);
+            ConvJob c = job(0, 2 * NB + j, 0, 1.0); add(c, 3 * j + 2, 1, 0, j); fwd.push_back(c);
+        }
+    }
+    if (tpt1) {
+        for (int u = 0; u < MB; ++u)
+            for (int e = 0; e < 2; ++e) {
+                const int bt = d->bond_types[2 * u + e];
+                const double s = d->diameters[bt];
+                ConvJob a = job(0, 3 * NB + 2 * u + e, 1, 1.0 / (M_PI * s * s));
+                add(a, 3 * bt + 0, 0, 0, NB + 2 * u + e);
+                fwd.push_back(a);
+            }
+    }
+    for (int j = 0; j < NB; ++j) {
+        ConvJob a = job(1, j, 0, 1.0);
+        if (fmt) {
+            add(a, 3 * j + 0, 0, 1, j);
+            add(a, 3 * j + 1, 0, 1, NB + j);
+            add(a, 3 * j + 2, 2, 1, 2 * NB + j);
+        } else if (msa) {
+            add(a, 3 * j + 0, 0, 1, j);
+        }
+        back.push_back(a);
+    }
+    for (int u = 0; u < MB; ++u)
+        for (int e = 0; e < 2; ++e) {
+            const int bt = d->bond_types[2 * u + e];
+            ConvJob a = job(1, NB + 2 * u + e, 0, 1.0);
+            if (tpt1) add(a, 3 * bt + 0, 0, 1, 3 * NB + 2 * u + e);
+            back.push_back(a);
+        }
+    ConvJob psi = job(1, NB + 2 * MB, 0, 1.0);
+    add(psi, 3 * NB + MB, 0, 1, 3 * NB + 2 * MB);
+    back.push_back(psi);
+    return SDFT_OK;
+}
+
+extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
+    if (!out || !d) {
+        set_error(nullptr, SDFT_E_ARG, "null argument");
+        return SDFT_E_ARG;
+    }
+    *out = nullptr;
+    if (d->abi_version != SDFT_ABI_VERSION) {
+        set_error(nullptr, SDFT_E_ARG, "abi_version %d != %d", d->abi_version, SDFT_ABI_VERSION);
+        return SDFT_E_ARG;
+    }
+    const int P = d->n_points, NB = d->NB, MB = d->MB, nsp = d->n_species;
+    if (P <= 0 || NB <= 0 || NB > SDFT_MAX_NB || MB < 0 || MB > SDFT_MAX_MB || nsp <= 0 || nsp > SDFT_MAX_SPECIES) {
+        set_error(nullptr, SDFT_E_ARG, "bad sizes P=%d NB=%d MB=%d n_species=%d", P, NB, MB, nsp);
+        return SDFT_E_ARG;
+    }
+    const int F = d->functionals;
+    if (((F & SDFT_F_MFMT) && (F & SDFT_F_TPT1MFMT)) || ((F & SDFT_F_MSA) && (F & SDFT_F_TPT1MSA))) {
+        set_error(nullptr, SDFT_E_UNSUPPORTED, "list at most one FMT and one MSA functional");
+        return SDFT_E_UNSUPPORTED;
+    }
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) {
+        set_error(nullptr, SDFT_E_HIP, "no HIP device available");
+        return SDFT_E_HIP;
+    }
+    sdft_ctx* ctx = new sdft_ctx();
+    (void)hipGetDevice(&ctx->device);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error(ctx, SDFT_E_HIP, "hipStreamCreate failed");
+        delete ctx;
+        return SDFT_E_HIP;
+    }
+    int rc = SDFT_OK;
+    auto fail = [&](int code) {
+        sdft_destroy(ctx);
+        return code;
+    };
+    Ctx& c = ctx->c;
+    c.P = P; c.NB = NB; c.MB = MB; c.nsp = nsp; c.K = 3 * NB + MB + 1;
+    ctx->lB = d->bjerrum;
+    ctx->valences.assign(d->valences, d->valences + NB);
+
+    // ---- system
+    DevSystem sys;
+    memset(&sys, 0, sizeof(sys));
+    sys.NB = NB; sys.MB = MB; sys.n_species = nsp; sys.K = c.K; sys.functionals = F; sys.lB = d->bjerrum;
+    for (int j = 0; j < NB; ++j) {
+        if (!(d->diameters[j] > 0.0)) {
+            set_error(ctx, SDFT_E_ARG, "diameter of bead %d must be positive", j);
+            return fail(SDFT_E_ARG);
+        }
+        sys.sigma[j] = d->diameters[j];
+        sys.z[j] = d->valences[j];
+    }
+    for (int u = 0; u < MB; ++u) {
+        sys.bond_i[u] = d->bond_types[2 * u];
+        sys.bond_j[u] = d->bond_types[2 * u + 1];
+        if (sys.bond_i[u] < 0 || sys.bond_j[u] >= NB || sys.bond_i[u] > sys.bond_j[u]) {
+            set_error(ctx, SDFT_E_ARG, "bond %d must satisfy 0 <= i <= j < NB", u);
+            return fail(SDFT_E_ARG);
+        }
+    }
+    sys.hard_walls = d->hard_walls;
+    sys.charged_walls = d->wall_charge != nullptr;
+    ctx->hard_walls = d->hard_walls;
+    ctx->charged_walls = d->wall_charge != nullptr;
+
+    // ---- species
+    std::vector<DevSpecies> spec(nsp);
+    std::vector<int64_t> goffs(nsp);
+    int state_off = 0, maxd_off = 0;
+    int64_t goff = 0;
+    c.species.resize(nsp);
+    for (int u = 0; u < nsp; ++u) {
+        const int N = d->species_nseg[u], S = d->species_nstates[u];
+        if (N <= 0 || S <= 0 || S > SDFT_MAX_S) {
+            set_error(ctx, SDFT_E_UNSUPPORTED, "species %d: nseg=%d states=%d (at most %d states per segment)", u, N, S, SDFT_MAX_S);
+            return fail(SDFT_E_UNSUPPORTED);
+        }
+        if (d->species_model[u] == SDFT_MODEL_MONOMERBEAD && N > 1) {
+            set_error(ctx, SDFT_E_UNSUPPORTED, "species %d: multi-bead monomerbead species are not supported", u);
+            return fail(SDFT_E_UNSUPPORTED);
+        }
+        const int fseg = d->species_fixed_seg ? d->species_fixed_seg[u] : -1;
+        if (fseg > 0 || (fseg == 0 && N < 2)) {
+            set_error(ctx, SDFT_E_UNSUPPORTED, "species %d: only a grafted first segment of a chain is supported", u);
+            return fail(SDFT_E_UNSUPPORTED);
+        }
+        DevSpecies& s = spec[u];
+        memset(&s, 0, sizeof(s));
+        s.nseg = N; s.S = S; s.model = d->species_model[u]; s.fixed_seg = fseg;
+        s.state_off = state_off; s.maxd_off = maxd_off;
+        s.fixed_density = d->fixed_density ? d->fixed_density[u] : 0.0;
+        int multi = 0;
+        for (int q = 0; q < N; ++q) {
+            int cnt = 0;
+            for (int a = 0; a < S; ++a) {
+                const int b = d->seg_states[state_off + q * S + a];
+                if (b >= NB) {
+                    set_error(ctx, SDFT_E_ARG, "species %d segment %d: bead type %d out of range", u, q, b);
+                    return fail(SDFT_E_ARG);
+                }
+                if (b >= 0) ++cnt;
+            }
+            if (cnt == 0 || d->seg_states[state_off + q * S] < 0) {
+                set_error(ctx, SDFT_E_ARG, "species %d segment %d has no leading state", u, q);
+                return fail(SDFT_E_ARG);
+            }
+            if (cnt > 1) multi = 1;
+        }
+        s.multistate = multi;
+        goffs[u] = goff;
+        SpeciesHost& sh = c.species[u];
+        sh.nseg = N; sh.S = S; sh.model = s.model; sh.fixed_seg = fseg; sh.multistate = multi;
+        sh.state_off = state_off; sh.maxd_off = maxd_off; sh.goff = goff;
+        ctx->nseg.push_back(N);
+        ctx->S.push_back(S);
+        ctx->state_off.push_back(state_off);
+        ctx->multistate.push_back(multi);
+        ctx->S_max = std::max(ctx->S_max, S);
+        goff += (int64_t)N * S * S;
+        state_off += N * S;
+        maxd_off += N;
+    }
+    ctx->seg_states_h.assign(d->seg_states, d->seg_states + state_off);
+    c.GF = (int)goff;
+    c.Frt = NB + 2 * MB;
+    c.Fnw = 3 * NB + 2 * MB;
+    c.Fdp = 3 * NB + 2 * MB + 1;
+
+    // ---- points and kernel sets (one set per distinct (NP, dz))
+    std::map<std::pair<int, uint64_t>, int> setmap;
+    std::vector<int32_t> set_L;
+    std::vector<int64_t> set_kbase;
+    std::vector<double> set_dz;
+    ctx->pts.resize(P);
+    int64_t base = 0, xoff = 0, kb = 0;
+    for (int p = 0; p < P; ++p) {
+        DevPoint& q = ctx->pts[p];
+        memset(&q, 0, sizeof(q));
+        const int NP = d->NP[p];
+        if (NP < 4 || !(d->dz[p] > 0.0)) {
+            set_error(ctx, SDFT_E_ARG, "point %d: NP=%d dz=%g", p, NP, d->dz[p]);
+            return fail(SDFT_E_ARG);
+        }
+        q.NP = NP; q.L = 2 * NP; q.base = base; q.xoff = xoff;
+        q.n = (int64_t)NP * (NB + 2 * MB + 1) + 1;
+        q.dz = d->dz[p];
+        q.total_charge = d->total_charge ? d->total_charge[p] : 0.0;
+        uint64_t bits;
+        memcpy(&bits, &d->dz[p], 8);
+        auto key = std::make_pair(NP, bits);
+        auto it = setmap.find(key);
+        if (it == setmap.end()) {
+            const int sidx = (int)set_L.size();
+            setmap[key] = sidx;
+            set_L.push_back(2 * NP);
+            set_kbase.push_back(kb);
+            set_dz.push_back(d->dz[p]);
+            kb += 2 * NP;
+            q.kset = sidx;
+        } else {
+            q.kset = it->second;
+        }
+        q.kbase = set_kbase[q.kset];
+        for (int j = 0; j < NB; ++j) q.delta_muH[j] = d->delta_muH[(int64_t)p * NB + j];
+        for (int u = 0; u < nsp; ++u) {
+            q.mu_species[u] = d->mu_species[(int64_t)p * nsp + u];
+            q.fixed_coord[u] = d->fixed_coord ? d->fixed_coord[(int64_t)p * nsp + u] : -1;
+            if (spec[u].fixed_seg >= 0 && (q.fixed_coord[u] < 0 || q.fixed_coord[u] >= NP)) {
+                set_error(ctx, SDFT_E_ARG, "point %d species %d: grafting coordinate %d outside [0, %d)", p, u, q.fixed_coord[u], NP);
+                return fail(SDFT_E_ARG);
+            }
+            // normative (child_propagators.jl:41-50): prod over segments of (sum_states e^{-dmuH})^(1/N)
+            const int N = spec[u].nseg, S = spec[u].S;
+            double normative = 1.0;
+            for (int s2 = N - 1; s2 >= 0; --s2) {
+                double t = 0.0;
+                for (int a = 0; a < S; ++a) {
+                    const int b = d->seg_states[spec[u].state_off + s2 * S + a];
+                    if (b >= 0) t += exp(-q.delta_muH[b]);
+                }
+                normative *= pow(t, 1.0 / N);
+            }
+            q.normative[u] = normative;
+            q.chain_norm[u] = pow(normative, (double)(N - 1));
+        }
+        for (int j = 0; j < NB; ++j) {
+            q.wall_lo[j] = d->wall_lo ? d->wall_lo[(int64_t)p * NB + j] : 0;
+            q.wall_hi[j] = d->wall_hi ? d->wall_hi[(int64_t)p * NB + j] : 0;
+        }
+        if (d->wall_charge) { q.wall_q[0] = d->wall_charge[2 * p]; q.wall_q[1] = d->wall_charge[2 * p + 1]; }
+        base += NP;
+        xoff += q.n;
+        c.maxNP = std::max(c.maxNP, NP);
+        c.max_n = std::max(c.max_n, q.n);
+    }
+    c.sumNP = base;
+    c.total_n = xoff;
+    c.sumL_sets = kb;
+    ctx->nsets = (int)set_L.size();
+
+    // ---- jobs
+    std::vector<ConvJob> fwd, back;
+    build_jobs(d, fwd, back);
+    c.n_fwd_jobs = (int)fwd.size();
+    c.n_back_jobs = (int)back.size();
+    std::vector<ConvJob> jobs(fwd);
+    jobs.insert(jobs.end(), back.begin(), back.end());
+
+    // ---- uploads
+    if ((rc = upload(ctx, &c.d_sys, &sys, 1)) != SDFT_OK) return fail(rc);
+    if ((rc = upload(ctx, &c.d_pts, ctx->pts.data(), P)) != SDFT_OK) return fail(rc);
+    if ((rc = upload(ctx, &c.d_spec, spec.data(), nsp)) != SDFT_OK) return fail(rc);
+    if ((rc = upload(ctx, &c.d_goffs, goffs.data(), nsp)) != SDFT_OK) return fail(rc);
+    if ((rc = upload(ctx, &c.d_seg_states, d->seg_states, state_off)) != SDFT_OK) return fail(rc);
+    std::vector<double> maxd(maxd_off, 0.0);
+    if (d->max_distance) maxd.assign(d->max_distance, d->max_distance + maxd_off);
+    if ((rc = upload(ctx, &c.d_maxd, maxd.data(), maxd_off)) != SDFT_OK) return fail(rc);
+    if ((rc = upload(ctx, &c.d_jobs, jobs.data(), (int64_t)jobs.size())) != SDFT_OK) return fail(rc);
+    if ((rc = upload(ctx, &c.d_Ext, d->Ext, c.sumNP * NB)) != SDFT_OK) return fail(rc);
+    std::vector<double> surf(c.sumNP, 0.0);
+    if (d->surf) surf.assign(d->surf, d->surf + c.sumNP);
+    if ((rc = upload(ctx, &c.d_surf, surf.data(), c.sumNP)) != SDFT_OK) return fail(rc);
+    double* d_trap64 = nullptr;
+    if ((rc = upload(ctx, &d_trap64, d->trapez, c.sumNP * NB)) != SDFT_OK) return fail(rc);
+    ctx->p64.trapez = d_trap64;
+    if ((rc = dalloc(ctx, &ctx->p32.trapez, c.sumNP * NB)) != SDFT_OK) return fail(rc);
+    launch_convert_f32(ctx->stream, d_trap64, ctx->p32.trapez, c.sumNP * NB);
+
+    // kernel vectors
+    int32_t* d_setL = nullptr;
+    int64_t* d_setkb = nullptr;
+    double* d_setdz = nullptr;
+    double* d_wbuf = nullptr;
+    if ((rc = upload(ctx, &d_setL, set_L.data(), ctx->nsets)) != SDFT_OK) return fail(rc);
+    if ((rc = upload(ctx, &d_setkb, set_kbase.data(), ctx->nsets)) != SDFT_OK) return fail(rc);
+    if ((rc = upload(ctx, &d_setdz, set_dz.data(), ctx->nsets)) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &d_wbuf, c.sumL_sets * c.K)) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &ctx->p64.kern, c.sumL_sets * c.K)) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &ctx->p32.kern, c.sumL_sets * c.K)) != SDFT_OK) return fail(rc);
+    int maxL = 0;
+    for (int L : set_L) maxL = std::max(maxL, L);
+    launch_build_kernels(ctx->stream, c.d_sys, c.K, ctx->nsets, maxL, d_setL, d_setkb, d_setdz, d_wbuf, ctx->p64.kern,
+                         ctx->p32.kern);
+    HIP_CHECK_RET(hipGetLastError());
+    // banded half-width of the bond kernels (FP32 phase): beyond it every |c| <= 1e-9 max|c|
+    {
+        std::vector<double> hk((size_t)c.sumL_sets * c.K);
+        HIP_CHECK_RET(hipMemcpyAsync(hk.data(), ctx->p64.kern, hk.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+        std::vector<int32_t> band(ctx->nsets, 0);
+        for (int s = 0; s < ctx->nsets; ++s) {
+            const int L = set_L[s];
+            int B = 0;
+            for (int u = 0; u < MB; ++u) {
+                const double* cv = hk.data() + set_kbase[s] * c.K + (int64_t)(3 * NB + u) * L;
+                double mx = 0.0;
+                for (int i = 0; i < L; ++i) mx = std::max(mx, fabs(cv[i]));
+                int b = 0;
+                for (int i = 0; i <= L / 2; ++i)
+                    if (fabs(cv[i]) > 1e-9 * mx || fabs(cv[(L - i) % L]) > 1e-9 * mx) b = i;
+                B = std::max(B, b);
+            }
+            band[s] = B;
+        }
+        if ((rc = upload(ctx, &c.d_band, band.data(), ctx->nsets)) != SDFT_OK) return fail(rc);
+        HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    }
+
+    // ---- work buffers
+    auto alloc_pipe = [&](auto& pp) -> int {
+        TRY(dalloc(ctx, &pp.elam, c.sumNP * NB));
+        TRY(dalloc(ctx, &pp.hl, c.sumNP * 2 * MB));
+        TRY(dalloc(ctx, &pp.gC, c.sumNP * c.GF));
+        TRY(dalloc(ctx, &pp.gP, c.sumNP * c.GF));
+        TRY(dalloc(ctx, &pp.rho_beads, c.sumNP * NB));
+        TRY(dalloc(ctx, &pp.rho_bonds, c.sumNP * 2 * MB));
+        TRY(dalloc(ctx, &pp.rhoT, c.sumNP * c.Frt));
+        TRY(dalloc(ctx, &pp.nw, c.sumNP * c.Fnw));
+        TRY(dalloc(ctx, &pp.dphi, c.sumNP * c.Fdp));
+        return SDFT_OK;
+    };
+    if ((rc = alloc_pipe(ctx->p32)) != SDFT_OK) return fail(rc);
+    if ((rc = alloc_pipe(ctx->p64)) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &c.d_evalflag, P)) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &c.d_errflag, P)) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &c.d_acc_species, (int64_t)P * 2 * SDFT_MAX_SPECIES)) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &c.d_fixed_lambda, (int64_t)P * SDFT_MAX_SPECIES)) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &c.d_x, c.total_n)) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &c.d_fx, c.total_n)) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &c.d_gram, (int64_t)P * (SDFT_MAX_M * SDFT_MAX_M + SDFT_MAX_M))) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &c.d_errbits, P)) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &c.d_state, P)) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &c.d_n_active, 1)) != SDFT_OK) return fail(rc);
+    ctx->hard_lo.assign(d->wall_lo ? d->wall_lo : nullptr, d->wall_lo ? d->wall_lo + (int64_t)P * NB : nullptr);
+    ctx->hard_hi.assign(d->wall_hi ? d->wall_hi : nullptr, d->wall_hi ? d->wall_hi + (int64_t)P * NB : nullptr);
+    if (d->wall_charge) ctx->wall_q.assign(d->wall_charge, d->wall_charge + 2 * P);
+    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    *out = ctx;
+    return SDFT_OK;
+}
+
+extern "C" int64_t sdft_solution_length(const sdft_ctx_t* ctx) { return ctx ? ctx->c.total_n : -1; }
+
+extern "C" int32_t sdft_solution_offsets(const sdft_ctx_t* ctx, int64_t* offsets) {
+    if (!ctx || !offsets) return SDFT_E_ARG;
+    for (int p = 0; p < ctx->c.P; ++p) offsets[p] = ctx->pts[p].xoff;
+    offsets[ctx->c.P] = ctx->c.total_n;
+    return SDFT_OK;
+}
+
+static int check_errflags(sdft_ctx* ctx) {
+    std::vector<int32_t> ef(ctx->c.P);
+    HIP_CHECK_RET(hipMemcpyAsync(ef.data(), ctx->c.d_errflag, ef.size() * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    for (int p = 0; p < ctx->c.P; ++p)
+        if (ef[p] != 0) {
+            set_error(ctx, ef[p], "Normalization error: non-finite or non-positive chain normalisation at point %d", p);
+            return ef[p];
+        }
+    return SDFT_OK;
+}
+
+static void run_objective(sdft_ctx* ctx) {
+    ctx->p32.objective(ctx->stream, ctx->c, 32, ctx->c.d_x, ctx->c.d_fx);
+    ctx->p64.objective(ctx->stream, ctx->c, 64, ctx->c.d_x, ctx->c.d_fx);
+}
+
+extern "C" int32_t sdft_objective(sdft_ctx_t* ctx, const double* x, double* fx, int32_t precision) {
+    if (!ctx || !x || !fx) return SDFT_E_ARG;
+    if (precision != SDFT_PREC_FP32 && precision != SDFT_PREC_FP64) {
+        set_error(ctx, SDFT_E_ARG, "precision must be 32 or 64");
+        return SDFT_E_ARG;
+    }
+    Ctx& c = ctx->c;
+    HIP_CHECK_RET(hipMemcpyAsync(c.d_x, x, c.total_n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    launch_zero_i32(ctx->stream, c.d_errflag, c.P);
+    launch_set_evalflag(ctx->stream, c.d_evalflag, precision, c.P);
+    // record the evaluation precision in the solve state so observables read the right pipeline
+    std::vector<DevSolveState> st(c.P);
+    for (auto& s : st) { memset(&s, 0, sizeof(s)); s.prec = precision; s.converged = 1; s.mode = -1; }
+    HIP_CHECK_RET(hipMemcpyAsync(c.d_state, st.data(), st.size() * sizeof(DevSolveState), hipMemcpyHostToDevice, ctx->stream));
+    run_objective(ctx);
+    HIP_CHECK_RET(hipGetLastError());
+    HIP_CHECK_RET(hipMemcpyAsync(fx, c.d_fx, c.total_n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    ctx->have_eval = true;
+    ctx->solved = false;
+    return check_errflags(ctx);
+}
+
+extern "C" int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* o, double* x_inout, int32_t* iters, double* err) {
+    if (!ctx || !o || !x_inout) return SDFT_E_ARG;
+    Ctx& c = ctx->c;
+    if (o->anderm < 2 || o->anderm > SDFT_MAX_M || o->max_iters < 1) {
+        set_error(ctx, SDFT_E_ARG, "anderm must be in [2, %d] and max_iters >= 1", SDFT_MAX_M);
+        return SDFT_E_ARG;
+    }
+    const int prec = o->precision;
+    if (prec != SDFT_PREC_FP32 && prec != SDFT_PREC_FP64 && prec != SDFT_PREC_MIXED) {
+        set_error(ctx, SDFT_E_ARG, "bad precision %d", prec);
+        return SDFT_E_ARG;
+    }
+    // (re)size the Anderson history
+    if (!c.d_Xh || c.m != o->anderm) {
+        TRY(dalloc(ctx, &c.d_Xh, c.total_n * o->anderm));
+        TRY(dalloc(ctx, &c.d_Gh, c.total_n * o->anderm));
+        c.m = o->anderm;
+    }
+    c.band_fp32 = o->band_fp32;
+    SolveParams sp;
+    sp.tole = o->tole; sp.damping = o->damping; sp.mixing_max = o->mixing_max; sp.switch_tole = o->switch_tole;
+    sp.fp32_switch = o->fp32_switch; sp.m = o->anderm; sp.max_iters = o->max_iters;
+    sp.mixed = prec == SDFT_PREC_MIXED;
+    sp.start_prec = prec == SDFT_PREC_FP64 ? 64 : 32;
+    std::vector<DevSolveState> st(c.P);
+    for (auto& s : st) {
+        memset(&s, 0, sizeof(s));
+        s.use_picard = 1;
+        s.prec = sp.start_prec;
+        s.mode = -1;
+    }
+    HIP_CHECK_RET(hipMemcpyAsync(c.d_state, st.data(), st.size() * sizeof(DevSolveState), hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK_RET(hipMemcpyAsync(c.d_x, x_inout, c.total_n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    launch_zero_i32(ctx->stream, c.d_errflag, c.P);
+    launch_set_evalflag(ctx->stream, c.d_evalflag, sp.start_prec, c.P);
+    launch_zero_f64(ctx->stream, (double*)c.d_errbits, c.P);
+    hipEvent_t e0, e1;
+    HIP_CHECK_RET(hipEventCreate(&e0));
+    HIP_CHECK_RET(hipEventCreate(&e1));
+    HIP_CHECK_RET(hipEventRecord(e0, ctx->stream));
+    const int check = o->check_every > 0 ? o->check_every : 10;
+    int32_t n_active = c.P;
+    int it = 0;
+    for (it = 0; it < o->max_iters; ++it) {
+        run_objective(ctx);
+        launch_anderson(ctx->stream, c, sp);
+        if ((it + 1) % check == 0 || it + 1 == o->max_iters) {
+            HIP_CHECK_RET(hipMemcpyAsync(&n_active, c.d_n_active, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+            if (n_active == 0) break;
+        }
+    }
+    HIP_CHECK_RET(hipGetLastError());
+    HIP_CHECK_RET(hipEventRecord(e1, ctx->stream));
+    HIP_CHECK_RET(hipEventSynchronize(e1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ctx->last_solve_ms = ms;
+    HIP_CHECK_RET(hipMemcpyAsync(x_inout, c.d_x, c.total_n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK_RET(hipMemcpyAsync(st.data(), c.d_state, st.size() * sizeof(DevSolveState), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    int any_fail = 0;
+    double tot = 0.0;
+    for (int p = 0; p < c.P; ++p) {
+        if (iters) iters[p] = st[p].iter;
+        if (err) err[p] = st[p].err;
+        tot += st[p].iter;
+        if (st[p].failed || !st[p].converged) any_fail = 1;
+    }
+    ctx->last_iters_total = tot;
+    ctx->solved = true;
+    ctx->have_eval = true;
+    int rc = check_errflags(ctx);
+    if (rc != SDFT_OK) return rc;
+    if (any_fail) {
+        set_error(ctx, SDFT_E_NOCONV, "at least one state point did not converge (max_iters or non-finite residual)");
+        return SDFT_E_NOCONV;
+    }
+    return SDFT_OK;
+}
+
+extern "C" int32_t sdft_last_timing(const sdft_ctx_t* ctx, double* solve_ms, double* iterations_total) {
+    if (!ctx) return SDFT_E_ARG;
+    if (solve_ms) *solve_ms = ctx->last_solve_ms;
+    if (iterations_total) *iterations_total = ctx->last_iters_total;
+    return SDFT_OK;
+}
+
+template <typename T>
+static int download_field(sdft_ctx* ctx, const T* dptr, int64_t count, double* host) {
+    std::vector<T> tmp(count);
+    HIP_CHECK_RET(hipMemcpyAsync(tmp.data(), dptr, count * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    for (int64_t i = 0; i < count; ++i) host[i] = (double)tmp[i];
+    return SDFT_OK;
+}
+
+static int point_prec(sdft_ctx* ctx, int p) {
+    DevSolveState s;
+    (void)hipMemcpyAsync(&s, ctx->c.d_state + p, sizeof(s), hipMemcpyDeviceToHost, ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
+    return s.prec == 32 ? 32 : 64;
+}
+
+extern "C" int32_t sdft_get_field(sdft_ctx_t* ctx, int32_t field_id, int32_t point, double* out, int64_t capacity) {
+    if (!ctx || !out || point < 0 || point >= ctx->c.P) return SDFT_E_ARG;
+    const Ctx& c = ctx->c;
+    const DevPoint& P = ctx->pts[point];
+    int64_t need = 0;
+    switch (field_id) {
+        case SDFT_FIELD_RHO_BEADS: need = (int64_t)P.NP * c.NB; break;
+        case SDFT_FIELD_RHO_BONDS: need = (int64_t)P.NP * 2 * c.MB; break;
+        case SDFT_FIELD_X: case SDFT_FIELD_FX: need = P.n; break;
+        default: set_error(ctx, SDFT_E_ARG, "unknown field id %d", field_id); return SDFT_E_ARG;
+    }
+    if (capacity < need) {
+        set_error(ctx, SDFT_E_ARG, "capacity %lld < %lld", (long long)capacity, (long long)need);
+        return SDFT_E_ARG;
+    }
+    if (!ctx->have_eval) {
+        set_error(ctx, SDFT_E_STATE, "no evaluation has run yet");
+        return SDFT_E_STATE;
+    }
+    const int prec = point_prec(ctx, point);
+    if (field_id == SDFT_FIELD_X) return download_field(ctx, c.d_x + P.xoff, need, out);
+    if (field_id == SDFT_FIELD_FX) return download_field(ctx, c.d_fx + P.xoff, need, out);
+    if (field_id == SDFT_FIELD_RHO_BEADS)
+        return prec == 64 ? download_field(ctx, ctx->p64.rho_beads + P.base * c.NB, need, out)
+                          : download_field(ctx, ctx->p32.rho_beads + P.base * c.NB, need, out);
+    return prec == 64 ? download_field(ctx, ctx->p64.rho_bonds + P.base * 2 * c.MB, need, out)
+                      : download_field(ctx, ctx->p32.rho_bonds + P.base * 2 * c.MB, need, out);
+}
+
+extern "C" int32_t sdft_observables(sdft_ctx_t* ctx, double* omega, double* alpha, double* p_cvt) {
+    if (!ctx) return SDFT_E_ARG;
+    if (!ctx->have_eval) {
+        set_error(ctx, SDFT_E_STATE, "observables need a prior sdft_solve or sdft_objective");
+        return SDFT_E_STATE;
+    }
+    Ctx& c = ctx->c;
+    const int P = c.P, NB = c.NB;
+    double* d_om = nullptr;
+    TRY(dalloc(ctx, &d_om, P));
+    ctx->p32.omega(ctx->stream, c, c.d_x, d_om);
+    ctx->p64.omega(ctx->stream, c, c.d_x, d_om);
+    std::vector<double> om(P), accs((size_t)P * 2 * SDFT_MAX_SPECIES);
+    std::vector<DevSolveState> st(P);
+    HIP_CHECK_RET(hipMemcpyAsync(om.data(), d_om, P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK_RET(hipMemcpyAsync(accs.data(), c.d_acc_species, accs.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK_RET(hipMemcpyAsync(st.data(), c.d_state, st.size() * sizeof(DevSolveState), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    if (omega) {
+        for (int p = 0; p < P; ++p) {
+            double Nsum = 0.0;
+            for (int u = 0; u < c.nsp; ++u) Nsum += accs[(size_t)p * 2 * SDFT_MAX_SPECIES + 2 * u];
+            omega[p] = om[p] * ctx->pts[p].dz - Nsum;
+        }
+    }
+    if (alpha) {
+        const int Smax = ctx->S_max;
+        for (int64_t i = 0; i < (int64_t)P * c.nsp * Smax; ++i) alpha[i] = NAN;
+        for (int u = 0; u < c.nsp; ++u) {
+            if (!ctx->multistate[u]) continue;
+            const int N = ctx->nseg[u], S = ctx->S[u];
+            double* d_ss = nullptr;
+            TRY(dalloc(ctx, &d_ss, (int64_t)P * N * S));
+            ctx->p32.segment_sums(ctx->stream, c, u, d_ss, (int64_t)N * S);
+            ctx->p64.segment_sums(ctx->stream, c, u, d_ss, (int64_t)N * S);
+            std::vector<double> ss((size_t)P * N * S);
+            HIP_CHECK_RET(hipMemcpyAsync(ss.data(), d_ss, ss.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+            for (int p = 0; p < P; ++p) {
+                std::vector<double> acc(S, 0.0), cnt(S, 0.0);
+                for (int q = 0; q < N; ++q) {
+                    int ns = 0;
+                    for (int a = 0; a < S; ++a) if (ctx->seg_states_h[ctx->state_off[u] + q * S + a] >= 0) ++ns;
+                    if (ns == 1) continue;
+                    double tot = 0.0;
+                    for (int a = 0; a < ns; ++a) tot += ss[(size_t)p * N * S + q * S + a];
+                    for (int a = 0; a < ns; ++a) {
+                        acc[a] += tot > 0 ? ss[(size_t)p * N * S + q * S + a] / tot : 0.0;
+                        cnt[a] += 1.0;
+                    }
+                }
+                for (int a = 0; a < S; ++a)
+                    alpha[((int64_t)p * c.nsp + u) * Smax + a] = cnt[a] > 0 ? acc[a] / cnt[a] : NAN;
+            }
+        }
+    }
+    if (p_cvt) {
+        for (int p = 0; p < P; ++p) {
+            double l = 0.0, r = 0.0;
+            if (ctx->hard_walls && !ctx->hard_lo.empty()) {
+                const DevPoint& Pt = ctx->pts[p];
+                const int prec = st[p].prec == 32 ? 32 : 64;
+                for (int j = 0; j < NB; ++j) {
+                    const int lo = ctx->hard_lo[(size_t)p * NB + j], hi = ctx->hard_hi[(size_t)p * NB + j];
+                    double v[2];
+                    if (prec == 64) {
+                        HIP_CHECK_RET(hipMemcpy(&v[0], ctx->p64.rho_beads + Pt.base * NB + (int64_t)j * Pt.NP + lo, 8, hipMemcpyDeviceToHost));
+                        HIP_CHECK_RET(hipMemcpy(&v[1], ctx->p64.rho_beads + Pt.base * NB + (int64_t)j * Pt.NP + hi, 8, hipMemcpyDeviceToHost));
+                    } else {
+                        float f[2];
+                        HIP_CHECK_RET(hipMemcpy(&f[0], ctx->p32.rho_beads + Pt.base * NB + (int64_t)j * Pt.NP + lo, 4, hipMemcpyDeviceToHost));
+                        HIP_CHECK_RET(hipMemcpy(&f[1], ctx->p32.rho_beads + Pt.base * NB + (int64_t)j * Pt.NP + hi, 4, hipMemcpyDeviceToHost));
+                        v[0] = f[0]; v[1] = f[1];
+                    }
+                    l += v[0];
+                    r += v[1];
+                }
+            }
+            if (ctx->charged_walls && !ctx->wall_q.empty()) {
+                l += -2.0 * M_PI * ctx->lB * ctx->wall_q[2 * p] * ctx->wall_q[2 * p];
+                r += -2.0 * M_PI * ctx->lB * ctx->wall_q[2 * p + 1] * ctx->wall_q[2 * p + 1];
+            }
+            p_cvt[2 * p] = l;
+            p_cvt[2 * p + 1] = r;
+        }
+    }
+    return SDFT_OK;
+}

@@ -1,0 +1,124 @@
+// Internal device/host structures of libsoftdft_hip (gfx950).
+//
+// Memory layout in HBM (all per-point data contiguous per point):
+//   grid family with F fields:   fam[base_p * F + f * NP_p + n]
+//     base_p = sum_{q<p} NP_q,  n in [0, NP_p)
+//   convolution kernels:          kern[kbase_s * K + k * L_s + d]
+//     one set per distinct (NP, dz); L_s = 2 NP_s; K = 3 NB + MB + 1
+//   solution vectors (FP64):      x[xoff_p + i], i in [0, n_p), reference flatten order
+//   Anderson history (FP64):      hist[hoff_p + col * n_p + i]  (hoff_p = m * xoff_p)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SDFT_MAX_NB 16
+#define SDFT_MAX_MB 16
+#define SDFT_MAX_S 2
+#define SDFT_MAX_SPECIES 16
+#define SDFT_MAX_M 8
+
+// kernel-vector index inside a kernel set
+//   3*j + 0 : w2 (delta shell, R = sigma_j / 2)        even
+//   3*j + 1 : w3 (step, R = sigma_j / 2)               even
+//   3*j + 2 : wV (vector delta, i * w applied)          odd
+//   3*NB + u: bond u, delta(sigma_ij)/(4 pi sigma_ij^2) x spectral filter   even
+//   3*NB+MB : Coulomb 4 pi lB / lambda_FD(k)            even
+struct DevPoint {
+    int32_t NP;
+    int32_t L;
+    int32_t kset;        // index of the kernel set
+    int32_t pad0;
+    int64_t base;        // sum of NP over previous points
+    int64_t kbase;       // sum of L over previous kernel sets (of this point's set)
+    int64_t xoff;        // offset of this point's solution vector
+    int64_t n;           // solution length
+    double dz;
+    double total_charge;
+    int32_t fixed_coord[SDFT_MAX_SPECIES];
+    double mu_species[SDFT_MAX_SPECIES];
+    double delta_muH[SDFT_MAX_NB];
+    int32_t wall_lo[SDFT_MAX_NB];
+    int32_t wall_hi[SDFT_MAX_NB];
+    double wall_q[2];
+    // per species: normative = prod_seg (sum_states exp(-delta_muH))^(1/N) (child_propagators.jl:41-50)
+    // and the free-chain normalisation normative^(N-1) (handle_fixed.jl:105-107)
+    double normative[SDFT_MAX_SPECIES];
+    double chain_norm[SDFT_MAX_SPECIES];
+};
+
+struct DevSpecies {
+    int32_t nseg;
+    int32_t S;            // max states per segment
+    int32_t model;        // 0 freely jointed, 1 monomer bead
+    int32_t fixed_seg;    // -1 or grafted segment (only 0 supported)
+    int32_t multistate;   // any segment with > 1 state
+    int32_t state_off;    // offset into the seg_states table ([nseg][S])
+    int32_t segsum_off;   // offset into the per-point segment-sum table ([nseg][S][2])
+    int32_t maxd_off;     // offset into the max-distance table ([nseg])
+    double fixed_density;
+};
+
+// System-wide constants shared by every point of a context
+struct DevSystem {
+    int32_t NB, MB, n_species, K;
+    int32_t functionals;
+    int32_t segsum_len;   // per point
+    double lB;
+    double sigma[SDFT_MAX_NB];
+    double z[SDFT_MAX_NB];
+    int32_t bond_i[SDFT_MAX_MB];
+    int32_t bond_j[SDFT_MAX_MB];
+    int32_t hard_walls;
+    int32_t charged_walls;
+};
+
+// one term of a convolution job: out += sign-structured T(+-)H(kern) * in
+struct ConvTerm {
+    int32_t kern;     // kernel index within the set
+    int32_t odd;      // 0: A = c[n-n'] + c[n+n'+1];  1: forward odd  A = c[n-n'] + c[n+n'+1] (c odd)
+                      // 2: transpose of odd          A = -c[n-n'] + c[n+n'+1]
+    int32_t infam;    // 0: rhoT family, 1: dphi family
+    int32_t infield;
+};
+
+struct ConvJob {
+    int32_t nterms;
+    ConvTerm t[3];
+    int32_t outkind;   // 0: weighted-density family (T), 1: F(x) (double) at field offset
+    int32_t outfield;  // field index in the family, or field offset in units of NP within the solution vector
+    int32_t absval;    // take |.| of the result (scalar weighted densities)
+    double scale;
+};
+
+// Anderson / solve state per point
+struct DevSolveState {
+    int32_t iter;         // iterations done
+    int32_t hist_base;    // iterations done when the current history started (0, or the precision switch)
+    int32_t restarted;    // history was restarted at the precision switch
+    int32_t use_picard;
+    int32_t converged;
+    int32_t prec;         // 32 or 64: precision of the next objective
+    int32_t switch_iter;  // iteration of the FP32 -> FP64 switch (0 if none)
+    int32_t mode;         // 0 Picard, 1 Anderson, for the pending update
+    int32_t ncols;        // number of difference columns used by the pending update
+    int32_t failed;       // 1: max_iters reached, 2: non-finite residual
+    int32_t pad;
+    double err;
+    double amix;
+    double lam[SDFT_MAX_M];
+};
+
+struct SolveParams {
+    double tole, damping, mixing_max, switch_tole, fp32_switch;
+    int32_t m, max_iters, mixed, start_prec;
+};
+
+#define HIP_CHECK_RET(expr)                                                     \
+    do {                                                                        \
+        hipError_t _e = (expr);                                                 \
+        if (_e != hipSuccess) {                                                 \
+            set_error(ctx, SDFT_E_HIP, "%s failed: %s (%s:%d)", #expr,          \
+                      hipGetErrorString(_e), __FILE__, __LINE__);               \
+            return SDFT_E_HIP;                                                  \
+        }                                                                       \
+    } while (0)

@@ -1,0 +1,83 @@
+// Host-side context and per-precision pipeline declarations shared by the
+// kernel and API translation units.
+#pragma once
+#include <vector>
+#include <string>
+#include <algorithm>
+
+#include "../../include/softdft_hip.h"
+#include "sdft_internal.h"
+
+namespace sdft {
+
+struct SpeciesHost {
+    int nseg, S, model, fixed_seg, multistate;
+    int state_off, maxd_off;
+    int64_t goff;   // field offset in the propagator families
+};
+
+// Device pointers and sizes shared by both precisions.
+struct Ctx {
+    int P = 0, NB = 0, MB = 0, nsp = 0, K = 0;
+    int maxNP = 0;
+    int64_t sumNP = 0, sumL_sets = 0;
+    int64_t total_n = 0, max_n = 0;
+    int GF = 0;                     // propagator fields per point (sum over species of nseg*S*S)
+    int Frt = 0, Fnw = 0, Fdp = 0;  // family widths
+    int n_fwd_jobs = 0, n_back_jobs = 0;
+    int band_fp32 = 1;
+    int m = 5;                      // Anderson depth the history buffers were sized for
+    std::vector<SpeciesHost> species;
+
+    DevSystem* d_sys = nullptr;
+    DevPoint* d_pts = nullptr;
+    DevSpecies* d_spec = nullptr;
+    int64_t* d_goffs = nullptr;
+    int32_t* d_seg_states = nullptr;
+    double* d_maxd = nullptr;
+    ConvJob* d_jobs = nullptr;
+    int32_t* d_band = nullptr;      // banded half-width of the bond kernels per kernel set
+    double* d_Ext = nullptr;        // (sumNP * NB)
+    double* d_surf = nullptr;       // (sumNP)
+    int32_t* d_evalflag = nullptr;  // per point: 0 / 32 / 64 = evaluate in that precision
+    int32_t* d_errflag = nullptr;
+    double* d_acc_species = nullptr;   // [P][2*MAX_SPECIES]
+    double* d_fixed_lambda = nullptr;  // [P][MAX_SPECIES]
+    // solve
+    double* d_x = nullptr;
+    double* d_fx = nullptr;
+    double* d_Xh = nullptr;
+    double* d_Gh = nullptr;
+    double* d_gram = nullptr;
+    unsigned long long* d_errbits = nullptr;
+    DevSolveState* d_state = nullptr;
+    int32_t* d_n_active = nullptr;
+};
+
+template <typename T>
+struct Pipeline {
+    T* kern = nullptr;      // convolution kernel vectors
+    T* trapez = nullptr;    // (sumNP * NB)
+    T* elam = nullptr;
+    T* hl = nullptr;
+    T* gC = nullptr;
+    T* gP = nullptr;
+    T* rho_beads = nullptr;
+    T* rho_bonds = nullptr;
+    T* rhoT = nullptr;
+    T* nw = nullptr;
+    T* dphi = nullptr;
+    void objective(hipStream_t st, const Ctx& c, int prec, const double* x, double* fx);
+    void omega(hipStream_t st, const Ctx& c, const double* x, double* omega_acc);
+    void segment_sums(hipStream_t st, const Ctx& c, int u, double* out, int64_t stride);
+};
+
+void launch_build_kernels(hipStream_t st, const DevSystem* sys, int K, int nsets, int maxL, const int32_t* set_L,
+                          const int64_t* set_kbase, const double* set_dz, double* wbuf, double* c64, float* c32);
+void launch_anderson(hipStream_t st, const Ctx& c, const SolveParams& sp);
+void launch_set_evalflag(hipStream_t st, int32_t* evalflag, int value, int P);
+void launch_zero_i32(hipStream_t st, int32_t* a, int64_t n);
+void launch_zero_f64(hipStream_t st, double* a, int64_t n);
+void launch_convert_f32(hipStream_t st, const double* a, float* b, int64_t n);
+
+}  // namespace sdft
