@@ -158,8 +158,24 @@ int32_t sdft_observables(sdft_ctx_t* ctx, double* omega, double* alpha, double* 
 /* copy one field of one point to host (layouts above) */
 int32_t sdft_get_field(sdft_ctx_t* ctx, int32_t field_id, int32_t point, double* host_out, int64_t capacity);
 
-/* timing of the last sdft_solve: device milliseconds of the whole solve and of the objective only */
+/* timing of the last sdft_solve: device milliseconds of the whole solve and total iterations */
 int32_t sdft_last_timing(const sdft_ctx_t* ctx, double* solve_ms, double* iterations_total);
+
+/* kernel classes for sdft_kernel_profile */
+#define SDFT_KP_PROP_FP32 0      /* one propagator step, FP32, banded bond kernel */
+#define SDFT_KP_PROP_FP64 1      /* one propagator step, FP64, dense (exact) bond operator */
+#define SDFT_KP_CONV_FWD_FP32 2  /* weighted-density convolutions, FP32 */
+#define SDFT_KP_CONV_FWD_FP64 3
+#define SDFT_KP_CONV_BACK_FP32 4 /* back-projection convolutions, FP32 */
+#define SDFT_KP_CONV_BACK_FP64 5
+#define SDFT_KP_POINTWISE_FP64 6 /* FMT/MSA derivatives (FP64 arithmetic) on FP64 fields */
+
+/* Re-launch one kernel class `reps` times for every point on the context's
+ * stream, bracketed by HIP events, after a solve or objective has filled the
+ * fields.  Returns the average duration of one launch and its algorithmic
+ * flop count (2 flops per multiply-add of the operator actually applied).
+ * Profiling aid; no counterpart in the reference. */
+int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kernel_class, int32_t reps, double* avg_ms, double* flops);
 
 #ifdef __cplusplus
 }

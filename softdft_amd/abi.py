@@ -28,7 +28,9 @@ FIELD_RHO_BEADS, FIELD_RHO_BONDS, FIELD_X, FIELD_FX = 1, 2, 3, 4
 
 EXPORTED = ("sdft_abi_version", "sdft_last_error", "sdft_create", "sdft_destroy", "sdft_solution_length",
             "sdft_solution_offsets", "sdft_objective", "sdft_solve", "sdft_observables", "sdft_get_field",
-            "sdft_last_timing")
+            "sdft_last_timing", "sdft_kernel_profile")
+KP_PROP_FP32, KP_PROP_FP64, KP_CONV_FWD_FP32, KP_CONV_FWD_FP64, KP_CONV_BACK_FP32, KP_CONV_BACK_FP64, \
+    KP_POINTWISE_FP64 = range(7)
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
@@ -92,6 +94,8 @@ def load_library(path: str = LIB_PATH):
     lib.sdft_get_field.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _f64p, C.c_int64]
     lib.sdft_last_timing.restype = C.c_int32
     lib.sdft_last_timing.argtypes = [C.c_void_p, _f64p, _f64p]
+    lib.sdft_kernel_profile.restype = C.c_int32
+    lib.sdft_kernel_profile.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _f64p, _f64p]
     if lib.sdft_abi_version() != SDFT_ABI_VERSION:
         raise RuntimeError("libsoftdft_hip ABI version mismatch")
     _lib = lib
@@ -268,6 +272,14 @@ class Batch:
         it = C.c_double()
         self._check(self.lib.sdft_last_timing(self.h, C.byref(ms), C.byref(it)))
         return ms.value, it.value
+
+    def kernel_profile(self, kernel_class: int, reps: int = 20):
+        """(average ms per launch, algorithmic flops per launch) measured with HIP events on the
+        context's stream."""
+        ms = C.c_double()
+        fl = C.c_double()
+        self._check(self.lib.sdft_kernel_profile(self.h, kernel_class, reps, C.byref(ms), C.byref(fl)))
+        return ms.value, fl.value
 
     def observables(self):
         P = len(self.points)

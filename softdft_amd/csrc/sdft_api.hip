@@ -37,6 +37,7 @@ struct sdft_ctx {
     int hard_walls = 0, charged_walls = 0;
     std::vector<double> valences;
     double lB = 0.0;
+    int fwd_terms = 0, back_terms = 0;   // convolution terms per point (algorithmic flop count)
 };
 
 static thread_local std::string g_last_error;
@@ -297,7 +298,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         DevPoint& q = ctx->pts[p];
         memset(&q, 0, sizeof(q));
         const int NP = d->NP[p];
-        if (NP < 4 || !(d->dz[p] > 0.0)) {
+        if (NP < 4 || NP > SDFT_MAX_TILES * 256 || !(d->dz[p] > 0.0)) {
             set_error(ctx, SDFT_E_ARG, "point %d: NP=%d dz=%g", p, NP, d->dz[p]);
             return fail(SDFT_E_ARG);
         }
@@ -363,6 +364,8 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     build_jobs(d, fwd, back);
     c.n_fwd_jobs = (int)fwd.size();
     c.n_back_jobs = (int)back.size();
+    for (const ConvJob& j : fwd) ctx->fwd_terms += j.nterms;
+    for (const ConvJob& j : back) ctx->back_terms += j.nterms;
     std::vector<ConvJob> jobs(fwd);
     jobs.insert(jobs.end(), back.begin(), back.end());
 
@@ -444,10 +447,11 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     if ((rc = dalloc(ctx, &c.d_evalflag, P)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_errflag, P)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_acc_species, (int64_t)P * 2 * SDFT_MAX_SPECIES)) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &c.d_acc_part, (int64_t)P * SDFT_MAX_TILES * 2 * SDFT_MAX_SPECIES)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_fixed_lambda, (int64_t)P * SDFT_MAX_SPECIES)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_x, c.total_n)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_fx, c.total_n)) != SDFT_OK) return fail(rc);
-    if ((rc = dalloc(ctx, &c.d_gram, (int64_t)P * (SDFT_MAX_M * SDFT_MAX_M + SDFT_MAX_M))) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &c.d_gram, (int64_t)P * SDFT_GRAM_BLOCKS * SDFT_GRAM_LEN)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_errbits, P)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_state, P)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_n_active, 1)) != SDFT_OK) return fail(rc);
@@ -652,15 +656,17 @@ extern "C" int32_t sdft_observables(sdft_ctx_t* ctx, double* omega, double* alph
     Ctx& c = ctx->c;
     const int P = c.P, NB = c.NB;
     double* d_om = nullptr;
-    TRY(dalloc(ctx, &d_om, P));
+    TRY(dalloc(ctx, &d_om, (int64_t)P * SDFT_OMEGA_BLOCKS));
     ctx->p32.omega(ctx->stream, c, c.d_x, d_om);
     ctx->p64.omega(ctx->stream, c, c.d_x, d_om);
-    std::vector<double> om(P), accs((size_t)P * 2 * SDFT_MAX_SPECIES);
+    std::vector<double> omp((size_t)P * SDFT_OMEGA_BLOCKS), om(P, 0.0), accs((size_t)P * 2 * SDFT_MAX_SPECIES);
     std::vector<DevSolveState> st(P);
-    HIP_CHECK_RET(hipMemcpyAsync(om.data(), d_om, P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK_RET(hipMemcpyAsync(omp.data(), d_om, omp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_CHECK_RET(hipMemcpyAsync(accs.data(), c.d_acc_species, accs.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_CHECK_RET(hipMemcpyAsync(st.data(), c.d_state, st.size() * sizeof(DevSolveState), hipMemcpyDeviceToHost, ctx->stream));
     HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    for (int p = 0; p < P; ++p)
+        for (int b = 0; b < SDFT_OMEGA_BLOCKS; ++b) om[p] += omp[(size_t)p * SDFT_OMEGA_BLOCKS + b];
     if (omega) {
         for (int p = 0; p < P; ++p) {
             double Nsum = 0.0;
@@ -728,6 +734,89 @@ extern "C" int32_t sdft_observables(sdft_ctx_t* ctx, double* omega, double* alph
             p_cvt[2 * p] = l;
             p_cvt[2 * p + 1] = r;
         }
+    }
+    return SDFT_OK;
+}
+
+extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps, double* avg_ms, double* flops) {
+    if (!ctx || reps < 1) return SDFT_E_ARG;
+    if (!ctx->have_eval) {
+        set_error(ctx, SDFT_E_STATE, "kernel profiling needs a prior sdft_solve or sdft_objective");
+        return SDFT_E_STATE;
+    }
+    Ctx& c = ctx->c;
+    const int prec = (kc == SDFT_KP_PROP_FP32 || kc == SDFT_KP_CONV_FWD_FP32 || kc == SDFT_KP_CONV_BACK_FP32) ? 32 : 64;
+    // pick the longest chain species for the propagator classes
+    int u = 0;
+    for (int k = 0; k < c.nsp; ++k)
+        if (c.species[k].nseg > c.species[u].nseg) u = k;
+    if ((kc == SDFT_KP_PROP_FP32 || kc == SDFT_KP_PROP_FP64) && c.species[u].nseg < 2) {
+        set_error(ctx, SDFT_E_ARG, "no chain species to profile");
+        return SDFT_E_ARG;
+    }
+    // scratch output so the solution buffers are untouched
+    double* d_fx = nullptr;
+    TRY(dalloc(ctx, &d_fx, c.total_n));
+    std::vector<int32_t> saved(c.P);
+    HIP_CHECK_RET(hipMemcpyAsync(saved.data(), c.d_evalflag, c.P * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    launch_set_evalflag(ctx->stream, c.d_evalflag, prec, c.P);
+    const int s_mid = c.species[u].nseg / 2 > 0 ? c.species[u].nseg / 2 : 1;
+    auto once = [&]() {
+        switch (kc) {
+            case SDFT_KP_PROP_FP32: ctx->p32.launch_prop_step(ctx->stream, c, 32, u, s_mid, c.band_fp32 != 0); break;
+            case SDFT_KP_PROP_FP64: ctx->p64.launch_prop_step(ctx->stream, c, 64, u, s_mid, false); break;
+            case SDFT_KP_CONV_FWD_FP32: ctx->p32.launch_conv(ctx->stream, c, 32, false, d_fx); break;
+            case SDFT_KP_CONV_FWD_FP64: ctx->p64.launch_conv(ctx->stream, c, 64, false, d_fx); break;
+            case SDFT_KP_CONV_BACK_FP32: ctx->p32.launch_conv(ctx->stream, c, 32, true, d_fx); break;
+            case SDFT_KP_CONV_BACK_FP64: ctx->p64.launch_conv(ctx->stream, c, 64, true, d_fx); break;
+            default: ctx->p64.launch_pointwise(ctx->stream, c, 64); break;
+        }
+    };
+    once();  // warm
+    hipEvent_t e0, e1;
+    HIP_CHECK_RET(hipEventCreate(&e0));
+    HIP_CHECK_RET(hipEventCreate(&e1));
+    HIP_CHECK_RET(hipEventRecord(e0, ctx->stream));
+    for (int r = 0; r < reps; ++r) once();
+    HIP_CHECK_RET(hipEventRecord(e1, ctx->stream));
+    HIP_CHECK_RET(hipEventSynchronize(e1));
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIP_CHECK_RET(hipMemcpyAsync(c.d_evalflag, saved.data(), c.P * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    HIP_CHECK_RET(hipGetLastError());
+    if (avg_ms) *avg_ms = ms / reps;
+    if (flops) {
+        // algorithmic flops: 2 per multiply-add of the operator applied
+        std::vector<int32_t> band(ctx->nsets);
+        HIP_CHECK_RET(hipMemcpy(band.data(), c.d_band, ctx->nsets * sizeof(int32_t), hipMemcpyDeviceToHost));
+        double f = 0.0;
+        const int S = c.species[u].S;
+        int npair = 0;
+        {
+            // valid (a, b) pairs at the profiled step: states of the two segments involved
+            const int N = c.species[u].nseg;
+            const int so = c.species[u].state_off;
+            auto cnt = [&](int seg) { int k = 0; for (int a = 0; a < S; ++a) if (ctx->seg_states_h[so + seg * S + a] >= 0) ++k; return k; };
+            npair = cnt(N - 1 - s_mid) * cnt(N - s_mid);  // child sweep; parent sweep similar
+            (void)N;
+        }
+        for (int p = 0; p < c.P; ++p) {
+            const double NP = ctx->pts[p].NP;
+            if (kc == SDFT_KP_PROP_FP32 || kc == SDFT_KP_PROP_FP64) {
+                const bool bd = kc == SDFT_KP_PROP_FP32 && c.band_fp32;
+                const double taps = bd ? std::min(NP, 2.0 * band[ctx->pts[p].kset] + 1.0) : NP;
+                f += 2.0 /*sweeps*/ * npair * 2.0 * NP * taps;
+            } else if (kc == SDFT_KP_POINTWISE_FP64) {
+                f += 0.0;
+            } else {
+                const bool back = kc == SDFT_KP_CONV_BACK_FP32 || kc == SDFT_KP_CONV_BACK_FP64;
+                f += (back ? ctx->back_terms : ctx->fwd_terms) * 2.0 * NP * NP;
+            }
+        }
+        *flops = f;
     }
     return SDFT_OK;
 }

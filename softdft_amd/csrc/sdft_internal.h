@@ -16,6 +16,10 @@
 #define SDFT_MAX_S 2
 #define SDFT_MAX_SPECIES 16
 #define SDFT_MAX_M 8
+#define SDFT_MAX_TILES 128          // 256-point tiles per grid (NP <= 32768)
+#define SDFT_GRAM_BLOCKS 64         // blocks per point for the Anderson reductions
+#define SDFT_GRAM_LEN 44            // SDFT_MAX_M*(SDFT_MAX_M+1)/2 + SDFT_MAX_M
+#define SDFT_OMEGA_BLOCKS 8
 
 // kernel-vector index inside a kernel set
 //   3*j + 0 : w2 (delta shell, R = sigma_j / 2)        even

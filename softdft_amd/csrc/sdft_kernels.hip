@@ -32,8 +32,10 @@ __device__ __forceinline__ double warp_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+// NaN-propagating max (fmax would drop a NaN residual)
+__device__ __forceinline__ double nanmax(double a, double b) { return (a > b || a != a) ? a : b; }
 __device__ __forceinline__ double warp_max(double v) {
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, 64));
     return v;
 }
 
@@ -128,14 +130,11 @@ __global__ void __launch_bounds__(256) k_build_cvec(const DevSystem* __restrict_
 template <typename T>
 __global__ void k_prepare(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
                           const int32_t* __restrict__ evalflag, int prec, const double* __restrict__ x,
-                          const double* __restrict__ Ext, T* __restrict__ elam, T* __restrict__ hl,
-                          double* __restrict__ acc_species) {
+                          const double* __restrict__ Ext, T* __restrict__ elam, T* __restrict__ hl) {
     const DevSystem& sys = *psys;
     const int p = blockIdx.y;
     if (evalflag[p] != prec) return;
     const DevPoint& P = pts[p];
-    if (blockIdx.x == 0 && threadIdx.x < 2 * sys.n_species)
-        acc_species[(int64_t)p * 2 * SDFT_MAX_SPECIES + threadIdx.x] = 0.0;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= P.NP) return;
     const int NB = sys.NB, MB = sys.MB, NP = P.NP;
@@ -415,7 +414,7 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
                                                    const T* __restrict__ gC, const T* __restrict__ gP,
                                                    const double* __restrict__ fixed_lambda,
                                                    T* __restrict__ rho_beads, T* __restrict__ rho_bonds, T* __restrict__ rhoT,
-                                                   double* __restrict__ acc_species, int32_t* __restrict__ errflag) {
+                                                   double* __restrict__ acc_part, int32_t* __restrict__ errflag) {
     const DevSystem& sys = *psys;
     const int p = blockIdx.y;
     if (evalflag[p] != prec) return;
@@ -517,10 +516,11 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
         }
     }
     __syncthreads();
-    if (threadIdx.x < 2 * sys.n_species) {
+    // per-block partial sums, reduced in a fixed order by k_neutrality (deterministic, batch-invariant)
+    if (threadIdx.x < 2 * sys.n_species && blockIdx.x < SDFT_MAX_TILES) {
         double s = 0.0;
         for (int w = 0; w < TB / 64; ++w) s += red[threadIdx.x][w];
-        atomicAdd(&acc_species[(int64_t)p * 2 * SDFT_MAX_SPECIES + threadIdx.x], s);
+        acc_part[((int64_t)p * SDFT_MAX_TILES + blockIdx.x) * 2 * SDFT_MAX_SPECIES + threadIdx.x] = s;
     }
     if (!valid) return;
     for (int j = 0; j < NB; ++j) {
@@ -872,7 +872,7 @@ __global__ void __launch_bounds__(256) k_pointwise(const DevSystem* __restrict__
 // --------------------------------------------------------------------------
 __global__ void k_neutrality(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
                              const DevSpecies* __restrict__ spec, const int32_t* __restrict__ evalflag, int prec,
-                             const double* __restrict__ acc_species, const double* __restrict__ x,
+                             const double* __restrict__ acc_part, double* __restrict__ acc_species, const double* __restrict__ x,
                              double* __restrict__ fx, int P_count) {
     const DevSystem& sys = *psys;
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -881,6 +881,12 @@ __global__ void k_neutrality(const DevSystem* __restrict__ psys, const DevPoint*
     const int nsp = sys.n_species;
     double Num[SDFT_MAX_SPECIES], qs[SDFT_MAX_SPECIES];
     double Qtot = P.total_charge;
+    const int ntiles = (P.NP + 255) / 256;
+    for (int k = 0; k < 2 * nsp; ++k) {
+        double t = 0.0;
+        for (int b = 0; b < ntiles; ++b) t += acc_part[((int64_t)p * SDFT_MAX_TILES + b) * 2 * SDFT_MAX_SPECIES + k];
+        acc_species[(int64_t)p * 2 * SDFT_MAX_SPECIES + k] = t;
+    }
     for (int u = 0; u < nsp; ++u) {
         Num[u] = acc_species[(int64_t)p * 2 * SDFT_MAX_SPECIES + 2 * u];
         const double ch = acc_species[(int64_t)p * 2 * SDFT_MAX_SPECIES + 2 * u + 1];
@@ -926,8 +932,6 @@ __global__ void k_and_residual(const DevPoint* __restrict__ pts, int32_t* __rest
     const DevSolveState& S = stt[p];
     if (S.converged) return;
     const DevPoint& P = pts[p];
-    if (blockIdx.x == 0 && threadIdx.x < SDFT_MAX_M * SDFT_MAX_M + SDFT_MAX_M)
-        gram[(int64_t)p * (SDFT_MAX_M * SDFT_MAX_M + SDFT_MAX_M) + threadIdx.x] = 0.0;
     const int col = S.iter % m;
     const int64_t hoff = P.xoff * m;
     double mx = 0.0;
@@ -945,7 +949,7 @@ __global__ void k_and_residual(const DevPoint* __restrict__ pts, int32_t* __rest
     __syncthreads();
     if (threadIdx.x == 0) {
         double v = 0.0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, red[w]);
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v = nanmax(v, red[w]);
         if (v != v) v = INFINITY;
         atomicMax(&errbits[p], (unsigned long long)__double_as_longlong(v));
     }
@@ -993,7 +997,7 @@ __global__ void k_and_gram(const DevPoint* __restrict__ pts, const DevSolveState
     if ((int)threadIdx.x < ntri + nc) {
         double s = 0.0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[threadIdx.x][w];
-        atomicAdd(&gram[(int64_t)p * (SDFT_MAX_M * SDFT_MAX_M + SDFT_MAX_M) + threadIdx.x], s);
+        gram[((int64_t)p * SDFT_GRAM_BLOCKS + blockIdx.x) * SDFT_GRAM_LEN + threadIdx.x] = s;
     }
 }
 
@@ -1087,8 +1091,13 @@ __global__ void k_and_solve(DevSolveState* __restrict__ stt, int32_t* __restrict
     } else {
         S.use_picard = 0;
         const int nc = (local < m ? local : m) - 1;
-        const double* gp = gram + (int64_t)p * (SDFT_MAX_M * SDFT_MAX_M + SDFT_MAX_M);
         const int ntri = nc * (nc + 1) / 2;
+        double gp[SDFT_GRAM_LEN];
+        for (int k = 0; k < ntri + nc; ++k) {
+            double t = 0.0;
+            for (int b = 0; b < SDFT_GRAM_BLOCKS; ++b) t += gram[((int64_t)p * SDFT_GRAM_BLOCKS + b) * SDFT_GRAM_LEN + k];
+            gp[k] = t;
+        }
         double rhs[SDFT_MAX_M];
         for (int a = 0; a < nc; ++a) rhs[a] = gp[ntri + a];
         small_lstsq(nc, gp, rhs, S.lam);
@@ -1211,7 +1220,7 @@ __global__ void __launch_bounds__(256) k_omega(const DevSystem* __restrict__ psy
     if (threadIdx.x == 0) {
         double s = 0.0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
-        atomicAdd(&omega_acc[p], s);
+        omega_acc[(int64_t)p * gridDim.x + blockIdx.x] = s;
     }
 }
 
@@ -1305,7 +1314,7 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
     const int P = c.P;
     const int maxNP = c.maxNP;
     hipLaunchKernelGGL(k_prepare<T>, grid2(maxNP, 256, P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag, prec, x,
-                       c.d_Ext, elam, hl, c.d_acc_species);
+                       c.d_Ext, elam, hl);
     for (int u = 0; u < c.nsp; ++u) {
         const SpeciesHost& sh = c.species[u];
         PropArgs A;
@@ -1324,7 +1333,7 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
     }
     hipLaunchKernelGGL((k_accumulate<T, 256>), grid2(maxNP, 256, P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag,
                        prec, c.d_spec, c.d_goffs, c.GF, c.d_seg_states, c.d_maxd, elam, trapez, gC, gP,
-                       c.d_fixed_lambda, rho_beads, rho_bonds, rhoT, c.d_acc_species, c.d_errflag);
+                       c.d_fixed_lambda, rho_beads, rho_bonds, rhoT, c.d_acc_part, c.d_errflag);
     if (c.n_fwd_jobs > 0)
         hipLaunchKernelGGL((k_conv<T, 256>), grid2(maxNP, 256, c.n_fwd_jobs, P), dim3(256), 0, st, c.d_sys, c.d_pts,
                            c.d_evalflag, prec, c.d_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx);
@@ -1333,12 +1342,39 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
     hipLaunchKernelGGL((k_conv<T, 256>), grid2(maxNP, 256, c.n_back_jobs, P), dim3(256), 0, st, c.d_sys, c.d_pts,
                        c.d_evalflag, prec, c.d_jobs + c.n_fwd_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx);
     hipLaunchKernelGGL(k_neutrality, dim3((P + 63) / 64), dim3(64), 0, st, c.d_sys, c.d_pts, c.d_spec, c.d_evalflag,
-                       prec, c.d_acc_species, x, fx, P);
+                       prec, c.d_acc_part, c.d_acc_species, x, fx, P);
+}
+
+template <typename T>
+void Pipeline<T>::launch_prop_step(hipStream_t st, const Ctx& c, int prec, int u, int s, bool banded) {
+    const SpeciesHost& sh = c.species[u];
+    PropArgs A;
+    A.u = u; A.nseg = sh.nseg; A.S = sh.S; A.fixed_seg = sh.fixed_seg;
+    A.state_off = sh.state_off; A.maxd_off = sh.maxd_off; A.goff = sh.goff; A.GF = c.GF;
+    A.band = banded ? 1 : -1;
+    A.s = s;
+    hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(c.maxNP, 256, c.P, 2), dim3(256), 0, st, c.d_sys, c.d_pts,
+                       c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl, trapez, gC, gP, c.d_band);
+}
+
+template <typename T>
+void Pipeline<T>::launch_conv(hipStream_t st, const Ctx& c, int prec, bool back, double* fx) {
+    const int nj = back ? c.n_back_jobs : c.n_fwd_jobs;
+    if (nj == 0) return;
+    hipLaunchKernelGGL((k_conv<T, 256>), grid2(c.maxNP, 256, nj, c.P), dim3(256), 0, st, c.d_sys, c.d_pts,
+                       c.d_evalflag, prec, c.d_jobs + (back ? c.n_fwd_jobs : 0), kern, rhoT, c.Frt, dphi, c.Fdp, nw,
+                       c.Fnw, fx);
+}
+
+template <typename T>
+void Pipeline<T>::launch_pointwise(hipStream_t st, const Ctx& c, int prec) {
+    hipLaunchKernelGGL(k_pointwise<T>, grid2(c.maxNP, 256, c.P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag,
+                       prec, nw, c.Fnw, rhoT, c.Frt, c.d_surf, dphi, c.Fdp);
 }
 
 template <typename T>
 void Pipeline<T>::omega(hipStream_t st, const Ctx& c, const double* x, double* omega_acc) {
-    hipLaunchKernelGGL(k_omega<T>, dim3(8, c.P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_state, nw, c.Fnw, rhoT, c.Frt,
+    hipLaunchKernelGGL(k_omega<T>, dim3(SDFT_OMEGA_BLOCKS, c.P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_state, nw, c.Fnw, rhoT, c.Frt,
                        rho_bonds, x, omega_acc);
 }
 
@@ -1354,7 +1390,7 @@ template struct Pipeline<double>;
 
 void launch_anderson(hipStream_t st, const Ctx& c, const SolveParams& sp) {
     const int P = c.P;
-    const int tiles = (int)std::min<int64_t>((c.max_n + 255) / 256, 64);
+    const int tiles = SDFT_GRAM_BLOCKS;  // fixed per point: the reductions do not depend on the batch
     hipLaunchKernelGGL(k_and_residual, dim3(tiles, P), dim3(256), 0, st, c.d_pts, c.d_evalflag, c.d_state, c.d_x, c.d_fx,
                        c.d_Xh, c.d_Gh, c.d_errbits, c.d_gram, sp.m, c.d_n_active);
     hipLaunchKernelGGL(k_and_gram, dim3(tiles, P), dim3(256), 0, st, c.d_pts, c.d_state, c.d_Xh, c.d_Gh, c.d_gram, sp.m);

@@ -41,7 +41,8 @@ struct Ctx {
     double* d_surf = nullptr;       // (sumNP)
     int32_t* d_evalflag = nullptr;  // per point: 0 / 32 / 64 = evaluate in that precision
     int32_t* d_errflag = nullptr;
-    double* d_acc_species = nullptr;   // [P][2*MAX_SPECIES]
+    double* d_acc_species = nullptr;   // [P][2*MAX_SPECIES] totals (written by k_neutrality)
+    double* d_acc_part = nullptr;      // [P][MAX_TILES][2*MAX_SPECIES] per-tile partial sums
     double* d_fixed_lambda = nullptr;  // [P][MAX_SPECIES]
     // solve
     double* d_x = nullptr;
@@ -70,6 +71,10 @@ struct Pipeline {
     void objective(hipStream_t st, const Ctx& c, int prec, const double* x, double* fx);
     void omega(hipStream_t st, const Ctx& c, const double* x, double* omega_acc);
     void segment_sums(hipStream_t st, const Ctx& c, int u, double* out, int64_t stride);
+    // single-launch entry points used by sdft_kernel_profile
+    void launch_prop_step(hipStream_t st, const Ctx& c, int prec, int u, int s, bool banded);
+    void launch_conv(hipStream_t st, const Ctx& c, int prec, bool back, double* fx);
+    void launch_pointwise(hipStream_t st, const Ctx& c, int prec);
 };
 
 void launch_build_kernels(hipStream_t st, const DevSystem* sys, int K, int nsets, int maxL, const int32_t* set_L,

@@ -1,0 +1,152 @@
+"""GPU parity of libsoftdft_hip against the FP64 oracle, through the C ABI.
+
+Tolerances (relative to max |oracle|):
+  * FP64 objective F(x):                1e-11  (same arithmetic, different summation order)
+  * FP32 objective F(x):                5e-5   (single-precision convolutions and propagators)
+  * converged fields / Omega, FP64:     1e-7 abs / 1e-8 rel (both solved to ||F(x)-x||_inf < 1e-8)
+  * converged Omega, mixed precision:   1e-8   (the benchmark's acceptance bound)
+  * degree of ionization / CVT:         1e-8 absolute
+"""
+import numpy as np
+import pytest
+
+from softdft_amd import abi, configs
+from softdft_amd.problem import build_state_point
+from oracle.dft_oracle import OracleDFT, anderson_solve
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems():
+    return {
+        "chains": configs.hard_sphere_chains(H=6.0, dz=0.05, n_seg=5),
+        "weak_pe": configs.weak_polyelectrolyte(H=6.0, dz=0.05, n_seg=6, solvent=0.2, pH=5.0),
+        "weak_pe_charged_walls": configs.weak_polyelectrolyte(H=5.0, dz=0.05, n_seg=4, solvent=0.2, pH=6.0,
+                                                              wall_charge=(-0.05, -0.02)),
+        "mfmt_msa": configs.weak_polyelectrolyte(H=5.0, dz=0.05, n_seg=4, solvent=0.2, pH=4.0, msa="msa"),
+        "brush": configs.weak_polyelectrolyte(H=8.0, dz=0.05, n_seg=10, pH=6.0, pKa=5.0, grafted=0.05),
+    }
+
+
+SYSTEMS = _systems()
+
+
+def _mfmt_variant(ds):
+    ds = dict(ds)
+    ds["model"] = ["ideal", "mfmt", "msa", "coul"]
+    return ds
+
+
+SYSTEMS["mfmt_msa"] = _mfmt_variant(SYSTEMS["mfmt_msa"])
+
+
+@pytest.mark.parametrize("name", list(SYSTEMS))
+def test_objective_fp64_matches_oracle(name):
+    sp = build_state_point(SYSTEMS[name])
+    b = abi.Batch([sp])
+    o = OracleDFT(sp)
+    rng = np.random.default_rng(7)
+    x0 = b.x0()
+    for x in (x0, x0 + 0.05 * rng.standard_normal(len(x0))):
+        print(f"case objective_fp64 {name} n={len(x)}", flush=True)
+        fx = b.objective(x, abi.PREC_FP64)
+        ref = o.objective(x)
+        scale = max(1.0, float(np.max(np.abs(ref))))
+        assert np.max(np.abs(fx - ref)) / scale < 1e-11
+        rb = b.get_field(abi.FIELD_RHO_BEADS, 0)
+        assert np.allclose(rb, o.rho_beads, rtol=1e-11, atol=1e-14)
+    b.close()
+
+
+@pytest.mark.parametrize("name", ["chains", "weak_pe", "brush"])
+def test_objective_fp32_is_close(name):
+    sp = build_state_point(SYSTEMS[name])
+    b = abi.Batch([sp])
+    x0 = b.x0()
+    print(f"case objective_fp32 {name}", flush=True)
+    fx = b.objective(x0, abi.PREC_FP32)
+    ref = OracleDFT(sp).objective(x0)
+    assert np.max(np.abs(fx - ref)) / max(1.0, float(np.max(np.abs(ref)))) < 5e-5
+    b.close()
+
+
+@pytest.mark.parametrize("name", ["chains", "weak_pe", "weak_pe_charged_walls", "brush"])
+def test_solve_matches_oracle(name):
+    sp = build_state_point(SYSTEMS[name])
+    b = abi.Batch([sp])
+    o = OracleDFT(sp)
+    num = sp.numerics
+    print(f"case solve {name}", flush=True)
+    x, it, err = b.solve(b.x0(), tole=num["tole"], damping=num["damping"], mixing_max=num["mixing_max"],
+                         switch_tole=num["switch_tole"], anderm=int(num["anderm"]), max_iters=3000)
+    om, al, pc = b.observables()
+    r = anderson_solve(o.objective, sp.x0(), num)
+    om_ref = o.grand_potential(r.x)
+    assert err[0] < num["tole"]
+    # both sides stop at ||F(x)-x||_inf < 1e-8, so converged fields agree to ~that and Omega to ~1e-9
+    assert np.max(np.abs(x - r.x)) < 1e-7
+    assert abs(om[0] - om_ref) / abs(om_ref) < 1e-8
+    fr = o.state_fractions()
+    for u, f in enumerate(fr):
+        if f is not None:
+            assert np.allclose(al[0, u, :len(f)], f, atol=1e-8)
+    assert np.allclose(pc[0], o.contact_pressure(), atol=1e-8)
+    print(f"case solve mixed {name}", flush=True)
+    x2, it2, err2 = b.solve(b.x0(), tole=num["tole"], damping=num["damping"], mixing_max=num["mixing_max"],
+                            switch_tole=num["switch_tole"], anderm=int(num["anderm"]), max_iters=3000,
+                            precision=abi.PREC_MIXED)
+    om2, _, _ = b.observables()
+    assert abs(om2[0] - om_ref) / abs(om_ref) < 1e-8
+    b.close()
+
+
+def test_batched_points_equal_individual_solves():
+    dss = [configs.weak_polyelectrolyte(H=5.0, dz=0.05, n_seg=6, solvent=0.2, pH=pH) for pH in (3.0, 4.5, 6.0, 9.0)]
+    sps = [build_state_point(d) for d in dss]
+    b = abi.Batch(sps)
+    print("case batch-vs-single", flush=True)
+    xb, itb, errb = b.solve(b.x0(), max_iters=3000)
+    omb, alb, _ = b.observables()
+    for p, sp in enumerate(sps):
+        bi = abi.Batch([sp])
+        xi, iti, erri = bi.solve(bi.x0(), max_iters=3000)
+        omi, ali, _ = bi.observables()
+        # reductions use a fixed number of blocks per point and a fixed summation order, so a point
+        # solved inside a batch follows exactly the same trajectory as when solved alone
+        assert iti[0] == itb[p]
+        assert np.array_equal(xi, b.split(xb)[p])
+        assert omi[0] == omb[p]
+        bi.close()
+    b.close()
+
+
+def test_ragged_batch_of_slit_widths():
+    dss = [configs.weak_polyelectrolyte(H=H, dz=0.05, n_seg=4, solvent=0.2, pH=7.0) for H in (3.0, 4.1, 5.55)]
+    sps = [build_state_point(d) for d in dss]
+    assert len({sp.NP for sp in sps}) == 3
+    b = abi.Batch(sps)
+    print("case ragged", flush=True)
+    x0 = b.x0()
+    fx = b.objective(x0)
+    for p, sp in enumerate(sps):
+        ref = OracleDFT(sp).objective(sp.x0())
+        assert np.max(np.abs(b.split(fx)[p] - ref)) / max(1.0, np.max(np.abs(ref))) < 1e-11
+    x, it, err = b.solve(x0, max_iters=3000, precision=abi.PREC_MIXED)
+    assert np.all(err < 1e-8)
+    b.close()
+
+
+def test_nonconvergence_and_nonfinite_inputs_are_reported():
+    sp = build_state_point(SYSTEMS["weak_pe"])
+    b = abi.Batch([sp])
+    print("case max_iters", flush=True)
+    with pytest.raises(abi.SdftError) as e:
+        b.solve(b.x0(), max_iters=3)
+    assert e.value.code == -4
+    x = b.x0()
+    x[5] = np.nan
+    print("case nan", flush=True)
+    with pytest.raises(abi.SdftError) as e:
+        b.solve(x, max_iters=50)
+    assert e.value.code == -4
+    b.close()

@@ -1,0 +1,60 @@
+"""CPU-side checks of the C-ABI library: it loads, exports every entry point the
+header declares, validates descriptors before touching a device, and fails
+loudly (no silent CPU path) when no HIP device is present."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from softdft_amd import abi, configs
+from softdft_amd.problem import build_state_point
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "softdft_hip.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(sdft_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_entry_point():
+    lib = abi.load_library()
+    names = _declared()
+    assert set(abi.EXPORTED) <= set(names)
+    for n in names:
+        assert hasattr(lib, n), n
+    assert lib.sdft_abi_version() == abi.SDFT_ABI_VERSION
+
+
+def test_descriptor_validation_happens_before_device_use():
+    lib = abi.load_library()
+    d = abi.SdftDesc()
+    d.abi_version = 999
+    h = C.c_void_p()
+    assert lib.sdft_create(C.byref(d), C.byref(h)) == -1
+    assert b"abi_version" in lib.sdft_last_error(None)
+    d.abi_version = abi.SDFT_ABI_VERSION
+    d.n_points = 0
+    assert lib.sdft_create(C.byref(d), C.byref(h)) == -1
+    d.n_points, d.NB, d.MB, d.n_species = 1, 2, 1, 1
+    d.functionals = abi.F_MFMT | abi.F_TPT1MFMT
+    assert lib.sdft_create(C.byref(d), C.byref(h)) == -5
+    assert lib.sdft_solution_length(None) == -1
+
+
+def test_batch_requires_a_shared_layout():
+    a = build_state_point(configs.hard_sphere_chains(H=3.0, dz=0.05, n_seg=3))
+    b = build_state_point(configs.weak_polyelectrolyte(H=3.0, dz=0.05, n_seg=3))
+    with pytest.raises(ValueError):
+        abi.Batch([a, b])
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a HIP device is present")
+def test_no_device_fails_loudly():
+    sp = build_state_point(configs.hard_sphere_chains(H=3.0, dz=0.05, n_seg=3))
+    with pytest.raises(abi.SdftError) as e:
+        abi.Batch([sp])
+    assert e.value.code == -2
