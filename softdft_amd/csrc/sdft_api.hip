@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <map>
 #include <tuple>
 
@@ -154,6 +155,173 @@ static int build_jobs(const sdft_desc_t* d, std::vector<ConvJob>& fwd, std::vect
     ConvJob psi = job(1, NB + 2 * MB, 0, 1.0);
     add(psi, 3 * NB + MB, 0, 1, 3 * NB + 2 * MB);
     back.push_back(psi);
+    return SDFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// MFMA operator-GEMM plans: group columns that share a kernel set and the same
+// (canonical) operator terms, then cut each group into row x column tiles.
+// ---------------------------------------------------------------------------
+template <typename V>
+static int upload_vec(sdft_ctx* ctx, void** dst, const std::vector<V>& v) {
+    V* d = nullptr;
+    TRY(upload(ctx, &d, v.data(), (int64_t)v.size()));
+    *dst = d;
+    return SDFT_OK;
+}
+
+static int build_conv_plan(sdft_ctx* ctx, const std::vector<ConvJob>& jobs, int job_base, int njobs,
+                           const std::vector<std::vector<int>>& canon, const std::vector<int>& set_points,
+                           const std::vector<int32_t>& set_L, const std::vector<int64_t>& set_kbase, OpPlan& plan) {
+    Ctx& c = ctx->c;
+    struct Key {
+        int kset, nterms, kern[3], sign[3];
+        bool operator<(const Key& o) const {
+            if (kset != o.kset) return kset < o.kset;
+            if (nterms != o.nterms) return nterms < o.nterms;
+            for (int i = 0; i < 3; ++i) {
+                if (kern[i] != o.kern[i]) return kern[i] < o.kern[i];
+                if (sign[i] != o.sign[i]) return sign[i] < o.sign[i];
+            }
+            return false;
+        }
+    };
+    std::map<Key, std::vector<OpCol>> groups;
+    for (int p = 0; p < c.P; ++p) {
+        const int s = ctx->pts[p].kset;
+        for (int j = 0; j < njobs; ++j) {
+            const ConvJob& J = jobs[job_base + j];
+            Key k;
+            memset(&k, 0, sizeof(k));
+            k.kset = s;
+            k.nterms = J.nterms;
+            for (int t = 0; t < J.nterms; ++t) {
+                k.kern[t] = canon[s][J.t[t].kern];
+                k.sign[t] = J.t[t].odd == 2 ? -1 : 1;
+            }
+            groups[k].push_back(OpCol{p, job_base + j});
+        }
+    }
+    (void)set_points;
+    // tile shape: wide (64 x 64) when groups are large, tall (128 x 16) otherwise
+    size_t tot = 0;
+    for (auto& g : groups) tot += g.second.size();
+    plan.wide = groups.empty() ? true : (tot / groups.size() >= 48);
+    const int TM = opgemm_tile_rows(plan.wide), TN = opgemm_tile_cols(plan.wide);
+    std::vector<OpGroup> gv;
+    std::vector<OpCol> cols;
+    std::vector<OpTile> tiles;
+    for (auto& kv : groups) {
+        const Key& k = kv.first;
+        OpGroup g;
+        memset(&g, 0, sizeof(g));
+        g.kset = k.kset;
+        g.L = set_L[k.kset];
+        g.NP = g.L / 2;
+        g.nterms = k.nterms;
+        for (int t = 0; t < 3; ++t) { g.kern[t] = k.kern[t]; g.sign[t] = k.sign[t]; }
+        g.col_off = (int32_t)cols.size();
+        g.ncols = (int32_t)kv.second.size();
+        g.kbase = set_kbase[k.kset];
+        g.band = -1;
+        const int gi = (int)gv.size();
+        gv.push_back(g);
+        cols.insert(cols.end(), kv.second.begin(), kv.second.end());
+        for (int r = 0; r < g.NP; r += TM)
+            for (int c0 = 0; c0 < g.ncols; c0 += TN) tiles.push_back(OpTile{gi, r, c0, std::min(TN, g.ncols - c0)});
+    }
+    plan.ntiles = (int)tiles.size();
+    if (plan.ntiles == 0) return SDFT_OK;
+    TRY(upload_vec(ctx, &plan.d_tiles, tiles));
+    TRY(upload_vec(ctx, &plan.d_groups, gv));
+    TRY(upload_vec(ctx, &plan.d_cols, cols));
+    return SDFT_OK;
+}
+
+static int bond_of(const DevSystem& sys, int MB, int si, int sj) {
+    const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
+    for (int u = 0; u < MB; ++u)
+        if (sys.bond_i[u] == lo && sys.bond_j[u] == hi) return u;
+    return -1;
+}
+
+// Propagator plan of species u: valid only if every (sweep, a, b) column applies the same
+// canonical bond operator at every step (homogeneous chains); otherwise the direct kernel runs.
+static int build_prop_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const std::vector<std::vector<int>>& canon,
+                           const std::vector<int32_t>& set_L, const std::vector<int64_t>& set_kbase,
+                           const std::vector<int32_t>& band, OpPlan& plan) {
+    Ctx& c = ctx->c;
+    const SpeciesHost& sh = c.species[u];
+    const int N = sh.nseg, S = sh.S, NB = c.NB, MB = c.MB;
+    if (N < 2) return SDFT_OK;
+    const int* st = ctx->seg_states_h.data() + sh.state_off;
+    // canonical bond kernel of each (sweep, a, b); -2 = unset, -1 = varies
+    std::vector<int> ck(2 * S * S, -2);
+    for (int sw = 0; sw < 2; ++sw)
+        for (int s = 1; s < N; ++s) {
+            const int outseg = sw == 0 ? N - 1 - s : s;
+            const int inseg = sw == 0 ? outseg + 1 : outseg - 1;
+            for (int a = 0; a < S; ++a)
+                for (int b = 0; b < S; ++b) {
+                    const int si = st[outseg * S + a], sj = st[inseg * S + b];
+                    if (si < 0 || sj < 0) continue;
+                    const int bond = bond_of(sys, MB, si, sj);
+                    if (bond < 0) return SDFT_OK;
+                    for (size_t ks = 0; ks < canon.size(); ++ks) {
+                        const int k = canon[ks][3 * NB + bond];
+                        int& slot = ck[(sw * S + a) * S + b];
+                        if (slot == -2) slot = k;
+                        else if (slot != k) return SDFT_OK;  // not homogeneous: keep the direct kernel
+                    }
+                }
+        }
+    struct Key { int kset, kern; bool operator<(const Key& o) const { return kset != o.kset ? kset < o.kset : kern < o.kern; } };
+    std::map<Key, std::vector<PropCol>> groups;
+    for (int p = 0; p < c.P; ++p)
+        for (int sw = 0; sw < 2; ++sw)
+            for (int a = 0; a < S; ++a)
+                for (int b = 0; b < S; ++b) {
+                    const int k = ck[(sw * S + a) * S + b];
+                    if (k < 0) continue;
+                    groups[Key{ctx->pts[p].kset, k}].push_back(PropCol{p, sw, a, b});
+                }
+    size_t tot = 0;
+    for (auto& g : groups) tot += g.second.size();
+    plan.wide = groups.empty() ? true : (tot / groups.size() >= 48);
+    // fused chaining needs every (sweep, a, b) column, b fastest, in one group per kernel set
+    {
+        bool full = S <= 2;
+        for (int k = 0; k < 2 * S * S; ++k) full = full && ck[k] >= 0 && ck[k] == ck[0];
+        plan.full_pairs = full;
+    }
+    const int TM = opgemm_tile_rows(plan.wide), TN = opgemm_tile_cols(plan.wide);
+    std::vector<OpGroup> gv;
+    std::vector<PropCol> cols;
+    std::vector<OpTile> tiles;
+    for (auto& kv : groups) {
+        OpGroup g;
+        memset(&g, 0, sizeof(g));
+        g.kset = kv.first.kset;
+        g.L = set_L[g.kset];
+        g.NP = g.L / 2;
+        g.nterms = 1;
+        g.kern[0] = kv.first.kern;
+        g.sign[0] = 1;
+        g.col_off = (int32_t)cols.size();
+        g.ncols = (int32_t)kv.second.size();
+        g.kbase = set_kbase[g.kset];
+        g.band = band[g.kset];
+        const int gi = (int)gv.size();
+        gv.push_back(g);
+        cols.insert(cols.end(), kv.second.begin(), kv.second.end());
+        for (int r = 0; r < g.NP; r += TM)
+            for (int c0 = 0; c0 < g.ncols; c0 += TN) tiles.push_back(OpTile{gi, r, c0, std::min(TN, g.ncols - c0)});
+    }
+    plan.ntiles = (int)tiles.size();
+    if (plan.ntiles == 0) return SDFT_OK;
+    TRY(upload_vec(ctx, &plan.d_tiles, tiles));
+    TRY(upload_vec(ctx, &plan.d_groups, gv));
+    TRY(upload_vec(ctx, &plan.d_cols, cols));
     return SDFT_OK;
 }
 
@@ -427,6 +595,29 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         }
         if ((rc = upload(ctx, &c.d_band, band.data(), ctx->nsets)) != SDFT_OK) return fail(rc);
         HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+        // canonical kernel index per set: identical vectors (equal diameters) share one operator
+        std::vector<std::vector<int>> canon(ctx->nsets, std::vector<int>(c.K));
+        for (int s = 0; s < ctx->nsets; ++s) {
+            const int L = set_L[s];
+            for (int k = 0; k < c.K; ++k) {
+                canon[s][k] = k;
+                const double* ck = hk.data() + set_kbase[s] * c.K + (int64_t)k * L;
+                for (int k2 = 0; k2 < k; ++k2) {
+                    const double* c2 = hk.data() + set_kbase[s] * c.K + (int64_t)k2 * L;
+                    if (memcmp(ck, c2, (size_t)L * sizeof(double)) == 0) { canon[s][k] = canon[s][k2]; break; }
+                }
+            }
+        }
+        const char* kenv = getenv("SOFTDFT_KERNELS");
+        c.use_gemm = !(kenv && strcmp(kenv, "direct") == 0);
+        std::vector<int> dummy;
+        if ((rc = build_conv_plan(ctx, jobs, 0, c.n_fwd_jobs, canon, dummy, set_L, set_kbase, c.fwd_plan)) != SDFT_OK)
+            return fail(rc);
+        if ((rc = build_conv_plan(ctx, jobs, c.n_fwd_jobs, c.n_back_jobs, canon, dummy, set_L, set_kbase, c.back_plan)) != SDFT_OK)
+            return fail(rc);
+        c.prop_plan.assign(nsp, OpPlan());
+        for (int u = 0; u < nsp; ++u)
+            if ((rc = build_prop_plan(ctx, sys, u, canon, set_L, set_kbase, band, c.prop_plan[u])) != SDFT_OK) return fail(rc);
     }
 
     // ---- work buffers
@@ -442,6 +633,12 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         TRY(dalloc(ctx, &pp.dphi, c.sumNP * c.Fdp));
         return SDFT_OK;
     };
+    {
+        int Smax = 1;
+        for (const auto& sh : c.species) Smax = std::max(Smax, sh.S);
+        if ((rc = dalloc(ctx, &ctx->p32.fbuf, 2 * c.sumNP * 2 * Smax * Smax)) != SDFT_OK) return fail(rc);
+        if ((rc = dalloc(ctx, &ctx->p64.fbuf, 2 * c.sumNP * 2 * Smax * Smax)) != SDFT_OK) return fail(rc);
+    }
     if ((rc = alloc_pipe(ctx->p32)) != SDFT_OK) return fail(rc);
     if ((rc = alloc_pipe(ctx->p64)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_evalflag, P)) != SDFT_OK) return fail(rc);
@@ -531,6 +728,7 @@ extern "C" int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* o, doub
         c.m = o->anderm;
     }
     c.band_fp32 = o->band_fp32;
+    c.band_fp64 = (prec == SDFT_PREC_MIXED && o->band_fp32) ? 1 : 0;  // the plain FP64 path stays dense
     SolveParams sp;
     sp.tole = o->tole; sp.damping = o->damping; sp.mixing_max = o->mixing_max; sp.switch_tole = o->switch_tole;
     sp.fp32_switch = o->fp32_switch; sp.m = o->anderm; sp.max_iters = o->max_iters;

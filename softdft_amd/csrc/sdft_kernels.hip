@@ -188,7 +188,8 @@ __global__ void __launch_bounds__(TB) k_prop_step(const DevSystem* __restrict__ 
                                                   const int32_t* __restrict__ seg_states, const double* __restrict__ maxd_tab,
                                                   const T* __restrict__ kern, const T* __restrict__ elam,
                                                   const T* __restrict__ hl, const T* __restrict__ trapez,
-                                                  T* __restrict__ gC, T* __restrict__ gP, const int32_t* __restrict__ band_of_set) {
+                                                  T* __restrict__ gC, T* __restrict__ gP, const int32_t* __restrict__ band_of_set,
+                                                  int sweep_base) {
     const DevSystem& sys = *psys;
     const int p = blockIdx.y;
     if (evalflag[p] != prec) return;
@@ -196,7 +197,7 @@ __global__ void __launch_bounds__(TB) k_prop_step(const DevSystem* __restrict__ 
     const int NP = P.NP, L = P.L, NB = sys.NB, MB = sys.MB, K = sys.K;
     const int n0 = blockIdx.x * TB;
     if (n0 >= NP) return;
-    const int sweep = blockIdx.z;
+    const int sweep = blockIdx.z + sweep_base;
     const int N = A.nseg, S = A.S;
     const int outseg = sweep == 0 ? N - 1 - A.s : A.s;
     const int inseg = sweep == 0 ? outseg + 1 : outseg - 1;
@@ -1320,11 +1321,28 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
         PropArgs A;
         A.u = u; A.nseg = sh.nseg; A.S = sh.S; A.fixed_seg = sh.fixed_seg;
         A.state_off = sh.state_off; A.maxd_off = sh.maxd_off; A.goff = sh.goff; A.GF = c.GF;
-        A.band = (sizeof(T) == 4 && c.band_fp32) ? 1 : -1;
+        const bool banded = (sizeof(T) == 4) ? c.band_fp32 != 0 : c.band_fp64 != 0;
+        A.band = banded ? 1 : -1;
+        const bool gemm = c.use_gemm && c.prop_plan[u].ntiles > 0;
         for (int s = 1; s < sh.nseg; ++s) {
             A.s = s;
-            hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(maxNP, 256, P, 2), dim3(256), 0, st, c.d_sys, c.d_pts,
-                               c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl, trapez, gC, gP, c.d_band);
+            // sweeps whose parent is the grafted segment use the direct real-space sums
+            const bool fix0 = sh.fixed_seg == 0 && s == sh.nseg - 1;  // child sweep, parent = segment 0
+            const bool fix1 = sh.fixed_seg == 0 && s == 1;            // parent sweep, parent = segment 0
+            if (!gemm) {
+                hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(maxNP, 256, P, 2), dim3(256), 0, st, c.d_sys,
+                                   c.d_pts, c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl, trapez,
+                                   gC, gP, c.d_band, 0);
+                continue;
+            }
+            const int skip = (fix0 && fix1) ? 2 : fix0 ? 0 : fix1 ? 1 : -1;
+            if (skip != 2)
+                launch_opgemm_prop<T>(st, c, prec, c.prop_plan[u], u, s, skip, banded, kern, elam, hl, trapez, gC, gP,
+                                      fbuf);
+            if (skip >= 0)
+                hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(maxNP, 256, P, skip == 2 ? 2 : 1), dim3(256), 0, st,
+                                   c.d_sys, c.d_pts, c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl,
+                                   trapez, gC, gP, c.d_band, skip == 2 ? 0 : skip);
         }
         if (sh.fixed_seg >= 0) {
             hipLaunchKernelGGL(k_fixed_lambda<T>, dim3(P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag, prec, A,
@@ -1334,13 +1352,16 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
     hipLaunchKernelGGL((k_accumulate<T, 256>), grid2(maxNP, 256, P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag,
                        prec, c.d_spec, c.d_goffs, c.GF, c.d_seg_states, c.d_maxd, elam, trapez, gC, gP,
                        c.d_fixed_lambda, rho_beads, rho_bonds, rhoT, c.d_acc_part, c.d_errflag);
-    if (c.n_fwd_jobs > 0)
+    if (c.use_gemm) launch_opgemm_conv<T>(st, c, prec, c.fwd_plan, kern, rhoT, dphi, nw, fx);
+    else if (c.n_fwd_jobs > 0)
         hipLaunchKernelGGL((k_conv<T, 256>), grid2(maxNP, 256, c.n_fwd_jobs, P), dim3(256), 0, st, c.d_sys, c.d_pts,
                            c.d_evalflag, prec, c.d_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx);
     hipLaunchKernelGGL(k_pointwise<T>, grid2(maxNP, 256, P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag, prec,
                        nw, c.Fnw, rhoT, c.Frt, c.d_surf, dphi, c.Fdp);
-    hipLaunchKernelGGL((k_conv<T, 256>), grid2(maxNP, 256, c.n_back_jobs, P), dim3(256), 0, st, c.d_sys, c.d_pts,
-                       c.d_evalflag, prec, c.d_jobs + c.n_fwd_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx);
+    if (c.use_gemm) launch_opgemm_conv<T>(st, c, prec, c.back_plan, kern, rhoT, dphi, nw, fx);
+    else
+        hipLaunchKernelGGL((k_conv<T, 256>), grid2(maxNP, 256, c.n_back_jobs, P), dim3(256), 0, st, c.d_sys, c.d_pts,
+                           c.d_evalflag, prec, c.d_jobs + c.n_fwd_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx);
     hipLaunchKernelGGL(k_neutrality, dim3((P + 63) / 64), dim3(64), 0, st, c.d_sys, c.d_pts, c.d_spec, c.d_evalflag,
                        prec, c.d_acc_part, c.d_acc_species, x, fx, P);
 }
@@ -1353,14 +1374,22 @@ void Pipeline<T>::launch_prop_step(hipStream_t st, const Ctx& c, int prec, int u
     A.state_off = sh.state_off; A.maxd_off = sh.maxd_off; A.goff = sh.goff; A.GF = c.GF;
     A.band = banded ? 1 : -1;
     A.s = s;
-    hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(c.maxNP, 256, c.P, 2), dim3(256), 0, st, c.d_sys, c.d_pts,
-                       c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl, trapez, gC, gP, c.d_band);
+    if (c.use_gemm && c.prop_plan[u].ntiles > 0 && sh.fixed_seg < 0)
+        launch_opgemm_prop<T>(st, c, prec, c.prop_plan[u], u, s, -1, banded, kern, elam, hl, trapez, gC, gP,
+                              fbuf);
+    else
+        hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(c.maxNP, 256, c.P, 2), dim3(256), 0, st, c.d_sys, c.d_pts,
+                           c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl, trapez, gC, gP, c.d_band, 0);
 }
 
 template <typename T>
 void Pipeline<T>::launch_conv(hipStream_t st, const Ctx& c, int prec, bool back, double* fx) {
     const int nj = back ? c.n_back_jobs : c.n_fwd_jobs;
     if (nj == 0) return;
+    if (c.use_gemm) {
+        launch_opgemm_conv<T>(st, c, prec, back ? c.back_plan : c.fwd_plan, kern, rhoT, dphi, nw, fx);
+        return;
+    }
     hipLaunchKernelGGL((k_conv<T, 256>), grid2(c.maxNP, 256, nj, c.P), dim3(256), 0, st, c.d_sys, c.d_pts,
                        c.d_evalflag, prec, c.d_jobs + (back ? c.n_fwd_jobs : 0), kern, rhoT, c.Frt, dphi, c.Fdp, nw,
                        c.Fnw, fx);

@@ -10,6 +10,37 @@
 
 namespace sdft {
 
+// tile / group / column tables of the MFMA operator-GEMM (sdft_opgemm.hip)
+struct OpTile {
+    int32_t group;
+    int32_t row0;
+    int32_t col0;
+    int32_t ncols;
+};
+struct OpGroup {
+    int32_t kset;
+    int32_t NP, L;
+    int32_t nterms;
+    int32_t kern[3];
+    int32_t sign[3];
+    int32_t col_off;
+    int32_t ncols;
+    int64_t kbase;
+    int32_t band;
+    int32_t pad;
+};
+struct OpCol { int32_t p, job; };
+struct PropCol { int32_t p, sweep, a, b; };
+
+struct OpPlan {
+    int ntiles = 0;
+    bool wide = true;
+    bool full_pairs = false;   // propagator plans: every (sweep, a, b) column present, b fastest
+    void* d_tiles = nullptr;
+    void* d_groups = nullptr;
+    void* d_cols = nullptr;
+};
+
 struct SpeciesHost {
     int nseg, S, model, fixed_seg, multistate;
     int state_off, maxd_off;
@@ -27,6 +58,10 @@ struct Ctx {
     int n_fwd_jobs = 0, n_back_jobs = 0;
     int band_fp32 = 1;
     int m = 5;                      // Anderson depth the history buffers were sized for
+    int band_fp64 = 0;              // FP64 propagators banded (mixed-precision polish only)
+    bool use_gemm = true;           // MFMA operator-GEMM path (else the direct reference kernels)
+    OpPlan fwd_plan, back_plan;
+    std::vector<OpPlan> prop_plan;  // per species (ntiles == 0: direct kernel)
     std::vector<SpeciesHost> species;
 
     DevSystem* d_sys = nullptr;
@@ -68,6 +103,7 @@ struct Pipeline {
     T* rhoT = nullptr;
     T* nw = nullptr;
     T* dphi = nullptr;
+    T* fbuf = nullptr;      // chained propagator inputs, two step buffers of (2 sweeps x S x S) fields
     void objective(hipStream_t st, const Ctx& c, int prec, const double* x, double* fx);
     void omega(hipStream_t st, const Ctx& c, const double* x, double* omega_acc);
     void segment_sums(hipStream_t st, const Ctx& c, int u, double* out, int64_t stride);
@@ -76,6 +112,16 @@ struct Pipeline {
     void launch_conv(hipStream_t st, const Ctx& c, int prec, bool back, double* fx);
     void launch_pointwise(hipStream_t st, const Ctx& c, int prec);
 };
+
+template <typename T>
+void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, const T* kern, const T* rhoT,
+                        const T* dphi, T* nw, double* fx);
+template <typename T>
+void launch_opgemm_prop(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, int u, int s, int skip_sweep,
+                        bool banded, const T* kern, const T* elam, const T* hl, const T* trapez, T* gC, T* gP,
+                        T* fbuf);
+int opgemm_tile_rows(bool wide);
+int opgemm_tile_cols(bool wide);
 
 void launch_build_kernels(hipStream_t st, const DevSystem* sys, int K, int nsets, int maxL, const int32_t* set_L,
                           const int64_t* set_kbase, const double* set_dz, double* wbuf, double* c64, float* c32);

@@ -1,0 +1,384 @@
+// MFMA operator-GEMM for CDNA4 (gfx950).
+//
+// Every convolution of the SCF objective is y = A x with the Toeplitz +- Hankel
+// operator A[n][n'] = s*c[n-n'] + c[n+n'+1] (indices mod L = 2 NP), c being the
+// periodic IDFT of a sampled weight.  All state points of a kernel set (same NP,
+// dz) share c, so one convolution stage over a batch is a GEMM
+//     Y[NP x cols] = sum_terms A_term[NP x NP] * X_term[NP x cols]
+// whose columns are (state point, field) pairs.  The A operand is generated on
+// the fly from two LDS windows of c (no NP x NP matrix is ever stored); B is
+// staged through LDS; products run on v_mfma_f32_16x16x4f32 /
+// v_mfma_f64_16x16x4f64.  Lane layouts (verified on gfx950):
+//   A: lane l holds A[l % 16][l / 16]      B: lane l holds B[l / 16][l % 16]
+//   D f32: D[4 (l/16) + v][l % 16]         D f64: D[4 v + (l/16)][l % 16]
+//
+// Two input modes share the core:
+//   MODE_CONV: X columns are plain fields (weighted densities / back projections)
+//   MODE_PROP: X columns are propagator inputs exp(-lambda) exp(lng/2) Q trapez
+//              built while staging; the epilogue writes |y| exp(lng/2)/normative
+//              into the gC / gP families (child_propagators.jl:118-141).
+#include <cstring>
+
+#include "sdft_launch.h"
+
+namespace sdft {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<float> {
+    typedef f32x4 acc_t;
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+template <> struct Mfma<double> {
+    typedef f64x4 acc_t;
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
+};
+
+__device__ __forceinline__ int wrapLg(int d, int L) {
+    d %= L;
+    return d < 0 ? d + L : d;
+}
+
+// Workgroup tile: 4 waves stacked along rows; each wave WM x WN blocks of 16 x 16.
+template <int WM, int WN>
+struct TileShape {
+    static constexpr int TM = 4 * WM * 16;  // rows per workgroup
+    static constexpr int TN = WN * 16;      // columns per workgroup
+    static constexpr int KC = 16;           // n' per staged chunk
+    static constexpr int CW = TM + KC;      // c window length (TM + KC - 1 used)
+};
+
+struct PropStep {       // per-launch propagator parameters
+    int32_t u, s, nseg, S, fixed_seg, state_off, maxd_off, GF, MB, NB;
+    int64_t goff;
+    int32_t skip_sweep; // sweep whose parent is grafted at this step (handled elsewhere), or -1
+    int32_t banded;     // apply the group's band half-width
+    int32_t fused;      // inputs of step s >= 2 come from fin; the epilogue writes fout for step s + 1
+    int32_t pad;
+};
+
+template <typename T, int WM, int WN, bool PROP>
+__global__ void __launch_bounds__(256) k_opgemm(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
+                                                const int32_t* __restrict__ evalflag, int prec,
+                                                const OpTile* __restrict__ tiles, const OpGroup* __restrict__ groups,
+                                                const void* __restrict__ colsv, const ConvJob* __restrict__ jobs,
+                                                const T* __restrict__ kern,
+                                                // MODE_CONV inputs / outputs
+                                                const T* __restrict__ rhoT, int Frt, const T* __restrict__ dphi, int Fdp,
+                                                T* __restrict__ nw, int Fnw, double* __restrict__ fx,
+                                                // MODE_PROP inputs / outputs
+                                                PropStep ps, const int32_t* __restrict__ seg_states,
+                                                const double* __restrict__ maxd_tab, const T* __restrict__ elam,
+                                                const T* __restrict__ hl, const T* __restrict__ trapez,
+                                                T* __restrict__ gC, T* __restrict__ gP,
+                                                const T* __restrict__ fin, T* __restrict__ fout) {
+    typedef TileShape<WM, WN> Sh;
+    constexpr int TM = Sh::TM, TN = Sh::TN, KC = Sh::KC;
+    const OpTile tile = tiles[blockIdx.x];
+    const OpGroup G = groups[tile.group];
+    const int NP = G.NP, L = G.L;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const DevSystem& sys = *psys;
+
+    __shared__ T xs[KC][TN];
+    __shared__ T cmw[Sh::CW];
+    __shared__ T cpw[Sh::CW];
+    __shared__ int32_t colp[TN];        // point of each column (-1 inactive)
+    __shared__ int32_t colaux[TN][4];   // MODE_CONV: job; MODE_PROP: sweep, a, b, bond
+    __shared__ int32_t colpp[TN][2];    // MODE_PROP: pi, pj
+    __shared__ int any_active;
+
+    if (tid == 0) any_active = 0;
+    __syncthreads();
+    if (tid < TN) {
+        int p = -1;
+        if (tid < tile.ncols) {
+            if (!PROP) {
+                const OpCol c = ((const OpCol*)colsv)[G.col_off + tile.col0 + tid];
+                p = c.p;
+                colaux[tid][0] = c.job;
+            } else {
+                const PropCol c = ((const PropCol*)colsv)[G.col_off + tile.col0 + tid];
+                p = c.p;
+                const int N = ps.nseg, S = ps.S;
+                const int outseg = c.sweep == 0 ? N - 1 - ps.s : ps.s;
+                const int inseg = c.sweep == 0 ? outseg + 1 : outseg - 1;
+                const int si = seg_states[ps.state_off + outseg * S + c.a];
+                const int sj = seg_states[ps.state_off + inseg * S + c.b];
+                if (si < 0 || sj < 0 || c.sweep == ps.skip_sweep) p = -1;
+                int bond = 0;
+                if (p >= 0) {
+                    const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
+                    for (int u2 = 0; u2 < ps.MB; ++u2)
+                        if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
+                }
+                colaux[tid][0] = c.sweep;
+                colaux[tid][1] = c.a;
+                colaux[tid][2] = c.b;
+                colaux[tid][3] = bond;
+                colpp[tid][0] = si <= sj ? 0 : 1;
+                colpp[tid][1] = si <= sj ? 1 : 0;
+            }
+            if (p >= 0 && evalflag[p] != prec) p = -1;
+        }
+        colp[tid] = p;
+        if (p >= 0) atomicOr(&any_active, 1);
+    }
+    __syncthreads();
+    if (!any_active) return;
+
+    typename Mfma<T>::acc_t acc[WM][WN];
+    for (int i = 0; i < WM; ++i)
+        for (int j = 0; j < WN; ++j)
+            for (int v = 0; v < 4; ++v) acc[i][j][v] = T(0);
+
+    const T* cset = kern + G.kbase * (int64_t)sys.K;
+    const int r0 = tile.row0;
+    int k_lo = 0, k_hi = NP;
+    if (PROP && ps.banded && G.band >= 0) {
+        k_lo = max(0, r0 - G.band);
+        k_hi = min(NP, r0 + TM + G.band);
+    }
+    const int nterms = PROP ? 1 : G.nterms;
+    // staging assignment: each thread loads KC*TN/256 consecutive-k elements of one column
+    constexpr int PER = KC * TN / 256;
+    const int scol = tid / (KC / PER);
+    const int sk0 = (tid % (KC / PER)) * PER;
+
+    for (int term = 0; term < nterms; ++term) {
+        const T* c = cset + (int64_t)G.kern[term] * L;
+        const T sgn = G.sign[term] < 0 ? T(-1) : T(1);
+        // input base pointer of the staging column for this term (MODE_CONV)
+        const T* inp = nullptr;
+        int sp_ = -1;
+        if (scol < TN) sp_ = colp[scol];
+        if (!PROP && sp_ >= 0) {
+            const ConvJob& J = jobs[colaux[scol][0]];
+            const DevPoint& P = pts[sp_];
+            const ConvTerm tm = J.t[term];
+            inp = tm.infam == 0 ? rhoT + P.base * (int64_t)Frt + (int64_t)tm.infield * P.NP
+                                : dphi + P.base * (int64_t)Fdp + (int64_t)tm.infield * P.NP;
+        }
+        for (int k0 = k_lo; k0 < k_hi; k0 += KC) {
+            __syncthreads();
+            // ---- stage X chunk (transposed into xs[k][col])
+            if (scol < TN) {
+                for (int q = 0; q < PER; ++q) {
+                    const int nn = k0 + sk0 + q;
+                    T v = T(0);
+                    if (sp_ >= 0 && nn < NP) {
+                        if (!PROP) {
+                            v = inp[nn];
+                        } else if (ps.fused && ps.s >= 2) {
+                            const DevPoint& P = pts[sp_];
+                            const int sweep = colaux[scol][0], a = colaux[scol][1], b = colaux[scol][2];
+                            v = fin[P.base * (int64_t)(2 * ps.S * ps.S) + (((int64_t)sweep * ps.S + a) * ps.S + b) * P.NP + nn];
+                        } else {
+                            const DevPoint& P = pts[sp_];
+                            const int sweep = colaux[scol][0], b = colaux[scol][2], bond = colaux[scol][3];
+                            const int N = ps.nseg, S = ps.S;
+                            const int outseg = sweep == 0 ? N - 1 - ps.s : ps.s;
+                            const int inseg = sweep == 0 ? outseg + 1 : outseg - 1;
+                            const int sj = seg_states[ps.state_off + inseg * S + b];
+                            const bool stored = sweep == 0 ? (inseg < N - 1) : (inseg > 0);
+                            T q_in;
+                            const T* gin = sweep == 0 ? gC : gP;
+                            if (!stored) {
+                                if (ps.fixed_seg < 0) {
+                                    q_in = T(1);
+                                } else {
+                                    const int Kf = P.fixed_coord[ps.u];
+                                    const double* maxd = maxd_tab + ps.maxd_off;
+                                    if (inseg == ps.fixed_seg) q_in = nn == Kf ? T(1) : T(0);
+                                    else {
+                                        const double d = (double)(nn - Kf) * P.dz;
+                                        q_in = (d * d > maxd[inseg] * maxd[inseg]) ? T(0) : T(1);
+                                    }
+                                }
+                            } else {
+                                q_in = T(0);
+                                const int64_t gb = P.base * (int64_t)ps.GF + ps.goff;
+                                for (int b2 = 0; b2 < S; ++b2)
+                                    if (seg_states[ps.state_off + inseg * S + b2] >= 0)
+                                        q_in += gin[gb + (((int64_t)inseg * S + b2) * S + b) * P.NP + nn];
+                            }
+                            const int64_t eb = P.base * (int64_t)ps.NB;
+                            v = elam[eb + (int64_t)sj * P.NP + nn] *
+                                hl[P.base * (int64_t)(2 * ps.MB) + (int64_t)(2 * bond + colpp[scol][1]) * P.NP + nn] * q_in *
+                                trapez[eb + (int64_t)sj * P.NP + nn];
+                        }
+                    }
+                    xs[sk0 + q][scol] = v;
+                }
+            }
+            // ---- stage the operator windows: cm[i] = s*c[r0 - k0 - (KC-1) + i], cp[i] = c[r0 + k0 + 1 + i]
+            for (int i = tid; i < TM + KC - 1; i += 256) {
+                cmw[i] = sgn * c[wrapLg(r0 - k0 - (KC - 1) + i, L)];
+                cpw[i] = c[wrapLg(r0 + k0 + 1 + i, L)];
+            }
+            __syncthreads();
+            // ---- MFMA over the chunk: KC/4 k-steps
+#pragma unroll
+            for (int ks = 0; ks < KC / 4; ++ks) {
+                const int kl = ks * 4 + (lane >> 4);  // local k of this lane's A/B element
+                T bv[WN];
+#pragma unroll
+                for (int j = 0; j < WN; ++j) bv[j] = xs[kl][j * 16 + (lane & 15)];
+#pragma unroll
+                for (int i = 0; i < WM; ++i) {
+                    const int rl = wave * WM * 16 + i * 16 + (lane & 15);  // local row
+                    const T av = cmw[rl - kl + KC - 1] + cpw[rl + kl];
+#pragma unroll
+                    for (int j = 0; j < WN; ++j) acc[i][j] = Mfma<T>::mma(av, bv[j], acc[i][j]);
+                }
+            }
+        }
+    }
+
+    // ---- epilogue
+#pragma unroll
+    for (int i = 0; i < WM; ++i) {
+#pragma unroll
+        for (int j = 0; j < WN; ++j) {
+            const int cl = j * 16 + (lane & 15);
+            const int p = colp[cl];
+            if (!PROP) {
+                if (p < 0) continue;
+                const DevPoint& P = pts[p];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int n = r0 + wave * WM * 16 + i * 16 + Mfma<T>::row(lane, v);
+                    if (n >= NP) continue;
+                    const ConvJob& J = jobs[colaux[cl][0]];
+                    double yy = (double)acc[i][j][v] * J.scale;
+                    if (J.absval) yy = fabs(yy);
+                    if (J.outkind == 0) nw[P.base * (int64_t)Fnw + (int64_t)J.outfield * P.NP + n] = (T)yy;
+                    else fx[P.xoff + (int64_t)J.outfield * P.NP + n] = yy;
+                }
+            } else {
+                // every lane takes part (the state sum below is a lane shuffle)
+                const int sweep = colaux[cl][0], a = colaux[cl][1], b = colaux[cl][2], bond = colaux[cl][3];
+                const int N = ps.nseg, S = ps.S;
+                const int outseg = sweep == 0 ? N - 1 - ps.s : ps.s;
+                const DevPoint& P = pts[p >= 0 ? p : 0];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int n = r0 + wave * WM * 16 + i * 16 + Mfma<T>::row(lane, v);
+                    const bool ok = p >= 0 && n < NP;
+                    const T y = acc[i][j][v];
+                    T val = T(0);
+                    if (ok) {
+                        val = (y < T(0) ? -y : y) *
+                              hl[P.base * (int64_t)(2 * ps.MB) + (int64_t)(2 * bond + colpp[cl][0]) * P.NP + n] /
+                              (T)P.normative[ps.u];
+                        if (ps.fixed_seg >= 0) {
+                            const int Kf = P.fixed_coord[ps.u];
+                            const double* maxd = maxd_tab + ps.maxd_off;
+                            const double d = (double)(n - Kf) * P.dz;
+                            if (d * d > maxd[outseg] * maxd[outseg]) val = T(0);
+                        }
+                        T* gout = sweep == 0 ? gC : gP;
+                        gout[P.base * (int64_t)ps.GF + ps.goff + (((int64_t)outseg * S + b) * S + a) * P.NP + n] = val;
+                    }
+                    if (ps.fused) {
+                        // Q(a) = sum_b g[outseg][b][a]: the S columns of one (point, sweep, a) are adjacent lanes
+                        T q = val;
+                        if (S == 2) q += __shfl_xor(val, 1, 64);
+                        const int nextout = sweep == 0 ? outseg - 1 : outseg + 1;
+                        if (ok && b == 0 && ps.s + 1 < N) {
+                            const int sj = seg_states[ps.state_off + outseg * S + a];  // input bead of step s+1
+                            const int64_t eb = P.base * (int64_t)ps.NB + (int64_t)sj * P.NP + n;
+                            const T e = elam[eb] * q * trapez[eb];
+                            for (int a2 = 0; a2 < S; ++a2) {
+                                const int si2 = seg_states[ps.state_off + nextout * S + a2];
+                                if (si2 < 0) continue;
+                                const int lo = si2 <= sj ? si2 : sj, hi = si2 <= sj ? sj : si2;
+                                int bond2 = 0;
+                                for (int u2 = 0; u2 < ps.MB; ++u2)
+                                    if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond2 = u2;
+                                const int pj2 = si2 <= sj ? 1 : 0;
+                                fout[P.base * (int64_t)(2 * S * S) + (((int64_t)sweep * S + a2) * S + a) * P.NP + n] =
+                                    e * hl[P.base * (int64_t)(2 * ps.MB) + (int64_t)(2 * bond2 + pj2) * P.NP + n];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
+// host launchers
+// --------------------------------------------------------------------------
+template <typename T>
+void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, const T* kern, const T* rhoT,
+                        const T* dphi, T* nw, double* fx) {
+    if (plan.ntiles == 0) return;
+    PropStep ps;
+    memset(&ps, 0, sizeof(ps));
+    if (plan.wide)
+        hipLaunchKernelGGL((k_opgemm<T, 1, 4, false>), dim3(plan.ntiles), dim3(256), 0, st, c.d_sys, c.d_pts,
+                           c.d_evalflag, prec, (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups, plan.d_cols,
+                           c.d_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx, ps, c.d_seg_states, c.d_maxd,
+                           (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (T*)nullptr, (T*)nullptr,
+                           (const T*)nullptr, (T*)nullptr);
+    else
+        hipLaunchKernelGGL((k_opgemm<T, 2, 1, false>), dim3(plan.ntiles), dim3(256), 0, st, c.d_sys, c.d_pts,
+                           c.d_evalflag, prec, (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups, plan.d_cols,
+                           c.d_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx, ps, c.d_seg_states, c.d_maxd,
+                           (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (T*)nullptr, (T*)nullptr,
+                           (const T*)nullptr, (T*)nullptr);
+}
+
+template <typename T>
+void launch_opgemm_prop(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, int u, int s, int skip_sweep,
+                        bool banded, const T* kern, const T* elam, const T* hl, const T* trapez, T* gC, T* gP,
+                        T* fbuf) {
+    if (plan.ntiles == 0) return;
+    const SpeciesHost& sh = c.species[u];
+    PropStep ps;
+    memset(&ps, 0, sizeof(ps));
+    ps.u = u; ps.s = s; ps.nseg = sh.nseg; ps.S = sh.S; ps.fixed_seg = sh.fixed_seg;
+    ps.state_off = sh.state_off; ps.maxd_off = sh.maxd_off; ps.GF = c.GF; ps.MB = c.MB; ps.NB = c.NB;
+    ps.goff = sh.goff; ps.skip_sweep = skip_sweep; ps.banded = banded ? 1 : 0;
+    // fused input chaining: homogeneous, ungrafted chains whose every (sweep, a, b) column is present
+    ps.fused = (plan.full_pairs && sh.fixed_seg < 0 && fbuf != nullptr) ? 1 : 0;
+    const int64_t fstride = c.sumNP * 2 * sh.S * sh.S;
+    const T* fin = fbuf ? fbuf + (int64_t)(s & 1) * fstride : nullptr;
+    T* fout = fbuf ? fbuf + (int64_t)((s + 1) & 1) * fstride : nullptr;
+    if (plan.wide)
+        hipLaunchKernelGGL((k_opgemm<T, 1, 4, true>), dim3(plan.ntiles), dim3(256), 0, st, c.d_sys, c.d_pts,
+                           c.d_evalflag, prec, (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups, plan.d_cols,
+                           c.d_jobs, kern, (const T*)nullptr, c.Frt, (const T*)nullptr, c.Fdp, (T*)nullptr, c.Fnw,
+                           (double*)nullptr, ps, c.d_seg_states, c.d_maxd, elam, hl, trapez, gC, gP, fin, fout);
+    else
+        hipLaunchKernelGGL((k_opgemm<T, 2, 1, true>), dim3(plan.ntiles), dim3(256), 0, st, c.d_sys, c.d_pts,
+                           c.d_evalflag, prec, (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups, plan.d_cols,
+                           c.d_jobs, kern, (const T*)nullptr, c.Frt, (const T*)nullptr, c.Fdp, (T*)nullptr, c.Fnw,
+                           (double*)nullptr, ps, c.d_seg_states, c.d_maxd, elam, hl, trapez, gC, gP, fin, fout);
+}
+
+template void launch_opgemm_conv<float>(hipStream_t, const Ctx&, int, const OpPlan&, const float*, const float*,
+                                        const float*, float*, double*);
+template void launch_opgemm_conv<double>(hipStream_t, const Ctx&, int, const OpPlan&, const double*, const double*,
+                                         const double*, double*, double*);
+template void launch_opgemm_prop<float>(hipStream_t, const Ctx&, int, const OpPlan&, int, int, int, bool, const float*,
+                                        const float*, const float*, const float*, float*, float*, float*);
+template void launch_opgemm_prop<double>(hipStream_t, const Ctx&, int, const OpPlan&, int, int, int, bool, const double*,
+                                         const double*, const double*, const double*, double*, double*, double*);
+
+// row/column tile sizes of the two shapes (for the host planner)
+int opgemm_tile_rows(bool wide) { return wide ? TileShape<1, 4>::TM : TileShape<2, 1>::TM; }
+int opgemm_tile_cols(bool wide) { return wide ? TileShape<1, 4>::TN : TileShape<2, 1>::TN; }
+
+}  // namespace sdft
