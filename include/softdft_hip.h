@@ -161,9 +161,19 @@ int32_t sdft_get_field(sdft_ctx_t* ctx, int32_t field_id, int32_t point, double*
 /* timing of the last sdft_solve: device milliseconds of the whole solve and total iterations */
 int32_t sdft_last_timing(const sdft_ctx_t* ctx, double* solve_ms, double* iterations_total);
 
+/* bond operator applied by sdft_objective at each precision: 1 = banded (the filtered bond kernel
+ * truncated where |c| <= 1e-9 max|c|, half-width rounded up to a multiple of 8), 0 = dense.
+ * Defaults: FP32 banded, FP64 dense.  sdft_solve takes its own choice from its options.
+ * (precision control; no counterpart in the reference) */
+int32_t sdft_set_operator_band(sdft_ctx_t* ctx, int32_t fp32_banded, int32_t fp64_banded);
+
+/* batch iterations of the last sdft_solve during which the FP32 and the FP64 pipeline ran
+ * (instrumentation; no counterpart in the reference) */
+int32_t sdft_last_pipeline_iters(const sdft_ctx_t* ctx, int32_t* fp32_iters, int32_t* fp64_iters);
+
 /* kernel classes for sdft_kernel_profile */
 #define SDFT_KP_PROP_FP32 0      /* one propagator step, FP32, banded bond kernel */
-#define SDFT_KP_PROP_FP64 1      /* one propagator step, FP64, dense (exact) bond operator */
+#define SDFT_KP_PROP_FP64 1      /* one propagator step, FP64, bond operator as in the last solve */
 #define SDFT_KP_CONV_FWD_FP32 2  /* weighted-density convolutions, FP32 */
 #define SDFT_KP_CONV_FWD_FP64 3
 #define SDFT_KP_CONV_BACK_FP32 4 /* back-projection convolutions, FP32 */

@@ -28,7 +28,7 @@ FIELD_RHO_BEADS, FIELD_RHO_BONDS, FIELD_X, FIELD_FX = 1, 2, 3, 4
 
 EXPORTED = ("sdft_abi_version", "sdft_last_error", "sdft_create", "sdft_destroy", "sdft_solution_length",
             "sdft_solution_offsets", "sdft_objective", "sdft_solve", "sdft_observables", "sdft_get_field",
-            "sdft_last_timing", "sdft_kernel_profile")
+            "sdft_last_timing", "sdft_last_pipeline_iters", "sdft_set_operator_band", "sdft_kernel_profile")
 KP_PROP_FP32, KP_PROP_FP64, KP_CONV_FWD_FP32, KP_CONV_FWD_FP64, KP_CONV_BACK_FP32, KP_CONV_BACK_FP64, \
     KP_POINTWISE_FP64 = range(7)
 
@@ -94,6 +94,10 @@ def load_library(path: str = LIB_PATH):
     lib.sdft_get_field.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _f64p, C.c_int64]
     lib.sdft_last_timing.restype = C.c_int32
     lib.sdft_last_timing.argtypes = [C.c_void_p, _f64p, _f64p]
+    lib.sdft_last_pipeline_iters.restype = C.c_int32
+    lib.sdft_last_pipeline_iters.argtypes = [C.c_void_p, _i32p, _i32p]
+    lib.sdft_set_operator_band.restype = C.c_int32
+    lib.sdft_set_operator_band.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     lib.sdft_kernel_profile.restype = C.c_int32
     lib.sdft_kernel_profile.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _f64p, _f64p]
     if lib.sdft_abi_version() != SDFT_ABI_VERSION:
@@ -272,6 +276,17 @@ class Batch:
         it = C.c_double()
         self._check(self.lib.sdft_last_timing(self.h, C.byref(ms), C.byref(it)))
         return ms.value, it.value
+
+    def set_operator_band(self, fp32: bool = True, fp64: bool = False):
+        """Bond operator of objective() per precision: banded (True) or dense (False)."""
+        self._check(self.lib.sdft_set_operator_band(self.h, int(fp32), int(fp64)))
+
+    def pipeline_iters(self):
+        """(FP32, FP64) batch iterations of the last solve during which each pipeline ran."""
+        a = C.c_int32()
+        b = C.c_int32()
+        self._check(self.lib.sdft_last_pipeline_iters(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def kernel_profile(self, kernel_class: int, reps: int = 20):
         """(average ms per launch, algorithmic flops per launch) measured with HIP events on the

@@ -26,6 +26,8 @@ struct sdft_ctx {
     bool have_eval = false;
     double last_solve_ms = 0.0;
     double last_iters_total = 0.0;
+    int32_t last_iters32 = 0, last_iters64 = 0;  // batch iterations that ran each pipeline
+    int32_t obj_band32 = 1, obj_band64 = 0;       // bond operator of sdft_objective per precision
     int nsets = 0;
     int S_max = 1;
     std::vector<int> nseg;
@@ -245,18 +247,16 @@ static int bond_of(const DevSystem& sys, int MB, int si, int sj) {
     return -1;
 }
 
-// Propagator plan of species u: valid only if every (sweep, a, b) column applies the same
-// canonical bond operator at every step (homogeneous chains); otherwise the direct kernel runs.
-static int build_prop_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const std::vector<std::vector<int>>& canon,
-                           const std::vector<int32_t>& set_L, const std::vector<int64_t>& set_kbase,
-                           const std::vector<int32_t>& band, OpPlan& plan) {
+// Canonical bond operator of every (sweep, a, b) column of chain species u (ck[(sweep*S + a)*S + b],
+// -1 where the column never occurs). False if a column's operator changes along the chain or a
+// bond type is missing: such chains keep the direct kernel.
+static bool column_kernels(sdft_ctx* ctx, const DevSystem& sys, int u, const std::vector<std::vector<int>>& canon,
+                           std::vector<int>& ck) {
     Ctx& c = ctx->c;
     const SpeciesHost& sh = c.species[u];
     const int N = sh.nseg, S = sh.S, NB = c.NB, MB = c.MB;
-    if (N < 2) return SDFT_OK;
     const int* st = ctx->seg_states_h.data() + sh.state_off;
-    // canonical bond kernel of each (sweep, a, b); -2 = unset, -1 = varies
-    std::vector<int> ck(2 * S * S, -2);
+    ck.assign(2 * S * S, -2);
     for (int sw = 0; sw < 2; ++sw)
         for (int s = 1; s < N; ++s) {
             const int outseg = sw == 0 ? N - 1 - s : s;
@@ -266,15 +266,60 @@ static int build_prop_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const std
                     const int si = st[outseg * S + a], sj = st[inseg * S + b];
                     if (si < 0 || sj < 0) continue;
                     const int bond = bond_of(sys, MB, si, sj);
-                    if (bond < 0) return SDFT_OK;
+                    if (bond < 0) return false;
                     for (size_t ks = 0; ks < canon.size(); ++ks) {
                         const int k = canon[ks][3 * NB + bond];
                         int& slot = ck[(sw * S + a) * S + b];
                         if (slot == -2) slot = k;
-                        else if (slot != k) return SDFT_OK;  // not homogeneous: keep the direct kernel
+                        else if (slot != k) return false;
                     }
                 }
         }
+    for (int& k : ck)
+        if (k == -2) k = -1;
+    return true;
+}
+
+// Persistent banded chain plan of species u: free (ungrafted) linear chains with fixed column
+// operators whose band fits the mirror construction (2 Be + 1 <= NP) and whose working set fits LDS.
+static void build_chain_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const std::vector<std::vector<int>>& canon,
+                             const std::vector<int32_t>& band, ChainPlanHost& ch) {
+    Ctx& c = ctx->c;
+    const SpeciesHost& sh = c.species[u];
+    ch = ChainPlanHost();
+    const int N = sh.nseg, S = sh.S;
+    if (N < 2 || sh.fixed_seg >= 0 || S > SDFT_MAX_S) return;
+    std::vector<int> ck;
+    if (!column_kernels(ctx, sys, u, canon, ck)) return;
+    for (int sw = 0; sw < 2; ++sw)
+        for (int k = 0; k < SDFT_MAX_S * SDFT_MAX_S; ++k) ch.ck[sw][k] = k < S * S ? ck[sw * S * S + k] : -1;
+    const int* st = ctx->seg_states_h.data() + sh.state_off;
+    ch.homog = true;
+    for (int q = 1; q < N; ++q)
+        for (int a = 0; a < S; ++a) ch.homog = ch.homog && st[q * S + a] == st[a];
+    ch.S = S;
+    for (int p = 0; p < c.P; ++p) {
+        const int NP = ctx->pts[p].NP;
+        const int Be = (band[ctx->pts[p].kset] + 7) & ~7;  // the chain kernel's rounded half-width
+        if (2 * Be + 1 > NP) return;
+        ch.npad_max = std::max(ch.npad_max, (NP + 255) / 256 * 256);
+        ch.be_max = std::max(ch.be_max, Be);
+    }
+    if (chain_lds_bytes(ch, sizeof(double)) > 160 * 1024) return;
+    ch.ok = true;
+}
+
+// Propagator plan of species u: valid only if every (sweep, a, b) column applies the same
+// canonical bond operator at every step (homogeneous chains); otherwise the direct kernel runs.
+static int build_prop_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const std::vector<std::vector<int>>& canon,
+                           const std::vector<int32_t>& set_L, const std::vector<int64_t>& set_kbase,
+                           const std::vector<int32_t>& band, OpPlan& plan) {
+    Ctx& c = ctx->c;
+    const SpeciesHost& sh = c.species[u];
+    const int S = sh.S;
+    if (sh.nseg < 2) return SDFT_OK;
+    std::vector<int> ck;
+    if (!column_kernels(ctx, sys, u, canon, ck)) return SDFT_OK;  // not homogeneous: keep the direct kernel
     struct Key { int kset, kern; bool operator<(const Key& o) const { return kset != o.kset ? kset < o.kset : kern < o.kern; } };
     std::map<Key, std::vector<PropCol>> groups;
     for (int p = 0; p < c.P; ++p)
@@ -378,6 +423,10 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         }
         sys.sigma[j] = d->diameters[j];
         sys.z[j] = d->valences[j];
+        int k = 0;
+        while (k < sys.ncls && sys.cls_sigma[k] != sys.sigma[j]) ++k;
+        if (k == sys.ncls) sys.cls_sigma[sys.ncls++] = sys.sigma[j];
+        sys.cls[j] = k;
     }
     for (int u = 0; u < MB; ++u) {
         sys.bond_i[u] = d->bond_types[2 * u];
@@ -387,6 +436,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
             return fail(SDFT_E_ARG);
         }
     }
+    c.ncls = sys.ncls;
     sys.hard_walls = d->hard_walls;
     sys.charged_walls = d->wall_charge != nullptr;
     ctx->hard_walls = d->hard_walls;
@@ -568,6 +618,13 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     if ((rc = dalloc(ctx, &d_wbuf, c.sumL_sets * c.K)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &ctx->p64.kern, c.sumL_sets * c.K)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &ctx->p32.kern, c.sumL_sets * c.K)) != SDFT_OK) return fail(rc);
+    {
+        const hipError_t e = prepare_kernel_attributes(c);
+        if (e != hipSuccess) {
+            set_error(ctx, SDFT_E_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+            return fail(SDFT_E_HIP);
+        }
+    }
     int maxL = 0;
     for (int L : set_L) maxL = std::max(maxL, L);
     launch_build_kernels(ctx->stream, c.d_sys, c.K, ctx->nsets, maxL, d_setL, d_setkb, d_setdz, d_wbuf, ctx->p64.kern,
@@ -618,6 +675,18 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         c.prop_plan.assign(nsp, OpPlan());
         for (int u = 0; u < nsp; ++u)
             if ((rc = build_prop_plan(ctx, sys, u, canon, set_L, set_kbase, band, c.prop_plan[u])) != SDFT_OK) return fail(rc);
+        c.chain_plan.assign(nsp, ChainPlanHost());
+        const char* cenv = getenv("SOFTDFT_CHAIN");
+        const bool chain_on = c.use_gemm && !(cenv && strcmp(cenv, "off") == 0);
+        for (int u = 0; u < nsp && chain_on; ++u) {
+            build_chain_plan(ctx, sys, u, canon, band, c.chain_plan[u]);
+            if (!c.chain_plan[u].ok) continue;
+            const hipError_t e = chain_prepare(c.chain_plan[u]);
+            if (e != hipSuccess) {
+                set_error(ctx, SDFT_E_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+                return fail(SDFT_E_HIP);
+            }
+        }
     }
 
     // ---- work buffers
@@ -651,7 +720,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     if ((rc = dalloc(ctx, &c.d_gram, (int64_t)P * SDFT_GRAM_BLOCKS * SDFT_GRAM_LEN)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_errbits, P)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_state, P)) != SDFT_OK) return fail(rc);
-    if ((rc = dalloc(ctx, &c.d_n_active, 1)) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &c.d_n_active, 3)) != SDFT_OK) return fail(rc);
     ctx->hard_lo.assign(d->wall_lo ? d->wall_lo : nullptr, d->wall_lo ? d->wall_lo + (int64_t)P * NB : nullptr);
     ctx->hard_hi.assign(d->wall_hi ? d->wall_hi : nullptr, d->wall_hi ? d->wall_hi + (int64_t)P * NB : nullptr);
     if (d->wall_charge) ctx->wall_q.assign(d->wall_charge, d->wall_charge + 2 * P);
@@ -681,9 +750,10 @@ static int check_errflags(sdft_ctx* ctx) {
     return SDFT_OK;
 }
 
-static void run_objective(sdft_ctx* ctx) {
-    ctx->p32.objective(ctx->stream, ctx->c, 32, ctx->c.d_x, ctx->c.d_fx);
-    ctx->p64.objective(ctx->stream, ctx->c, 64, ctx->c.d_x, ctx->c.d_fx);
+// evaluates F(x) for every point whose evalflag is 32 or 64; a pipeline with no such point is skipped
+static void run_objective(sdft_ctx* ctx, bool run32, bool run64) {
+    if (run32) ctx->p32.objective(ctx->stream, ctx->c, 32, ctx->c.d_x, ctx->c.d_fx);
+    if (run64) ctx->p64.objective(ctx->stream, ctx->c, 64, ctx->c.d_x, ctx->c.d_fx);
 }
 
 extern "C" int32_t sdft_objective(sdft_ctx_t* ctx, const double* x, double* fx, int32_t precision) {
@@ -700,7 +770,9 @@ extern "C" int32_t sdft_objective(sdft_ctx_t* ctx, const double* x, double* fx, 
     std::vector<DevSolveState> st(c.P);
     for (auto& s : st) { memset(&s, 0, sizeof(s)); s.prec = precision; s.converged = 1; s.mode = -1; }
     HIP_CHECK_RET(hipMemcpyAsync(c.d_state, st.data(), st.size() * sizeof(DevSolveState), hipMemcpyHostToDevice, ctx->stream));
-    run_objective(ctx);
+    c.band_fp32 = ctx->obj_band32;
+    c.band_fp64 = ctx->obj_band64;
+    run_objective(ctx, precision == 32, precision == 64);
     HIP_CHECK_RET(hipGetLastError());
     HIP_CHECK_RET(hipMemcpyAsync(fx, c.d_fx, c.total_n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
@@ -751,15 +823,24 @@ extern "C" int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* o, doub
     HIP_CHECK_RET(hipEventCreate(&e1));
     HIP_CHECK_RET(hipEventRecord(e0, ctx->stream));
     const int check = o->check_every > 0 ? o->check_every : 10;
-    int32_t n_active = c.P;
-    int it = 0;
+    // the host polls the active counts every `check` iterations; mixed-precision switches are
+    // applied at these polls, so between two polls the set of precisions in use is fixed and a
+    // pipeline with no point at its precision is not launched
+    int32_t counts[3] = {c.P, sp.start_prec == 32 ? c.P : 0, 0};
+    bool run32 = sp.start_prec == 32, run64 = sp.start_prec == 64;
+    int it = 0, n32 = 0, n64 = 0;
     for (it = 0; it < o->max_iters; ++it) {
-        run_objective(ctx);
+        run_objective(ctx, run32, run64);
+        n32 += run32;
+        n64 += run64;
         launch_anderson(ctx->stream, c, sp);
         if ((it + 1) % check == 0 || it + 1 == o->max_iters) {
-            HIP_CHECK_RET(hipMemcpyAsync(&n_active, c.d_n_active, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+            HIP_CHECK_RET(hipMemcpyAsync(counts, c.d_n_active, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
             HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
-            if (n_active == 0) break;
+            if (counts[0] == 0) break;
+            if (counts[2] > 0) launch_apply_switch(ctx->stream, c);
+            run32 = counts[1] - counts[2] > 0;
+            run64 = counts[0] - counts[1] + counts[2] > 0;
         }
     }
     HIP_CHECK_RET(hipGetLastError());
@@ -770,6 +851,8 @@ extern "C" int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* o, doub
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     ctx->last_solve_ms = ms;
+    ctx->last_iters32 = n32;
+    ctx->last_iters64 = n64;
     HIP_CHECK_RET(hipMemcpyAsync(x_inout, c.d_x, c.total_n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_CHECK_RET(hipMemcpyAsync(st.data(), c.d_state, st.size() * sizeof(DevSolveState), hipMemcpyDeviceToHost, ctx->stream));
     HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
@@ -797,6 +880,20 @@ extern "C" int32_t sdft_last_timing(const sdft_ctx_t* ctx, double* solve_ms, dou
     if (!ctx) return SDFT_E_ARG;
     if (solve_ms) *solve_ms = ctx->last_solve_ms;
     if (iterations_total) *iterations_total = ctx->last_iters_total;
+    return SDFT_OK;
+}
+
+extern "C" int32_t sdft_set_operator_band(sdft_ctx_t* ctx, int32_t fp32_banded, int32_t fp64_banded) {
+    if (!ctx) return SDFT_E_ARG;
+    ctx->obj_band32 = fp32_banded ? 1 : 0;
+    ctx->obj_band64 = fp64_banded ? 1 : 0;
+    return SDFT_OK;
+}
+
+extern "C" int32_t sdft_last_pipeline_iters(const sdft_ctx_t* ctx, int32_t* fp32_iters, int32_t* fp64_iters) {
+    if (!ctx) return SDFT_E_ARG;
+    if (fp32_iters) *fp32_iters = ctx->last_iters32;
+    if (fp64_iters) *fp64_iters = ctx->last_iters64;
     return SDFT_OK;
 }
 
@@ -962,7 +1059,7 @@ extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps
     auto once = [&]() {
         switch (kc) {
             case SDFT_KP_PROP_FP32: ctx->p32.launch_prop_step(ctx->stream, c, 32, u, s_mid, c.band_fp32 != 0); break;
-            case SDFT_KP_PROP_FP64: ctx->p64.launch_prop_step(ctx->stream, c, 64, u, s_mid, false); break;
+            case SDFT_KP_PROP_FP64: ctx->p64.launch_prop_step(ctx->stream, c, 64, u, s_mid, c.band_fp64 != 0); break;
             case SDFT_KP_CONV_FWD_FP32: ctx->p32.launch_conv(ctx->stream, c, 32, false, d_fx); break;
             case SDFT_KP_CONV_FWD_FP64: ctx->p64.launch_conv(ctx->stream, c, 64, false, d_fx); break;
             case SDFT_KP_CONV_BACK_FP32: ctx->p32.launch_conv(ctx->stream, c, 32, true, d_fx); break;
@@ -1004,7 +1101,7 @@ extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps
         for (int p = 0; p < c.P; ++p) {
             const double NP = ctx->pts[p].NP;
             if (kc == SDFT_KP_PROP_FP32 || kc == SDFT_KP_PROP_FP64) {
-                const bool bd = kc == SDFT_KP_PROP_FP32 && c.band_fp32;
+                const bool bd = kc == SDFT_KP_PROP_FP32 ? c.band_fp32 != 0 : c.band_fp64 != 0;
                 const double taps = bd ? std::min(NP, 2.0 * band[ctx->pts[p].kset] + 1.0) : NP;
                 f += 2.0 /*sweeps*/ * npair * 2.0 * NP * taps;
             } else if (kc == SDFT_KP_POINTWISE_FP64) {

@@ -1,0 +1,279 @@
+// Persistent banded chain propagation on MFMA (gfx950).
+//
+// One workgroup owns one (state point, sweep) of a linear chain species and runs
+// all N-1 propagator steps (child_propagators.jl:93-141 for sweep 0, the parent
+// recursion for sweep 1) without leaving the kernel: the running segment sums Q
+// stay in LDS, only the per-segment propagators gC / gP are written to HBM for
+// the density accumulation.
+//
+// The bond operator of one step is A[n][n'] = c[n - n'] + c[n + n' + 1]
+// (indices mod L = 2 NP) with c truncated to the half-width Be.  On the
+// mirror-extended input
+//     x_ext[m] = x[m] (0 <= m < NP),  x[-1 - m] (m < 0),  x[2 NP - 1 - m] (m >= NP)
+// the Toeplitz and the Hankel part together are one plain convolution
+//     y[n] = sum_{|j| <= Be} c[j] x_ext[n - j].
+// Writing n = 16 p + r, a 256-output tile is the GEMM
+//     D[r][p] = sum_t A[r][t] X[t][p],   A[r][t] = c[r - t + Be],  X[t][p] = x_ext[16 p + t - Be]
+// with K = 2 Be + 16: M = 16 output phases, N = 16 position blocks.  A is a
+// sliding window of c in LDS, X the extended input in LDS; both feed
+// v_mfma_f32_16x16x4f32 / v_mfma_f64_16x16x4f64 directly.
+//
+// Per step and column (a, b) (a: state of the output segment, b: state of the
+// input segment):
+//     x_ab = exp(-lambda_sj) exp(lng_bond / 2) Q_b trapez_sj            (input)
+//     g[outseg][b][a] = |A x_ab| exp(lng_bond / 2) / normative          (output)
+//     Q_a (next step) = sum_b g[outseg][b][a]
+// The weights W_ab = exp(-lambda) exp(lng/2) trapez do not change along a chain
+// whose segments all carry the same states; they are then kept in registers.
+#include <cstring>
+
+#include "sdft_launch.h"
+#include "sdft_mfma.h"
+
+namespace sdft {
+
+constexpr int CHAIN_THREADS = 512;
+constexpr int CHAIN_WAVES = CHAIN_THREADS / 64;
+
+struct ChainArgs {
+    int32_t u, nseg, S, state_off;
+    int32_t GF, MB, NB, homog;     // homog: every segment has the same state beads
+    int64_t goff;
+    int32_t ck[2][SDFT_MAX_S * SDFT_MAX_S];  // canonical bond kernel of (sweep, a * S + b)
+    int32_t npad_max, be_max;                // LDS carve-out (largest point of the launch)
+};
+
+struct ChainCol {      // one (a, b) column at one step
+    int32_t valid, sj, hl_in, hl_out;   // input bead, hl field of the input / output end
+};
+
+__device__ __forceinline__ int chain_phys(int i) { return i + (i >> 4); }  // 1 pad word per 16: conflict-free B reads
+
+template <typename T, int EPC>
+__global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __restrict__ psys,
+                                                         const DevPoint* __restrict__ pts,
+                                                         const int32_t* __restrict__ evalflag, int prec, ChainArgs ca,
+                                                         const int32_t* __restrict__ seg_states,
+                                                         const int32_t* __restrict__ band, const T* __restrict__ kern,
+                                                         const T* __restrict__ elam, const T* __restrict__ hl,
+                                                         const T* __restrict__ trapez, T* __restrict__ gC,
+                                                         T* __restrict__ gP) {
+    const int p = blockIdx.x, sw = blockIdx.y;
+    if (evalflag[p] != prec) return;
+    const DevSystem& sys = *psys;
+    const DevPoint& P = pts[p];
+    const int NP = P.NP, L = P.L;
+    const int B = band[P.kset];
+    const int Be = (B + 7) & ~7;               // half-width rounded up to a multiple of 8
+    const int KS = Be / 2 + 4;                 // K / 4 with K = 2 Be + 16, a multiple of 4
+    const int ntile = (NP + 255) >> 8;
+    const int XL = ntile * 256 + 2 * Be;       // extended input length of this point
+    const int S = ca.S, C = S * S, N = ca.nseg;
+    const int KC = 2 * ca.be_max + 32;
+    const int XLm = ca.npad_max + 2 * ca.be_max;
+    const int XLP = XLm + (XLm >> 4) + 1;
+    const int NPQ = ca.npad_max;
+
+    extern __shared__ unsigned char chain_lds[];
+    T* cw = reinterpret_cast<T*>(chain_lds);   // [C][KC]
+    T* X = cw + C * KC;                        // [C][XLP]
+    T* Q = X + C * XLP;                        // [S][NPQ]
+    __shared__ ChainCol info[2][SDFT_MAX_S * SDFT_MAX_S];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int* st = seg_states + ca.state_off;
+    const double norm = P.normative[ca.u];
+
+    // operator windows: cw[col][ci] = c[ci - 15 - Be] for |ci - 15 - Be| <= Be, else 0
+    {
+        const T* cset = kern + (int64_t)P.kbase * sys.K;
+        for (int e = tid; e < C * KC; e += CHAIN_THREADS) {
+            const int col = e / KC, ci = e - col * KC;
+            const int k = ca.ck[sw][col];
+            const int j = ci - 15 - Be;
+            T v = T(0);
+            if (k >= 0 && j >= -Be && j <= Be) {
+                int jj = j % L;
+                if (jj < 0) jj += L;
+                v = cset[(int64_t)k * L + jj];
+            }
+            cw[e] = v;
+        }
+    }
+
+    // input weights W_ab of the extended element mi of each column; with EPC > 0 each thread keeps
+    // the weights of its elements (mi = tid + k * CHAIN_THREADS) in registers
+    T wreg[SDFT_MAX_S * SDFT_MAX_S][EPC > 0 ? EPC : 1];
+    const int64_t eb = P.base * (int64_t)ca.NB;
+    const int64_t hb = P.base * (int64_t)(2 * ca.MB);
+    auto ext_index = [&](int mi) -> int {  // point index of extended element mi, -1 beyond the mirror
+        const int m = mi - Be;
+        return m < 0 ? -1 - m : (m < NP ? m : 2 * NP - 1 - m);
+    };
+
+    for (int s = 1; s < N; ++s) {
+        const int outseg = sw == 0 ? N - 1 - s : s;
+        const int inseg = sw == 0 ? outseg + 1 : outseg - 1;
+        ChainCol* ci = info[s & 1];
+        if (tid < C) {
+            const int a = tid / S, b = tid - a * S;
+            const int si = st[outseg * S + a], sj = st[inseg * S + b];
+            ChainCol cc;
+            cc.valid = (si >= 0 && sj >= 0 && ca.ck[sw][tid] >= 0) ? 1 : 0;
+            cc.sj = sj < 0 ? 0 : sj;
+            int bond = 0;
+            if (cc.valid) {
+                const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
+                for (int u2 = 0; u2 < ca.MB; ++u2)
+                    if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
+            }
+            cc.hl_in = 2 * bond + (si <= sj ? 1 : 0);
+            cc.hl_out = 2 * bond + (si <= sj ? 0 : 1);
+            ci[tid] = cc;
+        }
+        __syncthreads();  // column info of this step; Q of the previous step complete
+        // ---- extended inputs X = W * Q (Q = 1 at the free chain end)
+        const bool first = s == 1;
+#pragma unroll
+        for (int col = 0; col < SDFT_MAX_S * SDFT_MAX_S; ++col) {
+            if (col < C) {
+                const ChainCol cc = ci[col];
+                const T* el = elam + eb + (int64_t)cc.sj * NP;
+                const T* tz = trapez + eb + (int64_t)cc.sj * NP;
+                const T* hh = hl + hb + (int64_t)cc.hl_in * NP;
+                const T* qb = Q + (col % S) * NPQ;
+                T* xc = X + col * XLP;
+                if (EPC > 0) {
+                    if (first || !ca.homog) {
+#pragma unroll
+                        for (int k = 0; k < (EPC > 0 ? EPC : 1); ++k) {
+                            const int mi = tid + k * CHAIN_THREADS;
+                            const int n = ext_index(mi);
+                            wreg[col][k] = (mi < XL && cc.valid && n >= 0) ? el[n] * hh[n] * tz[n] : T(0);
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < (EPC > 0 ? EPC : 1); ++k) {
+                        const int mi = tid + k * CHAIN_THREADS;
+                        if (mi < XL) {
+                            const int n = ext_index(mi);
+                            const T v = (first || n < 0) ? wreg[col][k] : wreg[col][k] * qb[n];
+                            xc[chain_phys(mi)] = v;
+                        }
+                    }
+                } else {
+                    for (int mi = tid; mi < XL; mi += CHAIN_THREADS) {
+                        const int n = ext_index(mi);
+                        T v = (cc.valid && n >= 0) ? el[n] * hh[n] * tz[n] : T(0);
+                        if (!first && n >= 0) v *= qb[n];
+                        xc[chain_phys(mi)] = v;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // ---- MFMA tiles: unit = (a, tile), summing the S input states into Q_a
+        T* gout = sw == 0 ? gC : gP;
+        const int64_t gb = P.base * (int64_t)ca.GF + ca.goff;
+        const int q = lane & 15, g = lane >> 4;
+        for (int unit = wave; unit < S * ntile; unit += CHAIN_WAVES) {
+            const int a = unit / ntile, tile = unit - a * ntile;
+            const int p0 = tile * 16;
+            T qs[4] = {T(0), T(0), T(0), T(0)};
+            for (int b = 0; b < S; ++b) {
+                const int col = a * S + b;
+                const ChainCol cc = ci[col];
+                if (!cc.valid) continue;
+                const T* cwc = cw + col * KC + 15 + q - g + 2 * Be;
+                const T* xc = X + col * XLP;
+                const int xb = 16 * (p0 + q) + g;
+                typename Mfma<T>::acc_t acc;
+                for (int v = 0; v < 4; ++v) acc[v] = T(0);
+                for (int ks = 0; ks < KS; ks += 4) {  // KS is a multiple of 4
+#pragma unroll
+                    for (int k4 = 0; k4 < 4; ++k4) {
+                        const int mi = xb + 4 * (ks + k4);
+                        acc = Mfma<T>::mma(cwc[-4 * (ks + k4)], xc[chain_phys(mi)], acc);
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int n = 16 * (p0 + q) + Mfma<T>::row(lane, v);
+                    if (n < NP) {
+                        const T y = acc[v];
+                        const T val = (y < T(0) ? -y : y) * hl[hb + (int64_t)cc.hl_out * NP + n] / (T)norm;
+                        gout[gb + (((int64_t)outseg * S + b) * S + a) * NP + n] = val;
+                        qs[v] += val;
+                    }
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int n = 16 * (p0 + q) + Mfma<T>::row(lane, v);
+                if (n < NP) Q[a * NPQ + n] = qs[v];
+            }
+        }
+        // the next step's first barrier orders these Q writes before their reads
+    }
+}
+
+size_t chain_lds_bytes(const ChainPlanHost& ch, size_t elem) {
+    const int S = ch.S, C = S * S;
+    const int KC = 2 * ch.be_max + 32;
+    const int XLm = ch.npad_max + 2 * ch.be_max;
+    const int XLP = XLm + (XLm >> 4) + 1;
+    return (size_t)(C * KC + C * XLP + S * ch.npad_max) * elem;
+}
+
+template <typename T, int EPC>
+static hipError_t chain_set_lds(size_t bytes) {
+    if (bytes <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chain<T, EPC>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t chain_prepare(const ChainPlanHost& ch) {
+    const size_t b4 = chain_lds_bytes(ch, 4), b8 = chain_lds_bytes(ch, 8);
+    hipError_t e;
+    if ((e = chain_set_lds<float, 0>(b4)) != hipSuccess) return e;
+    if ((e = chain_set_lds<float, 2>(b4)) != hipSuccess) return e;
+    if ((e = chain_set_lds<float, 3>(b4)) != hipSuccess) return e;
+    if ((e = chain_set_lds<float, 4>(b4)) != hipSuccess) return e;
+    if ((e = chain_set_lds<double, 0>(b8)) != hipSuccess) return e;
+    if ((e = chain_set_lds<double, 2>(b8)) != hipSuccess) return e;
+    if ((e = chain_set_lds<double, 3>(b8)) != hipSuccess) return e;
+    return chain_set_lds<double, 4>(b8);
+}
+
+template <typename T>
+void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, const T* elam, const T* hl,
+                  const T* trapez, T* gC, T* gP) {
+    const SpeciesHost& sh = c.species[u];
+    const ChainPlanHost& ch = c.chain_plan[u];
+    ChainArgs ca;
+    memset(&ca, 0, sizeof(ca));
+    ca.u = u; ca.nseg = sh.nseg; ca.S = sh.S; ca.state_off = sh.state_off;
+    ca.GF = c.GF; ca.MB = c.MB; ca.NB = c.NB; ca.homog = ch.homog ? 1 : 0;
+    ca.goff = sh.goff;
+    for (int sw = 0; sw < 2; ++sw)
+        for (int k = 0; k < SDFT_MAX_S * SDFT_MAX_S; ++k) ca.ck[sw][k] = ch.ck[sw][k];
+    ca.npad_max = ch.npad_max; ca.be_max = ch.be_max;
+    const size_t lds = chain_lds_bytes(ch, sizeof(T));
+    const int xl = ch.npad_max + 2 * ch.be_max;
+    const int epc = (xl + CHAIN_THREADS - 1) / CHAIN_THREADS;
+    auto go = [&](auto kfn) {
+        hipLaunchKernelGGL(kfn, dim3(c.P, 2), dim3(CHAIN_THREADS), lds, st, c.d_sys, c.d_pts, c.d_evalflag, prec, ca,
+                           c.d_seg_states, c.d_band, kern, elam, hl, trapez, gC, gP);
+    };
+    if (epc <= 2) go(k_chain<T, 2>);
+    else if (epc <= 3) go(k_chain<T, 3>);
+    else if (epc <= 4) go(k_chain<T, 4>);
+    else go(k_chain<T, 0>);
+}
+
+template void launch_chain<float>(hipStream_t, const Ctx&, int, int, const float*, const float*, const float*,
+                                  const float*, float*, float*);
+template void launch_chain<double>(hipStream_t, const Ctx&, int, int, const double*, const double*, const double*,
+                                   const double*, double*, double*);
+
+}  // namespace sdft

@@ -17,7 +17,8 @@
 #define SDFT_MAX_SPECIES 16
 #define SDFT_MAX_M 8
 #define SDFT_MAX_TILES 128          // 256-point tiles per grid (NP <= 32768)
-#define SDFT_GRAM_BLOCKS 64         // blocks per point for the Anderson reductions
+#define SDFT_GRAM_BLOCKS 64         // most blocks per point for the Anderson reductions
+#define SDFT_GRAM_CHUNK 2048        // solution entries per reduction block
 #define SDFT_GRAM_LEN 44            // SDFT_MAX_M*(SDFT_MAX_M+1)/2 + SDFT_MAX_M
 #define SDFT_OMEGA_BLOCKS 8
 
@@ -74,6 +75,9 @@ struct DevSystem {
     int32_t bond_j[SDFT_MAX_MB];
     int32_t hard_walls;
     int32_t charged_walls;
+    int32_t ncls;                     // distinct bead diameters (the MSA sums run over these classes)
+    int32_t cls[SDFT_MAX_NB];         // class of each bead
+    double cls_sigma[SDFT_MAX_NB];    // diameter of each class
 };
 
 // one term of a convolution job: out += sign-structured T(+-)H(kern) * in
@@ -106,7 +110,7 @@ struct DevSolveState {
     int32_t mode;         // 0 Picard, 1 Anderson, for the pending update
     int32_t ncols;        // number of difference columns used by the pending update
     int32_t failed;       // 1: max_iters reached, 2: non-finite residual
-    int32_t pad;
+    int32_t want64;       // mixed mode: residual below the switch threshold, switch at the next poll
     double err;
     double amix;
     double lam[SDFT_MAX_M];

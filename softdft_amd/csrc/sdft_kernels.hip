@@ -11,6 +11,7 @@
 // kernels below apply that operator directly (Toeplitz +- Hankel), exact to
 // round-off, so no arbitrary-length FFT is needed; the c vectors are built
 // once per distinct (NP, dz) by k_build_cvec.
+#include <type_traits>
 #include "sdft_launch.h"
 
 #include <math.h>
@@ -424,10 +425,12 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
     const int n = blockIdx.x * TB + threadIdx.x;
     const bool valid = n < NP;
     const int nc = valid ? n : NP - 1;
-    double rb[SDFT_MAX_NB];
-    double rbo[2 * SDFT_MAX_MB];
-    for (int j = 0; j < NB; ++j) rb[j] = 0.0;
-    for (int f = 0; f < 2 * MB; ++f) rbo[f] = 0.0;
+    // per-thread accumulators in LDS, one column per thread: rb[j * TB], rbo[f * TB]
+    extern __shared__ double acc_lds[];
+    double* rb = acc_lds + threadIdx.x;
+    double* rbo = acc_lds + NB * TB + threadIdx.x;
+    for (int j = 0; j < NB; ++j) rb[j * TB] = 0.0;
+    for (int f = 0; f < 2 * MB; ++f) rbo[f * TB] = 0.0;
     __shared__ double red[2 * SDFT_MAX_SPECIES][TB / 64];
     for (int u = 0; u < sys.n_species; ++u) {
         const DevSpecies sp = spec[u];
@@ -473,12 +476,12 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
                 if (valid) {
                     if (q == sp.fixed_seg) {
                         if (nc == Kf) {
-                            rb[si] += rs / P.dz / tz;
+                            rb[si * TB] += rs / P.dz / tz;
                             num += rs;
                             chg += rs * sys.z[si];
                         }
                     } else {
-                        rb[si] += rs;
+                        rb[si * TB] += rs;
                         num += rs * tz * P.dz;
                         chg += rs * tz * P.dz * sys.z[si];
                     }
@@ -490,7 +493,7 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
                             int bond = 0;
                             for (int u2 = 0; u2 < MB; ++u2)
                                 if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
-                            rbo[2 * bond + 0] += contr * (double)gC[gidx(q, b, a, nc)];
+                            rbo[(2 * bond + 0) * TB] += contr * (double)gC[gidx(q, b, a, nc)];
                         }
                     }
                     if (q > 0) {
@@ -502,7 +505,7 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
                             int bond = 0;
                             for (int u2 = 0; u2 < MB; ++u2)
                                 if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
-                            rbo[2 * bond + 1] += c1 * (double)gP[gidx(q, b, a, nc)];
+                            rbo[(2 * bond + 1) * TB] += c1 * (double)gP[gidx(q, b, a, nc)];
                         }
                     }
                 }
@@ -525,14 +528,14 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
     }
     if (!valid) return;
     for (int j = 0; j < NB; ++j) {
-        rho_beads[fidx(P, NB, j, n)] = (T)rb[j];
-        rhoT[fidx(P, NB + 2 * MB, j, n)] = (T)(rb[j] * (double)trapez[fidx(P, NB, j, n)]);
+        rho_beads[fidx(P, NB, j, n)] = (T)rb[j * TB];
+        rhoT[fidx(P, NB + 2 * MB, j, n)] = (T)(rb[j * TB] * (double)trapez[fidx(P, NB, j, n)]);
     }
     for (int u = 0; u < MB; ++u) {
         for (int e = 0; e < 2; ++e) {
             const int bt = e == 0 ? sys.bond_i[u] : sys.bond_j[u];
-            rho_bonds[fidx(P, 2 * MB, 2 * u + e, n)] = (T)rbo[2 * u + e];
-            rhoT[fidx(P, NB + 2 * MB, NB + 2 * u + e, n)] = (T)(rbo[2 * u + e] * (double)trapez[fidx(P, NB, bt, n)]);
+            rho_bonds[fidx(P, 2 * MB, 2 * u + e, n)] = (T)rbo[(2 * u + e) * TB];
+            rhoT[fidx(P, NB + 2 * MB, NB + 2 * u + e, n)] = (T)(rbo[(2 * u + e) * TB] * (double)trapez[fidx(P, NB, bt, n)]);
         }
     }
 }
@@ -597,148 +600,26 @@ __global__ void __launch_bounds__(TB) k_conv(const DevSystem* __restrict__ psys,
 //   dphi[3NB + 2MB]  = surf + sum_j z_j rho_j trapez_j                                     -> Coulomb
 // --------------------------------------------------------------------------
 
-struct GammaOut { double G, eta, H; };
-
-__device__ GammaOut gamma_msa(const double* rho, const DevSystem& sys) {
-    const int NB = sys.NB;
-    const double tol = 1e-10;
-    double s_rz = 0.0, s_r = 0.0, s_rs = 0.0, s_rs3 = 0.0;
-    for (int j = 0; j < NB; ++j) {
-        s_rz += rho[j] * sys.z[j] * sys.z[j];
-        s_r += rho[j];
-        s_rs += rho[j] * sys.sigma[j];
-        s_rs3 += rho[j] * sys.sigma[j] * sys.sigma[j] * sys.sigma[j];
-    }
-    const double kap = sqrt(4.0 * M_PI * sys.lB * s_rz);
-    GammaOut o;
-    if (kap < tol) {
-        o.G = kap / 2.0; o.eta = 0.0; o.H = 2.0 / M_PI;
-        return o;
-    }
-    const double seff = s_rs / s_r;
-    const double x = seff * sqrt(4.0 * M_PI * sys.lB * s_rz);
-    double G = (sqrt(1.0 + 2.0 * x) - 1.0) / (2.0 * seff);
-    const double term2 = 2.0 / M_PI * (1.0 - M_PI / 6.0 * s_rs3);
-    double H = 0.0, eta = 0.0, G1 = 0.0;
-    int it = 0;
-    while (true) {
-        double h = 0.0, e = 0.0;
-        for (int j = 0; j < NB; ++j) {
-            const double s = sys.sigma[j];
-            h += rho[j] * s * s * s / (1.0 + G * s);
-            e += rho[j] * sys.z[j] * s / (1.0 + G * s);
-        }
-        H = h + term2;
-        eta = e / H;
-        double g1 = 0.0;
-        for (int j = 0; j < NB; ++j) {
-            const double s = sys.sigma[j];
-            const double q = sys.z[j] - eta * s * s;
-            g1 += rho[j] * q * q / (1.0 + G * s) / (1.0 + G * s);
-        }
-        G1 = sqrt(M_PI * sys.lB * g1);
-        const double Gn = 0.9 * G + 0.1 * G1;
-        ++it;
-        if (fabs(G1 - G) < tol || it >= 1000) {
-            G = Gn;
-            break;
-        }
-        G = Gn;
-    }
-    o.G = G; o.eta = eta; o.H = H;
-    return o;
-}
-
-// compute_msa_deriv! (iLST msa.jl:160-170)
-__device__ void msa_deriv(const double* rho, const DevSystem& sys, const GammaOut& g, double* dphi) {
-    const int NB = sys.NB;
-    double ustar = 0.0;
-    for (int j = 0; j < NB; ++j) {
-        const double s = sys.sigma[j];
-        ustar += rho[j] * s * s * ((sys.z[j] - g.eta * s * s) / (1.0 + g.G * s) + sys.z[j]);
-    }
-    ustar *= -M_PI / 6.0;
-    for (int j = 0; j < NB; ++j) {
-        const double s = sys.sigma[j], z = sys.z[j];
-        dphi[j] = -sys.lB * (g.G * z * z / (1.0 + g.G * s) +
-                             g.eta * s * ((2.0 * z - g.eta * s * s) / (1.0 + g.G * s) + g.eta * s * s / 3.0) + z * ustar);
-    }
-}
-
-// compute_TPT1msa_deriv! (iLST TPT1msa.jl:210-282) -- adds the chain term to dphi, fills lng
-__device__ void tpt1msa_extra(const double* rho, const double* rbond /*[2MB]*/, const DevSystem& sys, const GammaOut& g,
-                              double* dphi, double* lng) {
-    const int NB = sys.NB, MB = sys.MB;
-    const double lB = sys.lB, G = g.G, eta = g.eta, H = g.H;
-    for (int u = 0; u < MB; ++u) {
-        const int i = sys.bond_i[u], j = sys.bond_j[u];
-        const double sij = (sys.sigma[i] + sys.sigma[j]) / 2.0;
-        if (G == 0.0) {
-            lng[u] = -lB / sij * sys.z[i] * sys.z[j];
-        } else {
-            const double Xi = (sys.z[i] - eta * sys.sigma[i] * sys.sigma[i]) / (1.0 + G * sys.sigma[i]);
-            const double Xj = (sys.z[j] - eta * sys.sigma[j] * sys.sigma[j]) / (1.0 + G * sys.sigma[j]);
-            lng[u] = -lB / sij * Xi * Xj;
-        }
-    }
-    if (G == 0.0) return;
-    double dfG = 0.0, dfe = 0.0, deG = 0.0, dHG = 0.0;
-    for (int w = 0; w < NB; ++w) {
-        const double s = sys.sigma[w], z = sys.z[w];
-        const double opg = 1.0 + G * s;
-        const double q = z - eta * s * s;
-        dfG += rho[w] * s * q * q / opg / opg / opg;
-        dfe += rho[w] * s * s * q / opg / opg;
-        deG += rho[w] * z * s * s / opg / opg;
-        dHG += rho[w] * s * s * s * s / opg / opg;
-    }
-    dfG *= -2.0 * M_PI * lB;
-    dfe *= -2.0 * M_PI * lB;
-    deG *= -1.0 / H;
-    dHG = -dHG;
-    const double deH = -eta / H;
-    const double detaG = deG + deH * dHG;
-    const double denom = 1.0 / (2.0 * G - dfe * detaG - dfG);
-    for (int w = 0; w < NB; ++w) {
-        const double s = sys.sigma[w], z = sys.z[w];
-        const double ig = 1.0 / (1.0 + G * s);
-        const double Xw = (z - eta * s * s) * ig;
-        const double dfr = M_PI * lB * Xw * Xw;
-        const double der = s / H * z * ig;
-        const double dHr = s * s * s * (ig - 1.0 / 3.0);
-        const double dGr = (dfr + dfe * (der + dfe * deH * dHr)) * denom;
-        const double detr = detaG * dGr;
-        double add = 0.0;
-        for (int u = 0; u < MB; ++u) {
-            const int i = sys.bond_i[u], j = sys.bond_j[u];
-            const double sij = (sys.sigma[i] + sys.sigma[j]) / 2.0;
-            const double igi = 1.0 / (1.0 + G * sys.sigma[i]);
-            const double Xi = (sys.z[i] - eta * sys.sigma[i] * sys.sigma[i]) * igi;
-            const double dXi = -(sys.sigma[i] * sys.sigma[i] * igi * detr + sys.sigma[i] * Xi * igi * dGr);
-            const double igj = 1.0 / (1.0 + G * sys.sigma[j]);
-            const double Xj = (sys.z[j] - eta * sys.sigma[j] * sys.sigma[j]) * igj;
-            const double dXj = -(sys.sigma[j] * sys.sigma[j] * igj * detr + sys.sigma[j] * Xj * igj * dGr);
-            add += (rbond[2 * u] + rbond[2 * u + 1]) / 2.0 * lB / sij * (dXi * Xj + Xi * dXj);
-        }
-        dphi[w] += add;
-    }
-}
-
 struct FmtIn {
     double n0, n1, n2, n3, nV1, nV2;
     double s3;   // sum of capped n3_i (TPT1) or capped n3
 };
 
-__device__ FmtIn fmt_collect(const double* n2j, const double* n3j, const double* nVj, const DevSystem& sys) {
+// weighted densities of the mixture at point n, read from the nw family
+template <typename T>
+__device__ FmtIn fmt_load(const DevSystem& sys, const DevPoint& P, int n, const T* __restrict__ nw, int Fnw) {
     FmtIn f{0, 0, 0, 0, 0, 0, 0};
-    for (int j = 0; j < sys.NB; ++j) {
+    const int NB = sys.NB;
+    for (int j = 0; j < NB; ++j) {
         const double s = sys.sigma[j];
-        f.n0 += n2j[j] / (M_PI * s * s);
-        f.n1 += n2j[j] / (2.0 * M_PI * s);
-        f.n2 += n2j[j];
-        f.n3 += n3j[j];
-        f.nV1 += nVj[j] / (2.0 * M_PI * s);
-        f.nV2 += nVj[j];
+        const double n2 = (double)nw[fidx(P, Fnw, j, n)];
+        const double nv = (double)nw[fidx(P, Fnw, 2 * NB + j, n)];
+        f.n0 += n2 / (M_PI * s * s);
+        f.n1 += n2 / (2.0 * M_PI * s);
+        f.n2 += n2;
+        f.n3 += (double)nw[fidx(P, Fnw, NB + j, n)];
+        f.nV1 += nv / (2.0 * M_PI * s);
+        f.nV2 += nv;
     }
     if (f.n3 > 0.95) {  // cap (mfmt.jl:126-131 / TPT1mfmt.jl:154-159: uniform scaling)
         const double fac = 0.95 / f.n3;
@@ -777,19 +658,113 @@ __device__ double fmt_phi(const FmtIn& f) {
     return -f.n0 * lg + (f.n1 * f.n2 - dot12) / om + cubic;
 }
 
-template <typename T>
-__device__ void load_point_inputs(const DevSystem& sys, const DevPoint& P, int n, const T* nw, int Fnw,
-                                  double* n2j, double* n3j, double* nVj, double* rbond) {
-    const int NB = sys.NB, MB = sys.MB;
-    for (int j = 0; j < NB; ++j) {
-        n2j[j] = (double)nw[fidx(P, Fnw, j, n)];
-        n3j[j] = (double)nw[fidx(P, Fnw, NB + j, n)];
-        nVj[j] = (double)nw[fidx(P, Fnw, 2 * NB + j, n)];
+// per-class sums of the MSA densities over the beads of one diameter:
+// C = sum rho, B = sum rho z, A = sum rho z^2. Every MSA sum over beads
+// (msa.jl:181-230, TPT1msa.jl:210-282) depends on a bead only through
+// (sigma, rho, rho z, rho z^2), so it runs over the NCL classes instead.
+template <int NCL>
+struct MsaState {
+    double C[NCL], B[NCL], A[NCL];
+    double ig[NCL];           // 1 / (1 + G sigma_c) at the final G
+    double G, eta, H;
+};
+
+// Gamma_msa (msa.jl:181-230): damped fixed point G <- 0.9 G + 0.1 G1, stop at |G1 - G| < 1e-10
+template <int NCL>
+__device__ __forceinline__ void gamma_msa_cls(MsaState<NCL>& m, const DevSystem& sys) {
+    const double tol = 1e-10, lB = sys.lB;
+    double s_rz = 0.0, s_r = 0.0, s_rs = 0.0, s_rs3 = 0.0;
+#pragma unroll
+    for (int c = 0; c < NCL; ++c) {
+        const double s = sys.cls_sigma[c];
+        s_rz += m.A[c];
+        s_r += m.C[c];
+        s_rs += m.C[c] * s;
+        s_rs3 += m.C[c] * s * s * s;
     }
-    for (int f = 0; f < 2 * MB; ++f) rbond[f] = (double)nw[fidx(P, Fnw, 3 * NB + f, n)];
+    const double kap = sqrt(4.0 * M_PI * lB * s_rz);
+    if (kap < tol) {
+        m.G = kap / 2.0; m.eta = 0.0; m.H = 2.0 / M_PI;
+#pragma unroll
+        for (int c = 0; c < NCL; ++c) m.ig[c] = 1.0 / (1.0 + m.G * sys.cls_sigma[c]);
+        return;
+    }
+    const double seff = s_rs / s_r;
+    const double x = seff * kap;
+    double G = (sqrt(1.0 + 2.0 * x) - 1.0) / (2.0 * seff);
+    const double term2 = 2.0 / M_PI * (1.0 - M_PI / 6.0 * s_rs3);
+    double c3[NCL], bs[NCL];
+#pragma unroll
+    for (int c = 0; c < NCL; ++c) {
+        const double s = sys.cls_sigma[c];
+        c3[c] = m.C[c] * s * s * s;
+        bs[c] = m.B[c] * s;
+    }
+    double H = 0.0, eta = 0.0;
+    for (int it = 1;; ++it) {
+        double ig[NCL], h = 0.0, e = 0.0;
+#pragma unroll
+        for (int c = 0; c < NCL; ++c) {
+            ig[c] = 1.0 / (1.0 + G * sys.cls_sigma[c]);
+            h += c3[c] * ig[c];
+            e += bs[c] * ig[c];
+        }
+        H = h + term2;
+        eta = e / H;
+        double g1 = 0.0;
+#pragma unroll
+        for (int c = 0; c < NCL; ++c) {
+            const double s2 = sys.cls_sigma[c] * sys.cls_sigma[c];
+            // sum over the class of rho (z - eta s^2)^2
+            const double q2 = m.A[c] - 2.0 * eta * s2 * m.B[c] + eta * eta * s2 * s2 * m.C[c];
+            g1 += q2 * ig[c] * ig[c];
+        }
+        const double G1 = sqrt(M_PI * lB * g1);
+        const double Gn = 0.9 * G + 0.1 * G1;
+        const bool done = fabs(G1 - G) < tol || it >= 1000;
+        G = Gn;
+        if (done) break;
+    }
+    m.G = G; m.eta = eta; m.H = H;
+#pragma unroll
+    for (int c = 0; c < NCL; ++c) m.ig[c] = 1.0 / (1.0 + G * sys.cls_sigma[c]);
 }
 
-template <typename T>
+// class sums of rho = n2 / (pi sigma^2) at point n
+template <typename T, int NCL>
+__device__ __forceinline__ void msa_load(MsaState<NCL>& m, const DevSystem& sys, const DevPoint& P, int n, const T* __restrict__ nw,
+                         int Fnw) {
+#pragma unroll
+    for (int c = 0; c < NCL; ++c) m.C[c] = m.B[c] = m.A[c] = 0.0;
+    for (int j = 0; j < sys.NB; ++j) {
+        const double s = sys.sigma[j], z = sys.z[j];
+        const double rho = (double)nw[fidx(P, Fnw, j, n)] / (M_PI * s * s);
+        const int cj = sys.cls[j];
+#pragma unroll
+        for (int c = 0; c < NCL; ++c) {
+            if (c == cj) {
+                m.C[c] += rho;
+                m.B[c] += rho * z;
+                m.A[c] += rho * z * z;
+            }
+        }
+    }
+}
+
+// calls f(std::integral_constant<int, NCL>) with the number of diameter classes
+// rounded up to a power of two (padding classes hold zero density and add exact zeros)
+template <class F>
+static void cls_dispatch(int ncls, F&& f) {
+    if (ncls <= 1) f(std::integral_constant<int, 1>{});
+    else if (ncls <= 2) f(std::integral_constant<int, 2>{});
+    else if (ncls <= 4) f(std::integral_constant<int, 4>{});
+    else if (ncls <= 8) f(std::integral_constant<int, 8>{});
+    else f(std::integral_constant<int, SDFT_MAX_NB>{});
+}
+
+// pointwise functional derivatives, FP64 arithmetic, no per-bead arrays:
+// the weighted densities are re-read from nw where a second pass needs them
+template <typename T, int NCL>
 __global__ void __launch_bounds__(256) k_pointwise(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
                                                   const int32_t* __restrict__ evalflag, int prec,
                                                   const T* __restrict__ nw, int Fnw, const T* __restrict__ rhoT, int Frt,
@@ -801,70 +776,134 @@ __global__ void __launch_bounds__(256) k_pointwise(const DevSystem* __restrict__
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= P.NP) return;
     const int NB = sys.NB, MB = sys.MB;
-    double n2j[SDFT_MAX_NB], n3j[SDFT_MAX_NB], nVj[SDFT_MAX_NB], rbond[2 * SDFT_MAX_MB];
-    load_point_inputs(sys, P, n, nw, Fnw, n2j, n3j, nVj, rbond);
-    double ch0[SDFT_MAX_NB], ch1[SDFT_MAX_NB], chv[SDFT_MAX_NB], lngs[SDFT_MAX_MB];
-    for (int j = 0; j < NB; ++j) ch0[j] = ch1[j] = chv[j] = 0.0;
-    for (int u = 0; u < MB; ++u) lngs[u] = 0.0;
     const int F = sys.functionals;
-    if (F & (SDFT_F_MFMT | SDFT_F_TPT1MFMT)) {
-        const FmtIn f = fmt_collect(n2j, n3j, nVj, sys);
-        if (f.s3 > 1e-10) {
-            double d[6];
-            fmt_deriv(f, d);
-            if (F & SDFT_F_TPT1MFMT) {
-                // TPT1 chain term (TPT1mfmt.jl:387-445) with the bond weighted densities
-                const double Xi = fmax(1.0 - f.nV2 * f.nV2 / (f.n2 * f.n2), 0.0);
-                const double i1 = 1.0 / (1.0 - f.n3);
-                const double t2 = f.n2 * Xi / 4.0 * i1 * i1;
-                const double t3 = f.n2 * f.n2 * Xi / 72.0 * i1 * i1 * i1;
-                for (int u = 0; u < MB; ++u) {
-                    const double si = sys.sigma[sys.bond_i[u]], sj = sys.sigma[sys.bond_j[u]];
-                    const double sij = (si + sj) / 2.0, tau = si * sj / sij;
-                    const double y = i1 + t2 * tau + t3 * tau * tau;
-                    const double dn2 = (tau / 4.0 * i1 * i1 + f.n2 * tau * tau * i1 * i1 * i1 / 36.0) / y;
-                    const double dn3 = (i1 * i1 + f.n2 * Xi / 2.0 * i1 * i1 * i1 * tau +
-                                        f.n2 * f.n2 * Xi / 24.0 * i1 * i1 * i1 * i1 * tau * tau) / y;
-                    const double dv = -f.nV2 * (tau / (2.0 * f.n2) * i1 * i1 + tau * tau / 36.0 * i1 * i1 * i1) / y;
-                    const double rb = rbond[2 * u] + rbond[2 * u + 1];
-                    d[2] -= 0.5 * rb * dn2;
-                    d[3] -= 0.5 * rb * dn3;
-                    d[5] -= 0.5 * rb * dv;
-                    lngs[u] += log(y);
-                }
+    const bool has_fmt = (F & (SDFT_F_MFMT | SDFT_F_TPT1MFMT)) != 0;
+    const bool has_msa = (F & (SDFT_F_MSA | SDFT_F_TPT1MSA)) != 0;
+    const bool tpt1msa = (F & SDFT_F_TPT1MSA) != 0;
+
+    // ---- FMT (mfmt.jl:106-199, TPT1mfmt.jl:135-241)
+    FmtIn f{0, 0, 0, 0, 0, 0, 0};
+    bool fmt_on = false;
+    double d[6] = {0, 0, 0, 0, 0, 0};
+    if (has_fmt) {
+        f = fmt_load(sys, P, n, nw, Fnw);
+        fmt_on = f.s3 > 1e-10;
+        if (fmt_on) fmt_deriv(f, d);
+    }
+
+    // ---- MSA (msa.jl:160-230, TPT1msa.jl:210-282)
+    MsaState<NCL> m;
+    double ustar = 0.0, dfe = 0.0, deH = 0.0, detaG = 0.0, denom = 0.0;
+    if (has_msa) {
+        msa_load<T, NCL>(m, sys, P, n, nw, Fnw);
+        gamma_msa_cls<NCL>(m, sys);
+#pragma unroll
+        for (int c = 0; c < NCL; ++c) {
+            const double s = sys.cls_sigma[c];
+            ustar += s * s * (m.ig[c] * (m.B[c] - m.eta * s * s * m.C[c]) + m.B[c]);
+        }
+        ustar *= -M_PI / 6.0;
+        if (tpt1msa && m.G != 0.0) {
+            double dfG = 0.0, deG = 0.0, dHG = 0.0;
+#pragma unroll
+            for (int c = 0; c < NCL; ++c) {
+                const double s = sys.cls_sigma[c], s2 = s * s, ig = m.ig[c];
+                const double q2 = m.A[c] - 2.0 * m.eta * s2 * m.B[c] + m.eta * m.eta * s2 * s2 * m.C[c];
+                dfG += s * q2 * ig * ig * ig;
+                dfe += s2 * (m.B[c] - m.eta * s2 * m.C[c]) * ig * ig;
+                deG += s2 * m.B[c] * ig * ig;
+                dHG += s2 * s2 * m.C[c] * ig * ig;
             }
-            for (int j = 0; j < NB; ++j) {
-                const double s = sys.sigma[j];
-                ch0[j] = d[0] / (M_PI * s * s) + d[1] / (2.0 * M_PI * s) + d[2];
-                ch1[j] = d[3];
-                chv[j] = d[4] / (2.0 * M_PI * s) + d[5];
-            }
+            dfG *= -2.0 * M_PI * sys.lB;
+            dfe *= -2.0 * M_PI * sys.lB;
+            deG *= -1.0 / m.H;
+            dHG = -dHG;
+            deH = -m.eta / m.H;
+            detaG = deG + deH * dHG;
+            denom = 1.0 / (2.0 * m.G - dfe * detaG - dfG);
         }
     }
-    if (F & (SDFT_F_MSA | SDFT_F_TPT1MSA)) {
-        double rho[SDFT_MAX_NB], dm[SDFT_MAX_NB], lm[SDFT_MAX_MB];
-        for (int j = 0; j < NB; ++j) rho[j] = n2j[j] / (M_PI * sys.sigma[j] * sys.sigma[j]);
-        const GammaOut g = gamma_msa(rho, sys);
-        msa_deriv(rho, sys, g, dm);
-        if (F & SDFT_F_TPT1MSA) {
-            tpt1msa_extra(rho, rbond, sys, g, dm, lm);
-            for (int u = 0; u < MB; ++u) lngs[u] += lm[u];
-        }
-        for (int j = 0; j < NB; ++j) ch0[j] += dm[j] / (M_PI * sys.sigma[j] * sys.sigma[j]);
-    }
-    for (int j = 0; j < NB; ++j) {
-        dphi[fidx(P, Fdp, j, n)] = (T)ch0[j];
-        dphi[fidx(P, Fdp, NB + j, n)] = (T)ch1[j];
-        dphi[fidx(P, Fdp, 2 * NB + j, n)] = (T)chv[j];
+
+    // ---- bonds: TPT1 chain terms and the bond outputs (TPT1mfmt.jl:387-445, TPT1msa.jl:232-282)
+    double Sa = 0.0, Sb = 0.0;
+    const bool tpt1fmt_on = fmt_on && (F & SDFT_F_TPT1MFMT);
+    double Xif = 0.0, i1 = 0.0, t2 = 0.0, t3 = 0.0;
+    if (tpt1fmt_on) {
+        Xif = fmax(1.0 - f.nV2 * f.nV2 / (f.n2 * f.n2), 0.0);
+        i1 = 1.0 / (1.0 - f.n3);
+        t2 = f.n2 * Xif / 4.0 * i1 * i1;
+        t3 = f.n2 * f.n2 * Xif / 72.0 * i1 * i1 * i1;
     }
     for (int u = 0; u < MB; ++u) {
-        for (int e = 0; e < 2; ++e) {
-            const int bt = e == 0 ? sys.bond_i[u] : sys.bond_j[u];
-            dphi[fidx(P, Fdp, 3 * NB + 2 * u + e, n)] = (T)(lngs[u] / (M_PI * sys.sigma[bt] * sys.sigma[bt]));
+        const int bi = sys.bond_i[u], bj = sys.bond_j[u];
+        const double si = sys.sigma[bi], sj = sys.sigma[bj];
+        const double sij = (si + sj) / 2.0;
+        double lng = 0.0;
+        if (tpt1fmt_on || tpt1msa) {
+            const double rb = (double)nw[fidx(P, Fnw, 3 * NB + 2 * u, n)] + (double)nw[fidx(P, Fnw, 3 * NB + 2 * u + 1, n)];
+            if (tpt1fmt_on) {
+                const double tau = si * sj / sij;
+                const double y = i1 + t2 * tau + t3 * tau * tau;
+                const double dn2 = (tau / 4.0 * i1 * i1 + f.n2 * tau * tau * i1 * i1 * i1 / 36.0) / y;
+                const double dn3 = (i1 * i1 + f.n2 * Xif / 2.0 * i1 * i1 * i1 * tau +
+                                    f.n2 * f.n2 * Xif / 24.0 * i1 * i1 * i1 * i1 * tau * tau) / y;
+                const double dv = -f.nV2 * (tau / (2.0 * f.n2) * i1 * i1 + tau * tau / 36.0 * i1 * i1 * i1) / y;
+                d[2] -= 0.5 * rb * dn2;
+                d[3] -= 0.5 * rb * dn3;
+                d[5] -= 0.5 * rb * dv;
+                lng += log(y);
+            }
+            if (tpt1msa) {
+                const double zi = sys.z[bi], zj = sys.z[bj];
+                if (m.G == 0.0) {
+                    lng += -sys.lB / sij * zi * zj;
+                } else {
+                    const double igi = 1.0 / (1.0 + m.G * si), igj = 1.0 / (1.0 + m.G * sj);
+                    const double Xi = (zi - m.eta * si * si) * igi, Xj = (zj - m.eta * sj * sj) * igj;
+                    lng += -sys.lB / sij * Xi * Xj;
+                    // d(X_i X_j)/d rho_w = detr_w (a_i X_j + X_i a_j) + dGr_w (b_i X_j + X_i b_j)
+                    const double cu = rb / 2.0 * sys.lB / sij;
+                    const double ai = -si * si * igi, aj = -sj * sj * igj;
+                    const double b_i = -si * Xi * igi, b_j = -sj * Xj * igj;
+                    Sa += cu * (ai * Xj + Xi * aj);
+                    Sb += cu * (b_i * Xj + Xi * b_j);
+                }
+            }
         }
+        dphi[fidx(P, Fdp, 3 * NB + 2 * u, n)] = (T)(lng / (M_PI * si * si));
+        dphi[fidx(P, Fdp, 3 * NB + 2 * u + 1, n)] = (T)(lng / (M_PI * sj * sj));
     }
+
+    // ---- bead outputs
     double q = surf[P.base + n];
-    for (int j = 0; j < NB; ++j) q += sys.z[j] * (double)rhoT[fidx(P, Frt, j, n)];
+    for (int j = 0; j < NB; ++j) {
+        const double s = sys.sigma[j], z = sys.z[j];
+        double ch0 = 0.0, ch1 = 0.0, chv = 0.0;
+        if (fmt_on) {
+            ch0 = d[0] / (M_PI * s * s) + d[1] / (2.0 * M_PI * s) + d[2];
+            ch1 = d[3];
+            chv = d[4] / (2.0 * M_PI * s) + d[5];
+        }
+        if (has_msa) {
+            const double ig = 1.0 / (1.0 + m.G * s);
+            const double G = m.G, eta = m.eta;
+            double dm = -sys.lB * (G * z * z * ig + eta * s * ((2.0 * z - eta * s * s) * ig + eta * s * s / 3.0) + z * ustar);
+            if (tpt1msa && G != 0.0) {
+                const double Xw = (z - eta * s * s) * ig;
+                const double dfr = M_PI * sys.lB * Xw * Xw;
+                const double der = s / m.H * z * ig;
+                const double dHr = s * s * s * (ig - 1.0 / 3.0);
+                const double dGr = (dfr + dfe * (der + dfe * deH * dHr)) * denom;
+                const double detr = detaG * dGr;
+                dm += detr * Sa + dGr * Sb;
+            }
+            ch0 += dm / (M_PI * s * s);
+        }
+        dphi[fidx(P, Fdp, j, n)] = (T)ch0;
+        dphi[fidx(P, Fdp, NB + j, n)] = (T)ch1;
+        dphi[fidx(P, Fdp, 2 * NB + j, n)] = (T)chv;
+        q += z * (double)rhoT[fidx(P, Frt, j, n)];
+    }
     dphi[fidx(P, Fdp, 3 * NB + 2 * MB, n)] = (T)q;
 }
 
@@ -923,37 +962,17 @@ __global__ void k_neutrality(const DevSystem* __restrict__ psys, const DevPoint*
 // --------------------------------------------------------------------------
 // Anderson mixing on the device (AndersonMixing.jl:56-169)
 // --------------------------------------------------------------------------
-__global__ void k_and_residual(const DevPoint* __restrict__ pts, int32_t* __restrict__ evalflag_unused,
-                               const DevSolveState* __restrict__ stt, const double* __restrict__ x,
-                               const double* __restrict__ fx, double* __restrict__ Xh, double* __restrict__ Gh,
-                               unsigned long long* __restrict__ errbits, double* __restrict__ gram, int m,
-                               int32_t* __restrict__ n_active) {
-    const int p = blockIdx.y;
-    if (blockIdx.x == 0 && p == 0 && threadIdx.x == 0) *n_active = 0;
-    const DevSolveState& S = stt[p];
-    if (S.converged) return;
-    const DevPoint& P = pts[p];
-    const int col = S.iter % m;
-    const int64_t hoff = P.xoff * m;
-    double mx = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double xi = x[P.xoff + i];
-        const double g = fx[P.xoff + i] - xi;
-        Xh[hoff + (int64_t)col * P.n + i] = xi;
-        Gh[hoff + (int64_t)col * P.n + i] = g;
-        const double a = fabs(g);
-        mx = (a > mx || a != a) ? a : mx;  // NaN propagates
-    }
-    __shared__ double red[4];
-    mx = warp_max(mx);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double v = 0.0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v = nanmax(v, red[w]);
-        if (v != v) v = INFINITY;
-        atomicMax(&errbits[p], (unsigned long long)__double_as_longlong(v));
-    }
+// The history depth M is a template parameter so that the Gram accumulators,
+// column offsets and the small least-squares solve live in registers; a point
+// whose history holds fewer than M columns masks the missing ones to zero,
+// which leaves every sum and every Jacobi rotation unchanged.
+//
+// Per-point block count of the reductions: a function of the point's own
+// length only, so the summation order (and the result) never depends on the
+// rest of the batch.
+__host__ __device__ __forceinline__ int gram_blocks(int64_t n) {
+    const int64_t b = (n + SDFT_GRAM_CHUNK - 1) / SDFT_GRAM_CHUNK;
+    return (int)(b < 1 ? 1 : (b > SDFT_GRAM_BLOCKS ? SDFT_GRAM_BLOCKS : b));
 }
 
 __device__ __forceinline__ int hist_len(const DevSolveState& S, int m) {
@@ -962,77 +981,127 @@ __device__ __forceinline__ int hist_len(const DevSolveState& S, int m) {
     return local < m ? local : m;
 }
 
-__global__ void k_and_gram(const DevPoint* __restrict__ pts, const DevSolveState* __restrict__ stt,
-                           const double* __restrict__ Xh, const double* __restrict__ Gh, double* __restrict__ gram, int m) {
+// packed layout of the normal equations for NC columns: the upper triangle
+// (a <= b) row by row, then the NC right-hand sides
+template <int NC> struct GramLayout {
+    static constexpr int NTRI = NC * (NC + 1) / 2;
+    static constexpr int LEN = NTRI + NC;
+    static_assert(LEN <= SDFT_GRAM_LEN, "Gram layout exceeds its slot");
+    __host__ __device__ static constexpr int tri(int a, int b) { return a * NC - a * (a - 1) / 2 + (b - a); }
+};
+
+// residual g = F(x) - x into the history, its max norm, and the per-block
+// partial Gram sums  <dG_a, dG_b>, <dG_a, g>  with dG_a = G_{col-1-a} - g
+template <int M>
+__global__ void __launch_bounds__(256) k_and_resgram(const DevPoint* __restrict__ pts,
+                                                    const DevSolveState* __restrict__ stt, const double* __restrict__ x,
+                                                    const double* __restrict__ fx, double* __restrict__ Xh,
+                                                    double* __restrict__ Gh, unsigned long long* __restrict__ errbits,
+                                                    double* __restrict__ gram, int32_t* __restrict__ n_active) {
+    constexpr int NC = M - 1;
+    using L = GramLayout<NC>;
     const int p = blockIdx.y;
+    if (blockIdx.x == 0 && p == 0 && threadIdx.x < 3) n_active[threadIdx.x] = 0;
     const DevSolveState& S = stt[p];
     if (S.converged) return;
-    const int hl = hist_len(S, m);
-    const int nc = hl - 1;
-    if (nc < 1) return;
     const DevPoint& P = pts[p];
-    const int col = S.iter % m;
-    const int64_t hoff = P.xoff * m;
-    const double* g = Gh + hoff + (int64_t)col * P.n;
-    double acc[SDFT_MAX_M * (SDFT_MAX_M + 1) / 2 + SDFT_MAX_M];
-    const int ntri = nc * (nc + 1) / 2;
-    for (int k = 0; k < ntri + nc; ++k) acc[k] = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += (int64_t)gridDim.x * blockDim.x) {
-        double dg[SDFT_MAX_M];
-        const double gi = g[i];
-        for (int a = 0; a < nc; ++a) {
-            const int ca = ((col - 1 - a) % m + m) % m;
-            dg[a] = Gh[hoff + (int64_t)ca * P.n + i] - gi;
+    const int nb = gram_blocks(P.n);
+    if ((int)blockIdx.x >= nb) return;
+    const int col = S.iter % M;
+    const int nc = hist_len(S, M) - 1;
+    const int64_t hoff = P.xoff * M, n = P.n;
+    const int64_t cur = hoff + (int64_t)col * n;
+    int64_t coff[NC > 0 ? NC : 1];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) coff[a] = hoff + (int64_t)((col - 1 - a + M) % M) * n;
+    double acc[L::LEN];
+#pragma unroll
+    for (int k = 0; k < L::LEN; ++k) acc[k] = 0.0;
+    double mx = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)nb * blockDim.x) {
+        const double xi = x[P.xoff + i];
+        const double g = fx[P.xoff + i] - xi;
+        Xh[cur + i] = xi;
+        Gh[cur + i] = g;
+        const double ag = fabs(g);
+        mx = (ag > mx || ag != ag) ? ag : mx;  // NaN propagates
+        double dg[NC > 0 ? NC : 1];
+#pragma unroll
+        for (int a = 0; a < NC; ++a) dg[a] = a < nc ? Gh[coff[a] + i] - g : 0.0;
+#pragma unroll
+        for (int a = 0; a < NC; ++a) {
+#pragma unroll
+            for (int b = a; b < NC; ++b) acc[L::tri(a, b)] += dg[a] * dg[b];
+            acc[L::NTRI + a] += dg[a] * g;
         }
-        int k = 0;
-        for (int a = 0; a < nc; ++a)
-            for (int b = a; b < nc; ++b) acc[k++] += dg[a] * dg[b];
-        for (int a = 0; a < nc; ++a) acc[ntri + a] += dg[a] * gi;
     }
-    __shared__ double red[SDFT_MAX_M * (SDFT_MAX_M + 1) / 2 + SDFT_MAX_M][4];
-    for (int k = 0; k < ntri + nc; ++k) {
+    __shared__ double red[L::LEN + 1][4];
+    const int w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < L::LEN; ++k) {
         const double v = warp_sum(acc[k]);
-        if ((threadIdx.x & 63) == 0) red[k][threadIdx.x >> 6] = v;
+        if ((threadIdx.x & 63) == 0) red[k][w] = v;
     }
+    mx = warp_max(mx);
+    if ((threadIdx.x & 63) == 0) red[L::LEN][w] = mx;
     __syncthreads();
-    if ((int)threadIdx.x < ntri + nc) {
-        double s = 0.0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[threadIdx.x][w];
+    if ((int)threadIdx.x < L::LEN) {
+        const double s = ((red[threadIdx.x][0] + red[threadIdx.x][1]) + red[threadIdx.x][2]) + red[threadIdx.x][3];
         gram[((int64_t)p * SDFT_GRAM_BLOCKS + blockIdx.x) * SDFT_GRAM_LEN + threadIdx.x] = s;
+    } else if ((int)threadIdx.x == L::LEN) {
+        double v = nanmax(nanmax(red[L::LEN][0], red[L::LEN][1]), nanmax(red[L::LEN][2], red[L::LEN][3]));
+        if (v != v) v = INFINITY;
+        atomicMax(&errbits[p], (unsigned long long)__double_as_longlong(v));
     }
 }
 
 // minimum-norm least squares through the normal equations with a Jacobi
-// eigen-decomposition and a relative cutoff (the reference calls lsqr!)
-__device__ void small_lstsq(int nc, const double* tri, const double* rhs, double* lam) {
-    double A[SDFT_MAX_M][SDFT_MAX_M], V[SDFT_MAX_M][SDFT_MAX_M];
-    int k = 0;
-    for (int a = 0; a < nc; ++a)
-        for (int b = a; b < nc; ++b) { A[a][b] = tri[k]; A[b][a] = tri[k]; ++k; }
-    for (int a = 0; a < nc; ++a)
-        for (int b = 0; b < nc; ++b) V[a][b] = a == b ? 1.0 : 0.0;
+// eigen-decomposition and a relative eigenvalue cutoff (the reference calls
+// lsqr!). Columns a >= nc are zero-padded: their rotations are skipped and
+// their zero eigenvalues fall under the cutoff.
+template <int NC>
+__device__ void small_lstsq(int nc, const double* __restrict__ gp, double* __restrict__ lam) {
+    using L = GramLayout<NC>;
+    double A[NC][NC], V[NC][NC], rhs[NC];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) {
+        rhs[a] = a < nc ? gp[L::NTRI + a] : 0.0;
+#pragma unroll
+        for (int b = 0; b < NC; ++b) {
+            const int lo = a < b ? a : b, hi = a < b ? b : a;
+            A[a][b] = (a < nc && b < nc) ? gp[L::tri(lo, hi)] : 0.0;
+            V[a][b] = a == b ? 1.0 : 0.0;
+        }
+    }
     for (int sweep = 0; sweep < 30; ++sweep) {
         double off = 0.0;
-        for (int a = 0; a < nc; ++a)
-            for (int b = a + 1; b < nc; ++b) off += A[a][b] * A[a][b];
+#pragma unroll
+        for (int a = 0; a < NC; ++a)
+#pragma unroll
+            for (int b = a + 1; b < NC; ++b) off += A[a][b] * A[a][b];
         if (off < 1e-300) break;
-        for (int a = 0; a < nc; ++a) {
-            for (int b = a + 1; b < nc; ++b) {
+#pragma unroll
+        for (int a = 0; a < NC; ++a) {
+#pragma unroll
+            for (int b = a + 1; b < NC; ++b) {
                 if (fabs(A[a][b]) < 1e-300) continue;
                 const double th = (A[b][b] - A[a][a]) / (2.0 * A[a][b]);
                 const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
                 const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
-                for (int r = 0; r < nc; ++r) {
+#pragma unroll
+                for (int r = 0; r < NC; ++r) {
                     const double ar = A[r][a], br = A[r][b];
                     A[r][a] = c * ar - s * br;
                     A[r][b] = s * ar + c * br;
                 }
-                for (int r = 0; r < nc; ++r) {
+#pragma unroll
+                for (int r = 0; r < NC; ++r) {
                     const double ar = A[a][r], br = A[b][r];
                     A[a][r] = c * ar - s * br;
                     A[b][r] = s * ar + c * br;
                 }
-                for (int r = 0; r < nc; ++r) {
+#pragma unroll
+                for (int r = 0; r < NC; ++r) {
                     const double vr = V[r][a], wr = V[r][b];
                     V[r][a] = c * vr - s * wr;
                     V[r][b] = s * vr + c * wr;
@@ -1041,31 +1110,55 @@ __device__ void small_lstsq(int nc, const double* tri, const double* rhs, double
         }
     }
     double emax = 0.0;
-    for (int a = 0; a < nc; ++a) emax = fmax(emax, fabs(A[a][a]));
-    for (int a = 0; a < nc; ++a) lam[a] = 0.0;
-    for (int e = 0; e < nc; ++e) {
+#pragma unroll
+    for (int a = 0; a < NC; ++a) emax = fmax(emax, fabs(A[a][a]));
+    double out[NC];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) out[a] = 0.0;
+#pragma unroll
+    for (int e = 0; e < NC; ++e) {
         const double ev = A[e][e];
         if (!(ev > 1e-14 * emax)) continue;
         double proj = 0.0;
-        for (int r = 0; r < nc; ++r) proj += V[r][e] * rhs[r];
+#pragma unroll
+        for (int r = 0; r < NC; ++r) proj += V[r][e] * rhs[r];
         proj /= ev;
-        for (int r = 0; r < nc; ++r) lam[r] -= V[r][e] * proj;  // solves M lam = -rhs
+#pragma unroll
+        for (int r = 0; r < NC; ++r) out[r] -= V[r][e] * proj;  // solves M lam = -rhs
     }
+#pragma unroll
+    for (int a = 0; a < NC; ++a) lam[a] = out[a];
 }
 
-__global__ void k_and_solve(DevSolveState* __restrict__ stt, int32_t* __restrict__ evalflag,
-                            unsigned long long* __restrict__ errbits, const double* __restrict__ gram,
-                            SolveParams sp, int P_count, int32_t* __restrict__ n_active, const int32_t* __restrict__ errflag) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P_count) return;
-    DevSolveState S = stt[p];
-    if (S.converged) {
-        S.mode = -1;
-        evalflag[p] = 0;
-        stt[p] = S;
+// one wavefront per point: lanes add the block partials in a fixed order,
+// lane 0 takes the Picard/Anderson decision (AndersonMixing.jl:95-150)
+template <int M>
+__global__ void __launch_bounds__(64) k_and_solve(const DevPoint* __restrict__ pts, DevSolveState* __restrict__ stt,
+                                                  int32_t* __restrict__ evalflag, unsigned long long* __restrict__ errbits,
+                                                  const double* __restrict__ gram, SolveParams sp,
+                                                  int32_t* __restrict__ n_active, const int32_t* __restrict__ errflag) {
+    constexpr int NC = M - 1;
+    using L = GramLayout<NC>;
+    const int p = blockIdx.x;
+    const int lane = threadIdx.x;
+    __shared__ double gp[L::LEN];
+    if (stt[p].converged) {
+        if (lane == 0) {
+            stt[p].mode = -1;
+            evalflag[p] = 0;
+        }
         return;
     }
-    const int m = sp.m;
+    if (lane < L::LEN) {
+        const int nb = gram_blocks(pts[p].n);
+        const double* src = gram + (int64_t)p * SDFT_GRAM_BLOCKS * SDFT_GRAM_LEN + lane;
+        double t = 0.0;
+        for (int b = 0; b < nb; ++b) t += src[(int64_t)b * SDFT_GRAM_LEN];
+        gp[lane] = t;
+    }
+    __syncthreads();
+    if (lane != 0) return;
+    DevSolveState S = stt[p];
     double err = __longlong_as_double((long long)errbits[p]);
     errbits[p] = 0ull;
     if (errflag[p] != 0) err = INFINITY;
@@ -1084,24 +1177,15 @@ __global__ void k_and_solve(DevSolveState* __restrict__ stt, int32_t* __restrict
         stt[p] = S;
         return;
     }
-    bool picard = (local < m) || (err > sp.switch_tole && S.use_picard);
+    bool picard = (local < M) || (err > sp.switch_tole && S.use_picard);
     if (S.restarted && local >= 2 && !(err > sp.switch_tole && S.use_picard)) picard = false;
     if (picard) {
         S.mode = 0;
         S.ncols = 0;
     } else {
         S.use_picard = 0;
-        const int nc = (local < m ? local : m) - 1;
-        const int ntri = nc * (nc + 1) / 2;
-        double gp[SDFT_GRAM_LEN];
-        for (int k = 0; k < ntri + nc; ++k) {
-            double t = 0.0;
-            for (int b = 0; b < SDFT_GRAM_BLOCKS; ++b) t += gram[((int64_t)p * SDFT_GRAM_BLOCKS + b) * SDFT_GRAM_LEN + k];
-            gp[k] = t;
-        }
-        double rhs[SDFT_MAX_M];
-        for (int a = 0; a < nc; ++a) rhs[a] = gp[ntri + a];
-        small_lstsq(nc, gp, rhs, S.lam);
+        const int nc = (local < M ? local : M) - 1;
+        small_lstsq<NC>(nc, gp, S.lam);
         S.mode = 1;
         S.ncols = nc;
     }
@@ -1112,40 +1196,78 @@ __global__ void k_and_solve(DevSolveState* __restrict__ stt, int32_t* __restrict
         S.converged = 1;
         S.failed = 1;
     }
-    if (!S.converged && sp.mixed && S.prec == 32 && err < sp.fp32_switch) {
-        S.prec = 64;
-        S.switch_iter = it;
-        S.hist_base = it;
-        S.restarted = 1;
-    }
+    // the FP32 -> FP64 switch itself is applied by k_apply_switch at the host's next poll, so that
+    // the host always knows which precisions have active points
+    if (!S.converged && sp.mixed && S.prec == 32 && err < sp.fp32_switch) S.want64 = 1;
     evalflag[p] = S.converged ? 0 : S.prec;
-    if (!S.converged) atomicAdd(n_active, 1);
+    if (!S.converged) {
+        atomicAdd(&n_active[0], 1);
+        if (S.prec == 32) atomicAdd(&n_active[1], 1);
+        if (S.want64) atomicAdd(&n_active[2], 1);
+    }
     stt[p] = S;
 }
 
-__global__ void k_and_update(const DevPoint* __restrict__ pts, const DevSolveState* __restrict__ stt,
-                             double* __restrict__ x, const double* __restrict__ fx, const double* __restrict__ Xh,
-                             const double* __restrict__ Gh, int m) {
+// mixed precision: move the points that asked for it to FP64 and restart their history
+// (the FP32 history is not reused by the FP64 iterations)
+__global__ void k_apply_switch(DevSolveState* __restrict__ stt, int32_t* __restrict__ evalflag, int P_count) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P_count) return;
+    DevSolveState S = stt[p];
+    if (!S.want64) return;
+    S.want64 = 0;
+    if (!S.converged) {
+        S.prec = 64;
+        S.switch_iter = S.iter;
+        S.hist_base = S.iter;
+        S.restarted = 1;
+        evalflag[p] = 64;
+    }
+    stt[p] = S;
+}
+
+// x <- Picard mix or the Anderson extrapolation (AndersonMixing.jl:120-150)
+template <int M>
+__global__ void __launch_bounds__(256) k_and_update(const DevPoint* __restrict__ pts,
+                                                   const DevSolveState* __restrict__ stt, double* __restrict__ x,
+                                                   const double* __restrict__ fx, const double* __restrict__ Xh,
+                                                   const double* __restrict__ Gh) {
+    constexpr int NC = M - 1;
     const int p = blockIdx.y;
     const DevSolveState& S = stt[p];
-    if (S.mode < 0) return;
+    const int mode = S.mode;
+    if (mode < 0) return;
     const DevPoint& P = pts[p];
-    const int64_t hoff = P.xoff * m;
-    const int col = (S.iter - 1) % m;  // column written by this iteration
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < P.n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double xi = Xh[hoff + (int64_t)col * P.n + i];
-        const double gi = Gh[hoff + (int64_t)col * P.n + i];
+    const int nb = gram_blocks(P.n);
+    if ((int)blockIdx.x >= nb) return;
+    const int64_t hoff = P.xoff * M, n = P.n;
+    const int col = (S.iter - 1) % M;  // column written by this iteration
+    const int64_t cur = hoff + (int64_t)col * n;
+    const double amix = S.amix;
+    const int nc = mode == 1 ? S.ncols : 0;
+    int64_t coff[NC > 0 ? NC : 1];
+    double lam[NC > 0 ? NC : 1];
+#pragma unroll
+    for (int a = 0; a < NC; ++a) {
+        coff[a] = hoff + (int64_t)((col - 1 - a + M) % M) * n;
+        lam[a] = S.lam[a];
+    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)nb * blockDim.x) {
+        const double xi = Xh[cur + i];
         double nx;
-        if (S.mode == 0) {
-            nx = (1.0 - S.amix) * xi + S.amix * fx[P.xoff + i];
+        if (mode == 0) {
+            nx = (1.0 - amix) * xi + amix * fx[P.xoff + i];
         } else {
+            const double gi = Gh[cur + i];
             double dx = 0.0, dg = 0.0;
-            for (int a = 0; a < S.ncols; ++a) {
-                const int ca = ((col - 1 - a) % m + m) % m;
-                dx += (Xh[hoff + (int64_t)ca * P.n + i] - xi) * S.lam[a];
-                dg += (Gh[hoff + (int64_t)ca * P.n + i] - gi) * S.lam[a];
+#pragma unroll
+            for (int a = 0; a < NC; ++a) {
+                if (a < nc) {
+                    dx += (Xh[coff[a] + i] - xi) * lam[a];
+                    dg += (Gh[coff[a] + i] - gi) * lam[a];
+                }
             }
-            nx = xi + dx + S.amix * (gi + dg);
+            nx = xi + dx + amix * (gi + dg);
         }
         x[P.xoff + i] = nx;
     }
@@ -1156,7 +1278,7 @@ __global__ void k_and_update(const DevPoint* __restrict__ pts, const DevSolveSta
 // --------------------------------------------------------------------------
 // grand potential integrand on the primary points (grand_potential.jl:4-28):
 //   f_ex(K) - sum_j rho_j mu_ex_j trapez_j + 1/2 sum rho_bond lng
-template <typename T>
+template <typename T, int NCL>
 __global__ void __launch_bounds__(256) k_omega(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
                                               const DevSolveState* __restrict__ stt,
                                               const T* __restrict__ nw, int Fnw, const T* __restrict__ rhoT, int Frt,
@@ -1167,14 +1289,12 @@ __global__ void __launch_bounds__(256) k_omega(const DevSystem* __restrict__ psy
     if ((stt[p].prec == 64 ? 64 : 32) != (sizeof(T) == 8 ? 64 : 32)) return;
     const DevPoint& P = pts[p];
     const int NB = sys.NB, MB = sys.MB, NP = P.NP;
+    const int F = sys.functionals;
     double acc = 0.0;
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < NP; n += gridDim.x * blockDim.x) {
-        double n2j[SDFT_MAX_NB], n3j[SDFT_MAX_NB], nVj[SDFT_MAX_NB], rbond[2 * SDFT_MAX_MB];
-        load_point_inputs(sys, P, n, nw, Fnw, n2j, n3j, nVj, rbond);
-        const int F = sys.functionals;
         double f = 0.0;
         if (F & (SDFT_F_MFMT | SDFT_F_TPT1MFMT)) {
-            const FmtIn fi = fmt_collect(n2j, n3j, nVj, sys);
+            const FmtIn fi = fmt_load(sys, P, n, nw, Fnw);
             f += fmt_phi(fi);
             if (F & SDFT_F_TPT1MFMT) {
                 const double Xi = 1.0 - fi.nV2 * fi.nV2 / (fi.n2 * fi.n2);  // no max(., 0) in the free energy
@@ -1185,26 +1305,31 @@ __global__ void __launch_bounds__(256) k_omega(const DevSystem* __restrict__ psy
                     const double si = sys.sigma[sys.bond_i[u]], sj = sys.sigma[sys.bond_j[u]];
                     const double sij = (si + sj) / 2.0, tau = si * sj / sij;
                     const double y = i1 + t2 * tau + t3 * tau * tau;
-                    f -= 0.5 * (rbond[2 * u] + rbond[2 * u + 1]) * log(y);
+                    const double rb = (double)nw[fidx(P, Fnw, 3 * NB + 2 * u, n)] +
+                                      (double)nw[fidx(P, Fnw, 3 * NB + 2 * u + 1, n)];
+                    f -= 0.5 * rb * log(y);
                 }
             }
         }
         if (F & (SDFT_F_MSA | SDFT_F_TPT1MSA)) {
-            double rho[SDFT_MAX_NB];
-            for (int j = 0; j < NB; ++j) rho[j] = n2j[j] / (M_PI * sys.sigma[j] * sys.sigma[j]);
-            const GammaOut g = gamma_msa(rho, sys);
-            double fe = g.G * g.G * g.G / (3.0 * M_PI);
-            for (int j = 0; j < NB; ++j) {
-                const double s = sys.sigma[j], z = sys.z[j];
-                fe -= sys.lB * rho[j] * z * (z * g.G + g.eta * s) / (1.0 + g.G * s);
-            }
+            // free_energy_msa (msa.jl:232-250): G^3/(3 pi) - lB sum rho z (z G + eta s)/(1 + G s)
+            MsaState<NCL> m;
+            msa_load<T, NCL>(m, sys, P, n, nw, Fnw);
+            gamma_msa_cls<NCL>(m, sys);
+            double fe = m.G * m.G * m.G / (3.0 * M_PI);
+#pragma unroll
+            for (int c = 0; c < NCL; ++c)
+                fe -= sys.lB * m.ig[c] * (m.A[c] * m.G + m.eta * sys.cls_sigma[c] * m.B[c]);
             if (F & SDFT_F_TPT1MSA) {
                 for (int u = 0; u < MB; ++u) {
                     const int i = sys.bond_i[u], j = sys.bond_j[u];
-                    const double sij = (sys.sigma[i] + sys.sigma[j]) / 2.0;
-                    const double Xi = (sys.z[i] - g.eta * sys.sigma[i] * sys.sigma[i]) / (1.0 + g.G * sys.sigma[i]);
-                    const double Xj = (sys.z[j] - g.eta * sys.sigma[j] * sys.sigma[j]) / (1.0 + g.G * sys.sigma[j]);
-                    fe -= 0.5 * (rbond[2 * u] + rbond[2 * u + 1]) * (-sys.lB / sij * Xi * Xj);
+                    const double si = sys.sigma[i], sj = sys.sigma[j];
+                    const double sij = (si + sj) / 2.0;
+                    const double Xi = (sys.z[i] - m.eta * si * si) / (1.0 + m.G * si);
+                    const double Xj = (sys.z[j] - m.eta * sj * sj) / (1.0 + m.G * sj);
+                    const double rb = (double)nw[fidx(P, Fnw, 3 * NB + 2 * u, n)] +
+                                      (double)nw[fidx(P, Fnw, 3 * NB + 2 * u + 1, n)];
+                    fe -= 0.5 * rb * (-sys.lB / sij * Xi * Xj);
                 }
             }
             f += fe;
@@ -1310,6 +1435,19 @@ void launch_build_kernels(hipStream_t st, const DevSystem* sys, int K, int nsets
     hipLaunchKernelGGL(k_build_cvec, grid2(maxL, 256, K, nsets), dim3(256), 0, st, sys, set_L, set_kbase, wbuf, c64, c32);
 }
 
+// LDS of k_accumulate: one FP64 accumulator column per bead and per bond end
+static size_t accumulate_lds_bytes(const Ctx& c) { return (size_t)(c.NB + 2 * c.MB) * 256 * sizeof(double); }
+
+hipError_t prepare_kernel_attributes(const Ctx& c) {
+    const size_t lds = accumulate_lds_bytes(c);
+    if (lds <= 65536) return hipSuccess;
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_accumulate<float, 256>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_accumulate<double, 256>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
 template <typename T>
 void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double* x, double* fx) {
     const int P = c.P;
@@ -1324,6 +1462,10 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
         const bool banded = (sizeof(T) == 4) ? c.band_fp32 != 0 : c.band_fp64 != 0;
         A.band = banded ? 1 : -1;
         const bool gemm = c.use_gemm && c.prop_plan[u].ntiles > 0;
+        if (banded && c.chain_plan[u].ok) {
+            launch_chain<T>(st, c, prec, u, kern, elam, hl, trapez, gC, gP);
+            continue;
+        }
         for (int s = 1; s < sh.nseg; ++s) {
             A.s = s;
             // sweeps whose parent is the grafted segment use the direct real-space sums
@@ -1349,15 +1491,15 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
                                c.d_seg_states, c.d_maxd, elam, trapez, gC, gP, c.d_fixed_lambda);
         }
     }
-    hipLaunchKernelGGL((k_accumulate<T, 256>), grid2(maxNP, 256, P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag,
+    hipLaunchKernelGGL((k_accumulate<T, 256>), grid2(maxNP, 256, P), dim3(256), accumulate_lds_bytes(c), st, c.d_sys,
+                       c.d_pts, c.d_evalflag,
                        prec, c.d_spec, c.d_goffs, c.GF, c.d_seg_states, c.d_maxd, elam, trapez, gC, gP,
                        c.d_fixed_lambda, rho_beads, rho_bonds, rhoT, c.d_acc_part, c.d_errflag);
     if (c.use_gemm) launch_opgemm_conv<T>(st, c, prec, c.fwd_plan, kern, rhoT, dphi, nw, fx);
     else if (c.n_fwd_jobs > 0)
         hipLaunchKernelGGL((k_conv<T, 256>), grid2(maxNP, 256, c.n_fwd_jobs, P), dim3(256), 0, st, c.d_sys, c.d_pts,
                            c.d_evalflag, prec, c.d_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx);
-    hipLaunchKernelGGL(k_pointwise<T>, grid2(maxNP, 256, P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag, prec,
-                       nw, c.Fnw, rhoT, c.Frt, c.d_surf, dphi, c.Fdp);
+    launch_pointwise(st, c, prec);
     if (c.use_gemm) launch_opgemm_conv<T>(st, c, prec, c.back_plan, kern, rhoT, dphi, nw, fx);
     else
         hipLaunchKernelGGL((k_conv<T, 256>), grid2(maxNP, 256, c.n_back_jobs, P), dim3(256), 0, st, c.d_sys, c.d_pts,
@@ -1397,14 +1539,23 @@ void Pipeline<T>::launch_conv(hipStream_t st, const Ctx& c, int prec, bool back,
 
 template <typename T>
 void Pipeline<T>::launch_pointwise(hipStream_t st, const Ctx& c, int prec) {
-    hipLaunchKernelGGL(k_pointwise<T>, grid2(c.maxNP, 256, c.P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag,
-                       prec, nw, c.Fnw, rhoT, c.Frt, c.d_surf, dphi, c.Fdp);
+    const T *nw_ = nw, *rhoT_ = rhoT;
+    T* dphi_ = dphi;
+    cls_dispatch(c.ncls, [&](auto k) {
+        constexpr int NCL = decltype(k)::value;
+        hipLaunchKernelGGL((k_pointwise<T, NCL>), grid2(c.maxNP, 256, c.P), dim3(256), 0, st, c.d_sys, c.d_pts,
+                           c.d_evalflag, prec, nw_, c.Fnw, rhoT_, c.Frt, c.d_surf, dphi_, c.Fdp);
+    });
 }
 
 template <typename T>
 void Pipeline<T>::omega(hipStream_t st, const Ctx& c, const double* x, double* omega_acc) {
-    hipLaunchKernelGGL(k_omega<T>, dim3(SDFT_OMEGA_BLOCKS, c.P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_state, nw, c.Fnw, rhoT, c.Frt,
-                       rho_bonds, x, omega_acc);
+    const T *nw_ = nw, *rhoT_ = rhoT, *rb_ = rho_bonds;
+    cls_dispatch(c.ncls, [&](auto k) {
+        constexpr int NCL = decltype(k)::value;
+        hipLaunchKernelGGL((k_omega<T, NCL>), dim3(SDFT_OMEGA_BLOCKS, c.P), dim3(256), 0, st, c.d_sys, c.d_pts,
+                           c.d_state, nw_, c.Fnw, rhoT_, c.Frt, rb_, x, omega_acc);
+    });
 }
 
 template <typename T>
@@ -1417,16 +1568,32 @@ void Pipeline<T>::segment_sums(hipStream_t st, const Ctx& c, int u, double* out,
 template struct Pipeline<float>;
 template struct Pipeline<double>;
 
-void launch_anderson(hipStream_t st, const Ctx& c, const SolveParams& sp) {
+template <int M>
+static void launch_anderson_m(hipStream_t st, const Ctx& c, const SolveParams& sp) {
     const int P = c.P;
-    const int tiles = SDFT_GRAM_BLOCKS;  // fixed per point: the reductions do not depend on the batch
-    hipLaunchKernelGGL(k_and_residual, dim3(tiles, P), dim3(256), 0, st, c.d_pts, c.d_evalflag, c.d_state, c.d_x, c.d_fx,
-                       c.d_Xh, c.d_Gh, c.d_errbits, c.d_gram, sp.m, c.d_n_active);
-    hipLaunchKernelGGL(k_and_gram, dim3(tiles, P), dim3(256), 0, st, c.d_pts, c.d_state, c.d_Xh, c.d_Gh, c.d_gram, sp.m);
-    hipLaunchKernelGGL(k_and_solve, dim3((P + 63) / 64), dim3(64), 0, st, c.d_state, c.d_evalflag, c.d_errbits, c.d_gram,
-                       sp, P, c.d_n_active, c.d_errflag);
-    hipLaunchKernelGGL(k_and_update, dim3(tiles, P), dim3(256), 0, st, c.d_pts, c.d_state, c.d_x, c.d_fx, c.d_Xh, c.d_Gh,
-                       sp.m);
+    const int tiles = gram_blocks(c.max_n);  // each point uses gram_blocks(its n) of these
+    hipLaunchKernelGGL(k_and_resgram<M>, dim3(tiles, P), dim3(256), 0, st, c.d_pts, c.d_state, c.d_x, c.d_fx, c.d_Xh,
+                       c.d_Gh, c.d_errbits, c.d_gram, c.d_n_active);
+    hipLaunchKernelGGL(k_and_solve<M>, dim3(P), dim3(64), 0, st, c.d_pts, c.d_state, c.d_evalflag, c.d_errbits, c.d_gram,
+                       sp, c.d_n_active, c.d_errflag);
+    hipLaunchKernelGGL(k_and_update<M>, dim3(tiles, P), dim3(256), 0, st, c.d_pts, c.d_state, c.d_x, c.d_fx, c.d_Xh,
+                       c.d_Gh);
+}
+
+void launch_anderson(hipStream_t st, const Ctx& c, const SolveParams& sp) {
+    switch (sp.m) {
+        case 2: launch_anderson_m<2>(st, c, sp); break;
+        case 3: launch_anderson_m<3>(st, c, sp); break;
+        case 4: launch_anderson_m<4>(st, c, sp); break;
+        case 5: launch_anderson_m<5>(st, c, sp); break;
+        case 6: launch_anderson_m<6>(st, c, sp); break;
+        case 7: launch_anderson_m<7>(st, c, sp); break;
+        default: launch_anderson_m<8>(st, c, sp); break;
+    }
+}
+
+void launch_apply_switch(hipStream_t st, const Ctx& c) {
+    hipLaunchKernelGGL(k_apply_switch, dim3((c.P + 63) / 64), dim3(64), 0, st, c.d_state, c.d_evalflag, c.P);
 }
 
 void launch_set_evalflag(hipStream_t st, int32_t* evalflag, int value, int P) {

@@ -41,6 +41,15 @@ struct OpPlan {
     void* d_cols = nullptr;
 };
 
+// persistent banded chain kernel of one species (ok: eligible for the banded operator)
+struct ChainPlanHost {
+    bool ok = false;
+    bool homog = false;                            // every segment carries the same state beads
+    int S = 1;
+    int ck[2][SDFT_MAX_S * SDFT_MAX_S];            // canonical bond kernel of (sweep, a * S + b), -1 absent
+    int npad_max = 0, be_max = 0;                  // largest padded NP and even band half-width
+};
+
 struct SpeciesHost {
     int nseg, S, model, fixed_seg, multistate;
     int state_off, maxd_off;
@@ -50,6 +59,7 @@ struct SpeciesHost {
 // Device pointers and sizes shared by both precisions.
 struct Ctx {
     int P = 0, NB = 0, MB = 0, nsp = 0, K = 0;
+    int ncls = 1;                   // distinct bead diameters
     int maxNP = 0;
     int64_t sumNP = 0, sumL_sets = 0;
     int64_t total_n = 0, max_n = 0;
@@ -62,6 +72,7 @@ struct Ctx {
     bool use_gemm = true;           // MFMA operator-GEMM path (else the direct reference kernels)
     OpPlan fwd_plan, back_plan;
     std::vector<OpPlan> prop_plan;  // per species (ntiles == 0: direct kernel)
+    std::vector<ChainPlanHost> chain_plan;  // per species
     std::vector<SpeciesHost> species;
 
     DevSystem* d_sys = nullptr;
@@ -87,7 +98,7 @@ struct Ctx {
     double* d_gram = nullptr;
     unsigned long long* d_errbits = nullptr;
     DevSolveState* d_state = nullptr;
-    int32_t* d_n_active = nullptr;
+    int32_t* d_n_active = nullptr;  // [active, active at FP32, waiting for the FP64 switch]
 };
 
 template <typename T>
@@ -116,6 +127,11 @@ struct Pipeline {
 template <typename T>
 void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, const T* kern, const T* rhoT,
                         const T* dphi, T* nw, double* fx);
+size_t chain_lds_bytes(const ChainPlanHost& ch, size_t elem);
+hipError_t chain_prepare(const ChainPlanHost& ch);
+template <typename T>
+void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, const T* elam, const T* hl,
+                  const T* trapez, T* gC, T* gP);
 template <typename T>
 void launch_opgemm_prop(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, int u, int s, int skip_sweep,
                         bool banded, const T* kern, const T* elam, const T* hl, const T* trapez, T* gC, T* gP,
@@ -126,6 +142,8 @@ int opgemm_tile_cols(bool wide);
 void launch_build_kernels(hipStream_t st, const DevSystem* sys, int K, int nsets, int maxL, const int32_t* set_L,
                           const int64_t* set_kbase, const double* set_dz, double* wbuf, double* c64, float* c32);
 void launch_anderson(hipStream_t st, const Ctx& c, const SolveParams& sp);
+hipError_t prepare_kernel_attributes(const Ctx& c);
+void launch_apply_switch(hipStream_t st, const Ctx& c);
 void launch_set_evalflag(hipStream_t st, int32_t* evalflag, int value, int P);
 void launch_zero_i32(hipStream_t st, int32_t* a, int64_t n);
 void launch_zero_f64(hipStream_t st, double* a, int64_t n);

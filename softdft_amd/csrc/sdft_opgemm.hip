@@ -20,27 +20,9 @@
 #include <cstring>
 
 #include "sdft_launch.h"
+#include "sdft_mfma.h"
 
 namespace sdft {
-
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-
-template <typename T> struct Mfma;
-template <> struct Mfma<float> {
-    typedef f32x4 acc_t;
-    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
-};
-template <> struct Mfma<double> {
-    typedef f64x4 acc_t;
-    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
-};
 
 __device__ __forceinline__ int wrapLg(int d, int L) {
     d %= L;
