@@ -30,6 +30,34 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def copy_probe(mib):
+    """A plain device-to-device copy of a known size: rocprofv3's FETCH_SIZE / WRITE_SIZE for it
+    give the counter-to-bytes factor applied to the kernels' readings."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy probe: 5 copies of {mib} MiB ({mib * 1024} KiB read and written per copy)")
+
+
+def traffic_from_profiles(kernel_name):
+    """HBM bytes per launch of `kernel_name` measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and
+    scaled by the copy-probe factor, as recorded in profiles/*.md lines of the form
+    `traffic_bytes_per_launch: <kernel name> = <bytes>`; None when not measured."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    name, _, val = line[len("traffic_bytes_per_launch:"):].rpartition("=")
+                    if name.strip() == kernel_name:
+                        return float(val)
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -38,7 +66,14 @@ def main():
     ap.add_argument("--points", type=int, default=256)
     ap.add_argument("--singles", type=int, default=6, help="one-at-a-time FP64 sample size")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--kernel-only", choices=["chain32", "chain64", "fwd32", "fwd64", "back32", "back64"],
+                    help="after one mixed solve, only re-launch this kernel class 20 times (for --pmc runs)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; state points shard across GPUs with no "
                                    "communication, but only the single-GPU figure is measured"}))
@@ -55,6 +90,14 @@ def main():
     x0 = batch.x0()
     num = dict(tole=1e-8, damping=0.1, mixing_max=0.02, switch_tole=100.0, anderm=5, max_iters=3000)
 
+    if args.kernel_only:
+        batch.solve(x0, precision=abi.PREC_MIXED, **num)
+        kc = {"chain32": abi.KP_CHAIN_FP32, "chain64": abi.KP_CHAIN_FP64, "fwd32": abi.KP_CONV_FWD_FP32,
+              "fwd64": abi.KP_CONV_FWD_FP64, "back32": abi.KP_CONV_BACK_FP32, "back64": abi.KP_CONV_BACK_FP64}
+        ms, fl = batch.kernel_profile(kc[args.kernel_only], reps=20)
+        log(f"{args.kernel_only}: {ms * 1e3:.1f} us/launch, {fl:.4g} algorithmic flops/launch")
+        batch.close()
+        return
     for w in range(args.warmup):
         batch.solve(x0, precision=abi.PREC_MIXED, **num)
         log(f"warmup {w} done")
@@ -73,15 +116,15 @@ def main():
 
     # per-launch time of the hot kernel classes, HIP events on the library stream, with the
     # operators the mixed solve used; weighted by launches per evaluation x iterations run
-    n_prop = sum(max(sp.n_seg - 1, 0) for sp in sps[0].ms.species)
+    # (one chain launch runs every propagator step of the chain species)
     prof = {}
     for name, kc, prec, per_eval, iters_run in [
-            ("opgemm propagator step <float>", abi.KP_PROP_FP32, 32, n_prop, it32),
-            ("opgemm propagator step <double>", abi.KP_PROP_FP64, 64, n_prop, it64),
-            ("opgemm forward convolutions <float>", abi.KP_CONV_FWD_FP32, 32, 1, it32),
-            ("opgemm forward convolutions <double>", abi.KP_CONV_FWD_FP64, 64, 1, it64),
-            ("opgemm back-projection convolutions <float>", abi.KP_CONV_BACK_FP32, 32, 1, it32),
-            ("opgemm back-projection convolutions <double>", abi.KP_CONV_BACK_FP64, 64, 1, it64)]:
+            ("k_chain<float> (banded propagators, all steps)", abi.KP_CHAIN_FP32, 32, 1, it32),
+            ("k_chain<double> (banded propagators, all steps)", abi.KP_CHAIN_FP64, 64, 1, it64),
+            ("k_convgemm<float> (forward weighted densities)", abi.KP_CONV_FWD_FP32, 32, 1, it32),
+            ("k_convgemm<double> (forward weighted densities)", abi.KP_CONV_FWD_FP64, 64, 1, it64),
+            ("k_convgemm<float> (back projections)", abi.KP_CONV_BACK_FP32, 32, 1, it32),
+            ("k_convgemm<double> (back projections)", abi.KP_CONV_BACK_FP64, 64, 1, it64)]:
         ms, fl = batch.kernel_profile(kc, reps=20)
         prof[name] = (ms, fl, prec, ms * per_eval * iters_run)
         log(f"kernel {name}: {ms * 1e3:.1f} us/launch, {fl / ms / 1e9:.2f} TFLOP/s, "
@@ -142,7 +185,7 @@ def main():
         "max_rel_dOmega_vs_fp64": float(rel.max()),
         "mean_iterations": float(np.mean([i.mean() for i in iters])),
         "roofline": {"bound": "mfma", "kernel": name, "achieved": achieved, "peak": PEAK[prec],
-                     "unit": "TFLOP/s", "frac": achieved / PEAK[prec], "traffic": None},
+                     "unit": "TFLOP/s", "frac": achieved / PEAK[prec], "traffic": traffic_from_profiles(name)},
         "cpu_baseline": cpu,
         "kernels_us": {k: v[0] * 1e3 for k, v in prof.items()},
         "kernels_ms_per_solve_est": {k: v[3] for k, v in prof.items()},

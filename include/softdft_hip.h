@@ -179,6 +179,8 @@ int32_t sdft_last_pipeline_iters(const sdft_ctx_t* ctx, int32_t* fp32_iters, int
 #define SDFT_KP_CONV_BACK_FP32 4 /* back-projection convolutions, FP32 */
 #define SDFT_KP_CONV_BACK_FP64 5
 #define SDFT_KP_POINTWISE_FP64 6 /* FMT/MSA derivatives (FP64 arithmetic) on FP64 fields */
+#define SDFT_KP_CHAIN_FP32 7     /* persistent banded chain propagation (all steps, both sweeps), FP32 */
+#define SDFT_KP_CHAIN_FP64 8     /* the same in FP64 */
 
 /* Re-launch one kernel class `reps` times for every point on the context's
  * stream, bracketed by HIP events, after a solve or objective has filled the

@@ -30,7 +30,7 @@ EXPORTED = ("sdft_abi_version", "sdft_last_error", "sdft_create", "sdft_destroy"
             "sdft_solution_offsets", "sdft_objective", "sdft_solve", "sdft_observables", "sdft_get_field",
             "sdft_last_timing", "sdft_last_pipeline_iters", "sdft_set_operator_band", "sdft_kernel_profile")
 KP_PROP_FP32, KP_PROP_FP64, KP_CONV_FWD_FP32, KP_CONV_FWD_FP64, KP_CONV_BACK_FP32, KP_CONV_BACK_FP64, \
-    KP_POINTWISE_FP64 = range(7)
+    KP_POINTWISE_FP64, KP_CHAIN_FP32, KP_CHAIN_FP64 = range(9)
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)

@@ -172,6 +172,31 @@ static int upload_vec(sdft_ctx* ctx, void** dst, const std::vector<V>& v) {
     return SDFT_OK;
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin by ID, each XCD with its own L2. Reorder the
+// tiles (built column block by column block, rows fastest) so that all row tiles of one column
+// block land on the same XCD and share its L2 copy of the block's inputs: column block k goes to
+// XCD k % 8 and the lists are interleaved, padded with empty tiles (ncols 0 exits at once).
+static std::vector<OpTile> xcd_order(const std::vector<OpTile>& tiles) {
+    constexpr int NX = 8;
+    std::vector<std::vector<OpTile>> by(NX);
+    int blk = -1;
+    OpTile prev{-1, -1, -1, 0};
+    for (const OpTile& t : tiles) {
+        if (t.group != prev.group || t.col0 != prev.col0) ++blk;
+        by[blk % NX].push_back(t);
+        prev = t;
+    }
+    size_t mx = 0;
+    for (auto& v : by) mx = std::max(mx, v.size());
+    std::vector<OpTile> out;
+    out.reserve(mx * NX);
+    for (size_t k = 0; k < mx; ++k)
+        for (int x = 0; x < NX; ++x) out.push_back(k < by[x].size() ? by[x][k] : OpTile{0, 0, 0, 0});
+    // drop trailing empty tiles
+    while (!out.empty() && out.back().ncols == 0) out.pop_back();
+    return out;
+}
+
 static int build_conv_plan(sdft_ctx* ctx, const std::vector<ConvJob>& jobs, int job_base, int njobs,
                            const std::vector<std::vector<int>>& canon, const std::vector<int>& set_points,
                            const std::vector<int32_t>& set_L, const std::vector<int64_t>& set_kbase, OpPlan& plan) {
@@ -229,9 +254,10 @@ static int build_conv_plan(sdft_ctx* ctx, const std::vector<ConvJob>& jobs, int 
         const int gi = (int)gv.size();
         gv.push_back(g);
         cols.insert(cols.end(), kv.second.begin(), kv.second.end());
-        for (int r = 0; r < g.NP; r += TM)
-            for (int c0 = 0; c0 < g.ncols; c0 += TN) tiles.push_back(OpTile{gi, r, c0, std::min(TN, g.ncols - c0)});
+        for (int c0 = 0; c0 < g.ncols; c0 += TN)
+            for (int r = 0; r < g.NP; r += TM) tiles.push_back(OpTile{gi, r, c0, std::min(TN, g.ncols - c0)});
     }
+    tiles = xcd_order(tiles);
     plan.ntiles = (int)tiles.size();
     if (plan.ntiles == 0) return SDFT_OK;
     TRY(upload_vec(ctx, &plan.d_tiles, tiles));
@@ -300,13 +326,14 @@ static void build_chain_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const s
     ch.S = S;
     for (int p = 0; p < c.P; ++p) {
         const int NP = ctx->pts[p].NP;
-        const int Be = (band[ctx->pts[p].kset] + 7) & ~7;  // the chain kernel's rounded half-width
+        const int Be = (band[ctx->pts[p].kset] + 7) / 16 * 16 + 8;  // the chain kernel's half-width (8 mod 16)
         if (2 * Be + 1 > NP) return;
         ch.npad_max = std::max(ch.npad_max, (NP + 255) / 256 * 256);
         ch.be_max = std::max(ch.be_max, Be);
     }
-    if (chain_lds_bytes(ch, sizeof(double)) > 160 * 1024) return;
-    ch.ok = true;
+    ch.ok32 = chain_lds_bytes(ch, sizeof(float)) <= 160 * 1024;
+    ch.ok64 = chain_lds_bytes(ch, sizeof(double)) <= 160 * 1024;
+    ch.ok = ch.ok32 || ch.ok64;
 }
 
 // Propagator plan of species u: valid only if every (sweep, a, b) column applies the same
@@ -359,9 +386,10 @@ static int build_prop_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const std
         const int gi = (int)gv.size();
         gv.push_back(g);
         cols.insert(cols.end(), kv.second.begin(), kv.second.end());
-        for (int r = 0; r < g.NP; r += TM)
-            for (int c0 = 0; c0 < g.ncols; c0 += TN) tiles.push_back(OpTile{gi, r, c0, std::min(TN, g.ncols - c0)});
+        for (int c0 = 0; c0 < g.ncols; c0 += TN)
+            for (int r = 0; r < g.NP; r += TM) tiles.push_back(OpTile{gi, r, c0, std::min(TN, g.ncols - c0)});
     }
+    tiles = xcd_order(tiles);
     plan.ntiles = (int)tiles.size();
     if (plan.ntiles == 0) return SDFT_OK;
     TRY(upload_vec(ctx, &plan.d_tiles, tiles));
@@ -1040,13 +1068,23 @@ extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps
         return SDFT_E_STATE;
     }
     Ctx& c = ctx->c;
-    const int prec = (kc == SDFT_KP_PROP_FP32 || kc == SDFT_KP_CONV_FWD_FP32 || kc == SDFT_KP_CONV_BACK_FP32) ? 32 : 64;
+    if (kc < SDFT_KP_PROP_FP32 || kc > SDFT_KP_CHAIN_FP64) {
+        set_error(ctx, SDFT_E_ARG, "unknown kernel class %d", kc);
+        return SDFT_E_ARG;
+    }
+    const bool chain = kc == SDFT_KP_CHAIN_FP32 || kc == SDFT_KP_CHAIN_FP64;
+    const int prec = (kc == SDFT_KP_PROP_FP32 || kc == SDFT_KP_CONV_FWD_FP32 || kc == SDFT_KP_CONV_BACK_FP32 ||
+                      kc == SDFT_KP_CHAIN_FP32) ? 32 : 64;
     // pick the longest chain species for the propagator classes
     int u = 0;
     for (int k = 0; k < c.nsp; ++k)
         if (c.species[k].nseg > c.species[u].nseg) u = k;
-    if ((kc == SDFT_KP_PROP_FP32 || kc == SDFT_KP_PROP_FP64) && c.species[u].nseg < 2) {
+    if ((kc == SDFT_KP_PROP_FP32 || kc == SDFT_KP_PROP_FP64 || chain) && c.species[u].nseg < 2) {
         set_error(ctx, SDFT_E_ARG, "no chain species to profile");
+        return SDFT_E_ARG;
+    }
+    if (chain && !(prec == 32 ? c.chain_plan[u].ok32 : c.chain_plan[u].ok64)) {
+        set_error(ctx, SDFT_E_ARG, "species %d does not run on the banded chain kernel", u);
         return SDFT_E_ARG;
     }
     // scratch output so the solution buffers are untouched
@@ -1064,6 +1102,14 @@ extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps
             case SDFT_KP_CONV_FWD_FP64: ctx->p64.launch_conv(ctx->stream, c, 64, false, d_fx); break;
             case SDFT_KP_CONV_BACK_FP32: ctx->p32.launch_conv(ctx->stream, c, 32, true, d_fx); break;
             case SDFT_KP_CONV_BACK_FP64: ctx->p64.launch_conv(ctx->stream, c, 64, true, d_fx); break;
+            case SDFT_KP_CHAIN_FP32:
+                launch_chain<float>(ctx->stream, c, 32, u, ctx->p32.kern, ctx->p32.elam, ctx->p32.hl, ctx->p32.trapez,
+                                    ctx->p32.gC, ctx->p32.gP);
+                break;
+            case SDFT_KP_CHAIN_FP64:
+                launch_chain<double>(ctx->stream, c, 64, u, ctx->p64.kern, ctx->p64.elam, ctx->p64.hl,
+                                     ctx->p64.trapez, ctx->p64.gC, ctx->p64.gP);
+                break;
             default: ctx->p64.launch_pointwise(ctx->stream, c, 64); break;
         }
     };
@@ -1098,9 +1144,19 @@ extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps
             npair = cnt(N - 1 - s_mid) * cnt(N - s_mid);  // child sweep; parent sweep similar
             (void)N;
         }
+        // chain kernel: valid (a, b) columns summed over all steps of both sweeps, 2 Be + 1 taps
+        double chain_cols = 0.0;
+        if (chain) {
+            const int N = c.species[u].nseg, S = c.species[u].S, so = c.species[u].state_off;
+            auto cnt = [&](int seg) { int k = 0; for (int a = 0; a < S; ++a) if (ctx->seg_states_h[so + seg * S + a] >= 0) ++k; return k; };
+            for (int s = 1; s < N; ++s) chain_cols += cnt(N - 1 - s) * cnt(N - s) + cnt(s) * cnt(s - 1);
+        }
         for (int p = 0; p < c.P; ++p) {
             const double NP = ctx->pts[p].NP;
-            if (kc == SDFT_KP_PROP_FP32 || kc == SDFT_KP_PROP_FP64) {
+            if (chain) {
+                const int Be = (band[ctx->pts[p].kset] + 7) / 16 * 16 + 8;
+                f += chain_cols * 2.0 * NP * (2.0 * Be + 1.0);
+            } else if (kc == SDFT_KP_PROP_FP32 || kc == SDFT_KP_PROP_FP64) {
                 const bool bd = kc == SDFT_KP_PROP_FP32 ? c.band_fp32 != 0 : c.band_fp64 != 0;
                 const double taps = bd ? std::min(NP, 2.0 * band[ctx->pts[p].kset] + 1.0) : NP;
                 f += 2.0 /*sweeps*/ * npair * 2.0 * NP * taps;

@@ -49,6 +49,11 @@ struct ChainCol {      // one (a, b) column at one step
 
 __device__ __forceinline__ int chain_phys(int i) { return i + (i >> 4); }  // 1 pad word per 16: conflict-free B reads
 
+// applied half-width: the smallest Be >= B with Be = 8 (mod 16), so that K / 16 = Be / 8 + 1 is even
+__host__ __device__ __forceinline__ int chain_half_width(int B) { return (B + 7) / 16 * 16 + 8; }
+
+constexpr int CHAIN_LDS_SLACK = 64;  // elements after the carve-out: the operand prefetch may read one group past the end
+
 template <typename T, int EPC>
 __global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __restrict__ psys,
                                                          const DevPoint* __restrict__ pts,
@@ -64,8 +69,8 @@ __global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __rest
     const DevPoint& P = pts[p];
     const int NP = P.NP, L = P.L;
     const int B = band[P.kset];
-    const int Be = (B + 7) & ~7;               // half-width rounded up to a multiple of 8
-    const int KS = Be / 2 + 4;                 // K / 4 with K = 2 Be + 16, a multiple of 4
+    const int Be = chain_half_width(B);
+    const int KS = Be / 2 + 4;                 // K / 4 with K = 2 Be + 16: an even number of 4-step groups
     const int ntile = (NP + 255) >> 8;
     const int XL = ntile * 256 + 2 * Be;       // extended input length of this point
     const int S = ca.S, C = S * S, N = ca.nseg;
@@ -84,13 +89,14 @@ __global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __rest
     const int* st = seg_states + ca.state_off;
     const double norm = P.normative[ca.u];
 
-    // operator windows: cw[col][ci] = c[ci - 15 - Be] for |ci - 15 - Be| <= Be, else 0
+    // operator windows, stored reversed so that the A reads ascend with k:
+    // cw[col][ri] = c[j], j = (KC - 1 - ri) - 15 - Be, for |j| <= Be, else 0
     {
         const T* cset = kern + (int64_t)P.kbase * sys.K;
         for (int e = tid; e < C * KC; e += CHAIN_THREADS) {
-            const int col = e / KC, ci = e - col * KC;
+            const int col = e / KC, ri = e - col * KC;
             const int k = ca.ck[sw][col];
-            const int j = ci - 15 - Be;
+            const int j = (KC - 1 - ri) - 15 - Be;
             T v = T(0);
             if (k >= 0 && j >= -Be && j <= Be) {
                 int jj = j % L;
@@ -179,29 +185,60 @@ __global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __rest
         for (int unit = wave; unit < S * ntile; unit += CHAIN_WAVES) {
             const int a = unit / ntile, tile = unit - a * ntile;
             const int p0 = tile * 16;
+            // epilogue factors exp(lng/2) of the output end, fetched before the products
+            T hv[SDFT_MAX_S][4];
+#pragma unroll
+            for (int b = 0; b < SDFT_MAX_S; ++b) {
+                const ChainCol cc = ci[(a * S + b) % C];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int n = 16 * (p0 + q) + Mfma<T>::row(lane, v);
+                    hv[b][v] = (b < S && cc.valid && n < NP) ? hl[hb + (int64_t)cc.hl_out * NP + n] : T(0);
+                }
+            }
             T qs[4] = {T(0), T(0), T(0), T(0)};
-            for (int b = 0; b < S; ++b) {
+#pragma unroll
+            for (int b = 0; b < SDFT_MAX_S; ++b) {
+                if (b >= S) break;
                 const int col = a * S + b;
                 const ChainCol cc = ci[col];
                 if (!cc.valid) continue;
-                const T* cwc = cw + col * KC + 15 + q - g + 2 * Be;
-                const T* xc = X + col * XLP;
-                const int xb = 16 * (p0 + q) + g;
+                // k-step ks = 4 K + k4 (t = 4 ks + g): A[q][t] = c[q - t + Be] = cw[col][KC - 16 - 2 Be - q + g + 4 ks],
+                // B = X[col][phys(16 (p0 + q) + g + 4 ks)] = X[col][17 (p0 + q) + g + 17 K + 4 k4]
+                const T* ap = cw + col * KC + (KC - 16 - 2 * Be) - q + g;
+                const T* xp = X + col * XLP + 17 * (p0 + q) + g;
                 typename Mfma<T>::acc_t acc;
                 for (int v = 0; v < 4; ++v) acc[v] = T(0);
-                for (int ks = 0; ks < KS; ks += 4) {  // KS is a multiple of 4
+                // ping-pong operand buffers: the next group's LDS reads are in flight while
+                // the current group runs on the matrix core
+                T a0[4], b0[4], a1[4], b1[4];
+                auto fetch = [&](int K, T (&av)[4], T (&bv)[4]) {
 #pragma unroll
                     for (int k4 = 0; k4 < 4; ++k4) {
-                        const int mi = xb + 4 * (ks + k4);
-                        acc = Mfma<T>::mma(cwc[-4 * (ks + k4)], xc[chain_phys(mi)], acc);
+                        av[k4] = ap[16 * K + 4 * k4];
+                        bv[k4] = xp[17 * K + 4 * k4];
                     }
+                };
+                const int KG = KS / 4;  // even
+                fetch(0, a0, b0);
+                for (int K = 0; K < KG; K += 2) {
+                    fetch(K + 1, a1, b1);
+                    __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the products
+#pragma unroll
+                    for (int k4 = 0; k4 < 4; ++k4) acc = Mfma<T>::mma(a0[k4], b0[k4], acc);
+                    __builtin_amdgcn_sched_barrier(0);
+                    fetch(K + 2, a0, b0);  // past the end on the last pass: read from the slack, unused
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int k4 = 0; k4 < 4; ++k4) acc = Mfma<T>::mma(a1[k4], b1[k4], acc);
+                    __builtin_amdgcn_sched_barrier(0);
                 }
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int n = 16 * (p0 + q) + Mfma<T>::row(lane, v);
                     if (n < NP) {
                         const T y = acc[v];
-                        const T val = (y < T(0) ? -y : y) * hl[hb + (int64_t)cc.hl_out * NP + n] / (T)norm;
+                        const T val = (y < T(0) ? -y : y) * hv[b][v] / (T)norm;
                         gout[gb + (((int64_t)outseg * S + b) * S + a) * NP + n] = val;
                         qs[v] += val;
                     }
@@ -222,7 +259,7 @@ size_t chain_lds_bytes(const ChainPlanHost& ch, size_t elem) {
     const int KC = 2 * ch.be_max + 32;
     const int XLm = ch.npad_max + 2 * ch.be_max;
     const int XLP = XLm + (XLm >> 4) + 1;
-    return (size_t)(C * KC + C * XLP + S * ch.npad_max) * elem;
+    return (size_t)(C * KC + C * XLP + S * ch.npad_max + CHAIN_LDS_SLACK) * elem;
 }
 
 template <typename T, int EPC>
@@ -233,16 +270,22 @@ static hipError_t chain_set_lds(size_t bytes) {
 }
 
 hipError_t chain_prepare(const ChainPlanHost& ch) {
-    const size_t b4 = chain_lds_bytes(ch, 4), b8 = chain_lds_bytes(ch, 8);
-    hipError_t e;
-    if ((e = chain_set_lds<float, 0>(b4)) != hipSuccess) return e;
-    if ((e = chain_set_lds<float, 2>(b4)) != hipSuccess) return e;
-    if ((e = chain_set_lds<float, 3>(b4)) != hipSuccess) return e;
-    if ((e = chain_set_lds<float, 4>(b4)) != hipSuccess) return e;
-    if ((e = chain_set_lds<double, 0>(b8)) != hipSuccess) return e;
-    if ((e = chain_set_lds<double, 2>(b8)) != hipSuccess) return e;
-    if ((e = chain_set_lds<double, 3>(b8)) != hipSuccess) return e;
-    return chain_set_lds<double, 4>(b8);
+    hipError_t e = hipSuccess;
+    if (ch.ok32) {
+        const size_t b = chain_lds_bytes(ch, sizeof(float));
+        if ((e = chain_set_lds<float, 0>(b)) != hipSuccess) return e;
+        if ((e = chain_set_lds<float, 2>(b)) != hipSuccess) return e;
+        if ((e = chain_set_lds<float, 3>(b)) != hipSuccess) return e;
+        if ((e = chain_set_lds<float, 4>(b)) != hipSuccess) return e;
+    }
+    if (ch.ok64) {
+        const size_t b = chain_lds_bytes(ch, sizeof(double));
+        if ((e = chain_set_lds<double, 0>(b)) != hipSuccess) return e;
+        if ((e = chain_set_lds<double, 2>(b)) != hipSuccess) return e;
+        if ((e = chain_set_lds<double, 3>(b)) != hipSuccess) return e;
+        e = chain_set_lds<double, 4>(b);
+    }
+    return e;
 }
 
 template <typename T>

@@ -456,23 +456,42 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
         };
         double num = 0.0, chg = 0.0;
         for (int q = 0; q < N; ++q) {
-            for (int a = 0; a < S; ++a) {
-                const int si = st[q * S + a];
+            // every propagator value of this segment is loaded up front (independent loads in flight),
+            // gcv[b][a] = gC[q][b][a] (child side, b: state of q+1), gpv[b][a] = gP[q][b][a] (parent side)
+            T gcv[SDFT_MAX_S][SDFT_MAX_S], gpv[SDFT_MAX_S][SDFT_MAX_S], elv[SDFT_MAX_S], tzv[SDFT_MAX_S];
+#pragma unroll
+            for (int a = 0; a < SDFT_MAX_S; ++a) {
+                const int si = a < S ? st[q * S + a] : -1;
+                elv[a] = si >= 0 ? elam[fidx(P, NB, si, nc)] : T(0);
+                tzv[a] = si >= 0 ? trapez[fidx(P, NB, si, nc)] : T(0);
+#pragma unroll
+                for (int b = 0; b < SDFT_MAX_S; ++b) {
+                    const bool okc = si >= 0 && b < S && q < N - 1 && st[(q + 1) * S + b] >= 0;
+                    const bool okp = si >= 0 && b < S && q > 0 && st[(q - 1) * S + b] >= 0;
+                    gcv[b][a] = okc ? gC[gidx(q, b, a, nc)] : T(0);
+                    gpv[b][a] = okp ? gP[gidx(q, b, a, nc)] : T(0);
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < SDFT_MAX_S; ++a) {
+                const int si = a < S ? st[q * S + a] : -1;
                 if (si < 0) continue;
                 double Gp = 0.0, Qc = 0.0;
                 if (q == 0) Gp = (double)base_value<T>(0, nc, sp.fixed_seg, Kf, P.dz, maxd);
                 else
-                    for (int b = 0; b < S; ++b)
-                        if (st[(q - 1) * S + b] >= 0) Gp += (double)gP[gidx(q, b, a, nc)];
+#pragma unroll
+                    for (int b = 0; b < SDFT_MAX_S; ++b)
+                        if (b < S && st[(q - 1) * S + b] >= 0) Gp += (double)gpv[b][a];
                 // leaf: sum gC[K, :, 1, idx_i, leaf] is the base case 1 for every state
                 if (q == N - 1) Qc = (double)base_value<T>(q, nc, sp.fixed_seg, Kf, P.dz, maxd);
                 else
-                    for (int b = 0; b < S; ++b)
-                        if (st[(q + 1) * S + b] >= 0) Qc += (double)gC[gidx(q, b, a, nc)];
-                const double el = (double)elam[fidx(P, NB, si, nc)];
+#pragma unroll
+                    for (int b = 0; b < SDFT_MAX_S; ++b)
+                        if (b < S && st[(q + 1) * S + b] >= 0) Qc += (double)gcv[b][a];
+                const double el = (double)elv[a];
                 const double contr = pref * el * Gp;
                 const double rs = contr * Qc;
-                const double tz = (double)trapez[fidx(P, NB, si, nc)];
+                const double tz = (double)tzv[a];
                 if (valid) {
                     if (q == sp.fixed_seg) {
                         if (nc == Kf) {
@@ -486,26 +505,28 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
                         chg += rs * tz * P.dz * sys.z[si];
                     }
                     if (q < N - 1) {
-                        for (int b = 0; b < S; ++b) {
-                            const int sj = st[(q + 1) * S + b];
+#pragma unroll
+                        for (int b = 0; b < SDFT_MAX_S; ++b) {
+                            const int sj = b < S ? st[(q + 1) * S + b] : -1;
                             if (sj < 0) continue;
                             const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
                             int bond = 0;
                             for (int u2 = 0; u2 < MB; ++u2)
                                 if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
-                            rbo[(2 * bond + 0) * TB] += contr * (double)gC[gidx(q, b, a, nc)];
+                            rbo[(2 * bond + 0) * TB] += contr * (double)gcv[b][a];
                         }
                     }
                     if (q > 0) {
                         const double c1 = pref * el * Qc;
-                        for (int b = 0; b < S; ++b) {
-                            const int sj = st[(q - 1) * S + b];
+#pragma unroll
+                        for (int b = 0; b < SDFT_MAX_S; ++b) {
+                            const int sj = b < S ? st[(q - 1) * S + b] : -1;
                             if (sj < 0) continue;
                             const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
                             int bond = 0;
                             for (int u2 = 0; u2 < MB; ++u2)
                                 if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
-                            rbo[(2 * bond + 1) * TB] += c1 * (double)gP[gidx(q, b, a, nc)];
+                            rbo[(2 * bond + 1) * TB] += c1 * (double)gpv[b][a];
                         }
                     }
                 }
@@ -910,44 +931,45 @@ __global__ void __launch_bounds__(256) k_pointwise(const DevSystem* __restrict__
 // --------------------------------------------------------------------------
 // neutrality shift (compute_neutrality_shift, neutrality_shift.jl:1-99): one thread per point
 // --------------------------------------------------------------------------
-__global__ void k_neutrality(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
-                             const DevSpecies* __restrict__ spec, const int32_t* __restrict__ evalflag, int prec,
-                             const double* __restrict__ acc_part, double* __restrict__ acc_species, const double* __restrict__ x,
-                             double* __restrict__ fx, int P_count) {
+__global__ void __launch_bounds__(64) k_neutrality(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
+                                                   const DevSpecies* __restrict__ spec,
+                                                   const int32_t* __restrict__ evalflag, int prec,
+                                                   const double* __restrict__ acc_part,
+                                                   double* __restrict__ acc_species, const double* __restrict__ x,
+                                                   double* __restrict__ fx) {
+    // one wavefront per point: lane k < 2 n_species adds the per-tile partials of quantity k in
+    // tile order; lane u < n_species then carries species u through the Newton iteration, whose
+    // sums over species are butterfly reductions (identical in every lane)
     const DevSystem& sys = *psys;
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P_count || evalflag[p] != prec) return;
+    const int p = blockIdx.x, lane = threadIdx.x;
+    if (evalflag[p] != prec) return;
     const DevPoint& P = pts[p];
     const int nsp = sys.n_species;
-    double Num[SDFT_MAX_SPECIES], qs[SDFT_MAX_SPECIES];
-    double Qtot = P.total_charge;
     const int ntiles = (P.NP + 255) / 256;
-    for (int k = 0; k < 2 * nsp; ++k) {
-        double t = 0.0;
-        for (int b = 0; b < ntiles; ++b) t += acc_part[((int64_t)p * SDFT_MAX_TILES + b) * 2 * SDFT_MAX_SPECIES + k];
-        acc_species[(int64_t)p * 2 * SDFT_MAX_SPECIES + k] = t;
+    double t = 0.0;
+    if (lane < 2 * nsp) {
+        for (int b = 0; b < ntiles; ++b) t += acc_part[((int64_t)p * SDFT_MAX_TILES + b) * 2 * SDFT_MAX_SPECIES + lane];
+        acc_species[(int64_t)p * 2 * SDFT_MAX_SPECIES + lane] = t;
     }
-    for (int u = 0; u < nsp; ++u) {
-        Num[u] = acc_species[(int64_t)p * 2 * SDFT_MAX_SPECIES + 2 * u];
-        const double ch = acc_species[(int64_t)p * 2 * SDFT_MAX_SPECIES + 2 * u + 1];
-        qs[u] = Num[u] <= 0.0 ? 0.0 : ch / Num[u];
-        if (spec[u].multistate || spec[u].fixed_seg >= 0) {
-            Qtot += Num[u] * qs[u];
-            qs[u] = 0.0;
+    const double num_u = __shfl(t, (2 * lane) & 63, 64);
+    const double ch_u = __shfl(t, (2 * lane + 1) & 63, 64);
+    double Num = 0.0, q = 0.0, fold = 0.0;
+    if (lane < nsp) {
+        Num = num_u;
+        q = Num <= 0.0 ? 0.0 : ch_u / Num;
+        if (spec[lane].multistate || spec[lane].fixed_seg >= 0) {  // folded into the fixed charge
+            fold = Num * q;
+            q = 0.0;
         }
     }
-    double s = 0.0;
-    for (int u = 0; u < nsp; ++u) s += Num[u] * qs[u];
-    double err = fabs(Qtot + s);
+    const double Qtot = P.total_charge + warp_sum(fold);
+    double err = fabs(Qtot + warp_sum(Num * q));
     double psi1 = 0.0, cst = 1.0;
     int it = 1;
     while (err > 1e-10) {
-        double f = Qtot, f1 = 0.0;
-        for (int i = 0; i < nsp; ++i) {
-            const double t = pow(cst, qs[i]) * Num[i] * qs[i];
-            f += t;
-            f1 -= t * qs[i];
-        }
+        const double tq = lane < nsp ? pow(cst, q) * Num * q : 0.0;
+        const double f = Qtot + warp_sum(tq);
+        const double f1 = -warp_sum(tq * q);
         err = fabs(f);
         if (fabs(f1) < 1e-10) break;
         const double del = fmax(1.0, fabs(f / f1));
@@ -956,7 +978,7 @@ __global__ void k_neutrality(const DevSystem* __restrict__ psys, const DevPoint*
         ++it;
         if (it > 1000) break;
     }
-    fx[P.xoff + P.n - 1] = x[P.xoff + P.n - 1] + psi1;
+    if (lane == 0) fx[P.xoff + P.n - 1] = x[P.xoff + P.n - 1] + psi1;
 }
 
 // --------------------------------------------------------------------------
@@ -1073,13 +1095,19 @@ __device__ void small_lstsq(int nc, const double* __restrict__ gp, double* __res
             V[a][b] = a == b ? 1.0 : 0.0;
         }
     }
+    // cyclic Jacobi until the off-diagonal mass is at round-off level of the matrix
+    double fro = 0.0;
+#pragma unroll
+    for (int a = 0; a < NC; ++a)
+#pragma unroll
+        for (int b = 0; b < NC; ++b) fro += A[a][b] * A[a][b];
     for (int sweep = 0; sweep < 30; ++sweep) {
         double off = 0.0;
 #pragma unroll
         for (int a = 0; a < NC; ++a)
 #pragma unroll
             for (int b = a + 1; b < NC; ++b) off += A[a][b] * A[a][b];
-        if (off < 1e-300) break;
+        if (!(off > 1e-32 * fro)) break;
 #pragma unroll
         for (int a = 0; a < NC; ++a) {
 #pragma unroll
@@ -1462,7 +1490,7 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
         const bool banded = (sizeof(T) == 4) ? c.band_fp32 != 0 : c.band_fp64 != 0;
         A.band = banded ? 1 : -1;
         const bool gemm = c.use_gemm && c.prop_plan[u].ntiles > 0;
-        if (banded && c.chain_plan[u].ok) {
+        if (banded && (sizeof(T) == 4 ? c.chain_plan[u].ok32 : c.chain_plan[u].ok64)) {
             launch_chain<T>(st, c, prec, u, kern, elam, hl, trapez, gC, gP);
             continue;
         }
@@ -1504,8 +1532,8 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
     else
         hipLaunchKernelGGL((k_conv<T, 256>), grid2(maxNP, 256, c.n_back_jobs, P), dim3(256), 0, st, c.d_sys, c.d_pts,
                            c.d_evalflag, prec, c.d_jobs + c.n_fwd_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx);
-    hipLaunchKernelGGL(k_neutrality, dim3((P + 63) / 64), dim3(64), 0, st, c.d_sys, c.d_pts, c.d_spec, c.d_evalflag,
-                       prec, c.d_acc_part, c.d_acc_species, x, fx, P);
+    hipLaunchKernelGGL(k_neutrality, dim3(P), dim3(64), 0, st, c.d_sys, c.d_pts, c.d_spec, c.d_evalflag, prec,
+                       c.d_acc_part, c.d_acc_species, x, fx);
 }
 
 template <typename T>

@@ -43,7 +43,8 @@ struct OpPlan {
 
 // persistent banded chain kernel of one species (ok: eligible for the banded operator)
 struct ChainPlanHost {
-    bool ok = false;
+    bool ok = false;                               // eligible in at least one precision
+    bool ok32 = false, ok64 = false;               // the working set fits LDS at that precision
     bool homog = false;                            // every segment carries the same state beads
     int S = 1;
     int ck[2][SDFT_MAX_S * SDFT_MAX_S];            // canonical bond kernel of (sweep, a * S + b), -1 absent

@@ -300,6 +300,158 @@ __global__ void __launch_bounds__(256) k_opgemm(const DevSystem* __restrict__ ps
 }
 
 // --------------------------------------------------------------------------
+// Dense convolution GEMM (forward weighted densities, back projections).
+// Tile 64 rows x 16*WN columns, 4 waves stacked along rows, K chunks of 64 with the
+// next chunk (inputs and operator windows) prefetched into registers while the
+// current one runs on the matrix cores; sums over a job's terms chain the chunks.
+// xs rows are padded to TN + 16 so that the B reads of the four lane groups fall
+// in distinct LDS banks.
+// --------------------------------------------------------------------------
+template <typename T, int WN>
+__global__ void __launch_bounds__(256) k_convgemm(const DevPoint* __restrict__ pts,
+                                                  const int32_t* __restrict__ evalflag, int prec,
+                                                  const OpTile* __restrict__ tiles, const OpGroup* __restrict__ groups,
+                                                  const OpCol* __restrict__ cols, const ConvJob* __restrict__ jobs,
+                                                  const T* __restrict__ kern, int Ksys, const T* __restrict__ rhoT,
+                                                  int Frt, const T* __restrict__ dphi, int Fdp, T* __restrict__ nw,
+                                                  int Fnw, double* __restrict__ fx) {
+    constexpr int TM = 64, TN = WN * 16, KC = 64;
+    constexpr int XS = TN + 16;            // padded row of xs
+    constexpr int PER = KC * TN / 256;     // staged inputs per thread and chunk
+    constexpr int TPC = KC / PER;          // threads per staged column
+    constexpr int CWN = TM + KC - 1;       // operator window length
+    const OpTile tile = tiles[blockIdx.x];
+    const OpGroup* __restrict__ Gp = groups + tile.group;  // read fields in place (dynamic term index)
+    const int NP = Gp->NP, L = Gp->L;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    __shared__ T xs[KC * XS];
+    __shared__ T cmw[TM + KC];
+    __shared__ T cpw[TM + KC];
+    __shared__ int32_t colp[TN];
+    __shared__ int32_t coljob[TN];
+    __shared__ int any_active;
+
+    if (tid == 0) any_active = 0;
+    __syncthreads();
+    if (tid < TN) {
+        int p = -1, job = 0;
+        if (tid < tile.ncols) {
+            const OpCol c = cols[Gp->col_off + tile.col0 + tid];
+            p = c.p;
+            job = c.job;
+            if (evalflag[p] != prec) p = -1;
+        }
+        colp[tid] = p;
+        coljob[tid] = job;
+        if (p >= 0) atomicOr(&any_active, 1);
+    }
+    __syncthreads();
+    if (!any_active) return;
+
+    const int r0 = tile.row0;
+    const int scol = tid / TPC, sk0 = (tid % TPC) * PER;
+    const int sp = colp[scol];
+    const T* cset = kern + Gp->kbase * (int64_t)Ksys;
+    const int nk = (NP + KC - 1) / KC;
+    const int nchunks = Gp->nterms * nk;
+
+    T xr[PER];
+    T cr = T(0);
+    auto fetch = [&](int ch) {
+        const int term = ch / nk, k0 = (ch - term * nk) * KC;
+        const T* in = nullptr;
+        if (sp >= 0) {
+            const DevPoint& P = pts[sp];
+            const ConvTerm* tm = jobs[coljob[scol]].t + term;
+            const int infield = tm->infield;
+            in = tm->infam == 0 ? rhoT + P.base * (int64_t)Frt + (int64_t)infield * P.NP
+                                : dphi + P.base * (int64_t)Fdp + (int64_t)infield * P.NP;
+        }
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int nn = k0 + sk0 + q;
+            xr[q] = (in != nullptr && nn < NP) ? in[nn] : T(0);
+        }
+        const T* c = cset + (int64_t)Gp->kern[term] * L;
+        if (tid < CWN) {
+            const T sgn = Gp->sign[term] < 0 ? T(-1) : T(1);
+            cr = sgn * c[wrapLg(r0 - k0 - (KC - 1) + tid, L)];
+        } else if (tid >= 128 && tid < 128 + CWN) {
+            cr = c[wrapLg(r0 + k0 + 1 + (tid - 128), L)];
+        }
+    };
+
+    typename Mfma<T>::acc_t acc[WN];
+#pragma unroll
+    for (int j = 0; j < WN; ++j)
+        for (int v = 0; v < 4; ++v) acc[j][v] = T(0);
+
+    fetch(0);
+    const int rl = wave * 16 + (lane & 15);
+    for (int ch = 0; ch < nchunks; ++ch) {
+        __syncthreads();  // the previous chunk's products are done with xs / windows
+#pragma unroll
+        for (int q = 0; q < PER; ++q) xs[(sk0 + q) * XS + scol] = xr[q];
+        if (tid < CWN) cmw[tid] = cr;
+        else if (tid >= 128 && tid < 128 + CWN) cpw[tid - 128] = cr;
+        __syncthreads();
+        if (ch + 1 < nchunks) fetch(ch + 1);  // in flight during the products below
+        // operands of k-step pairs in ping-pong registers: the LDS reads of the next pair
+        // are in flight while the current pair runs on the matrix core
+        T cm0[2], cp0[2], bv0[2][WN], cm1[2], cp1[2], bv1[2][WN];
+        auto lds_fetch = [&](int kp, T (&cm)[2], T (&cp)[2], T (&bv)[2][WN]) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int kl = (2 * kp + h) * 4 + (lane >> 4);
+                cm[h] = cmw[rl - kl + KC - 1];
+                cp[h] = cpw[rl + kl];
+#pragma unroll
+                for (int j = 0; j < WN; ++j) bv[h][j] = xs[kl * XS + j * 16 + (lane & 15)];
+            }
+        };
+        auto products = [&](T (&cm)[2], T (&cp)[2], T (&bv)[2][WN]) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const T av = cm[h] + cp[h];
+#pragma unroll
+                for (int j = 0; j < WN; ++j) acc[j] = Mfma<T>::mma(av, bv[h][j], acc[j]);
+            }
+        };
+        lds_fetch(0, cm0, cp0, bv0);
+#pragma unroll
+        for (int kp = 0; kp < KC / 8; kp += 2) {
+            lds_fetch(kp + 1, cm1, cp1, bv1);
+            __builtin_amdgcn_sched_barrier(0);
+            products(cm0, cp0, bv0);
+            __builtin_amdgcn_sched_barrier(0);
+            if (kp + 2 < KC / 8) lds_fetch(kp + 2, cm0, cp0, bv0);
+            __builtin_amdgcn_sched_barrier(0);
+            products(cm1, cp1, bv1);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+
+#pragma unroll
+    for (int j = 0; j < WN; ++j) {
+        const int cl = j * 16 + (lane & 15);
+        const int p = colp[cl];
+        if (p < 0) continue;
+        const DevPoint& P = pts[p];
+        const ConvJob& J = jobs[coljob[cl]];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int n = r0 + wave * 16 + Mfma<T>::row(lane, v);
+            if (n >= NP) continue;
+            double yy = (double)acc[j][v] * J.scale;
+            if (J.absval) yy = fabs(yy);
+            if (J.outkind == 0) nw[P.base * (int64_t)Fnw + (int64_t)J.outfield * P.NP + n] = (T)yy;
+            else fx[P.xoff + (int64_t)J.outfield * P.NP + n] = yy;
+        }
+    }
+}
+
+// --------------------------------------------------------------------------
 // host launchers
 // --------------------------------------------------------------------------
 template <typename T>
@@ -309,11 +461,9 @@ void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& pl
     PropStep ps;
     memset(&ps, 0, sizeof(ps));
     if (plan.wide)
-        hipLaunchKernelGGL((k_opgemm<T, 1, 4, false>), dim3(plan.ntiles), dim3(256), 0, st, c.d_sys, c.d_pts,
-                           c.d_evalflag, prec, (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups, plan.d_cols,
-                           c.d_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx, ps, c.d_seg_states, c.d_maxd,
-                           (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (T*)nullptr, (T*)nullptr,
-                           (const T*)nullptr, (T*)nullptr);
+        hipLaunchKernelGGL((k_convgemm<T, 4>), dim3(plan.ntiles), dim3(256), 0, st, c.d_pts, c.d_evalflag, prec,
+                           (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups, (const OpCol*)plan.d_cols,
+                           c.d_jobs, kern, c.K, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx);
     else
         hipLaunchKernelGGL((k_opgemm<T, 2, 1, false>), dim3(plan.ntiles), dim3(256), 0, st, c.d_sys, c.d_pts,
                            c.d_evalflag, prec, (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups, plan.d_cols,
