@@ -70,6 +70,8 @@ def main():
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--kernel-only", choices=["chain32", "chain64", "fwd32", "fwd64", "back32", "back64"],
                     help="after one mixed solve, only re-launch this kernel class 20 times (for --pmc runs)")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="stop after the timed mixed solves (for rocprofv3 --kernel-trace of the solve itself)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -113,6 +115,10 @@ def main():
     value = args.points * args.steps / total
     it32, it64 = batch.pipeline_iters()
     log(f"batch iterations with the FP32 / FP64 pipeline running: {it32} / {it64}")
+    if args.solve_only:
+        log(f"solve-only: {value:.1f} state points/s over {args.steps} timed solves")
+        batch.close()
+        return
 
     # per-launch time of the hot kernel classes, HIP events on the library stream, with the
     # operators the mixed solve used; weighted by launches per evaluation x iterations run

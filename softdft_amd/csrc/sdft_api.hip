@@ -103,7 +103,17 @@ extern "C" void sdft_destroy(sdft_ctx_t* ctx) {
     delete ctx;
 }
 
-static int build_jobs(const sdft_desc_t* d, std::vector<ConvJob>& fwd, std::vector<ConvJob>& back) {
+// Convolution job table (functionals A14-A20): forward weighted densities; shared back-projection
+// partials together with the lng and Psi projections; the bead back projections into F(x).
+//  * n_V = w_V * rho_j enters only summed over beads (no |.|), so it is one job per diameter class on
+//    the class sum of rho_j trapez_j.
+//  * mu_ex_j = w2_j * ch0_j + w3_j * ch1_j - wV_j * chv_j. With FMT, ch1 (dPhi4) is the same field for
+//    every bead and chv (dPhi5/(2 pi s) + dPhi8) the same within a diameter class, whose beads share
+//    w3 and wV: w3 * ch1 - wV * chv is computed once per class (outkind 2) and added to each bead's
+//    w2 * ch0_j (linearity; the reference sums the three per bead).
+//  * the two ends of a bond with equal diameters share kernel and input: one job writes both lng fields.
+static int build_jobs(const sdft_desc_t* d, const DevSystem& sys, std::vector<ConvJob>& fwd,
+                      std::vector<ConvJob>& pre, std::vector<ConvJob>& back, int& Fws) {
     const int NB = d->NB, MB = d->MB;
     const int F = d->functionals;
     const bool fmt = F & (SDFT_F_MFMT | SDFT_F_TPT1MFMT);
@@ -113,6 +123,7 @@ static int build_jobs(const sdft_desc_t* d, std::vector<ConvJob>& fwd, std::vect
         ConvJob j;
         memset(&j, 0, sizeof(j));
         j.outkind = outkind; j.outfield = outfield; j.absval = absval; j.scale = scale;
+        j.addfield = -1; j.outfield2 = -1;
         return j;
     };
     auto add = [](ConvJob& j, int kern, int odd, int infam, int infield) {
@@ -121,9 +132,16 @@ static int build_jobs(const sdft_desc_t* d, std::vector<ConvJob>& fwd, std::vect
     };
     for (int j = 0; j < NB; ++j) {
         if (fmt || msa) { ConvJob a = job(0, j, 1, 1.0); add(a, 3 * j + 0, 0, 0, j); fwd.push_back(a); }
-        if (fmt) {
-            ConvJob b = job(0, NB + j, 1, 1.0); add(b, 3 * j + 1, 0, 0, j); fwd.push_back(b);
-            ConvJob c = job(0, 2 * NB + j, 0, 1.0); add(c, 3 * j + 2, 1, 0, j); fwd.push_back(c);
+        if (fmt) { ConvJob b = job(0, NB + j, 1, 1.0); add(b, 3 * j + 1, 0, 0, j); fwd.push_back(b); }
+    }
+    // vector weighted density (no |.|, linear): one per diameter class on the class density sum
+    if (fmt) {
+        for (int c = 0; c < sys.ncls; ++c) {
+            int j0 = 0;
+            while (sys.cls[j0] != c) ++j0;
+            ConvJob v = job(0, 2 * NB + c, 0, 1.0);
+            add(v, 3 * j0 + 2, 1, 0, NB + 2 * MB + c);
+            fwd.push_back(v);
         }
     }
     if (tpt1) {
@@ -136,27 +154,39 @@ static int build_jobs(const sdft_desc_t* d, std::vector<ConvJob>& fwd, std::vect
                 fwd.push_back(a);
             }
     }
+    Fws = 0;
+    if (fmt) {
+        Fws = sys.ncls;
+        for (int c = 0; c < sys.ncls; ++c) {
+            int j0 = 0;
+            while (sys.cls[j0] != c) ++j0;
+            ConvJob w = job(2, c, 0, 1.0);
+            add(w, 3 * j0 + 1, 0, 1, NB + j0);
+            add(w, 3 * j0 + 2, 2, 1, 2 * NB + j0);
+            pre.push_back(w);
+        }
+    }
     for (int j = 0; j < NB; ++j) {
         ConvJob a = job(1, j, 0, 1.0);
-        if (fmt) {
-            add(a, 3 * j + 0, 0, 1, j);
-            add(a, 3 * j + 1, 0, 1, NB + j);
-            add(a, 3 * j + 2, 2, 1, 2 * NB + j);
-        } else if (msa) {
-            add(a, 3 * j + 0, 0, 1, j);
-        }
+        if (fmt || msa) add(a, 3 * j + 0, 0, 1, j);
+        if (fmt) a.addfield = sys.cls[j];
         back.push_back(a);
     }
-    for (int u = 0; u < MB; ++u)
+    // lng and Psi need no shared partial: they run in the same launch as the partials
+    for (int u = 0; u < MB; ++u) {
+        const bool same = d->diameters[d->bond_types[2 * u]] == d->diameters[d->bond_types[2 * u + 1]];
         for (int e = 0; e < 2; ++e) {
+            if (e == 1 && same) continue;  // written by the e = 0 job
             const int bt = d->bond_types[2 * u + e];
             ConvJob a = job(1, NB + 2 * u + e, 0, 1.0);
             if (tpt1) add(a, 3 * bt + 0, 0, 1, 3 * NB + 2 * u + e);
-            back.push_back(a);
+            if (same) a.outfield2 = NB + 2 * u + 1;
+            pre.push_back(a);
         }
+    }
     ConvJob psi = job(1, NB + 2 * MB, 0, 1.0);
     add(psi, 3 * NB + MB, 0, 1, 3 * NB + 2 * MB);
-    back.push_back(psi);
+    pre.push_back(psi);
     return SDFT_OK;
 }
 
@@ -263,6 +293,13 @@ static int build_conv_plan(sdft_ctx* ctx, const std::vector<ConvJob>& jobs, int 
     TRY(upload_vec(ctx, &plan.d_tiles, tiles));
     TRY(upload_vec(ctx, &plan.d_groups, gv));
     TRY(upload_vec(ctx, &plan.d_cols, cols));
+    plan.ngroups = (int)gv.size();
+    for (int k = 0; k < 2; ++k) {
+        OpCol* act = nullptr;
+        TRY(dalloc(ctx, &act, (int64_t)cols.size()));
+        plan.d_cols_act[k] = act;
+        TRY(dalloc(ctx, &plan.d_count[k], (int64_t)gv.size()));
+    }
     return SDFT_OK;
 }
 
@@ -529,7 +566,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     }
     ctx->seg_states_h.assign(d->seg_states, d->seg_states + state_off);
     c.GF = (int)goff;
-    c.Frt = NB + 2 * MB;
+    c.Frt = NB + 2 * MB + sys.ncls;  // bead and bond-end densities x trapez, then one class sum per diameter
     c.Fnw = 3 * NB + 2 * MB;
     c.Fdp = 3 * NB + 2 * MB + 1;
 
@@ -544,7 +581,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         DevPoint& q = ctx->pts[p];
         memset(&q, 0, sizeof(q));
         const int NP = d->NP[p];
-        if (NP < 4 || NP > SDFT_MAX_TILES * 256 || !(d->dz[p] > 0.0)) {
+        if (NP < 4 || NP > SDFT_MAX_TILES * SDFT_ACC_POINTS || !(d->dz[p] > 0.0)) {
             set_error(ctx, SDFT_E_ARG, "point %d: NP=%d dz=%g", p, NP, d->dz[p]);
             return fail(SDFT_E_ARG);
         }
@@ -606,13 +643,16 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     ctx->nsets = (int)set_L.size();
 
     // ---- jobs
-    std::vector<ConvJob> fwd, back;
-    build_jobs(d, fwd, back);
+    std::vector<ConvJob> fwd, pre, back;
+    build_jobs(d, sys, fwd, pre, back, c.Fws);
     c.n_fwd_jobs = (int)fwd.size();
+    c.n_pre_jobs = (int)pre.size();
     c.n_back_jobs = (int)back.size();
     for (const ConvJob& j : fwd) ctx->fwd_terms += j.nterms;
+    for (const ConvJob& j : pre) ctx->back_terms += j.nterms;
     for (const ConvJob& j : back) ctx->back_terms += j.nterms;
     std::vector<ConvJob> jobs(fwd);
+    jobs.insert(jobs.end(), pre.begin(), pre.end());
     jobs.insert(jobs.end(), back.begin(), back.end());
 
     // ---- uploads
@@ -698,7 +738,10 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         std::vector<int> dummy;
         if ((rc = build_conv_plan(ctx, jobs, 0, c.n_fwd_jobs, canon, dummy, set_L, set_kbase, c.fwd_plan)) != SDFT_OK)
             return fail(rc);
-        if ((rc = build_conv_plan(ctx, jobs, c.n_fwd_jobs, c.n_back_jobs, canon, dummy, set_L, set_kbase, c.back_plan)) != SDFT_OK)
+        if ((rc = build_conv_plan(ctx, jobs, c.n_fwd_jobs, c.n_pre_jobs, canon, dummy, set_L, set_kbase, c.pre_plan)) != SDFT_OK)
+            return fail(rc);
+        if ((rc = build_conv_plan(ctx, jobs, c.n_fwd_jobs + c.n_pre_jobs, c.n_back_jobs, canon, dummy, set_L, set_kbase,
+                                  c.back_plan)) != SDFT_OK)
             return fail(rc);
         c.prop_plan.assign(nsp, OpPlan());
         for (int u = 0; u < nsp; ++u)
@@ -746,6 +789,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     if ((rc = dalloc(ctx, &c.d_x, c.total_n)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_fx, c.total_n)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_gram, (int64_t)P * SDFT_GRAM_BLOCKS * SDFT_GRAM_LEN)) != SDFT_OK) return fail(rc);
+    if (c.Fws > 0 && (rc = dalloc(ctx, &c.d_wsum, c.sumNP * c.Fws)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_errbits, P)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_state, P)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_n_active, 3)) != SDFT_OK) return fail(rc);
@@ -800,6 +844,7 @@ extern "C" int32_t sdft_objective(sdft_ctx_t* ctx, const double* x, double* fx, 
     HIP_CHECK_RET(hipMemcpyAsync(c.d_state, st.data(), st.size() * sizeof(DevSolveState), hipMemcpyHostToDevice, ctx->stream));
     c.band_fp32 = ctx->obj_band32;
     c.band_fp64 = ctx->obj_band64;
+    if (c.use_gemm) launch_compact_cols(ctx->stream, c, precision);
     run_objective(ctx, precision == 32, precision == 64);
     HIP_CHECK_RET(hipGetLastError());
     HIP_CHECK_RET(hipMemcpyAsync(fx, c.d_fx, c.total_n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -856,6 +901,12 @@ extern "C" int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* o, doub
     // pipeline with no point at its precision is not launched
     int32_t counts[3] = {c.P, sp.start_prec == 32 ? c.P : 0, 0};
     bool run32 = sp.start_prec == 32, run64 = sp.start_prec == 64;
+    auto compact = [&]() {
+        if (!c.use_gemm) return;
+        if (run32) launch_compact_cols(ctx->stream, c, 32);
+        if (run64) launch_compact_cols(ctx->stream, c, 64);
+    };
+    compact();
     int it = 0, n32 = 0, n64 = 0;
     for (it = 0; it < o->max_iters; ++it) {
         run_objective(ctx, run32, run64);
@@ -869,6 +920,7 @@ extern "C" int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* o, doub
             if (counts[2] > 0) launch_apply_switch(ctx->stream, c);
             run32 = counts[1] - counts[2] > 0;
             run64 = counts[0] - counts[1] + counts[2] > 0;
+            compact();
         }
     }
     HIP_CHECK_RET(hipGetLastError());
@@ -1093,6 +1145,7 @@ extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps
     std::vector<int32_t> saved(c.P);
     HIP_CHECK_RET(hipMemcpyAsync(saved.data(), c.d_evalflag, c.P * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     launch_set_evalflag(ctx->stream, c.d_evalflag, prec, c.P);
+    if (c.use_gemm) launch_compact_cols(ctx->stream, c, prec);  // every point active in the profiled launches
     const int s_mid = c.species[u].nseg / 2 > 0 ? c.species[u].nseg / 2 : 1;
     auto once = [&]() {
         switch (kc) {

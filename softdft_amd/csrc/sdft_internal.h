@@ -16,7 +16,8 @@
 #define SDFT_MAX_S 2
 #define SDFT_MAX_SPECIES 16
 #define SDFT_MAX_M 8
-#define SDFT_MAX_TILES 128          // 256-point tiles per grid (NP <= 32768)
+#define SDFT_ACC_POINTS 64          // grid points per density-accumulation block
+#define SDFT_MAX_TILES 256          // accumulation blocks per grid (NP <= 256 * 64 = 16384)
 #define SDFT_GRAM_BLOCKS 64         // most blocks per point for the Anderson reductions
 #define SDFT_GRAM_CHUNK 2048        // solution entries per reduction block
 #define SDFT_GRAM_LEN 44            // SDFT_MAX_M*(SDFT_MAX_M+1)/2 + SDFT_MAX_M
@@ -92,9 +93,13 @@ struct ConvTerm {
 struct ConvJob {
     int32_t nterms;
     ConvTerm t[3];
-    int32_t outkind;   // 0: weighted-density family (T), 1: F(x) (double) at field offset
+    int32_t outkind;   // 0: weighted-density family (T), 1: F(x) (double) at field offset,
+                       // 2: shared back-projection partial (double, one field per diameter class)
     int32_t outfield;  // field index in the family, or field offset in units of NP within the solution vector
     int32_t absval;    // take |.| of the result (scalar weighted densities)
+    int32_t addfield;  // outkind 1: shared partial added to the result (-1: none)
+    int32_t outfield2; // outkind 1: second F(x) field receiving the same values (-1: none)
+    int32_t pad;
     double scale;
 };
 

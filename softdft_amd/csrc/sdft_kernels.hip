@@ -399,6 +399,10 @@ __global__ void __launch_bounds__(256) k_fixed_lambda(const DevSystem* __restric
     }
 }
 
+constexpr int ACC_POINTS = SDFT_ACC_POINTS;  // grid points per k_accumulate block (one wavefront wide)
+constexpr int ACC_GROUPS = 4;   // wavefronts per block, each taking every 4th segment
+static_assert(ACC_POINTS * ACC_GROUPS == 256, "k_accumulate runs 256-thread blocks");
+
 // --------------------------------------------------------------------------
 // densities (accumulate_densities!, accumulate_densities.jl:95-184; update_spatial_structures!)
 // rho_seg(q,a) = norm e^{mu} exp(-lambda_si) Gp(q,a) Qc(q,a)
@@ -416,13 +420,18 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
                                                    const T* __restrict__ gC, const T* __restrict__ gP,
                                                    const double* __restrict__ fixed_lambda,
                                                    T* __restrict__ rho_beads, T* __restrict__ rho_bonds, T* __restrict__ rhoT,
-                                                   double* __restrict__ acc_part, int32_t* __restrict__ errflag) {
+                                                   int Frt, double* __restrict__ acc_part, int32_t* __restrict__ errflag) {
+    // A block covers ACC_POINTS grid points; its ACC_GROUPS wavefronts split the segments of every
+    // species (segment q goes to group q % ACC_GROUPS), so more independent loads are in flight per
+    // point; the groups' partial densities are added in group order at the end.
     const DevSystem& sys = *psys;
     const int p = blockIdx.y;
     if (evalflag[p] != prec) return;
     const DevPoint& P = pts[p];
     const int NP = P.NP, NB = sys.NB, MB = sys.MB;
-    const int n = blockIdx.x * TB + threadIdx.x;
+    const int nl = threadIdx.x % ACC_POINTS;
+    const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x / ACC_POINTS);  // wave-uniform: scalar segment loop
+    const int n = blockIdx.x * ACC_POINTS + nl;
     const bool valid = n < NP;
     const int nc = valid ? n : NP - 1;
     // per-thread accumulators in LDS, one column per thread: rb[j * TB], rbo[f * TB]
@@ -455,21 +464,28 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
             return P.base * (int64_t)GF + (goffs[u] + ((int64_t)seg * S + b) * S + a) * NP + nn;
         };
         double num = 0.0, chg = 0.0;
-        for (int q = 0; q < N; ++q) {
+        for (int q = grp; q < N; q += ACC_GROUPS) {
             // every propagator value of this segment is loaded up front (independent loads in flight),
             // gcv[b][a] = gC[q][b][a] (child side, b: state of q+1), gpv[b][a] = gP[q][b][a] (parent side)
+            // The loads are unconditional, from clamped (always in-bounds) indices, and masked after
+            // the fact, so that they issue back to back instead of one conditional load at a time.
             T gcv[SDFT_MAX_S][SDFT_MAX_S], gpv[SDFT_MAX_S][SDFT_MAX_S], elv[SDFT_MAX_S], tzv[SDFT_MAX_S];
 #pragma unroll
             for (int a = 0; a < SDFT_MAX_S; ++a) {
+                const int ac = a < S ? a : 0;
                 const int si = a < S ? st[q * S + a] : -1;
-                elv[a] = si >= 0 ? elam[fidx(P, NB, si, nc)] : T(0);
-                tzv[a] = si >= 0 ? trapez[fidx(P, NB, si, nc)] : T(0);
+                const int sic = si >= 0 ? si : 0;
+                const T e = elam[fidx(P, NB, sic, nc)], t = trapez[fidx(P, NB, sic, nc)];
+                elv[a] = si >= 0 ? e : T(0);
+                tzv[a] = si >= 0 ? t : T(0);
 #pragma unroll
                 for (int b = 0; b < SDFT_MAX_S; ++b) {
+                    const int bc = b < S ? b : 0;
                     const bool okc = si >= 0 && b < S && q < N - 1 && st[(q + 1) * S + b] >= 0;
                     const bool okp = si >= 0 && b < S && q > 0 && st[(q - 1) * S + b] >= 0;
-                    gcv[b][a] = okc ? gC[gidx(q, b, a, nc)] : T(0);
-                    gpv[b][a] = okp ? gP[gidx(q, b, a, nc)] : T(0);
+                    const T vc = gC[gidx(q, bc, ac, nc)], vp = gP[gidx(q, bc, ac, nc)];
+                    gcv[b][a] = okc ? vc : T(0);
+                    gpv[b][a] = okp ? vp : T(0);
                 }
             }
 #pragma unroll
@@ -547,17 +563,32 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
         for (int w = 0; w < TB / 64; ++w) s += red[threadIdx.x][w];
         acc_part[((int64_t)p * SDFT_MAX_TILES + blockIdx.x) * 2 * SDFT_MAX_SPECIES + threadIdx.x] = s;
     }
-    if (!valid) return;
+    if (!valid || grp != 0) return;
+    // densities of this grid point: the segment groups' partials in group order
+    auto total = [&](int f) {
+        double v = 0.0;
+        for (int g = 0; g < ACC_GROUPS; ++g) v += acc_lds[f * TB + g * ACC_POINTS + nl];
+        return v;
+    };
     for (int j = 0; j < NB; ++j) {
-        rho_beads[fidx(P, NB, j, n)] = (T)rb[j * TB];
-        rhoT[fidx(P, NB + 2 * MB, j, n)] = (T)(rb[j * TB] * (double)trapez[fidx(P, NB, j, n)]);
+        const double v = total(j);
+        rho_beads[fidx(P, NB, j, n)] = (T)v;
+        rhoT[fidx(P, Frt, j, n)] = (T)(v * (double)trapez[fidx(P, NB, j, n)]);
     }
     for (int u = 0; u < MB; ++u) {
         for (int e = 0; e < 2; ++e) {
             const int bt = e == 0 ? sys.bond_i[u] : sys.bond_j[u];
-            rho_bonds[fidx(P, 2 * MB, 2 * u + e, n)] = (T)rbo[(2 * u + e) * TB];
-            rhoT[fidx(P, NB + 2 * MB, NB + 2 * u + e, n)] = (T)(rbo[(2 * u + e) * TB] * (double)trapez[fidx(P, NB, bt, n)]);
+            const double v = total(NB + 2 * u + e);
+            rho_bonds[fidx(P, 2 * MB, 2 * u + e, n)] = (T)v;
+            rhoT[fidx(P, Frt, NB + 2 * u + e, n)] = (T)(v * (double)trapez[fidx(P, NB, bt, n)]);
         }
+    }
+    // per diameter class: sum of rho_j trapez_j over its beads (input of the class-shared vector weight)
+    for (int c = 0; c < sys.ncls; ++c) {
+        double t = 0.0;
+        for (int j = 0; j < NB; ++j)
+            if (sys.cls[j] == c) t += total(j) * (double)trapez[fidx(P, NB, j, n)];
+        rhoT[fidx(P, Frt, NB + 2 * MB + c, n)] = (T)t;
     }
 }
 
@@ -569,7 +600,8 @@ __global__ void __launch_bounds__(TB) k_conv(const DevSystem* __restrict__ psys,
                                              const int32_t* __restrict__ evalflag, int prec,
                                              const ConvJob* __restrict__ jobs, const T* __restrict__ kern,
                                              const T* __restrict__ rhoT, int Frt, const T* __restrict__ dphi, int Fdp,
-                                             T* __restrict__ nw, int Fnw, double* __restrict__ fx) {
+                                             T* __restrict__ nw, int Fnw, double* __restrict__ fx,
+                                             double* __restrict__ wsum, int Fws) {
     const DevSystem& sys = *psys;
     const int p = blockIdx.z;
     if (evalflag[p] != prec) return;
@@ -603,10 +635,7 @@ __global__ void __launch_bounds__(TB) k_conv(const DevSystem* __restrict__ psys,
         }
     }
     if (n >= NP) return;
-    double y = (double)acc * J.scale;
-    if (J.absval) y = fabs(y);
-    if (J.outkind == 0) nw[fidx(P, Fnw, J.outfield, n)] = (T)y;
-    else fx[P.xoff + (int64_t)J.outfield * NP + n] = y;
+    conv_store<T>(J, P, n, (double)acc, nw, Fnw, fx, wsum, Fws);
 }
 
 // --------------------------------------------------------------------------
@@ -634,11 +663,15 @@ __device__ FmtIn fmt_load(const DevSystem& sys, const DevPoint& P, int n, const 
     for (int j = 0; j < NB; ++j) {
         const double s = sys.sigma[j];
         const double n2 = (double)nw[fidx(P, Fnw, j, n)];
-        const double nv = (double)nw[fidx(P, Fnw, 2 * NB + j, n)];
         f.n0 += n2 / (M_PI * s * s);
         f.n1 += n2 / (2.0 * M_PI * s);
         f.n2 += n2;
         f.n3 += (double)nw[fidx(P, Fnw, NB + j, n)];
+    }
+    // the vector weighted densities are linear in rho: one per diameter class (beads share w_V)
+    for (int c = 0; c < sys.ncls; ++c) {
+        const double s = sys.cls_sigma[c];
+        const double nv = (double)nw[fidx(P, Fnw, 2 * NB + c, n)];
         f.nV1 += nv / (2.0 * M_PI * s);
         f.nV2 += nv;
     }
@@ -945,7 +978,7 @@ __global__ void __launch_bounds__(64) k_neutrality(const DevSystem* __restrict__
     if (evalflag[p] != prec) return;
     const DevPoint& P = pts[p];
     const int nsp = sys.n_species;
-    const int ntiles = (P.NP + 255) / 256;
+    const int ntiles = (P.NP + ACC_POINTS - 1) / ACC_POINTS;
     double t = 0.0;
     if (lane < 2 * nsp) {
         for (int b = 0; b < ntiles; ++b) t += acc_part[((int64_t)p * SDFT_MAX_TILES + b) * 2 * SDFT_MAX_SPECIES + lane];
@@ -1519,19 +1552,13 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
                                c.d_seg_states, c.d_maxd, elam, trapez, gC, gP, c.d_fixed_lambda);
         }
     }
-    hipLaunchKernelGGL((k_accumulate<T, 256>), grid2(maxNP, 256, P), dim3(256), accumulate_lds_bytes(c), st, c.d_sys,
+    hipLaunchKernelGGL((k_accumulate<T, 256>), grid2(maxNP, ACC_POINTS, P), dim3(256), accumulate_lds_bytes(c), st, c.d_sys,
                        c.d_pts, c.d_evalflag,
                        prec, c.d_spec, c.d_goffs, c.GF, c.d_seg_states, c.d_maxd, elam, trapez, gC, gP,
-                       c.d_fixed_lambda, rho_beads, rho_bonds, rhoT, c.d_acc_part, c.d_errflag);
-    if (c.use_gemm) launch_opgemm_conv<T>(st, c, prec, c.fwd_plan, kern, rhoT, dphi, nw, fx);
-    else if (c.n_fwd_jobs > 0)
-        hipLaunchKernelGGL((k_conv<T, 256>), grid2(maxNP, 256, c.n_fwd_jobs, P), dim3(256), 0, st, c.d_sys, c.d_pts,
-                           c.d_evalflag, prec, c.d_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx);
+                       c.d_fixed_lambda, rho_beads, rho_bonds, rhoT, c.Frt, c.d_acc_part, c.d_errflag);
+    launch_conv(st, c, prec, false, fx);
     launch_pointwise(st, c, prec);
-    if (c.use_gemm) launch_opgemm_conv<T>(st, c, prec, c.back_plan, kern, rhoT, dphi, nw, fx);
-    else
-        hipLaunchKernelGGL((k_conv<T, 256>), grid2(maxNP, 256, c.n_back_jobs, P), dim3(256), 0, st, c.d_sys, c.d_pts,
-                           c.d_evalflag, prec, c.d_jobs + c.n_fwd_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx);
+    launch_conv(st, c, prec, true, fx);
     hipLaunchKernelGGL(k_neutrality, dim3(P), dim3(64), 0, st, c.d_sys, c.d_pts, c.d_spec, c.d_evalflag, prec,
                        c.d_acc_part, c.d_acc_species, x, fx);
 }
@@ -1552,17 +1579,25 @@ void Pipeline<T>::launch_prop_step(hipStream_t st, const Ctx& c, int prec, int u
                            c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl, trapez, gC, gP, c.d_band, 0);
 }
 
+// forward: weighted densities; back: shared partials first, then the projections that add them
 template <typename T>
 void Pipeline<T>::launch_conv(hipStream_t st, const Ctx& c, int prec, bool back, double* fx) {
-    const int nj = back ? c.n_back_jobs : c.n_fwd_jobs;
-    if (nj == 0) return;
-    if (c.use_gemm) {
-        launch_opgemm_conv<T>(st, c, prec, back ? c.back_plan : c.fwd_plan, kern, rhoT, dphi, nw, fx);
-        return;
+    auto run = [&](const OpPlan& plan, int job0, int nj) {
+        if (nj == 0) return;
+        if (c.use_gemm) {
+            launch_opgemm_conv<T>(st, c, prec, plan, kern, rhoT, dphi, nw, fx);
+            return;
+        }
+        hipLaunchKernelGGL((k_conv<T, 256>), grid2(c.maxNP, 256, nj, c.P), dim3(256), 0, st, c.d_sys, c.d_pts,
+                           c.d_evalflag, prec, c.d_jobs + job0, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx,
+                           c.d_wsum, c.Fws);
+    };
+    if (!back) {
+        run(c.fwd_plan, 0, c.n_fwd_jobs);
+    } else {
+        run(c.pre_plan, c.n_fwd_jobs, c.n_pre_jobs);
+        run(c.back_plan, c.n_fwd_jobs + c.n_pre_jobs, c.n_back_jobs);
     }
-    hipLaunchKernelGGL((k_conv<T, 256>), grid2(c.maxNP, 256, nj, c.P), dim3(256), 0, st, c.d_sys, c.d_pts,
-                       c.d_evalflag, prec, c.d_jobs + (back ? c.n_fwd_jobs : 0), kern, rhoT, c.Frt, dphi, c.Fdp, nw,
-                       c.Fnw, fx);
 }
 
 template <typename T>

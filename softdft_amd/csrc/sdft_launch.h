@@ -39,6 +39,11 @@ struct OpPlan {
     void* d_tiles = nullptr;
     void* d_groups = nullptr;
     void* d_cols = nullptr;
+    int ngroups = 0;
+    // convolution plans, per precision (0: FP32, 1: FP64): the columns of each group whose point
+    // evaluates at that precision, compacted in order, and their number per group
+    void* d_cols_act[2] = {nullptr, nullptr};
+    int32_t* d_count[2] = {nullptr, nullptr};
 };
 
 // persistent banded chain kernel of one species (ok: eligible for the banded operator)
@@ -66,12 +71,13 @@ struct Ctx {
     int64_t total_n = 0, max_n = 0;
     int GF = 0;                     // propagator fields per point (sum over species of nseg*S*S)
     int Frt = 0, Fnw = 0, Fdp = 0;  // family widths
-    int n_fwd_jobs = 0, n_back_jobs = 0;
+    int n_fwd_jobs = 0, n_pre_jobs = 0, n_back_jobs = 0;  // job table: forward ++ shared partials ++ back
+    int Fws = 0;                    // shared back-projection partials per point (diameter classes)
     int band_fp32 = 1;
     int m = 5;                      // Anderson depth the history buffers were sized for
     int band_fp64 = 0;              // FP64 propagators banded (mixed-precision polish only)
     bool use_gemm = true;           // MFMA operator-GEMM path (else the direct reference kernels)
-    OpPlan fwd_plan, back_plan;
+    OpPlan fwd_plan, pre_plan, back_plan;
     std::vector<OpPlan> prop_plan;  // per species (ntiles == 0: direct kernel)
     std::vector<ChainPlanHost> chain_plan;  // per species
     std::vector<SpeciesHost> species;
@@ -83,6 +89,7 @@ struct Ctx {
     int32_t* d_seg_states = nullptr;
     double* d_maxd = nullptr;
     ConvJob* d_jobs = nullptr;
+    double* d_wsum = nullptr;       // (sumNP * Fws) shared back-projection partials
     int32_t* d_band = nullptr;      // banded half-width of the bond kernels per kernel set
     double* d_Ext = nullptr;        // (sumNP * NB)
     double* d_surf = nullptr;       // (sumNP)
@@ -101,6 +108,23 @@ struct Ctx {
     DevSolveState* d_state = nullptr;
     int32_t* d_n_active = nullptr;  // [active, active at FP32, waiting for the FP64 switch]
 };
+
+// stores one convolution column's value y at grid point n of point P (ConvJob semantics)
+template <typename T>
+__device__ __forceinline__ void conv_store(const ConvJob& J, const DevPoint& P, int n, double y, T* nw, int Fnw,
+                                           double* fx, double* wsum, int Fws) {
+    y *= J.scale;
+    if (J.absval) y = fabs(y);
+    if (J.outkind == 0) {
+        nw[P.base * (int64_t)Fnw + (int64_t)J.outfield * P.NP + n] = (T)y;
+    } else if (J.outkind == 2) {
+        wsum[P.base * (int64_t)Fws + (int64_t)J.outfield * P.NP + n] = y;
+    } else {
+        if (J.addfield >= 0) y += wsum[P.base * (int64_t)Fws + (int64_t)J.addfield * P.NP + n];
+        fx[P.xoff + (int64_t)J.outfield * P.NP + n] = y;
+        if (J.outfield2 >= 0) fx[P.xoff + (int64_t)J.outfield2 * P.NP + n] = y;
+    }
+}
 
 template <typename T>
 struct Pipeline {
@@ -125,6 +149,7 @@ struct Pipeline {
     void launch_pointwise(hipStream_t st, const Ctx& c, int prec);
 };
 
+void launch_compact_cols(hipStream_t st, const Ctx& c, int prec);
 template <typename T>
 void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, const T* kern, const T* rhoT,
                         const T* dphi, T* nw, double* fx);

@@ -56,6 +56,7 @@ __global__ void __launch_bounds__(256) k_opgemm(const DevSystem* __restrict__ ps
                                                 // MODE_CONV inputs / outputs
                                                 const T* __restrict__ rhoT, int Frt, const T* __restrict__ dphi, int Fdp,
                                                 T* __restrict__ nw, int Fnw, double* __restrict__ fx,
+                                                double* __restrict__ wsum, int Fws,
                                                 // MODE_PROP inputs / outputs
                                                 PropStep ps, const int32_t* __restrict__ seg_states,
                                                 const double* __restrict__ maxd_tab, const T* __restrict__ elam,
@@ -240,11 +241,7 @@ __global__ void __launch_bounds__(256) k_opgemm(const DevSystem* __restrict__ ps
                 for (int v = 0; v < 4; ++v) {
                     const int n = r0 + wave * WM * 16 + i * 16 + Mfma<T>::row(lane, v);
                     if (n >= NP) continue;
-                    const ConvJob& J = jobs[colaux[cl][0]];
-                    double yy = (double)acc[i][j][v] * J.scale;
-                    if (J.absval) yy = fabs(yy);
-                    if (J.outkind == 0) nw[P.base * (int64_t)Fnw + (int64_t)J.outfield * P.NP + n] = (T)yy;
-                    else fx[P.xoff + (int64_t)J.outfield * P.NP + n] = yy;
+                    conv_store<T>(jobs[colaux[cl][0]], P, n, (double)acc[i][j][v], nw, Fnw, fx, wsum, Fws);
                 }
             } else {
                 // every lane takes part (the state sum below is a lane shuffle)
@@ -300,43 +297,111 @@ __global__ void __launch_bounds__(256) k_opgemm(const DevSystem* __restrict__ ps
 }
 
 // --------------------------------------------------------------------------
+// Active-column compaction of the convolution plans, per precision: points in the other precision
+// or converged are dropped from every column group (order kept), so that in the late iterations of
+// a batch, when few points are still active, the GEMM tiles hold active columns only. It runs when
+// the set can grow (solve start, the host's polls, where precision switches happen) and the GEMMs
+// still check each column's point, so points that converge between two polls just idle. A column's
+// result does not depend on the other columns of its tile, so this leaves every value unchanged.
+// --------------------------------------------------------------------------
+struct CompactArgs {
+    const OpGroup* groups[3];
+    const OpCol* cols[3];
+    OpCol* act[3];
+    int32_t* count[3];
+    int32_t ngroups[3];
+};
+
+__global__ void __launch_bounds__(256) k_compact_cols(CompactArgs ca, const int32_t* __restrict__ evalflag, int prec) {
+    int g = blockIdx.x, pl = 0;
+    while (pl < 2 && g >= ca.ngroups[pl]) g -= ca.ngroups[pl++];
+    if (g >= ca.ngroups[pl]) return;
+    const OpGroup& G = ca.groups[pl][g];
+    const OpCol* in = ca.cols[pl] + G.col_off;
+    OpCol* out = ca.act[pl] + G.col_off;
+    __shared__ int wsum[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int base = 0;
+    for (int c0 = 0; c0 < G.ncols; c0 += 256) {
+        const int ci = c0 + tid;
+        OpCol c = {0, 0};
+        bool act = false;
+        if (ci < G.ncols) {
+            c = in[ci];
+            act = evalflag[c.p] == prec;
+        }
+        const unsigned long long m = __ballot(act);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        __syncthreads();
+        if (lane == 0) wsum[wave] = __popcll(m);
+        __syncthreads();
+        int off = base;
+        for (int w = 0; w < wave; ++w) off += wsum[w];
+        if (act) out[off + before] = c;
+        base += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    }
+    if (tid == 0) ca.count[pl][g] = base;
+}
+
+void launch_compact_cols(hipStream_t st, const Ctx& c, int prec) {
+    CompactArgs ca;
+    memset(&ca, 0, sizeof(ca));
+    const OpPlan* pl[3] = {&c.fwd_plan, &c.pre_plan, &c.back_plan};
+    int tot = 0;
+    for (int i = 0; i < 3; ++i) {
+        ca.groups[i] = (const OpGroup*)pl[i]->d_groups;
+        ca.cols[i] = (const OpCol*)pl[i]->d_cols;
+        ca.act[i] = (OpCol*)pl[i]->d_cols_act[prec == 64 ? 1 : 0];
+        ca.count[i] = pl[i]->d_count[prec == 64 ? 1 : 0];
+        ca.ngroups[i] = pl[i]->ntiles > 0 ? pl[i]->ngroups : 0;
+        tot += ca.ngroups[i];
+    }
+    if (tot > 0) hipLaunchKernelGGL(k_compact_cols, dim3(tot), dim3(256), 0, st, ca, c.d_evalflag, prec);
+}
+
+// --------------------------------------------------------------------------
 // Dense convolution GEMM (forward weighted densities, back projections).
 // Tile 64 rows x 16*WN columns, 4 waves stacked along rows, K chunks of 64 with the
 // next chunk (inputs and operator windows) prefetched into registers while the
 // current one runs on the matrix cores; sums over a job's terms chain the chunks.
-// xs rows are padded to TN + 16 so that the B reads of the four lane groups fall
-// in distinct LDS banks.
+// Inputs are staged column-major in LDS ([column][k], rows padded by 2): each staging
+// load reads 64 consecutive inputs of one column, and both the staging writes and the
+// B reads are free of bank conflicts.
 // --------------------------------------------------------------------------
 template <typename T, int WN>
 __global__ void __launch_bounds__(256) k_convgemm(const DevPoint* __restrict__ pts,
                                                   const int32_t* __restrict__ evalflag, int prec,
                                                   const OpTile* __restrict__ tiles, const OpGroup* __restrict__ groups,
-                                                  const OpCol* __restrict__ cols, const ConvJob* __restrict__ jobs,
+                                                  const OpCol* __restrict__ cols, const int32_t* __restrict__ count,
+                                                  const ConvJob* __restrict__ jobs,
                                                   const T* __restrict__ kern, int Ksys, const T* __restrict__ rhoT,
                                                   int Frt, const T* __restrict__ dphi, int Fdp, T* __restrict__ nw,
-                                                  int Fnw, double* __restrict__ fx) {
+                                                  int Fnw, double* __restrict__ fx, double* __restrict__ wsum,
+                                                  int Fws) {
     constexpr int TM = 64, TN = WN * 16, KC = 64;
-    constexpr int XS = TN + 16;            // padded row of xs
+    constexpr int XK = KC + 2;             // xs row (one column's KC inputs); the pad of 2 puts the B reads
+                                           // of lanes (q, g) on banks 2q + g: conflict-free
     constexpr int PER = KC * TN / 256;     // staged inputs per thread and chunk
-    constexpr int TPC = KC / PER;          // threads per staged column
     constexpr int CWN = TM + KC - 1;       // operator window length
+    static_assert(KC == 64, "one wavefront stages one column chunk");
     const OpTile tile = tiles[blockIdx.x];
     const OpGroup* __restrict__ Gp = groups + tile.group;  // read fields in place (dynamic term index)
     const int NP = Gp->NP, L = Gp->L;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
-    __shared__ T xs[KC * XS];
+    __shared__ T xs[TN * XK];              // [column][k]
     __shared__ T cmw[TM + KC];
     __shared__ T cpw[TM + KC];
     __shared__ int32_t colp[TN];
     __shared__ int32_t coljob[TN];
+    __shared__ const T* colin[3][TN];      // input field of each column and term (nullptr: inactive)
     __shared__ int any_active;
 
     if (tid == 0) any_active = 0;
     __syncthreads();
     if (tid < TN) {
         int p = -1, job = 0;
-        if (tid < tile.ncols) {
+        if (tid < tile.ncols && tile.col0 + tid < count[tile.group]) {
             const OpCol c = cols[Gp->col_off + tile.col0 + tid];
             p = c.p;
             job = c.job;
@@ -348,29 +413,36 @@ __global__ void __launch_bounds__(256) k_convgemm(const DevPoint* __restrict__ p
     }
     __syncthreads();
     if (!any_active) return;
+    if (tid < TN) {
+        const int sp = colp[tid];
+        for (int term = 0; term < 3; ++term) {
+            const T* in = nullptr;
+            if (sp >= 0 && term < Gp->nterms) {
+                const DevPoint& P = pts[sp];
+                const ConvTerm* tm = jobs[coljob[tid]].t + term;
+                in = tm->infam == 0 ? rhoT + P.base * (int64_t)Frt + (int64_t)tm->infield * P.NP
+                                    : dphi + P.base * (int64_t)Fdp + (int64_t)tm->infield * P.NP;
+            }
+            colin[term][tid] = in;
+        }
+    }
+    __syncthreads();
 
     const int r0 = tile.row0;
-    const int scol = tid / TPC, sk0 = (tid % TPC) * PER;
-    const int sp = colp[scol];
     const T* cset = kern + Gp->kbase * (int64_t)Ksys;
     const int nk = (NP + KC - 1) / KC;
     const int nchunks = Gp->nterms * nk;
 
+    // staging: wavefront w loads columns w, w + 4, ... (PER of them), lane = k within the chunk,
+    // so every load instruction reads 64 consecutive inputs of one column
     T xr[PER];
     T cr = T(0);
     auto fetch = [&](int ch) {
         const int term = ch / nk, k0 = (ch - term * nk) * KC;
-        const T* in = nullptr;
-        if (sp >= 0) {
-            const DevPoint& P = pts[sp];
-            const ConvTerm* tm = jobs[coljob[scol]].t + term;
-            const int infield = tm->infield;
-            in = tm->infam == 0 ? rhoT + P.base * (int64_t)Frt + (int64_t)infield * P.NP
-                                : dphi + P.base * (int64_t)Fdp + (int64_t)infield * P.NP;
-        }
+        const int nn = k0 + lane;
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
-            const int nn = k0 + sk0 + q;
+            const T* in = colin[term][4 * q + wave];
             xr[q] = (in != nullptr && nn < NP) ? in[nn] : T(0);
         }
         const T* c = cset + (int64_t)Gp->kern[term] * L;
@@ -392,7 +464,7 @@ __global__ void __launch_bounds__(256) k_convgemm(const DevPoint* __restrict__ p
     for (int ch = 0; ch < nchunks; ++ch) {
         __syncthreads();  // the previous chunk's products are done with xs / windows
 #pragma unroll
-        for (int q = 0; q < PER; ++q) xs[(sk0 + q) * XS + scol] = xr[q];
+        for (int q = 0; q < PER; ++q) xs[(4 * q + wave) * XK + lane] = xr[q];
         if (tid < CWN) cmw[tid] = cr;
         else if (tid >= 128 && tid < 128 + CWN) cpw[tid - 128] = cr;
         __syncthreads();
@@ -407,7 +479,7 @@ __global__ void __launch_bounds__(256) k_convgemm(const DevPoint* __restrict__ p
                 cm[h] = cmw[rl - kl + KC - 1];
                 cp[h] = cpw[rl + kl];
 #pragma unroll
-                for (int j = 0; j < WN; ++j) bv[h][j] = xs[kl * XS + j * 16 + (lane & 15)];
+                for (int j = 0; j < WN; ++j) bv[h][j] = xs[(j * 16 + (lane & 15)) * XK + kl];
             }
         };
         auto products = [&](T (&cm)[2], T (&cp)[2], T (&bv)[2][WN]) {
@@ -443,10 +515,7 @@ __global__ void __launch_bounds__(256) k_convgemm(const DevPoint* __restrict__ p
         for (int v = 0; v < 4; ++v) {
             const int n = r0 + wave * 16 + Mfma<T>::row(lane, v);
             if (n >= NP) continue;
-            double yy = (double)acc[j][v] * J.scale;
-            if (J.absval) yy = fabs(yy);
-            if (J.outkind == 0) nw[P.base * (int64_t)Fnw + (int64_t)J.outfield * P.NP + n] = (T)yy;
-            else fx[P.xoff + (int64_t)J.outfield * P.NP + n] = yy;
+            conv_store<T>(J, P, n, (double)acc[j][v], nw, Fnw, fx, wsum, Fws);
         }
     }
 }
@@ -462,12 +531,14 @@ void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& pl
     memset(&ps, 0, sizeof(ps));
     if (plan.wide)
         hipLaunchKernelGGL((k_convgemm<T, 4>), dim3(plan.ntiles), dim3(256), 0, st, c.d_pts, c.d_evalflag, prec,
-                           (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups, (const OpCol*)plan.d_cols,
-                           c.d_jobs, kern, c.K, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx);
+                           (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups,
+                           (const OpCol*)plan.d_cols_act[prec == 64 ? 1 : 0], plan.d_count[prec == 64 ? 1 : 0],
+                           c.d_jobs, kern, c.K, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx, c.d_wsum, c.Fws);
     else
         hipLaunchKernelGGL((k_opgemm<T, 2, 1, false>), dim3(plan.ntiles), dim3(256), 0, st, c.d_sys, c.d_pts,
                            c.d_evalflag, prec, (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups, plan.d_cols,
-                           c.d_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx, ps, c.d_seg_states, c.d_maxd,
+                           c.d_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx, c.d_wsum, c.Fws, ps,
+                           c.d_seg_states, c.d_maxd,
                            (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (T*)nullptr, (T*)nullptr,
                            (const T*)nullptr, (T*)nullptr);
 }
@@ -492,12 +563,12 @@ void launch_opgemm_prop(hipStream_t st, const Ctx& c, int prec, const OpPlan& pl
         hipLaunchKernelGGL((k_opgemm<T, 1, 4, true>), dim3(plan.ntiles), dim3(256), 0, st, c.d_sys, c.d_pts,
                            c.d_evalflag, prec, (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups, plan.d_cols,
                            c.d_jobs, kern, (const T*)nullptr, c.Frt, (const T*)nullptr, c.Fdp, (T*)nullptr, c.Fnw,
-                           (double*)nullptr, ps, c.d_seg_states, c.d_maxd, elam, hl, trapez, gC, gP, fin, fout);
+                           (double*)nullptr, (double*)nullptr, 0, ps, c.d_seg_states, c.d_maxd, elam, hl, trapez, gC, gP, fin, fout);
     else
         hipLaunchKernelGGL((k_opgemm<T, 2, 1, true>), dim3(plan.ntiles), dim3(256), 0, st, c.d_sys, c.d_pts,
                            c.d_evalflag, prec, (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups, plan.d_cols,
                            c.d_jobs, kern, (const T*)nullptr, c.Frt, (const T*)nullptr, c.Fdp, (T*)nullptr, c.Fnw,
-                           (double*)nullptr, ps, c.d_seg_states, c.d_maxd, elam, hl, trapez, gC, gP, fin, fout);
+                           (double*)nullptr, (double*)nullptr, 0, ps, c.d_seg_states, c.d_maxd, elam, hl, trapez, gC, gP, fin, fout);
 }
 
 template void launch_opgemm_conv<float>(hipStream_t, const Ctx&, int, const OpPlan&, const float*, const float*,

@@ -3,8 +3,11 @@
 Tolerances (relative to max |oracle|):
   * FP64 objective F(x):                1e-11  (same arithmetic, different summation order)
   * FP32 objective F(x):                5e-5   (single-precision convolutions and propagators)
-  * converged fields / Omega, FP64:     1e-7 abs / 1e-8 rel (both solved to ||F(x)-x||_inf < 1e-8)
-  * converged Omega, mixed precision:   1e-8   (the benchmark's acceptance bound)
+  * converged fields / Omega, FP64:     1e-7 abs / 1e-8 relative to max(1, |Omega|) (both solved to
+                                        ||F(x)-x||_inf < 1e-8, an absolute bound on O(1) fields, so Omega
+                                        agrees to ~1e-8 absolute where |Omega| < 1, e.g. the brush)
+  * converged Omega, mixed precision:   the same (the benchmark's acceptance bound, 1e-8 relative, on
+                                        its |Omega| ~ 10-20 state points)
   * degree of ionization / CVT:         1e-8 absolute
 """
 import numpy as np
@@ -85,7 +88,7 @@ def test_solve_matches_oracle(name):
     assert err[0] < num["tole"]
     # both sides stop at ||F(x)-x||_inf < 1e-8, so converged fields agree to ~that and Omega to ~1e-9
     assert np.max(np.abs(x - r.x)) < 1e-7
-    assert abs(om[0] - om_ref) / abs(om_ref) < 1e-8
+    assert abs(om[0] - om_ref) < 1e-8 * max(1.0, abs(om_ref))
     fr = o.state_fractions()
     for u, f in enumerate(fr):
         if f is not None:
@@ -96,7 +99,7 @@ def test_solve_matches_oracle(name):
                             switch_tole=num["switch_tole"], anderm=int(num["anderm"]), max_iters=3000,
                             precision=abi.PREC_MIXED)
     om2, _, _ = b.observables()
-    assert abs(om2[0] - om_ref) / abs(om_ref) < 1e-8
+    assert abs(om2[0] - om_ref) < 1e-8 * max(1.0, abs(om_ref))
     b.close()
 
 
