@@ -74,6 +74,10 @@ extern "C" {
 #define SDFT_FIELD_X 3           /* current solution vector (n_p) */
 #define SDFT_FIELD_FX 4          /* last F(x) (n_p) */
 #define SDFT_FIELD_SEGMENT_SUMS 5 /* per species/segment/state: sum_K rho_seg (raw), then sum_K rho_seg*trapez*dz */
+/* species u's segment density profiles rho_K.segments[u] (accumulate_densities.jl:95-184), layout
+ * (NP, S_u, N_u) with S_u the species' states per segment (0 for absent states): field id
+ * SDFT_FIELD_RHO_SEGMENTS + u */
+#define SDFT_FIELD_RHO_SEGMENTS 16
 
 typedef struct sdft_ctx sdft_ctx_t;
 

@@ -25,6 +25,7 @@ ERRORS = {-1: "SDFT_E_ARG", -2: "SDFT_E_HIP", -3: "SDFT_E_NORMALIZATION", -4: "S
 F_MFMT, F_TPT1MFMT, F_MSA, F_TPT1MSA = 1, 2, 4, 8
 PREC_FP64, PREC_FP32, PREC_MIXED = 64, 32, 3264
 FIELD_RHO_BEADS, FIELD_RHO_BONDS, FIELD_X, FIELD_FX = 1, 2, 3, 4
+FIELD_RHO_SEGMENTS = 16  # + species index: rho_K.segments[u], (NP, S_u, N_u)
 
 EXPORTED = ("sdft_abi_version", "sdft_last_error", "sdft_create", "sdft_destroy", "sdft_solution_length",
             "sdft_solution_offsets", "sdft_objective", "sdft_solve", "sdft_observables", "sdft_get_field",
@@ -225,6 +226,8 @@ class Batch:
         lib.sdft_solution_offsets(h, _ptr(offs, C.c_int64))
         self.offsets = offs
         self.S_max = int(max(nst))
+        self.species_states = [int(v) for v in nst]
+        self.species_segments = [int(v) for v in nseg]
 
     # ------------------------------------------------------------------
     def _check(self, rc: int):
@@ -315,3 +318,15 @@ class Batch:
         if field_id == FIELD_RHO_BONDS:
             return out.reshape((sp.NP, 2, self.MB), order="F")
         return out
+
+    def segment_profile(self, species: int, point: int) -> np.ndarray:
+        """Segment densities of one species at one point, (NP, S_u, N_u) with S_u the species'
+        states per segment (rho_K.segments, accumulate_densities.jl:95-184)."""
+        if not 0 <= species < self.nsp:
+            raise ValueError(f"species index {species} out of range")
+        sp = self.points[point]
+        S, N = self.species_states[species], self.species_segments[species]
+        out = np.zeros(sp.NP * S * N)
+        self._check(self.lib.sdft_get_field(self.h, FIELD_RHO_SEGMENTS + species, point,
+                                            _ptr(out, C.c_double), out.size))
+        return out.reshape((sp.NP, S, N), order="F")

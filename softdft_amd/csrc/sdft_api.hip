@@ -41,6 +41,7 @@ struct sdft_ctx {
     std::vector<double> valences;
     double lB = 0.0;
     int fwd_terms = 0, back_terms = 0;   // convolution terms per point (algorithmic flop count)
+    double* d_om_blocks = nullptr;       // per-point Omega block partials, allocated on first use
 };
 
 static thread_local std::string g_last_error;
@@ -222,9 +223,7 @@ static std::vector<OpTile> xcd_order(const std::vector<OpTile>& tiles) {
     out.reserve(mx * NX);
     for (size_t k = 0; k < mx; ++k)
         for (int x = 0; x < NX; ++x) out.push_back(k < by[x].size() ? by[x][k] : OpTile{0, 0, 0, 0});
-    // drop trailing empty tiles
-    while (!out.empty() && out.back().ncols == 0) out.pop_back();
-    return out;
+    return out;  // a multiple of 8 long: split launches (ID = tile + ntiles * part) keep parts on one XCD
 }
 
 static int build_conv_plan(sdft_ctx* ctx, const std::vector<ConvJob>& jobs, int job_base, int njobs,
@@ -264,7 +263,7 @@ static int build_conv_plan(sdft_ctx* ctx, const std::vector<ConvJob>& jobs, int 
     size_t tot = 0;
     for (auto& g : groups) tot += g.second.size();
     plan.wide = groups.empty() ? true : (tot / groups.size() >= 48);
-    const int TM = opgemm_tile_rows(plan.wide), TN = opgemm_tile_cols(plan.wide);
+    const int TM = conv_tile_rows(plan.wide), TN = opgemm_tile_cols(plan.wide);
     std::vector<OpGroup> gv;
     std::vector<OpCol> cols;
     std::vector<OpTile> tiles;
@@ -1002,7 +1001,14 @@ extern "C" int32_t sdft_get_field(sdft_ctx_t* ctx, int32_t field_id, int32_t poi
         case SDFT_FIELD_RHO_BEADS: need = (int64_t)P.NP * c.NB; break;
         case SDFT_FIELD_RHO_BONDS: need = (int64_t)P.NP * 2 * c.MB; break;
         case SDFT_FIELD_X: case SDFT_FIELD_FX: need = P.n; break;
-        default: set_error(ctx, SDFT_E_ARG, "unknown field id %d", field_id); return SDFT_E_ARG;
+        default:
+            if (field_id >= SDFT_FIELD_RHO_SEGMENTS && field_id < SDFT_FIELD_RHO_SEGMENTS + c.nsp) {
+                const SpeciesHost& sh = c.species[field_id - SDFT_FIELD_RHO_SEGMENTS];
+                need = (int64_t)P.NP * sh.S * sh.nseg;
+                break;
+            }
+            set_error(ctx, SDFT_E_ARG, "unknown field id %d", field_id);
+            return SDFT_E_ARG;
     }
     if (capacity < need) {
         set_error(ctx, SDFT_E_ARG, "capacity %lld < %lld", (long long)capacity, (long long)need);
@@ -1013,6 +1019,22 @@ extern "C" int32_t sdft_get_field(sdft_ctx_t* ctx, int32_t field_id, int32_t poi
         return SDFT_E_STATE;
     }
     const int prec = point_prec(ctx, point);
+    if (field_id >= SDFT_FIELD_RHO_SEGMENTS) {
+        double* d = nullptr;
+        HIP_CHECK_RET(hipMalloc(&d, need * sizeof(double)));
+        const int u = field_id - SDFT_FIELD_RHO_SEGMENTS;
+        if (prec == 64) ctx->p64.segment_profile(ctx->stream, c, u, point, d);
+        else ctx->p32.segment_profile(ctx->stream, c, u, point, d);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(out, d, need * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        (void)hipFree(d);
+        if (e != hipSuccess) {
+            set_error(ctx, SDFT_E_HIP, "segment profile: %s", hipGetErrorString(e));
+            return SDFT_E_HIP;
+        }
+        return SDFT_OK;
+    }
     if (field_id == SDFT_FIELD_X) return download_field(ctx, c.d_x + P.xoff, need, out);
     if (field_id == SDFT_FIELD_FX) return download_field(ctx, c.d_fx + P.xoff, need, out);
     if (field_id == SDFT_FIELD_RHO_BEADS)
@@ -1030,8 +1052,8 @@ extern "C" int32_t sdft_observables(sdft_ctx_t* ctx, double* omega, double* alph
     }
     Ctx& c = ctx->c;
     const int P = c.P, NB = c.NB;
-    double* d_om = nullptr;
-    TRY(dalloc(ctx, &d_om, (int64_t)P * SDFT_OMEGA_BLOCKS));
+    if (!ctx->d_om_blocks) TRY(dalloc(ctx, &ctx->d_om_blocks, (int64_t)P * SDFT_OMEGA_BLOCKS));
+    double* d_om = ctx->d_om_blocks;
     ctx->p32.omega(ctx->stream, c, c.d_x, d_om);
     ctx->p64.omega(ctx->stream, c, c.d_x, d_om);
     std::vector<double> omp((size_t)P * SDFT_OMEGA_BLOCKS), om(P, 0.0), accs((size_t)P * 2 * SDFT_MAX_SPECIES);

@@ -1465,6 +1465,54 @@ __global__ void __launch_bounds__(256) k_segment_sums(const DevSystem* __restric
     }
 }
 
+// segment density profiles of species u at point p (rho_K.segments, accumulate_densities.jl:95-184):
+// out[n + NP * (a + S * q)] = norm e^{mu} e^{-lambda} Gp Qc, 0 for absent states
+template <typename T>
+__global__ void __launch_bounds__(256) k_segment_profile(const DevSystem* __restrict__ psys,
+                                                        const DevPoint* __restrict__ pts,
+                                                        const DevSpecies* __restrict__ spec,
+                                                        const int64_t* __restrict__ goffs, int GF,
+                                                        const int32_t* __restrict__ seg_states,
+                                                        const double* __restrict__ maxd_tab,
+                                                        const T* __restrict__ elam, const T* __restrict__ gC,
+                                                        const T* __restrict__ gP,
+                                                        const double* __restrict__ fixed_lambda, int u, int p,
+                                                        double* __restrict__ out) {
+    const DevSystem& sys = *psys;
+    const DevPoint& P = pts[p];
+    const DevSpecies sp = spec[u];
+    const int q = blockIdx.y;
+    const int S = sp.S, N = sp.nseg, NP = P.NP, NB = sys.NB;
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= NP) return;
+    const int* st = seg_states + sp.state_off;
+    const double* maxd = maxd_tab + sp.maxd_off;
+    const int Kf = sp.fixed_seg >= 0 ? P.fixed_coord[u] : -1;
+    const double norm = sp.fixed_seg >= 0 ? sp.fixed_density / fixed_lambda[(int64_t)p * SDFT_MAX_SPECIES + u]
+                                          : P.chain_norm[u];
+    const double pref = norm * exp(P.mu_species[u]);
+    auto gidx = [&](int seg, int b, int a) -> int64_t {
+        return P.base * (int64_t)GF + (goffs[u] + ((int64_t)seg * S + b) * S + a) * NP + n;
+    };
+    for (int a = 0; a < S; ++a) {
+        const int si = st[q * S + a];
+        double v = 0.0;
+        if (si >= 0) {
+            double Gp = 0.0, Qc = 0.0;
+            if (q == 0) Gp = (double)base_value<T>(0, n, sp.fixed_seg, Kf, P.dz, maxd);
+            else
+                for (int b = 0; b < S; ++b)
+                    if (st[(q - 1) * S + b] >= 0) Gp += (double)gP[gidx(q, b, a)];
+            if (q == N - 1) Qc = (double)base_value<T>(q, n, sp.fixed_seg, Kf, P.dz, maxd);
+            else
+                for (int b = 0; b < S; ++b)
+                    if (st[(q + 1) * S + b] >= 0) Qc += (double)gC[gidx(q, b, a)];
+            v = pref * (double)elam[fidx(P, NB, si, n)] * Gp * Qc;
+        }
+        out[n + (int64_t)NP * (a + (int64_t)S * q)] = v;
+    }
+}
+
 __global__ void k_set_evalflag(int32_t* evalflag, int value, int P_count) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < P_count) evalflag[p] = value;
@@ -1626,6 +1674,14 @@ void Pipeline<T>::segment_sums(hipStream_t st, const Ctx& c, int u, double* out,
     const SpeciesHost& sh = c.species[u];
     hipLaunchKernelGGL(k_segment_sums<T>, dim3(sh.nseg, c.P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_state, c.d_spec,
                        c.d_goffs, c.GF, c.d_seg_states, c.d_maxd, elam, gC, gP, c.d_fixed_lambda, u, out, stride);
+}
+
+template <typename T>
+void Pipeline<T>::segment_profile(hipStream_t st, const Ctx& c, int u, int p, double* out) {
+    const SpeciesHost& sh = c.species[u];
+    hipLaunchKernelGGL(k_segment_profile<T>, dim3((c.maxNP + 255) / 256, sh.nseg), dim3(256), 0, st, c.d_sys,
+                       c.d_pts, c.d_spec, c.d_goffs, c.GF, c.d_seg_states, c.d_maxd, elam, gC, gP,
+                       c.d_fixed_lambda, u, p, out);
 }
 
 template struct Pipeline<float>;

@@ -143,6 +143,7 @@ struct Pipeline {
     void objective(hipStream_t st, const Ctx& c, int prec, const double* x, double* fx);
     void omega(hipStream_t st, const Ctx& c, const double* x, double* omega_acc);
     void segment_sums(hipStream_t st, const Ctx& c, int u, double* out, int64_t stride);
+    void segment_profile(hipStream_t st, const Ctx& c, int u, int p, double* out);
     // single-launch entry points used by sdft_kernel_profile
     void launch_prop_step(hipStream_t st, const Ctx& c, int prec, int u, int s, bool banded);
     void launch_conv(hipStream_t st, const Ctx& c, int prec, bool back, double* fx);
@@ -163,6 +164,7 @@ void launch_opgemm_prop(hipStream_t st, const Ctx& c, int prec, const OpPlan& pl
                         bool banded, const T* kern, const T* elam, const T* hl, const T* trapez, T* gC, T* gP,
                         T* fbuf);
 int opgemm_tile_rows(bool wide);
+int conv_tile_rows(bool wide);   // rows per tile of the convolution GEMMs
 int opgemm_tile_cols(bool wide);
 
 void launch_build_kernels(hipStream_t st, const DevSystem* sys, int K, int nsets, int maxL, const int32_t* set_L,

@@ -368,7 +368,12 @@ void launch_compact_cols(hipStream_t st, const Ctx& c, int prec) {
 // load reads 64 consecutive inputs of one column, and both the staging writes and the
 // B reads are free of bank conflicts.
 // --------------------------------------------------------------------------
-template <typename T, int WN>
+constexpr int CONV_WM = 1;  // plan tile rows = 64 * CONV_WM (2 measured slower on MI355X: fewer workgroups)
+
+// A plan tile of 64 * WM * RSPLIT rows is computed by RSPLIT workgroups of 64 * WM rows
+// (blockIdx.x = tile + ntiles * part). With CONV_WM > 1, FP32 takes whole tiles and FP64 splits
+// them (WM = 1: its accumulators and operands already fill the register file).
+template <typename T, int WM, int WN, int RSPLIT>
 __global__ void __launch_bounds__(256) k_convgemm(const DevPoint* __restrict__ pts,
                                                   const int32_t* __restrict__ evalflag, int prec,
                                                   const OpTile* __restrict__ tiles, const OpGroup* __restrict__ groups,
@@ -378,15 +383,20 @@ __global__ void __launch_bounds__(256) k_convgemm(const DevPoint* __restrict__ p
                                                   int Frt, const T* __restrict__ dphi, int Fdp, T* __restrict__ nw,
                                                   int Fnw, double* __restrict__ fx, double* __restrict__ wsum,
                                                   int Fws) {
-    constexpr int TM = 64, TN = WN * 16, KC = 64;
+    constexpr int TM = 64 * WM, TN = WN * 16, KC = 64;
     constexpr int XK = KC + 2;             // xs row (one column's KC inputs); the pad of 2 puts the B reads
                                            // of lanes (q, g) on banks 2q + g: conflict-free
     constexpr int PER = KC * TN / 256;     // staged inputs per thread and chunk
     constexpr int CWN = TM + KC - 1;       // operator window length
     static_assert(KC == 64, "one wavefront stages one column chunk");
-    const OpTile tile = tiles[blockIdx.x];
+    static_assert(CWN <= 256, "one window element per thread");
+    const int ntiles = gridDim.x / RSPLIT;
+    const int part = blockIdx.x / ntiles;
+    const OpTile tile = tiles[blockIdx.x - part * ntiles];
     const OpGroup* __restrict__ Gp = groups + tile.group;  // read fields in place (dynamic term index)
     const int NP = Gp->NP, L = Gp->L;
+    const int r0 = tile.row0 + part * TM;
+    if (r0 >= NP) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 
     __shared__ T xs[TN * XK];              // [column][k]
@@ -428,7 +438,6 @@ __global__ void __launch_bounds__(256) k_convgemm(const DevPoint* __restrict__ p
     }
     __syncthreads();
 
-    const int r0 = tile.row0;
     const T* cset = kern + Gp->kbase * (int64_t)Ksys;
     const int nk = (NP + KC - 1) / KC;
     const int nchunks = Gp->nterms * nk;
@@ -436,7 +445,7 @@ __global__ void __launch_bounds__(256) k_convgemm(const DevPoint* __restrict__ p
     // staging: wavefront w loads columns w, w + 4, ... (PER of them), lane = k within the chunk,
     // so every load instruction reads 64 consecutive inputs of one column
     T xr[PER];
-    T cr = T(0);
+    T crm = T(0), crp = T(0);
     auto fetch = [&](int ch) {
         const int term = ch / nk, k0 = (ch - term * nk) * KC;
         const int nn = k0 + lane;
@@ -445,49 +454,58 @@ __global__ void __launch_bounds__(256) k_convgemm(const DevPoint* __restrict__ p
             const T* in = colin[term][4 * q + wave];
             xr[q] = (in != nullptr && nn < NP) ? in[nn] : T(0);
         }
-        const T* c = cset + (int64_t)Gp->kern[term] * L;
         if (tid < CWN) {
+            const T* c = cset + (int64_t)Gp->kern[term] * L;
             const T sgn = Gp->sign[term] < 0 ? T(-1) : T(1);
-            cr = sgn * c[wrapLg(r0 - k0 - (KC - 1) + tid, L)];
-        } else if (tid >= 128 && tid < 128 + CWN) {
-            cr = c[wrapLg(r0 + k0 + 1 + (tid - 128), L)];
+            crm = sgn * c[wrapLg(r0 - k0 - (KC - 1) + tid, L)];
+            crp = c[wrapLg(r0 + k0 + 1 + tid, L)];
         }
     };
 
-    typename Mfma<T>::acc_t acc[WN];
+    typename Mfma<T>::acc_t acc[WM][WN];
 #pragma unroll
-    for (int j = 0; j < WN; ++j)
-        for (int v = 0; v < 4; ++v) acc[j][v] = T(0);
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j)
+            for (int v = 0; v < 4; ++v) acc[i][j][v] = T(0);
 
     fetch(0);
-    const int rl = wave * 16 + (lane & 15);
+    const int rw = wave * 16 * WM + (lane & 15);  // row of this lane in row block 0
     for (int ch = 0; ch < nchunks; ++ch) {
         __syncthreads();  // the previous chunk's products are done with xs / windows
 #pragma unroll
         for (int q = 0; q < PER; ++q) xs[(4 * q + wave) * XK + lane] = xr[q];
-        if (tid < CWN) cmw[tid] = cr;
-        else if (tid >= 128 && tid < 128 + CWN) cpw[tid - 128] = cr;
+        if (tid < CWN) {
+            cmw[tid] = crm;
+            cpw[tid] = crp;
+        }
         __syncthreads();
         if (ch + 1 < nchunks) fetch(ch + 1);  // in flight during the products below
         // operands of k-step pairs in ping-pong registers: the LDS reads of the next pair
         // are in flight while the current pair runs on the matrix core
-        T cm0[2], cp0[2], bv0[2][WN], cm1[2], cp1[2], bv1[2][WN];
-        auto lds_fetch = [&](int kp, T (&cm)[2], T (&cp)[2], T (&bv)[2][WN]) {
+        T cm0[2][WM], cp0[2][WM], bv0[2][WN], cm1[2][WM], cp1[2][WM], bv1[2][WN];
+        auto lds_fetch = [&](int kp, T (&cm)[2][WM], T (&cp)[2][WM], T (&bv)[2][WN]) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int kl = (2 * kp + h) * 4 + (lane >> 4);
-                cm[h] = cmw[rl - kl + KC - 1];
-                cp[h] = cpw[rl + kl];
+#pragma unroll
+                for (int i = 0; i < WM; ++i) {
+                    cm[h][i] = cmw[rw + 16 * i - kl + KC - 1];
+                    cp[h][i] = cpw[rw + 16 * i + kl];
+                }
 #pragma unroll
                 for (int j = 0; j < WN; ++j) bv[h][j] = xs[(j * 16 + (lane & 15)) * XK + kl];
             }
         };
-        auto products = [&](T (&cm)[2], T (&cp)[2], T (&bv)[2][WN]) {
+        auto products = [&](T (&cm)[2][WM], T (&cp)[2][WM], T (&bv)[2][WN]) {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const T av = cm[h] + cp[h];
 #pragma unroll
-                for (int j = 0; j < WN; ++j) acc[j] = Mfma<T>::mma(av, bv[h][j], acc[j]);
+                for (int i = 0; i < WM; ++i) {
+                    const T av = cm[h][i] + cp[h][i];
+#pragma unroll
+                    for (int j = 0; j < WN; ++j) acc[i][j] = Mfma<T>::mma(av, bv[h][j], acc[i][j]);
+                }
             }
         };
         lds_fetch(0, cm0, cp0, bv0);
@@ -512,10 +530,13 @@ __global__ void __launch_bounds__(256) k_convgemm(const DevPoint* __restrict__ p
         const DevPoint& P = pts[p];
         const ConvJob& J = jobs[coljob[cl]];
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int n = r0 + wave * 16 + Mfma<T>::row(lane, v);
-            if (n >= NP) continue;
-            conv_store<T>(J, P, n, (double)acc[j][v], nw, Fnw, fx, wsum, Fws);
+        for (int i = 0; i < WM; ++i) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int n = r0 + wave * 16 * WM + 16 * i + Mfma<T>::row(lane, v);
+                if (n >= NP) continue;
+                conv_store<T>(J, P, n, (double)acc[i][j][v], nw, Fnw, fx, wsum, Fws);
+            }
         }
     }
 }
@@ -529,18 +550,20 @@ void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& pl
     if (plan.ntiles == 0) return;
     PropStep ps;
     memset(&ps, 0, sizeof(ps));
-    if (plan.wide)
-        hipLaunchKernelGGL((k_convgemm<T, 4>), dim3(plan.ntiles), dim3(256), 0, st, c.d_pts, c.d_evalflag, prec,
-                           (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups,
+    if (plan.wide) {
+        constexpr int WM = sizeof(T) == 4 ? CONV_WM : 1, RS = CONV_WM / WM;
+        hipLaunchKernelGGL((k_convgemm<T, WM, 4, RS>), dim3(plan.ntiles * RS), dim3(256), 0, st, c.d_pts,
+                           c.d_evalflag, prec, (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups,
                            (const OpCol*)plan.d_cols_act[prec == 64 ? 1 : 0], plan.d_count[prec == 64 ? 1 : 0],
                            c.d_jobs, kern, c.K, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx, c.d_wsum, c.Fws);
-    else
+    } else {
         hipLaunchKernelGGL((k_opgemm<T, 2, 1, false>), dim3(plan.ntiles), dim3(256), 0, st, c.d_sys, c.d_pts,
                            c.d_evalflag, prec, (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups, plan.d_cols,
                            c.d_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx, c.d_wsum, c.Fws, ps,
                            c.d_seg_states, c.d_maxd,
                            (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (T*)nullptr, (T*)nullptr,
                            (const T*)nullptr, (T*)nullptr);
+    }
 }
 
 template <typename T>
@@ -582,6 +605,7 @@ template void launch_opgemm_prop<double>(hipStream_t, const Ctx&, int, const OpP
 
 // row/column tile sizes of the two shapes (for the host planner)
 int opgemm_tile_rows(bool wide) { return wide ? TileShape<1, 4>::TM : TileShape<2, 1>::TM; }
+int conv_tile_rows(bool wide) { return wide ? 64 * CONV_WM : TileShape<2, 1>::TM; }
 int opgemm_tile_cols(bool wide) { return wide ? TileShape<1, 4>::TN : TileShape<2, 1>::TN; }
 
 }  // namespace sdft

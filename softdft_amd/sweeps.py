@@ -1,0 +1,129 @@
+"""Continuation sweeps on top of the batched solver.
+
+The reference's slit-width continuation (routines/adjust_distanceZ.jl:15-155) steps the wall
+separation H, warm-starts each width from the previous (narrower) converged system and records per
+width the contact pressure P (contact_value_theorem, first wall), the disjoining pressure
+Pi = P - P_bulk and the energy Omega + P_bulk * H (adjust_distanceZ.jl:95-110); its summary file
+(adjust_distanceZ.jl:138-148) lists every step against the last one.
+
+The reference driver is stale: it calls eval_inhomogeneous(dft_system, 1), a method that does not
+exist (src/iDFT/evaluation/eval_inhomogeneous.jl:4 defines only the one-argument form), after
+copying the previous densities into the new system. Here the warm start is done on the solution
+vector itself: the excess fields mu_ex, lng and Psi of the new width take the previous width's
+values on the left half (1-based index k < round(NP_new / 2) + 2, Julia round-half-even) and the
+mirror image on the right half, which is the reference's copy rule (adjust_distanceZ.jl:73-88), and
+PsiC is copied (adjust_distanceZ.jl:90).
+
+Batching: width i is warm-started from width i - 1. The widths are solved in `waves` rounds: round r
+holds every width i with i % waves == r, so each round is one batched solve and every width except
+those of round 0 (cold, from the bulk guess) starts from its immediate narrower neighbour, as in
+the reference.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+from . import abi
+from .problem import StatePoint, build_state_point, flatten, unflatten
+
+
+def julia_round(x: float) -> int:
+    """Julia's round(Int, x): nearest integer, ties to even (Python's round does the same)."""
+    return int(round(x))
+
+
+def remap_solution(sp_old: StatePoint, x_old: np.ndarray, sp_new: StatePoint) -> np.ndarray:
+    """Warm start of width `sp_new` from the converged `x_old` of width `sp_old`
+    (adjust_distanceZ.jl:64-90 applied to the solution vector)."""
+    mu_o, lng_o, psi_o, psic_o = unflatten(x_old, sp_old.NP, sp_old.NB, sp_old.MB)
+    mu, lng, psi, _ = unflatten(sp_new.x0(), sp_new.NP, sp_new.NB, sp_new.MB)
+    mu, lng, psi = mu.copy(), lng.copy(), psi.copy()
+    NP = sp_new.NP
+    last = julia_round(NP / 2) + 2  # 1-based k < last
+    for k in range(1, last):
+        if k > sp_old.NP:
+            raise ValueError(f"width step too large: the old grid (NP={sp_old.NP}) has no index {k}")
+        k0, k1 = k - 1, NP - k  # 0-based index and its mirror (NP - k + 1 in 1-based)
+        mu[k0] = mu_o[k0]
+        mu[k1] = mu_o[k0]
+        lng[k0] = lng_o[k0]
+        lng[k1] = lng_o[k0]
+        psi[k0] = psi_o[k0]
+        psi[k1] = psi_o[k0]
+    return flatten(mu, lng, psi, float(psic_o))
+
+
+@dataclass
+class WidthResult:
+    step: int            # 1-based sweep index
+    distance: float      # H
+    pressure: float      # contact-value pressure at the first wall
+    delta_p: float       # Pi(H) = P - P_bulk
+    energy: float        # Omega + P_bulk * H
+    omega: float
+    iterations: int
+    warm: bool
+
+
+def width_sweep(datasets: Sequence[dict], *, waves: int = 8, precision: int = abi.PREC_MIXED,
+                numerics: Optional[dict] = None, max_iters: int = 3000,
+                on_round: Optional[Callable[[int, float], None]] = None) -> List[WidthResult]:
+    """Solve the slit widths of `datasets` (increasing H, one chemistry) with neighbour warm starts.
+
+    Returns one WidthResult per width, in sweep order."""
+    sps = [build_state_point(d) for d in datasets]
+    n = len(sps)
+    num = dict(numerics or sps[0].numerics)
+    opts = dict(tole=num["tole"], damping=num["damping"], mixing_max=num["mixing_max"],
+                switch_tole=num["switch_tole"], anderm=int(num["anderm"]), max_iters=max_iters,
+                precision=precision)
+    xs: List[Optional[np.ndarray]] = [None] * n
+    res: List[Optional[WidthResult]] = [None] * n
+    waves = max(1, min(waves, n))
+    for r in range(waves):
+        idx = list(range(r, n, waves))
+        if not idx:
+            continue
+        batch = abi.Batch([sps[i] for i in idx])
+        x0 = []
+        for i in idx:
+            if r > 0:
+                x0.append(remap_solution(sps[i - 1], xs[i - 1], sps[i]))
+            else:
+                x0.append(sps[i].x0())
+        x, it, err = batch.solve(np.concatenate(x0), **opts)
+        om, _, pc = batch.observables()
+        parts = batch.split(x)
+        for k, i in enumerate(idx):
+            xs[i] = parts[k].copy()
+            sp = sps[i]
+            p_bulk = float(sp.bulk.pressure)
+            res[i] = WidthResult(step=i + 1, distance=float(sp.H), pressure=float(pc[k, 0]),
+                                 delta_p=float(pc[k, 0] - p_bulk), energy=float(om[k] + p_bulk * sp.H),
+                                 omega=float(om[k]), iterations=int(it[k]), warm=r > 0)
+        batch.close()
+        if on_round is not None:
+            on_round(r, float(np.max(err)))
+    return [r for r in res if r is not None]
+
+
+def write_sweep_summary(results: Sequence[WidthResult], path: str) -> None:
+    """adjust_distancesZ_results.csv (adjust_distanceZ.jl:138-148): every step against the last."""
+    final_e, final_dp = results[-1].energy, results[-1].delta_p
+    with open(path, "w") as f:
+        f.write("step,distance,Pressure,deltaP,energy,deltaE\n")
+        for r in results:
+            f.write(f"{r.step},{r.distance!r},{r.pressure!r},{r.delta_p - final_dp!r},{r.energy!r},"
+                    f"{r.energy - final_e!r}\n")
+
+
+def write_run_info(r: WidthResult, directory: str) -> None:
+    """Per-width run_info.csv (adjust_distanceZ.jl:112-116)."""
+    os.makedirs(directory, exist_ok=True)
+    with open(os.path.join(directory, "run_info.csv"), "w") as f:
+        f.write("step,distance,Pressure,deltaP,energy\n")
+        f.write(f"{r.step},{r.distance!r},{r.pressure!r},{r.delta_p!r},{r.energy!r}\n")

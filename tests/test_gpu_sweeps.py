@@ -1,0 +1,32 @@
+"""GPU: slit-width continuation with neighbour warm starts against independent cold solves."""
+import numpy as np
+import pytest
+
+from softdft_amd import abi, configs
+from softdft_amd.problem import build_state_point
+from softdft_amd.sweeps import width_sweep
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("precision", [abi.PREC_FP64, abi.PREC_MIXED])
+def test_width_sweep_matches_cold_solves(precision):
+    Hs = [4.0 + 0.05 * i for i in range(6)]
+    dss = [configs.weak_polyelectrolyte(H=H, dz=0.05, n_seg=4, solvent=0.2, pH=7.0) for H in Hs]
+    print(f"case width_sweep precision={precision}", flush=True)
+    res = width_sweep(dss, waves=3, precision=precision)
+    # round r of 3 solves widths i % 3 == r; round 0 starts cold, the others from width i - 1
+    assert [r.warm for r in res] == [i % 3 != 0 for i in range(6)]
+    sps = [build_state_point(d) for d in dss]
+    b = abi.Batch(sps)  # cold: every width from its bulk guess
+    num = sps[0].numerics
+    x, it, err = b.solve(b.x0(), tole=num["tole"], damping=num["damping"], mixing_max=num["mixing_max"],
+                         switch_tole=num["switch_tole"], anderm=int(num["anderm"]), max_iters=3000,
+                         precision=abi.PREC_FP64)
+    om, _, pc = b.observables()
+    for k, (r, sp) in enumerate(zip(res, sps)):
+        # the warm start changes the path, not the fixed point: both stop at ||F(x) - x|| < 1e-8
+        assert abs(r.omega - om[k]) < 1e-8 * max(1.0, abs(om[k]))
+        assert abs(r.delta_p - (pc[k, 0] - sp.bulk.pressure)) < 1e-7
+        assert abs(r.energy - (om[k] + sp.bulk.pressure * sp.H)) < 1e-7 * max(1.0, abs(r.energy))
+    b.close()
