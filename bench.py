@@ -69,7 +69,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--kernel-only", choices=["chain32", "chain64", "fwd32", "fwd64", "back32", "back64"],
-                    help="after one mixed solve, only re-launch this kernel class 20 times (for --pmc runs)")
+                    help="after one full-load evaluation, only re-launch this kernel class 21 times "
+                         "(for rocprofv3 --kernel-trace / --pmc runs of that kernel)")
     ap.add_argument("--solve-only", action="store_true",
                     help="stop after the timed mixed solves (for rocprofv3 --kernel-trace of the solve itself)")
     args = ap.parse_args()
@@ -93,7 +94,12 @@ def main():
     num = dict(tole=1e-8, damping=0.1, mixing_max=0.02, switch_tole=100.0, anderm=5, max_iters=3000)
 
     if args.kernel_only:
-        batch.solve(x0, precision=abi.PREC_MIXED, **num)
+        # one evaluation of every point in the class's precision with the banded operators, so that
+        # every launch of the kernel in this run has all 256 points active (the full-load launch the
+        # roofline figure times)
+        prec = abi.PREC_FP32 if args.kernel_only.endswith("32") else abi.PREC_FP64
+        batch.set_operator_band(1, 1)
+        batch.objective(x0, prec)
         kc = {"chain32": abi.KP_CHAIN_FP32, "chain64": abi.KP_CHAIN_FP64, "fwd32": abi.KP_CONV_FWD_FP32,
               "fwd64": abi.KP_CONV_FWD_FP64, "back32": abi.KP_CONV_BACK_FP32, "back64": abi.KP_CONV_BACK_FP64}
         ms, fl = batch.kernel_profile(kc[args.kernel_only], reps=20)

@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define SDFT_ABI_VERSION 1
+#define SDFT_ABI_VERSION 2
 
 /* error codes */
 #define SDFT_OK 0
@@ -58,6 +58,7 @@ extern "C" {
 #define SDFT_F_TPT1MFMT 2
 #define SDFT_F_MSA 4
 #define SDFT_F_TPT1MSA 8
+#define SDFT_F_SW 16      /* mean-field square well (square_well.jl:19-123); needs the sw_* arrays below */
 
 /* species models */
 #define SDFT_MODEL_FREELY_JOINTED 0
@@ -124,6 +125,12 @@ typedef struct sdft_desc {
     const int32_t* wall_lo;        /* [P][NB] contact index of the left wall */
     const int32_t* wall_hi;        /* [P][NB] */
     const double* wall_charge;     /* [P][2] or NULL */
+
+    /* square-well pairs (properties.pairs[:square_well], process_pairs.jl:42-95), used with SDFT_F_SW */
+    int32_t n_sw_pairs;
+    const int32_t* sw_pairs;       /* [n_sw_pairs][2] bead types, i <= j */
+    const double* sw_energy;       /* [n_sw_pairs] well depth eps_ij */
+    const double* sw_lambda;       /* [n_sw_pairs] well width multiplier lambda_ij */
 } sdft_desc_t;
 
 typedef struct sdft_solver_opts {

@@ -20,6 +20,8 @@ convention), following the reference files line by line, quirks included:
                           src/iLST/free_energy/excluded_volume/{mfmt,TPT1mfmt}.jl
   msa / TPT1msa           src/iDFT/functionals/electrostaticcorrelations/{msa,TPT1msa}.jl,
                           src/iLST/free_energy/electrostaticcorrelations/{msa,TPT1msa}.jl
+  square well             src/iDFT/functionals/short_range_interactions/square_well.jl:19-123
+                          (mu: real part of the k-space product, :120; fe: full IFFT, :88-95)
   eval_mu_functionals     src/iDFT/evaluation/objective/functions/eval_mu_functional.jl:1-52
   Poisson + neutrality    src/iDFT/coulomb/models/coulomb.jl:84-125, src/iDFT/coulomb/neutrality_shift.jl:1-99
   Anderson mixing         src/NonlinearSolvers/fixed_point/AndersonMixing.jl:56-169
@@ -299,6 +301,13 @@ class OracleDFT:
             self.wbond[:, u] = w_delta(k, sij) / (4.0 * PI * sij ** 2)
             self.wbond_filt[:, u] = self.wbond[:, u] * filt
         self.wcoul = w_coulomb(k, self.dz, ms.bjerrum)
+        # square well: step(lambda sigma_ij) - step(sigma_ij) per pair (square_well.jl:37-50)
+        self.wsw = None
+        if ms.sw is not None:
+            self.wsw = np.zeros((self.L, len(ms.sw.pairs)))
+            for u, (i, j) in enumerate(ms.sw.pairs):
+                sij = (sig[i] + sig[j]) / 2.0
+                self.wsw[:, u] = w_step(k, ms.sw.lambdas[u] * sij) - w_step(k, sij)
         self.surf_hat = np.fft.fft(sp.surf)
         self.mu_species = sp.mu_species.copy()
         self.fix_lambda = [1.0] * len(ms.species)
@@ -661,9 +670,23 @@ class OracleDFT:
                     self._add_lng_back(lngP, self.wmsa, lng_hat)
                 for j in range(NB):
                     mu_ex_hat[:, j] += np.fft.fft(d[:, j]) * self.wmsa[:, j]
+            elif fn == "sw":
+                # eval_mu_functional! (square_well.jl:98-123): the real part is taken in k-space
+                for i in ms.sw.species:
+                    mu_ex_hat[:, i] -= self._sw_hat(i).real
         mu = np.fft.ifft(mu_ex_hat, axis=0).real[:self.NP]
         lng = np.fft.ifft(lng_hat, axis=0).real[:self.NP]
         return mu, lng
+
+    def _sw_hat(self, i: int) -> np.ndarray:
+        """sum_j rho_hat_j w_ij eps_ij over the pairs of bead type i (square_well.jl:75-84, 107-116)."""
+        sw = self.ms.sw
+        f = np.zeros(self.L, complex)
+        for j in sw.species:
+            u = sw.index(i, j)
+            if u is not None:
+                f += self.rho_beads_hat[:, j] * self.wsw[:, u] * sw.energys[u]
+        return f
 
     # -------------------------------------------------------- Coulomb
     def _coulomb(self, PsiC_old):
@@ -795,6 +818,11 @@ class OracleDFT:
                     wtb = self._bond_weighted(self.wmsa)[:NP]
                     f = free_energy_tpt1msa_vec(wtb, ms.bond_types, rb, sig, Z, lB)
                 om += float(np.sum(f))
+            elif fn == "sw":
+                # eval_fe_functional (square_well.jl:64-95): full complex IFFT, then its real part
+                for i in ms.sw.species:
+                    f = np.fft.ifft(self._sw_hat(i)).real[:NP]
+                    om -= 0.5 * float(np.sum(self.rho_beads[:, i] * f))
         om -= float(np.sum(self.rho_beads * mu_ex * sp.trapez))
         om += float(np.sum(self.rho_bonds * lng)) / 2.0
         om *= self.dz

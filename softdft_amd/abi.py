@@ -18,11 +18,11 @@ from .problem import StatePoint
 LIB_NAME = "libsoftdft_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
-SDFT_ABI_VERSION = 1
+SDFT_ABI_VERSION = 2
 SDFT_OK = 0
 ERRORS = {-1: "SDFT_E_ARG", -2: "SDFT_E_HIP", -3: "SDFT_E_NORMALIZATION", -4: "SDFT_E_NOCONV",
           -5: "SDFT_E_UNSUPPORTED", -6: "SDFT_E_STATE"}
-F_MFMT, F_TPT1MFMT, F_MSA, F_TPT1MSA = 1, 2, 4, 8
+F_MFMT, F_TPT1MFMT, F_MSA, F_TPT1MSA, F_SW = 1, 2, 4, 8, 16
 PREC_FP64, PREC_FP32, PREC_MIXED = 64, 32, 3264
 FIELD_RHO_BEADS, FIELD_RHO_BONDS, FIELD_X, FIELD_FX = 1, 2, 3, 4
 FIELD_RHO_SEGMENTS = 16  # + species index: rho_K.segments[u], (NP, S_u, N_u)
@@ -48,7 +48,8 @@ class SdftDesc(C.Structure):
                 ("NP", _i32p), ("dz", _f64p), ("delta_muH", _f64p), ("mu_species", _f64p),
                 ("total_charge", _f64p), ("fixed_coord", _i32p),
                 ("Ext", _f64p), ("trapez", _f64p), ("surf", _f64p),
-                ("hard_walls", C.c_int32), ("wall_lo", _i32p), ("wall_hi", _i32p), ("wall_charge", _f64p)]
+                ("hard_walls", C.c_int32), ("wall_lo", _i32p), ("wall_hi", _i32p), ("wall_charge", _f64p),
+                ("n_sw_pairs", C.c_int32), ("sw_pairs", _i32p), ("sw_energy", _f64p), ("sw_lambda", _f64p)]
 
 
 class SdftSolverOpts(C.Structure):
@@ -114,7 +115,7 @@ def _ptr(a: np.ndarray, ctype):
 def functional_flags(names: Sequence[str]) -> int:
     f = 0
     for n in names:
-        f |= {"mfmt": F_MFMT, "TPT1mfmt": F_TPT1MFMT, "msa": F_MSA, "TPT1msa": F_TPT1MSA}[n]
+        f |= {"mfmt": F_MFMT, "TPT1mfmt": F_TPT1MFMT, "msa": F_MSA, "TPT1msa": F_TPT1MSA, "sw": F_SW}[n]
     return f
 
 
@@ -133,7 +134,8 @@ class Batch:
             m2 = sp.ms
             if (m2.monomers != ms.monomers or m2.bond_types != ms.bond_types or len(m2.species) != len(ms.species)
                     or not np.array_equal(m2.diameters, ms.diameters) or not np.array_equal(m2.valences, ms.valences)
-                    or sp.functionals != p0.functionals or abs(m2.bjerrum - ms.bjerrum) > 0):
+                    or sp.functionals != p0.functionals or abs(m2.bjerrum - ms.bjerrum) > 0
+                    or m2.sw != ms.sw):
                 raise ValueError("state points in one batch must share the molecular-system layout")
         self.NB, self.MB, self.nsp = ms.NB, ms.MB, len(ms.species)
         P = len(points)
@@ -215,6 +217,14 @@ class Batch:
         d.wall_lo = _ptr(keep["lo"], C.c_int32)
         d.wall_hi = _ptr(keep["hi"], C.c_int32)
         d.wall_charge = _ptr(keep["wq"], C.c_double) if has_q else None
+        if ms.sw is not None and "sw" in p0.functionals:
+            keep["sw_pairs"] = np.ascontiguousarray(np.array(ms.sw.pairs, dtype=np.int32).reshape(-1))
+            keep["sw_eps"] = np.ascontiguousarray(np.array(ms.sw.energys[:len(ms.sw.pairs)], dtype=np.float64))
+            keep["sw_lam"] = np.ascontiguousarray(np.array(ms.sw.lambdas[:len(ms.sw.pairs)], dtype=np.float64))
+            d.n_sw_pairs = len(ms.sw.pairs)
+            d.sw_pairs = _ptr(keep["sw_pairs"], C.c_int32)
+            d.sw_energy = _ptr(keep["sw_eps"], C.c_double)
+            d.sw_lambda = _ptr(keep["sw_lam"], C.c_double)
         self._desc = d
         h = C.c_void_p()
         rc = lib.sdft_create(C.byref(d), C.byref(h))

@@ -148,6 +148,39 @@ def free_energy_tpt1msa(rho_bonds, bond_types, rho, sig, Z, lB):
     return f
 
 
+def sw_volumes(ms: MolSys) -> np.ndarray:
+    """V_ij = 4 pi / 3 sigma_ij^3 (lambda_ij^3 - 1) per square-well pair
+    (src/iLST/free_energy/short_range_interactions/square_well.jl:73-74)."""
+    sw, sig = ms.sw, ms.diameters
+    return np.array([4.0 * PI / 3.0 * ((sig[i] + sig[j]) / 2.0) ** 3 * (lam ** 3 - 1.0)
+                     for (i, j), lam in zip(sw.pairs, sw.lambdas)])
+
+
+def free_energy_sw(ms: MolSys, rho) -> float:
+    """Mean-field square-well free energy density (square_well.jl:64-81): the double loop runs over
+    the participating bead types, so each unlike pair enters twice with the factor 1/2."""
+    sw, V = ms.sw, sw_volumes(ms)
+    f = 0.0
+    for i in sw.species:
+        for j in sw.species:
+            k = sw.index(i, j)
+            if k is not None:
+                f -= 0.5 * rho[i] * rho[j] * sw.energys[k] * V[k]
+    return f
+
+
+def sw_deriv(ms: MolSys, rho) -> np.ndarray:
+    """d f_sw / d rho_i (compute_sw_deriv!, square_well.jl:129-150)."""
+    sw, V = ms.sw, sw_volumes(ms)
+    d = np.zeros(ms.NB)
+    for i in sw.species:
+        for j in sw.species:
+            k = sw.index(i, j)
+            if k is not None:
+                d[i] -= rho[j] * sw.energys[k] * V[k]
+    return d
+
+
 def _fmt_scalars(rho, sig):
     return rho, rho * sig / 2, rho * PI * sig ** 2, rho * PI * sig ** 3 / 6
 
@@ -192,6 +225,9 @@ def bulk_fe_terms(ms: MolSys, rho_beads, rho_bonds):
             mu += d
             lng += l
             f += free_energy_tpt1msa(rho_bonds, bt, rho_beads, sig, Z, lB)
+        elif m == "sw":
+            mu += sw_deriv(ms, rho_beads)
+            f += free_energy_sw(ms, rho_beads)
     return mu, lng, f
 
 

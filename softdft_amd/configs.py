@@ -54,6 +54,26 @@ def hard_sphere_chains(H: float = 16.0, dz: float = 0.02, n_seg: int = 20,
             "numerical": dict(numerics or SCF_NUMERICS), "model": ["ideal", "TPT1mfmt"]}
 
 
+def square_well_chains(H: float = 8.0, dz: float = 0.05, n_seg: int = 6, rho_poly: float = 0.1,
+                       rho_solvent: float = 0.3, eps_AA: float = 0.4, eps_AS: float = 0.2,
+                       eps_SS: float = 0.1, lam: Optional[float] = 1.5, base: str = "TPT1mfmt",
+                       numerics: Optional[dict] = None) -> dict:
+    """Hard-sphere chains in solvent with mean-field square-well attractions between all bead
+    pairs (model [ideal, <base>, sw]; pairs: square_well with energy and, unless lam is None,
+    lambda per pair)."""
+    ds = hard_sphere_chains(H=H, dz=dz, n_seg=n_seg, rho_poly=rho_poly, rho_solvent=rho_solvent,
+                            numerics=numerics)
+    ds["model"] = ["ideal", base, "sw"]
+
+    def entry(e):
+        d = {"energy": e}
+        if lam is not None:
+            d["lambda"] = lam
+        return d
+    ds["pairs"] = {"square_well": {"A, A": entry(eps_AA), "A, S": entry(eps_AS), "S, S": entry(eps_SS)}}
+    return ds
+
+
 def weak_polyelectrolyte(H: float = 30.0, dz: float = 0.01, n_seg: int = 100, rho_poly: float = 0.05,
                          pKa: float = 4.5, pH: float = 6.0, salt_M: float = 0.010,
                          solvent: float = 0.0, numerics: Optional[dict] = None,

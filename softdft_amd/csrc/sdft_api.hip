@@ -113,6 +113,11 @@ extern "C" void sdft_destroy(sdft_ctx_t* ctx) {
 //    w3 and wV: w3 * ch1 - wV * chv is computed once per class (outkind 2) and added to each bead's
 //    w2 * ch0_j (linearity; the reference sums the three per bead).
 //  * the two ends of a bond with equal diameters share kernel and input: one job writes both lng fields.
+//  * square well (square_well.jl:98-123): mu_i -= Re(sum_j rho_hat_j w_ij eps_ij) in k-space. With the
+//    half-sample mirror the real part is the average of the convolution at n and n-1, so it is one
+//    convolution with the kernel averaged over d and d-1, on rho_j trapez_j: a shared partial per bead
+//    type, added to that bead's back projection. The plain convolution (the free energy's, :88-95) is a
+//    forward weighted field read only by the grand potential.
 static int build_jobs(const sdft_desc_t* d, const DevSystem& sys, std::vector<ConvJob>& fwd,
                       std::vector<ConvJob>& pre, std::vector<ConvJob>& back, int& Fws) {
     const int NB = d->NB, MB = d->MB;
@@ -124,7 +129,7 @@ static int build_jobs(const sdft_desc_t* d, const DevSystem& sys, std::vector<Co
         ConvJob j;
         memset(&j, 0, sizeof(j));
         j.outkind = outkind; j.outfield = outfield; j.absval = absval; j.scale = scale;
-        j.addfield = -1; j.outfield2 = -1;
+        j.addfield = -1; j.outfield2 = -1; j.addfield2 = -1;
         return j;
     };
     auto add = [](ConvJob& j, int kern, int odd, int infam, int infield) {
@@ -167,10 +172,34 @@ static int build_jobs(const sdft_desc_t* d, const DevSystem& sys, std::vector<Co
             pre.push_back(w);
         }
     }
+    // square-well partners of each participating bead type: for every participating j the first pair
+    // (min(i, j), max(i, j)) in list order (findfirst, square_well.jl:78-79)
+    std::vector<int> sw_slot(NB, -1);
+    const int sw_ws = Fws;
+    for (int s = 0; s < sys.n_sw_beads; ++s) {
+        const int i = sys.sw_bead[s];
+        ConvJob f = job(0, sys.nw_sw_base + s, 0, 1.0);
+        ConvJob m = job(2, sw_ws + s, 0, -1.0);
+        for (int t = 0; t < sys.n_sw_beads; ++t) {
+            const int j = sys.sw_bead[t];
+            const int lo = std::min(i, j), hi = std::max(i, j);
+            int u = 0;
+            while (u < sys.n_sw && !(sys.sw_i[u] == lo && sys.sw_j[u] == hi)) ++u;
+            if (u == sys.n_sw) continue;
+            if (f.nterms == SDFT_SW_TERMS) return SDFT_E_UNSUPPORTED;
+            add(f, sys.sw_kbase + 2 * u + 1, 0, 0, j);
+            add(m, sys.sw_kbase + 2 * u, 0, 0, j);
+        }
+        fwd.push_back(f);
+        pre.push_back(m);
+        sw_slot[i] = sw_ws + s;
+    }
+    Fws += sys.n_sw_beads;
     for (int j = 0; j < NB; ++j) {
         ConvJob a = job(1, j, 0, 1.0);
         if (fmt || msa) add(a, 3 * j + 0, 0, 1, j);
         if (fmt) a.addfield = sys.cls[j];
+        a.addfield2 = sw_slot[j];
         back.push_back(a);
     }
     // lng and Psi need no shared partial: they run in the same launch as the partials
@@ -472,7 +501,12 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         return code;
     };
     Ctx& c = ctx->c;
-    c.P = P; c.NB = NB; c.MB = MB; c.nsp = nsp; c.K = 3 * NB + MB + 1;
+    const int n_sw = (F & SDFT_F_SW) ? d->n_sw_pairs : 0;
+    if ((F & SDFT_F_SW) && (n_sw <= 0 || n_sw > SDFT_MAX_SW || !d->sw_pairs || !d->sw_energy || !d->sw_lambda)) {
+        set_error(ctx, SDFT_E_ARG, "square well: need 1..%d pairs with energies and lambdas (got %d)", SDFT_MAX_SW, n_sw);
+        return fail(SDFT_E_ARG);
+    }
+    c.P = P; c.NB = NB; c.MB = MB; c.nsp = nsp; c.K = 3 * NB + MB + 1 + 2 * n_sw;
     ctx->lB = d->bjerrum;
     ctx->valences.assign(d->valences, d->valences + NB);
 
@@ -501,6 +535,25 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         }
     }
     c.ncls = sys.ncls;
+    // square-well pairs and the bead types they involve (ascending)
+    sys.n_sw = n_sw;
+    sys.sw_kbase = 3 * NB + MB + 1;
+    for (int u = 0; u < n_sw; ++u) {
+        sys.sw_i[u] = d->sw_pairs[2 * u];
+        sys.sw_j[u] = d->sw_pairs[2 * u + 1];
+        sys.sw_eps[u] = d->sw_energy[u];
+        sys.sw_lam[u] = d->sw_lambda[u];
+        if (sys.sw_i[u] < 0 || sys.sw_j[u] >= NB || sys.sw_i[u] > sys.sw_j[u]) {
+            set_error(ctx, SDFT_E_ARG, "square-well pair %d must satisfy 0 <= i <= j < NB", u);
+            return fail(SDFT_E_ARG);
+        }
+    }
+    for (int j = 0; j < NB; ++j) {
+        bool in = false;
+        for (int u = 0; u < n_sw; ++u) in = in || sys.sw_i[u] == j || sys.sw_j[u] == j;
+        if (in) sys.sw_bead[sys.n_sw_beads++] = j;
+    }
+    sys.nw_sw_base = 3 * NB + 2 * MB;
     sys.hard_walls = d->hard_walls;
     sys.charged_walls = d->wall_charge != nullptr;
     ctx->hard_walls = d->hard_walls;
@@ -566,7 +619,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     ctx->seg_states_h.assign(d->seg_states, d->seg_states + state_off);
     c.GF = (int)goff;
     c.Frt = NB + 2 * MB + sys.ncls;  // bead and bond-end densities x trapez, then one class sum per diameter
-    c.Fnw = 3 * NB + 2 * MB;
+    c.Fnw = 3 * NB + 2 * MB + sys.n_sw_beads;  // ... then one square-well field per participating bead type
     c.Fdp = 3 * NB + 2 * MB + 1;
 
     // ---- points and kernel sets (one set per distinct (NP, dz))
@@ -643,7 +696,10 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
 
     // ---- jobs
     std::vector<ConvJob> fwd, pre, back;
-    build_jobs(d, sys, fwd, pre, back, c.Fws);
+    if (build_jobs(d, sys, fwd, pre, back, c.Fws) != SDFT_OK) {
+        set_error(ctx, SDFT_E_UNSUPPORTED, "square well: at most %d partner bead types per bead type", SDFT_SW_TERMS);
+        return fail(SDFT_E_UNSUPPORTED);
+    }
     c.n_fwd_jobs = (int)fwd.size();
     c.n_pre_jobs = (int)pre.size();
     c.n_back_jobs = (int)back.size();
@@ -1219,7 +1275,8 @@ extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps
             npair = cnt(N - 1 - s_mid) * cnt(N - s_mid);  // child sweep; parent sweep similar
             (void)N;
         }
-        // chain kernel: valid (a, b) columns summed over all steps of both sweeps, 2 Be + 1 taps
+        // chain kernel: valid (a, b) columns summed over all steps of both sweeps, 2 B + 1 taps of the
+        // banded operator (B: the bond kernel's band; the kernel's own padding to Be is not counted)
         double chain_cols = 0.0;
         if (chain) {
             const int N = c.species[u].nseg, S = c.species[u].S, so = c.species[u].state_off;
@@ -1229,8 +1286,7 @@ extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps
         for (int p = 0; p < c.P; ++p) {
             const double NP = ctx->pts[p].NP;
             if (chain) {
-                const int Be = (band[ctx->pts[p].kset] + 7) / 16 * 16 + 8;
-                f += chain_cols * 2.0 * NP * (2.0 * Be + 1.0);
+                f += chain_cols * 2.0 * NP * (2.0 * band[ctx->pts[p].kset] + 1.0);
             } else if (kc == SDFT_KP_PROP_FP32 || kc == SDFT_KP_PROP_FP64) {
                 const bool bd = kc == SDFT_KP_PROP_FP32 ? c.band_fp32 != 0 : c.band_fp64 != 0;
                 const double taps = bd ? std::min(NP, 2.0 * band[ctx->pts[p].kset] + 1.0) : NP;

@@ -4,7 +4,7 @@
 //   grid family with F fields:   fam[base_p * F + f * NP_p + n]
 //     base_p = sum_{q<p} NP_q,  n in [0, NP_p)
 //   convolution kernels:          kern[kbase_s * K + k * L_s + d]
-//     one set per distinct (NP, dz); L_s = 2 NP_s; K = 3 NB + MB + 1
+//     one set per distinct (NP, dz); L_s = 2 NP_s; K = 3 NB + MB + 1 + 2 n_sw
 //   solution vectors (FP64):      x[xoff_p + i], i in [0, n_p), reference flatten order
 //   Anderson history (FP64):      hist[hoff_p + col * n_p + i]  (hoff_p = m * xoff_p)
 #pragma once
@@ -15,6 +15,8 @@
 #define SDFT_MAX_MB 16
 #define SDFT_MAX_S 2
 #define SDFT_MAX_SPECIES 16
+#define SDFT_MAX_SW 16          // square-well pairs
+#define SDFT_SW_TERMS 3         // square-well partners per bead type (terms of one convolution job)
 #define SDFT_MAX_M 8
 #define SDFT_ACC_POINTS 64          // grid points per density-accumulation block
 #define SDFT_MAX_TILES 256          // accumulation blocks per grid (NP <= 256 * 64 = 16384)
@@ -29,6 +31,9 @@
 //   3*j + 2 : wV (vector delta, i * w applied)          odd
 //   3*NB + u: bond u, delta(sigma_ij)/(4 pi sigma_ij^2) x spectral filter   even
 //   3*NB+MB : Coulomb 4 pi lB / lambda_FD(k)            even
+//   3*NB+MB+1+2u   : square-well pair u, eps_u (step(lambda_u sigma_ij) - step(sigma_ij)),
+//                    averaged over d and d-1 (the real part of the k-space product)
+//   3*NB+MB+1+2u+1 : the same kernel unshifted (free energy)
 struct DevPoint {
     int32_t NP;
     int32_t L;
@@ -79,6 +84,14 @@ struct DevSystem {
     int32_t ncls;                     // distinct bead diameters (the MSA sums run over these classes)
     int32_t cls[SDFT_MAX_NB];         // class of each bead
     double cls_sigma[SDFT_MAX_NB];    // diameter of each class
+    // square well: pair u couples bead types sw_i[u] <= sw_j[u]; its two kernels (depth times the
+    // shell weight step(lambda sigma_ij) - step(sigma_ij)) sit at sw_kbase + 2u (the half-step
+    // averaged form used for mu) and sw_kbase + 2u + 1 (plain, for the free energy)
+    int32_t n_sw, sw_kbase;
+    int32_t sw_i[SDFT_MAX_SW], sw_j[SDFT_MAX_SW];
+    double sw_eps[SDFT_MAX_SW], sw_lam[SDFT_MAX_SW];
+    int32_t n_sw_beads, nw_sw_base;   // bead types with a pair; their weighted fields start at nw_sw_base
+    int32_t sw_bead[SDFT_MAX_NB];
 };
 
 // one term of a convolution job: out += sign-structured T(+-)H(kern) * in
@@ -99,7 +112,7 @@ struct ConvJob {
     int32_t absval;    // take |.| of the result (scalar weighted densities)
     int32_t addfield;  // outkind 1: shared partial added to the result (-1: none)
     int32_t outfield2; // outkind 1: second F(x) field receiving the same values (-1: none)
-    int32_t pad;
+    int32_t addfield2; // outkind 1: second shared partial added to the result (-1: none)
     double scale;
 };
 

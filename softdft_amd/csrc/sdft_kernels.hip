@@ -69,9 +69,17 @@ __device__ double weight_value(const DevSystem& sys, int k, int m, int L, double
         const double kc = 0.5 * (M_PI / dz);
         const double r = a / kc;
         return w * exp(-(r * r) * (r * r));
-    } else {  // coul_func_fft!: 2 pi lB / ((1 - cos(k dz)) / dz^2), k = 0 -> 0
+    } else if (k == 3 * NB + MB) {  // coul_func_fft!: 2 pi lB / ((1 - cos(k dz)) / dz^2), k = 0 -> 0
         if (m == 0) return 0.0;
         return 2.0 * M_PI * sys.lB / ((1.0 - cos(kv * dz)) / (dz * dz));
+    } else {  // square well (square_well.jl:37-50): eps (step(lambda sigma_ij) - step(sigma_ij))
+        const int u = (k - sys.sw_kbase) >> 1;
+        const double sij = (sys.sigma[sys.sw_i[u]] + sys.sigma[sys.sw_j[u]]) / 2.0;
+        auto step = [&](double b) {
+            return (m == 0) ? (4.0 / 3.0) * M_PI * b * b * b
+                            : 4.0 * M_PI * (sin(b * a) - b * a * cos(b * a)) / (a * a * a);
+        };
+        return sys.sw_eps[u] * (step(sys.sw_lam[u] * sij) - step(sij));
     }
 }
 
@@ -88,6 +96,8 @@ __global__ void k_build_weights(const DevSystem* __restrict__ psys, const int32_
 
 // c[d] = (1/L) sum_m w(m) cos(2 pi m d / L)      (even kernels)
 // c[d] = -(1/L) sum_m w(m) sin(2 pi m d / L)     (odd vector kernels: Re IFFT(i w))
+// square-well mu kernels: (c[d] + c[d-1]) / 2, the real part of the k-space product on the
+// half-sample mirrored grid (x_ext[-n] = x_ext[n-1])
 __global__ void __launch_bounds__(256) k_build_cvec(const DevSystem* __restrict__ psys, const int32_t* __restrict__ set_L,
                                                     const int64_t* __restrict__ set_kbase, const double* __restrict__ wbuf,
                                                     double* __restrict__ c64, float* __restrict__ c32) {
@@ -96,9 +106,11 @@ __global__ void __launch_bounds__(256) k_build_cvec(const DevSystem* __restrict_
     const int L = set_L[s];
     const int d = blockIdx.x * blockDim.x + threadIdx.x;
     const bool odd = (k < 3 * sys.NB) && (k % 3 == 2);
+    const bool halfshift = sys.n_sw > 0 && k >= sys.sw_kbase && ((k - sys.sw_kbase) & 1) == 0;
+    const int dm = d > 0 ? d - 1 : L - 1;
     const double* w = wbuf + set_kbase[s] * sys.K + (int64_t)k * L;
     __shared__ double ws[256];
-    double acc = 0.0;
+    double acc = 0.0, acc2 = 0.0;
     for (int m0 = 0; m0 < L; m0 += 256) {
         __syncthreads();
         ws[threadIdx.x] = (m0 + threadIdx.x < L) ? w[m0 + threadIdx.x] : 0.0;
@@ -111,10 +123,15 @@ __global__ void __launch_bounds__(256) k_build_cvec(const DevSystem* __restrict_
                 double sn, cs;
                 sincospi(2.0 * (double)r / (double)L, &sn, &cs);
                 acc += ws[i] * (odd ? sn : cs);
+                if (halfshift) {
+                    const long long r2 = ((long long)m * dm) % L;
+                    acc2 += ws[i] * cospi(2.0 * (double)r2 / (double)L);
+                }
             }
         }
     }
     if (d < L) {
+        if (halfshift) acc = 0.5 * (acc + acc2);
         const double v = (odd ? -acc : acc) / (double)L;
         const int64_t o = set_kbase[s] * sys.K + (int64_t)k * L + d;
         c64[o] = v;
@@ -1343,8 +1360,8 @@ template <typename T, int NCL>
 __global__ void __launch_bounds__(256) k_omega(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
                                               const DevSolveState* __restrict__ stt,
                                               const T* __restrict__ nw, int Fnw, const T* __restrict__ rhoT, int Frt,
-                                              const T* __restrict__ rho_bonds, const double* __restrict__ x,
-                                              double* __restrict__ omega_acc) {
+                                              const T* __restrict__ rho_bonds, const T* __restrict__ rho_beads,
+                                              const double* __restrict__ x, double* __restrict__ omega_acc) {
     const DevSystem& sys = *psys;
     const int p = blockIdx.y;
     if ((stt[p].prec == 64 ? 64 : 32) != (sizeof(T) == 8 ? 64 : 32)) return;
@@ -1354,6 +1371,12 @@ __global__ void __launch_bounds__(256) k_omega(const DevSystem* __restrict__ psy
     double acc = 0.0;
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < NP; n += gridDim.x * blockDim.x) {
         double f = 0.0;
+        if (F & SDFT_F_SW) {
+            // eval_fe_functional (square_well.jl:64-95): -1/2 rho_i sum_j eps_ij (w_ij * rho_j trapez_j)
+            for (int s = 0; s < sys.n_sw_beads; ++s)
+                f -= 0.5 * (double)rho_beads[fidx(P, NB, sys.sw_bead[s], n)] *
+                     (double)nw[fidx(P, Fnw, sys.nw_sw_base + s, n)];
+        }
         if (F & (SDFT_F_MFMT | SDFT_F_TPT1MFMT)) {
             const FmtIn fi = fmt_load(sys, P, n, nw, Fnw);
             f += fmt_phi(fi);
@@ -1661,11 +1684,11 @@ void Pipeline<T>::launch_pointwise(hipStream_t st, const Ctx& c, int prec) {
 
 template <typename T>
 void Pipeline<T>::omega(hipStream_t st, const Ctx& c, const double* x, double* omega_acc) {
-    const T *nw_ = nw, *rhoT_ = rhoT, *rb_ = rho_bonds;
+    const T *nw_ = nw, *rhoT_ = rhoT, *rb_ = rho_bonds, *rbe_ = rho_beads;
     cls_dispatch(c.ncls, [&](auto k) {
         constexpr int NCL = decltype(k)::value;
         hipLaunchKernelGGL((k_omega<T, NCL>), dim3(SDFT_OMEGA_BLOCKS, c.P), dim3(256), 0, st, c.d_sys, c.d_pts,
-                           c.d_state, nw_, c.Fnw, rhoT_, c.Frt, rb_, x, omega_acc);
+                           c.d_state, nw_, c.Fnw, rhoT_, c.Frt, rb_, rbe_, x, omega_acc);
     });
 }
 

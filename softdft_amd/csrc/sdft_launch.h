@@ -121,6 +121,7 @@ __device__ __forceinline__ void conv_store(const ConvJob& J, const DevPoint& P, 
         wsum[P.base * (int64_t)Fws + (int64_t)J.outfield * P.NP + n] = y;
     } else {
         if (J.addfield >= 0) y += wsum[P.base * (int64_t)Fws + (int64_t)J.addfield * P.NP + n];
+        if (J.addfield2 >= 0) y += wsum[P.base * (int64_t)Fws + (int64_t)J.addfield2 * P.NP + n];
         fx[P.xoff + (int64_t)J.outfield * P.NP + n] = y;
         if (J.outfield2 >= 0) fx[P.xoff + (int64_t)J.outfield2 * P.NP + n] = y;
     }

@@ -36,7 +36,7 @@ E0 = 1.60217653e-19
 K_B = 1.3806505e-23
 N_A = 6.0221415e23
 
-KNOWN_FE_MODELS = ("ideal", "mfmt", "TPT1mfmt", "msa", "TPT1msa")
+KNOWN_FE_MODELS = ("ideal", "mfmt", "TPT1mfmt", "msa", "TPT1msa", "sw")
 
 
 class SetupError(ValueError):
@@ -105,6 +105,7 @@ class MolSys:
     bond_types: List[Tuple[int, int]]
     fe_model: List[str]
     species_charge: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    sw: Optional["SquareWellPairs"] = None    # pairs: square_well (process_pairs.jl:42-95)
 
     @property
     def NB(self) -> int:
@@ -122,6 +123,54 @@ class MolSys:
         else:
             pair, pi, pj = (sj, si), 1, 0
         return self.bond_types.index(pair), pi, pj
+
+
+@dataclass
+class SquareWellPairs:
+    """properties.pairs[:square_well] (src/molecular_system/setup/species/properties/process_pairs.jl:42-95):
+    ordered bead-type pairs (i <= j, 0-based), their well depths and width multipliers, and the
+    bead types that take part."""
+    pairs: List[Tuple[int, int]]
+    energys: List[float]
+    lambdas: List[float]
+    species: List[int]
+
+    def index(self, i: int, j: int) -> Optional[int]:
+        """findfirst(==((min, max)), pairs): the first matching pair, None if absent."""
+        key = (min(i, j), max(i, j))
+        for k, pr in enumerate(self.pairs):
+            if pr == key:
+                return k
+        return None
+
+
+def process_square_well(monomers: List[str], dataset: dict) -> Optional[SquareWellPairs]:
+    """process_pairs.jl:42-95 for the `square_well` pair type. Pairs are visited as Julia's
+    Iterators.product(sorted monomers, sorted monomers) visits them (first name fastest), a YAML key
+    "A, B" naming an ordered pair; each interaction's properties are appended to per-property lists
+    (energy -> energys, lambda -> lambdas); lambdas default to 1.5 when no pair gives one."""
+    pair_data = dataset.get("pairs") or {}
+    if "square_well" not in pair_data:
+        return None
+    pd = pair_data["square_well"] or {}
+    pairs: List[Tuple[int, int]] = []
+    props: Dict[str, List[float]] = {}
+    for j2, name2 in enumerate(monomers):
+        for j1, name1 in enumerate(monomers):
+            key = f"{name1}, {name2}"
+            if key not in pd:
+                continue
+            pairs.append((j1, j2) if j1 < j2 else (j2, j1))
+            for prop, value in (pd[key] or {}).items():
+                props.setdefault(str(prop) + "s", []).append(float(value))
+    energys = props.get("energys", [])
+    if "energys" in props and "lambdas" not in props:
+        props["lambdas"] = [1.5] * len(energys)
+    lambdas = props.get("lambdas", [])
+    if len(energys) < len(pairs) or len(lambdas) < len(pairs):
+        raise SetupError("square_well: every pair needs an energy (and a lambda once any pair gives one)")
+    involved = sorted({i for pr in pairs for i in pr})
+    return SquareWellPairs(pairs=pairs, energys=energys, lambdas=lambdas, species=involved)
 
 
 # ---------------------------------------------------------------- topology
@@ -500,6 +549,10 @@ def build_molsys(dataset: dict) -> MolSys:
     for k, sp in enumerate(species):
         sp.pSpecies = pS[k]
 
+    sw = process_square_well(monomers, dataset)
+    if "sw" in fe_model and (sw is None or not sw.pairs):
+        raise SetupError("model 'sw' needs pairs: square_well entries")
     return MolSys(constants=constants, temperature=T, dielectric=er, bjerrum=lB,
                   monomers=monomers, diameters=diam, valences=val, delta_muH=dmu,
-                  species=species, reactions=reactions, bond_types=bond_types, fe_model=fe_model)
+                  species=species, reactions=reactions, bond_types=bond_types, fe_model=fe_model,
+                  sw=sw if "sw" in fe_model else None)

@@ -31,7 +31,7 @@ import numpy as np
 from .bulk import BulkState, solve_bulk
 from .molsys import MolSys, SetupError, build_molsys
 
-FUNCTIONAL_KEYS = ("mfmt", "TPT1mfmt", "msa", "TPT1msa")
+FUNCTIONAL_KEYS = ("mfmt", "TPT1mfmt", "msa", "TPT1msa", "sw")
 
 
 def julia_cld_plus1(H: float, dz: float) -> int:

@@ -28,6 +28,8 @@ def _systems():
                                                               wall_charge=(-0.05, -0.02)),
         "mfmt_msa": configs.weak_polyelectrolyte(H=5.0, dz=0.05, n_seg=4, solvent=0.2, pH=4.0, msa="msa"),
         "brush": configs.weak_polyelectrolyte(H=8.0, dz=0.05, n_seg=10, pH=6.0, pKa=5.0, grafted=0.05),
+        "square_well": configs.square_well_chains(H=6.0, dz=0.05, n_seg=5),
+        "square_well_monomers": configs.square_well_chains(H=6.0, dz=0.05, n_seg=1, rho_poly=0.2, base="mfmt"),
     }
 
 
@@ -61,7 +63,7 @@ def test_objective_fp64_matches_oracle(name):
     b.close()
 
 
-@pytest.mark.parametrize("name", ["chains", "weak_pe", "brush"])
+@pytest.mark.parametrize("name", ["chains", "weak_pe", "brush", "square_well"])
 def test_objective_fp32_is_close(name):
     sp = build_state_point(SYSTEMS[name])
     b = abi.Batch([sp])
@@ -73,7 +75,8 @@ def test_objective_fp32_is_close(name):
     b.close()
 
 
-@pytest.mark.parametrize("name", ["chains", "weak_pe", "weak_pe_charged_walls", "brush"])
+@pytest.mark.parametrize("name", ["chains", "weak_pe", "weak_pe_charged_walls", "brush", "square_well",
+                                  "square_well_monomers"])
 def test_solve_matches_oracle(name):
     sp = build_state_point(SYSTEMS[name])
     b = abi.Batch([sp])

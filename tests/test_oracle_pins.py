@@ -18,6 +18,12 @@ closed forms, finite differences, invariants or brute force:
   propagators / densities (species_model/)     ideal chain without walls reproduces bulk weights; grafted chain
                                                integrates to sigma_graft per segment; Henderson-Hasselbalch alpha;
                                                dimer density equals the explicit one-bond integral
+  square well (square_well.jl, iLST + iDFT)    bulk derivative by central differences of f_sw; uniform slab
+                                               without walls: F(x0) = x0 and Omega/V = -P_bulk (weight
+                                               normalisation V_ij, signs, the 1/2 of the double loop); the
+                                               k-space real-part quirk equals the half-step average of the
+                                               mirrored convolution (the library's form) and differs from the
+                                               plain convolution on a non-uniform profile
   Anderson (AndersonMixing.jl:56-169)          linear contraction converges to (I - A)^-1 b
   objective_dft end to end                     bulk-consistency KAT F(x0) = x0; Omega/V = -P_bulk
   bulk state (iLST)                            Carnahan-Starling pressure; HH dissociation
@@ -386,11 +392,63 @@ def test_picard_phase_is_the_damped_update():
     assert np.allclose(seen[1], (1 - a) * seen[0] + a * (0.5 * seen[0] + 1.0))
 
 
+# ----------------------------------------------------------------- square well
+def test_square_well_bulk_derivative_matches_finite_differences():
+    from softdft_amd.bulk import free_energy_sw, sw_deriv
+    ms = build_molsys(configs.square_well_chains(eps_AA=0.4, eps_AS=0.25, eps_SS=0.1, lam=1.7))
+    rho = np.array([0.13, 0.31])
+    d = sw_deriv(ms, rho)
+    for i in range(2):
+        h = 1e-6
+        rp, rm = rho.copy(), rho.copy()
+        rp[i] += h
+        rm[i] -= h
+        assert d[i] == pytest.approx((free_energy_sw(ms, rp) - free_energy_sw(ms, rm)) / (2 * h), rel=1e-8)
+    # one bead type: f = -1/2 rho^2 eps V, V = 4 pi/3 sigma^3 (lambda^3 - 1)
+    ms1 = build_molsys(configs.square_well_chains(n_seg=1, rho_solvent=0.0, eps_AA=0.3, lam=1.5))
+    V = 4 * PI / 3 * (1.5 ** 3 - 1)
+    assert free_energy_sw(ms1, np.array([0.2])) == pytest.approx(-0.5 * 0.04 * 0.3 * V, rel=1e-14)
+    assert sw_deriv(ms1, np.array([0.2]))[0] == pytest.approx(-0.2 * 0.3 * V, rel=1e-14)
+
+
+def test_square_well_default_lambda_and_pair_order():
+    ds = configs.square_well_chains(lam=None)
+    ms = build_molsys(ds)
+    assert ms.sw.lambdas == [1.5, 1.5, 1.5]
+    # Iterators.product visits (A, A), (S, A), (A, S), (S, S): the "A, S" key is the third visit
+    assert ms.sw.pairs == [(0, 0), (0, 1), (1, 1)]
+    assert ms.sw.species == [0, 1]
+
+
+def test_square_well_real_part_is_the_half_step_average():
+    """mu_sw from Re(rho_hat w) (square_well.jl:120) equals the half-step average of the mirrored
+    convolution, y[n] = (u[n] + u[n-1]) / 2 with u[-1] = u[0], i.e. the operator with the kernel
+    vector (c[d] + c[d-1]) / 2; the free energy's plain convolution u differs from it."""
+    rng = np.random.default_rng(5)
+    NP, dz = 41, 0.05
+    L = 2 * NP
+    k = od.kgrid(L, dz)
+    w = 0.37 * (od.w_step(k, 1.5) - od.w_step(k, 1.0))
+    f = rng.random(NP)
+    fh = np.fft.fft(od.mirror(f, L))
+    quirk = np.fft.ifft((fh * w).real).real[:NP]
+    plain = np.fft.ifft(fh * w).real[:NP]
+    avg = 0.5 * (plain + np.concatenate([[plain[0]], plain[:-1]]))
+    assert np.allclose(quirk, avg, rtol=1e-12, atol=1e-13)
+    c = _cvec(w, False)
+    c_half = 0.5 * (c + np.roll(c, 1))
+    assert np.allclose(_tph_operator(c_half, NP, 1) @ f, quirk, rtol=1e-12, atol=1e-13)
+    assert np.max(np.abs(quirk - plain)) > 1e-4 * np.max(np.abs(plain))
+
+
 # ----------------------------------------------------------------- end to end KATs
-@pytest.mark.parametrize("builder", ["neutral", "charged"])
+@pytest.mark.parametrize("builder", ["neutral", "charged", "square_well"])
 def test_bulk_consistency_kat(builder):
     if builder == "neutral":
         ds = configs.hard_sphere_chains(H=3.0, dz=0.05, n_seg=5, walls=False)
+    elif builder == "square_well":
+        ds = configs.square_well_chains(H=3.0, dz=0.05, n_seg=4, eps_AA=0.4, eps_AS=0.25, eps_SS=0.1, lam=1.6)
+        ds["geometry"]["external_field"] = {}
     else:
         ds = configs.weak_polyelectrolyte(H=3.0, dz=0.05, n_seg=5, solvent=0.2, pH=5.0)
         ds["geometry"]["external_field"] = {}
@@ -399,7 +457,7 @@ def test_bulk_consistency_kat(builder):
     fx = o.objective(x0)
     mu, lng, psi, psic = unflatten(fx - x0, sp.NP, sp.NB, sp.MB)
     assert np.max(np.abs(mu)) < 1e-13 and np.max(np.abs(lng)) < 1e-13 and np.max(np.abs(psi)) < 1e-13
-    if builder == "neutral":
+    if builder != "charged":
         assert abs(psic) < 1e-15
     om = o.grand_potential(x0)
     assert om / (sp.NP * sp.dz) == pytest.approx(-sp.bulk.pressure, rel=1e-12)
