@@ -14,6 +14,12 @@ values on the left half (1-based index k < round(NP_new / 2) + 2, Julia round-ha
 mirror image on the right half, which is the reference's copy rule (adjust_distanceZ.jl:73-88), and
 PsiC is copied (adjust_distanceZ.jl:90).
 
+The pH continuation (routines/adjust_pH.jl:16-124) steps the pSpecies of H+ ("Y"), warm-starts each pH
+from the previous one with the same copy rule (adjust_pH.jl:50-80, on a fixed grid), and records per
+step the brush height of the grafted species (brush_height.jl) and the average degree of ionization
+(the `avg` state-2 column of total_state_fractions, degree_of_dissociation.jl), stopping once alpha
+reaches 0.99 on an increasing sweep (0.01 on a decreasing one, adjust_pH.jl:107-113).
+
 Batching: width i is warm-started from width i - 1. The widths are solved in `waves` rounds: round r
 holds every width i with i % waves == r, so each round is one batched solve and every width except
 those of round 0 (cold, from the bulk guess) starts from its immediate narrower neighbour, as in
@@ -27,7 +33,9 @@ from typing import Callable, List, Optional, Sequence
 
 import numpy as np
 
-from . import abi
+import math
+
+from . import abi, analysis
 from .problem import StatePoint, build_state_point, flatten, unflatten
 
 
@@ -127,3 +135,84 @@ def write_run_info(r: WidthResult, directory: str) -> None:
     with open(os.path.join(directory, "run_info.csv"), "w") as f:
         f.write("step,distance,Pressure,deltaP,energy\n")
         f.write(f"{r.step},{r.distance!r},{r.pressure!r},{r.delta_p!r},{r.energy!r}\n")
+
+
+@dataclass
+class PHResult:
+    step: int             # 1-based sweep index
+    pH: float
+    brush_height: float   # NaN without exactly one grafted species (adjust_pH.jl:86-87)
+    alpha: float          # avg fraction of state 2 over multistate segments, NaN without one
+    omega: float
+    iterations: int
+    warm: bool
+
+
+def _waves_solve(sps, waves, opts, on_round, finish):
+    """Solve `sps` in `waves` batched rounds, point i warm-started from point i - 1 (cold in round 0);
+    finish(batch, idx, x_parts, iters, warm) is called per round while the context is alive."""
+    n = len(sps)
+    xs: List[Optional[np.ndarray]] = [None] * n
+    waves = max(1, min(waves, n))
+    for r in range(waves):
+        idx = list(range(r, n, waves))
+        if not idx:
+            continue
+        batch = abi.Batch([sps[i] for i in idx])
+        x0 = [remap_solution(sps[i - 1], xs[i - 1], sps[i]) if r > 0 else sps[i].x0() for i in idx]
+        x, it, err = batch.solve(np.concatenate(x0), **opts)
+        parts = batch.split(x)
+        for k, i in enumerate(idx):
+            xs[i] = parts[k].copy()
+        finish(batch, idx, x, it, r > 0)
+        batch.close()
+        if on_round is not None:
+            on_round(r, float(np.max(err)))
+
+
+def ph_sweep(datasets: Sequence[dict], pH_values: Sequence[float], *, waves: int = 8,
+             precision: int = abi.PREC_MIXED, numerics: Optional[dict] = None, max_iters: int = 3000,
+             on_round: Optional[Callable[[int, float], None]] = None) -> List[PHResult]:
+    """pH continuation over `datasets` (one per pH in `pH_values`, same grid and chemistry) with
+    neighbour warm starts; the result list ends at the reference's early-stop step."""
+    sps = [build_state_point(d) for d in datasets]
+    n = len(sps)
+    num = dict(numerics or sps[0].numerics)
+    opts = dict(tole=num["tole"], damping=num["damping"], mixing_max=num["mixing_max"],
+                switch_tole=num["switch_tole"], anderm=int(num["anderm"]), max_iters=max_iters,
+                precision=precision)
+    res: List[Optional[PHResult]] = [None] * n
+
+    def finish(batch, idx, x, it, warm):
+        om, al, _ = batch.observables()
+        for k, i in enumerate(idx):
+            sp = sps[i]
+            segs = [batch.segment_profile(u, k) for u in range(batch.nsp)]
+            hs = analysis.brush_heights(segs, sp)
+            multi = [u for u, s in enumerate(sp.ms.species) if any(len(f) > 1 for f in s.state_family)]
+            alpha = float(al[k, multi[0], 1]) if multi else math.nan
+            res[i] = PHResult(step=i + 1, pH=float(pH_values[i]),
+                              brush_height=float(hs[0][1]) if len(hs) == 1 else math.nan,
+                              alpha=alpha, omega=float(om[k]), iterations=int(it[k]), warm=warm)
+
+    _waves_solve(sps, waves, opts, on_round, finish)
+    out = [r for r in res if r is not None]
+    increasing = n < 2 or pH_values[-1] >= pH_values[0]
+    for k, r in enumerate(out):
+        if (increasing and r.alpha >= 0.99) or (not increasing and r.alpha <= 0.01):
+            return out[:k + 1]
+    return out
+
+
+def write_ph_summary(results: Sequence[PHResult], path: str) -> None:
+    """summary_brush_alpha.csv (adjust_pH.jl:298-310)."""
+    def g12(v: float) -> str:   # Julia's @printf "%.12g" (NaN / Inf spelled as Julia prints them)
+        if math.isnan(v):
+            return "NaN"
+        if math.isinf(v):
+            return "Inf" if v > 0 else "-Inf"
+        return "%.12g" % v
+    with open(path, "w") as f:
+        f.write("step,pH,brush_height,alpha\n")
+        for r in results:
+            f.write(f"{r.step},{g12(r.pH)},{g12(r.brush_height)},{g12(r.alpha)}\n")

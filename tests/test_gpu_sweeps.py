@@ -30,3 +30,30 @@ def test_width_sweep_matches_cold_solves(precision):
         assert abs(r.delta_p - (pc[k, 0] - sp.bulk.pressure)) < 1e-7
         assert abs(r.energy - (om[k] + sp.bulk.pressure * sp.H)) < 1e-7 * max(1.0, abs(r.energy))
     b.close()
+
+
+def test_ph_sweep_matches_cold_solves():
+    from softdft_amd import analysis
+    from softdft_amd.sweeps import ph_sweep
+    pHs = [4.0, 4.5, 5.0, 5.5, 6.0, 6.5]
+    dss = [configs.weak_polyelectrolyte(H=8.0, dz=0.05, n_seg=10, pH=pH, pKa=5.0, grafted=0.05) for pH in pHs]
+    print("case ph_sweep brush", flush=True)
+    res = ph_sweep(dss, pHs, waves=3, precision=abi.PREC_MIXED)
+    assert [r.step for r in res] == list(range(1, len(res) + 1))
+    assert [r.warm for r in res] == [i % 3 != 0 for i in range(len(res))]
+    sps = [build_state_point(d) for d in dss]
+    b = abi.Batch(sps)
+    num = sps[0].numerics
+    x, it, err = b.solve(b.x0(), tole=num["tole"], damping=num["damping"], mixing_max=num["mixing_max"],
+                         switch_tole=num["switch_tole"], anderm=int(num["anderm"]), max_iters=3000,
+                         precision=abi.PREC_FP64)
+    om, al, _ = b.observables()
+    for k, r in enumerate(res):
+        segs = [b.segment_profile(u, k) for u in range(b.nsp)]
+        (_, h), = analysis.brush_heights(segs, sps[k])
+        assert abs(r.omega - om[k]) < 1e-8 * max(1.0, abs(om[k]))
+        assert abs(r.alpha - al[k, 0, 1]) < 1e-7
+        assert abs(r.brush_height - h) < 1e-6 * h
+    # ionization grows with pH on a weak acid
+    assert all(res[k + 1].alpha > res[k].alpha for k in range(len(res) - 1))
+    b.close()

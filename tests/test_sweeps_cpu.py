@@ -52,3 +52,14 @@ def test_sweep_summary_is_relative_to_the_last_width(tmp_path):
     f = lines[1].split(",")
     assert float(f[3]) == 0.25 - 0.15 and float(f[5]) == -3.0 - (-2.5)
     assert lines[2].split(",")[3:6:2] == ["0.0", "0.0"]
+
+
+def test_ph_summary_format(tmp_path):
+    from softdft_amd.sweeps import PHResult, write_ph_summary
+    rs = [PHResult(step=1, pH=3.0, brush_height=float("nan"), alpha=0.123456789012345, omega=-1.0,
+                   iterations=10, warm=False),
+          PHResult(step=2, pH=3.05, brush_height=4.25, alpha=1.0 / 3.0, omega=-1.0, iterations=9, warm=True)]
+    p = tmp_path / "summary_brush_alpha.csv"
+    write_ph_summary(rs, str(p))
+    assert p.read_text() == ("step,pH,brush_height,alpha\n1,3,NaN,0.123456789012\n"
+                             "2,3.05,4.25,0.333333333333\n")
