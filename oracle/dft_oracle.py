@@ -52,6 +52,14 @@ import numpy as np
 PI = math.pi
 
 
+def _julia_pow(x: float, k: int) -> float:
+    """x^k as Julia's Float64 power evaluates it past the double range: Inf instead of an error."""
+    try:
+        return x ** k
+    except OverflowError:
+        return math.inf
+
+
 # ---------------------------------------------------------------- weights
 
 def kgrid(L: int, dz: float) -> np.ndarray:
@@ -485,11 +493,11 @@ class OracleDFT:
                 lamb += float(np.sum(contr * temp * trapez[:, si] * dz))
         else:
             lamb = 1.0
-            norm = normative ** (N - 1)
+            norm = _julia_pow(normative, N - 1)
         if not math.isfinite(lamb) or lamb <= 0:
             raise FloatingPointError(f"Normalization error: lambda = {lamb} for species {u}")
         norm *= 1.0 / lamb
-        lamb /= normative ** (N - 1)
+        lamb /= _julia_pow(normative, N - 1)
         self.fix_lambda[u] = lamb
 
         # ---- densities

@@ -233,10 +233,96 @@ def bulk_fe_terms(ms: MolSys, rho_beads, rho_bonds):
 
 # ------------------------------------------------------------ bulk chains
 
+def _lse(v) -> float:
+    v = np.asarray(v, dtype=np.float64)
+    m = float(np.max(v)) if v.size else -math.inf
+    if m == -math.inf:
+        return -math.inf
+    return m + math.log(float(np.sum(np.exp(v - m))))
+
+
+def _chain_bulk_log(ms: MolSys, u: int, mu_ex, lng):
+    """The recursion of _chain_bulk carried out on logarithms of the propagators, for chains whose
+    propagator products overflow a double (500-mers at high pH): (log Xi, rho_seg, bonds). The
+    reference evaluates the same products directly and returns Inf/NaN there."""
+    sp = ms.species[u]
+    fam, ch, pa, lv = sp.state_family, sp.topology.children, sp.topology.parents, sp.topology.levels
+    dmu = ms.delta_muH
+    N = sp.n_seg
+    S = max(len(f) for f in fam)
+    C = max(max((len(c) for c in ch), default=0), 1)
+    lC = np.full((S, C, S, N), -math.inf)
+    for level in reversed(lv):
+        for p in level:
+            for a, si in enumerate(fam[p]):
+                lC[0, 0, a, p] = 0.0
+                for c_idx, c in enumerate(ch[p]):
+                    for b, sj in enumerate(fam[c]):
+                        bond = ms.bond_types.index((min(si, sj), max(si, sj)))
+                        v = -dmu[sj] - mu_ex[sj] + lng[bond]
+                        for gci, _ in enumerate(ch[c]):
+                            v += _lse(lC[:, gci, b, c])
+                        lC[b, c_idx, a, p] = v
+    lP = np.full((S, S, N), -math.inf)
+    for level in lv:
+        for c in level:
+            for b, sj in enumerate(fam[c]):
+                lP[0, b, c] = 0.0
+                for p in pa[c]:
+                    for a, si in enumerate(fam[p]):
+                        bond = ms.bond_types.index((min(si, sj), max(si, sj)))
+                        v = -dmu[si] - mu_ex[si] + lng[bond] + _lse(lP[:, a, p])
+                        for oci, oc in enumerate(ch[p]):
+                            if oc != c:
+                                v += _lse(lC[:, oci, a, p])
+                        lP[a, b, c] = v
+    terms = []
+    for a, si in enumerate(fam[0]):
+        t = -(mu_ex[si] + dmu[si]) + _lse(lP[:, a, 0])
+        for ci, _ in enumerate(ch[0]):
+            t += _lse(lC[:, ci, a, 0])
+        terms.append(t)
+    logXi = _lse(terms)
+    seg = np.zeros((S, N))
+    bonds = np.zeros(ms.MB)
+    for i in range(N):
+        for a, si in enumerate(fam[i]):
+            lpref = -(mu_ex[si] + dmu[si]) + _lse(lP[:, a, i]) - logXi
+            if ch[i]:
+                c0 = ch[i][0]
+                cc = []
+                for b, _ in enumerate(fam[c0]):
+                    t = lC[b, 0, a, i]
+                    for oci, oc in enumerate(ch[i]):
+                        if oc != c0:
+                            t += _lse(lC[:, oci, a, i])
+                    cc.append(t)
+                lcc = _lse(cc)
+            else:
+                lcc = 0.0
+            lval = lpref + lcc
+            seg[a, i] = math.exp(lval)
+            for ci, c in enumerate(ch[i]):
+                lQ = _lse(lC[:, ci, a, i])
+                for b, sj in enumerate(fam[c]):
+                    bonds[ms.bond_types.index((min(si, sj), max(si, sj)))] += math.exp(lval + lC[b, ci, a, i] - lQ)
+    return logXi, seg, bonds
+
+
 def _chain_bulk(ms: MolSys, u: int, mu_ex, lng):
-    """(Xi, rho_seg[S,N], bond contributions [MB]) per unit species density;
+    """(Xi, rho_seg[S,N], bond contributions [MB], log Xi) per unit species density;
     src/iLST/species_model/freely_jointed/{child,parent}_propagators.jl,
-    partition_function.jl, accumulate_densities.jl:32-96."""
+    partition_function.jl, accumulate_densities.jl:32-96. Products that overflow are redone on
+    logarithms (_chain_bulk_log), which agrees with the direct form wherever that is finite."""
+    with np.errstate(over="ignore", invalid="ignore"):
+        Xi, seg, bonds = _chain_bulk_direct(ms, u, mu_ex, lng)
+    if math.isfinite(Xi) and Xi > 0 and np.all(np.isfinite(seg)) and np.all(np.isfinite(bonds)):
+        return Xi, seg, bonds, math.log(Xi)
+    logXi, seg, bonds = _chain_bulk_log(ms, u, mu_ex, lng)
+    return math.exp(logXi) if logXi < 709.0 else math.inf, seg, bonds, logXi
+
+
+def _chain_bulk_direct(ms: MolSys, u: int, mu_ex, lng):
     sp = ms.species[u]
     fam, ch, pa, lv = sp.state_family, sp.topology.children, sp.topology.parents, sp.topology.levels
     dmu = ms.delta_muH
@@ -348,15 +434,17 @@ def solve_bulk(ms: MolSys, tol: float = 1e-14, max_newton: int = 100) -> BulkSta
         rho = base.copy()
         Xi = np.zeros(nsp)
         segs, bonds = [], []
+        logXi = np.zeros(nsp)
         for u in range(nsp):
-            X, seg, b = _chain_bulk(ms, u, mu_ex, lng)
+            X, seg, b, lX = _chain_bulk(ms, u, mu_ex, lng)
             Xi[u] = X
+            logXi[u] = lX
             segs.append(seg)
             bonds.append(b)
         for u, sp in enumerate(ms.species):
             if not math.isnan(sp.pSpecies):
                 # mu_species = ln rho - ln Xi = ln(unit_conv) - pSpecies ln10
-                rho[u] = math.exp(math.log(unit_conv) - sp.pSpecies * LOG10 + math.log(Xi[u]))
+                rho[u] = math.exp(math.log(unit_conv) - sp.pSpecies * LOG10 + logXi[u])
         # charge balance (balance_charge.jl:21-79): Q_solver equals the net charge with
         # compensators at their input densities; opposite-sign compensators absorb it.
         comp = [sp.adjustable for sp in ms.species]
@@ -368,11 +456,11 @@ def solve_bulk(ms: MolSys, tol: float = 1e-14, max_newton: int = 100) -> BulkSta
                 raise SetupError(f"No compensating species available to absorb net charge {R}")
             for u in adj:
                 rho[u] = base[u] + max(-R / len(adj) / qch[u], 0.0)
-        return rho, Xi, segs, bonds
+        return rho, Xi, segs, bonds, logXi
 
     def state(y):
         mu_ex, lng = y[:ms.NB], y[ms.NB:]
-        rho, Xi, segs, bonds = species_densities(mu_ex, lng)
+        rho, Xi, segs, bonds, logXi = species_densities(mu_ex, lng)
         beads = np.zeros(ms.NB)
         rb = np.zeros((2, ms.MB))
         rseg = []
@@ -384,10 +472,10 @@ def solve_bulk(ms: MolSys, tol: float = 1e-14, max_newton: int = 100) -> BulkSta
                     beads[st] += s[a, i]
             rb[0] += bonds[u] * rho[u]
             rb[1] += bonds[u] * rho[u]
-        return rho, Xi, rseg, beads, rb
+        return rho, Xi, rseg, beads, rb, logXi
 
     def G(y):
-        _, _, _, beads, rb = state(y)
+        _, _, _, beads, rb, _ = state(y)
         mu, lng, _ = bulk_fe_terms(ms, beads, rb)
         return np.concatenate([mu, lng]) - y
 
@@ -410,12 +498,12 @@ def solve_bulk(ms: MolSys, tol: float = 1e-14, max_newton: int = 100) -> BulkSta
         y = y + dy
     else:
         raise SetupError("bulk state did not converge")
-    rho, Xi, rseg, beads, rb = state(y)
+    rho, Xi, rseg, beads, rb, logXi = state(y)
     mu_ex, lng, f_ex = bulk_fe_terms(ms, beads, rb)
     mu_sp = np.zeros(nsp)
     for u in range(nsp):
         ideal = math.log(rho[u]) if rho[u] > 1e-100 else -1e20
-        mu_sp[u] = ideal - math.log(Xi[u])
+        mu_sp[u] = ideal - logXi[u]
     P = float(np.sum(rho) - f_ex + np.sum(beads * mu_ex) - np.sum(rb[0] * lng) / 2 - np.sum(rb[1] * lng) / 2)
     return BulkState(rho_species=rho, rho_segments=rseg, rho_beads=beads, rho_bonds=rb,
                      mu_species=mu_sp, mu_ex=mu_ex, lng=lng, Xi=Xi, pressure=P, species_charge=qch)

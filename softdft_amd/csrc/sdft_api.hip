@@ -603,9 +603,13 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
             if (cnt > 1) multi = 1;
         }
         s.multistate = multi;
+        s.rescale = (N >= SDFT_RESCALE_MIN_SEG && s.model == SDFT_MODEL_FREELY_JOINTED) ? 1 : 0;
         goffs[u] = goff;
         SpeciesHost& sh = c.species[u];
         sh.nseg = N; sh.S = S; sh.model = s.model; sh.fixed_seg = fseg; sh.multistate = multi;
+        sh.rescale = s.rescale;
+        c.nmax_seg = std::max(c.nmax_seg, N);
+        c.any_rescale = c.any_rescale || s.rescale;
         sh.state_off = state_off; sh.maxd_off = maxd_off; sh.goff = goff;
         ctx->nseg.push_back(N);
         ctx->S.push_back(S);
@@ -821,6 +825,8 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         TRY(dalloc(ctx, &pp.hl, c.sumNP * 2 * MB));
         TRY(dalloc(ctx, &pp.gC, c.sumNP * c.GF));
         TRY(dalloc(ctx, &pp.gP, c.sumNP * c.GF));
+        TRY(dalloc(ctx, &pp.smax, (int64_t)P * c.nsp * 2 * c.nmax_seg));
+        TRY(dalloc(ctx, &pp.lsc, (int64_t)P * c.nsp * (c.nmax_seg + 1)));
         TRY(dalloc(ctx, &pp.rho_beads, c.sumNP * NB));
         TRY(dalloc(ctx, &pp.rho_bonds, c.sumNP * 2 * MB));
         TRY(dalloc(ctx, &pp.rhoT, c.sumNP * c.Frt));
@@ -1235,11 +1241,11 @@ extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps
             case SDFT_KP_CONV_BACK_FP64: ctx->p64.launch_conv(ctx->stream, c, 64, true, d_fx); break;
             case SDFT_KP_CHAIN_FP32:
                 launch_chain<float>(ctx->stream, c, 32, u, ctx->p32.kern, ctx->p32.elam, ctx->p32.hl, ctx->p32.trapez,
-                                    ctx->p32.gC, ctx->p32.gP);
+                                    ctx->p32.gC, ctx->p32.gP, ctx->p32.smax);
                 break;
             case SDFT_KP_CHAIN_FP64:
                 launch_chain<double>(ctx->stream, c, 64, u, ctx->p64.kern, ctx->p64.elam, ctx->p64.hl,
-                                     ctx->p64.trapez, ctx->p64.gC, ctx->p64.gP);
+                                     ctx->p64.trapez, ctx->p64.gC, ctx->p64.gP, ctx->p64.smax);
                 break;
             default: ctx->p64.launch_pointwise(ctx->stream, c, 64); break;
         }

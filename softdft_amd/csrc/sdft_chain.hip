@@ -41,6 +41,8 @@ struct ChainArgs {
     int64_t goff;
     int32_t ck[2][SDFT_MAX_S * SDFT_MAX_S];  // canonical bond kernel of (sweep, a * S + b)
     int32_t npad_max, be_max;                // LDS carve-out (largest point of the launch)
+    int32_t rescale, nsp, nmax, pad;         // per-step rescaling (scale_slot tables)
+    unsigned long long* smax;
 };
 
 struct ChainCol {      // one (a, b) column at one step
@@ -140,6 +142,32 @@ __global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __rest
         __syncthreads();  // column info of this step; Q of the previous step complete
         // ---- extended inputs X = W * Q (Q = 1 at the free chain end)
         const bool first = s == 1;
+        // per-step rescaling: divide the input by the largest Q of the previous step (recorded for
+        // the segment log scales)
+        T qscale = T(1);
+        if (ca.rescale && !first) {
+            __shared__ double redm[CHAIN_WAVES];
+            double mx = 0.0;
+            for (int a = 0; a < S; ++a)
+                for (int n = tid; n < NP; n += CHAIN_THREADS) {
+                    const double v = (double)Q[a * NPQ + n];
+                    mx = v > mx ? v : mx;
+                }
+            for (int o = 32; o > 0; o >>= 1) {
+                const double w = __shfl_xor(mx, o, 64);
+                mx = w > mx ? w : mx;
+            }
+            if (lane == 0) redm[wave] = mx;
+            __syncthreads();
+            double m = 0.0;
+            for (int w = 0; w < CHAIN_WAVES; ++w) m = redm[w] > m ? redm[w] : m;
+            if (tid == 0) {
+                const int64_t slot = scale_slot(p, ca.u, sw, s - 1, ca.nsp, ca.nmax);
+                ca.smax[slot] = (unsigned long long)__double_as_longlong(m);
+            }
+            m = (m > 0.0 && m <= 1.7976931348623157e308) ? m : 1.0;  // scale_read's rule
+            qscale = (T)(1.0 / m);
+        }
 #pragma unroll
         for (int col = 0; col < SDFT_MAX_S * SDFT_MAX_S; ++col) {
             if (col < C) {
@@ -163,7 +191,7 @@ __global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __rest
                         const int mi = tid + k * CHAIN_THREADS;
                         if (mi < XL) {
                             const int n = ext_index(mi);
-                            const T v = (first || n < 0) ? wreg[col][k] : wreg[col][k] * qb[n];
+                            const T v = (first || n < 0) ? wreg[col][k] : wreg[col][k] * (qb[n] * qscale);
                             xc[chain_phys(mi)] = v;
                         }
                     }
@@ -171,7 +199,7 @@ __global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __rest
                     for (int mi = tid; mi < XL; mi += CHAIN_THREADS) {
                         const int n = ext_index(mi);
                         T v = (cc.valid && n >= 0) ? el[n] * hh[n] * tz[n] : T(0);
-                        if (!first && n >= 0) v *= qb[n];
+                        if (!first && n >= 0) v *= qb[n] * qscale;
                         xc[chain_phys(mi)] = v;
                     }
                 }
@@ -290,7 +318,7 @@ hipError_t chain_prepare(const ChainPlanHost& ch) {
 
 template <typename T>
 void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, const T* elam, const T* hl,
-                  const T* trapez, T* gC, T* gP) {
+                  const T* trapez, T* gC, T* gP, unsigned long long* smax) {
     const SpeciesHost& sh = c.species[u];
     const ChainPlanHost& ch = c.chain_plan[u];
     ChainArgs ca;
@@ -301,6 +329,7 @@ void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, 
     for (int sw = 0; sw < 2; ++sw)
         for (int k = 0; k < SDFT_MAX_S * SDFT_MAX_S; ++k) ca.ck[sw][k] = ch.ck[sw][k];
     ca.npad_max = ch.npad_max; ca.be_max = ch.be_max;
+    ca.rescale = sh.rescale; ca.nsp = c.nsp; ca.nmax = c.nmax_seg; ca.smax = smax;
     const size_t lds = chain_lds_bytes(ch, sizeof(T));
     const int xl = ch.npad_max + 2 * ch.be_max;
     const int epc = (xl + CHAIN_THREADS - 1) / CHAIN_THREADS;
@@ -315,8 +344,8 @@ void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, 
 }
 
 template void launch_chain<float>(hipStream_t, const Ctx&, int, int, const float*, const float*, const float*,
-                                  const float*, float*, float*);
+                                  const float*, float*, float*, unsigned long long*);
 template void launch_chain<double>(hipStream_t, const Ctx&, int, int, const double*, const double*, const double*,
-                                   const double*, double*, double*);
+                                   const double*, double*, double*, unsigned long long*);
 
 }  // namespace sdft

@@ -66,8 +66,17 @@ struct DevSpecies {
     int32_t state_off;    // offset into the seg_states table ([nseg][S])
     int32_t segsum_off;   // offset into the per-point segment-sum table ([nseg][S][2])
     int32_t maxd_off;     // offset into the max-distance table ([nseg])
+    int32_t rescale;      // propagators carry per-step scale factors (long chains)
+    int32_t pad;
     double fixed_density;
 };
+
+// Chains of at least this many segments propagate with per-step rescaling: each step's input is
+// divided by the largest value of the previous step's output (per point and sweep), and the
+// logarithms of those factors are summed back in where densities and normalisations form products
+// of the two sweeps. The reference's only rescaling is the constant normative per bond, which
+// leaves 500-mer propagators outside the double range once a brush builds a potential.
+#define SDFT_RESCALE_MIN_SEG 50
 
 // System-wide constants shared by every point of a context
 struct DevSystem {

@@ -190,6 +190,9 @@ struct PropArgs {
     int64_t goff;     // field offset of this species in the gC / gP families
     int GF;           // fields per point of the gC / gP families
     int band;         // >= 0: banded Toeplitz window (FP32 phase), -1: dense
+    int rescale;      // per-step rescaling (scale_slot tables)
+    int nsp, nmax;
+    unsigned long long* smax;
 };
 
 template <typename T>
@@ -234,14 +237,19 @@ __global__ void __launch_bounds__(TB) k_prop_step(const DevSystem* __restrict__ 
     auto gidx = [&](int seg, int b, int a, int nn) -> int64_t {
         return P.base * (int64_t)A.GF + (A.goff + ((int64_t)seg * S + b) * S + a) * NP + nn;
     };
+    // input scale: 1 / the previous step's largest output (per-step rescaling)
+    const T qinv = (A.rescale && in_stored)
+                       ? (T)(1.0 / scale_read(A.smax, scale_slot(p, A.u, sweep, A.s - 1, A.nsp, A.nmax)))
+                       : T(1);
     // Q(n') = sum_b' g_in[inseg][b'][b](n')  or base
     auto qin = [&](int b, int nn) -> T {
         if (!in_stored) return base_value<T>(inseg, nn, A.fixed_seg, Kf, dz, maxd);
         T q = 0;
         for (int b2 = 0; b2 < S; ++b2)
             if (st[inseg * S + b2] >= 0) q += gin[gidx(inseg, b2, b, nn)];
-        return q;
+        return q * qinv;
     };
+    const int64_t oslot = scale_slot(p, A.u, sweep, A.s, A.nsp, A.nmax);
 
     int npair = 0;
     int pa[SDFT_MAX_S * SDFT_MAX_S], pb[SDFT_MAX_S * SDFT_MAX_S], pbond[SDFT_MAX_S * SDFT_MAX_S];
@@ -304,6 +312,7 @@ __global__ void __launch_bounds__(TB) k_prop_step(const DevSystem* __restrict__ 
                 if (d * d > maxd[outseg] * maxd[outseg]) val = 0.0;
             }
             gout[gidx(outseg, b, a, n)] = (T)val;
+            if (A.rescale) scale_note(A.smax, oslot, (double)(T)val);
         }
         return;
     }
@@ -352,16 +361,60 @@ __global__ void __launch_bounds__(TB) k_prop_step(const DevSystem* __restrict__ 
             acc[q] = a;
         }
     }
-    if (n >= NP) return;
-    for (int q = 0; q < npair; ++q) {
-        const int a = pa[q], b = pb[q], bond = pbond[q];
-        T v = absT(acc[q]) * hl[fidx(P, 2 * MB, 2 * bond + ppi[q], n)] / (T)normative;
-        if (A.fixed_seg >= 0) {
-            const double d = (double)(n - Kf) * dz;
-            if (d * d > maxd[outseg] * maxd[outseg]) v = T(0);
+    double vmax = 0.0;
+    if (n < NP) {
+        for (int q = 0; q < npair; ++q) {
+            const int a = pa[q], b = pb[q], bond = pbond[q];
+            T v = absT(acc[q]) * hl[fidx(P, 2 * MB, 2 * bond + ppi[q], n)] / (T)normative;
+            if (A.fixed_seg >= 0) {
+                const double d = (double)(n - Kf) * dz;
+                if (d * d > maxd[outseg] * maxd[outseg]) v = T(0);
+            }
+            gout[gidx(outseg, b, a, n)] = v;
+            vmax = (double)v > vmax ? (double)v : vmax;
         }
-        gout[gidx(outseg, b, a, n)] = v;
     }
+    if (A.rescale) {
+        vmax = warp_max(vmax);
+        if ((t & 63) == 0) scale_note(A.smax, oslot, vmax);
+    }
+}
+
+// --------------------------------------------------------------------------
+// per-step rescaling: E(q) = LC(q) + LP(q), the summed logarithms of the input factors divided
+// out on the way to gC[q] (child sweep step N-1-q) and gP[q] (parent sweep step q); a step's
+// stored output carries the factors of all earlier steps of its sweep. Slot nmax: E(1), the
+// grafted chain's normalisation segment.
+// --------------------------------------------------------------------------
+__global__ void k_scale_prefix(const int32_t* __restrict__ evalflag, int prec, int P, int u, int N, int fixed_seg,
+                               const unsigned long long* __restrict__ smax, double* __restrict__ lsc, int nsp,
+                               int nmax) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P || evalflag[p] != prec) return;
+    double* E = lsc + ((int64_t)p * nsp + u) * (nmax + 1);
+    double acc = 0.0;
+    for (int q = 0; q < N; ++q) {  // parent sweep: gP[q] from step q, inputs divided by steps 1..q-1
+        if (q >= 2) acc += log(scale_read(smax, scale_slot(p, u, 1, q - 1, nsp, nmax)));
+        E[q] = acc;
+    }
+    acc = 0.0;
+    for (int q = N - 1; q >= 0; --q) {  // child sweep: gC[q] from step N-1-q
+        const int s = N - 1 - q;
+        if (s >= 2) acc += log(scale_read(smax, scale_slot(p, u, 0, s - 1, nsp, nmax)));
+        E[q] += acc;
+    }
+    E[nmax] = (fixed_seg == 0 && N > 1) ? E[1] : 0.0;
+}
+
+// log of the factor restoring the unscaled product Gp(q) Qc(q) relative to the species' norm: for a
+// grafted chain norm = sigma / lambda with lambda from the stored (scaled) propagators at segment 1,
+// so the factor is exp(E(q) - E(1)); a free chain's norm normative^(N-1) is folded in as
+// (N-1) log normative + E(q) (the power alone can leave the double range)
+__device__ __forceinline__ double seg_log_scale(const DevSpecies& sp, const DevPoint& P, int p, int u, int q,
+                                                const double* __restrict__ lsc, int nsp, int nmax) {
+    const double* E = scale_row(lsc, p, u, nsp, nmax);
+    if (sp.fixed_seg >= 0) return E[q] - E[nmax];
+    return (double)(sp.nseg - 1) * log(P.normative[u]) + E[q];
 }
 
 // --------------------------------------------------------------------------
@@ -437,7 +490,8 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
                                                    const T* __restrict__ gC, const T* __restrict__ gP,
                                                    const double* __restrict__ fixed_lambda,
                                                    T* __restrict__ rho_beads, T* __restrict__ rho_bonds, T* __restrict__ rhoT,
-                                                   int Frt, double* __restrict__ acc_part, int32_t* __restrict__ errflag) {
+                                                   int Frt, double* __restrict__ acc_part, int32_t* __restrict__ errflag,
+                                                   const double* __restrict__ lsc, int nmax) {
     // A block covers ACC_POINTS grid points; its ACC_GROUPS wavefronts split the segments of every
     // species (segment q goes to group q % ACC_GROUPS), so more independent loads are in flight per
     // point; the groups' partial densities are added in group order at the end.
@@ -474,14 +528,16 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
                 norm = sp.fixed_density / lamb;
             }
         } else {
-            norm = P.chain_norm[u];
+            norm = sp.rescale ? 1.0 : P.chain_norm[u];  // rescaled: normative^(N-1) enters per segment
         }
-        const double pref = norm * exp(P.mu_species[u]);
+        const double pref0 = norm * exp(P.mu_species[u]);
         auto gidx = [&](int seg, int b, int a, int nn) -> int64_t {
             return P.base * (int64_t)GF + (goffs[u] + ((int64_t)seg * S + b) * S + a) * NP + nn;
         };
         double num = 0.0, chg = 0.0;
         for (int q = grp; q < N; q += ACC_GROUPS) {
+            const double pref = sp.rescale ? pref0 * exp(seg_log_scale(sp, P, p, u, q, lsc, sys.n_species, nmax))
+                                           : pref0;
             // every propagator value of this segment is loaded up front (independent loads in flight),
             // gcv[b][a] = gC[q][b][a] (child side, b: state of q+1), gpv[b][a] = gP[q][b][a] (parent side)
             // The loads are unconditional, from clamped (always in-bounds) indices, and masked after
@@ -1442,7 +1498,7 @@ __global__ void __launch_bounds__(256) k_segment_sums(const DevSystem* __restric
                                                      const int32_t* __restrict__ seg_states, const double* __restrict__ maxd_tab,
                                                      const T* __restrict__ elam, const T* __restrict__ gC, const T* __restrict__ gP,
                                                      const double* __restrict__ fixed_lambda, int u, double* __restrict__ out,
-                                                     int64_t out_stride) {
+                                                     int64_t out_stride, const double* __restrict__ lsc, int nmax) {
     const DevSystem& sys = *psys;
     const int p = blockIdx.y;
     if ((stt[p].prec == 64 ? 64 : 32) != (sizeof(T) == 8 ? 64 : 32)) return;
@@ -1453,8 +1509,10 @@ __global__ void __launch_bounds__(256) k_segment_sums(const DevSystem* __restric
     const int* st = seg_states + sp.state_off;
     const double* maxd = maxd_tab + sp.maxd_off;
     const int Kf = sp.fixed_seg >= 0 ? P.fixed_coord[u] : -1;
-    double norm = sp.fixed_seg >= 0 ? sp.fixed_density / fixed_lambda[(int64_t)p * SDFT_MAX_SPECIES + u] : P.chain_norm[u];
-    const double pref = norm * exp(P.mu_species[u]);
+    double norm = sp.fixed_seg >= 0 ? sp.fixed_density / fixed_lambda[(int64_t)p * SDFT_MAX_SPECIES + u]
+                                    : (sp.rescale ? 1.0 : P.chain_norm[u]);
+    double pref = norm * exp(P.mu_species[u]);
+    if (sp.rescale) pref *= exp(seg_log_scale(sp, P, p, u, q, lsc, sys.n_species, nmax));
     auto gidx = [&](int seg, int b, int a, int nn) -> int64_t {
         return P.base * (int64_t)GF + (goffs[u] + ((int64_t)seg * S + b) * S + a) * NP + nn;
     };
@@ -1500,7 +1558,8 @@ __global__ void __launch_bounds__(256) k_segment_profile(const DevSystem* __rest
                                                         const T* __restrict__ elam, const T* __restrict__ gC,
                                                         const T* __restrict__ gP,
                                                         const double* __restrict__ fixed_lambda, int u, int p,
-                                                        double* __restrict__ out) {
+                                                        double* __restrict__ out, const double* __restrict__ lsc,
+                                                        int nmax) {
     const DevSystem& sys = *psys;
     const DevPoint& P = pts[p];
     const DevSpecies sp = spec[u];
@@ -1512,8 +1571,9 @@ __global__ void __launch_bounds__(256) k_segment_profile(const DevSystem* __rest
     const double* maxd = maxd_tab + sp.maxd_off;
     const int Kf = sp.fixed_seg >= 0 ? P.fixed_coord[u] : -1;
     const double norm = sp.fixed_seg >= 0 ? sp.fixed_density / fixed_lambda[(int64_t)p * SDFT_MAX_SPECIES + u]
-                                          : P.chain_norm[u];
-    const double pref = norm * exp(P.mu_species[u]);
+                                          : (sp.rescale ? 1.0 : P.chain_norm[u]);
+    double pref = norm * exp(P.mu_species[u]);
+    if (sp.rescale) pref *= exp(seg_log_scale(sp, P, p, u, q, lsc, sys.n_species, nmax));
     auto gidx = [&](int seg, int b, int a) -> int64_t {
         return P.base * (int64_t)GF + (goffs[u] + ((int64_t)seg * S + b) * S + a) * NP + n;
     };
@@ -1586,16 +1646,25 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
     const int maxNP = c.maxNP;
     hipLaunchKernelGGL(k_prepare<T>, grid2(maxNP, 256, P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag, prec, x,
                        c.d_Ext, elam, hl);
+    if (c.any_rescale) {  // per-step maxima start at 0 every evaluation (a fill kernel: capture-safe)
+        const int64_t ns = (int64_t)P * c.nsp * 2 * c.nmax_seg;
+        hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st,
+                           reinterpret_cast<double*>(smax), ns);
+    }
     for (int u = 0; u < c.nsp; ++u) {
         const SpeciesHost& sh = c.species[u];
         PropArgs A;
         A.u = u; A.nseg = sh.nseg; A.S = sh.S; A.fixed_seg = sh.fixed_seg;
         A.state_off = sh.state_off; A.maxd_off = sh.maxd_off; A.goff = sh.goff; A.GF = c.GF;
+        A.rescale = sh.rescale; A.nsp = c.nsp; A.nmax = c.nmax_seg; A.smax = smax;
         const bool banded = (sizeof(T) == 4) ? c.band_fp32 != 0 : c.band_fp64 != 0;
         A.band = banded ? 1 : -1;
         const bool gemm = c.use_gemm && c.prop_plan[u].ntiles > 0;
         if (banded && (sizeof(T) == 4 ? c.chain_plan[u].ok32 : c.chain_plan[u].ok64)) {
-            launch_chain<T>(st, c, prec, u, kern, elam, hl, trapez, gC, gP);
+            launch_chain<T>(st, c, prec, u, kern, elam, hl, trapez, gC, gP, smax);
+            if (sh.rescale)
+                hipLaunchKernelGGL(k_scale_prefix, dim3((P + 63) / 64), dim3(64), 0, st, c.d_evalflag, prec, P, u,
+                                   sh.nseg, sh.fixed_seg, smax, lsc, c.nsp, c.nmax_seg);
             continue;
         }
         for (int s = 1; s < sh.nseg; ++s) {
@@ -1612,12 +1681,15 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
             const int skip = (fix0 && fix1) ? 2 : fix0 ? 0 : fix1 ? 1 : -1;
             if (skip != 2)
                 launch_opgemm_prop<T>(st, c, prec, c.prop_plan[u], u, s, skip, banded, kern, elam, hl, trapez, gC, gP,
-                                      fbuf);
+                                      fbuf, smax);
             if (skip >= 0)
                 hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(maxNP, 256, P, skip == 2 ? 2 : 1), dim3(256), 0, st,
                                    c.d_sys, c.d_pts, c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl,
                                    trapez, gC, gP, c.d_band, skip == 2 ? 0 : skip);
         }
+        if (sh.rescale)
+            hipLaunchKernelGGL(k_scale_prefix, dim3((P + 63) / 64), dim3(64), 0, st, c.d_evalflag, prec, P, u, sh.nseg,
+                               sh.fixed_seg, smax, lsc, c.nsp, c.nmax_seg);
         if (sh.fixed_seg >= 0) {
             hipLaunchKernelGGL(k_fixed_lambda<T>, dim3(P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag, prec, A,
                                c.d_seg_states, c.d_maxd, elam, trapez, gC, gP, c.d_fixed_lambda);
@@ -1626,7 +1698,7 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
     hipLaunchKernelGGL((k_accumulate<T, 256>), grid2(maxNP, ACC_POINTS, P), dim3(256), accumulate_lds_bytes(c), st, c.d_sys,
                        c.d_pts, c.d_evalflag,
                        prec, c.d_spec, c.d_goffs, c.GF, c.d_seg_states, c.d_maxd, elam, trapez, gC, gP,
-                       c.d_fixed_lambda, rho_beads, rho_bonds, rhoT, c.Frt, c.d_acc_part, c.d_errflag);
+                       c.d_fixed_lambda, rho_beads, rho_bonds, rhoT, c.Frt, c.d_acc_part, c.d_errflag, lsc, c.nmax_seg);
     launch_conv(st, c, prec, false, fx);
     launch_pointwise(st, c, prec);
     launch_conv(st, c, prec, true, fx);
@@ -1642,9 +1714,10 @@ void Pipeline<T>::launch_prop_step(hipStream_t st, const Ctx& c, int prec, int u
     A.state_off = sh.state_off; A.maxd_off = sh.maxd_off; A.goff = sh.goff; A.GF = c.GF;
     A.band = banded ? 1 : -1;
     A.s = s;
+    A.rescale = sh.rescale; A.nsp = c.nsp; A.nmax = c.nmax_seg; A.smax = smax;
     if (c.use_gemm && c.prop_plan[u].ntiles > 0 && sh.fixed_seg < 0)
         launch_opgemm_prop<T>(st, c, prec, c.prop_plan[u], u, s, -1, banded, kern, elam, hl, trapez, gC, gP,
-                              fbuf);
+                              fbuf, smax);
     else
         hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(c.maxNP, 256, c.P, 2), dim3(256), 0, st, c.d_sys, c.d_pts,
                            c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl, trapez, gC, gP, c.d_band, 0);
@@ -1696,7 +1769,8 @@ template <typename T>
 void Pipeline<T>::segment_sums(hipStream_t st, const Ctx& c, int u, double* out, int64_t stride) {
     const SpeciesHost& sh = c.species[u];
     hipLaunchKernelGGL(k_segment_sums<T>, dim3(sh.nseg, c.P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_state, c.d_spec,
-                       c.d_goffs, c.GF, c.d_seg_states, c.d_maxd, elam, gC, gP, c.d_fixed_lambda, u, out, stride);
+                       c.d_goffs, c.GF, c.d_seg_states, c.d_maxd, elam, gC, gP, c.d_fixed_lambda, u, out, stride, lsc,
+                       c.nmax_seg);
 }
 
 template <typename T>
@@ -1704,7 +1778,7 @@ void Pipeline<T>::segment_profile(hipStream_t st, const Ctx& c, int u, int p, do
     const SpeciesHost& sh = c.species[u];
     hipLaunchKernelGGL(k_segment_profile<T>, dim3((c.maxNP + 255) / 256, sh.nseg), dim3(256), 0, st, c.d_sys,
                        c.d_pts, c.d_spec, c.d_goffs, c.GF, c.d_seg_states, c.d_maxd, elam, gC, gP,
-                       c.d_fixed_lambda, u, p, out);
+                       c.d_fixed_lambda, u, p, out, lsc, c.nmax_seg);
 }
 
 template struct Pipeline<float>;

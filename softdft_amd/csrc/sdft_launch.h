@@ -57,7 +57,7 @@ struct ChainPlanHost {
 };
 
 struct SpeciesHost {
-    int nseg, S, model, fixed_seg, multistate;
+    int nseg, S, model, fixed_seg, multistate, rescale;
     int state_off, maxd_off;
     int64_t goff;   // field offset in the propagator families
 };
@@ -107,7 +107,29 @@ struct Ctx {
     unsigned long long* d_errbits = nullptr;
     DevSolveState* d_state = nullptr;
     int32_t* d_n_active = nullptr;  // [active, active at FP32, waiting for the FP64 switch]
+    int nmax_seg = 1;               // longest species (propagator scale tables)
+    bool any_rescale = false;       // some species propagates with per-step rescaling
 };
+
+// Per-step propagator scale factors (species with DevSpecies::rescale): smax holds, per
+// (point, species, sweep, step), the largest output value of that step as the bits of a
+// non-negative double (atomicMax on the bits orders them like the values); the next step divides its
+// input by it. lsc holds per (point, species, segment q) E(q), the summed logarithms of the factors
+// divided out of gC[q] and gP[q], and at q = nmax the value E(q*) of the grafted normalisation
+// segment: the stored product Gp(q) Qc(q) times exp(E(q)) is the unscaled one.
+__device__ __forceinline__ int64_t scale_slot(int p, int u, int sweep, int s, int nsp, int nmax) {
+    return (((int64_t)p * nsp + u) * 2 + sweep) * nmax + s;
+}
+__device__ __forceinline__ double scale_read(const unsigned long long* smax, int64_t i) {
+    const double m = __longlong_as_double((long long)smax[i]);
+    return (m > 0.0 && m <= 1.7976931348623157e308) ? m : 1.0;
+}
+__device__ __forceinline__ void scale_note(unsigned long long* smax, int64_t i, double v) {
+    if (v > 0.0) atomicMax(smax + i, (unsigned long long)__double_as_longlong(v));
+}
+__device__ __forceinline__ const double* scale_row(const double* lsc, int p, int u, int nsp, int nmax) {
+    return lsc + ((int64_t)p * nsp + u) * (nmax + 1);
+}
 
 // stores one convolution column's value y at grid point n of point P (ConvJob semantics)
 template <typename T>
@@ -141,6 +163,8 @@ struct Pipeline {
     T* nw = nullptr;
     T* dphi = nullptr;
     T* fbuf = nullptr;      // chained propagator inputs, two step buffers of (2 sweeps x S x S) fields
+    unsigned long long* smax = nullptr;  // per-step scale factors (see scale_slot)
+    double* lsc = nullptr;               // summed log scales per segment (see scale_row)
     void objective(hipStream_t st, const Ctx& c, int prec, const double* x, double* fx);
     void omega(hipStream_t st, const Ctx& c, const double* x, double* omega_acc);
     void segment_sums(hipStream_t st, const Ctx& c, int u, double* out, int64_t stride);
@@ -159,11 +183,11 @@ size_t chain_lds_bytes(const ChainPlanHost& ch, size_t elem);
 hipError_t chain_prepare(const ChainPlanHost& ch);
 template <typename T>
 void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, const T* elam, const T* hl,
-                  const T* trapez, T* gC, T* gP);
+                  const T* trapez, T* gC, T* gP, unsigned long long* smax);
 template <typename T>
 void launch_opgemm_prop(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, int u, int s, int skip_sweep,
                         bool banded, const T* kern, const T* elam, const T* hl, const T* trapez, T* gC, T* gP,
-                        T* fbuf);
+                        T* fbuf, unsigned long long* smax);
 int opgemm_tile_rows(bool wide);
 int conv_tile_rows(bool wide);   // rows per tile of the convolution GEMMs
 int opgemm_tile_cols(bool wide);

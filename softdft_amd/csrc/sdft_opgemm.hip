@@ -44,7 +44,10 @@ struct PropStep {       // per-launch propagator parameters
     int32_t skip_sweep; // sweep whose parent is grafted at this step (handled elsewhere), or -1
     int32_t banded;     // apply the group's band half-width
     int32_t fused;      // inputs of step s >= 2 come from fin; the epilogue writes fout for step s + 1
-    int32_t pad;
+    int32_t rescale;    // per-step rescaling: inputs from step s - 1 divided by its recorded maximum,
+                        // this step's maximum recorded (scale_slot)
+    int32_t nsp, nmax;
+    unsigned long long* smax;
 };
 
 template <typename T, int WM, int WN, bool PROP>
@@ -78,6 +81,7 @@ __global__ void __launch_bounds__(256) k_opgemm(const DevSystem* __restrict__ ps
     __shared__ int32_t colp[TN];        // point of each column (-1 inactive)
     __shared__ int32_t colaux[TN][4];   // MODE_CONV: job; MODE_PROP: sweep, a, b, bond
     __shared__ int32_t colpp[TN][2];    // MODE_PROP: pi, pj
+    __shared__ T colinv[TN];            // MODE_PROP: input scale of the column (1 / previous step's maximum)
     __shared__ int any_active;
 
     if (tid == 0) any_active = 0;
@@ -110,6 +114,12 @@ __global__ void __launch_bounds__(256) k_opgemm(const DevSystem* __restrict__ ps
                 colaux[tid][3] = bond;
                 colpp[tid][0] = si <= sj ? 0 : 1;
                 colpp[tid][1] = si <= sj ? 1 : 0;
+                // the input comes from step s - 1 when the input segment carries a stored propagator
+                const bool stored = c.sweep == 0 ? (inseg < N - 1) : (inseg > 0);
+                T inv = T(1);
+                if (p >= 0 && ps.rescale && stored)
+                    inv = (T)(1.0 / scale_read(ps.smax, scale_slot(p, ps.u, c.sweep, ps.s - 1, ps.nsp, ps.nmax)));
+                colinv[tid] = inv;
             }
             if (p >= 0 && evalflag[p] != prec) p = -1;
         }
@@ -164,7 +174,8 @@ __global__ void __launch_bounds__(256) k_opgemm(const DevSystem* __restrict__ ps
                         } else if (ps.fused && ps.s >= 2) {
                             const DevPoint& P = pts[sp_];
                             const int sweep = colaux[scol][0], a = colaux[scol][1], b = colaux[scol][2];
-                            v = fin[P.base * (int64_t)(2 * ps.S * ps.S) + (((int64_t)sweep * ps.S + a) * ps.S + b) * P.NP + nn];
+                            v = fin[P.base * (int64_t)(2 * ps.S * ps.S) + (((int64_t)sweep * ps.S + a) * ps.S + b) * P.NP + nn] *
+                                colinv[scol];
                         } else {
                             const DevPoint& P = pts[sp_];
                             const int sweep = colaux[scol][0], b = colaux[scol][2], bond = colaux[scol][3];
@@ -196,8 +207,8 @@ __global__ void __launch_bounds__(256) k_opgemm(const DevSystem* __restrict__ ps
                             }
                             const int64_t eb = P.base * (int64_t)ps.NB;
                             v = elam[eb + (int64_t)sj * P.NP + nn] *
-                                hl[P.base * (int64_t)(2 * ps.MB) + (int64_t)(2 * bond + colpp[scol][1]) * P.NP + nn] * q_in *
-                                trapez[eb + (int64_t)sj * P.NP + nn];
+                                hl[P.base * (int64_t)(2 * ps.MB) + (int64_t)(2 * bond + colpp[scol][1]) * P.NP + nn] *
+                                (q_in * colinv[scol]) * trapez[eb + (int64_t)sj * P.NP + nn];
                         }
                     }
                     xs[sk0 + q][scol] = v;
@@ -249,6 +260,7 @@ __global__ void __launch_bounds__(256) k_opgemm(const DevSystem* __restrict__ ps
                 const int N = ps.nseg, S = ps.S;
                 const int outseg = sweep == 0 ? N - 1 - ps.s : ps.s;
                 const DevPoint& P = pts[p >= 0 ? p : 0];
+                double vmax = 0.0;  // this lane's largest output (per-step rescaling)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int n = r0 + wave * WM * 16 + i * 16 + Mfma<T>::row(lane, v);
@@ -267,6 +279,7 @@ __global__ void __launch_bounds__(256) k_opgemm(const DevSystem* __restrict__ ps
                         }
                         T* gout = sweep == 0 ? gC : gP;
                         gout[P.base * (int64_t)ps.GF + ps.goff + (((int64_t)outseg * S + b) * S + a) * P.NP + n] = val;
+                        vmax = (double)val > vmax ? (double)val : vmax;
                     }
                     if (ps.fused) {
                         // Q(a) = sum_b g[outseg][b][a]: the S columns of one (point, sweep, a) are adjacent lanes
@@ -290,6 +303,15 @@ __global__ void __launch_bounds__(256) k_opgemm(const DevSystem* __restrict__ ps
                             }
                         }
                     }
+                }
+                if (ps.rescale) {
+                    // lanes l, l ^ 16, l ^ 32, l ^ 48 hold rows of the same column: one atomic per column
+                    for (int o = 16; o < 64; o <<= 1) {
+                        const double w = __shfl_xor(vmax, o, 64);
+                        vmax = w > vmax ? w : vmax;
+                    }
+                    if (lane < 16 && p >= 0)
+                        scale_note(ps.smax, scale_slot(p, ps.u, sweep, ps.s, ps.nsp, ps.nmax), vmax);
                 }
             }
         }
@@ -569,7 +591,7 @@ void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& pl
 template <typename T>
 void launch_opgemm_prop(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, int u, int s, int skip_sweep,
                         bool banded, const T* kern, const T* elam, const T* hl, const T* trapez, T* gC, T* gP,
-                        T* fbuf) {
+                        T* fbuf, unsigned long long* smax) {
     if (plan.ntiles == 0) return;
     const SpeciesHost& sh = c.species[u];
     PropStep ps;
@@ -577,6 +599,7 @@ void launch_opgemm_prop(hipStream_t st, const Ctx& c, int prec, const OpPlan& pl
     ps.u = u; ps.s = s; ps.nseg = sh.nseg; ps.S = sh.S; ps.fixed_seg = sh.fixed_seg;
     ps.state_off = sh.state_off; ps.maxd_off = sh.maxd_off; ps.GF = c.GF; ps.MB = c.MB; ps.NB = c.NB;
     ps.goff = sh.goff; ps.skip_sweep = skip_sweep; ps.banded = banded ? 1 : 0;
+    ps.rescale = sh.rescale; ps.nsp = c.nsp; ps.nmax = c.nmax_seg; ps.smax = smax;
     // fused input chaining: homogeneous, ungrafted chains whose every (sweep, a, b) column is present
     ps.fused = (plan.full_pairs && sh.fixed_seg < 0 && fbuf != nullptr) ? 1 : 0;
     const int64_t fstride = c.sumNP * 2 * sh.S * sh.S;
@@ -599,9 +622,11 @@ template void launch_opgemm_conv<float>(hipStream_t, const Ctx&, int, const OpPl
 template void launch_opgemm_conv<double>(hipStream_t, const Ctx&, int, const OpPlan&, const double*, const double*,
                                          const double*, double*, double*);
 template void launch_opgemm_prop<float>(hipStream_t, const Ctx&, int, const OpPlan&, int, int, int, bool, const float*,
-                                        const float*, const float*, const float*, float*, float*, float*);
+                                        const float*, const float*, const float*, float*, float*, float*,
+                                        unsigned long long*);
 template void launch_opgemm_prop<double>(hipStream_t, const Ctx&, int, const OpPlan&, int, int, int, bool, const double*,
-                                         const double*, const double*, const double*, double*, double*, double*);
+                                         const double*, const double*, const double*, double*, double*, double*,
+                                         unsigned long long*);
 
 // row/column tile sizes of the two shapes (for the host planner)
 int opgemm_tile_rows(bool wide) { return wide ? TileShape<1, 4>::TM : TileShape<2, 1>::TM; }

@@ -493,3 +493,26 @@ def test_monomer_order_and_wall_indices():
     # lo = round(0.5/0.02) + 1 = 26 (1-based), hi = round(3.5/0.02) + 1 = 176
     assert sp.wall_lo[0] == 25 and sp.wall_hi[0] == 175
     assert sp.trapez[25, 0] == 0.5 and sp.Ext[24, 0] == 1e20 and sp.Ext[25, 0] == 0.0
+
+
+def test_bulk_chain_log_domain_matches_direct_and_normalises():
+    """The overflow fallback of the bulk chain recursion equals the direct products where those are
+    finite, and for a 500-mer at pH 9 (where the direct products overflow) every segment's state
+    probabilities still sum to 1."""
+    from softdft_amd import bulk as bk
+    ms = build_molsys(configs.weak_polyelectrolyte(H=4.0, dz=0.05, n_seg=20, pH=7.0, solvent=0.2))
+    rng = np.random.default_rng(3)
+    mu_ex = 0.3 * rng.standard_normal(ms.NB)
+    lng = 0.1 * rng.standard_normal(ms.MB)
+    Xi, seg, bonds = bk._chain_bulk_direct(ms, 0, mu_ex, lng)
+    lX, seg2, bonds2 = bk._chain_bulk_log(ms, 0, mu_ex, lng)
+    assert lX == pytest.approx(math.log(Xi), rel=1e-13)
+    assert np.allclose(seg2, seg, rtol=1e-12, atol=0)
+    assert np.allclose(bonds2, bonds, rtol=1e-12, atol=0)
+    ms5 = build_molsys(configs.weak_polyelectrolyte(H=4.0, dz=0.05, n_seg=500, pH=9.0, pKa=5.0, grafted=0.05))
+    with np.errstate(over="ignore", invalid="ignore"):
+        Xd, _, _ = bk._chain_bulk_direct(ms5, 0, np.zeros(ms5.NB), np.zeros(ms5.MB))
+    assert not math.isfinite(Xd)
+    X5, seg5, b5, lX5 = bk._chain_bulk(ms5, 0, np.zeros(ms5.NB), np.zeros(ms5.MB))
+    assert math.isfinite(lX5) and np.all(np.isfinite(seg5))
+    assert np.allclose(seg5.sum(axis=0), 1.0, rtol=1e-12)
