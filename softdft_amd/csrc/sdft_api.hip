@@ -1,6 +1,7 @@
 // C ABI of libsoftdft_hip: context creation, objective, Anderson solve,
 // observables.  Host orchestration only; all grid work runs in the kernels of
 // sdft_kernels.hip on one HIP stream per context.
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -968,6 +969,11 @@ extern "C" int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* o, doub
         if (run64) launch_compact_cols(ctx->stream, c, 64);
     };
     compact();
+    // optional progress lines on stderr (SDFT_PROGRESS = seconds between lines), written at polls
+    const char* prog_env = getenv("SDFT_PROGRESS");
+    const double prog_every = prog_env ? atof(prog_env) : 0.0;
+    auto prog_t0 = std::chrono::steady_clock::now();
+    auto prog_last = prog_t0;
     int it = 0, n32 = 0, n64 = 0;
     for (it = 0; it < o->max_iters; ++it) {
         run_objective(ctx, run32, run64);
@@ -977,6 +983,22 @@ extern "C" int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* o, doub
         if ((it + 1) % check == 0 || it + 1 == o->max_iters) {
             HIP_CHECK_RET(hipMemcpyAsync(counts, c.d_n_active, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
             HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+            if (prog_every > 0.0) {
+                const auto now = std::chrono::steady_clock::now();
+                if (std::chrono::duration<double>(now - prog_last).count() >= prog_every) {
+                    prog_last = now;
+                    HIP_CHECK_RET(hipMemcpyAsync(st.data(), c.d_state, st.size() * sizeof(DevSolveState),
+                                                 hipMemcpyDeviceToHost, ctx->stream));
+                    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+                    double emax = 0.0;
+                    for (const DevSolveState& q : st)
+                        if (!q.converged) emax = q.err > emax || q.err != q.err ? q.err : emax;
+                    fprintf(stderr, "sdft_solve: iteration %d, %d of %d points active (%d in FP32), max residual %.3e, %.1f s\n",
+                            it + 1, counts[0], c.P, counts[1] - counts[2], emax,
+                            std::chrono::duration<double>(now - prog_t0).count());
+                    fflush(stderr);
+                }
+            }
             if (counts[0] == 0) break;
             if (counts[2] > 0) launch_apply_switch(ctx->stream, c);
             run32 = counts[1] - counts[2] > 0;

@@ -15,7 +15,8 @@ mirror image on the right half, which is the reference's copy rule (adjust_dista
 PsiC is copied (adjust_distanceZ.jl:90).
 
 The pH continuation (routines/adjust_pH.jl:16-124) steps the pSpecies of H+ ("Y"), warm-starts each pH
-from the previous one with the same copy rule (adjust_pH.jl:50-80, on a fixed grid), and records per
+from the previous one (on the fixed grid the solution vector is copied as is, see copy_solution), and
+records per
 step the brush height of the grafted species (brush_height.jl) and the average degree of ionization
 (the `avg` state-2 column of total_state_fractions, degree_of_dissociation.jl), stopping once alpha
 reaches 0.99 on an increasing sweep (0.01 on a decreasing one, adjust_pH.jl:107-113).
@@ -75,6 +76,7 @@ class WidthResult:
     omega: float
     iterations: int
     warm: bool
+    residual: float = 0.0  # ||F(x) - x||_inf at the end (above tole: not converged within max_iters)
 
 
 def width_sweep(datasets: Sequence[dict], *, waves: int = 8, precision: int = abi.PREC_MIXED,
@@ -89,33 +91,18 @@ def width_sweep(datasets: Sequence[dict], *, waves: int = 8, precision: int = ab
     opts = dict(tole=num["tole"], damping=num["damping"], mixing_max=num["mixing_max"],
                 switch_tole=num["switch_tole"], anderm=int(num["anderm"]), max_iters=max_iters,
                 precision=precision)
-    xs: List[Optional[np.ndarray]] = [None] * n
     res: List[Optional[WidthResult]] = [None] * n
-    waves = max(1, min(waves, n))
-    for r in range(waves):
-        idx = list(range(r, n, waves))
-        if not idx:
-            continue
-        batch = abi.Batch([sps[i] for i in idx])
-        x0 = []
-        for i in idx:
-            if r > 0:
-                x0.append(remap_solution(sps[i - 1], xs[i - 1], sps[i]))
-            else:
-                x0.append(sps[i].x0())
-        x, it, err = batch.solve(np.concatenate(x0), **opts)
+
+    def finish(batch, idx, x, it, err, warm):
         om, _, pc = batch.observables()
-        parts = batch.split(x)
         for k, i in enumerate(idx):
-            xs[i] = parts[k].copy()
             sp = sps[i]
             p_bulk = float(sp.bulk.pressure)
             res[i] = WidthResult(step=i + 1, distance=float(sp.H), pressure=float(pc[k, 0]),
                                  delta_p=float(pc[k, 0] - p_bulk), energy=float(om[k] + p_bulk * sp.H),
-                                 omega=float(om[k]), iterations=int(it[k]), warm=r > 0)
-        batch.close()
-        if on_round is not None:
-            on_round(r, float(np.max(err)))
+                                 omega=float(om[k]), iterations=int(it[k]), warm=warm, residual=float(err[k]))
+
+    _waves_solve(sps, waves, opts, on_round, finish)
     return [r for r in res if r is not None]
 
 
@@ -146,9 +133,19 @@ class PHResult:
     omega: float
     iterations: int
     warm: bool
+    residual: float = 0.0  # ||F(x) - x||_inf at the end (above tole: not converged within max_iters)
 
 
-def _waves_solve(sps, waves, opts, on_round, finish):
+def copy_solution(sp_old: StatePoint, x_old: np.ndarray, sp_new: StatePoint) -> np.ndarray:
+    """Warm start on an unchanged grid: the converged solution vector itself. The pH driver's mirror
+    copy (adjust_pH.jl:50-80) assumes a profile symmetric about the slit centre, which a brush grafted
+    to one wall is not, and the driver's stale solver call never reads the copied densities."""
+    if sp_old.NP != sp_new.NP or sp_old.NB != sp_new.NB or sp_old.MB != sp_new.MB:
+        return remap_solution(sp_old, x_old, sp_new)
+    return np.array(x_old, dtype=np.float64, copy=True)
+
+
+def _waves_solve(sps, waves, opts, on_round, finish, warm=remap_solution):
     """Solve `sps` in `waves` batched rounds, point i warm-started from point i - 1 (cold in round 0);
     finish(batch, idx, x_parts, iters, warm) is called per round while the context is alive."""
     n = len(sps)
@@ -159,12 +156,12 @@ def _waves_solve(sps, waves, opts, on_round, finish):
         if not idx:
             continue
         batch = abi.Batch([sps[i] for i in idx])
-        x0 = [remap_solution(sps[i - 1], xs[i - 1], sps[i]) if r > 0 else sps[i].x0() for i in idx]
-        x, it, err = batch.solve(np.concatenate(x0), **opts)
+        x0 = [warm(sps[i - 1], xs[i - 1], sps[i]) if r > 0 else sps[i].x0() for i in idx]
+        x, it, err = batch.solve(np.concatenate(x0), raise_on_noconv=False, **opts)
         parts = batch.split(x)
         for k, i in enumerate(idx):
             xs[i] = parts[k].copy()
-        finish(batch, idx, x, it, r > 0)
+        finish(batch, idx, x, it, err, r > 0)
         batch.close()
         if on_round is not None:
             on_round(r, float(np.max(err)))
@@ -183,7 +180,7 @@ def ph_sweep(datasets: Sequence[dict], pH_values: Sequence[float], *, waves: int
                 precision=precision)
     res: List[Optional[PHResult]] = [None] * n
 
-    def finish(batch, idx, x, it, warm):
+    def finish(batch, idx, x, it, err, warm):
         om, al, _ = batch.observables()
         for k, i in enumerate(idx):
             sp = sps[i]
@@ -193,9 +190,10 @@ def ph_sweep(datasets: Sequence[dict], pH_values: Sequence[float], *, waves: int
             alpha = float(al[k, multi[0], 1]) if multi else math.nan
             res[i] = PHResult(step=i + 1, pH=float(pH_values[i]),
                               brush_height=float(hs[0][1]) if len(hs) == 1 else math.nan,
-                              alpha=alpha, omega=float(om[k]), iterations=int(it[k]), warm=warm)
+                              alpha=alpha, omega=float(om[k]), iterations=int(it[k]), warm=warm,
+                              residual=float(err[k]))
 
-    _waves_solve(sps, waves, opts, on_round, finish)
+    _waves_solve(sps, waves, opts, on_round, finish, warm=copy_solution)
     out = [r for r in res if r is not None]
     increasing = n < 2 or pH_values[-1] >= pH_values[0]
     for k, r in enumerate(out):

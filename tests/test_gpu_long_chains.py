@@ -53,3 +53,23 @@ def test_rescaled_brush_solve_matches_oracle():
         assert err[0] < num["tole"]
         assert abs(om[0] - om_ref) < 1e-8 * max(1.0, abs(om_ref))
     b.close()
+
+
+@pytest.mark.parametrize("pH,salt", [(3.0, 0.001), (9.0, 0.1)])
+def test_500mer_brush_objective_matches_oracle(pH, salt):
+    """The C5 chain (500-mer, grafting 0.05, pKa 5, H = 81.9149) on a coarser grid (dz = 0.05) where the
+    oracle runs in seconds; pH 9 at 100 mM is the case whose unscaled propagators leave the double
+    range inside a brush."""
+    ds = configs.weak_polyelectrolyte(H=81.9149, dz=0.05, n_seg=500, pKa=5.0, pH=pH, salt_M=salt, grafted=0.05)
+    sp = build_state_point(ds)
+    b = abi.Batch([sp])
+    o = OracleDFT(sp)
+    x = b.x0()
+    print(f"case 500mer objective pH={pH}", flush=True)
+    ref = o.objective(x)
+    scale = max(1.0, float(np.max(np.abs(ref))))
+    fx = b.objective(x, abi.PREC_FP64)
+    assert np.max(np.abs(fx - ref)) / scale < 1e-11
+    fx32 = b.objective(x, abi.PREC_FP32)
+    assert np.max(np.abs(fx32 - ref)) / scale < 5e-5
+    b.close()
