@@ -58,6 +58,196 @@ def traffic_from_profiles(kernel_name):
     return None
 
 
+# ---------------------------------------------------------------------------------------------
+# per-config reports (`--config c1|c2|c4|c5`): one JSON line each, beside (not instead of) the
+# benchmark line; the errors are against the FP64 paths and the oracle, the CPU figure is the NumPy
+# oracle on one host core (a bounded sample where a full CPU solve would take minutes)
+# ---------------------------------------------------------------------------------------------
+def _finite(v):
+    """JSON-safe copy: NaN / Inf (quantities a config does not have) become null."""
+    import math
+    if isinstance(v, float):
+        return v if math.isfinite(v) else None
+    if isinstance(v, dict):
+        return {k: _finite(x) for k, x in v.items()}
+    if isinstance(v, (list, tuple)):
+        return [_finite(x) for x in v]
+    return v
+
+
+def _solve_opts(num, **kw):
+    d = dict(tole=num["tole"], damping=num["damping"], mixing_max=num["mixing_max"], switch_tole=num["switch_tole"],
+             anderm=int(num["anderm"]), max_iters=5000)
+    d.update(kw)
+    return d
+
+
+def _adsorption(sp, rho):
+    """Excess adsorption per bead type: sum_K (rho_j(K) - rho_j^bulk) trapez_j(K) dz."""
+    import numpy as np
+    return np.sum((rho - sp.bulk.rho_beads[None, :]) * sp.trapez[:sp.NP], axis=0) * sp.dz
+
+
+def _timed_solve(batch, x0, prec, opts):
+    t0 = time.perf_counter()
+    x, it, err = batch.solve(x0, precision=prec, raise_on_noconv=False, **opts)
+    return x, it, err, time.perf_counter() - t0
+
+
+def _oracle_eval_time(sp, x, reps=2):
+    from oracle.dft_oracle import OracleDFT
+    o = OracleDFT(sp)
+    o.objective(x)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        o.objective(x)
+    return (time.perf_counter() - t0) / reps, o
+
+
+def report_single(name, ds, cpu_full):
+    """C1 / C2: one state point. Plain FP64 GPU solve, mixed solve, CPU oracle (full solve or a bounded
+    sample of objective evaluations scaled by the FP64 iteration count)."""
+    import numpy as np
+    from softdft_amd import abi
+    from softdft_amd.problem import build_state_point
+    from oracle.dft_oracle import anderson_solve
+    sp = build_state_point(ds)
+    b = abi.Batch([sp])
+    opts = _solve_opts(sp.numerics)
+    x64, it64, e64, t64 = _timed_solve(b, b.x0(), abi.PREC_FP64, opts)
+    om64, al64, _ = b.observables()
+    rho64 = b.get_field(abi.FIELD_RHO_BEADS, 0)
+    xm, itm, em, tm = _timed_solve(b, b.x0(), abi.PREC_MIXED, opts)
+    omm, alm, _ = b.observables()
+    rhom = b.get_field(abi.FIELD_RHO_BEADS, 0)
+    b.close()
+    log(f"{name}: FP64 {it64[0]} iterations {t64:.2f} s, mixed {itm[0]} iterations {tm:.2f} s")
+    teval, o = _oracle_eval_time(sp, x64)
+    fo = o.objective(x64)
+    om_oracle = o.grand_potential(x64)
+    res_oracle = float(np.max(np.abs(fo - x64)))
+    rho_or = o.rho_beads
+    if cpu_full:
+        t0 = time.perf_counter()
+        r = anderson_solve(o.objective, sp.x0(), sp.numerics)
+        tcpu = time.perf_counter() - t0
+        cpu = {"value": 1.0 / tcpu, "unit": "state points/s", "cores": 1, "kind": "port",
+               "sample": f"full oracle solve, {r.iters} iterations in {tcpu:.1f} s"}
+    else:
+        tcpu = teval * int(it64[0])
+        cpu = {"value": 1.0 / tcpu, "unit": "state points/s", "cores": 1, "kind": "port",
+               "sample": f"3 oracle objective evaluations ({teval:.3f} s each) x the {int(it64[0])} iterations of the "
+                         f"FP64 solve (estimate)"}
+    scale = float(np.max(np.abs(rho64)))
+    out = {
+        "report": "config", "config": name, "NP": sp.NP,
+        "fp64_gpu": {"value": 1.0 / t64, "unit": "state points/s", "iterations": int(it64[0]), "residual": float(e64[0])},
+        "mixed_gpu": {"value": 1.0 / tm, "unit": "state points/s", "iterations": int(itm[0]), "residual": float(em[0])},
+        "cpu_baseline": cpu,
+        "speedup_mixed_vs_fp64_gpu": t64 / tm, "speedup_mixed_vs_cpu": tcpu / tm,
+        "errors": {
+            "omega_fp64_gpu": float(om64[0]), "omega_mixed_rel_vs_fp64": float(abs(omm[0] - om64[0]) / abs(om64[0])),
+            "omega_fp64_rel_vs_oracle_at_same_x": float(abs(om64[0] - om_oracle) / abs(om_oracle)),
+            "oracle_residual_at_gpu_fp64_solution": res_oracle,
+            "density_max_rel_mixed_vs_fp64": float(np.max(np.abs(rhom - rho64)) / scale),
+            "density_max_rel_fp64_vs_oracle": float(np.max(np.abs(rho64 - rho_or)) / scale),
+            "adsorption_fp64": _adsorption(sp, rho64).tolist(),
+            "adsorption_max_abs_mixed_vs_fp64": float(np.max(np.abs(_adsorption(sp, rhom) - _adsorption(sp, rho64)))),
+            "ionization_max_abs_mixed_vs_fp64": float(np.max(np.abs(alm - al64))),
+        },
+    }
+    print(json.dumps(_finite(out)), flush=True)
+
+
+def report_c4(args):
+    """C4: 512 slit widths H = 4..40 (NP 201..2001) with neighbour warm starts, mixed precision, in waves."""
+    import numpy as np
+    from softdft_amd import abi, configs
+    from softdft_amd.problem import build_state_points
+    from softdft_amd.sweeps import width_sweep
+    dss = configs.config4(args.points if args.points != 256 else 512)
+    t0 = time.perf_counter()
+    res = width_sweep(dss, waves=8, precision=abi.PREC_MIXED, workers=12,
+                      on_round=lambda r, e: log(f"c4 round {r}: max residual {e:.2e}"))
+    tsw = time.perf_counter() - t0
+    n = len(res)
+    # plain FP64 one width at a time (sample) and the same widths in one batched cold FP64 solve
+    sel = np.linspace(0, n - 1, 4).astype(int)
+    sps = build_state_points([dss[i] for i in sel])
+    t0 = time.perf_counter()
+    for sp in sps:
+        b1 = abi.Batch([sp])
+        b1.solve(b1.x0(), precision=abi.PREC_FP64, raise_on_noconv=False, **_solve_opts(sp.numerics))
+        b1.close()
+    t1 = (time.perf_counter() - t0) / len(sps)
+    b = abi.Batch(sps)
+    x, it, err = b.solve(b.x0(), precision=abi.PREC_FP64, raise_on_noconv=False, **_solve_opts(sps[0].numerics))
+    om, _, pc = b.observables()
+    b.close()
+    dom = [abs(res[i].omega - om[k]) / max(1.0, abs(om[k])) for k, i in enumerate(sel)]
+    dpi = [abs(res[i].delta_p - (pc[k, 0] - sps[k].bulk.pressure)) for k, i in enumerate(sel)]
+    teval, _ = _oracle_eval_time(sps[len(sps) // 2], x[b.offsets[len(sps) // 2]:b.offsets[len(sps) // 2 + 1]])
+    mean_it = float(np.mean([r.iterations for r in res]))
+    out = {"report": "config", "config": "C4", "widths": n, "NP_range": [min(s.NP for s in sps), max(s.NP for s in sps)],
+           "mixed_gpu_sweep": {"value": n / tsw, "unit": "state points/s", "seconds": tsw, "mean_iterations": mean_it,
+                               "unconverged": int(sum(r.residual >= 1e-8 for r in res))},
+           "fp64_gpu_one_at_a_time": {"value": 1.0 / t1, "unit": "state points/s", "sample_points": len(sps)},
+           "speedup_vs_fp64_one_at_a_time": (n / tsw) * t1,
+           "cpu_baseline": {"value": 1.0 / (teval * mean_it), "unit": "state points/s", "cores": 1, "kind": "port",
+                            "sample": f"3 oracle objective evaluations at NP={sps[len(sps) // 2].NP} "
+                                      f"({teval:.3f} s each) x the sweep's mean {mean_it:.0f} iterations (estimate)"},
+           "errors": {"omega_max_rel_sweep_vs_cold_fp64": float(max(dom)), "pi_max_abs_sweep_vs_cold_fp64": float(max(dpi))},
+           "pi_curve": [[r.distance, r.delta_p] for r in res[::16]]}
+    print(json.dumps(_finite(out)), flush=True)
+
+
+def report_c5(args):
+    """C5: 128 brush points at full size (NP 16384, 500-mers). Per-iteration throughput of the batch in
+    FP32 and FP64 (banded, the mixed path's polish) and the plain FP64 (dense) evaluation of one point;
+    convergence of the brush is reported from the solves themselves."""
+    import numpy as np
+    from softdft_amd import abi, configs
+    from softdft_amd.problem import build_state_points
+    dss = configs.config5()
+    t0 = time.perf_counter()
+    sps = build_state_points(dss, workers=12)
+    log(f"c5: {len(sps)} state points set up in {time.perf_counter() - t0:.1f} s")
+    b = abi.Batch(sps)
+    x0 = b.x0()
+    res = {}
+    for prec, label in ((abi.PREC_FP32, "fp32"), (abi.PREC_FP64, "fp64_banded")):
+        b.set_operator_band(1, 1)
+        b.objective(x0, prec)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            b.objective(x0, prec)
+        res[label] = (time.perf_counter() - t0) / 3
+        log(f"c5 {label}: {res[label] * 1e3:.1f} ms per batch evaluation")
+    b.close()
+    b1 = abi.Batch([sps[0]])
+    b1.objective(b1.x0(), abi.PREC_FP64)
+    t0 = time.perf_counter()
+    b1.objective(b1.x0(), abi.PREC_FP64)
+    t_plain = time.perf_counter() - t0
+    b1.close()
+    # converged solves of the first pH column (pH 3, every salt), mixed precision, batched
+    def ph_of(d):
+        return next(e["pSpecies"] for e in d["species"] if e["sequence"] == "Y")
+    col = [i for i in range(len(sps)) if abs(ph_of(dss[i]) - 3.0) < 1e-9]
+    bs = abi.Batch([sps[i] for i in col])
+    x, it, err, ts = _timed_solve(bs, bs.x0(), abi.PREC_MIXED, _solve_opts(sps[0].numerics, max_iters=3000))
+    bs.close()
+    out = {"report": "config", "config": "C5", "points": len(sps), "NP": sps[0].NP, "n_seg": 500,
+           "batch_evaluation_ms": {"fp32": res["fp32"] * 1e3, "fp64_banded": res["fp64_banded"] * 1e3},
+           "batch_iterations_per_s": {"fp32": 1.0 / res["fp32"], "fp64_banded": 1.0 / res["fp64_banded"]},
+           "plain_fp64_dense_one_point_evaluation_ms": t_plain * 1e3,
+           "per_point_evaluation_speedup_fp32_batch_vs_plain_fp64": t_plain / (res["fp32"] / len(sps)),
+           "pH3_column_mixed_solve": {"points": len(col), "seconds": ts, "iterations": it.tolist(),
+                                      "converged": int(np.sum(err < 1e-8)), "value": float(np.sum(err < 1e-8)) / ts,
+                                      "unit": "state points/s"}}
+    print(json.dumps(_finite(out)), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +261,8 @@ def main():
     ap.add_argument("--kernel-only", choices=["chain32", "chain64", "fwd32", "fwd64", "back32", "back64"],
                     help="after one full-load evaluation, only re-launch this kernel class 21 times "
                          "(for rocprofv3 --kernel-trace / --pmc runs of that kernel)")
+    ap.add_argument("--config", choices=["c1", "c2", "c4", "c5"],
+                    help="per-config report (one JSON line) instead of the benchmark line")
     ap.add_argument("--solve-only", action="store_true",
                     help="stop after the timed mixed solves (for rocprofv3 --kernel-trace of the solve itself)")
     args = ap.parse_args()
@@ -82,12 +274,24 @@ def main():
                                    "communication, but only the single-GPU figure is measured"}))
         sys.exit(2)
 
+    if args.config:
+        from softdft_amd import configs
+        if args.config == "c1":
+            report_single("C1", configs.config1(), cpu_full=True)
+        elif args.config == "c2":
+            report_single("C2", configs.config2(), cpu_full=False)
+        elif args.config == "c4":
+            report_c4(args)
+        else:
+            report_c5(args)
+        return
+
     import numpy as np
     from softdft_amd import abi, configs
-    from softdft_amd.problem import build_state_point
+    from softdft_amd.problem import build_state_points
 
     t = time.time()
-    sps = [build_state_point(d) for d in configs.config3(args.points)]
+    sps = build_state_points(configs.config3(args.points), workers=8)
     log(f"setup: {len(sps)} state points in {time.time() - t:.1f} s")
     batch = abi.Batch(sps)
     x0 = batch.x0()

@@ -25,6 +25,13 @@ RHO_SOLVENT = (1.0 - POLY_FRAC) * ETA_TOTAL * 6.0 / math.pi
 # from the first full history with a small mixing step.
 SCF_NUMERICS = {"damping": 0.1, "mixing_max": 0.1, "switch_tole": 1e-3, "tole": 1e-8, "anderm": 5}
 SCF_NUMERICS_CHARGED = {"damping": 0.1, "mixing_max": 0.02, "switch_tole": 100.0, "tole": 1e-8, "anderm": 5}
+# The 500-mer brush starts ~10^2-10^4 away from its fixed point; with the charged settings above the
+# Anderson residual stalls at O(10) (the oracle does the same). Anderson from the first full history with
+# a 0.01 step converges the uncharged and weakly charged brushes (pH <= 5.4 at 100 mM, measured).
+SCF_NUMERICS_BRUSH = {"damping": 0.01, "mixing_max": 0.01, "switch_tole": 1e9, "tole": 1e-8, "anderm": 5}
+# Wider charged slits stiffen the low-k Poisson modes further: at mixing_max 0.02 widths H >= 38 stall
+# (measured); 0.005 converges every width of the H = 4..40 sweep (230-900 iterations cold).
+SCF_NUMERICS_WIDE = {"damping": 0.1, "mixing_max": 0.005, "switch_tole": 100.0, "tole": 1e-8, "anderm": 5}
 
 
 def _monomers(extra_charged: bool) -> dict:
@@ -130,7 +137,7 @@ def config3(n_points: int = 256) -> List[dict]:
 def config4(n_points: int = 512) -> List[dict]:
     """Slit-width sweep H = 4..40 at dz = 0.02 on the config-3 chemistry at pH 7."""
     return [weak_polyelectrolyte(H=float(H), dz=0.02, n_seg=20, rho_poly=RHO_POLY_SEG, solvent=RHO_SOLVENT,
-                                 pKa=4.5, pH=7.0, salt_M=0.010)
+                                 pKa=4.5, pH=7.0, salt_M=0.010, numerics=SCF_NUMERICS_WIDE)
             for H in np.linspace(4.0, 40.0, n_points)]
 
 
@@ -140,5 +147,5 @@ def config5(n_pH: int = 16, n_salt: int = 8) -> List[dict]:
     for pH in np.linspace(3.0, 9.0, n_pH):
         for salt in np.logspace(math.log10(0.001), math.log10(0.1), n_salt):
             out.append(weak_polyelectrolyte(H=81.9149, dz=0.005, n_seg=500, pKa=5.0, pH=float(pH),
-                                            salt_M=float(salt), grafted=0.05))
+                                            salt_M=float(salt), grafted=0.05, numerics=SCF_NUMERICS_BRUSH))
     return out

@@ -24,7 +24,7 @@ import copy
 import math
 from dataclasses import dataclass, field
 from fractions import Fraction
-from typing import List, Optional, Tuple
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -200,3 +200,14 @@ def build_state_point(dataset: dict, label: str = "") -> StatePoint:
                       total_charge=total_charge, hard_walls=hard, wall_lo=lo, wall_hi=hi,
                       wall_charges=wall_q, fixed=fixed, mu_species=mu_sp, functionals=funcs,
                       numerics=num, label=label)
+
+
+def build_state_points(datasets: Sequence[dict], workers: int = 0) -> List[StatePoint]:
+    """build_state_point over many datasets; with workers > 1 the host setups (bulk solves) run in
+    that many spawned processes (no GPU state is shared with them), results in input order."""
+    if workers <= 1 or len(datasets) < 2:
+        return [build_state_point(d) for d in datasets]
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(min(workers, len(datasets)), mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(build_state_point, datasets, chunksize=max(1, len(datasets) // (4 * workers))))

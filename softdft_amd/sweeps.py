@@ -37,7 +37,7 @@ import numpy as np
 import math
 
 from . import abi, analysis
-from .problem import StatePoint, build_state_point, flatten, unflatten
+from .problem import StatePoint, build_state_points, flatten, unflatten
 
 
 def julia_round(x: float) -> int:
@@ -81,11 +81,12 @@ class WidthResult:
 
 def width_sweep(datasets: Sequence[dict], *, waves: int = 8, precision: int = abi.PREC_MIXED,
                 numerics: Optional[dict] = None, max_iters: int = 3000,
-                on_round: Optional[Callable[[int, float], None]] = None) -> List[WidthResult]:
-    """Solve the slit widths of `datasets` (increasing H, one chemistry) with neighbour warm starts.
+                on_round: Optional[Callable[[int, float], None]] = None, workers: int = 0) -> List[WidthResult]:
+    """Solve the slit widths of `datasets` (increasing H, one chemistry) with neighbour warm starts;
+    `workers` > 1 runs the host setups in parallel processes.
 
     Returns one WidthResult per width, in sweep order."""
-    sps = [build_state_point(d) for d in datasets]
+    sps = build_state_points(datasets, workers)
     n = len(sps)
     num = dict(numerics or sps[0].numerics)
     opts = dict(tole=num["tole"], damping=num["damping"], mixing_max=num["mixing_max"],
@@ -169,10 +170,10 @@ def _waves_solve(sps, waves, opts, on_round, finish, warm=remap_solution):
 
 def ph_sweep(datasets: Sequence[dict], pH_values: Sequence[float], *, waves: int = 8,
              precision: int = abi.PREC_MIXED, numerics: Optional[dict] = None, max_iters: int = 3000,
-             on_round: Optional[Callable[[int, float], None]] = None) -> List[PHResult]:
+             on_round: Optional[Callable[[int, float], None]] = None, workers: int = 0) -> List[PHResult]:
     """pH continuation over `datasets` (one per pH in `pH_values`, same grid and chemistry) with
     neighbour warm starts; the result list ends at the reference's early-stop step."""
-    sps = [build_state_point(d) for d in datasets]
+    sps = build_state_points(datasets, workers)
     n = len(sps)
     num = dict(numerics or sps[0].numerics)
     opts = dict(tole=num["tole"], damping=num["damping"], mixing_max=num["mixing_max"],
