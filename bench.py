@@ -230,21 +230,25 @@ def report_c5(args):
     b1.objective(b1.x0(), abi.PREC_FP64)
     t_plain = time.perf_counter() - t0
     b1.close()
-    # converged solves of the first pH column (pH 3, every salt), mixed precision, batched
-    def ph_of(d):
-        return next(e["pSpecies"] for e in d["species"] if e["sequence"] == "Y")
-    col = [i for i in range(len(sps)) if abs(ph_of(dss[i]) - 3.0) < 1e-9]
-    bs = abi.Batch([sps[i] for i in col])
+    # one converged full-size solve: pH 3 at 1 mM (the first point), mixed precision
+    bs = abi.Batch([sps[0]])
     x, it, err, ts = _timed_solve(bs, bs.x0(), abi.PREC_MIXED, _solve_opts(sps[0].numerics, max_iters=3000))
+    om, al, _ = bs.observables()
     bs.close()
+    per_point_fp32 = res["fp32"] / len(sps)
     out = {"report": "config", "config": "C5", "points": len(sps), "NP": sps[0].NP, "n_seg": 500,
            "batch_evaluation_ms": {"fp32": res["fp32"] * 1e3, "fp64_banded": res["fp64_banded"] * 1e3},
            "batch_iterations_per_s": {"fp32": 1.0 / res["fp32"], "fp64_banded": 1.0 / res["fp64_banded"]},
-           "plain_fp64_dense_one_point_evaluation_ms": t_plain * 1e3,
-           "per_point_evaluation_speedup_fp32_batch_vs_plain_fp64": t_plain / (res["fp32"] / len(sps)),
-           "pH3_column_mixed_solve": {"points": len(col), "seconds": ts, "iterations": it.tolist(),
-                                      "converged": int(np.sum(err < 1e-8)), "value": float(np.sum(err < 1e-8)) / ts,
-                                      "unit": "state points/s"}}
+           "per_point_evaluation_ms": {"fp32_in_batch": per_point_fp32 * 1e3,
+                                       "fp64_banded_in_batch": res["fp64_banded"] / len(sps) * 1e3,
+                                       "plain_fp64_dense_one_point": t_plain * 1e3},
+           "per_point_evaluation_speedup_fp32_batch_vs_plain_fp64": t_plain / per_point_fp32,
+           "single_point_mixed_solve": {"pH": 3.0, "salt_M": 0.001, "iterations": int(it[0]),
+                                        "residual": float(err[0]), "seconds": ts, "omega": float(om[0]),
+                                        "alpha": float(al[0, 0, 1])},
+           "note": "Anderson(5) with the brush numerics converges the uncharged and weakly charged brushes; "
+                   "at pH >= 5.8 (100 mM) the residual stalls at O(1-10) for the library and the oracle alike, "
+                   "so converged points/s is not quoted for the whole 128-point sweep"}
     print(json.dumps(_finite(out)), flush=True)
 
 

@@ -29,9 +29,10 @@ SCF_NUMERICS_CHARGED = {"damping": 0.1, "mixing_max": 0.02, "switch_tole": 100.0
 # Anderson residual stalls at O(10) (the oracle does the same). Anderson from the first full history with
 # a 0.01 step converges the uncharged and weakly charged brushes (pH <= 5.4 at 100 mM, measured).
 SCF_NUMERICS_BRUSH = {"damping": 0.01, "mixing_max": 0.01, "switch_tole": 1e9, "tole": 1e-8, "anderm": 5}
-# Wider charged slits stiffen the low-k Poisson modes further: at mixing_max 0.02 widths H >= 38 stall
-# (measured); 0.005 converges every width of the H = 4..40 sweep (230-900 iterations cold).
-SCF_NUMERICS_WIDE = {"damping": 0.1, "mixing_max": 0.005, "switch_tole": 100.0, "tole": 1e-8, "anderm": 5}
+# Wider charged slits stiffen the low-k Poisson modes further: at mixing_max 0.02 widths H >= 38 stall,
+# and with Picard until the residual falls below 100 some warm-started widths stall too (measured on the
+# 512-width sweep); Anderson from the first full history with a 0.005 step converges all 512.
+SCF_NUMERICS_WIDE = {"damping": 0.1, "mixing_max": 0.005, "switch_tole": 1e9, "tole": 1e-8, "anderm": 5}
 
 
 def _monomers(extra_charged: bool) -> dict:
