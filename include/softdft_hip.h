@@ -131,6 +131,12 @@ typedef struct sdft_desc {
     const int32_t* sw_pairs;       /* [n_sw_pairs][2] bead types, i <= j */
     const double* sw_energy;       /* [n_sw_pairs] well depth eps_ij */
     const double* sw_lambda;       /* [n_sw_pairs] well width multiplier lambda_ij */
+
+    /* square-well walls (sw_walls.jl:61-103), for the contact-value theorem only (their potential is
+     * part of Ext): per point and bead type, the grid index of the well edge at the left and right
+     * wall (-1: none) and exp(energy) - 1; both NULL without square-well walls */
+    const int32_t* sw_wall_idx;    /* [P][NB][2] */
+    const double* sw_wall_fac;     /* [P][NB][2] */
 } sdft_desc_t;
 
 typedef struct sdft_solver_opts {

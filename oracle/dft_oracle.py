@@ -28,7 +28,7 @@ convention), following the reference files line by line, quirks included:
   grand potential         src/iDFT/evaluation/analysis/grand_potential.jl:4-28, num_species.jl:1-38
   ionization fractions    src/iDFT/evaluation/analysis/degree_of_dissociation.jl:1-78
   contact value theorem   src/iDFT/evaluation/analysis/contact_value_theorem.jl:1-20,
-                          .../external_fields/{hard_walls.jl:49-79, charged_walls.jl:60-81}
+                          .../external_fields/{hard_walls.jl:49-79, charged_walls.jl:60-81, sw_walls.jl:61-103}
 
 Deliberate differences (documented, not silent):
   * the Anderson least squares uses numpy lstsq (exact minimum-norm LS)
@@ -884,6 +884,12 @@ class OracleDFT:
         if sp.wall_charges is not None:
             c[0] += -2.0 * PI * ms.bjerrum * sp.wall_charges[0] ** 2
             c[1] += -2.0 * PI * ms.bjerrum * sp.wall_charges[1] ** 2
+        if sp.sw_wall_idx is not None:
+            # sw_walls.jl:61-103: rho at the well edge times (exp(energy) - 1)
+            for j in range(ms.NB):
+                for w in range(2):
+                    if sp.sw_wall_idx[j, w] >= 0:
+                        c[w] += self.rho_beads[sp.sw_wall_idx[j, w], j] * sp.sw_wall_fac[j, w]
         return c
 
 

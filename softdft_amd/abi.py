@@ -49,7 +49,8 @@ class SdftDesc(C.Structure):
                 ("total_charge", _f64p), ("fixed_coord", _i32p),
                 ("Ext", _f64p), ("trapez", _f64p), ("surf", _f64p),
                 ("hard_walls", C.c_int32), ("wall_lo", _i32p), ("wall_hi", _i32p), ("wall_charge", _f64p),
-                ("n_sw_pairs", C.c_int32), ("sw_pairs", _i32p), ("sw_energy", _f64p), ("sw_lambda", _f64p)]
+                ("n_sw_pairs", C.c_int32), ("sw_pairs", _i32p), ("sw_energy", _f64p), ("sw_lambda", _f64p),
+                ("sw_wall_idx", _i32p), ("sw_wall_fac", _f64p)]
 
 
 class SdftSolverOpts(C.Structure):
@@ -217,6 +218,17 @@ class Batch:
         d.wall_lo = _ptr(keep["lo"], C.c_int32)
         d.wall_hi = _ptr(keep["hi"], C.c_int32)
         d.wall_charge = _ptr(keep["wq"], C.c_double) if has_q else None
+        if any(sp.sw_wall_idx is not None for sp in points):
+            idx = np.full((P, self.NB, 2), -1, dtype=np.int32)
+            fac = np.zeros((P, self.NB, 2))
+            for k, sp in enumerate(points):
+                if sp.sw_wall_idx is not None:
+                    idx[k] = sp.sw_wall_idx
+                    fac[k] = sp.sw_wall_fac
+            keep["swwi"] = np.ascontiguousarray(idx)
+            keep["swwf"] = np.ascontiguousarray(fac)
+            d.sw_wall_idx = _ptr(keep["swwi"], C.c_int32)
+            d.sw_wall_fac = _ptr(keep["swwf"], C.c_double)
         if ms.sw is not None and "sw" in p0.functionals:
             keep["sw_pairs"] = np.ascontiguousarray(np.array(ms.sw.pairs, dtype=np.int32).reshape(-1))
             keep["sw_eps"] = np.ascontiguousarray(np.array(ms.sw.energys[:len(ms.sw.pairs)], dtype=np.float64))

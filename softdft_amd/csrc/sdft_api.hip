@@ -38,6 +38,8 @@ struct sdft_ctx {
     std::vector<int> multistate;
     std::vector<int> hard_lo, hard_hi;
     std::vector<double> wall_q;
+    std::vector<int32_t> sw_wall_idx;   // [P][NB][2] well edges of square-well walls (-1 none)
+    std::vector<double> sw_wall_fac;    // [P][NB][2] exp(energy) - 1
     int hard_walls = 0, charged_walls = 0;
     std::vector<double> valences;
     double lB = 0.0;
@@ -559,6 +561,18 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     sys.charged_walls = d->wall_charge != nullptr;
     ctx->hard_walls = d->hard_walls;
     ctx->charged_walls = d->wall_charge != nullptr;
+    if (d->sw_wall_idx && d->sw_wall_fac) {
+        ctx->sw_wall_idx.assign(d->sw_wall_idx, d->sw_wall_idx + (size_t)P * NB * 2);
+        ctx->sw_wall_fac.assign(d->sw_wall_fac, d->sw_wall_fac + (size_t)P * NB * 2);
+        for (int p = 0; p < P; ++p)
+            for (int k = 0; k < NB * 2; ++k) {
+                const int32_t i = ctx->sw_wall_idx[(size_t)p * NB * 2 + k];
+                if (i < -1 || i >= d->NP[p]) {
+                    set_error(ctx, SDFT_E_ARG, "point %d: square-well wall index %d outside the grid", p, i);
+                    return fail(SDFT_E_ARG);
+                }
+            }
+    }
 
     // ---- species
     std::vector<DevSpecies> spec(nsp);
@@ -1211,6 +1225,28 @@ extern "C" int32_t sdft_observables(sdft_ctx_t* ctx, double* omega, double* alph
             if (ctx->charged_walls && !ctx->wall_q.empty()) {
                 l += -2.0 * M_PI * ctx->lB * ctx->wall_q[2 * p] * ctx->wall_q[2 * p];
                 r += -2.0 * M_PI * ctx->lB * ctx->wall_q[2 * p + 1] * ctx->wall_q[2 * p + 1];
+            }
+            if (!ctx->sw_wall_idx.empty()) {
+                // square-well walls: rho at the well edge times (exp(energy) - 1) (sw_walls.jl:61-103)
+                const DevPoint& Pt = ctx->pts[p];
+                const int prec = st[p].prec == 32 ? 32 : 64;
+                for (int j = 0; j < NB; ++j)
+                    for (int w = 0; w < 2; ++w) {
+                        const size_t k = ((size_t)p * NB + j) * 2 + w;
+                        const int32_t i = ctx->sw_wall_idx[k];
+                        if (i < 0) continue;
+                        double v;
+                        if (prec == 64) {
+                            HIP_CHECK_RET(hipMemcpy(&v, ctx->p64.rho_beads + Pt.base * NB + (int64_t)j * Pt.NP + i, 8,
+                                                    hipMemcpyDeviceToHost));
+                        } else {
+                            float f;
+                            HIP_CHECK_RET(hipMemcpy(&f, ctx->p32.rho_beads + Pt.base * NB + (int64_t)j * Pt.NP + i, 4,
+                                                    hipMemcpyDeviceToHost));
+                            v = f;
+                        }
+                        (w == 0 ? l : r) += v * ctx->sw_wall_fac[k];
+                    }
             }
             p_cvt[2 * p] = l;
             p_cvt[2 * p + 1] = r;

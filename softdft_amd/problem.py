@@ -75,6 +75,10 @@ class StatePoint:
     functionals: List[str]
     numerics: dict
     label: str = ""
+    # square-well walls (sw_walls.jl): per bead type the 0-based grid index of the well's edge at each
+    # wall (-1: no well for that bead) and the contact-theorem factor exp(energy) - 1
+    sw_wall_idx: Optional[np.ndarray] = None   # [NB, 2] int
+    sw_wall_fac: Optional[np.ndarray] = None   # [NB, 2]
 
     @property
     def NB(self) -> int:
@@ -126,6 +130,7 @@ def build_state_point(dataset: dict, label: str = "") -> StatePoint:
     surf = np.zeros(2 * NP)
     total_charge = 0.0
     ext = geom.get("external_field") or {}
+    sw_idx = sw_fac = None
     hard = "hard_walls" in ext
     lo = np.zeros(NB, dtype=np.int64)
     hi = np.zeros(NB, dtype=np.int64)
@@ -148,12 +153,16 @@ def build_state_point(dataset: dict, label: str = "") -> StatePoint:
                         trapez[K, j] = 1.0
                     elif idx == hi1:
                         trapez[K, j] = 0.5
-        elif name == "charged_walls":
+        elif name in ("charged_walls", "pointcharge_walls"):
+            # charged_walls.jl:6-58 / pointcharge_walls.jl:6-48: in one dimension both put q / dz on the
+            # wall's grid point; only charged walls add a contact-theorem term (contact_value_theorem
+            # has no method for point-charge walls and contributes zero)
             q = desc["charge"]
             if isinstance(q[0], (list, tuple)):
                 q = q[0]
             qL, qR = float(q[0]), float(q[1])
-            wall_q = (qL, qR)
+            if name == "charged_walls":
+                wall_q = (qL, qR)
             total_charge += qL + qR
             for qq, pos in ((qL, 0.0), (qR, H)):
                 g = julia_round(pos / dz) + 1
@@ -161,6 +170,45 @@ def build_state_point(dataset: dict, label: str = "") -> StatePoint:
                     surf[g - 1] += qq / dz
             for K in range(NP):           # apply_mirroring!
                 surf[2 * NP - 1 - K] = surf[K]
+        elif name == "sw_walls":
+            # sw_walls.jl:13-58 (geometry parsing: geometry/.../sw_walls.jl:73-131): a well of depth
+            # energy[w] up to grid index round((wall +- lambda[w] d / 2) / dz) + 1 at each wall
+            mons = (desc or {}).get("monomers")
+            if not mons:
+                raise SetupError("Missing required section: 'monomers' under 'sw_walls'.")
+            sw_idx = np.full((NB, 2), -1, dtype=np.int64)
+            sw_fac = np.zeros((NB, 2))
+            for j, mname in enumerate(ms.monomers):
+                if mname not in mons:
+                    continue
+                e, lam = mons[mname].get("energy"), mons[mname].get("lambda")
+                if e is None or lam is None:
+                    raise SetupError("Each monomer entry under 'sw_walls.monomers' must define both 'energy' and 'lambda'.")
+                e, lam = [float(v) for v in e[0]], [float(v) for v in lam[0]]
+                d = float(ms.diameters[j])
+                lo1 = julia_round((0.0 + lam[0] * d / 2.0) / dz) + 1
+                hi1 = julia_round((H - lam[1] * d / 2.0) / dz) + 1
+                for K in range(NP):
+                    idx = K + 1
+                    if idx <= lo1:
+                        Ext[K, j] += e[0]
+                    elif idx >= hi1:
+                        Ext[K, j] += e[1]
+                sw_idx[j] = (lo1 - 1, hi1 - 1)
+                sw_fac[j] = (math.exp(e[0]) - 1.0, math.exp(e[1]) - 1.0)
+            if np.any(sw_idx < -1) or np.any(sw_idx >= NP):
+                raise SetupError("sw_walls: well edge outside the grid")
+        elif name == "input_external":
+            # input_external.jl:6-28: rows "i, Ext_1, ..., Ext_NB" (0-based grid index) overwrite Ext
+            path = (desc or {}).get("file", "external_potential.csv")
+            data = np.loadtxt(path, delimiter=",", ndmin=2)
+            if data.shape[1] != 1 + NB:
+                raise SetupError(f"Expected {1 + NB} columns: dims + {NB} species.")
+            for row in data:
+                K = int(round(row[0]))
+                if not 0 <= K < NP:
+                    raise SetupError(f"input_external: grid index {K} outside 0..{NP - 1}")
+                Ext[K, :] = row[1:]
         else:
             raise SetupError(f"external field '{name}' is outside this build's scope")
 
@@ -196,10 +244,11 @@ def build_state_point(dataset: dict, label: str = "") -> StatePoint:
             mu_sp[u] = 0.0
     funcs = [m for m in ms.fe_model if m in FUNCTIONAL_KEYS]
     num = {k: float(v) for k, v in (ds.get("numerical") or {}).items()}
-    return StatePoint(ms=ms, bulk=bulk, H=H, dz=dz, NP=NP, Ext=Ext, trapez=trapez, surf=surf,
+    sp_out = StatePoint(ms=ms, bulk=bulk, H=H, dz=dz, NP=NP, Ext=Ext, trapez=trapez, surf=surf,
                       total_charge=total_charge, hard_walls=hard, wall_lo=lo, wall_hi=hi,
                       wall_charges=wall_q, fixed=fixed, mu_species=mu_sp, functionals=funcs,
-                      numerics=num, label=label)
+                      numerics=num, label=label, sw_wall_idx=sw_idx, sw_wall_fac=sw_fac)
+    return sp_out
 
 
 def build_state_points(datasets: Sequence[dict], workers: int = 0) -> List[StatePoint]:

@@ -516,3 +516,52 @@ def test_bulk_chain_log_domain_matches_direct_and_normalises():
     X5, seg5, b5, lX5 = bk._chain_bulk(ms5, 0, np.zeros(ms5.NB), np.zeros(ms5.MB))
     assert math.isfinite(lX5) and np.all(np.isfinite(seg5))
     assert np.allclose(seg5.sum(axis=0), 1.0, rtol=1e-12)
+
+
+# ----------------------------------------------------------------- other wall types
+def _ideal_gas_slit(ext):
+    return {"procedure": "iDFT",
+            "geometry": {"coordinate_system": "Cartesian", "dimensions": [4.0], "bin_width": [0.05],
+                         "external_field": ext},
+            "species": [{"sequence": "A", "density": 0.3}, {"sequence": "S", "density": 0.2}],
+            "monomers": {"A": {"diameter": 1.0, "valence": 0.0}, "S": {"diameter": 0.8, "valence": 0.0}},
+            "numerical": dict(configs.SCF_NUMERICS), "model": ["ideal"]}
+
+
+def test_square_well_walls_ideal_gas_contact_theorem():
+    """Ideal gas between hard walls with square-well wall attractions: rho = rho_b exp(-Ext) exactly,
+    so the hard-wall contact density plus rho(well edge) (exp(eps) - 1) is the bulk pressure at each
+    wall (sw_walls.jl:13-103)."""
+    ext = {"hard_walls": {"dims": [1]},
+           "sw_walls": {"dims": [1], "monomers": {"A": {"energy": [[-0.7, 0.4]], "lambda": [[1.5, 1.3]]}}}}
+    sp, o = _oracle_for(_ideal_gas_slit(ext))
+    j = sp.ms.monomers.index("A")
+    lo1 = round(1.5 * 1.0 / 2 / 0.05) + 1            # 16: last grid index (1-based) inside the left well
+    hi1 = round((4.0 - 1.3 * 1.0 / 2) / 0.05) + 1     # 68
+    assert tuple(sp.sw_wall_idx[j]) == (lo1 - 1, hi1 - 1)
+    assert sp.Ext[lo1 - 1, j] == pytest.approx(-0.7) and sp.Ext[lo1, j] == 0.0
+    assert sp.Ext[hi1 - 1, j] == pytest.approx(0.4) and sp.Ext[hi1 - 2, j] == 0.0
+    assert tuple(sp.sw_wall_idx[sp.ms.monomers.index("S")]) == (-1, -1)
+    x0 = sp.x0()
+    fx = o.objective(x0)
+    assert np.max(np.abs(fx - x0)) < 1e-14           # no excess functional: x0 is the fixed point
+    c = o.contact_pressure()
+    assert c[0] == pytest.approx(sp.bulk.pressure, rel=1e-12)
+    assert c[1] == pytest.approx(sp.bulk.pressure, rel=1e-12)
+
+
+def test_pointcharge_walls_place_charge_like_charged_walls_without_a_contact_term():
+    a = build_state_point(_ideal_gas_slit({"hard_walls": {"dims": [1]},
+                                           "charged_walls": {"dims": [1], "charge": [[-0.05, 0.02]]}}))
+    b = build_state_point(_ideal_gas_slit({"hard_walls": {"dims": [1]},
+                                           "pointcharge_walls": {"dims": [1], "charge": [[-0.05, 0.02]]}}))
+    assert np.array_equal(a.surf, b.surf) and a.total_charge == b.total_charge == pytest.approx(-0.03)
+    assert a.wall_charges == (-0.05, 0.02) and b.wall_charges is None
+
+
+def test_input_external_overwrites_rows(tmp_path):
+    f = tmp_path / "ext.csv"
+    f.write_text("0, 5.0, 6.0\n3, -1.5, 0.25\n")
+    sp = build_state_point(_ideal_gas_slit({"input_external": {"dims": [1], "file": str(f)}}))
+    assert list(sp.Ext[0]) == [5.0, 6.0] and list(sp.Ext[3]) == [-1.5, 0.25]
+    assert np.count_nonzero(sp.Ext) == 4
