@@ -334,6 +334,16 @@ def main():
         batch.close()
         return
 
+    # the same solve launched kernel by kernel instead of as captured poll windows (hipGraph)
+    batch.set_graph_capture(False)
+    t0 = time.perf_counter()
+    xe, ite, erre = batch.solve(x0, precision=abi.PREC_MIXED, **num)
+    eager_ms = (time.perf_counter() - t0) * 1e3
+    batch.set_graph_capture(True)
+    same = bool(np.array_equal(xe, x) and np.array_equal(ite, it))
+    log(f"eager launches: {eager_ms:.1f} ms per solve (graph windows: {sum(times) / len(times) * 1e3:.1f} ms), "
+        f"bitwise equal: {same}")
+
     # per-launch time of the hot kernel classes, HIP events on the library stream, with the
     # operators the mixed solve used; weighted by launches per evaluation x iterations run
     # (one chain launch runs every propagator step of the chain species)
@@ -410,6 +420,8 @@ def main():
         "kernels_us": {k: v[0] * 1e3 for k, v in prof.items()},
         "kernels_ms_per_solve_est": {k: v[3] for k, v in prof.items()},
         "pipeline_iterations": {"fp32": it32, "fp64": it64},
+        "launch": {"mode": "hipGraph per poll window", "eager_ms_per_step": eager_ms,
+                   "eager_bitwise_equal": same},
     }
     print(json.dumps(out))
 

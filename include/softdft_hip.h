@@ -189,6 +189,13 @@ int32_t sdft_set_operator_band(sdft_ctx_t* ctx, int32_t fp32_banded, int32_t fp6
  * (instrumentation; no counterpart in the reference) */
 int32_t sdft_last_pipeline_iters(const sdft_ctx_t* ctx, int32_t* fp32_iters, int32_t* fp64_iters);
 
+/* launch mode of sdft_solve: enable = 1 (default) runs each window of check_every iterations
+ * between two host polls as one captured hipGraph (instantiated once per pair of active
+ * pipelines and solve), 0 launches every kernel on the stream, < 0 leaves the mode unchanged.
+ * graphs_built (may be NULL) receives the number of graphs instantiated so far.  Results are
+ * identical in both modes.  (launch control; no counterpart in the reference) */
+int32_t sdft_set_graph_capture(sdft_ctx_t* ctx, int32_t enable, int64_t* graphs_built);
+
 /* kernel classes for sdft_kernel_profile */
 #define SDFT_KP_PROP_FP32 0      /* one propagator step, FP32, banded bond kernel */
 #define SDFT_KP_PROP_FP64 1      /* one propagator step, FP64, bond operator as in the last solve */

@@ -29,7 +29,8 @@ FIELD_RHO_SEGMENTS = 16  # + species index: rho_K.segments[u], (NP, S_u, N_u)
 
 EXPORTED = ("sdft_abi_version", "sdft_last_error", "sdft_create", "sdft_destroy", "sdft_solution_length",
             "sdft_solution_offsets", "sdft_objective", "sdft_solve", "sdft_observables", "sdft_get_field",
-            "sdft_last_timing", "sdft_last_pipeline_iters", "sdft_set_operator_band", "sdft_kernel_profile")
+            "sdft_last_timing", "sdft_last_pipeline_iters", "sdft_set_operator_band", "sdft_kernel_profile",
+            "sdft_set_graph_capture")
 KP_PROP_FP32, KP_PROP_FP64, KP_CONV_FWD_FP32, KP_CONV_FWD_FP64, KP_CONV_BACK_FP32, KP_CONV_BACK_FP64, \
     KP_POINTWISE_FP64, KP_CHAIN_FP32, KP_CHAIN_FP64 = range(9)
 
@@ -103,6 +104,8 @@ def load_library(path: str = LIB_PATH):
     lib.sdft_set_operator_band.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     lib.sdft_kernel_profile.restype = C.c_int32
     lib.sdft_kernel_profile.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _f64p, _f64p]
+    lib.sdft_set_graph_capture.restype = C.c_int32
+    lib.sdft_set_graph_capture.argtypes = [C.c_void_p, C.c_int32, _i64p]
     if lib.sdft_abi_version() != SDFT_ABI_VERSION:
         raise RuntimeError("libsoftdft_hip ABI version mismatch")
     _lib = lib
@@ -305,6 +308,14 @@ class Batch:
     def set_operator_band(self, fp32: bool = True, fp64: bool = False):
         """Bond operator of objective() per precision: banded (True) or dense (False)."""
         self._check(self.lib.sdft_set_operator_band(self.h, int(fp32), int(fp64)))
+
+    def set_graph_capture(self, enable: Optional[bool] = None) -> int:
+        """Launch mode of solve(): True runs each poll window as one captured hipGraph (default),
+        False launches kernel by kernel, None keeps the mode.  Returns the graphs built so far."""
+        n = C.c_int64()
+        mode = -1 if enable is None else int(bool(enable))
+        self._check(self.lib.sdft_set_graph_capture(self.h, mode, C.byref(n)))
+        return n.value
 
     def pipeline_iters(self):
         """(FP32, FP64) batch iterations of the last solve during which each pipeline ran."""

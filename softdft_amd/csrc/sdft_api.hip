@@ -45,6 +45,26 @@ struct sdft_ctx {
     double lB = 0.0;
     int fwd_terms = 0, back_terms = 0;   // convolution terms per point (algorithmic flop count)
     double* d_om_blocks = nullptr;       // per-point Omega block partials, allocated on first use
+    int32_t graph_capture = 1;           // sdft_solve runs its poll windows as captured hipGraphs
+    int64_t graphs_built = 0;            // graphs instantiated by sdft_solve over the context's life
+};
+
+// Executable graphs of one sdft_solve, one per pair of active pipelines (FP32 | FP64 << 1);
+// released when the solve returns, after the stream has drained.
+struct WindowGraphs {
+    hipStream_t stream;
+    hipGraphExec_t exec[4] = {nullptr, nullptr, nullptr, nullptr};
+    explicit WindowGraphs(hipStream_t s) : stream(s) {}
+    WindowGraphs(const WindowGraphs&) = delete;
+    WindowGraphs& operator=(const WindowGraphs&) = delete;
+    ~WindowGraphs() {
+        bool any = false;
+        for (hipGraphExec_t e : exec) any = any || e;
+        if (!any) return;
+        (void)hipStreamSynchronize(stream);
+        for (hipGraphExec_t& e : exec)
+            if (e) (void)hipGraphExecDestroy(e);
+    }
 };
 
 static thread_local std::string g_last_error;
@@ -988,13 +1008,49 @@ extern "C" int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* o, doub
     const double prog_every = prog_env ? atof(prog_env) : 0.0;
     auto prog_t0 = std::chrono::steady_clock::now();
     auto prog_last = prog_t0;
+    // The `check` iterations between two polls are a fixed launch sequence for a given pair of
+    // active pipelines (every grid size comes from the context, every per-iteration decision is
+    // read on the device), so each full window runs as one captured hipGraph, instantiated once per
+    // pipeline pair and solve. Windows cut short by max_iters run eagerly.
+    WindowGraphs graphs(ctx->stream);
+    const bool use_graph = ctx->graph_capture != 0;
     int it = 0, n32 = 0, n64 = 0;
-    for (it = 0; it < o->max_iters; ++it) {
-        run_objective(ctx, run32, run64);
-        n32 += run32;
-        n64 += run64;
-        launch_anderson(ctx->stream, c, sp);
-        if ((it + 1) % check == 0 || it + 1 == o->max_iters) {
+    while (it < o->max_iters) {
+        const int w = std::min(check, o->max_iters - it);
+        const int key = (run32 ? 1 : 0) | (run64 ? 2 : 0);
+        if (use_graph && w == check) {
+            if (!graphs.exec[key]) {
+                hipGraph_t g = nullptr;
+                HIP_CHECK_RET(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                for (int k = 0; k < w; ++k) {
+                    run_objective(ctx, run32, run64);
+                    launch_anderson(ctx->stream, c, sp);
+                }
+                const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+                if (ec != hipSuccess || !g) {
+                    set_error(ctx, SDFT_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+                    return SDFT_E_HIP;
+                }
+                const hipError_t ei = hipGraphInstantiate(&graphs.exec[key], g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (ei != hipSuccess) {
+                    graphs.exec[key] = nullptr;
+                    set_error(ctx, SDFT_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+                    return SDFT_E_HIP;
+                }
+                ++ctx->graphs_built;
+            }
+            HIP_CHECK_RET(hipGraphLaunch(graphs.exec[key], ctx->stream));
+        } else {
+            for (int k = 0; k < w; ++k) {
+                run_objective(ctx, run32, run64);
+                launch_anderson(ctx->stream, c, sp);
+            }
+        }
+        n32 += run32 ? w : 0;
+        n64 += run64 ? w : 0;
+        it += w;
+        {
             HIP_CHECK_RET(hipMemcpyAsync(counts, c.d_n_active, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
             HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
             if (prog_every > 0.0) {
@@ -1008,7 +1064,7 @@ extern "C" int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* o, doub
                     for (const DevSolveState& q : st)
                         if (!q.converged) emax = q.err > emax || q.err != q.err ? q.err : emax;
                     fprintf(stderr, "sdft_solve: iteration %d, %d of %d points active (%d in FP32), max residual %.3e, %.1f s\n",
-                            it + 1, counts[0], c.P, counts[1] - counts[2], emax,
+                            it, counts[0], c.P, counts[1] - counts[2], emax,
                             std::chrono::duration<double>(now - prog_t0).count());
                     fflush(stderr);
                 }
@@ -1071,6 +1127,13 @@ extern "C" int32_t sdft_last_pipeline_iters(const sdft_ctx_t* ctx, int32_t* fp32
     if (!ctx) return SDFT_E_ARG;
     if (fp32_iters) *fp32_iters = ctx->last_iters32;
     if (fp64_iters) *fp64_iters = ctx->last_iters64;
+    return SDFT_OK;
+}
+
+extern "C" int32_t sdft_set_graph_capture(sdft_ctx_t* ctx, int32_t enable, int64_t* graphs_built) {
+    if (!ctx) return SDFT_E_ARG;
+    if (enable >= 0) ctx->graph_capture = enable ? 1 : 0;
+    if (graphs_built) *graphs_built = ctx->graphs_built;
     return SDFT_OK;
 }
 
