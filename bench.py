@@ -260,6 +260,7 @@ def main():
     ap.add_argument("--points", type=int, default=256)
     ap.add_argument("--singles", type=int, default=6, help="one-at-a-time FP64 sample size")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="fixed launch rules instead of sdft_autotune")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--kernel-only", choices=["chain32", "chain64", "fwd32", "fwd64", "back32", "back64"],
@@ -300,6 +301,14 @@ def main():
     batch = abi.Batch(sps)
     x0 = batch.x0()
     num = dict(tole=1e-8, damping=0.1, mixing_max=0.02, switch_tole=100.0, anderm=5, max_iters=3000)
+    tuning = []
+    if not args.no_tune:
+        t = time.time()
+        tuning = batch.autotune(reps=3)
+        log(f"autotune: {len(tuning)} decisions in {time.time() - t:.2f} s")
+        for e in tuning:
+            log(f"  kind {e['kind']} key {e['key']} precision {e['precision']}: choice {e['choice']}, "
+                f"candidates (us) {[round(u, 1) for u in e['us'] if u >= 0]}")
 
     if args.kernel_only:
         # one evaluation of every point in the class's precision with the banded operators, so that
@@ -353,9 +362,13 @@ def main():
             ("k_chain<double> (banded propagators, all steps)", abi.KP_CHAIN_FP64, 64, 1, it64),
             ("k_convgemm<float> (forward weighted densities)", abi.KP_CONV_FWD_FP32, 32, 1, it32),
             ("k_convgemm<double> (forward weighted densities)", abi.KP_CONV_FWD_FP64, 64, 1, it64),
-            ("k_convgemm<float> (back projections)", abi.KP_CONV_BACK_FP32, 32, 1, it32),
-            ("k_convgemm<double> (back projections)", abi.KP_CONV_BACK_FP64, 64, 1, it64)]:
-        ms, fl = batch.kernel_profile(kc, reps=20)
+            ("conv GEMMs<float> (back projections: shared partials, then projections)", abi.KP_CONV_BACK_FP32, 32, 1, it32),
+            ("conv GEMMs<double> (back projections: shared partials, then projections)", abi.KP_CONV_BACK_FP64, 64, 1, it64)]:
+        try:
+            ms, fl = batch.kernel_profile(kc, reps=20)
+        except abi.SdftError as e:  # e.g. the autotuner chose the per-step propagators
+            log(f"kernel {name}: not profiled ({e})")
+            continue
         prof[name] = (ms, fl, prec, ms * per_eval * iters_run)
         log(f"kernel {name}: {ms * 1e3:.1f} us/launch, {fl / ms / 1e9:.2f} TFLOP/s, "
             f"{per_eval} launches/evaluation x {iters_run} iterations = {ms * per_eval * iters_run:.1f} ms per solve")
@@ -369,9 +382,16 @@ def main():
 
     # plain FP64, one state point at a time
     sel = np.linspace(0, args.points - 1, args.singles).astype(int)
+    single_tuning = []
+    if not args.no_tune:  # tuned once for the one-point layout, replayed on every one-point context
+        b1 = abi.Batch([sps[sel[0]]])
+        single_tuning = b1.autotune(reps=3)
+        b1.close()
     t0 = time.perf_counter()
     for i in sel:
         b1 = abi.Batch([sps[i]])
+        if single_tuning:
+            b1.set_tuning(single_tuning)
         b1.solve(b1.x0(), precision=abi.PREC_FP64, **num)
         b1.close()
     single_rate = len(sel) / (time.perf_counter() - t0)
@@ -422,6 +442,9 @@ def main():
         "pipeline_iterations": {"fp32": it32, "fp64": it64},
         "launch": {"mode": "hipGraph per poll window", "eager_ms_per_step": eager_ms,
                    "eager_bitwise_equal": same},
+        "tuning": [{"kind": {1: "propagator", 2: "conv_fwd", 3: "conv_pre", 4: "conv_back"}[e["kind"]],
+                    "key": e["key"], "precision": e["precision"], "choice": e["choice"],
+                    "us": [round(u, 2) for u in e["us"] if u >= 0]} for e in tuning],
     }
     print(json.dumps(out))
 

@@ -196,6 +196,42 @@ int32_t sdft_last_pipeline_iters(const sdft_ctx_t* ctx, int32_t* fp32_iters, int
  * identical in both modes.  (launch control; no counterpart in the reference) */
 int32_t sdft_set_graph_capture(sdft_ctx_t* ctx, int32_t enable, int64_t* graphs_built);
 
+/* Autotuner (no counterpart in the reference; the reference has one CPU code path).  Measures,
+ * with HIP events on the context's stream, the candidate launch configurations of this batch and
+ * keeps the fastest:
+ *   SDFT_TUNE_PROPAGATOR  per chain species and precision: the block-per-problem mapping of the
+ *                         propagator recursion.  choice 1 = persistent chain kernel (one workgroup per
+ *                         state point and sweep runs every step), 0 = one operator-GEMM launch per
+ *                         step (workgroups tile the grid of every point).  key = species index.
+ *   SDFT_TUNE_CONV_FWD / _PRE / _BACK
+ *                         per grid-length bin (kernel sets whose NP rounds up to the same multiple of
+ *                         256; key = that bound) and convolution stage: the tile shape, choice
+ *                         0 = 128 x 16, 1 = 64 x 64, 2 = 128 x 64 rows x columns.  Timed at FP32 and
+ *                         FP64 and chosen on the sum.
+ * us[k] is the measured microseconds per launch sequence of candidate k (< 0: not a candidate).
+ * Without a call the library uses fixed rules (chain kernel wherever eligible; 64 x 64 tiles where a
+ * grid length's operators average >= 48 columns, else 128 x 16).  The choice changes summation
+ * orders (FP64 results move in the last bits) and the banded operator's padding, never the algorithm.
+ * Call it before sdft_solve / sdft_objective: it overwrites the work buffers. */
+#define SDFT_TUNE_PROPAGATOR 1
+#define SDFT_TUNE_CONV_FWD 2
+#define SDFT_TUNE_CONV_PRE 3
+#define SDFT_TUNE_CONV_BACK 4
+typedef struct {
+    int32_t kind;        /* SDFT_TUNE_* */
+    int32_t key;         /* species index, or the bin's NP bound */
+    int32_t precision;   /* 32 / 64 (propagators), 0 (convolutions: FP32 + FP64) */
+    int32_t choice;
+    double us[3];
+} sdft_tune_entry_t;
+int32_t sdft_autotune(sdft_ctx_t* ctx, int32_t reps, sdft_tune_entry_t* entries, int32_t capacity,
+                      int32_t* n_entries);
+/* applies decisions without measuring (a saved sdft_autotune result for the same batch layout, or
+ * a forced configuration): PROPAGATOR entries with precision 0 apply to both precisions, and
+ * choice 1 takes effect only where the chain kernel is eligible; CONV entries with key 0 apply to
+ * every grid length, choice -1 restores the fixed rule.  us[] is ignored. */
+int32_t sdft_set_tuning(sdft_ctx_t* ctx, const sdft_tune_entry_t* entries, int32_t n);
+
 /* kernel classes for sdft_kernel_profile */
 #define SDFT_KP_PROP_FP32 0      /* one propagator step, FP32, banded bond kernel */
 #define SDFT_KP_PROP_FP64 1      /* one propagator step, FP64, bond operator as in the last solve */

@@ -30,7 +30,9 @@ FIELD_RHO_SEGMENTS = 16  # + species index: rho_K.segments[u], (NP, S_u, N_u)
 EXPORTED = ("sdft_abi_version", "sdft_last_error", "sdft_create", "sdft_destroy", "sdft_solution_length",
             "sdft_solution_offsets", "sdft_objective", "sdft_solve", "sdft_observables", "sdft_get_field",
             "sdft_last_timing", "sdft_last_pipeline_iters", "sdft_set_operator_band", "sdft_kernel_profile",
-            "sdft_set_graph_capture")
+            "sdft_set_graph_capture", "sdft_autotune", "sdft_set_tuning")
+TUNE_PROPAGATOR, TUNE_CONV_FWD, TUNE_CONV_PRE, TUNE_CONV_BACK = 1, 2, 3, 4
+CONV_TALL, CONV_WIDE64, CONV_WIDE128 = 0, 1, 2
 KP_PROP_FP32, KP_PROP_FP64, KP_CONV_FWD_FP32, KP_CONV_FWD_FP64, KP_CONV_BACK_FP32, KP_CONV_BACK_FP64, \
     KP_POINTWISE_FP64, KP_CHAIN_FP32, KP_CHAIN_FP64 = range(9)
 
@@ -59,6 +61,11 @@ class SdftSolverOpts(C.Structure):
                 ("switch_tole", C.c_double), ("anderm", C.c_int32), ("max_iters", C.c_int32),
                 ("precision", C.c_int32), ("fp32_switch", C.c_double), ("band_fp32", C.c_int32),
                 ("check_every", C.c_int32)]
+
+
+class SdftTuneEntry(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("key", C.c_int32), ("precision", C.c_int32), ("choice", C.c_int32),
+                ("us", C.c_double * 3)]
 
 
 class SdftError(RuntimeError):
@@ -104,6 +111,10 @@ def load_library(path: str = LIB_PATH):
     lib.sdft_set_operator_band.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
     lib.sdft_kernel_profile.restype = C.c_int32
     lib.sdft_kernel_profile.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _f64p, _f64p]
+    lib.sdft_autotune.restype = C.c_int32
+    lib.sdft_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(SdftTuneEntry), C.c_int32, _i32p]
+    lib.sdft_set_tuning.restype = C.c_int32
+    lib.sdft_set_tuning.argtypes = [C.c_void_p, C.POINTER(SdftTuneEntry), C.c_int32]
     lib.sdft_set_graph_capture.restype = C.c_int32
     lib.sdft_set_graph_capture.argtypes = [C.c_void_p, C.c_int32, _i64p]
     if lib.sdft_abi_version() != SDFT_ABI_VERSION:
@@ -308,6 +319,26 @@ class Batch:
     def set_operator_band(self, fp32: bool = True, fp64: bool = False):
         """Bond operator of objective() per precision: banded (True) or dense (False)."""
         self._check(self.lib.sdft_set_operator_band(self.h, int(fp32), int(fp64)))
+
+    def autotune(self, reps: int = 3) -> List[dict]:
+        """Measure the propagator mapping per chain species and precision and the convolution tile
+        shape per grid-length bin and stage, keep the fastest; returns the decisions."""
+        cap = 4 * 64 + 3 * 4096
+        buf = (SdftTuneEntry * cap)()
+        n = C.c_int32()
+        self._check(self.lib.sdft_autotune(self.h, reps, buf, cap, C.byref(n)))
+        return [dict(kind=e.kind, key=e.key, precision=e.precision, choice=e.choice, us=list(e.us))
+                for e in buf[:min(n.value, cap)]]
+
+    def set_tuning(self, entries: Sequence[dict]):
+        """Apply decisions (from autotune() or forced): dicts with kind, key, precision, choice."""
+        arr = (SdftTuneEntry * max(1, len(entries)))()
+        for i, e in enumerate(entries):
+            arr[i].kind = e["kind"]
+            arr[i].key = e.get("key", 0)
+            arr[i].precision = e.get("precision", 0)
+            arr[i].choice = e["choice"]
+        self._check(self.lib.sdft_set_tuning(self.h, arr, len(entries)))
 
     def set_graph_capture(self, enable: Optional[bool] = None) -> int:
         """Launch mode of solve(): True runs each poll window as one captured hipGraph (default),

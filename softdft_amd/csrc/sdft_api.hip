@@ -45,6 +45,13 @@ struct sdft_ctx {
     double lB = 0.0;
     int fwd_terms = 0, back_terms = 0;   // convolution terms per point (algorithmic flop count)
     double* d_om_blocks = nullptr;       // per-point Omega block partials, allocated on first use
+    // convolution plan inputs, kept to rebuild the plans with other tile shapes (sdft_autotune)
+    std::vector<ConvJob> jobs_h;
+    std::vector<std::vector<int>> canon;
+    std::vector<int32_t> set_L_h;
+    std::vector<int64_t> set_kbase_h;
+    std::vector<int> set_shape[3];       // per stage (fwd, pre, back) and kernel set: tile shape, -1 heuristic
+    std::vector<sdft_tune_entry_t> tuning;   // decisions of the last sdft_autotune
     int32_t graph_capture = 1;           // sdft_solve runs its poll windows as captured hipGraphs
     int64_t graphs_built = 0;            // graphs instantiated by sdft_solve over the context's life
 };
@@ -278,9 +285,14 @@ static std::vector<OpTile> xcd_order(const std::vector<OpTile>& tiles) {
     return out;  // a multiple of 8 long: split launches (ID = tile + ntiles * part) keep parts on one XCD
 }
 
+// Convolution plan of one stage: columns (point, job) grouped by (kernel set, operator terms), each
+// group tiled in the shape of its kernel set (set_shape[kset]; < 0: the heuristic, 64 x 64 tiles
+// when the set's groups average >= 48 columns, else 128 x 16). Points with point_mask[p] == 0 are
+// left out (the autotuner's per-grid-length plans).
 static int build_conv_plan(sdft_ctx* ctx, const std::vector<ConvJob>& jobs, int job_base, int njobs,
-                           const std::vector<std::vector<int>>& canon, const std::vector<int>& set_points,
-                           const std::vector<int32_t>& set_L, const std::vector<int64_t>& set_kbase, OpPlan& plan) {
+                           const std::vector<std::vector<int>>& canon, const std::vector<int>& set_shape,
+                           const std::vector<char>* point_mask, const std::vector<int32_t>& set_L,
+                           const std::vector<int64_t>& set_kbase, Ctx::ConvStage& stage) {
     Ctx& c = ctx->c;
     struct Key {
         int kset, nterms, kern[3], sign[3];
@@ -296,6 +308,7 @@ static int build_conv_plan(sdft_ctx* ctx, const std::vector<ConvJob>& jobs, int 
     };
     std::map<Key, std::vector<OpCol>> groups;
     for (int p = 0; p < c.P; ++p) {
+        if (point_mask && !(*point_mask)[p]) continue;
         const int s = ctx->pts[p].kset;
         for (int j = 0; j < njobs; ++j) {
             const ConvJob& J = jobs[job_base + j];
@@ -310,46 +323,88 @@ static int build_conv_plan(sdft_ctx* ctx, const std::vector<ConvJob>& jobs, int 
             groups[k].push_back(OpCol{p, job_base + j});
         }
     }
-    (void)set_points;
-    // tile shape: wide (64 x 64) when groups are large, tall (128 x 16) otherwise
-    size_t tot = 0;
-    for (auto& g : groups) tot += g.second.size();
-    plan.wide = groups.empty() ? true : (tot / groups.size() >= 48);
-    const int TM = conv_tile_rows(plan.wide), TN = opgemm_tile_cols(plan.wide);
-    std::vector<OpGroup> gv;
-    std::vector<OpCol> cols;
-    std::vector<OpTile> tiles;
+    // shape of each kernel set in this stage
+    std::map<int, std::pair<size_t, size_t>> set_cols;  // kset -> (columns, groups)
     for (auto& kv : groups) {
-        const Key& k = kv.first;
-        OpGroup g;
-        memset(&g, 0, sizeof(g));
-        g.kset = k.kset;
-        g.L = set_L[k.kset];
-        g.NP = g.L / 2;
-        g.nterms = k.nterms;
-        for (int t = 0; t < 3; ++t) { g.kern[t] = k.kern[t]; g.sign[t] = k.sign[t]; }
-        g.col_off = (int32_t)cols.size();
-        g.ncols = (int32_t)kv.second.size();
-        g.kbase = set_kbase[k.kset];
-        g.band = -1;
-        const int gi = (int)gv.size();
-        gv.push_back(g);
-        cols.insert(cols.end(), kv.second.begin(), kv.second.end());
-        for (int c0 = 0; c0 < g.ncols; c0 += TN)
-            for (int r = 0; r < g.NP; r += TM) tiles.push_back(OpTile{gi, r, c0, std::min(TN, g.ncols - c0)});
+        auto& e = set_cols[kv.first.kset];
+        e.first += kv.second.size();
+        e.second += 1;
     }
-    tiles = xcd_order(tiles);
-    plan.ntiles = (int)tiles.size();
-    if (plan.ntiles == 0) return SDFT_OK;
-    TRY(upload_vec(ctx, &plan.d_tiles, tiles));
-    TRY(upload_vec(ctx, &plan.d_groups, gv));
-    TRY(upload_vec(ctx, &plan.d_cols, cols));
-    plan.ngroups = (int)gv.size();
-    for (int k = 0; k < 2; ++k) {
-        OpCol* act = nullptr;
-        TRY(dalloc(ctx, &act, (int64_t)cols.size()));
-        plan.d_cols_act[k] = act;
-        TRY(dalloc(ctx, &plan.d_count[k], (int64_t)gv.size()));
+    auto shape_of = [&](int kset) {
+        if (kset < (int)set_shape.size() && set_shape[kset] >= 0) return set_shape[kset];
+        const auto& e = set_cols[kset];
+        return e.first >= 48 * e.second ? (int)CONV_WIDE64 : (int)CONV_TALL;
+    };
+    for (int sh = 0; sh < CONV_NSHAPES; ++sh) {
+        OpPlan& plan = stage.sub[sh];
+        plan = OpPlan();
+        plan.shape = sh;
+        plan.wide = sh != CONV_TALL;
+        const int TM = conv_tile_rows(sh), TN = conv_tile_cols(sh);
+        std::vector<OpGroup> gv;
+        std::vector<OpCol> cols;
+        std::vector<OpTile> tiles;
+        for (auto& kv : groups) {
+            const Key& k = kv.first;
+            if (shape_of(k.kset) != sh) continue;
+            OpGroup g;
+            memset(&g, 0, sizeof(g));
+            g.kset = k.kset;
+            g.L = set_L[k.kset];
+            g.NP = g.L / 2;
+            g.nterms = k.nterms;
+            for (int t = 0; t < 3; ++t) { g.kern[t] = k.kern[t]; g.sign[t] = k.sign[t]; }
+            g.col_off = (int32_t)cols.size();
+            g.ncols = (int32_t)kv.second.size();
+            g.kbase = set_kbase[k.kset];
+            g.band = -1;
+            const int gi = (int)gv.size();
+            gv.push_back(g);
+            cols.insert(cols.end(), kv.second.begin(), kv.second.end());
+            for (int c0 = 0; c0 < g.ncols; c0 += TN)
+                for (int r = 0; r < g.NP; r += TM) tiles.push_back(OpTile{gi, r, c0, std::min(TN, g.ncols - c0)});
+        }
+        tiles = xcd_order(tiles);
+        plan.ntiles = (int)tiles.size();
+        if (plan.ntiles == 0) continue;
+        TRY(upload_vec(ctx, &plan.d_tiles, tiles));
+        TRY(upload_vec(ctx, &plan.d_groups, gv));
+        TRY(upload_vec(ctx, &plan.d_cols, cols));
+        plan.ngroups = (int)gv.size();
+        for (int k = 0; k < 2; ++k) {
+            OpCol* act = nullptr;
+            TRY(dalloc(ctx, &act, (int64_t)cols.size()));
+            plan.d_cols_act[k] = act;
+            TRY(dalloc(ctx, &plan.d_count[k], (int64_t)gv.size()));
+        }
+    }
+    return SDFT_OK;
+}
+
+// releases the device tables of a stage's plans (autotuner candidates, replaced plans)
+static void free_stage(sdft_ctx* ctx, Ctx::ConvStage& stage) {
+    for (OpPlan& pl : stage.sub) {
+        void* ptrs[7] = {pl.d_tiles, pl.d_groups, pl.d_cols, pl.d_cols_act[0], pl.d_cols_act[1], pl.d_count[0],
+                         pl.d_count[1]};
+        for (void* q : ptrs) {
+            if (!q) continue;
+            auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), q);
+            if (it != ctx->allocs.end()) ctx->allocs.erase(it);
+            (void)hipFree(q);
+        }
+        pl = OpPlan();
+    }
+}
+
+static int rebuild_conv_plans(sdft_ctx* ctx) {
+    Ctx& c = ctx->c;
+    Ctx::ConvStage* st[3] = {&c.fwd_plan, &c.pre_plan, &c.back_plan};
+    const int base[3] = {0, c.n_fwd_jobs, c.n_fwd_jobs + c.n_pre_jobs};
+    const int n[3] = {c.n_fwd_jobs, c.n_pre_jobs, c.n_back_jobs};
+    for (int k = 0; k < 3; ++k) {
+        free_stage(ctx, *st[k]);
+        TRY(build_conv_plan(ctx, ctx->jobs_h, base[k], n[k], ctx->canon, ctx->set_shape[k], nullptr, ctx->set_L_h,
+                            ctx->set_kbase_h, *st[k]));
     }
     return SDFT_OK;
 }
@@ -422,6 +477,8 @@ static void build_chain_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const s
     ch.ok32 = chain_lds_bytes(ch, sizeof(float)) <= 160 * 1024;
     ch.ok64 = chain_lds_bytes(ch, sizeof(double)) <= 160 * 1024;
     ch.ok = ch.ok32 || ch.ok64;
+    ch.elig32 = ch.ok32;
+    ch.elig64 = ch.ok64;
 }
 
 // Propagator plan of species u: valid only if every (sweep, a, b) column applies the same
@@ -829,14 +886,12 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         }
         const char* kenv = getenv("SOFTDFT_KERNELS");
         c.use_gemm = !(kenv && strcmp(kenv, "direct") == 0);
-        std::vector<int> dummy;
-        if ((rc = build_conv_plan(ctx, jobs, 0, c.n_fwd_jobs, canon, dummy, set_L, set_kbase, c.fwd_plan)) != SDFT_OK)
-            return fail(rc);
-        if ((rc = build_conv_plan(ctx, jobs, c.n_fwd_jobs, c.n_pre_jobs, canon, dummy, set_L, set_kbase, c.pre_plan)) != SDFT_OK)
-            return fail(rc);
-        if ((rc = build_conv_plan(ctx, jobs, c.n_fwd_jobs + c.n_pre_jobs, c.n_back_jobs, canon, dummy, set_L, set_kbase,
-                                  c.back_plan)) != SDFT_OK)
-            return fail(rc);
+        ctx->jobs_h = jobs;
+        ctx->canon = canon;
+        ctx->set_L_h = set_L;
+        ctx->set_kbase_h = set_kbase;
+        for (auto& v : ctx->set_shape) v.assign(ctx->nsets, -1);
+        if ((rc = rebuild_conv_plans(ctx)) != SDFT_OK) return fail(rc);
         c.prop_plan.assign(nsp, OpPlan());
         for (int u = 0; u < nsp; ++u)
             if ((rc = build_prop_plan(ctx, sys, u, canon, set_L, set_kbase, band, c.prop_plan[u])) != SDFT_OK) return fail(rc);
@@ -1314,6 +1369,168 @@ extern "C" int32_t sdft_observables(sdft_ctx_t* ctx, double* omega, double* alph
             p_cvt[2 * p] = l;
             p_cvt[2 * p + 1] = r;
         }
+    }
+    return SDFT_OK;
+}
+
+// HIP-event time of fn() on the context's stream: one warm-up call, then reps calls; us per call
+template <class F>
+static int time_launches(sdft_ctx* ctx, hipEvent_t e0, hipEvent_t e1, int reps, F&& fn, double* us) {
+    fn();
+    HIP_CHECK_RET(hipEventRecord(e0, ctx->stream));
+    for (int r = 0; r < reps; ++r) fn();
+    HIP_CHECK_RET(hipEventRecord(e1, ctx->stream));
+    HIP_CHECK_RET(hipEventSynchronize(e1));
+    HIP_CHECK_RET(hipGetLastError());
+    float ms = 0.f;
+    HIP_CHECK_RET(hipEventElapsedTime(&ms, e0, e1));
+    *us = 1e3 * (double)ms / reps;
+    return SDFT_OK;
+}
+
+static int autotune_run(sdft_ctx* ctx, int reps, hipEvent_t e0, hipEvent_t e1) {
+    Ctx& c = ctx->c;
+    hipStream_t st = ctx->stream;
+    // ---- propagators: persistent chain kernel vs one operator-GEMM launch per step, on the banded
+    //      operator both take in the mixed path
+    for (int u = 0; u < c.nsp; ++u) {
+        if (!c.use_gemm || c.species[u].nseg < 2 || !c.chain_plan[u].ok) continue;
+        for (int prec : {32, 64}) {
+            bool& ok = prec == 32 ? c.chain_plan[u].ok32 : c.chain_plan[u].ok64;
+            if (!ok) continue;
+            launch_set_evalflag(st, c.d_evalflag, prec, c.P);
+            auto run = [&](bool chain) {
+                if (prec == 32) ctx->p32.propagate(st, c, 32, u, chain);
+                else ctx->p64.propagate(st, c, 64, u, chain);
+            };
+            double t_chain = 0.0, t_step = 0.0;
+            TRY(time_launches(ctx, e0, e1, reps, [&]() { run(true); }, &t_chain));
+            TRY(time_launches(ctx, e0, e1, reps, [&]() { run(false); }, &t_step));
+            ok = t_chain <= t_step;
+            sdft_tune_entry_t e;
+            e.kind = SDFT_TUNE_PROPAGATOR; e.key = u; e.precision = prec; e.choice = ok ? 1 : 0;
+            e.us[0] = t_step; e.us[1] = t_chain; e.us[2] = -1.0;
+            ctx->tuning.push_back(e);
+        }
+    }
+    // ---- convolution tile shapes per grid-length bin and stage
+    if (!c.use_gemm) return SDFT_OK;
+    std::map<int, std::vector<char>> bins;  // NP bound -> points of the bin
+    std::vector<int> set_bin(ctx->nsets, 0);
+    for (int p = 0; p < c.P; ++p) {
+        const int bound = (ctx->pts[p].NP + 255) / 256 * 256;
+        auto& m = bins[bound];
+        if (m.empty()) m.assign(c.P, 0);
+        m[p] = 1;
+        set_bin[ctx->pts[p].kset] = bound;
+    }
+    const int base[3] = {0, c.n_fwd_jobs, c.n_fwd_jobs + c.n_pre_jobs};
+    const int njob[3] = {c.n_fwd_jobs, c.n_pre_jobs, c.n_back_jobs};
+    for (auto& bin : bins) {
+        for (int k = 0; k < 3; ++k) {
+            if (njob[k] == 0) continue;
+            double t[CONV_NSHAPES];
+            for (int sh = 0; sh < CONV_NSHAPES; ++sh) {
+                const std::vector<int> forced(ctx->nsets, sh);
+                Ctx::ConvStage tmp;
+                TRY(build_conv_plan(ctx, ctx->jobs_h, base[k], njob[k], ctx->canon, forced, &bin.second,
+                                    ctx->set_L_h, ctx->set_kbase_h, tmp));
+                const OpPlan* pl[1] = {&tmp.sub[sh]};
+                t[sh] = 0.0;
+                for (int prec : {32, 64}) {
+                    launch_set_evalflag(st, c.d_evalflag, prec, c.P);
+                    launch_compact_plans(st, c, prec, pl, 1);
+                    double us = 0.0;
+                    TRY(time_launches(ctx, e0, e1, reps, [&]() {
+                        if (prec == 32)
+                            launch_opgemm_conv<float>(st, c, 32, tmp.sub[sh], ctx->p32.kern, ctx->p32.rhoT,
+                                                      ctx->p32.dphi, ctx->p32.nw, c.d_fx);
+                        else
+                            launch_opgemm_conv<double>(st, c, 64, tmp.sub[sh], ctx->p64.kern, ctx->p64.rhoT,
+                                                       ctx->p64.dphi, ctx->p64.nw, c.d_fx);
+                    }, &us));
+                    t[sh] += us;
+                }
+                HIP_CHECK_RET(hipStreamSynchronize(st));
+                free_stage(ctx, tmp);
+            }
+            int best = 0;
+            for (int sh = 1; sh < CONV_NSHAPES; ++sh)
+                if (t[sh] < t[best]) best = sh;
+            for (int s = 0; s < ctx->nsets; ++s)
+                if (set_bin[s] == bin.first) ctx->set_shape[k][s] = best;
+            sdft_tune_entry_t e;
+            e.kind = SDFT_TUNE_CONV_FWD + k; e.key = bin.first; e.precision = 0; e.choice = best;
+            for (int sh = 0; sh < CONV_NSHAPES; ++sh) e.us[sh] = t[sh];
+            ctx->tuning.push_back(e);
+        }
+    }
+    return rebuild_conv_plans(ctx);
+}
+
+extern "C" int32_t sdft_autotune(sdft_ctx_t* ctx, int32_t reps, sdft_tune_entry_t* entries, int32_t capacity,
+                                 int32_t* n_entries) {
+    if (!ctx || reps < 1 || (capacity > 0 && !entries)) return SDFT_E_ARG;
+    Ctx& c = ctx->c;
+    ctx->tuning.clear();
+    std::vector<int32_t> saved(c.P);
+    HIP_CHECK_RET(hipMemcpyAsync(saved.data(), c.d_evalflag, c.P * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    const int band32 = c.band_fp32, band64 = c.band_fp64;
+    c.band_fp32 = 1;
+    c.band_fp64 = 1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess ? SDFT_OK : SDFT_E_HIP;
+    if (rc == SDFT_OK) rc = autotune_run(ctx, reps, e0, e1);
+    else set_error(ctx, SDFT_E_HIP, "hipEventCreate failed");
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    c.band_fp32 = band32;
+    c.band_fp64 = band64;
+    ctx->solved = false;
+    if (rc != SDFT_OK) return rc;
+    HIP_CHECK_RET(hipMemcpyAsync(c.d_evalflag, saved.data(), c.P * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    const int n = (int)ctx->tuning.size();
+    for (int i = 0; i < n && i < capacity; ++i) entries[i] = ctx->tuning[i];
+    if (n_entries) *n_entries = n;
+    return SDFT_OK;
+}
+
+extern "C" int32_t sdft_set_tuning(sdft_ctx_t* ctx, const sdft_tune_entry_t* entries, int32_t n) {
+    if (!ctx || n < 0 || (n > 0 && !entries)) return SDFT_E_ARG;
+    Ctx& c = ctx->c;
+    bool conv = false;
+    for (int i = 0; i < n; ++i) {
+        const sdft_tune_entry_t& e = entries[i];
+        if (e.kind == SDFT_TUNE_PROPAGATOR) {
+            if (e.key < 0 || e.key >= c.nsp || (e.choice != 0 && e.choice != 1) ||
+                (e.precision != 0 && e.precision != 32 && e.precision != 64)) {
+                set_error(ctx, SDFT_E_ARG, "tuning entry %d: bad propagator key / choice / precision", i);
+                return SDFT_E_ARG;
+            }
+            ChainPlanHost& ch = c.chain_plan[e.key];
+            if (e.precision != 64) ch.ok32 = ch.elig32 && e.choice == 1;
+            if (e.precision != 32) ch.ok64 = ch.elig64 && e.choice == 1;
+        } else if (e.kind >= SDFT_TUNE_CONV_FWD && e.kind <= SDFT_TUNE_CONV_BACK) {
+            if (e.choice < -1 || e.choice >= CONV_NSHAPES) {
+                set_error(ctx, SDFT_E_ARG, "tuning entry %d: tile shape %d out of range", i, e.choice);
+                return SDFT_E_ARG;
+            }
+            const int k = e.kind - SDFT_TUNE_CONV_FWD;
+            for (int p = 0; p < c.P; ++p) {
+                const int bound = (ctx->pts[p].NP + 255) / 256 * 256;
+                if (e.key == 0 || e.key == bound) ctx->set_shape[k][ctx->pts[p].kset] = e.choice;
+            }
+            conv = true;
+        } else {
+            set_error(ctx, SDFT_E_ARG, "tuning entry %d: unknown kind %d", i, e.kind);
+            return SDFT_E_ARG;
+        }
+    }
+    if (conv && c.use_gemm) {
+        HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+        TRY(rebuild_conv_plans(ctx));
     }
     return SDFT_OK;
 }

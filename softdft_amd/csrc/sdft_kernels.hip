@@ -1640,6 +1640,56 @@ hipError_t prepare_kernel_attributes(const Ctx& c) {
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
+// propagators of species u (child_propagators.jl / parent_propagators.jl / handle_fixed.jl): the
+// persistent chain kernel where the plan allows it (and allow_chain), else one launch per step
+template <typename T>
+void Pipeline<T>::propagate(hipStream_t st, const Ctx& c, int prec, int u, bool allow_chain) {
+    const int P = c.P;
+    const int maxNP = c.maxNP;
+    const SpeciesHost& sh = c.species[u];
+    PropArgs A;
+    A.u = u; A.nseg = sh.nseg; A.S = sh.S; A.fixed_seg = sh.fixed_seg;
+    A.state_off = sh.state_off; A.maxd_off = sh.maxd_off; A.goff = sh.goff; A.GF = c.GF;
+    A.rescale = sh.rescale; A.nsp = c.nsp; A.nmax = c.nmax_seg; A.smax = smax;
+    const bool banded = (sizeof(T) == 4) ? c.band_fp32 != 0 : c.band_fp64 != 0;
+    A.band = banded ? 1 : -1;
+    const bool gemm = c.use_gemm && c.prop_plan[u].ntiles > 0;
+    if (allow_chain && banded && (sizeof(T) == 4 ? c.chain_plan[u].ok32 : c.chain_plan[u].ok64)) {
+        launch_chain<T>(st, c, prec, u, kern, elam, hl, trapez, gC, gP, smax);
+        if (sh.rescale)
+            hipLaunchKernelGGL(k_scale_prefix, dim3((P + 63) / 64), dim3(64), 0, st, c.d_evalflag, prec, P, u,
+                               sh.nseg, sh.fixed_seg, smax, lsc, c.nsp, c.nmax_seg);
+        return;
+    }
+    for (int s = 1; s < sh.nseg; ++s) {
+        A.s = s;
+        // sweeps whose parent is the grafted segment use the direct real-space sums
+        const bool fix0 = sh.fixed_seg == 0 && s == sh.nseg - 1;  // child sweep, parent = segment 0
+        const bool fix1 = sh.fixed_seg == 0 && s == 1;            // parent sweep, parent = segment 0
+        if (!gemm) {
+            hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(maxNP, 256, P, 2), dim3(256), 0, st, c.d_sys,
+                               c.d_pts, c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl, trapez,
+                               gC, gP, c.d_band, 0);
+            continue;
+        }
+        const int skip = (fix0 && fix1) ? 2 : fix0 ? 0 : fix1 ? 1 : -1;
+        if (skip != 2)
+            launch_opgemm_prop<T>(st, c, prec, c.prop_plan[u], u, s, skip, banded, kern, elam, hl, trapez, gC, gP,
+                                  fbuf, smax);
+        if (skip >= 0)
+            hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(maxNP, 256, P, skip == 2 ? 2 : 1), dim3(256), 0, st,
+                               c.d_sys, c.d_pts, c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl,
+                               trapez, gC, gP, c.d_band, skip == 2 ? 0 : skip);
+    }
+    if (sh.rescale)
+        hipLaunchKernelGGL(k_scale_prefix, dim3((P + 63) / 64), dim3(64), 0, st, c.d_evalflag, prec, P, u, sh.nseg,
+                           sh.fixed_seg, smax, lsc, c.nsp, c.nmax_seg);
+    if (sh.fixed_seg >= 0) {
+        hipLaunchKernelGGL(k_fixed_lambda<T>, dim3(P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag, prec, A,
+                           c.d_seg_states, c.d_maxd, elam, trapez, gC, gP, c.d_fixed_lambda);
+    }
+}
+
 template <typename T>
 void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double* x, double* fx) {
     const int P = c.P;
@@ -1651,50 +1701,7 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
         hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st,
                            reinterpret_cast<double*>(smax), ns);
     }
-    for (int u = 0; u < c.nsp; ++u) {
-        const SpeciesHost& sh = c.species[u];
-        PropArgs A;
-        A.u = u; A.nseg = sh.nseg; A.S = sh.S; A.fixed_seg = sh.fixed_seg;
-        A.state_off = sh.state_off; A.maxd_off = sh.maxd_off; A.goff = sh.goff; A.GF = c.GF;
-        A.rescale = sh.rescale; A.nsp = c.nsp; A.nmax = c.nmax_seg; A.smax = smax;
-        const bool banded = (sizeof(T) == 4) ? c.band_fp32 != 0 : c.band_fp64 != 0;
-        A.band = banded ? 1 : -1;
-        const bool gemm = c.use_gemm && c.prop_plan[u].ntiles > 0;
-        if (banded && (sizeof(T) == 4 ? c.chain_plan[u].ok32 : c.chain_plan[u].ok64)) {
-            launch_chain<T>(st, c, prec, u, kern, elam, hl, trapez, gC, gP, smax);
-            if (sh.rescale)
-                hipLaunchKernelGGL(k_scale_prefix, dim3((P + 63) / 64), dim3(64), 0, st, c.d_evalflag, prec, P, u,
-                                   sh.nseg, sh.fixed_seg, smax, lsc, c.nsp, c.nmax_seg);
-            continue;
-        }
-        for (int s = 1; s < sh.nseg; ++s) {
-            A.s = s;
-            // sweeps whose parent is the grafted segment use the direct real-space sums
-            const bool fix0 = sh.fixed_seg == 0 && s == sh.nseg - 1;  // child sweep, parent = segment 0
-            const bool fix1 = sh.fixed_seg == 0 && s == 1;            // parent sweep, parent = segment 0
-            if (!gemm) {
-                hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(maxNP, 256, P, 2), dim3(256), 0, st, c.d_sys,
-                                   c.d_pts, c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl, trapez,
-                                   gC, gP, c.d_band, 0);
-                continue;
-            }
-            const int skip = (fix0 && fix1) ? 2 : fix0 ? 0 : fix1 ? 1 : -1;
-            if (skip != 2)
-                launch_opgemm_prop<T>(st, c, prec, c.prop_plan[u], u, s, skip, banded, kern, elam, hl, trapez, gC, gP,
-                                      fbuf, smax);
-            if (skip >= 0)
-                hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(maxNP, 256, P, skip == 2 ? 2 : 1), dim3(256), 0, st,
-                                   c.d_sys, c.d_pts, c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl,
-                                   trapez, gC, gP, c.d_band, skip == 2 ? 0 : skip);
-        }
-        if (sh.rescale)
-            hipLaunchKernelGGL(k_scale_prefix, dim3((P + 63) / 64), dim3(64), 0, st, c.d_evalflag, prec, P, u, sh.nseg,
-                               sh.fixed_seg, smax, lsc, c.nsp, c.nmax_seg);
-        if (sh.fixed_seg >= 0) {
-            hipLaunchKernelGGL(k_fixed_lambda<T>, dim3(P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag, prec, A,
-                               c.d_seg_states, c.d_maxd, elam, trapez, gC, gP, c.d_fixed_lambda);
-        }
-    }
+    for (int u = 0; u < c.nsp; ++u) propagate(st, c, prec, u, true);
     hipLaunchKernelGGL((k_accumulate<T, 256>), grid2(maxNP, ACC_POINTS, P), dim3(256), accumulate_lds_bytes(c), st, c.d_sys,
                        c.d_pts, c.d_evalflag,
                        prec, c.d_spec, c.d_goffs, c.GF, c.d_seg_states, c.d_maxd, elam, trapez, gC, gP,
@@ -1726,10 +1733,11 @@ void Pipeline<T>::launch_prop_step(hipStream_t st, const Ctx& c, int prec, int u
 // forward: weighted densities; back: shared partials first, then the projections that add them
 template <typename T>
 void Pipeline<T>::launch_conv(hipStream_t st, const Ctx& c, int prec, bool back, double* fx) {
-    auto run = [&](const OpPlan& plan, int job0, int nj) {
+    auto run = [&](const Ctx::ConvStage& stage, int job0, int nj) {
         if (nj == 0) return;
         if (c.use_gemm) {
-            launch_opgemm_conv<T>(st, c, prec, plan, kern, rhoT, dphi, nw, fx);
+            for (int sh = 0; sh < CONV_NSHAPES; ++sh)
+                launch_opgemm_conv<T>(st, c, prec, stage.sub[sh], kern, rhoT, dphi, nw, fx);
             return;
         }
         hipLaunchKernelGGL((k_conv<T, 256>), grid2(c.maxNP, 256, nj, c.P), dim3(256), 0, st, c.d_sys, c.d_pts,

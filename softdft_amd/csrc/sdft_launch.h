@@ -32,9 +32,18 @@ struct OpGroup {
 struct OpCol { int32_t p, job; };
 struct PropCol { int32_t p, sweep, a, b; };
 
+// tile shapes of the convolution GEMMs: rows x columns per plan tile
+enum ConvShape : int {
+    CONV_TALL = 0,     // 128 x 16 (k_opgemm): few columns per operator (ragged batches)
+    CONV_WIDE64 = 1,   // 64 x 64 (k_convgemm)
+    CONV_WIDE128 = 2,  // 128 x 64 (k_convgemm; FP64 runs it as two 64-row workgroups per tile)
+    CONV_NSHAPES = 3
+};
+
 struct OpPlan {
     int ntiles = 0;
-    bool wide = true;
+    bool wide = true;          // propagator plans: 64 x 64 tiles (else 128 x 16)
+    int shape = CONV_WIDE64;   // convolution plans
     bool full_pairs = false;   // propagator plans: every (sweep, a, b) column present, b fastest
     void* d_tiles = nullptr;
     void* d_groups = nullptr;
@@ -49,7 +58,8 @@ struct OpPlan {
 // persistent banded chain kernel of one species (ok: eligible for the banded operator)
 struct ChainPlanHost {
     bool ok = false;                               // eligible in at least one precision
-    bool ok32 = false, ok64 = false;               // the working set fits LDS at that precision
+    bool ok32 = false, ok64 = false;               // runs at that precision (eligible and chosen)
+    bool elig32 = false, elig64 = false;           // the working set fits LDS at that precision
     bool homog = false;                            // every segment carries the same state beads
     int S = 1;
     int ck[2][SDFT_MAX_S * SDFT_MAX_S];            // canonical bond kernel of (sweep, a * S + b), -1 absent
@@ -77,7 +87,10 @@ struct Ctx {
     int m = 5;                      // Anderson depth the history buffers were sized for
     int band_fp64 = 0;              // FP64 propagators banded (mixed-precision polish only)
     bool use_gemm = true;           // MFMA operator-GEMM path (else the direct reference kernels)
-    OpPlan fwd_plan, pre_plan, back_plan;
+    // convolution stages (forward, shared back-projection partials, back projections), each split
+    // by tile shape: the groups of a kernel set (one grid length) all run in that set's shape
+    struct ConvStage { OpPlan sub[CONV_NSHAPES]; };
+    ConvStage fwd_plan, pre_plan, back_plan;
     std::vector<OpPlan> prop_plan;  // per species (ntiles == 0: direct kernel)
     std::vector<ChainPlanHost> chain_plan;  // per species
     std::vector<SpeciesHost> species;
@@ -171,6 +184,7 @@ struct Pipeline {
     void segment_profile(hipStream_t st, const Ctx& c, int u, int p, double* out);
     // single-launch entry points used by sdft_kernel_profile
     void launch_prop_step(hipStream_t st, const Ctx& c, int prec, int u, int s, bool banded);
+    void propagate(hipStream_t st, const Ctx& c, int prec, int u, bool allow_chain);
     void launch_conv(hipStream_t st, const Ctx& c, int prec, bool back, double* fx);
     void launch_pointwise(hipStream_t st, const Ctx& c, int prec);
 };
@@ -189,8 +203,11 @@ void launch_opgemm_prop(hipStream_t st, const Ctx& c, int prec, const OpPlan& pl
                         bool banded, const T* kern, const T* elam, const T* hl, const T* trapez, T* gC, T* gP,
                         T* fbuf, unsigned long long* smax);
 int opgemm_tile_rows(bool wide);
-int conv_tile_rows(bool wide);   // rows per tile of the convolution GEMMs
 int opgemm_tile_cols(bool wide);
+int conv_tile_rows(int shape);   // rows / columns per tile of the convolution GEMMs
+int conv_tile_cols(int shape);
+// compacts the active columns of the given convolution plans for one precision
+void launch_compact_plans(hipStream_t st, const Ctx& c, int prec, const OpPlan* const* plans, int nplans);
 
 void launch_build_kernels(hipStream_t st, const DevSystem* sys, int K, int nsets, int maxL, const int32_t* set_L,
                           const int64_t* set_kbase, const double* set_dz, double* wbuf, double* c64, float* c32);

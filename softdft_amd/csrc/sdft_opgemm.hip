@@ -326,17 +326,22 @@ __global__ void __launch_bounds__(256) k_opgemm(const DevSystem* __restrict__ ps
 // still check each column's point, so points that converge between two polls just idle. A column's
 // result does not depend on the other columns of its tile, so this leaves every value unchanged.
 // --------------------------------------------------------------------------
+constexpr int COMPACT_MAX_PLANS = 3 * CONV_NSHAPES;
+
 struct CompactArgs {
-    const OpGroup* groups[3];
-    const OpCol* cols[3];
-    OpCol* act[3];
-    int32_t* count[3];
-    int32_t ngroups[3];
+    const OpGroup* groups[COMPACT_MAX_PLANS];
+    const OpCol* cols[COMPACT_MAX_PLANS];
+    OpCol* act[COMPACT_MAX_PLANS];
+    int32_t* count[COMPACT_MAX_PLANS];
+    int32_t ngroups[COMPACT_MAX_PLANS];
+    int32_t nplans;
 };
 
+// one workgroup per group of the listed plans (in order): the group's columns whose point evaluates
+// at `prec`, compacted in order, and their number
 __global__ void __launch_bounds__(256) k_compact_cols(CompactArgs ca, const int32_t* __restrict__ evalflag, int prec) {
     int g = blockIdx.x, pl = 0;
-    while (pl < 2 && g >= ca.ngroups[pl]) g -= ca.ngroups[pl++];
+    while (pl < ca.nplans - 1 && g >= ca.ngroups[pl]) g -= ca.ngroups[pl++];
     if (g >= ca.ngroups[pl]) return;
     const OpGroup& G = ca.groups[pl][g];
     const OpCol* in = ca.cols[pl] + G.col_off;
@@ -365,20 +370,31 @@ __global__ void __launch_bounds__(256) k_compact_cols(CompactArgs ca, const int3
     if (tid == 0) ca.count[pl][g] = base;
 }
 
-void launch_compact_cols(hipStream_t st, const Ctx& c, int prec) {
+void launch_compact_plans(hipStream_t st, const Ctx& c, int prec, const OpPlan* const* plans, int nplans) {
     CompactArgs ca;
     memset(&ca, 0, sizeof(ca));
-    const OpPlan* pl[3] = {&c.fwd_plan, &c.pre_plan, &c.back_plan};
-    int tot = 0;
-    for (int i = 0; i < 3; ++i) {
-        ca.groups[i] = (const OpGroup*)pl[i]->d_groups;
-        ca.cols[i] = (const OpCol*)pl[i]->d_cols;
-        ca.act[i] = (OpCol*)pl[i]->d_cols_act[prec == 64 ? 1 : 0];
-        ca.count[i] = pl[i]->d_count[prec == 64 ? 1 : 0];
-        ca.ngroups[i] = pl[i]->ntiles > 0 ? pl[i]->ngroups : 0;
-        tot += ca.ngroups[i];
+    int tot = 0, k = 0;
+    for (int i = 0; i < nplans && k < COMPACT_MAX_PLANS; ++i) {
+        const OpPlan* pl = plans[i];
+        if (pl->ntiles == 0) continue;
+        ca.groups[k] = (const OpGroup*)pl->d_groups;
+        ca.cols[k] = (const OpCol*)pl->d_cols;
+        ca.act[k] = (OpCol*)pl->d_cols_act[prec == 64 ? 1 : 0];
+        ca.count[k] = pl->d_count[prec == 64 ? 1 : 0];
+        ca.ngroups[k] = pl->ngroups;
+        tot += pl->ngroups;
+        ++k;
     }
+    ca.nplans = k;
     if (tot > 0) hipLaunchKernelGGL(k_compact_cols, dim3(tot), dim3(256), 0, st, ca, c.d_evalflag, prec);
+}
+
+void launch_compact_cols(hipStream_t st, const Ctx& c, int prec) {
+    const OpPlan* pl[COMPACT_MAX_PLANS];
+    int n = 0;
+    for (const Ctx::ConvStage* stg : {&c.fwd_plan, &c.pre_plan, &c.back_plan})
+        for (int sh = 0; sh < CONV_NSHAPES; ++sh) pl[n++] = &stg->sub[sh];
+    launch_compact_plans(st, c, prec, pl, n);
 }
 
 // --------------------------------------------------------------------------
@@ -390,11 +406,10 @@ void launch_compact_cols(hipStream_t st, const Ctx& c, int prec) {
 // load reads 64 consecutive inputs of one column, and both the staging writes and the
 // B reads are free of bank conflicts.
 // --------------------------------------------------------------------------
-constexpr int CONV_WM = 1;  // plan tile rows = 64 * CONV_WM (2 measured slower on MI355X: fewer workgroups)
-
 // A plan tile of 64 * WM * RSPLIT rows is computed by RSPLIT workgroups of 64 * WM rows
-// (blockIdx.x = tile + ntiles * part). With CONV_WM > 1, FP32 takes whole tiles and FP64 splits
-// them (WM = 1: its accumulators and operands already fill the register file).
+// (blockIdx.x = tile + ntiles * part). The 128-row shape (CONV_WIDE128) runs FP32 as WM = 2 and
+// FP64 as WM = 1, RSPLIT = 2; which shape a grid length takes is a plan property (heuristic, or
+// measured by sdft_autotune).
 template <typename T, int WM, int WN, int RSPLIT>
 __global__ void __launch_bounds__(256) k_convgemm(const DevPoint* __restrict__ pts,
                                                   const int32_t* __restrict__ evalflag, int prec,
@@ -572,19 +587,30 @@ void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& pl
     if (plan.ntiles == 0) return;
     PropStep ps;
     memset(&ps, 0, sizeof(ps));
-    if (plan.wide) {
-        constexpr int WM = sizeof(T) == 4 ? CONV_WM : 1, RS = CONV_WM / WM;
-        hipLaunchKernelGGL((k_convgemm<T, WM, 4, RS>), dim3(plan.ntiles * RS), dim3(256), 0, st, c.d_pts,
-                           c.d_evalflag, prec, (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups,
-                           (const OpCol*)plan.d_cols_act[prec == 64 ? 1 : 0], plan.d_count[prec == 64 ? 1 : 0],
-                           c.d_jobs, kern, c.K, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx, c.d_wsum, c.Fws);
-    } else {
-        hipLaunchKernelGGL((k_opgemm<T, 2, 1, false>), dim3(plan.ntiles), dim3(256), 0, st, c.d_sys, c.d_pts,
-                           c.d_evalflag, prec, (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups, plan.d_cols,
-                           c.d_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx, c.d_wsum, c.Fws, ps,
-                           c.d_seg_states, c.d_maxd,
-                           (const T*)nullptr, (const T*)nullptr, (const T*)nullptr, (T*)nullptr, (T*)nullptr,
-                           (const T*)nullptr, (T*)nullptr);
+    const OpCol* act = (const OpCol*)plan.d_cols_act[prec == 64 ? 1 : 0];
+    const int32_t* cnt = plan.d_count[prec == 64 ? 1 : 0];
+    auto wide = [&](auto kfn, int rsplit) {
+        hipLaunchKernelGGL(kfn, dim3(plan.ntiles * rsplit), dim3(256), 0, st, c.d_pts, c.d_evalflag, prec,
+                           (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups, act, cnt, c.d_jobs, kern, c.K,
+                           rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx, c.d_wsum, c.Fws);
+    };
+    switch (plan.shape) {
+        case CONV_WIDE64:
+            wide(k_convgemm<T, 1, 4, 1>, 1);
+            break;
+        case CONV_WIDE128:
+            // FP32 takes a 128-row tile in one workgroup; FP64 (whose accumulators and operands already
+            // fill the register file at 64 rows) splits it over two
+            if (sizeof(T) == 4) wide(k_convgemm<T, 2, 4, 1>, 1);
+            else wide(k_convgemm<T, 1, 4, 2>, 2);
+            break;
+        default:
+            hipLaunchKernelGGL((k_opgemm<T, 2, 1, false>), dim3(plan.ntiles), dim3(256), 0, st, c.d_sys, c.d_pts,
+                               c.d_evalflag, prec, (const OpTile*)plan.d_tiles, (const OpGroup*)plan.d_groups,
+                               plan.d_cols, c.d_jobs, kern, rhoT, c.Frt, dphi, c.Fdp, nw, c.Fnw, fx, c.d_wsum, c.Fws,
+                               ps, c.d_seg_states, c.d_maxd, (const T*)nullptr, (const T*)nullptr, (const T*)nullptr,
+                               (T*)nullptr, (T*)nullptr, (const T*)nullptr, (T*)nullptr);
+            break;
     }
 }
 
@@ -630,7 +656,8 @@ template void launch_opgemm_prop<double>(hipStream_t, const Ctx&, int, const OpP
 
 // row/column tile sizes of the two shapes (for the host planner)
 int opgemm_tile_rows(bool wide) { return wide ? TileShape<1, 4>::TM : TileShape<2, 1>::TM; }
-int conv_tile_rows(bool wide) { return wide ? 64 * CONV_WM : TileShape<2, 1>::TM; }
 int opgemm_tile_cols(bool wide) { return wide ? TileShape<1, 4>::TN : TileShape<2, 1>::TN; }
+int conv_tile_rows(int shape) { return shape == CONV_WIDE64 ? 64 : shape == CONV_WIDE128 ? 128 : TileShape<2, 1>::TM; }
+int conv_tile_cols(int shape) { return shape == CONV_TALL ? TileShape<2, 1>::TN : 64; }
 
 }  // namespace sdft
