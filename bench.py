@@ -362,8 +362,8 @@ def main():
             ("k_chain<double> (banded propagators, all steps)", abi.KP_CHAIN_FP64, 64, 1, it64),
             ("k_convgemm<float> (forward weighted densities)", abi.KP_CONV_FWD_FP32, 32, 1, it32),
             ("k_convgemm<double> (forward weighted densities)", abi.KP_CONV_FWD_FP64, 64, 1, it64),
-            ("conv GEMMs<float> (back projections: shared partials, then projections)", abi.KP_CONV_BACK_FP32, 32, 1, it32),
-            ("conv GEMMs<double> (back projections: shared partials, then projections)", abi.KP_CONV_BACK_FP64, 64, 1, it64)]:
+            ("conv GEMMs<float> (back projections: shared partials and projections, one launch)", abi.KP_CONV_BACK_FP32, 32, 1, it32),
+            ("conv GEMMs<double> (back projections: shared partials and projections, one launch)", abi.KP_CONV_BACK_FP64, 64, 1, it64)]:
         try:
             ms, fl = batch.kernel_profile(kc, reps=20)
         except abi.SdftError as e:  # e.g. the autotuner chose the per-step propagators

@@ -396,14 +396,30 @@ static void free_stage(sdft_ctx* ctx, Ctx::ConvStage& stage) {
     }
 }
 
+// job range of convolution stage k (0 forward, 1 shared partials, 2 back projections); with the
+// merged back stage, stage 1 is empty and stage 2 takes the partials and the projections from the
+// table's second copy
+static void stage_range(const Ctx& c, int k, int* base, int* n) {
+    if (k == 0) {
+        *base = 0;
+        *n = c.n_fwd_jobs;
+    } else if (c.merge_back) {
+        *base = c.n_jobs + c.n_fwd_jobs;
+        *n = k == 1 ? 0 : c.n_pre_jobs + c.n_back_jobs;
+    } else {
+        *base = k == 1 ? c.n_fwd_jobs : c.n_fwd_jobs + c.n_pre_jobs;
+        *n = k == 1 ? c.n_pre_jobs : c.n_back_jobs;
+    }
+}
+
 static int rebuild_conv_plans(sdft_ctx* ctx) {
     Ctx& c = ctx->c;
     Ctx::ConvStage* st[3] = {&c.fwd_plan, &c.pre_plan, &c.back_plan};
-    const int base[3] = {0, c.n_fwd_jobs, c.n_fwd_jobs + c.n_pre_jobs};
-    const int n[3] = {c.n_fwd_jobs, c.n_pre_jobs, c.n_back_jobs};
     for (int k = 0; k < 3; ++k) {
         free_stage(ctx, *st[k]);
-        TRY(build_conv_plan(ctx, ctx->jobs_h, base[k], n[k], ctx->canon, ctx->set_shape[k], nullptr, ctx->set_L_h,
+        int base = 0, n = 0;
+        stage_range(c, k, &base, &n);
+        TRY(build_conv_plan(ctx, ctx->jobs_h, base, n, ctx->canon, ctx->set_shape[k], nullptr, ctx->set_L_h,
                             ctx->set_kbase_h, *st[k]));
     }
     return SDFT_OK;
@@ -805,6 +821,17 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     std::vector<ConvJob> jobs(fwd);
     jobs.insert(jobs.end(), pre.begin(), pre.end());
     jobs.insert(jobs.end(), back.begin(), back.end());
+    // second copy for the merged back stage: the projections are stored without the shared
+    // partials, which k_add_partials adds once the launch that computes both has finished
+    c.n_jobs = (int)jobs.size();
+    for (int j = 0; j < c.n_jobs; ++j) {
+        ConvJob J = jobs[j];
+        if (j >= c.n_fwd_jobs + c.n_pre_jobs) {
+            J.addfield = -1;
+            J.addfield2 = -1;
+        }
+        jobs.push_back(J);
+    }
 
     // ---- uploads
     if ((rc = upload(ctx, &c.d_sys, &sys, 1)) != SDFT_OK) return fail(rc);
@@ -886,6 +913,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         }
         const char* kenv = getenv("SOFTDFT_KERNELS");
         c.use_gemm = !(kenv && strcmp(kenv, "direct") == 0);
+        c.merge_back = c.use_gemm && c.n_pre_jobs > 0;
         ctx->jobs_h = jobs;
         ctx->canon = canon;
         ctx->set_L_h = set_L;
@@ -1424,8 +1452,8 @@ static int autotune_run(sdft_ctx* ctx, int reps, hipEvent_t e0, hipEvent_t e1) {
         m[p] = 1;
         set_bin[ctx->pts[p].kset] = bound;
     }
-    const int base[3] = {0, c.n_fwd_jobs, c.n_fwd_jobs + c.n_pre_jobs};
-    const int njob[3] = {c.n_fwd_jobs, c.n_pre_jobs, c.n_back_jobs};
+    int base[3], njob[3];
+    for (int k = 0; k < 3; ++k) stage_range(c, k, &base[k], &njob[k]);
     for (auto& bin : bins) {
         for (int k = 0; k < 3; ++k) {
             if (njob[k] == 0) continue;

@@ -1640,6 +1640,30 @@ hipError_t prepare_kernel_attributes(const Ctx& c) {
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
+// Merged back stage: the back projections were stored without their shared partials (computed in
+// the same launch); add them here, in the order of conv_store (projection + partial + second partial).
+__global__ void __launch_bounds__(256) k_add_partials(const DevPoint* __restrict__ pts,
+                                                      const int32_t* __restrict__ evalflag, int prec,
+                                                      const ConvJob* __restrict__ jobs, int njobs,
+                                                      double* __restrict__ fx, const double* __restrict__ wsum,
+                                                      int Fws) {
+    const int p = blockIdx.y;
+    if (evalflag[p] != prec) return;
+    const DevPoint& P = pts[p];
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= P.NP) return;
+    for (int j = 0; j < njobs; ++j) {
+        const ConvJob& J = jobs[j];
+        if (J.outkind != 1 || (J.addfield < 0 && J.addfield2 < 0)) continue;
+        double* f = fx + P.xoff + (int64_t)J.outfield * P.NP + n;
+        double y = *f;
+        if (J.addfield >= 0) y += wsum[P.base * (int64_t)Fws + (int64_t)J.addfield * P.NP + n];
+        if (J.addfield2 >= 0) y += wsum[P.base * (int64_t)Fws + (int64_t)J.addfield2 * P.NP + n];
+        *f = y;
+        if (J.outfield2 >= 0) fx[P.xoff + (int64_t)J.outfield2 * P.NP + n] = y;
+    }
+}
+
 // propagators of species u (child_propagators.jl / parent_propagators.jl / handle_fixed.jl): the
 // persistent chain kernel where the plan allows it (and allow_chain), else one launch per step
 template <typename T>
@@ -1746,6 +1770,11 @@ void Pipeline<T>::launch_conv(hipStream_t st, const Ctx& c, int prec, bool back,
     };
     if (!back) {
         run(c.fwd_plan, 0, c.n_fwd_jobs);
+    } else if (c.use_gemm && c.merge_back) {
+        run(c.back_plan, c.n_jobs + c.n_fwd_jobs, c.n_pre_jobs + c.n_back_jobs);
+        if (c.n_back_jobs > 0)
+            hipLaunchKernelGGL(k_add_partials, grid2(c.maxNP, 256, c.P), dim3(256), 0, st, c.d_pts, c.d_evalflag, prec,
+                               c.d_jobs + c.n_fwd_jobs + c.n_pre_jobs, c.n_back_jobs, fx, c.d_wsum, c.Fws);
     } else {
         run(c.pre_plan, c.n_fwd_jobs, c.n_pre_jobs);
         run(c.back_plan, c.n_fwd_jobs + c.n_pre_jobs, c.n_back_jobs);

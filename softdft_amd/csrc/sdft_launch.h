@@ -82,6 +82,10 @@ struct Ctx {
     int GF = 0;                     // propagator fields per point (sum over species of nseg*S*S)
     int Frt = 0, Fnw = 0, Fdp = 0;  // family widths
     int n_fwd_jobs = 0, n_pre_jobs = 0, n_back_jobs = 0;  // job table: forward ++ shared partials ++ back
+    int n_jobs = 0;                 // length of that table; d_jobs holds it twice, the second copy with
+                                    // the back projections' partial additions removed (merged stage)
+    bool merge_back = false;        // partials and back projections in one GEMM launch, the partials
+                                    // added to F(x) afterwards by k_add_partials
     int Fws = 0;                    // shared back-projection partials per point (diameter classes)
     int band_fp32 = 1;
     int m = 5;                      // Anderson depth the history buffers were sized for
