@@ -200,9 +200,11 @@ int32_t sdft_set_graph_capture(sdft_ctx_t* ctx, int32_t enable, int64_t* graphs_
  * with HIP events on the context's stream, the candidate launch configurations of this batch and
  * keeps the fastest:
  *   SDFT_TUNE_PROPAGATOR  per chain species and precision: the block-per-problem mapping of the
- *                         propagator recursion.  choice 1 = persistent chain kernel (one workgroup per
- *                         state point and sweep runs every step), 0 = one operator-GEMM launch per
- *                         step (workgroups tile the grid of every point).  key = species index.
+ *                         propagator recursion.  choice 0 = one operator-GEMM launch per step
+ *                         (workgroups tile the grid of every point), 1 = persistent chain kernel (one
+ *                         workgroup per state point and sweep runs every step) with the input weights
+ *                         held in registers, 2 = the same kernel re-reading them every step.
+ *                         key = species index.
  *   SDFT_TUNE_CONV_FWD / _PRE / _BACK
  *                         per grid-length bin (kernel sets whose NP rounds up to the same multiple of
  *                         256; key = that bound) and convolution stage: the tile shape, choice
@@ -228,7 +230,7 @@ int32_t sdft_autotune(sdft_ctx_t* ctx, int32_t reps, sdft_tune_entry_t* entries,
                       int32_t* n_entries);
 /* applies decisions without measuring (a saved sdft_autotune result for the same batch layout, or
  * a forced configuration): PROPAGATOR entries with precision 0 apply to both precisions, and
- * choice 1 takes effect only where the chain kernel is eligible; CONV entries with key 0 apply to
+ * choices 1 and 2 take effect only where the chain kernel is eligible; CONV entries with key 0 apply to
  * every grid length, choice -1 restores the fixed rule.  us[] is ignored. */
 int32_t sdft_set_tuning(sdft_ctx_t* ctx, const sdft_tune_entry_t* entries, int32_t n);
 

@@ -1419,25 +1419,33 @@ static int time_launches(sdft_ctx* ctx, hipEvent_t e0, hipEvent_t e1, int reps, 
 static int autotune_run(sdft_ctx* ctx, int reps, hipEvent_t e0, hipEvent_t e1) {
     Ctx& c = ctx->c;
     hipStream_t st = ctx->stream;
-    // ---- propagators: persistent chain kernel vs one operator-GEMM launch per step, on the banded
-    //      operator both take in the mixed path
+    // ---- propagators: one operator-GEMM launch per step vs the persistent chain kernel with its
+    //      input weights held in registers or re-read every step, on the banded operator all of
+    //      them take in the mixed path
     for (int u = 0; u < c.nsp; ++u) {
         if (!c.use_gemm || c.species[u].nseg < 2 || !c.chain_plan[u].ok) continue;
+        ChainPlanHost& ch = c.chain_plan[u];
         for (int prec : {32, 64}) {
-            bool& ok = prec == 32 ? c.chain_plan[u].ok32 : c.chain_plan[u].ok64;
+            bool& ok = prec == 32 ? ch.ok32 : ch.ok64;
+            bool& rr = prec == 32 ? ch.reread32 : ch.reread64;
             if (!ok) continue;
             launch_set_evalflag(st, c.d_evalflag, prec, c.P);
-            auto run = [&](bool chain) {
-                if (prec == 32) ctx->p32.propagate(st, c, 32, u, chain);
-                else ctx->p64.propagate(st, c, 64, u, chain);
+            auto run = [&](int mode) {
+                ok = mode != 0;
+                rr = mode == 2;
+                if (prec == 32) ctx->p32.propagate(st, c, 32, u, true);
+                else ctx->p64.propagate(st, c, 64, u, true);
             };
-            double t_chain = 0.0, t_step = 0.0;
-            TRY(time_launches(ctx, e0, e1, reps, [&]() { run(true); }, &t_chain));
-            TRY(time_launches(ctx, e0, e1, reps, [&]() { run(false); }, &t_step));
-            ok = t_chain <= t_step;
+            double t[3];
+            for (int mode = 0; mode < 3; ++mode) TRY(time_launches(ctx, e0, e1, reps, [&]() { run(mode); }, &t[mode]));
+            int best = 0;
+            for (int mode = 1; mode < 3; ++mode)
+                if (t[mode] < t[best]) best = mode;
+            ok = best != 0;
+            rr = best == 2;
             sdft_tune_entry_t e;
-            e.kind = SDFT_TUNE_PROPAGATOR; e.key = u; e.precision = prec; e.choice = ok ? 1 : 0;
-            e.us[0] = t_step; e.us[1] = t_chain; e.us[2] = -1.0;
+            e.kind = SDFT_TUNE_PROPAGATOR; e.key = u; e.precision = prec; e.choice = best;
+            for (int mode = 0; mode < 3; ++mode) e.us[mode] = t[mode];
             ctx->tuning.push_back(e);
         }
     }
@@ -1532,14 +1540,20 @@ extern "C" int32_t sdft_set_tuning(sdft_ctx_t* ctx, const sdft_tune_entry_t* ent
     for (int i = 0; i < n; ++i) {
         const sdft_tune_entry_t& e = entries[i];
         if (e.kind == SDFT_TUNE_PROPAGATOR) {
-            if (e.key < 0 || e.key >= c.nsp || (e.choice != 0 && e.choice != 1) ||
+            if (e.key < 0 || e.key >= c.nsp || e.choice < 0 || e.choice > 2 ||
                 (e.precision != 0 && e.precision != 32 && e.precision != 64)) {
                 set_error(ctx, SDFT_E_ARG, "tuning entry %d: bad propagator key / choice / precision", i);
                 return SDFT_E_ARG;
             }
             ChainPlanHost& ch = c.chain_plan[e.key];
-            if (e.precision != 64) ch.ok32 = ch.elig32 && e.choice == 1;
-            if (e.precision != 32) ch.ok64 = ch.elig64 && e.choice == 1;
+            if (e.precision != 64) {
+                ch.ok32 = ch.elig32 && e.choice != 0;
+                ch.reread32 = e.choice == 2;
+            }
+            if (e.precision != 32) {
+                ch.ok64 = ch.elig64 && e.choice != 0;
+                ch.reread64 = e.choice == 2;
+            }
         } else if (e.kind >= SDFT_TUNE_CONV_FWD && e.kind <= SDFT_TUNE_CONV_BACK) {
             if (e.choice < -1 || e.choice >= CONV_NSHAPES) {
                 set_error(ctx, SDFT_E_ARG, "tuning entry %d: tile shape %d out of range", i, e.choice);

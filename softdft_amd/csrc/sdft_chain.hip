@@ -56,6 +56,10 @@ __host__ __device__ __forceinline__ int chain_half_width(int B) { return (B + 7)
 
 constexpr int CHAIN_LDS_SLACK = 64;  // elements after the carve-out: the operand prefetch may read one group past the end
 
+// Register use decides how many 8-wave workgroups share a CU: two need <= 128 VGPRs per lane, which
+// the FP32 variants and the FP64 re-read variant (EPC = 0) meet; the FP64 register-cached variants
+// run one per CU with shorter steps (forcing 128 spills to scratch: faster at full load, slower in
+// the polish phase where few points are active, so the solve is slower overall).
 template <typename T, int EPC>
 __global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __restrict__ psys,
                                                          const DevPoint* __restrict__ pts,
@@ -337,10 +341,11 @@ void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, 
         hipLaunchKernelGGL(kfn, dim3(c.P, 2), dim3(CHAIN_THREADS), lds, st, c.d_sys, c.d_pts, c.d_evalflag, prec, ca,
                            c.d_seg_states, c.d_band, kern, elam, hl, trapez, gC, gP);
     };
-    if (epc <= 2) go(k_chain<T, 2>);
+    const bool reread = sizeof(T) == 4 ? ch.reread32 : ch.reread64;
+    if (reread || epc > 4) go(k_chain<T, 0>);
+    else if (epc <= 2) go(k_chain<T, 2>);
     else if (epc <= 3) go(k_chain<T, 3>);
-    else if (epc <= 4) go(k_chain<T, 4>);
-    else go(k_chain<T, 0>);
+    else go(k_chain<T, 4>);
 }
 
 template void launch_chain<float>(hipStream_t, const Ctx&, int, int, const float*, const float*, const float*,

@@ -60,6 +60,8 @@ struct ChainPlanHost {
     bool ok = false;                               // eligible in at least one precision
     bool ok32 = false, ok64 = false;               // runs at that precision (eligible and chosen)
     bool elig32 = false, elig64 = false;           // the working set fits LDS at that precision
+    bool reread32 = false, reread64 = false;       // input weights re-read every step instead of
+                                                   // held in registers (fewer registers per lane)
     bool homog = false;                            // every segment carries the same state beads
     int S = 1;
     int ck[2][SDFT_MAX_S * SDFT_MAX_S];            // canonical bond kernel of (sweep, a * S + b), -1 absent
