@@ -239,8 +239,13 @@ __global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __rest
                 // B = X[col][phys(16 (p0 + q) + g + 4 ks)] = X[col][17 (p0 + q) + g + 17 K + 4 k4]
                 const T* ap = cw + col * KC + (KC - 16 - 2 * Be) - q + g;
                 const T* xp = X + col * XLP + 17 * (p0 + q) + g;
-                typename Mfma<T>::acc_t acc;
-                for (int v = 0; v < 4; ++v) acc[v] = T(0);
+                // four independent accumulation chains (one per k-step of a group of four): a
+                // product-accumulate does not wait for the previous one's result, so the matrix
+                // core stays busy with only two waves per SIMD; summed in a fixed order at the end
+                typename Mfma<T>::acc_t acc4[4];
+#pragma unroll
+                for (int k4 = 0; k4 < 4; ++k4)
+                    for (int v = 0; v < 4; ++v) acc4[k4][v] = T(0);
                 // ping-pong operand buffers: the next group's LDS reads are in flight while
                 // the current group runs on the matrix core
                 T a0[4], b0[4], a1[4], b1[4];
@@ -257,14 +262,16 @@ __global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __rest
                     fetch(K + 1, a1, b1);
                     __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the products
 #pragma unroll
-                    for (int k4 = 0; k4 < 4; ++k4) acc = Mfma<T>::mma(a0[k4], b0[k4], acc);
+                    for (int k4 = 0; k4 < 4; ++k4) acc4[k4] = Mfma<T>::mma(a0[k4], b0[k4], acc4[k4]);
                     __builtin_amdgcn_sched_barrier(0);
                     fetch(K + 2, a0, b0);  // past the end on the last pass: read from the slack, unused
                     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int k4 = 0; k4 < 4; ++k4) acc = Mfma<T>::mma(a1[k4], b1[k4], acc);
+                    for (int k4 = 0; k4 < 4; ++k4) acc4[k4] = Mfma<T>::mma(a1[k4], b1[k4], acc4[k4]);
                     __builtin_amdgcn_sched_barrier(0);
                 }
+                typename Mfma<T>::acc_t acc;
+                for (int v = 0; v < 4; ++v) acc[v] = (acc4[0][v] + acc4[1][v]) + (acc4[2][v] + acc4[3][v]);
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int n = 16 * (p0 + q) + Mfma<T>::row(lane, v);
