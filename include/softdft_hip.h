@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define SDFT_ABI_VERSION 2
+#define SDFT_ABI_VERSION 3
 
 /* error codes */
 #define SDFT_OK 0

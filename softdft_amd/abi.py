@@ -18,7 +18,7 @@ from .problem import StatePoint
 LIB_NAME = "libsoftdft_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
-SDFT_ABI_VERSION = 2
+SDFT_ABI_VERSION = 3
 SDFT_OK = 0
 ERRORS = {-1: "SDFT_E_ARG", -2: "SDFT_E_HIP", -3: "SDFT_E_NORMALIZATION", -4: "SDFT_E_NOCONV",
           -5: "SDFT_E_UNSUPPORTED", -6: "SDFT_E_STATE"}
