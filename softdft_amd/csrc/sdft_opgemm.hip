@@ -599,8 +599,9 @@ void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& pl
             wide(k_convgemm<T, 1, 4, 1>, 1);
             break;
         case CONV_WIDE128:
-            // FP32 takes a 128-row tile in one workgroup; FP64 (whose accumulators and operands already
-            // fill the register file at 64 rows) splits it over two
+            // FP32 takes a 128-row tile in one workgroup; FP64 splits it over two 64-row workgroups
+            // (one workgroup with WM = 2 measured 389 vs 356 us for the benchmark's forward + back
+            // stages: its 266 registers allow one wave per SIMD)
             if (sizeof(T) == 4) wide(k_convgemm<T, 2, 4, 1>, 1);
             else wide(k_convgemm<T, 1, 4, 2>, 2);
             break;
