@@ -467,15 +467,15 @@ static bool column_kernels(sdft_ctx* ctx, const DevSystem& sys, int u, const std
 
 // Persistent banded chain plan of species u: free (ungrafted) linear chains with fixed column
 // operators whose band fits the mirror construction (2 Be + 1 <= NP) and whose working set fits LDS.
-static void build_chain_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const std::vector<std::vector<int>>& canon,
-                             const std::vector<int32_t>& band, ChainPlanHost& ch) {
+static int build_chain_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const std::vector<std::vector<int>>& canon,
+                            const std::vector<int32_t>& band, ChainPlanHost& ch) {
     Ctx& c = ctx->c;
     const SpeciesHost& sh = c.species[u];
     ch = ChainPlanHost();
     const int N = sh.nseg, S = sh.S;
-    if (N < 2 || sh.fixed_seg >= 0 || S > SDFT_MAX_S) return;
+    if (N < 2 || sh.fixed_seg >= 0 || S > SDFT_MAX_S) return SDFT_OK;
     std::vector<int> ck;
-    if (!column_kernels(ctx, sys, u, canon, ck)) return;
+    if (!column_kernels(ctx, sys, u, canon, ck)) return SDFT_OK;
     for (int sw = 0; sw < 2; ++sw)
         for (int k = 0; k < SDFT_MAX_S * SDFT_MAX_S; ++k) ch.ck[sw][k] = k < S * S ? ck[sw * S * S + k] : -1;
     const int* st = ctx->seg_states_h.data() + sh.state_off;
@@ -486,15 +486,52 @@ static void build_chain_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const s
     for (int p = 0; p < c.P; ++p) {
         const int NP = ctx->pts[p].NP;
         const int Be = (band[ctx->pts[p].kset] + 7) / 16 * 16 + 8;  // the chain kernel's half-width (8 mod 16)
-        if (2 * Be + 1 > NP) return;
+        if (2 * Be + 1 > NP) return SDFT_OK;
         ch.npad_max = std::max(ch.npad_max, (NP + 255) / 256 * 256);
         ch.be_max = std::max(ch.be_max, Be);
     }
-    ch.ok32 = chain_lds_bytes(ch, sizeof(float)) <= 160 * 1024;
-    ch.ok64 = chain_lds_bytes(ch, sizeof(double)) <= 160 * 1024;
+    ch.ok32 = chain_lds_bytes(ch, sizeof(float), false) <= 160 * 1024;
+    ch.ok64 = chain_lds_bytes(ch, sizeof(double), false) <= 160 * 1024;
+    // output-factor table in LDS for homogeneous chains where it fits without costing occupancy:
+    // within 80 KB when the workgroup fitted there before (two per CU), else within 160 KB
+    auto hv_fits = [&](size_t elem, bool ok) {
+        if (!ok || !ch.homog) return false;
+        const size_t b0 = chain_lds_bytes(ch, elem, false), b1 = chain_lds_bytes(ch, elem, true);
+        return b1 <= (b0 <= 80 * 1024 ? 80 * 1024 : 160 * 1024);
+    };
+    ch.hv32 = hv_fits(sizeof(float), ch.ok32);
+    ch.hv64 = hv_fits(sizeof(double), ch.ok64);
     ch.ok = ch.ok32 || ch.ok64;
     ch.elig32 = ch.ok32;
     ch.elig64 = ch.ok64;
+    if (!ch.ok) return SDFT_OK;
+    // column table of every step of both sweeps (child_propagators.jl / parent_propagators.jl walk
+    // the same linear chain from opposite ends): input bead and bond fields of each (a, b) pair
+    const int C = S * S;
+    std::vector<ChainCol> tab((size_t)2 * (N - 1) * C);
+    for (int sw = 0; sw < 2; ++sw)
+        for (int q = 1; q < N; ++q) {
+            const int outseg = sw == 0 ? N - 1 - q : q;
+            const int inseg = sw == 0 ? outseg + 1 : outseg - 1;
+            for (int col = 0; col < C; ++col) {
+                const int a = col / S, b = col - a * S;
+                const int si = st[outseg * S + a], sj = st[inseg * S + b];
+                ChainCol cc;
+                cc.valid = (si >= 0 && sj >= 0 && ch.ck[sw][col] >= 0) ? 1 : 0;
+                cc.sj = sj < 0 ? 0 : sj;
+                int bond = 0;
+                if (cc.valid) {
+                    const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
+                    for (int u2 = 0; u2 < c.MB; ++u2)
+                        if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
+                }
+                cc.hl_in = 2 * bond + (si <= sj ? 1 : 0);
+                cc.hl_out = 2 * bond + (si <= sj ? 0 : 1);
+                tab[((size_t)sw * (N - 1) + (q - 1)) * C + col] = cc;
+            }
+        }
+    TRY(upload(ctx, &ch.d_info, tab.data(), (int64_t)tab.size()));
+    return SDFT_OK;
 }
 
 // Propagator plan of species u: valid only if every (sweep, a, b) column applies the same
@@ -927,7 +964,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         const char* cenv = getenv("SOFTDFT_CHAIN");
         const bool chain_on = c.use_gemm && !(cenv && strcmp(cenv, "off") == 0);
         for (int u = 0; u < nsp && chain_on; ++u) {
-            build_chain_plan(ctx, sys, u, canon, band, c.chain_plan[u]);
+            if ((rc = build_chain_plan(ctx, sys, u, canon, band, c.chain_plan[u])) != SDFT_OK) return fail(rc);
             if (!c.chain_plan[u].ok) continue;
             const hipError_t e = chain_prepare(c.chain_plan[u]);
             if (e != hipSuccess) {

@@ -38,15 +38,14 @@ constexpr int CHAIN_WAVES = CHAIN_THREADS / 64;
 struct ChainArgs {
     int32_t u, nseg, S, state_off;
     int32_t GF, MB, NB, homog;     // homog: every segment has the same state beads
+    int32_t hv;                    // output factors exp(lng/2)/normative tabulated in LDS (homog only)
+    int32_t pad2;
     int64_t goff;
     int32_t ck[2][SDFT_MAX_S * SDFT_MAX_S];  // canonical bond kernel of (sweep, a * S + b)
     int32_t npad_max, be_max;                // LDS carve-out (largest point of the launch)
     int32_t rescale, nsp, nmax, pad;         // per-step rescaling (scale_slot tables)
     unsigned long long* smax;
-};
-
-struct ChainCol {      // one (a, b) column at one step
-    int32_t valid, sj, hl_in, hl_out;   // input bead, hl field of the input / output end
+    const ChainCol* info;                    // [2][N-1][S*S] column table of every step (host-built)
 };
 
 __device__ __forceinline__ int chain_phys(int i) { return i + (i >> 4); }  // 1 pad word per 16: conflict-free B reads
@@ -56,12 +55,12 @@ __host__ __device__ __forceinline__ int chain_half_width(int B) { return (B + 7)
 
 constexpr int CHAIN_LDS_SLACK = 64;  // elements after the carve-out: the operand prefetch may read one group past the end
 
-// Register use decides how many 8-wave workgroups share a CU: two need <= 128 VGPRs per lane, which
-// the FP32 variants and the FP64 re-read variant (EPC = 0) meet; the FP64 register-cached variants
-// run one per CU with shorter steps (forcing 128 spills to scratch: faster at full load, slower in
-// the polish phase where few points are active, so the solve is slower overall).
+// Register use decides how many 8-wave workgroups share a CU: two need <= 128 VGPRs per lane (at
+// least 4 waves per SIMD), which the FP32 variants are held to; the FP64 variants run one per CU
+// with shorter steps (forcing 128 spills to scratch: faster at full load, slower in the polish phase
+// where few points are active, so the solve is slower overall).
 template <typename T, int EPC>
-__global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __restrict__ psys,
+__global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain(const DevSystem* __restrict__ psys,
                                                          const DevPoint* __restrict__ pts,
                                                          const int32_t* __restrict__ evalflag, int prec, ChainArgs ca,
                                                          const int32_t* __restrict__ seg_states,
@@ -89,10 +88,10 @@ __global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __rest
     T* cw = reinterpret_cast<T*>(chain_lds);   // [C][KC]
     T* X = cw + C * KC;                        // [C][XLP]
     T* Q = X + C * XLP;                        // [S][NPQ]
+    T* HV = Q + S * NPQ;                       // [C][NPQ] exp(lng/2) of the output end / normative (ca.hv)
     __shared__ ChainCol info[2][SDFT_MAX_S * SDFT_MAX_S];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int* st = seg_states + ca.state_off;
     const double norm = P.normative[ca.u];
 
     // operator windows, stored reversed so that the A reads ascend with k:
@@ -123,25 +122,29 @@ __global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __rest
         return m < 0 ? -1 - m : (m < NP ? m : 2 * NP - 1 - m);
     };
 
+    // Column table of the steps (static topology, built on the host): threads tid < C copy the
+    // current step's entry into LDS and already load the next one, so no global-memory latency sits
+    // between two steps.
+    const ChainCol* tab = ca.info + (int64_t)sw * (N - 1) * C;
+    const int4* tab4 = reinterpret_cast<const int4*>(tab);  // one 16-byte load per entry, kept in registers
+    static_assert(sizeof(ChainCol) == sizeof(int4), "ChainCol is four int32");
+    int4 nxt = make_int4(0, 0, 0, 0);
+    if (tid < C) nxt = tab4[tid];
+    // homogeneous chain: every step has the same columns, so the output factors of the epilogue are
+    // read from HBM once here (the first step's barrier orders these writes before their reads)
+    if (ca.hv) {
+        for (int e = tid; e < C * NP; e += CHAIN_THREADS) {
+            const int col = e / NP, n = e - col * NP;
+            const ChainCol cc = tab[col];
+            HV[col * NPQ + n] = cc.valid ? hl[hb + (int64_t)cc.hl_out * NP + n] / (T)norm : T(0);
+        }
+    }
     for (int s = 1; s < N; ++s) {
         const int outseg = sw == 0 ? N - 1 - s : s;
-        const int inseg = sw == 0 ? outseg + 1 : outseg - 1;
         ChainCol* ci = info[s & 1];
         if (tid < C) {
-            const int a = tid / S, b = tid - a * S;
-            const int si = st[outseg * S + a], sj = st[inseg * S + b];
-            ChainCol cc;
-            cc.valid = (si >= 0 && sj >= 0 && ca.ck[sw][tid] >= 0) ? 1 : 0;
-            cc.sj = sj < 0 ? 0 : sj;
-            int bond = 0;
-            if (cc.valid) {
-                const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
-                for (int u2 = 0; u2 < ca.MB; ++u2)
-                    if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
-            }
-            cc.hl_in = 2 * bond + (si <= sj ? 1 : 0);
-            cc.hl_out = 2 * bond + (si <= sj ? 0 : 1);
-            ci[tid] = cc;
+            ci[tid] = ChainCol{nxt.x, nxt.y, nxt.z, nxt.w};
+            if (s + 1 < N) nxt = tab4[s * C + tid];
         }
         __syncthreads();  // column info of this step; Q of the previous step complete
         // ---- extended inputs X = W * Q (Q = 1 at the free chain end)
@@ -219,13 +222,15 @@ __global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __rest
             const int p0 = tile * 16;
             // epilogue factors exp(lng/2) of the output end, fetched before the products
             T hv[SDFT_MAX_S][4];
+            if (!ca.hv) {
 #pragma unroll
-            for (int b = 0; b < SDFT_MAX_S; ++b) {
-                const ChainCol cc = ci[(a * S + b) % C];
+                for (int b = 0; b < SDFT_MAX_S; ++b) {
+                    const ChainCol cc = ci[(a * S + b) % C];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int n = 16 * (p0 + q) + Mfma<T>::row(lane, v);
-                    hv[b][v] = (b < S && cc.valid && n < NP) ? hl[hb + (int64_t)cc.hl_out * NP + n] : T(0);
+                    for (int v = 0; v < 4; ++v) {
+                        const int n = 16 * (p0 + q) + Mfma<T>::row(lane, v);
+                        hv[b][v] = (b < S && cc.valid && n < NP) ? hl[hb + (int64_t)cc.hl_out * NP + n] : T(0);
+                    }
                 }
             }
             T qs[4] = {T(0), T(0), T(0), T(0)};
@@ -277,7 +282,8 @@ __global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __rest
                     const int n = 16 * (p0 + q) + Mfma<T>::row(lane, v);
                     if (n < NP) {
                         const T y = acc[v];
-                        const T val = (y < T(0) ? -y : y) * hv[b][v] / (T)norm;
+                        const T ay = y < T(0) ? -y : y;
+                        const T val = ca.hv ? ay * HV[col * NPQ + n] : ay * hv[b][v] / (T)norm;
                         gout[gb + (((int64_t)outseg * S + b) * S + a) * NP + n] = val;
                         qs[v] += val;
                     }
@@ -293,12 +299,13 @@ __global__ void __launch_bounds__(CHAIN_THREADS) k_chain(const DevSystem* __rest
     }
 }
 
-size_t chain_lds_bytes(const ChainPlanHost& ch, size_t elem) {
+size_t chain_lds_bytes(const ChainPlanHost& ch, size_t elem, bool hv_table) {
     const int S = ch.S, C = S * S;
     const int KC = 2 * ch.be_max + 32;
     const int XLm = ch.npad_max + 2 * ch.be_max;
     const int XLP = XLm + (XLm >> 4) + 1;
-    return (size_t)(C * KC + C * XLP + S * ch.npad_max + CHAIN_LDS_SLACK) * elem;
+    const int HV = hv_table ? C * ch.npad_max : 0;
+    return (size_t)(C * KC + C * XLP + S * ch.npad_max + HV + CHAIN_LDS_SLACK) * elem;
 }
 
 template <typename T, int EPC>
@@ -311,14 +318,14 @@ static hipError_t chain_set_lds(size_t bytes) {
 hipError_t chain_prepare(const ChainPlanHost& ch) {
     hipError_t e = hipSuccess;
     if (ch.ok32) {
-        const size_t b = chain_lds_bytes(ch, sizeof(float));
+        const size_t b = chain_lds_bytes(ch, sizeof(float), ch.hv32);
         if ((e = chain_set_lds<float, 0>(b)) != hipSuccess) return e;
         if ((e = chain_set_lds<float, 2>(b)) != hipSuccess) return e;
         if ((e = chain_set_lds<float, 3>(b)) != hipSuccess) return e;
         if ((e = chain_set_lds<float, 4>(b)) != hipSuccess) return e;
     }
     if (ch.ok64) {
-        const size_t b = chain_lds_bytes(ch, sizeof(double));
+        const size_t b = chain_lds_bytes(ch, sizeof(double), ch.hv64);
         if ((e = chain_set_lds<double, 0>(b)) != hipSuccess) return e;
         if ((e = chain_set_lds<double, 2>(b)) != hipSuccess) return e;
         if ((e = chain_set_lds<double, 3>(b)) != hipSuccess) return e;
@@ -341,7 +348,10 @@ void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, 
         for (int k = 0; k < SDFT_MAX_S * SDFT_MAX_S; ++k) ca.ck[sw][k] = ch.ck[sw][k];
     ca.npad_max = ch.npad_max; ca.be_max = ch.be_max;
     ca.rescale = sh.rescale; ca.nsp = c.nsp; ca.nmax = c.nmax_seg; ca.smax = smax;
-    const size_t lds = chain_lds_bytes(ch, sizeof(T));
+    ca.info = ch.d_info;
+    const bool hvt = sizeof(T) == 4 ? ch.hv32 : ch.hv64;
+    ca.hv = hvt ? 1 : 0;
+    const size_t lds = chain_lds_bytes(ch, sizeof(T), hvt);
     const int xl = ch.npad_max + 2 * ch.be_max;
     const int epc = (xl + CHAIN_THREADS - 1) / CHAIN_THREADS;
     auto go = [&](auto kfn) {

@@ -55,6 +55,12 @@ struct OpPlan {
     int32_t* d_count[2] = {nullptr, nullptr};
 };
 
+// one (a, b) column of one step of the chain kernel (a: state of the output segment, b: state of the
+// input segment): valid pair, input bead, field of exp(lng/2) at the input and at the output end
+struct ChainCol {
+    int32_t valid, sj, hl_in, hl_out;
+};
+
 // persistent banded chain kernel of one species (ok: eligible for the banded operator)
 struct ChainPlanHost {
     bool ok = false;                               // eligible in at least one precision
@@ -66,6 +72,9 @@ struct ChainPlanHost {
     int S = 1;
     int ck[2][SDFT_MAX_S * SDFT_MAX_S];            // canonical bond kernel of (sweep, a * S + b), -1 absent
     int npad_max = 0, be_max = 0;                  // largest padded NP and even band half-width
+    ChainCol* d_info = nullptr;                    // [2 sweeps][N-1 steps][S*S] column table (device)
+    bool hv32 = false, hv64 = false;               // homogeneous chain: the output factors
+                                                   // exp(lng/2)/normative tabulated in LDS
 };
 
 struct SpeciesHost {
@@ -199,7 +208,7 @@ void launch_compact_cols(hipStream_t st, const Ctx& c, int prec);
 template <typename T>
 void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, const T* kern, const T* rhoT,
                         const T* dphi, T* nw, double* fx);
-size_t chain_lds_bytes(const ChainPlanHost& ch, size_t elem);
+size_t chain_lds_bytes(const ChainPlanHost& ch, size_t elem, bool hv_table);
 hipError_t chain_prepare(const ChainPlanHost& ch);
 template <typename T>
 void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, const T* elam, const T* hl,
