@@ -208,10 +208,11 @@ int32_t sdft_set_graph_capture(sdft_ctx_t* ctx, int32_t enable, int64_t* graphs_
  *   SDFT_TUNE_CONV_FWD / _PRE / _BACK
  *                         per grid-length bin (kernel sets whose NP rounds up to the same multiple of
  *                         256; key = that bound) and convolution stage: the tile shape, choice
- *                         0 = 128 x 16, 1 = 64 x 64, 2 = 128 x 64 rows x columns.  Timed at FP32 and
+ *                         0 = 128 x 16, 1 = 64 x 64, 2 = 128 x 64, 3 = 64 x 32, 4 = 64 x 16 rows x
+ *                         columns.  Timed at FP32 and
  *                         FP64 and chosen on the sum.
  * us[k] is the measured microseconds per launch sequence of candidate k (< 0: not a candidate).
- * Without a call the library uses fixed rules (chain kernel wherever eligible; 64 x 64 tiles where a
+ * Without a call the library uses fixed rules (chain kernel wherever eligible; 64 x 32 tiles where a
  * grid length's operators average >= 48 columns, else 128 x 16).  The choice changes summation
  * orders (FP64 results move in the last bits) and the banded operator's padding, never the algorithm.
  * Call it before sdft_solve / sdft_objective: it overwrites the work buffers. */
@@ -224,7 +225,7 @@ typedef struct {
     int32_t key;         /* species index, or the bin's NP bound */
     int32_t precision;   /* 32 / 64 (propagators), 0 (convolutions: FP32 + FP64) */
     int32_t choice;
-    double us[3];
+    double us[5];
 } sdft_tune_entry_t;
 int32_t sdft_autotune(sdft_ctx_t* ctx, int32_t reps, sdft_tune_entry_t* entries, int32_t capacity,
                       int32_t* n_entries);

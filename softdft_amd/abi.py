@@ -32,7 +32,7 @@ EXPORTED = ("sdft_abi_version", "sdft_last_error", "sdft_create", "sdft_destroy"
             "sdft_last_timing", "sdft_last_pipeline_iters", "sdft_set_operator_band", "sdft_kernel_profile",
             "sdft_set_graph_capture", "sdft_autotune", "sdft_set_tuning")
 TUNE_PROPAGATOR, TUNE_CONV_FWD, TUNE_CONV_PRE, TUNE_CONV_BACK = 1, 2, 3, 4
-CONV_TALL, CONV_WIDE64, CONV_WIDE128 = 0, 1, 2
+CONV_TALL, CONV_WIDE64, CONV_WIDE128, CONV_MID, CONV_NARROW = 0, 1, 2, 3, 4
 KP_PROP_FP32, KP_PROP_FP64, KP_CONV_FWD_FP32, KP_CONV_FWD_FP64, KP_CONV_BACK_FP32, KP_CONV_BACK_FP64, \
     KP_POINTWISE_FP64, KP_CHAIN_FP32, KP_CHAIN_FP64 = range(9)
 
@@ -65,7 +65,7 @@ class SdftSolverOpts(C.Structure):
 
 class SdftTuneEntry(C.Structure):
     _fields_ = [("kind", C.c_int32), ("key", C.c_int32), ("precision", C.c_int32), ("choice", C.c_int32),
-                ("us", C.c_double * 3)]
+                ("us", C.c_double * 5)]
 
 
 class SdftError(RuntimeError):

@@ -286,7 +286,7 @@ static std::vector<OpTile> xcd_order(const std::vector<OpTile>& tiles) {
 }
 
 // Convolution plan of one stage: columns (point, job) grouped by (kernel set, operator terms), each
-// group tiled in the shape of its kernel set (set_shape[kset]; < 0: the heuristic, 64 x 64 tiles
+// group tiled in the shape of its kernel set (set_shape[kset]; < 0: the fixed rule, 64 x 32 tiles
 // when the set's groups average >= 48 columns, else 128 x 16). Points with point_mask[p] == 0 are
 // left out (the autotuner's per-grid-length plans).
 static int build_conv_plan(sdft_ctx* ctx, const std::vector<ConvJob>& jobs, int job_base, int njobs,
@@ -333,7 +333,7 @@ static int build_conv_plan(sdft_ctx* ctx, const std::vector<ConvJob>& jobs, int 
     auto shape_of = [&](int kset) {
         if (kset < (int)set_shape.size() && set_shape[kset] >= 0) return set_shape[kset];
         const auto& e = set_cols[kset];
-        return e.first >= 48 * e.second ? (int)CONV_WIDE64 : (int)CONV_TALL;
+        return e.first >= 48 * e.second ? (int)CONV_MID : (int)CONV_TALL;
     };
     for (int sh = 0; sh < CONV_NSHAPES; ++sh) {
         OpPlan& plan = stage.sub[sh];
@@ -1483,6 +1483,7 @@ static int autotune_run(sdft_ctx* ctx, int reps, hipEvent_t e0, hipEvent_t e1) {
             sdft_tune_entry_t e;
             e.kind = SDFT_TUNE_PROPAGATOR; e.key = u; e.precision = prec; e.choice = best;
             for (int mode = 0; mode < 3; ++mode) e.us[mode] = t[mode];
+            e.us[3] = e.us[4] = -1.0;
             ctx->tuning.push_back(e);
         }
     }

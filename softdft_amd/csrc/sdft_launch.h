@@ -37,7 +37,9 @@ enum ConvShape : int {
     CONV_TALL = 0,     // 128 x 16 (k_opgemm): few columns per operator (ragged batches)
     CONV_WIDE64 = 1,   // 64 x 64 (k_convgemm)
     CONV_WIDE128 = 2,  // 128 x 64 (k_convgemm; FP64 runs it as two 64-row workgroups per tile)
-    CONV_NSHAPES = 3
+    CONV_MID = 3,      // 64 x 32 (k_convgemm): twice the tiles of 64 x 64, fewer registers
+    CONV_NARROW = 4,   // 64 x 16 (k_convgemm)
+    CONV_NSHAPES = 5
 };
 
 struct OpPlan {

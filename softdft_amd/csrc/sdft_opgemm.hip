@@ -598,6 +598,12 @@ void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& pl
         case CONV_WIDE64:
             wide(k_convgemm<T, 1, 4, 1>, 1);
             break;
+        case CONV_MID:
+            wide(k_convgemm<T, 1, 2, 1>, 1);
+            break;
+        case CONV_NARROW:
+            wide(k_convgemm<T, 1, 1, 1>, 1);
+            break;
         case CONV_WIDE128:
             // FP32 takes a 128-row tile in one workgroup; FP64 splits it over two 64-row workgroups
             // (one workgroup with WM = 2 measured 389 vs 356 us for the benchmark's forward + back
@@ -658,7 +664,11 @@ template void launch_opgemm_prop<double>(hipStream_t, const Ctx&, int, const OpP
 // row/column tile sizes of the two shapes (for the host planner)
 int opgemm_tile_rows(bool wide) { return wide ? TileShape<1, 4>::TM : TileShape<2, 1>::TM; }
 int opgemm_tile_cols(bool wide) { return wide ? TileShape<1, 4>::TN : TileShape<2, 1>::TN; }
-int conv_tile_rows(int shape) { return shape == CONV_WIDE64 ? 64 : shape == CONV_WIDE128 ? 128 : TileShape<2, 1>::TM; }
-int conv_tile_cols(int shape) { return shape == CONV_TALL ? TileShape<2, 1>::TN : 64; }
+int conv_tile_rows(int shape) {
+    return shape == CONV_TALL ? TileShape<2, 1>::TM : shape == CONV_WIDE128 ? 128 : 64;
+}
+int conv_tile_cols(int shape) {
+    return shape == CONV_TALL ? TileShape<2, 1>::TN : shape == CONV_MID ? 32 : shape == CONV_NARROW ? 16 : 64;
+}
 
 }  // namespace sdft

@@ -40,7 +40,7 @@ def _banded_oracle(sp, half_width):
     return o
 
 
-@pytest.mark.parametrize("shape", [abi.CONV_TALL, abi.CONV_WIDE64, abi.CONV_WIDE128])
+@pytest.mark.parametrize("shape", [abi.CONV_TALL, abi.CONV_WIDE64, abi.CONV_WIDE128, abi.CONV_MID, abi.CONV_NARROW])
 def test_every_conv_tile_shape_matches_oracle(shape):
     sps = _points()
     b = abi.Batch(sps)
