@@ -33,6 +33,9 @@
 namespace sdft {
 
 constexpr int CHAIN_THREADS = 512;
+// accumulation chains per MFMA unit: one (measured: four chains did not shorten the launch, so the
+// registers go to occupancy instead)
+constexpr int NACC = 1;
 constexpr int CHAIN_WAVES = CHAIN_THREADS / 64;
 
 struct ChainArgs {
@@ -57,8 +60,8 @@ constexpr int CHAIN_LDS_SLACK = 64;  // elements after the carve-out: the operan
 
 // Register use decides how many 8-wave workgroups share a CU: two need <= 128 VGPRs per lane (at
 // least 4 waves per SIMD), which the FP32 variants are held to; the FP64 variants run one per CU
-// with shorter steps (forcing 128 spills to scratch: faster at full load, slower in the polish phase
-// where few points are active, so the solve is slower overall).
+// (held to 128 they spill the input weights to scratch: faster at full load, slower in the polish
+// phase where few points are active, so the solve is slower overall).
 template <typename T, int EPC>
 __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain(const DevSystem* __restrict__ psys,
                                                          const DevPoint* __restrict__ pts,
@@ -244,12 +247,11 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
                 // B = X[col][phys(16 (p0 + q) + g + 4 ks)] = X[col][17 (p0 + q) + g + 17 K + 4 k4]
                 const T* ap = cw + col * KC + (KC - 16 - 2 * Be) - q + g;
                 const T* xp = X + col * XLP + 17 * (p0 + q) + g;
-                // four independent accumulation chains (one per k-step of a group of four): a
-                // product-accumulate does not wait for the previous one's result, so the matrix
-                // core stays busy with only two waves per SIMD; summed in a fixed order at the end
-                typename Mfma<T>::acc_t acc4[4];
+                // NACC independent accumulation chains (k-step k4 goes to chain k4 % NACC), summed in
+                // a fixed order at the end
+                typename Mfma<T>::acc_t acc4[NACC];
 #pragma unroll
-                for (int k4 = 0; k4 < 4; ++k4)
+                for (int k4 = 0; k4 < NACC; ++k4)
                     for (int v = 0; v < 4; ++v) acc4[k4][v] = T(0);
                 // ping-pong operand buffers: the next group's LDS reads are in flight while
                 // the current group runs on the matrix core
@@ -267,16 +269,18 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
                     fetch(K + 1, a1, b1);
                     __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the products
 #pragma unroll
-                    for (int k4 = 0; k4 < 4; ++k4) acc4[k4] = Mfma<T>::mma(a0[k4], b0[k4], acc4[k4]);
+                    for (int k4 = 0; k4 < 4; ++k4) acc4[k4 % NACC] = Mfma<T>::mma(a0[k4], b0[k4], acc4[k4 % NACC]);
                     __builtin_amdgcn_sched_barrier(0);
                     fetch(K + 2, a0, b0);  // past the end on the last pass: read from the slack, unused
                     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int k4 = 0; k4 < 4; ++k4) acc4[k4] = Mfma<T>::mma(a1[k4], b1[k4], acc4[k4]);
+                    for (int k4 = 0; k4 < 4; ++k4) acc4[k4 % NACC] = Mfma<T>::mma(a1[k4], b1[k4], acc4[k4 % NACC]);
                     __builtin_amdgcn_sched_barrier(0);
                 }
-                typename Mfma<T>::acc_t acc;
-                for (int v = 0; v < 4; ++v) acc[v] = (acc4[0][v] + acc4[1][v]) + (acc4[2][v] + acc4[3][v]);
+                typename Mfma<T>::acc_t acc = acc4[0];
+#pragma unroll
+                for (int k4 = 1; k4 < NACC; ++k4)
+                    for (int v = 0; v < 4; ++v) acc[v] += acc4[k4][v];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int n = 16 * (p0 + q) + Mfma<T>::row(lane, v);
