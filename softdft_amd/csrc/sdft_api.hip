@@ -748,6 +748,10 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
             if (cnt > 1) multi = 1;
         }
         s.multistate = multi;
+        s.homog = 1;
+        for (int q = 1; q < N; ++q)
+            for (int a = 0; a < S; ++a)
+                if (d->seg_states[state_off + q * S + a] != d->seg_states[state_off + a]) s.homog = 0;
         s.rescale = (N >= SDFT_RESCALE_MIN_SEG && s.model == SDFT_MODEL_FREELY_JOINTED) ? 1 : 0;
         goffs[u] = goff;
         SpeciesHost& sh = c.species[u];

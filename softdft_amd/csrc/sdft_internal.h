@@ -67,7 +67,7 @@ struct DevSpecies {
     int32_t segsum_off;   // offset into the per-point segment-sum table ([nseg][S][2])
     int32_t maxd_off;     // offset into the max-distance table ([nseg])
     int32_t rescale;      // propagators carry per-step scale factors (long chains)
-    int32_t pad;
+    int32_t homog;        // every segment carries the same state beads (same bonds on every link)
     double fixed_density;
 };
 

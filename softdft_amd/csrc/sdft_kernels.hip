@@ -535,6 +535,17 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
             return P.base * (int64_t)GF + (goffs[u] + ((int64_t)seg * S + b) * S + a) * NP + nn;
         };
         double num = 0.0, chg = 0.0;
+        // homogeneous species (every segment carries the same state beads): the bead of state a and
+        // the bond of each (a, b) link are the same at every segment, so the contributions are summed
+        // in registers and added to the LDS accumulators once per species
+        const bool hom = sp.homog != 0;
+        double racc[SDFT_MAX_S], rc0[SDFT_MAX_S][SDFT_MAX_S], rc1[SDFT_MAX_S][SDFT_MAX_S];
+#pragma unroll
+        for (int a = 0; a < SDFT_MAX_S; ++a) {
+            racc[a] = 0.0;
+#pragma unroll
+            for (int b = 0; b < SDFT_MAX_S; ++b) rc0[a][b] = rc1[a][b] = 0.0;
+        }
         for (int q = grp; q < N; q += ACC_GROUPS) {
             const double pref = sp.rescale ? pref0 * exp(seg_log_scale(sp, P, p, u, q, lsc, sys.n_species, nmax))
                                            : pref0;
@@ -589,7 +600,8 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
                             chg += rs * sys.z[si];
                         }
                     } else {
-                        rb[si * TB] += rs;
+                        if (hom) racc[a] += rs;
+                        else rb[si * TB] += rs;
                         num += rs * tz * P.dz;
                         chg += rs * tz * P.dz * sys.z[si];
                     }
@@ -598,6 +610,10 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
                         for (int b = 0; b < SDFT_MAX_S; ++b) {
                             const int sj = b < S ? st[(q + 1) * S + b] : -1;
                             if (sj < 0) continue;
+                            if (hom) {
+                                rc0[a][b] += contr * (double)gcv[b][a];
+                                continue;
+                            }
                             const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
                             int bond = 0;
                             for (int u2 = 0; u2 < MB; ++u2)
@@ -611,6 +627,10 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
                         for (int b = 0; b < SDFT_MAX_S; ++b) {
                             const int sj = b < S ? st[(q - 1) * S + b] : -1;
                             if (sj < 0) continue;
+                            if (hom) {
+                                rc1[a][b] += c1 * (double)gpv[b][a];
+                                continue;
+                            }
                             const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
                             int bond = 0;
                             for (int u2 = 0; u2 < MB; ++u2)
@@ -618,6 +638,26 @@ __global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__
                             rbo[(2 * bond + 1) * TB] += c1 * (double)gpv[b][a];
                         }
                     }
+                }
+            }
+        }
+        if (hom && valid) {
+            // segment 0's states and their links stand for every segment (homogeneous species)
+#pragma unroll
+            for (int a = 0; a < SDFT_MAX_S; ++a) {
+                const int si = a < S ? st[a] : -1;
+                if (si < 0) continue;
+                rb[si * TB] += racc[a];
+#pragma unroll
+                for (int b = 0; b < SDFT_MAX_S; ++b) {
+                    const int sj = b < S ? st[b] : -1;
+                    if (sj < 0 || N < 2) continue;
+                    const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
+                    int bond = 0;
+                    for (int u2 = 0; u2 < MB; ++u2)
+                        if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
+                    rbo[(2 * bond + 0) * TB] += rc0[a][b];
+                    rbo[(2 * bond + 1) * TB] += rc1[a][b];
                 }
             }
         }
