@@ -140,7 +140,7 @@ def cpu_points_parallel(points, cores):
     return int(sum(its)), dt
 
 
-def report_single(name, ds, cpu_full):
+def report_single(name, ds, cpu_full, args):
     """C1 / C2: one state point. Plain FP64 GPU solve, mixed solve, CPU oracle (full solve or a bounded
     sample of objective evaluations scaled by the FP64 iteration count)."""
     import numpy as np
@@ -149,6 +149,10 @@ def report_single(name, ds, cpu_full):
     from oracle.dft_oracle import anderson_solve
     sp = build_state_point(ds)
     b = abi.Batch([sp])
+    if not args.no_tune:
+        dec = b.autotune(reps=3)
+        log(f"{name}: autotune " + "; ".join(f"kind {e['kind']} key {e['key']} prec {e['precision']} -> {e['choice']}"
+                                             for e in dec))
     opts = _solve_opts(sp.numerics)
     x64, it64, e64, t64 = _timed_solve(b, b.x0(), abi.PREC_FP64, opts)
     om64, al64, _ = b.observables()
@@ -204,7 +208,7 @@ def report_c4(args):
     from softdft_amd.sweeps import width_sweep
     dss = configs.config4(args.points if args.points != 256 else 512)
     t0 = time.perf_counter()
-    res = width_sweep(dss, waves=8, precision=abi.PREC_MIXED, workers=12,
+    res = width_sweep(dss, waves=8, precision=abi.PREC_MIXED, workers=12, tune=not args.no_tune,
                       on_round=lambda r, e: log(f"c4 round {r}: max residual {e:.2e}"))
     tsw = time.perf_counter() - t0
     n = len(res)
@@ -320,9 +324,9 @@ def main():
     if args.config:
         from softdft_amd import configs
         if args.config == "c1":
-            report_single("C1", configs.config1(), cpu_full=True)
+            report_single("C1", configs.config1(), cpu_full=True, args=args)
         elif args.config == "c2":
-            report_single("C2", configs.config2(), cpu_full=False)
+            report_single("C2", configs.config2(), cpu_full=False, args=args)
         elif args.config == "c4":
             report_c4(args)
         else:

@@ -81,9 +81,11 @@ class WidthResult:
 
 def width_sweep(datasets: Sequence[dict], *, waves: int = 8, precision: int = abi.PREC_MIXED,
                 numerics: Optional[dict] = None, max_iters: int = 3000,
-                on_round: Optional[Callable[[int, float], None]] = None, workers: int = 0) -> List[WidthResult]:
+                on_round: Optional[Callable[[int, float], None]] = None, workers: int = 0,
+                tune: bool = False) -> List[WidthResult]:
     """Solve the slit widths of `datasets` (increasing H, one chemistry) with neighbour warm starts;
-    `workers` > 1 runs the host setups in parallel processes.
+    `workers` > 1 runs the host setups in parallel processes; `tune` measures the launch
+    configuration on the first round's context and replays it on the others.
 
     Returns one WidthResult per width, in sweep order."""
     sps = build_state_points(datasets, workers)
@@ -103,7 +105,7 @@ def width_sweep(datasets: Sequence[dict], *, waves: int = 8, precision: int = ab
                                  delta_p=float(pc[k, 0] - p_bulk), energy=float(om[k] + p_bulk * sp.H),
                                  omega=float(om[k]), iterations=int(it[k]), warm=warm, residual=float(err[k]))
 
-    _waves_solve(sps, waves, opts, on_round, finish)
+    _waves_solve(sps, waves, opts, on_round, finish, tune=tune)
     return [r for r in res if r is not None]
 
 
@@ -146,17 +148,25 @@ def copy_solution(sp_old: StatePoint, x_old: np.ndarray, sp_new: StatePoint) -> 
     return np.array(x_old, dtype=np.float64, copy=True)
 
 
-def _waves_solve(sps, waves, opts, on_round, finish, warm=remap_solution):
+def _waves_solve(sps, waves, opts, on_round, finish, warm=remap_solution, tune=False):
     """Solve `sps` in `waves` batched rounds, point i warm-started from point i - 1 (cold in round 0);
-    finish(batch, idx, x_parts, iters, warm) is called per round while the context is alive."""
+    finish(batch, idx, x_parts, iters, warm) is called per round while the context is alive. With
+    `tune`, round 0's context is autotuned and its decisions (keyed by species and grid-length bin,
+    which every round shares) are replayed on the later rounds."""
     n = len(sps)
     xs: List[Optional[np.ndarray]] = [None] * n
     waves = max(1, min(waves, n))
+    tuning = None
     for r in range(waves):
         idx = list(range(r, n, waves))
         if not idx:
             continue
         batch = abi.Batch([sps[i] for i in idx])
+        if tune:
+            if tuning is None:
+                tuning = batch.autotune(reps=2)
+            else:
+                batch.set_tuning(tuning)
         x0 = [warm(sps[i - 1], xs[i - 1], sps[i]) if r > 0 else sps[i].x0() for i in idx]
         x, it, err = batch.solve(np.concatenate(x0), raise_on_noconv=False, **opts)
         parts = batch.split(x)
@@ -170,9 +180,11 @@ def _waves_solve(sps, waves, opts, on_round, finish, warm=remap_solution):
 
 def ph_sweep(datasets: Sequence[dict], pH_values: Sequence[float], *, waves: int = 8,
              precision: int = abi.PREC_MIXED, numerics: Optional[dict] = None, max_iters: int = 3000,
-             on_round: Optional[Callable[[int, float], None]] = None, workers: int = 0) -> List[PHResult]:
+             on_round: Optional[Callable[[int, float], None]] = None, workers: int = 0,
+             tune: bool = False) -> List[PHResult]:
     """pH continuation over `datasets` (one per pH in `pH_values`, same grid and chemistry) with
-    neighbour warm starts; the result list ends at the reference's early-stop step."""
+    neighbour warm starts; the result list ends at the reference's early-stop step (`tune` as in
+    width_sweep)."""
     sps = build_state_points(datasets, workers)
     n = len(sps)
     num = dict(numerics or sps[0].numerics)
@@ -194,7 +206,7 @@ def ph_sweep(datasets: Sequence[dict], pH_values: Sequence[float], *, waves: int
                               alpha=alpha, omega=float(om[k]), iterations=int(it[k]), warm=warm,
                               residual=float(err[k]))
 
-    _waves_solve(sps, waves, opts, on_round, finish, warm=copy_solution)
+    _waves_solve(sps, waves, opts, on_round, finish, warm=copy_solution, tune=tune)
     out = [r for r in res if r is not None]
     increasing = n < 2 or pH_values[-1] >= pH_values[0]
     for k, r in enumerate(out):

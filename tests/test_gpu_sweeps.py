@@ -9,12 +9,12 @@ from softdft_amd.sweeps import width_sweep
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("precision", [abi.PREC_FP64, abi.PREC_MIXED])
-def test_width_sweep_matches_cold_solves(precision):
+@pytest.mark.parametrize("precision,tune", [(abi.PREC_FP64, False), (abi.PREC_MIXED, False), (abi.PREC_MIXED, True)])
+def test_width_sweep_matches_cold_solves(precision, tune):
     Hs = [4.0 + 0.05 * i for i in range(6)]
     dss = [configs.weak_polyelectrolyte(H=H, dz=0.05, n_seg=4, solvent=0.2, pH=7.0) for H in Hs]
-    print(f"case width_sweep precision={precision}", flush=True)
-    res = width_sweep(dss, waves=3, precision=precision)
+    print(f"case width_sweep precision={precision} tune={tune}", flush=True)
+    res = width_sweep(dss, waves=3, precision=precision, tune=tune)
     # round r of 3 solves widths i % 3 == r; round 0 starts cold, the others from width i - 1
     assert [r.warm for r in res] == [i % 3 != 0 for i in range(6)]
     sps = [build_state_point(d) for d in dss]
