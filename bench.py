@@ -184,7 +184,6 @@ def report_single(name, ds, cpu_full, args):
         "fp64_gpu": {"value": 1.0 / t64, "unit": "state points/s", "iterations": int(it64[0]), "residual": float(e64[0])},
         "mixed_gpu": {"value": 1.0 / tm, "unit": "state points/s", "iterations": int(itm[0]), "residual": float(em[0])},
         "cpu_baseline": cpu,
-        "cpu_baseline_one_core": cpu1,
         "speedup_mixed_vs_fp64_gpu": t64 / tm, "speedup_mixed_vs_cpu": tcpu / tm,
         "errors": {
             "omega_fp64_gpu": float(om64[0]), "omega_mixed_rel_vs_fp64": float(abs(omm[0] - om64[0]) / abs(om64[0])),
