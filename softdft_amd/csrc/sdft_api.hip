@@ -961,6 +961,21 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         ctx->set_kbase_h = set_kbase;
         for (auto& v : ctx->set_shape) v.assign(ctx->nsets, -1);
         if ((rc = rebuild_conv_plans(ctx)) != SDFT_OK) return fail(rc);
+        // banded propagator steps on k_bandstep: every point's applied half-width (the band rounded up
+        // to 8 mod 16) fits its grid, and the workgroup's operator windows and inputs fit 64 KB of LDS
+        for (int u = 0; u < nsp; ++u) {
+            SpeciesHost& sh = c.species[u];
+            sh.band_step = 0;
+            sh.band_be = 0;
+            if (sh.nseg < 2 || sh.S > SDFT_MAX_S) continue;
+            bool ok = true;
+            for (int p = 0; p < P; ++p) {
+                const int Be = (band[ctx->pts[p].kset] + 7) / 16 * 16 + 8;
+                ok = ok && 2 * Be + 1 <= ctx->pts[p].NP;
+                sh.band_be = std::max(sh.band_be, Be);
+            }
+            sh.band_step = ok && bandstep_lds_bytes(sh.S, sh.band_be, sizeof(double)) <= 65536 ? 1 : 0;
+        }
         c.prop_plan.assign(nsp, OpPlan());
         for (int u = 0; u < nsp; ++u)
             if ((rc = build_prop_plan(ctx, sys, u, canon, set_L, set_kbase, band, c.prop_plan[u])) != SDFT_OK) return fail(rc);

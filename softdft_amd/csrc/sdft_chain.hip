@@ -374,4 +374,211 @@ template void launch_chain<float>(hipStream_t, const Ctx&, int, int, const float
 template void launch_chain<double>(hipStream_t, const Ctx&, int, int, const double*, const double*, const double*,
                                    const double*, double*, double*, unsigned long long*);
 
+
+// --------------------------------------------------------------------------
+// One banded propagator step for long grids (k_bandstep): the chain kernel's
+// tile GEMM D[r][p] = sum_t c[r - t + Be] x_ext[16 p + t - Be] on the
+// mirror-extended input, one launch per step, many workgroups per problem.
+// A workgroup owns (point, sweep, output state a) and 1024 consecutive outputs
+// (4 waves x 256); it stages the operator windows of the S input states b and
+// their extended inputs x_ab = exp(-lambda) exp(lng/2) Q_b trapez over the
+// 1024 + 2 Be elements the outputs need, then every wave runs its 256-output
+// tile per b and writes g[outseg][b][a] = |A x_ab| exp(lng/2) / normative
+// (child_propagators.jl:93-141 / parent_propagators.jl, per step), with the
+// grafted chains' max-distance cut and the per-step scale maximum as in the
+// per-step operator GEMM. Steps whose parent is the grafted segment are not
+// convolutions (direct sums) and stay with k_prop_step.
+// --------------------------------------------------------------------------
+constexpr int BAND_OUT = 1024;  // outputs per workgroup (4 waves x 16 positions x 16 phases)
+
+struct BandStepArgs {
+    int32_t u, s, nseg, S, fixed_seg, state_off, maxd_off, GF, MB, NB;
+    int64_t goff;
+    int32_t skip_sweep, rescale, nsp, nmax;
+    int32_t be_max, pad;
+    unsigned long long* smax;
+};
+
+size_t bandstep_lds_bytes(int S, int be, size_t elem) {
+    const int KC = 2 * be + 32, XL = BAND_OUT + 2 * be, XLP = XL + (XL >> 4) + 1;
+    return (size_t)(S * KC + S * XLP + CHAIN_LDS_SLACK) * elem;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bandstep(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
+                                                  const int32_t* __restrict__ evalflag, int prec, BandStepArgs A,
+                                                  const int32_t* __restrict__ seg_states,
+                                                  const double* __restrict__ maxd_tab,
+                                                  const int32_t* __restrict__ band, const T* __restrict__ kern,
+                                                  const T* __restrict__ elam, const T* __restrict__ hl,
+                                                  const T* __restrict__ trapez, T* __restrict__ gC,
+                                                  T* __restrict__ gP) {
+    const int S = A.S, N = A.nseg;
+    const int a = blockIdx.y % S;
+    const int sw = (blockIdx.y / S) & 1;
+    const int p = blockIdx.y / (2 * S);
+    if (evalflag[p] != prec || sw == A.skip_sweep) return;
+    const DevSystem& sys = *psys;
+    const DevPoint& P = pts[p];
+    const int NP = P.NP, L = P.L;
+    const int n0 = blockIdx.x * BAND_OUT;
+    if (n0 >= NP) return;
+    const int outseg = sw == 0 ? N - 1 - A.s : A.s;
+    const int inseg = sw == 0 ? outseg + 1 : outseg - 1;
+    const int* st = seg_states + A.state_off;
+    const int si = st[outseg * S + a];
+    if (si < 0) return;
+    const int Be = chain_half_width(band[P.kset]);
+    const int KC = 2 * Be + 32, XL = BAND_OUT + 2 * Be, XLP = XL + (XL >> 4) + 1;
+    const int KS = Be / 2 + 4;  // K / 4 with K = 2 Be + 16, an even number of 4-step groups
+    extern __shared__ unsigned char band_lds[];
+    T* cw = reinterpret_cast<T*>(band_lds);  // [S][KC] operator windows, reversed
+    T* X = cw + S * KC;                      // [S][XLP] extended inputs (one pad word per 16)
+    __shared__ int32_t colv[SDFT_MAX_S], colsj[SDFT_MAX_S], colbond[SDFT_MAX_S], colpi[SDFT_MAX_S], colpj[SDFT_MAX_S];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid < S) {
+        const int sj = st[inseg * S + tid];
+        int bond = 0;
+        if (sj >= 0) {
+            const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
+            for (int u2 = 0; u2 < A.MB; ++u2)
+                if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
+        }
+        colv[tid] = sj >= 0 ? 1 : 0;
+        colsj[tid] = sj >= 0 ? sj : 0;
+        colbond[tid] = bond;
+        colpi[tid] = si <= sj ? 0 : 1;  // output end of the bond
+        colpj[tid] = si <= sj ? 1 : 0;  // input end
+    }
+    __syncthreads();
+    // operator windows: cw[b][ri] = c[j], j = (KC - 1 - ri) - 15 - Be for |j| <= Be, else 0
+    const T* cset = kern + (int64_t)P.kbase * sys.K;
+    for (int e = tid; e < S * KC; e += 256) {
+        const int b = e / KC, ri = e - b * KC;
+        const int j = (KC - 1 - ri) - 15 - Be;
+        T v = T(0);
+        if (colv[b] && j >= -Be && j <= Be) {
+            int jj = j % L;
+            if (jj < 0) jj += L;
+            v = cset[(int64_t)(3 * A.NB + colbond[b]) * L + jj];
+        }
+        cw[e] = v;
+    }
+    // extended inputs: element mi is grid point reflect(n0 - Be + mi)
+    const double* maxd = maxd_tab + A.maxd_off;
+    const int Kf = A.fixed_seg >= 0 ? P.fixed_coord[A.u] : -1;
+    const bool stored = sw == 0 ? (inseg < N - 1) : (inseg > 0);
+    const T qinv = (A.rescale && stored)
+                       ? (T)(1.0 / scale_read(A.smax, scale_slot(p, A.u, sw, A.s - 1, A.nsp, A.nmax)))
+                       : T(1);
+    const T* gin = sw == 0 ? gC : gP;
+    const int64_t gb = P.base * (int64_t)A.GF + A.goff;
+    const int64_t eb = P.base * (int64_t)A.NB, hb = P.base * (int64_t)(2 * A.MB);
+    for (int e = tid; e < S * XL; e += 256) {
+        const int b = e / XL, mi = e - b * XL;
+        const int m = n0 - Be + mi;
+        const int n = m < 0 ? -1 - m : (m < NP ? m : 2 * NP - 1 - m);
+        T v = T(0);
+        if (colv[b] && n >= 0 && n < NP) {
+            T q;
+            if (!stored) {
+                q = base_value<T>(inseg, n, A.fixed_seg, Kf, P.dz, maxd);
+            } else {
+                q = T(0);
+                for (int b2 = 0; b2 < S; ++b2)
+                    if (st[inseg * S + b2] >= 0) q += gin[gb + ((int64_t)(inseg * S + b2) * S + b) * NP + n];
+            }
+            const int sj = colsj[b];
+            v = elam[eb + (int64_t)sj * NP + n] * hl[hb + (int64_t)(2 * colbond[b] + colpj[b]) * NP + n] *
+                (q * qinv) * trapez[eb + (int64_t)sj * NP + n];
+        }
+        X[b * XLP + chain_phys(mi)] = v;
+    }
+    __syncthreads();
+    // ---- per wave: the 256 outputs n0 + 256 wave + 16 q' + r, for every input state b
+    const int q = lane & 15, g = lane >> 4;
+    const int p0 = 16 * wave;
+    T* gout = sw == 0 ? gC : gP;
+    const double norm = P.normative[A.u];
+    double vmax = 0.0;
+    if (n0 + 256 * wave < NP) {
+        for (int b = 0; b < S; ++b) {
+            if (!colv[b]) continue;
+            T hv[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int n = n0 + 16 * (p0 + q) + Mfma<T>::row(lane, v);
+                hv[v] = n < NP ? hl[hb + (int64_t)(2 * colbond[b] + colpi[b]) * NP + n] : T(0);
+            }
+            const T* ap = cw + b * KC + (KC - 16 - 2 * Be) - q + g;
+            const T* xp = X + b * XLP + 17 * (p0 + q) + g;
+            typename Mfma<T>::acc_t acc;
+            for (int v = 0; v < 4; ++v) acc[v] = T(0);
+            T a0[4], b0[4], a1[4], b1[4];
+            auto fetch = [&](int K, T (&av)[4], T (&bv)[4]) {
+#pragma unroll
+                for (int k4 = 0; k4 < 4; ++k4) {
+                    av[k4] = ap[16 * K + 4 * k4];
+                    bv[k4] = xp[17 * K + 4 * k4];
+                }
+            };
+            const int KG = KS / 4;  // even
+            fetch(0, a0, b0);
+            for (int K = 0; K < KG; K += 2) {
+                fetch(K + 1, a1, b1);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int k4 = 0; k4 < 4; ++k4) acc = Mfma<T>::mma(a0[k4], b0[k4], acc);
+                __builtin_amdgcn_sched_barrier(0);
+                fetch(K + 2, a0, b0);  // past the end on the last pass: read from the slack, unused
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int k4 = 0; k4 < 4; ++k4) acc = Mfma<T>::mma(a1[k4], b1[k4], acc);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int n = n0 + 16 * (p0 + q) + Mfma<T>::row(lane, v);
+                if (n >= NP) continue;
+                const T y = acc[v];
+                T val = (y < T(0) ? -y : y) * hv[v] / (T)norm;
+                if (A.fixed_seg >= 0) {
+                    const double d = (double)(n - Kf) * P.dz;
+                    if (d * d > maxd[outseg] * maxd[outseg]) val = T(0);
+                }
+                gout[gb + ((int64_t)(outseg * S + b) * S + a) * NP + n] = val;
+                vmax = (double)val > vmax ? (double)val : vmax;
+            }
+        }
+    }
+    if (A.rescale) {
+        for (int o = 32; o > 0; o >>= 1) {
+            const double w = __shfl_xor(vmax, o, 64);
+            vmax = w > vmax ? w : vmax;
+        }
+        if (lane == 0) scale_note(A.smax, scale_slot(p, A.u, sw, A.s, A.nsp, A.nmax), vmax);
+    }
+}
+
+template <typename T>
+void launch_bandstep(hipStream_t st, const Ctx& c, int prec, int u, int s, int skip_sweep, const T* kern,
+                     const T* elam, const T* hl, const T* trapez, T* gC, T* gP, unsigned long long* smax) {
+    const SpeciesHost& sh = c.species[u];
+    BandStepArgs A;
+    memset(&A, 0, sizeof(A));
+    A.u = u; A.s = s; A.nseg = sh.nseg; A.S = sh.S; A.fixed_seg = sh.fixed_seg;
+    A.state_off = sh.state_off; A.maxd_off = sh.maxd_off; A.GF = c.GF; A.MB = c.MB; A.NB = c.NB;
+    A.goff = sh.goff; A.skip_sweep = skip_sweep; A.rescale = sh.rescale; A.nsp = c.nsp; A.nmax = c.nmax_seg;
+    A.be_max = sh.band_be; A.smax = smax;
+    const size_t lds = bandstep_lds_bytes(sh.S, sh.band_be, sizeof(T));
+    hipLaunchKernelGGL(k_bandstep<T>, dim3((c.maxNP + BAND_OUT - 1) / BAND_OUT, c.P * 2 * sh.S), dim3(256), lds, st,
+                       c.d_sys, c.d_pts, c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, c.d_band, kern, elam, hl,
+                       trapez, gC, gP);
+}
+
+template void launch_bandstep<float>(hipStream_t, const Ctx&, int, int, int, int, const float*, const float*,
+                                     const float*, const float*, float*, float*, unsigned long long*);
+template void launch_bandstep<double>(hipStream_t, const Ctx&, int, int, int, int, const double*, const double*,
+                                      const double*, const double*, double*, double*, unsigned long long*);
+
 }  // namespace sdft

@@ -195,13 +195,6 @@ struct PropArgs {
     unsigned long long* smax;
 };
 
-template <typename T>
-__device__ __forceinline__ T base_value(int seg, int n, int fixed_seg, int Kf, double dz, const double* maxd) {
-    if (fixed_seg < 0) return T(1);
-    if (seg == fixed_seg) return n == Kf ? T(1) : T(0);
-    const double d = (double)(n - Kf) * dz;
-    return (d * d > maxd[seg] * maxd[seg]) ? T(0) : T(1);
-}
 
 template <typename T, int TB>
 __global__ void __launch_bounds__(TB) k_prop_step(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
@@ -1725,11 +1718,21 @@ void Pipeline<T>::propagate(hipStream_t st, const Ctx& c, int prec, int u, bool 
                                sh.nseg, sh.fixed_seg, smax, lsc, c.nsp, c.nmax_seg);
         return;
     }
+    const bool bandstep = banded && sh.band_step != 0;
     for (int s = 1; s < sh.nseg; ++s) {
         A.s = s;
         // sweeps whose parent is the grafted segment use the direct real-space sums
         const bool fix0 = sh.fixed_seg == 0 && s == sh.nseg - 1;  // child sweep, parent = segment 0
         const bool fix1 = sh.fixed_seg == 0 && s == 1;            // parent sweep, parent = segment 0
+        if (bandstep) {
+            const int skip = (fix0 && fix1) ? 2 : fix0 ? 0 : fix1 ? 1 : -1;
+            if (skip != 2) launch_bandstep<T>(st, c, prec, u, s, skip, kern, elam, hl, trapez, gC, gP, smax);
+            if (skip >= 0)
+                hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(maxNP, 256, P, skip == 2 ? 2 : 1), dim3(256), 0, st,
+                                   c.d_sys, c.d_pts, c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl,
+                                   trapez, gC, gP, c.d_band, skip == 2 ? 0 : skip);
+            continue;
+        }
         if (!gemm) {
             hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(maxNP, 256, P, 2), dim3(256), 0, st, c.d_sys,
                                c.d_pts, c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl, trapez,

@@ -83,6 +83,8 @@ struct SpeciesHost {
     int nseg, S, model, fixed_seg, multistate, rescale;
     int state_off, maxd_off;
     int64_t goff;   // field offset in the propagator families
+    int band_step = 0;   // banded steps run k_bandstep (every point's band fits its grid)
+    int band_be = 0;     // largest applied half-width over the batch's grids (k_bandstep LDS)
 };
 
 // Device pointers and sizes shared by both precisions.
@@ -161,6 +163,17 @@ __device__ __forceinline__ const double* scale_row(const double* lsc, int p, int
     return lsc + ((int64_t)p * nsp + u) * (nmax + 1);
 }
 
+// propagator input at an end of the recursion: 1 at a free chain end; for a grafted chain 1 within
+// reach (max_distance) of the grafting point and 0 beyond, and at the grafted segment itself only at
+// the grafting point (handle_fixed.jl)
+template <typename T>
+__device__ __forceinline__ T base_value(int seg, int n, int fixed_seg, int Kf, double dz, const double* maxd) {
+    if (fixed_seg < 0) return T(1);
+    if (seg == fixed_seg) return n == Kf ? T(1) : T(0);
+    const double d = (double)(n - Kf) * dz;
+    return (d * d > maxd[seg] * maxd[seg]) ? T(0) : T(1);
+}
+
 // stores one convolution column's value y at grid point n of point P (ConvJob semantics)
 template <typename T>
 __device__ __forceinline__ void conv_store(const ConvJob& J, const DevPoint& P, int n, double y, T* nw, int Fnw,
@@ -215,6 +228,10 @@ hipError_t chain_prepare(const ChainPlanHost& ch);
 template <typename T>
 void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, const T* elam, const T* hl,
                   const T* trapez, T* gC, T* gP, unsigned long long* smax);
+template <typename T>
+void launch_bandstep(hipStream_t st, const Ctx& c, int prec, int u, int s, int skip_sweep, const T* kern,
+                     const T* elam, const T* hl, const T* trapez, T* gC, T* gP, unsigned long long* smax);
+size_t bandstep_lds_bytes(int S, int be, size_t elem);
 template <typename T>
 void launch_opgemm_prop(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, int u, int s, int skip_sweep,
                         bool banded, const T* kern, const T* elam, const T* hl, const T* trapez, T* gC, T* gP,
