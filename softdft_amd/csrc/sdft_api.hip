@@ -493,11 +493,13 @@ static int build_chain_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const st
     ch.ok32 = chain_lds_bytes(ch, sizeof(float), false) <= 160 * 1024;
     ch.ok64 = chain_lds_bytes(ch, sizeof(double), false) <= 160 * 1024;
     // output-factor table in LDS for homogeneous chains where it fits without costing occupancy:
-    // within 80 KB when the workgroup fitted there before (two per CU), else within 160 KB
+    // FP32 runs two workgroups per CU, so within 80 KB when the workgroup fitted there before; the
+    // FP64 variants run one per CU (registers), so within 160 KB
     auto hv_fits = [&](size_t elem, bool ok) {
         if (!ok || !ch.homog) return false;
         const size_t b0 = chain_lds_bytes(ch, elem, false), b1 = chain_lds_bytes(ch, elem, true);
-        return b1 <= (b0 <= 80 * 1024 ? 80 * 1024 : 160 * 1024);
+        const size_t lim = (elem == 4 && b0 <= 80 * 1024) ? 80 * 1024 : 160 * 1024;
+        return b1 <= lim;
     };
     ch.hv32 = hv_fits(sizeof(float), ch.ok32);
     ch.hv64 = hv_fits(sizeof(double), ch.ok64);
