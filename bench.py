@@ -429,7 +429,8 @@ def main():
     xd, itd, errd = batch.solve(x0, precision=abi.PREC_FP64, **num)
     om_fp64, al_fp64, _ = batch.observables()
     rel = np.abs(om_mixed - om_fp64) / np.abs(om_fp64)
-    log(f"max rel dOmega mixed vs FP64: {rel.max():.3e}")
+    dal = float(np.nanmax(np.abs(al_mixed[:, 0, 1] - al_fp64[:, 0, 1])))  # chain species' charged fraction
+    log(f"max rel dOmega mixed vs FP64: {rel.max():.3e}, max |d alpha|: {dal:.3e}")
     batch.close()
 
     # plain FP64, one state point at a time
@@ -494,6 +495,7 @@ def main():
         "fp64_one_at_a_time": {"value": single_rate, "unit": "state points/s", "sample_points": int(len(sel))},
         "speedup_vs_fp64_one_at_a_time": value / single_rate,
         "max_rel_dOmega_vs_fp64": float(rel.max()),
+        "max_abs_dalpha_vs_fp64": dal,
         "mean_iterations": float(np.mean([i.mean() for i in iters])),
         "roofline": {"bound": "mfma", "kernel": name, "achieved": achieved, "peak": PEAK[prec],
                      "unit": "TFLOP/s", "frac": achieved / PEAK[prec], "traffic": traffic_from_profiles(name)},
