@@ -45,6 +45,7 @@ struct sdft_ctx {
     double lB = 0.0;
     int fwd_terms = 0, back_terms = 0;   // convolution terms per point (algorithmic flop count)
     double* d_om_blocks = nullptr;       // per-point Omega block partials, allocated on first use
+    double* d_profile_fx = nullptr;      // sdft_kernel_profile's scratch F(x), allocated on first use
     // convolution plan inputs, kept to rebuild the plans with other tile shapes (sdft_autotune)
     std::vector<ConvJob> jobs_h;
     std::vector<std::vector<int>> canon;
@@ -1662,9 +1663,9 @@ extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps
         set_error(ctx, SDFT_E_ARG, "species %d does not run on the banded chain kernel", u);
         return SDFT_E_ARG;
     }
-    // scratch output so the solution buffers are untouched
-    double* d_fx = nullptr;
-    TRY(dalloc(ctx, &d_fx, c.total_n));
+    // scratch output so the solution buffers are untouched (allocated on the first call, kept)
+    if (!ctx->d_profile_fx) TRY(dalloc(ctx, &ctx->d_profile_fx, c.total_n));
+    double* d_fx = ctx->d_profile_fx;
     std::vector<int32_t> saved(c.P);
     HIP_CHECK_RET(hipMemcpyAsync(saved.data(), c.d_evalflag, c.P * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
     launch_set_evalflag(ctx->stream, c.d_evalflag, prec, c.P);
