@@ -395,15 +395,20 @@ def main():
         batch.close()
         return
 
-    # the same solve launched kernel by kernel instead of as captured poll windows (hipGraph)
-    batch.set_graph_capture(False)
+    # the same solve launched kernel by kernel instead of as captured poll windows (hipGraph), and
+    # with the FP32 / FP64 pipelines in turn instead of on two streams
+    batch.set_launch_mode(graph=False)
     t0 = time.perf_counter()
     xe, ite, erre = batch.solve(x0, precision=abi.PREC_MIXED, **num)
     eager_ms = (time.perf_counter() - t0) * 1e3
-    batch.set_graph_capture(True)
-    same = bool(np.array_equal(xe, x) and np.array_equal(ite, it))
-    log(f"eager launches: {eager_ms:.1f} ms per solve (graph windows: {sum(times) / len(times) * 1e3:.1f} ms), "
-        f"bitwise equal: {same}")
+    batch.set_launch_mode(graph=True, concurrent=False)
+    t0 = time.perf_counter()
+    xs_, its_, errs_ = batch.solve(x0, precision=abi.PREC_MIXED, **num)
+    seq_ms = (time.perf_counter() - t0) * 1e3
+    batch.set_launch_mode(concurrent=True)
+    same = bool(np.array_equal(xe, x) and np.array_equal(ite, it) and np.array_equal(xs_, x))
+    log(f"eager launches: {eager_ms:.1f} ms per solve, pipelines in turn: {seq_ms:.1f} ms "
+        f"(default: {sum(times) / len(times) * 1e3:.1f} ms), bitwise equal: {same}")
 
     # per-launch time of the hot kernel classes, HIP events on the library stream, with the
     # operators the mixed solve used; weighted by launches per evaluation x iterations run
@@ -504,8 +509,9 @@ def main():
         "kernels_us": {k: v[0] * 1e3 for k, v in prof.items()},
         "kernels_ms_per_solve_est": {k: v[3] for k, v in prof.items()},
         "pipeline_iterations": {"fp32": it32, "fp64": it64},
-        "launch": {"mode": "hipGraph per poll window", "eager_ms_per_step": eager_ms,
-                   "eager_bitwise_equal": same},
+        "launch": {"mode": "hipGraph per poll window, FP32 and FP64 pipelines on two streams",
+                   "eager_ms_per_step": eager_ms, "pipelines_in_turn_ms_per_step": seq_ms,
+                   "bitwise_equal": same},
         "tuning": [{"kind": {1: "propagator", 2: "conv_fwd", 3: "conv_pre", 4: "conv_back"}[e["kind"]],
                     "key": e["key"], "precision": e["precision"], "choice": e["choice"],
                     "us": [round(u, 2) for u in e["us"] if u >= 0]} for e in tuning],

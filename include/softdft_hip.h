@@ -189,12 +189,14 @@ int32_t sdft_set_operator_band(sdft_ctx_t* ctx, int32_t fp32_banded, int32_t fp6
  * (instrumentation; no counterpart in the reference) */
 int32_t sdft_last_pipeline_iters(const sdft_ctx_t* ctx, int32_t* fp32_iters, int32_t* fp64_iters);
 
-/* launch mode of sdft_solve: enable = 1 (default) runs each window of check_every iterations
+/* launch mode of sdft_solve: graph = 1 (default) runs each window of check_every iterations
  * between two host polls as one captured hipGraph (instantiated once per pair of active
- * pipelines and solve), 0 launches every kernel on the stream, < 0 leaves the mode unchanged.
- * graphs_built (may be NULL) receives the number of graphs instantiated so far.  Results are
- * identical in both modes.  (launch control; no counterpart in the reference) */
-int32_t sdft_set_graph_capture(sdft_ctx_t* ctx, int32_t enable, int64_t* graphs_built);
+ * pipelines and solve), 0 launches every kernel on the stream; concurrent = 1 (default) runs the
+ * FP32 and FP64 pipelines on two streams in iterations where both have points (they evaluate
+ * disjoint points), 0 one after the other; < 0 leaves a setting unchanged.  graphs_built (may be
+ * NULL) receives the number of graphs instantiated so far.  Results are identical in every mode.
+ * (launch control; no counterpart in the reference) */
+int32_t sdft_set_launch_mode(sdft_ctx_t* ctx, int32_t graph, int32_t concurrent, int64_t* graphs_built);
 
 /* Autotuner (no counterpart in the reference; the reference has one CPU code path).  Measures,
  * with HIP events on the context's stream, the candidate launch configurations of this batch and

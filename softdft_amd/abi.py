@@ -30,7 +30,7 @@ FIELD_RHO_SEGMENTS = 16  # + species index: rho_K.segments[u], (NP, S_u, N_u)
 EXPORTED = ("sdft_abi_version", "sdft_last_error", "sdft_create", "sdft_destroy", "sdft_solution_length",
             "sdft_solution_offsets", "sdft_objective", "sdft_solve", "sdft_observables", "sdft_get_field",
             "sdft_last_timing", "sdft_last_pipeline_iters", "sdft_set_operator_band", "sdft_kernel_profile",
-            "sdft_set_graph_capture", "sdft_autotune", "sdft_set_tuning")
+            "sdft_set_launch_mode", "sdft_autotune", "sdft_set_tuning")
 TUNE_PROPAGATOR, TUNE_CONV_FWD, TUNE_CONV_PRE, TUNE_CONV_BACK = 1, 2, 3, 4
 CONV_TALL, CONV_WIDE64, CONV_WIDE128, CONV_MID, CONV_NARROW = 0, 1, 2, 3, 4
 KP_PROP_FP32, KP_PROP_FP64, KP_CONV_FWD_FP32, KP_CONV_FWD_FP64, KP_CONV_BACK_FP32, KP_CONV_BACK_FP64, \
@@ -115,8 +115,8 @@ def load_library(path: str = LIB_PATH):
     lib.sdft_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(SdftTuneEntry), C.c_int32, _i32p]
     lib.sdft_set_tuning.restype = C.c_int32
     lib.sdft_set_tuning.argtypes = [C.c_void_p, C.POINTER(SdftTuneEntry), C.c_int32]
-    lib.sdft_set_graph_capture.restype = C.c_int32
-    lib.sdft_set_graph_capture.argtypes = [C.c_void_p, C.c_int32, _i64p]
+    lib.sdft_set_launch_mode.restype = C.c_int32
+    lib.sdft_set_launch_mode.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _i64p]
     if lib.sdft_abi_version() != SDFT_ABI_VERSION:
         raise RuntimeError("libsoftdft_hip ABI version mismatch")
     _lib = lib
@@ -340,13 +340,19 @@ class Batch:
             arr[i].choice = e["choice"]
         self._check(self.lib.sdft_set_tuning(self.h, arr, len(entries)))
 
-    def set_graph_capture(self, enable: Optional[bool] = None) -> int:
-        """Launch mode of solve(): True runs each poll window as one captured hipGraph (default),
-        False launches kernel by kernel, None keeps the mode.  Returns the graphs built so far."""
+    def set_launch_mode(self, graph: Optional[bool] = None, concurrent: Optional[bool] = None) -> int:
+        """Launch mode of solve(): graph windows (True, default) or kernel-by-kernel launches; FP32 and
+        FP64 pipelines side by side (True, default) or in turn; None keeps a setting. Returns the
+        graphs built so far."""
         n = C.c_int64()
-        mode = -1 if enable is None else int(bool(enable))
-        self._check(self.lib.sdft_set_graph_capture(self.h, mode, C.byref(n)))
+        g = -1 if graph is None else int(bool(graph))
+        cc = -1 if concurrent is None else int(bool(concurrent))
+        self._check(self.lib.sdft_set_launch_mode(self.h, g, cc, C.byref(n)))
         return n.value
+
+    def set_graph_capture(self, enable: Optional[bool] = None) -> int:
+        """Graph windows on / off (None: unchanged); returns the graphs built so far."""
+        return self.set_launch_mode(graph=enable)
 
     def pipeline_iters(self):
         """(FP32, FP64) batch iterations of the last solve during which each pipeline ran."""

@@ -19,6 +19,9 @@ struct sdft_ctx {
     Pipeline<float> p32;
     Pipeline<double> p64;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;       // the FP64 pipeline's stream while both precisions run
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int32_t concurrent = 1;              // run the two pipelines side by side
     int device = 0;
     std::vector<DevPoint> pts;
     std::vector<void*> allocs;
@@ -130,7 +133,11 @@ extern "C" const char* sdft_last_error(const sdft_ctx_t* ctx) {
 extern "C" void sdft_destroy(sdft_ctx_t* ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     for (void* p : ctx->allocs) (void)hipFree(p);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -631,6 +638,15 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         delete ctx;
         return SDFT_E_HIP;
     }
+    // second stream and fork / join events: iterations with points in both precisions run the FP32
+    // and FP64 pipelines side by side (they touch disjoint points)
+    if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+        set_error(ctx, SDFT_E_HIP, "hipStreamCreate / hipEventCreate failed");
+        sdft_destroy(ctx);
+        return SDFT_E_HIP;
+    }
     int rc = SDFT_OK;
     auto fail = [&](int code) {
         sdft_destroy(ctx);
@@ -1061,7 +1077,20 @@ static int check_errflags(sdft_ctx* ctx) {
 }
 
 // evaluates F(x) for every point whose evalflag is 32 or 64; a pipeline with no such point is skipped
+// When both precisions have points, the FP64 pipeline forks onto the second stream and joins before
+// the Anderson step: a point is evaluated at one precision per iteration, and every buffer the two
+// pipelines share (x, F(x), densities' partial sums, shared partials, error flags) is written per
+// point, so they touch disjoint memory. Under graph capture the fork and join become graph edges.
 static void run_objective(sdft_ctx* ctx, bool run32, bool run64) {
+    if (run32 && run64 && ctx->concurrent) {
+        (void)hipEventRecord(ctx->ev_fork, ctx->stream);
+        (void)hipStreamWaitEvent(ctx->stream2, ctx->ev_fork, 0);
+        ctx->p64.objective(ctx->stream2, ctx->c, 64, ctx->c.d_x, ctx->c.d_fx);
+        ctx->p32.objective(ctx->stream, ctx->c, 32, ctx->c.d_x, ctx->c.d_fx);
+        (void)hipEventRecord(ctx->ev_join, ctx->stream2);
+        (void)hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
+        return;
+    }
     if (run32) ctx->p32.objective(ctx->stream, ctx->c, 32, ctx->c.d_x, ctx->c.d_fx);
     if (run64) ctx->p64.objective(ctx->stream, ctx->c, 64, ctx->c.d_x, ctx->c.d_fx);
 }
@@ -1272,9 +1301,10 @@ extern "C" int32_t sdft_last_pipeline_iters(const sdft_ctx_t* ctx, int32_t* fp32
     return SDFT_OK;
 }
 
-extern "C" int32_t sdft_set_graph_capture(sdft_ctx_t* ctx, int32_t enable, int64_t* graphs_built) {
+extern "C" int32_t sdft_set_launch_mode(sdft_ctx_t* ctx, int32_t graph, int32_t concurrent, int64_t* graphs_built) {
     if (!ctx) return SDFT_E_ARG;
-    if (enable >= 0) ctx->graph_capture = enable ? 1 : 0;
+    if (graph >= 0) ctx->graph_capture = graph ? 1 : 0;
+    if (concurrent >= 0) ctx->concurrent = concurrent ? 1 : 0;
     if (graphs_built) *graphs_built = ctx->graphs_built;
     return SDFT_OK;
 }

@@ -1,4 +1,5 @@
-"""sdft_solve's captured poll windows (hipGraph) against kernel-by-kernel launches, through the C ABI.
+"""sdft_solve's launch modes through the C ABI: captured poll windows (hipGraph) against
+kernel-by-kernel launches, and the FP32 / FP64 pipelines on two streams against one after the other.
 
 The SCF kernels are deterministic (fixed reduction orders, no atomics on values that are read back
 in the same iteration), so both launch modes must give bitwise-identical solutions, iteration
@@ -15,8 +16,8 @@ from softdft_amd.problem import build_state_point
 pytestmark = pytest.mark.gpu
 
 
-def _solve(b, graphs, **kw):
-    b.set_graph_capture(graphs)
+def _solve(b, graphs, concurrent=True, **kw):
+    b.set_launch_mode(graph=graphs, concurrent=concurrent)
     x, it, err = b.solve(b.x0(), **kw)
     om, alpha, pc = b.observables()
     return x, it, err, om, alpha
@@ -53,6 +54,10 @@ def test_graph_windows_equal_eager_launches(precision):
     _assert_same(eager, g2)
     eager2 = _solve(b, False, max_iters=3000, precision=precision)
     _assert_same(eager, eager2)
+    # the FP32 and FP64 pipelines in turn instead of on two streams: the same solve
+    for graphs in (False, True):
+        seq = _solve(b, graphs, concurrent=False, max_iters=3000, precision=precision)
+        _assert_same(eager, seq)
     b.close()
 
 
