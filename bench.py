@@ -430,6 +430,19 @@ def main():
         log(f"kernel {name}: {ms * 1e3:.1f} us/launch, {fl / ms / 1e9:.2f} TFLOP/s, "
             f"{per_eval} launches/evaluation x {iters_run} iterations = {ms * per_eval * iters_run:.1f} ms per solve")
 
+    # density accumulation (memory-bound): bytes of one launch = the stored propagators gC and gP
+    # read once (2 N S^2 NP values per chain species) + the bead and bond densities written twice
+    # (plain and trapezoid-weighted)
+    sp0 = sps[0]
+    prop_vals = sum(2 * len(s.sequence) * max(len(f) for f in s.state_family) ** 2
+                    for s in sp0.ms.species if len(s.sequence) > 1)
+    for name, kc, w, iters_run in [("k_accumulate<float> (densities)", abi.KP_ACCUM_FP32, 4, it32),
+                                   ("k_accumulate<double> (densities)", abi.KP_ACCUM_FP64, 8, it64)]:
+        ms, _ = batch.kernel_profile(kc, reps=20)
+        nbytes = args.points * sp0.NP * w * (prop_vals + 2 * (sp0.NB + 2 * sp0.MB))
+        log(f"kernel {name}: {ms * 1e3:.1f} us/launch, {nbytes / ms / 1e9:.2f} TB/s of {nbytes / 1e6:.0f} MB, "
+            f"x {iters_run} iterations = {ms * iters_run:.1f} ms per solve")
+
     # FP64 reference for the acceptance bound (untimed)
     xd, itd, errd = batch.solve(x0, precision=abi.PREC_FP64, **num)
     om_fp64, al_fp64, _ = batch.observables()

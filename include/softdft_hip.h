@@ -247,6 +247,8 @@ int32_t sdft_set_tuning(sdft_ctx_t* ctx, const sdft_tune_entry_t* entries, int32
 #define SDFT_KP_POINTWISE_FP64 6 /* FMT/MSA derivatives (FP64 arithmetic) on FP64 fields */
 #define SDFT_KP_CHAIN_FP32 7     /* persistent banded chain propagation (all steps, both sweeps), FP32 */
 #define SDFT_KP_CHAIN_FP64 8     /* the same in FP64 */
+#define SDFT_KP_ACCUM_FP32 9     /* segment, bead and bond densities from the stored propagators, FP32 */
+#define SDFT_KP_ACCUM_FP64 10    /* the same in FP64 */
 
 /* Re-launch one kernel class `reps` times for every point on the context's
  * stream, bracketed by HIP events, after a solve or objective has filled the

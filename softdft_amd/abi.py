@@ -34,7 +34,7 @@ EXPORTED = ("sdft_abi_version", "sdft_last_error", "sdft_create", "sdft_destroy"
 TUNE_PROPAGATOR, TUNE_CONV_FWD, TUNE_CONV_PRE, TUNE_CONV_BACK = 1, 2, 3, 4
 CONV_TALL, CONV_WIDE64, CONV_WIDE128, CONV_MID, CONV_NARROW = 0, 1, 2, 3, 4
 KP_PROP_FP32, KP_PROP_FP64, KP_CONV_FWD_FP32, KP_CONV_FWD_FP64, KP_CONV_BACK_FP32, KP_CONV_BACK_FP64, \
-    KP_POINTWISE_FP64, KP_CHAIN_FP32, KP_CHAIN_FP64 = range(9)
+    KP_POINTWISE_FP64, KP_CHAIN_FP32, KP_CHAIN_FP64, KP_ACCUM_FP32, KP_ACCUM_FP64 = range(11)
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)

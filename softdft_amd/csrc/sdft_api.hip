@@ -1674,13 +1674,13 @@ extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps
         return SDFT_E_STATE;
     }
     Ctx& c = ctx->c;
-    if (kc < SDFT_KP_PROP_FP32 || kc > SDFT_KP_CHAIN_FP64) {
+    if (kc < SDFT_KP_PROP_FP32 || kc > SDFT_KP_ACCUM_FP64) {
         set_error(ctx, SDFT_E_ARG, "unknown kernel class %d", kc);
         return SDFT_E_ARG;
     }
     const bool chain = kc == SDFT_KP_CHAIN_FP32 || kc == SDFT_KP_CHAIN_FP64;
     const int prec = (kc == SDFT_KP_PROP_FP32 || kc == SDFT_KP_CONV_FWD_FP32 || kc == SDFT_KP_CONV_BACK_FP32 ||
-                      kc == SDFT_KP_CHAIN_FP32) ? 32 : 64;
+                      kc == SDFT_KP_CHAIN_FP32 || kc == SDFT_KP_ACCUM_FP32) ? 32 : 64;
     // pick the longest chain species for the propagator classes
     int u = 0;
     for (int k = 0; k < c.nsp; ++k)
@@ -1717,6 +1717,8 @@ extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps
                 launch_chain<double>(ctx->stream, c, 64, u, ctx->p64.kern, ctx->p64.elam, ctx->p64.hl,
                                      ctx->p64.trapez, ctx->p64.gC, ctx->p64.gP, ctx->p64.smax);
                 break;
+            case SDFT_KP_ACCUM_FP32: ctx->p32.launch_accumulate(ctx->stream, c, 32); break;
+            case SDFT_KP_ACCUM_FP64: ctx->p64.launch_accumulate(ctx->stream, c, 64); break;
             default: ctx->p64.launch_pointwise(ctx->stream, c, 64); break;
         }
     };
@@ -1767,8 +1769,8 @@ extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps
                 const bool bd = kc == SDFT_KP_PROP_FP32 ? c.band_fp32 != 0 : c.band_fp64 != 0;
                 const double taps = bd ? std::min(NP, 2.0 * band[ctx->pts[p].kset] + 1.0) : NP;
                 f += 2.0 /*sweeps*/ * npair * 2.0 * NP * taps;
-            } else if (kc == SDFT_KP_POINTWISE_FP64) {
-                f += 0.0;
+            } else if (kc == SDFT_KP_POINTWISE_FP64 || kc == SDFT_KP_ACCUM_FP32 || kc == SDFT_KP_ACCUM_FP64) {
+                f += 0.0;  // not counted in flops (memory-bound classes)
             } else {
                 const bool back = kc == SDFT_KP_CONV_BACK_FP32 || kc == SDFT_KP_CONV_BACK_FP64;
                 f += (back ? ctx->back_terms : ctx->fwd_terms) * 2.0 * NP * NP;

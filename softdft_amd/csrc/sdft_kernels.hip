@@ -1769,15 +1769,20 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
                            reinterpret_cast<double*>(smax), ns);
     }
     for (int u = 0; u < c.nsp; ++u) propagate(st, c, prec, u, true);
-    hipLaunchKernelGGL((k_accumulate<T, 256>), grid2(maxNP, ACC_POINTS, P), dim3(256), accumulate_lds_bytes(c), st, c.d_sys,
-                       c.d_pts, c.d_evalflag,
-                       prec, c.d_spec, c.d_goffs, c.GF, c.d_seg_states, c.d_maxd, elam, trapez, gC, gP,
-                       c.d_fixed_lambda, rho_beads, rho_bonds, rhoT, c.Frt, c.d_acc_part, c.d_errflag, lsc, c.nmax_seg);
+    launch_accumulate(st, c, prec);
     launch_conv(st, c, prec, false, fx);
     launch_pointwise(st, c, prec);
     launch_conv(st, c, prec, true, fx);
     hipLaunchKernelGGL(k_neutrality, dim3(P), dim3(64), 0, st, c.d_sys, c.d_pts, c.d_spec, c.d_evalflag, prec,
                        c.d_acc_part, c.d_acc_species, x, fx);
+}
+
+template <typename T>
+void Pipeline<T>::launch_accumulate(hipStream_t st, const Ctx& c, int prec) {
+    hipLaunchKernelGGL((k_accumulate<T, 256>), grid2(c.maxNP, ACC_POINTS, c.P), dim3(256), accumulate_lds_bytes(c), st,
+                       c.d_sys, c.d_pts, c.d_evalflag, prec, c.d_spec, c.d_goffs, c.GF, c.d_seg_states, c.d_maxd, elam,
+                       trapez, gC, gP, c.d_fixed_lambda, rho_beads, rho_bonds, rhoT, c.Frt, c.d_acc_part, c.d_errflag,
+                       lsc, c.nmax_seg);
 }
 
 template <typename T>

@@ -217,6 +217,7 @@ struct Pipeline {
     void propagate(hipStream_t st, const Ctx& c, int prec, int u, bool allow_chain);
     void launch_conv(hipStream_t st, const Ctx& c, int prec, bool back, double* fx);
     void launch_pointwise(hipStream_t st, const Ctx& c, int prec);
+    void launch_accumulate(hipStream_t st, const Ctx& c, int prec);
 };
 
 void launch_compact_cols(hipStream_t st, const Ctx& c, int prec);
