@@ -315,7 +315,7 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=16, help="host cores of the all-core CPU baseline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--kernel-only", choices=["chain32", "chain64", "fwd32", "fwd64", "back32", "back64"],
+    ap.add_argument("--kernel-only", choices=["chain32", "chain64", "fwd32", "fwd64", "back32", "back64", "acc32", "acc64"],
                     help="after one full-load evaluation, only re-launch this kernel class 21 times "
                          "(for rocprofv3 --kernel-trace / --pmc runs of that kernel)")
     ap.add_argument("--config", choices=["c1", "c2", "c4", "c5"],
@@ -370,7 +370,8 @@ def main():
         batch.set_operator_band(1, 1)
         batch.objective(x0, prec)
         kc = {"chain32": abi.KP_CHAIN_FP32, "chain64": abi.KP_CHAIN_FP64, "fwd32": abi.KP_CONV_FWD_FP32,
-              "fwd64": abi.KP_CONV_FWD_FP64, "back32": abi.KP_CONV_BACK_FP32, "back64": abi.KP_CONV_BACK_FP64}
+              "fwd64": abi.KP_CONV_FWD_FP64, "back32": abi.KP_CONV_BACK_FP32, "back64": abi.KP_CONV_BACK_FP64,
+              "acc32": abi.KP_ACCUM_FP32, "acc64": abi.KP_ACCUM_FP64}
         ms, fl = batch.kernel_profile(kc[args.kernel_only], reps=20)
         log(f"{args.kernel_only}: {ms * 1e3:.1f} us/launch, {fl:.4g} algorithmic flops/launch")
         batch.close()
