@@ -112,3 +112,23 @@ def test_autotune_decisions_and_replay():
     assert np.max(np.abs(om - om3) / np.maximum(1.0, np.abs(om3))) < 1e-8
     for bb in (b, b2, b3):
         bb.close()
+
+
+def test_kernel_profiling_leaves_the_context_unchanged():
+    """Every kernel class can be timed after an evaluation, and the timing launches (which write the
+    library's own buffers and a scratch F(x)) do not change what the next evaluation returns."""
+    sps = _points()
+    b = abi.Batch(sps)
+    x = b.x0()
+    before = {p: b.objective(x, p) for p in (abi.PREC_FP32, abi.PREC_FP64)}
+    print("case kernel profiling", flush=True)
+    for kc in range(abi.KP_ACCUM_FP64 + 1):
+        try:
+            ms, fl = b.kernel_profile(kc, reps=2)
+        except abi.SdftError as e:  # a class the context does not run (e.g. no chain-kernel species)
+            assert "chain" in str(e)
+            continue
+        assert ms > 0.0 and np.isfinite(fl) and fl >= 0.0
+    for p, fx in before.items():
+        assert np.array_equal(b.objective(x, p), fx)
+    b.close()
