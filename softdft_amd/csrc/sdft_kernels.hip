@@ -474,8 +474,11 @@ static_assert(ACC_POINTS * ACC_GROUPS == 256, "k_accumulate runs 256-thread bloc
 // Also per-species totals for the neutrality shift (neutrality_shift.jl:20-47):
 //   acc[u][0] = sum_seg,state num / N_u,  acc[u][1] = sum num * z
 // --------------------------------------------------------------------------
+// The FP64 variant is held to 4 waves per SIMD (128 VGPRs, a 12-byte spill): the kernel is bound by
+// load latency, and the extra wave made the full-load launch 156 -> 133 us. The FP32 variant reaches
+// 4 waves without a bound (held to 5 it spills 84 bytes per lane).
 template <typename T, int TB>
-__global__ void __launch_bounds__(TB) k_accumulate(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
+__global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
                                                    const int32_t* __restrict__ evalflag, int prec,
                                                    const DevSpecies* __restrict__ spec, const int64_t* __restrict__ goffs, int GF,
                                                    const int32_t* __restrict__ seg_states, const double* __restrict__ maxd_tab,

@@ -410,8 +410,10 @@ void launch_compact_cols(hipStream_t st, const Ctx& c, int prec) {
 // (blockIdx.x = tile + ntiles * part). The 128-row shape (CONV_WIDE128) runs FP32 as WM = 2 and
 // FP64 as WM = 1, RSPLIT = 2; which shape a grid length takes is a plan property (heuristic, or
 // measured by sdft_autotune).
+// The FP64 64x32 tile (WM = 1, WN = 2) is held to 5 waves per SIMD (96 VGPRs instead of 98, a
+// 12-byte spill): at full load the forward stage went 177 -> 169 us and the back stage 139 -> 134 us.
 template <typename T, int WM, int WN, int RSPLIT>
-__global__ void __launch_bounds__(256) k_convgemm(const DevPoint* __restrict__ pts,
+__global__ void __launch_bounds__(256, (sizeof(T) == 8 && WM == 1 && WN == 2) ? 5 : 1) k_convgemm(const DevPoint* __restrict__ pts,
                                                   const int32_t* __restrict__ evalflag, int prec,
                                                   const OpTile* __restrict__ tiles, const OpGroup* __restrict__ groups,
                                                   const OpCol* __restrict__ cols, const int32_t* __restrict__ count,
