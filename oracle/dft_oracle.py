@@ -727,27 +727,8 @@ class OracleDFT:
             if multi or self.sp.fixed[u].any or sp.evaluation == "simulation":
                 Qtot += Num[u] * qs[u]
                 qs[u] = 0.0
-        err = abs(Qtot + float(np.sum(Num * qs)))
-        psi1 = 0.0
-        cst = math.exp(-psi1)
-        it = 1
-        while err > 1e-10:
-            fx = Qtot
-            fx1 = 0.0
-            for i in range(nsp):
-                t = (cst ** qs[i]) * Num[i] * qs[i]
-                fx += t
-                fx1 -= t * qs[i]
-            err = abs(fx)
-            if abs(fx1) < 1e-10:
-                break
-            dl = max(1.0, abs(fx / fx1))
-            psi1 -= fx / (fx1 * dl)
-            cst = math.exp(-psi1)
-            it += 1
-            if it > 1000:
-                break
-        return Psi, PsiC_old + psi1
+        self.neutrality_inputs = (Qtot, Num.copy(), qs.copy())
+        return Psi, PsiC_old + neutrality_newton(Qtot, Num, qs)
 
     # -------------------------------------------------------- objective
     def objective(self, x: np.ndarray) -> np.ndarray:
@@ -891,6 +872,36 @@ class OracleDFT:
                     if sp.sw_wall_idx[j, w] >= 0:
                         c[w] += self.rho_beads[sp.sw_wall_idx[j, w], j] * sp.sw_wall_fac[j, w]
         return c
+
+
+# ---------------------------------------------------------------- neutrality shift
+
+def neutrality_newton(Qtot: float, Num: np.ndarray, qs: np.ndarray) -> float:
+    """Shift dPsi_C with Qtot + sum_u Num_u qs_u exp(-qs_u dPsi_C) = 0 (neutrality_shift.jl:60-96):
+    Newton from 0 with the step capped at 1 (del = max(1, |f/f'|)), stopping once |f| <= 1e-10 at the
+    start of an iteration, when |f'| < 1e-10, or after 1000 iterations. Species whose charge is fixed
+    (multistate, grafted, simulated) are folded into Qtot with qs_u = 0 by the caller (:49-58)."""
+    err = abs(Qtot + float(np.sum(Num * qs)))
+    psi1 = 0.0
+    cst = math.exp(-psi1)
+    it = 1
+    while err > 1e-10:
+        fx = Qtot
+        fx1 = 0.0
+        for i in range(len(qs)):
+            t = (cst ** qs[i]) * Num[i] * qs[i]
+            fx += t
+            fx1 -= t * qs[i]
+        err = abs(fx)
+        if abs(fx1) < 1e-10:
+            break
+        dl = max(1.0, abs(fx / fx1))
+        psi1 -= fx / (fx1 * dl)
+        cst = math.exp(-psi1)
+        it += 1
+        if it > 1000:
+            break
+    return psi1
 
 
 # ---------------------------------------------------------------- Anderson

@@ -134,6 +134,121 @@ def functional_flags(names: Sequence[str]) -> int:
     return f
 
 
+def build_descriptor(points: List[StatePoint]):
+    """The sdft_desc_t of a batch of state points that share one molecular-system layout, with the
+    numpy arrays it points into (keep them alive while the descriptor is used)."""
+    if not points:
+        raise ValueError("empty batch")
+    p0 = points[0]
+    ms = p0.ms
+    for sp in points[1:]:
+        m2 = sp.ms
+        if (m2.monomers != ms.monomers or m2.bond_types != ms.bond_types or len(m2.species) != len(ms.species)
+                or not np.array_equal(m2.diameters, ms.diameters) or not np.array_equal(m2.valences, ms.valences)
+                or sp.functionals != p0.functionals or abs(m2.bjerrum - ms.bjerrum) > 0
+                or m2.sw != ms.sw):
+            raise ValueError("state points in one batch must share the molecular-system layout")
+    NB, MB, nsp = ms.NB, ms.MB, len(ms.species)
+    P = len(points)
+    keep = {}
+    keep["diam"] = np.ascontiguousarray(ms.diameters, dtype=np.float64)
+    keep["val"] = np.ascontiguousarray(ms.valences, dtype=np.float64)
+    keep["bonds"] = np.ascontiguousarray(np.array(ms.bond_types, dtype=np.int32).reshape(-1))
+    nseg, model, nst, states, fseg, fden, maxd = [], [], [], [], [], [], []
+    for u, s in enumerate(ms.species):
+        for q in range(s.n_seg):
+            if len(s.topology.children[q]) > 1:
+                raise SdftError(-5, f"species {u}: branched chains are outside this build")
+        S = max(len(f) for f in s.state_family)
+        nseg.append(s.n_seg)
+        model.append(0 if s.model == "freely_jointed" else 1)
+        nst.append(S)
+        for f in s.state_family:
+            states.extend(list(f) + [-1] * (S - len(f)))
+        fx = p0.fixed[u]
+        if fx.any:
+            segs = np.nonzero(fx.segments)[0]
+            if len(segs) != 1 or segs[0] != 0:
+                raise SdftError(-5, f"species {u}: only a grafted first segment is supported")
+            fseg.append(0)
+            fden.append(fx.density)
+            maxd.extend(list(s.max_distance[:, 0]))
+        else:
+            fseg.append(-1)
+            fden.append(0.0)
+            maxd.extend([0.0] * s.n_seg)
+    keep["nseg"] = np.array(nseg, dtype=np.int32)
+    keep["model"] = np.array(model, dtype=np.int32)
+    keep["nst"] = np.array(nst, dtype=np.int32)
+    keep["states"] = np.array(states, dtype=np.int32)
+    keep["fseg"] = np.array(fseg, dtype=np.int32)
+    keep["fden"] = np.array(fden, dtype=np.float64)
+    keep["maxd"] = np.array(maxd, dtype=np.float64)
+    keep["NP"] = np.array([sp.NP for sp in points], dtype=np.int32)
+    keep["dz"] = np.array([sp.dz for sp in points], dtype=np.float64)
+    keep["dmu"] = np.ascontiguousarray(np.stack([sp.ms.delta_muH for sp in points]).astype(np.float64))
+    keep["mu"] = np.ascontiguousarray(np.stack([sp.mu_species for sp in points]).astype(np.float64))
+    keep["qtot"] = np.array([sp.total_charge for sp in points], dtype=np.float64)
+    keep["fcoord"] = np.ascontiguousarray(np.array([[int(sp.fixed[u].coords[0]) if sp.fixed[u].any else -1
+                                                     for u in range(nsp)] for sp in points], dtype=np.int32))
+    keep["Ext"] = np.ascontiguousarray(np.concatenate([sp.Ext.ravel(order="F") for sp in points]))
+    keep["trapez"] = np.ascontiguousarray(np.concatenate([sp.trapez.ravel(order="F") for sp in points]))
+    keep["surf"] = np.ascontiguousarray(np.concatenate([sp.surf[:sp.NP] for sp in points]))
+    keep["lo"] = np.ascontiguousarray(np.stack([sp.wall_lo for sp in points]).astype(np.int32))
+    keep["hi"] = np.ascontiguousarray(np.stack([sp.wall_hi for sp in points]).astype(np.int32))
+    has_q = any(sp.wall_charges is not None for sp in points)
+    keep["wq"] = np.ascontiguousarray(np.array([sp.wall_charges or (0.0, 0.0) for sp in points], dtype=np.float64))
+    d = SdftDesc()
+    d.abi_version = SDFT_ABI_VERSION
+    d.n_points = P
+    d.NB, d.MB, d.n_species = NB, MB, nsp
+    d.functionals = functional_flags(p0.functionals)
+    d.bjerrum = ms.bjerrum
+    d.diameters = _ptr(keep["diam"], C.c_double)
+    d.valences = _ptr(keep["val"], C.c_double)
+    d.bond_types = _ptr(keep["bonds"], C.c_int32)
+    d.species_nseg = _ptr(keep["nseg"], C.c_int32)
+    d.species_model = _ptr(keep["model"], C.c_int32)
+    d.species_nstates = _ptr(keep["nst"], C.c_int32)
+    d.seg_states = _ptr(keep["states"], C.c_int32)
+    d.species_fixed_seg = _ptr(keep["fseg"], C.c_int32)
+    d.fixed_density = _ptr(keep["fden"], C.c_double)
+    d.max_distance = _ptr(keep["maxd"], C.c_double)
+    d.NP = _ptr(keep["NP"], C.c_int32)
+    d.dz = _ptr(keep["dz"], C.c_double)
+    d.delta_muH = _ptr(keep["dmu"], C.c_double)
+    d.mu_species = _ptr(keep["mu"], C.c_double)
+    d.total_charge = _ptr(keep["qtot"], C.c_double)
+    d.fixed_coord = _ptr(keep["fcoord"], C.c_int32)
+    d.Ext = _ptr(keep["Ext"], C.c_double)
+    d.trapez = _ptr(keep["trapez"], C.c_double)
+    d.surf = _ptr(keep["surf"], C.c_double)
+    d.hard_walls = int(any(sp.hard_walls for sp in points))
+    d.wall_lo = _ptr(keep["lo"], C.c_int32)
+    d.wall_hi = _ptr(keep["hi"], C.c_int32)
+    d.wall_charge = _ptr(keep["wq"], C.c_double) if has_q else None
+    if any(sp.sw_wall_idx is not None for sp in points):
+        idx = np.full((P, NB, 2), -1, dtype=np.int32)
+        fac = np.zeros((P, NB, 2))
+        for k, sp in enumerate(points):
+            if sp.sw_wall_idx is not None:
+                idx[k] = sp.sw_wall_idx
+                fac[k] = sp.sw_wall_fac
+        keep["swwi"] = np.ascontiguousarray(idx)
+        keep["swwf"] = np.ascontiguousarray(fac)
+        d.sw_wall_idx = _ptr(keep["swwi"], C.c_int32)
+        d.sw_wall_fac = _ptr(keep["swwf"], C.c_double)
+    if ms.sw is not None and "sw" in p0.functionals:
+        keep["sw_pairs"] = np.ascontiguousarray(np.array(ms.sw.pairs, dtype=np.int32).reshape(-1))
+        keep["sw_eps"] = np.ascontiguousarray(np.array(ms.sw.energys[:len(ms.sw.pairs)], dtype=np.float64))
+        keep["sw_lam"] = np.ascontiguousarray(np.array(ms.sw.lambdas[:len(ms.sw.pairs)], dtype=np.float64))
+        d.n_sw_pairs = len(ms.sw.pairs)
+        d.sw_pairs = _ptr(keep["sw_pairs"], C.c_int32)
+        d.sw_energy = _ptr(keep["sw_eps"], C.c_double)
+        d.sw_lambda = _ptr(keep["sw_lam"], C.c_double)
+    return d, keep, nst, nseg
+
+
 class Batch:
     """A library context over state points that share one molecular-system layout."""
 
@@ -143,114 +258,11 @@ class Batch:
         lib = load_library()
         self.lib = lib
         self.points = points
-        p0 = points[0]
-        ms = p0.ms
-        for sp in points[1:]:
-            m2 = sp.ms
-            if (m2.monomers != ms.monomers or m2.bond_types != ms.bond_types or len(m2.species) != len(ms.species)
-                    or not np.array_equal(m2.diameters, ms.diameters) or not np.array_equal(m2.valences, ms.valences)
-                    or sp.functionals != p0.functionals or abs(m2.bjerrum - ms.bjerrum) > 0
-                    or m2.sw != ms.sw):
-                raise ValueError("state points in one batch must share the molecular-system layout")
+        d, keep, nst, nseg = build_descriptor(points)
+        ms = points[0].ms
         self.NB, self.MB, self.nsp = ms.NB, ms.MB, len(ms.species)
         P = len(points)
-        keep = {}
-        keep["diam"] = np.ascontiguousarray(ms.diameters, dtype=np.float64)
-        keep["val"] = np.ascontiguousarray(ms.valences, dtype=np.float64)
-        keep["bonds"] = np.ascontiguousarray(np.array(ms.bond_types, dtype=np.int32).reshape(-1))
-        nseg, model, nst, states, fseg, fden, maxd = [], [], [], [], [], [], []
-        for u, s in enumerate(ms.species):
-            for q in range(s.n_seg):
-                if len(s.topology.children[q]) > 1:
-                    raise SdftError(-5, f"species {u}: branched chains are outside this build")
-            S = max(len(f) for f in s.state_family)
-            nseg.append(s.n_seg)
-            model.append(0 if s.model == "freely_jointed" else 1)
-            nst.append(S)
-            for f in s.state_family:
-                states.extend(list(f) + [-1] * (S - len(f)))
-            fx = p0.fixed[u]
-            if fx.any:
-                segs = np.nonzero(fx.segments)[0]
-                if len(segs) != 1 or segs[0] != 0:
-                    raise SdftError(-5, f"species {u}: only a grafted first segment is supported")
-                fseg.append(0)
-                fden.append(fx.density)
-                maxd.extend(list(s.max_distance[:, 0]))
-            else:
-                fseg.append(-1)
-                fden.append(0.0)
-                maxd.extend([0.0] * s.n_seg)
-        keep["nseg"] = np.array(nseg, dtype=np.int32)
-        keep["model"] = np.array(model, dtype=np.int32)
-        keep["nst"] = np.array(nst, dtype=np.int32)
-        keep["states"] = np.array(states, dtype=np.int32)
-        keep["fseg"] = np.array(fseg, dtype=np.int32)
-        keep["fden"] = np.array(fden, dtype=np.float64)
-        keep["maxd"] = np.array(maxd, dtype=np.float64)
-        keep["NP"] = np.array([sp.NP for sp in points], dtype=np.int32)
-        keep["dz"] = np.array([sp.dz for sp in points], dtype=np.float64)
-        keep["dmu"] = np.ascontiguousarray(np.stack([sp.ms.delta_muH for sp in points]).astype(np.float64))
-        keep["mu"] = np.ascontiguousarray(np.stack([sp.mu_species for sp in points]).astype(np.float64))
-        keep["qtot"] = np.array([sp.total_charge for sp in points], dtype=np.float64)
-        keep["fcoord"] = np.ascontiguousarray(np.array([[int(sp.fixed[u].coords[0]) if sp.fixed[u].any else -1
-                                                         for u in range(self.nsp)] for sp in points], dtype=np.int32))
-        keep["Ext"] = np.ascontiguousarray(np.concatenate([sp.Ext.ravel(order="F") for sp in points]))
-        keep["trapez"] = np.ascontiguousarray(np.concatenate([sp.trapez.ravel(order="F") for sp in points]))
-        keep["surf"] = np.ascontiguousarray(np.concatenate([sp.surf[:sp.NP] for sp in points]))
-        keep["lo"] = np.ascontiguousarray(np.stack([sp.wall_lo for sp in points]).astype(np.int32))
-        keep["hi"] = np.ascontiguousarray(np.stack([sp.wall_hi for sp in points]).astype(np.int32))
-        has_q = any(sp.wall_charges is not None for sp in points)
-        keep["wq"] = np.ascontiguousarray(np.array([sp.wall_charges or (0.0, 0.0) for sp in points], dtype=np.float64))
         self._keep = keep
-        d = SdftDesc()
-        d.abi_version = SDFT_ABI_VERSION
-        d.n_points = P
-        d.NB, d.MB, d.n_species = self.NB, self.MB, self.nsp
-        d.functionals = functional_flags(p0.functionals)
-        d.bjerrum = ms.bjerrum
-        d.diameters = _ptr(keep["diam"], C.c_double)
-        d.valences = _ptr(keep["val"], C.c_double)
-        d.bond_types = _ptr(keep["bonds"], C.c_int32)
-        d.species_nseg = _ptr(keep["nseg"], C.c_int32)
-        d.species_model = _ptr(keep["model"], C.c_int32)
-        d.species_nstates = _ptr(keep["nst"], C.c_int32)
-        d.seg_states = _ptr(keep["states"], C.c_int32)
-        d.species_fixed_seg = _ptr(keep["fseg"], C.c_int32)
-        d.fixed_density = _ptr(keep["fden"], C.c_double)
-        d.max_distance = _ptr(keep["maxd"], C.c_double)
-        d.NP = _ptr(keep["NP"], C.c_int32)
-        d.dz = _ptr(keep["dz"], C.c_double)
-        d.delta_muH = _ptr(keep["dmu"], C.c_double)
-        d.mu_species = _ptr(keep["mu"], C.c_double)
-        d.total_charge = _ptr(keep["qtot"], C.c_double)
-        d.fixed_coord = _ptr(keep["fcoord"], C.c_int32)
-        d.Ext = _ptr(keep["Ext"], C.c_double)
-        d.trapez = _ptr(keep["trapez"], C.c_double)
-        d.surf = _ptr(keep["surf"], C.c_double)
-        d.hard_walls = int(any(sp.hard_walls for sp in points))
-        d.wall_lo = _ptr(keep["lo"], C.c_int32)
-        d.wall_hi = _ptr(keep["hi"], C.c_int32)
-        d.wall_charge = _ptr(keep["wq"], C.c_double) if has_q else None
-        if any(sp.sw_wall_idx is not None for sp in points):
-            idx = np.full((P, self.NB, 2), -1, dtype=np.int32)
-            fac = np.zeros((P, self.NB, 2))
-            for k, sp in enumerate(points):
-                if sp.sw_wall_idx is not None:
-                    idx[k] = sp.sw_wall_idx
-                    fac[k] = sp.sw_wall_fac
-            keep["swwi"] = np.ascontiguousarray(idx)
-            keep["swwf"] = np.ascontiguousarray(fac)
-            d.sw_wall_idx = _ptr(keep["swwi"], C.c_int32)
-            d.sw_wall_fac = _ptr(keep["swwf"], C.c_double)
-        if ms.sw is not None and "sw" in p0.functionals:
-            keep["sw_pairs"] = np.ascontiguousarray(np.array(ms.sw.pairs, dtype=np.int32).reshape(-1))
-            keep["sw_eps"] = np.ascontiguousarray(np.array(ms.sw.energys[:len(ms.sw.pairs)], dtype=np.float64))
-            keep["sw_lam"] = np.ascontiguousarray(np.array(ms.sw.lambdas[:len(ms.sw.pairs)], dtype=np.float64))
-            d.n_sw_pairs = len(ms.sw.pairs)
-            d.sw_pairs = _ptr(keep["sw_pairs"], C.c_int32)
-            d.sw_energy = _ptr(keep["sw_eps"], C.c_double)
-            d.sw_lambda = _ptr(keep["sw_lam"], C.c_double)
         self._desc = d
         h = C.c_void_p()
         rc = lib.sdft_create(C.byref(d), C.byref(h))

@@ -39,16 +39,22 @@ struct sdft_ctx {
     std::vector<int> seg_states_h;
     std::vector<int> state_off;
     std::vector<int> multistate;
-    std::vector<int> hard_lo, hard_hi;
-    std::vector<double> wall_q;
-    std::vector<int32_t> sw_wall_idx;   // [P][NB][2] well edges of square-well walls (-1 none)
-    std::vector<double> sw_wall_fac;    // [P][NB][2] exp(energy) - 1
     int hard_walls = 0, charged_walls = 0;
     std::vector<double> valences;
     double lB = 0.0;
     int fwd_terms = 0, back_terms = 0;   // convolution terms per point (algorithmic flop count)
     double* d_om_blocks = nullptr;       // per-point Omega block partials, allocated on first use
     double* d_profile_fx = nullptr;      // sdft_kernel_profile's scratch F(x), allocated on first use
+    double* d_segsums = nullptr;         // [P][max_u N_u S_u] segment sums of sdft_observables, first use
+    // contact-value theorem table (built by sdft_create): per point the entries off[p]..off[p+1) of
+    // (wall side, bead field offset j*NP + index or -1 for a constant term, factor); one gather kernel
+    // sums them into d_cvt[P][2]
+    int32_t* d_cvt_off = nullptr;
+    int32_t* d_cvt_idx = nullptr;
+    int32_t* d_cvt_side = nullptr;
+    double* d_cvt_fac = nullptr;
+    double* d_cvt = nullptr;
+    bool cvt_any = false;
     // convolution plan inputs, kept to rebuild the plans with other tile shapes (sdft_autotune)
     std::vector<ConvJob> jobs_h;
     std::vector<std::vector<int>> canon;
@@ -76,6 +82,47 @@ struct WindowGraphs {
         for (hipGraphExec_t& e : exec)
             if (e) (void)hipGraphExecDestroy(e);
     }
+};
+
+// Two timing events destroyed on every exit path.
+struct EventPair {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    EventPair() = default;
+    EventPair(const EventPair&) = delete;
+    EventPair& operator=(const EventPair&) = delete;
+    hipError_t create() {
+        hipError_t e = hipEventCreate(&e0);
+        return e != hipSuccess ? e : hipEventCreate(&e1);
+    }
+    ~EventPair() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    }
+};
+
+// Saves the per-point evaluation flags and writes them back on every exit path (profiling and
+// tuning force every point to one precision while they run).
+struct EvalflagGuard {
+    hipStream_t stream;
+    int32_t* d_flags;
+    std::vector<int32_t> saved;
+    bool armed = false;
+    EvalflagGuard(hipStream_t s, int32_t* d, int P) : stream(s), d_flags(d), saved(P) {}
+    EvalflagGuard(const EvalflagGuard&) = delete;
+    EvalflagGuard& operator=(const EvalflagGuard&) = delete;
+    hipError_t save() {
+        hipError_t e = hipMemcpyAsync(saved.data(), d_flags, saved.size() * sizeof(int32_t), hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        armed = e == hipSuccess;
+        return e;
+    }
+    hipError_t restore() {
+        if (!armed) return hipSuccess;
+        armed = false;
+        hipError_t e = hipMemcpyAsync(d_flags, saved.data(), saved.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream);
+        return e != hipSuccess ? e : hipStreamSynchronize(stream);
+    }
+    ~EvalflagGuard() { (void)restore(); }
 };
 
 static thread_local std::string g_last_error;
@@ -108,6 +155,18 @@ static int dalloc(sdft_ctx* ctx, T** p, int64_t count) {
     ctx->allocs.push_back(q);
     *p = (T*)q;
     return SDFT_OK;
+}
+
+// releases a buffer from dalloc before the context ends
+static void dfree(sdft_ctx* ctx, void* p) {
+    if (!p) return;
+    for (size_t i = 0; i < ctx->allocs.size(); ++i)
+        if (ctx->allocs[i] == p) {
+            (void)hipStreamSynchronize(ctx->stream);
+            (void)hipFree(p);
+            ctx->allocs.erase(ctx->allocs.begin() + (std::ptrdiff_t)i);
+            return;
+        }
 }
 
 #define TRY(expr)                      \
@@ -606,6 +665,51 @@ static int build_prop_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const std
     return SDFT_OK;
 }
 
+// Host-only checks of a descriptor, run before any device call: required tables present, grid sizes
+// in range, and every contact index inside its point's grid.
+static int validate_desc(const sdft_desc_t* d) {
+    const int P = d->n_points, NB = d->NB, nsp = d->n_species;
+    if (!d->diameters || !d->valences || (d->MB > 0 && !d->bond_types) || !d->species_nseg || !d->species_model ||
+        !d->species_nstates || !d->seg_states || !d->NP || !d->dz || !d->delta_muH || !d->mu_species || !d->Ext ||
+        !d->trapez || !d->surf) {
+        set_error(nullptr, SDFT_E_ARG, "a required descriptor table is NULL");
+        return SDFT_E_ARG;
+    }
+    for (int p = 0; p < P; ++p)
+        if (d->NP[p] < 4 || d->NP[p] > SDFT_MAX_TILES * SDFT_ACC_POINTS || !(d->dz[p] > 0.0)) {
+            set_error(nullptr, SDFT_E_ARG, "point %d: NP=%d dz=%g", p, d->NP[p], d->dz[p]);
+            return SDFT_E_ARG;
+        }
+    if (d->hard_walls && (!d->wall_lo || !d->wall_hi)) {
+        set_error(nullptr, SDFT_E_ARG, "hard_walls = 1 needs both contact-index tables wall_lo and wall_hi");
+        return SDFT_E_ARG;
+    }
+    if ((d->sw_wall_idx != nullptr) != (d->sw_wall_fac != nullptr)) {
+        set_error(nullptr, SDFT_E_ARG, "square-well walls need both sw_wall_idx and sw_wall_fac");
+        return SDFT_E_ARG;
+    }
+    for (int p = 0; p < P; ++p) {
+        const int NPp = d->NP[p];
+        for (int j = 0; j < NB && d->hard_walls; ++j) {
+            const int32_t lo = d->wall_lo[(size_t)p * NB + j], hi = d->wall_hi[(size_t)p * NB + j];
+            if (lo < 0 || lo >= NPp || hi < 0 || hi >= NPp) {
+                set_error(nullptr, SDFT_E_ARG, "point %d bead %d: hard-wall contact indices (%d, %d) outside [0, %d)",
+                          p, j, lo, hi, NPp);
+                return SDFT_E_ARG;
+            }
+        }
+        for (int k = 0; k < 2 * NB && d->sw_wall_idx; ++k) {
+            const int32_t i = d->sw_wall_idx[(size_t)p * NB * 2 + k];
+            if (i < -1 || i >= NPp) {
+                set_error(nullptr, SDFT_E_ARG, "point %d: square-well wall index %d outside the grid", p, i);
+                return SDFT_E_ARG;
+            }
+        }
+    }
+    (void)nsp;
+    return SDFT_OK;
+}
+
 extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     if (!out || !d) {
         set_error(nullptr, SDFT_E_ARG, "null argument");
@@ -626,6 +730,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         set_error(nullptr, SDFT_E_UNSUPPORTED, "list at most one FMT and one MSA functional");
         return SDFT_E_UNSUPPORTED;
     }
+    if (validate_desc(d) != SDFT_OK) return SDFT_E_ARG;
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) {
         set_error(nullptr, SDFT_E_HIP, "no HIP device available");
@@ -710,18 +815,44 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     sys.charged_walls = d->wall_charge != nullptr;
     ctx->hard_walls = d->hard_walls;
     ctx->charged_walls = d->wall_charge != nullptr;
-    if (d->sw_wall_idx && d->sw_wall_fac) {
-        ctx->sw_wall_idx.assign(d->sw_wall_idx, d->sw_wall_idx + (size_t)P * NB * 2);
-        ctx->sw_wall_fac.assign(d->sw_wall_fac, d->sw_wall_fac + (size_t)P * NB * 2);
-        for (int p = 0; p < P; ++p)
-            for (int k = 0; k < NB * 2; ++k) {
-                const int32_t i = ctx->sw_wall_idx[(size_t)p * NB * 2 + k];
-                if (i < -1 || i >= d->NP[p]) {
-                    set_error(ctx, SDFT_E_ARG, "point %d: square-well wall index %d outside the grid", p, i);
-                    return fail(SDFT_E_ARG);
-                }
+    // Contact-value theorem table (contact_value_theorem.jl:1-20): hard walls contribute
+    // sum_j rho_j at the contact index of each wall (hard_walls.jl:49-79), charged walls the constant
+    // -2 pi lB q^2 (charged_walls.jl:60-81), square-well walls rho_j at the well edge times
+    // (exp(energy) - 1) (sw_walls.jl:61-103). validate_desc has checked every index against its grid.
+    std::vector<int32_t> cvt_off(P + 1, 0), cvt_idx, cvt_side;
+    std::vector<double> cvt_fac;
+    for (int p = 0; p < P; ++p) {
+        const int NPp = d->NP[p];
+        if (d->hard_walls) {
+            for (int j = 0; j < NB; ++j) {
+                const int32_t lo = d->wall_lo[(size_t)p * NB + j], hi = d->wall_hi[(size_t)p * NB + j];
+                cvt_side.push_back(0); cvt_idx.push_back(j * NPp + lo); cvt_fac.push_back(1.0);
+                cvt_side.push_back(1); cvt_idx.push_back(j * NPp + hi); cvt_fac.push_back(1.0);
             }
+        }
+        if (d->wall_charge) {
+            for (int w = 0; w < 2; ++w) {
+                const double q = d->wall_charge[2 * p + w];
+                cvt_side.push_back(w); cvt_idx.push_back(-1); cvt_fac.push_back(-2.0 * M_PI * d->bjerrum * q * q);
+            }
+        }
+        if (d->sw_wall_idx) {
+            for (int j = 0; j < NB; ++j)
+                for (int w = 0; w < 2; ++w) {
+                    const size_t k = ((size_t)p * NB + j) * 2 + w;
+                    const int32_t i = d->sw_wall_idx[k];
+                    if (i < 0) continue;
+                    cvt_side.push_back(w); cvt_idx.push_back(j * NPp + i); cvt_fac.push_back(d->sw_wall_fac[k]);
+                }
+        }
+        cvt_off[p + 1] = (int32_t)cvt_idx.size();
     }
+    ctx->cvt_any = !cvt_idx.empty();
+    if ((rc = upload(ctx, &ctx->d_cvt_off, cvt_off.data(), P + 1)) != SDFT_OK) return fail(rc);
+    if ((rc = upload(ctx, &ctx->d_cvt_idx, cvt_idx.data(), (int64_t)cvt_idx.size())) != SDFT_OK) return fail(rc);
+    if ((rc = upload(ctx, &ctx->d_cvt_side, cvt_side.data(), (int64_t)cvt_side.size())) != SDFT_OK) return fail(rc);
+    if ((rc = upload(ctx, &ctx->d_cvt_fac, cvt_fac.data(), (int64_t)cvt_fac.size())) != SDFT_OK) return fail(rc);
+    if ((rc = dalloc(ctx, &ctx->d_cvt, 2 * (int64_t)P)) != SDFT_OK) return fail(rc);
 
     // ---- species
     std::vector<DevSpecies> spec(nsp);
@@ -1047,9 +1178,6 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     if ((rc = dalloc(ctx, &c.d_errbits, P)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_state, P)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &c.d_n_active, 3)) != SDFT_OK) return fail(rc);
-    ctx->hard_lo.assign(d->wall_lo ? d->wall_lo : nullptr, d->wall_lo ? d->wall_lo + (int64_t)P * NB : nullptr);
-    ctx->hard_hi.assign(d->wall_hi ? d->wall_hi : nullptr, d->wall_hi ? d->wall_hi + (int64_t)P * NB : nullptr);
-    if (d->wall_charge) ctx->wall_q.assign(d->wall_charge, d->wall_charge + 2 * P);
     HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
     *out = ctx;
     return SDFT_OK;
@@ -1135,6 +1263,9 @@ extern "C" int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* o, doub
     }
     // (re)size the Anderson history
     if (!c.d_Xh || c.m != o->anderm) {
+        dfree(ctx, c.d_Xh);
+        dfree(ctx, c.d_Gh);
+        c.d_Xh = c.d_Gh = nullptr;
         TRY(dalloc(ctx, &c.d_Xh, c.total_n * o->anderm));
         TRY(dalloc(ctx, &c.d_Gh, c.total_n * o->anderm));
         c.m = o->anderm;
@@ -1158,9 +1289,9 @@ extern "C" int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* o, doub
     launch_zero_i32(ctx->stream, c.d_errflag, c.P);
     launch_set_evalflag(ctx->stream, c.d_evalflag, sp.start_prec, c.P);
     launch_zero_f64(ctx->stream, (double*)c.d_errbits, c.P);
-    hipEvent_t e0, e1;
-    HIP_CHECK_RET(hipEventCreate(&e0));
-    HIP_CHECK_RET(hipEventCreate(&e1));
+    EventPair ev;
+    HIP_CHECK_RET(ev.create());
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     HIP_CHECK_RET(hipEventRecord(e0, ctx->stream));
     const int check = o->check_every > 0 ? o->check_every : 10;
     // the host polls the active counts every `check` iterations; mixed-precision switches are
@@ -1252,8 +1383,6 @@ extern "C" int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* o, doub
     HIP_CHECK_RET(hipEventSynchronize(e1));
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     ctx->last_solve_ms = ms;
     ctx->last_iters32 = n32;
     ctx->last_iters64 = n64;
@@ -1384,16 +1513,30 @@ extern "C" int32_t sdft_observables(sdft_ctx_t* ctx, double* omega, double* alph
         return SDFT_E_STATE;
     }
     Ctx& c = ctx->c;
-    const int P = c.P, NB = c.NB;
+    const int P = c.P;
+    // work buffers are allocated once per context and reused by every later call
     if (!ctx->d_om_blocks) TRY(dalloc(ctx, &ctx->d_om_blocks, (int64_t)P * SDFT_OMEGA_BLOCKS));
+    int64_t ss_stride = 0;
+    for (int u = 0; u < c.nsp; ++u)
+        if (ctx->multistate[u]) ss_stride = std::max<int64_t>(ss_stride, (int64_t)ctx->nseg[u] * ctx->S[u]);
+    if (alpha && ss_stride > 0 && !ctx->d_segsums) TRY(dalloc(ctx, &ctx->d_segsums, (int64_t)P * ss_stride));
     double* d_om = ctx->d_om_blocks;
     ctx->p32.omega(ctx->stream, c, c.d_x, d_om);
     ctx->p64.omega(ctx->stream, c, c.d_x, d_om);
+    // contact values of every point in one gather, read from the pipeline the point was evaluated in
+    if (p_cvt && ctx->cvt_any)
+        launch_cvt_gather(ctx->stream, c, ctx->d_cvt_off, ctx->d_cvt_idx, ctx->d_cvt_side, ctx->d_cvt_fac,
+                          ctx->p32.rho_beads, ctx->p64.rho_beads, ctx->d_cvt);
+    HIP_CHECK_RET(hipGetLastError());
     std::vector<double> omp((size_t)P * SDFT_OMEGA_BLOCKS), om(P, 0.0), accs((size_t)P * 2 * SDFT_MAX_SPECIES);
-    std::vector<DevSolveState> st(P);
     HIP_CHECK_RET(hipMemcpyAsync(omp.data(), d_om, omp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIP_CHECK_RET(hipMemcpyAsync(accs.data(), c.d_acc_species, accs.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_CHECK_RET(hipMemcpyAsync(st.data(), c.d_state, st.size() * sizeof(DevSolveState), hipMemcpyDeviceToHost, ctx->stream));
+    if (p_cvt) {
+        if (ctx->cvt_any)
+            HIP_CHECK_RET(hipMemcpyAsync(p_cvt, ctx->d_cvt, 2 * (size_t)P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        else
+            for (int i = 0; i < 2 * P; ++i) p_cvt[i] = 0.0;
+    }
     HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
     for (int p = 0; p < P; ++p)
         for (int b = 0; b < SDFT_OMEGA_BLOCKS; ++b) om[p] += omp[(size_t)p * SDFT_OMEGA_BLOCKS + b];
@@ -1410,8 +1553,7 @@ extern "C" int32_t sdft_observables(sdft_ctx_t* ctx, double* omega, double* alph
         for (int u = 0; u < c.nsp; ++u) {
             if (!ctx->multistate[u]) continue;
             const int N = ctx->nseg[u], S = ctx->S[u];
-            double* d_ss = nullptr;
-            TRY(dalloc(ctx, &d_ss, (int64_t)P * N * S));
+            double* d_ss = ctx->d_segsums;
             ctx->p32.segment_sums(ctx->stream, c, u, d_ss, (int64_t)N * S);
             ctx->p64.segment_sums(ctx->stream, c, u, d_ss, (int64_t)N * S);
             std::vector<double> ss((size_t)P * N * S);
@@ -1433,58 +1575,6 @@ extern "C" int32_t sdft_observables(sdft_ctx_t* ctx, double* omega, double* alph
                 for (int a = 0; a < S; ++a)
                     alpha[((int64_t)p * c.nsp + u) * Smax + a] = cnt[a] > 0 ? acc[a] / cnt[a] : NAN;
             }
-        }
-    }
-    if (p_cvt) {
-        for (int p = 0; p < P; ++p) {
-            double l = 0.0, r = 0.0;
-            if (ctx->hard_walls && !ctx->hard_lo.empty()) {
-                const DevPoint& Pt = ctx->pts[p];
-                const int prec = st[p].prec == 32 ? 32 : 64;
-                for (int j = 0; j < NB; ++j) {
-                    const int lo = ctx->hard_lo[(size_t)p * NB + j], hi = ctx->hard_hi[(size_t)p * NB + j];
-                    double v[2];
-                    if (prec == 64) {
-                        HIP_CHECK_RET(hipMemcpy(&v[0], ctx->p64.rho_beads + Pt.base * NB + (int64_t)j * Pt.NP + lo, 8, hipMemcpyDeviceToHost));
-                        HIP_CHECK_RET(hipMemcpy(&v[1], ctx->p64.rho_beads + Pt.base * NB + (int64_t)j * Pt.NP + hi, 8, hipMemcpyDeviceToHost));
-                    } else {
-                        float f[2];
-                        HIP_CHECK_RET(hipMemcpy(&f[0], ctx->p32.rho_beads + Pt.base * NB + (int64_t)j * Pt.NP + lo, 4, hipMemcpyDeviceToHost));
-                        HIP_CHECK_RET(hipMemcpy(&f[1], ctx->p32.rho_beads + Pt.base * NB + (int64_t)j * Pt.NP + hi, 4, hipMemcpyDeviceToHost));
-                        v[0] = f[0]; v[1] = f[1];
-                    }
-                    l += v[0];
-                    r += v[1];
-                }
-            }
-            if (ctx->charged_walls && !ctx->wall_q.empty()) {
-                l += -2.0 * M_PI * ctx->lB * ctx->wall_q[2 * p] * ctx->wall_q[2 * p];
-                r += -2.0 * M_PI * ctx->lB * ctx->wall_q[2 * p + 1] * ctx->wall_q[2 * p + 1];
-            }
-            if (!ctx->sw_wall_idx.empty()) {
-                // square-well walls: rho at the well edge times (exp(energy) - 1) (sw_walls.jl:61-103)
-                const DevPoint& Pt = ctx->pts[p];
-                const int prec = st[p].prec == 32 ? 32 : 64;
-                for (int j = 0; j < NB; ++j)
-                    for (int w = 0; w < 2; ++w) {
-                        const size_t k = ((size_t)p * NB + j) * 2 + w;
-                        const int32_t i = ctx->sw_wall_idx[k];
-                        if (i < 0) continue;
-                        double v;
-                        if (prec == 64) {
-                            HIP_CHECK_RET(hipMemcpy(&v, ctx->p64.rho_beads + Pt.base * NB + (int64_t)j * Pt.NP + i, 8,
-                                                    hipMemcpyDeviceToHost));
-                        } else {
-                            float f;
-                            HIP_CHECK_RET(hipMemcpy(&f, ctx->p32.rho_beads + Pt.base * NB + (int64_t)j * Pt.NP + i, 4,
-                                                    hipMemcpyDeviceToHost));
-                            v = f;
-                        }
-                        (w == 0 ? l : r) += v * ctx->sw_wall_fac[k];
-                    }
-            }
-            p_cvt[2 * p] = l;
-            p_cvt[2 * p + 1] = r;
         }
     }
     return SDFT_OK;
@@ -1599,24 +1689,20 @@ extern "C" int32_t sdft_autotune(sdft_ctx_t* ctx, int32_t reps, sdft_tune_entry_
     if (!ctx || reps < 1 || (capacity > 0 && !entries)) return SDFT_E_ARG;
     Ctx& c = ctx->c;
     ctx->tuning.clear();
-    std::vector<int32_t> saved(c.P);
-    HIP_CHECK_RET(hipMemcpyAsync(saved.data(), c.d_evalflag, c.P * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    EvalflagGuard flags(ctx->stream, c.d_evalflag, c.P);
+    HIP_CHECK_RET(flags.save());
     const int band32 = c.band_fp32, band64 = c.band_fp64;
     c.band_fp32 = 1;
     c.band_fp64 = 1;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    int rc = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess ? SDFT_OK : SDFT_E_HIP;
-    if (rc == SDFT_OK) rc = autotune_run(ctx, reps, e0, e1);
+    EventPair ev;
+    int rc = ev.create() == hipSuccess ? SDFT_OK : SDFT_E_HIP;
+    if (rc == SDFT_OK) rc = autotune_run(ctx, reps, ev.e0, ev.e1);
     else set_error(ctx, SDFT_E_HIP, "hipEventCreate failed");
-    if (e0) (void)hipEventDestroy(e0);
-    if (e1) (void)hipEventDestroy(e1);
     c.band_fp32 = band32;
     c.band_fp64 = band64;
     ctx->solved = false;
     if (rc != SDFT_OK) return rc;
-    HIP_CHECK_RET(hipMemcpyAsync(c.d_evalflag, saved.data(), c.P * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    HIP_CHECK_RET(flags.restore());
     const int n = (int)ctx->tuning.size();
     for (int i = 0; i < n && i < capacity; ++i) entries[i] = ctx->tuning[i];
     if (n_entries) *n_entries = n;
@@ -1696,8 +1782,8 @@ extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps
     // scratch output so the solution buffers are untouched (allocated on the first call, kept)
     if (!ctx->d_profile_fx) TRY(dalloc(ctx, &ctx->d_profile_fx, c.total_n));
     double* d_fx = ctx->d_profile_fx;
-    std::vector<int32_t> saved(c.P);
-    HIP_CHECK_RET(hipMemcpyAsync(saved.data(), c.d_evalflag, c.P * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    EvalflagGuard flags(ctx->stream, c.d_evalflag, c.P);
+    HIP_CHECK_RET(flags.save());
     launch_set_evalflag(ctx->stream, c.d_evalflag, prec, c.P);
     if (c.use_gemm) launch_compact_cols(ctx->stream, c, prec);  // every point active in the profiled launches
     const int s_mid = c.species[u].nseg / 2 > 0 ? c.species[u].nseg / 2 : 1;
@@ -1723,19 +1809,15 @@ extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps
         }
     };
     once();  // warm
-    hipEvent_t e0, e1;
-    HIP_CHECK_RET(hipEventCreate(&e0));
-    HIP_CHECK_RET(hipEventCreate(&e1));
-    HIP_CHECK_RET(hipEventRecord(e0, ctx->stream));
+    EventPair ev;
+    HIP_CHECK_RET(ev.create());
+    HIP_CHECK_RET(hipEventRecord(ev.e0, ctx->stream));
     for (int r = 0; r < reps; ++r) once();
-    HIP_CHECK_RET(hipEventRecord(e1, ctx->stream));
-    HIP_CHECK_RET(hipEventSynchronize(e1));
+    HIP_CHECK_RET(hipEventRecord(ev.e1, ctx->stream));
+    HIP_CHECK_RET(hipEventSynchronize(ev.e1));
     float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    HIP_CHECK_RET(hipMemcpyAsync(c.d_evalflag, saved.data(), c.P * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream));
-    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    (void)hipEventElapsedTime(&ms, ev.e0, ev.e1);
+    HIP_CHECK_RET(flags.restore());
     HIP_CHECK_RET(hipGetLastError());
     if (avg_ms) *avg_ms = ms / reps;
     if (flops) {

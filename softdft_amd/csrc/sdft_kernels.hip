@@ -1632,6 +1632,28 @@ __global__ void __launch_bounds__(256) k_segment_profile(const DevSystem* __rest
     }
 }
 
+// Contact-value theorem (contact_value_theorem.jl:1-20): per point and wall side the sum of the
+// table's entries, rho_beads at a (bead field, grid index) offset times a factor, or a constant term
+// (index < 0: charged walls). Densities come from the pipeline the point was last evaluated in.
+__global__ void k_cvt_gather(const DevPoint* __restrict__ pts, const DevSolveState* __restrict__ stt,
+                             const int32_t* __restrict__ off, const int32_t* __restrict__ idx,
+                             const int32_t* __restrict__ side, const double* __restrict__ fac,
+                             const float* __restrict__ rb32, const double* __restrict__ rb64, int NB, int P_count,
+                             double* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P_count) return;
+    const int64_t base = pts[p].base * (int64_t)NB;
+    const bool f32 = stt[p].prec == 32;
+    double s[2] = {0.0, 0.0};
+    for (int e = off[p]; e < off[p + 1]; ++e) {
+        const int32_t i = idx[e];
+        const double v = i < 0 ? 1.0 : (f32 ? (double)rb32[base + i] : rb64[base + i]);
+        s[side[e]] += v * fac[e];
+    }
+    out[2 * p] = s[0];
+    out[2 * p + 1] = s[1];
+}
+
 __global__ void k_set_evalflag(int32_t* evalflag, int value, int P_count) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < P_count) evalflag[p] = value;
@@ -1898,6 +1920,12 @@ void launch_anderson(hipStream_t st, const Ctx& c, const SolveParams& sp) {
 
 void launch_apply_switch(hipStream_t st, const Ctx& c) {
     hipLaunchKernelGGL(k_apply_switch, dim3((c.P + 63) / 64), dim3(64), 0, st, c.d_state, c.d_evalflag, c.P);
+}
+
+void launch_cvt_gather(hipStream_t st, const Ctx& c, const int32_t* off, const int32_t* idx, const int32_t* side,
+                       const double* fac, const float* rb32, const double* rb64, double* out) {
+    hipLaunchKernelGGL(k_cvt_gather, dim3((c.P + 63) / 64), dim3(64), 0, st, c.d_pts, c.d_state, off, idx, side, fac,
+                       rb32, rb64, c.NB, c.P, out);
 }
 
 void launch_set_evalflag(hipStream_t st, int32_t* evalflag, int value, int P) {

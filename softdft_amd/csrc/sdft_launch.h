@@ -249,6 +249,8 @@ void launch_build_kernels(hipStream_t st, const DevSystem* sys, int K, int nsets
 void launch_anderson(hipStream_t st, const Ctx& c, const SolveParams& sp);
 hipError_t prepare_kernel_attributes(const Ctx& c);
 void launch_apply_switch(hipStream_t st, const Ctx& c);
+void launch_cvt_gather(hipStream_t st, const Ctx& c, const int32_t* off, const int32_t* idx, const int32_t* side,
+                       const double* fac, const float* rb32, const double* rb64, double* out);
 void launch_set_evalflag(hipStream_t st, int32_t* evalflag, int value, int P);
 void launch_zero_i32(hipStream_t st, int32_t* a, int64_t n);
 void launch_zero_f64(hipStream_t st, double* a, int64_t n);

@@ -58,3 +58,32 @@ def test_no_device_fails_loudly():
     with pytest.raises(abi.SdftError) as e:
         abi.Batch([sp])
     assert e.value.code == -2
+
+
+def test_hard_walls_need_contact_tables_and_indices_inside_the_grid():
+    """hard_walls = 1 with a NULL contact-index table, or an index outside [0, NP), is rejected
+    before any device call (the contact-value theorem would otherwise drop or misread the hard-wall
+    term, hard_walls.jl:49-79)."""
+    lib = abi.load_library()
+    sp = build_state_point(configs.hard_sphere_chains(H=3.0, dz=0.05, n_seg=3))
+    assert sp.hard_walls
+    d, keep, _, _ = abi.build_descriptor([sp])
+    h = C.c_void_p()
+    d.wall_lo = None
+    assert lib.sdft_create(C.byref(d), C.byref(h)) == -1
+    assert b"wall_lo and wall_hi" in lib.sdft_last_error(None)
+    d.wall_lo = abi._ptr(keep["lo"], C.c_int32)
+    d.wall_hi = None
+    assert lib.sdft_create(C.byref(d), C.byref(h)) == -1
+    d.wall_hi = abi._ptr(keep["hi"], C.c_int32)
+    bad = keep["hi"].copy()
+    bad[0, 0] = sp.NP
+    d.wall_hi = abi._ptr(bad, C.c_int32)
+    assert lib.sdft_create(C.byref(d), C.byref(h)) == -1
+    assert b"contact indices" in lib.sdft_last_error(None)
+    bad[0, 0] = -1
+    assert lib.sdft_create(C.byref(d), C.byref(h)) == -1
+    d.wall_hi = abi._ptr(keep["hi"], C.c_int32)
+    if not os.path.exists("/dev/kfd"):
+        # a valid descriptor passes validation and reaches the device check
+        assert lib.sdft_create(C.byref(d), C.byref(h)) == -2

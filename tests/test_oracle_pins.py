@@ -15,6 +15,13 @@ closed forms, finite differences, invariants or brute force:
   weights + mirrored FFT convolution           uniform field -> n3 = sum rho pi sigma^3/6 exactly; brute-force
     (fourier_transforms/*.jl, apply_*.jl)      circulant matvec; Toeplitz +- Hankel operator == FFT path
   Poisson (coulomb.jl:96-125)                  discrete Poisson identity on the periodic extension
+  neutrality shift (neutrality_shift.jl:1-99)  closed form e^{dPsi} = (Q + sqrt(Q^2 + 4 N+ N-))/(2 N-) for free
+                                               +-1 ions; root residual with multivalent species; the shift of
+                                               F(x0) in a charged bulk, recomputed from the bead densities;
+                                               converged charged slits are electroneutral
+  charged-wall contact term (charged_walls.jl:60-81)  ideal +-1 electrolyte between charged hard walls:
+                                               P_cvt = sum rho(contact) - 2 pi lB q^2 equals the mid-slit
+                                               pressure, with second-order convergence in dz
   propagators / densities (species_model/)     ideal chain without walls reproduces bulk weights; grafted chain
                                                integrates to sigma_graft per segment; Henderson-Hasselbalch alpha;
                                                dimer density equals the explicit one-bond integral
@@ -459,6 +466,24 @@ def test_bulk_consistency_kat(builder):
     assert np.max(np.abs(mu)) < 1e-13 and np.max(np.abs(lng)) < 1e-13 and np.max(np.abs(psi)) < 1e-13
     if builder != "charged":
         assert abs(psic) < 1e-15
+    else:
+        # The reference's bulk charge balance counts the ideal dissociation while the SCF counts the
+        # interacting one, so the shift is not zero; it must still make the slab neutral. Recomputed
+        # from the bead densities: the multistate chain's charge is fixed, the free ions scale by
+        # exp(-z dPsi) (neutrality_shift.jl:49-60).
+        ms = sp.ms
+        chain_beads = {b for s in ms.species if any(len(f) > 1 for f in s.state_family)
+                       for f in s.state_family for b in f}
+        q = sp.total_charge
+        free = []
+        for j in range(ms.NB):
+            n_j = float(np.sum(o.rho_beads[:, j] * sp.trapez[:, j])) * sp.dz
+            if j in chain_beads:
+                q += ms.valences[j] * n_j
+            elif ms.valences[j] != 0:
+                free.append((n_j, ms.valences[j]))
+        assert abs(psic) > 1e-8
+        assert abs(q + sum(n * z * math.exp(-z * psic) for n, z in free)) < 1e-10
     om = o.grand_potential(x0)
     assert om / (sp.NP * sp.dz) == pytest.approx(-sp.bulk.pressure, rel=1e-12)
 
@@ -565,3 +590,79 @@ def test_input_external_overwrites_rows(tmp_path):
     sp = build_state_point(_ideal_gas_slit({"input_external": {"dims": [1], "file": str(f)}}))
     assert list(sp.Ext[0]) == [5.0, 6.0] and list(sp.Ext[3]) == [-1.5, 0.25]
     assert np.count_nonzero(sp.Ext) == 4
+
+
+# ----------------------------------------------------------------- neutrality shift
+@pytest.mark.parametrize("Q,Np,Nm", [(0.0, 0.3, 0.3), (0.05, 0.3, 0.2), (-0.4, 0.01, 0.5), (3.0, 0.2, 0.1),
+                                     (-2.0, 1e-3, 1e-3)])
+def test_neutrality_newton_closed_form_for_monovalent_ions(Q, Np, Nm):
+    """Q + N+ e^{-d} - N- e^{d} = 0 gives e^{d} = (Q + sqrt(Q^2 + 4 N+ N-)) / (2 N-); the last two
+    cases start with |f/f'| > 1 and exercise the capped Newton step."""
+    d = od.neutrality_newton(Q, np.array([Np, Nm]), np.array([1.0, -1.0]))
+    exact = math.log((Q + math.sqrt(Q * Q + 4 * Np * Nm)) / (2 * Nm))
+    assert d == pytest.approx(exact, abs=1e-9)
+    assert abs(Q + Np * math.exp(-d) - Nm * math.exp(d)) < 1e-10
+
+
+def test_neutrality_newton_root_with_multivalent_species():
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        qs = np.array([1.0, -1.0, 2.0, -3.0, 0.0])
+        Num = rng.uniform(0.01, 0.5, size=5)
+        Q = rng.uniform(-0.5, 0.5)
+        d = od.neutrality_newton(Q, Num, qs)
+        assert abs(Q + float(np.sum(Num * qs * np.exp(-qs * d)))) < 1e-10
+    # already neutral: no shift
+    assert od.neutrality_newton(0.1, np.array([0.2, 0.3]), np.array([1.0, -1.0])) == 0.0
+
+
+def _salt_slit(H, dz, q, salt=0.1):
+    ext = {"hard_walls": {"dims": [1]}, "charged_walls": {"dims": [1], "charge": [[q, q]]}}
+    geo = {"coordinate_system": "Cartesian", "dimensions": [H], "bin_width": [dz], "external_field": ext}
+    return {"procedure": "iDFT", "geometry": geo,
+            "species": [{"sequence": "c", "concentration": salt}, {"sequence": "a", "concentration": salt}],
+            "monomers": {"c": {"diameter": 1.0, "valence": 1.0}, "a": {"diameter": 1.0, "valence": -1.0}},
+            "numerical": dict(configs.SCF_NUMERICS_CHARGED), "model": ["ideal", "coul"]}
+
+
+def _solve_tight(ds, tole=1e-10):
+    sp, o = _oracle_for(ds)
+    num = dict(sp.numerics)
+    num["tole"] = tole
+    r = od.anderson_solve(o.objective, sp.x0(), num, max_iters=20000)
+    assert r.err < tole
+    return sp, o, r
+
+
+@pytest.mark.parametrize("system", ["salt", "weak_pe"])
+def test_converged_charged_slit_is_electroneutral(system):
+    """At the fixed point the neutrality shift is zero, i.e. the wall charge plus every ion's and
+    segment's charge integrates to zero (neutrality_shift.jl:17-60)."""
+    if system == "salt":
+        ds = _salt_slit(12.0, 0.05, 0.03)
+    else:
+        ds = configs.weak_polyelectrolyte(H=5.0, dz=0.05, n_seg=4, solvent=0.2, pH=6.0, wall_charge=(-0.05, -0.02))
+    sp, o, r = _solve_tight(ds)
+    ms = sp.ms
+    q = sp.total_charge + sum(ms.valences[j] * float(np.sum(o.rho_beads[:, j] * sp.trapez[:, j])) * sp.dz
+                              for j in range(ms.NB))
+    assert abs(sp.total_charge) > 1e-3
+    assert abs(q) < 1e-9
+
+
+def test_contact_theorem_with_charged_walls_ideal_electrolyte():
+    """Poisson-Boltzmann contact theorem: between charged hard walls the normal pressure is
+    sum_j rho_j(contact) - 2 pi lB q^2 at each wall and sum_j rho_j where the field vanishes (mid-slit).
+    The library's P_cvt (hard_walls.jl:49-79 + charged_walls.jl:60-81) must match the mid-slit value to
+    discretisation error, which falls ~4x when dz halves (the charge term is 1.5x the pressure here, so
+    a wrong sign or factor fails by orders of magnitude)."""
+    errs = []
+    for dz in (0.05, 0.025):
+        sp, o, r = _solve_tight(_salt_slit(30.0, dz, 0.05))
+        pc = o.contact_pressure()
+        pmid = float(np.sum(o.rho_beads[sp.NP // 2]))
+        assert pc[0] == pytest.approx(pc[1], rel=1e-12)
+        errs.append(abs(pc[0] - pmid) / pmid)
+        assert 2 * PI * sp.ms.bjerrum * 0.05 ** 2 > pmid
+    assert errs[0] < 1e-3
+    assert errs[1] < errs[0] / 3.0
