@@ -149,7 +149,9 @@ typedef struct sdft_solver_opts {
     int32_t precision;    /* SDFT_PREC_* */
     double fp32_switch;   /* mixed: switch to FP64 once ||F(x)-x||_inf < fp32_switch */
     int32_t band_fp32;    /* 1: the mixed mode's propagators (FP32 phase and FP64 polish) use the banded
-                             bond kernel; the plain FP64 mode is always dense */
+                             bond kernel, the plain FP64 mode the dense one; 2: every mode banded (long
+                             grafted chains, whose propagators span more decades than the dense or FFT
+                             operator's round-off floor resolves); 0: dense everywhere */
     int32_t check_every;  /* host polls convergence every this many iterations */
 } sdft_solver_opts_t;
 

@@ -325,12 +325,46 @@ class OracleDFT:
         self.rho_beads_hat = None
         self.rho_bonds_hat = None
         self.n_objective = 0
+        self.bond_band = None      # set by use_banded_direct_propagators
+        self.bond_taps = None
 
     # -------------------------------------------------------- convolution
     def _prop_conv(self, f: np.ndarray, bond: int) -> np.ndarray:
         """mirror -> FFT -> filter -> x w_bond -> IFFT -> |Re| on the primary half."""
+        if self.bond_band is not None:
+            return self._prop_conv_banded(f, bond)
         g = np.fft.ifft(np.fft.fft(mirror(f, self.L)) * self.wbond_filt[:, bond])
         return np.abs(g.real[:self.NP])
+
+    def use_banded_direct_propagators(self, rel_cut: float = 1e-9):
+        """Evaluate the propagator convolutions as direct banded sums instead of through the FFT
+        (a deliberate deviation, off by default). The filtered bond kernel's periodic IDFT c is kept
+        for |d| <= Be, where B is the last offset with |c| > rel_cut * max|c| over the bond types and
+        Be = B rounded up to 8 mod 16 (the band the library's banded operator applies), and
+        y[n] = sum_{|j| <= Be} c[j] x_ext[n - j] on the mirror-extended input.
+
+        Why: an FFT (or any dense) convolution carries an absolute round-off floor of ~1e-16 of the
+        largest input value. For long grafted chains in a brush the propagators of one segment span
+        more than 16 decades across the slit, and the product with the other sweep's propagator (up to
+        ~1e18 larger in the dilute region) lifts that floor into the segment densities: the reference's
+        FFT evaluation is then round-off dominated (grafted segment integrals are no longer sigma_g).
+        The banded direct sum has no floor (each output is a short sum of non-negative terms), and the
+        dropped tail is below 1e-9 of the kernel's peak."""
+        L = self.L
+        d = np.arange(L)
+        dd = np.minimum(d, L - d)
+        cs = [np.fft.ifft(self.wbond_filt[:, u]).real for u in range(self.ms.MB)]
+        B = max(int(dd[np.abs(c) > rel_cut * np.abs(c).max()].max()) for c in cs)
+        Be = (B + 7) // 16 * 16 + 8
+        if Be >= self.NP:
+            raise ValueError("band wider than the grid")
+        self.bond_band = Be
+        self.bond_taps = [np.concatenate([c[L - Be:], c[:Be + 1]]) for c in cs]   # c[-Be..Be]
+
+    def _prop_conv_banded(self, f: np.ndarray, bond: int) -> np.ndarray:
+        Be, NP = self.bond_band, self.NP
+        ext = np.concatenate([f[Be - 1::-1], f, f[NP - 1:NP - 1 - Be:-1]])          # mirror-extended
+        return np.abs(np.convolve(ext, self.bond_taps[bond], mode="valid"))
 
     # -------------------------------------------------------- densities
     def _species_density(self, u, lam, lng, rho_seg_u, rho_bonds):

@@ -1270,8 +1270,13 @@ extern "C" int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* o, doub
         TRY(dalloc(ctx, &c.d_Gh, c.total_n * o->anderm));
         c.m = o->anderm;
     }
-    c.band_fp32 = o->band_fp32;
-    c.band_fp64 = (prec == SDFT_PREC_MIXED && o->band_fp32) ? 1 : 0;  // the plain FP64 path stays dense
+    if (o->band_fp32 < 0 || o->band_fp32 > 2) {
+        set_error(ctx, SDFT_E_ARG, "band_fp32 must be 0, 1 or 2");
+        return SDFT_E_ARG;
+    }
+    c.band_fp32 = o->band_fp32 ? 1 : 0;
+    // the plain FP64 path stays dense unless every precision is asked to run banded (2)
+    c.band_fp64 = ((prec == SDFT_PREC_MIXED && o->band_fp32) || o->band_fp32 == 2) ? 1 : 0;
     SolveParams sp;
     sp.tole = o->tole; sp.damping = o->damping; sp.mixing_max = o->mixing_max; sp.switch_tole = o->switch_tole;
     sp.fp32_switch = o->fp32_switch; sp.m = o->anderm; sp.max_iters = o->max_iters;

@@ -4,8 +4,11 @@ Tolerances (relative to max |oracle|):
   * FP64 objective F(x):                1e-11  (same arithmetic, different summation order)
   * FP32 objective F(x):                5e-5   (single-precision convolutions and propagators)
   * converged densities, FP64 and mixed: 1e-8 relative (max norm per field family), both sides solved
-                                        to ||F(x)-x||_inf < 1e-11 so the comparison resolves the bound
-  * converged Omega, FP64 and mixed:    1e-10 relative to max(1, |Omega|)
+                                        to ||F(x)-x||_inf < 1e-12 so the comparison resolves the bound
+  * Omega at the same state (objective at x, then observables): 1e-12 relative
+  * converged Omega, FP64 and mixed:    1e-10 relative to max(1, |Omega|), plus the width of the
+                                        reference's fixed-point family where the neutrality shift's
+                                        1e-10 charge tolerance leaves Psi_C a band (omega_band)
   * degree of ionization / CVT:         1e-9 absolute
 """
 import numpy as np
@@ -14,6 +17,7 @@ import pytest
 from softdft_amd import abi, configs
 from softdft_amd.problem import build_state_point
 from oracle.dft_oracle import OracleDFT, anderson_solve
+from parity_helpers import hip_free_memory, omega_band
 
 pytestmark = pytest.mark.gpu
 
@@ -73,7 +77,7 @@ def test_objective_fp32_is_close(name):
     b.close()
 
 
-TIGHT = 1e-11
+TIGHT = 1e-12
 
 
 def _rel_maxdiff(a, ref):
@@ -83,9 +87,11 @@ def _rel_maxdiff(a, ref):
 @pytest.mark.parametrize("name", ["chains", "weak_pe", "weak_pe_charged_walls", "brush", "square_well",
                                   "square_well_monomers"])
 def test_solve_matches_oracle(name):
-    """Both sides solved to ||F(x)-x||_inf < 1e-11, three decades below the 1e-8 bound being checked:
-    converged bead and bond densities within 1e-8 relative, Omega within 1e-10 relative (plain FP64
-    and mixed precision), degree of ionization and contact pressure within 1e-9."""
+    """Both sides solved to ||F(x)-x||_inf < 1e-12, four decades below the 1e-8 bound being checked:
+    converged bead and bond densities within 1e-8 relative, Omega within 1e-10 relative (plain FP64 and
+    mixed precision) plus the width of the reference's own fixed-point family (the neutrality shift's
+    1e-10 charge tolerance leaves Psi_C free in a narrow band: ~1e-10 in Omega for the brush, measured
+    by omega_band), degree of ionization and contact pressure within 1e-9."""
     sp = build_state_point(SYSTEMS[name])
     b = abi.Batch([sp])
     o = OracleDFT(sp)
@@ -106,7 +112,8 @@ def test_solve_matches_oracle(name):
     assert np.allclose(rb, rb_ref, rtol=1e-8, atol=1e-12 * float(np.max(rb_ref)))
     if sp.MB:
         assert _rel_maxdiff(rbo, rbo_ref) < 1e-8
-    assert abs(om[0] - om_ref) < 1e-10 * max(1.0, abs(om_ref))
+    band = omega_band(sp, o, r, om_ref, num)
+    assert abs(om[0] - om_ref) < 1e-10 * max(1.0, abs(om_ref)) + band
     fr = o.state_fractions()
     for u, f in enumerate(fr):
         if f is not None:
@@ -118,24 +125,39 @@ def test_solve_matches_oracle(name):
     rb2 = b.get_field(abi.FIELD_RHO_BEADS, 0)
     assert err2[0] < TIGHT
     assert _rel_maxdiff(rb2, rb_ref) < 1e-8
-    assert abs(om2[0] - om_ref) < 1e-10 * max(1.0, abs(om_ref))
+    assert abs(om2[0] - om_ref) < 1e-10 * max(1.0, abs(om_ref)) + band
+    b.close()
+
+
+@pytest.mark.parametrize("name", ["weak_pe_charged_walls", "brush", "square_well", "mfmt_msa"])
+def test_omega_at_the_same_state_matches_oracle(name):
+    """grand_potential (grand_potential.jl:4-28) with rho from the objective at x and mu_ex, lng from x."""
+    sp = build_state_point(SYSTEMS[name])
+    b = abi.Batch([sp])
+    o = OracleDFT(sp)
+    rng = np.random.default_rng(5)
+    x = b.x0() + 0.01 * rng.standard_normal(sp.n)
+    print(f"case omega same state {name}", flush=True)
+    b.objective(x, abi.PREC_FP64)
+    om, al, pc = b.observables()
+    o.objective(x)
+    om_ref = o.grand_potential(x)
+    assert abs(om[0] - om_ref) < 1e-12 * max(1.0, abs(om_ref))
+    assert np.allclose(pc[0], o.contact_pressure(), rtol=1e-12, atol=1e-14)
     b.close()
 
 
 def test_observables_reuse_their_buffers():
     """sdft_observables allocates its work buffers once per context: 100 calls leave the device's
     free memory unchanged."""
-    import torch
     sp = build_state_point(SYSTEMS["weak_pe_charged_walls"])
     b = abi.Batch([sp, sp])
     b.objective(b.x0(), abi.PREC_FP64)
     first = b.observables()
-    torch.cuda.synchronize()
-    free0, _ = torch.cuda.mem_get_info()
+    free0 = hip_free_memory()
     for _ in range(100):
         om, al, pc = b.observables()
-    torch.cuda.synchronize()
-    free1, _ = torch.cuda.mem_get_info()
+    free1 = hip_free_memory()
     assert free1 == free0
     assert np.array_equal(om, first[0]) and np.array_equal(pc, first[2])
     assert np.array_equal(al, first[1], equal_nan=True)

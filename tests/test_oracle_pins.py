@@ -666,3 +666,76 @@ def test_contact_theorem_with_charged_walls_ideal_electrolyte():
         assert 2 * PI * sp.ms.bjerrum * 0.05 ** 2 > pmid
     assert errs[0] < 1e-3
     assert errs[1] < errs[0] / 3.0
+
+
+def test_neutrality_tolerance_leaves_a_band_of_fixed_points():
+    """compute_neutrality_shift returns no shift once |Q + sum N q| <= 1e-10 (neutrality_shift.jl:60-69),
+    so a converged state moved in Psi_C stays converged at a different Psi_C: every state whose charge
+    imbalance lies inside the tolerance is a fixed point. Psi_C is therefore defined only to about
+    2e-10 / |f'|, which the converged-Omega parity bounds account for."""
+    sp, o, r = _solve_tight(_salt_slit(12.0, 0.05, 0.03), tole=1e-13)
+    Q, N, q = o.neutrality_inputs
+    band = 2e-10 / float(np.sum(N * q * q))
+    x1 = r.x.copy()
+    x1[-1] += 3.0 * band
+    o1 = od.OracleDFT(sp)
+    num = dict(sp.numerics)
+    num["tole"] = 1e-13
+    r1 = od.anderson_solve(o1.objective, x1, num, max_iters=20000)
+    Q1, N1, q1 = o1.neutrality_inputs
+    moved = r1.x[-1] - r.x[-1]
+    assert abs(Q1 + float(np.sum(N1 * q1))) <= 1e-10
+    assert 1e-13 < abs(moved) <= band * 1.01
+
+
+# ----------------------------------------------------------------- banded direct propagators
+def test_banded_direct_propagators_equal_the_fft_of_the_truncated_kernel():
+    """On a well-conditioned state the direct banded sum equals the FFT convolution with the same
+    truncated kernel (the operator is the same; only the evaluation differs)."""
+    sp = build_state_point(configs.weak_polyelectrolyte(H=6.0, dz=0.05, n_seg=6, solvent=0.2, pH=5.0))
+    x = sp.x0() + 0.03 * np.random.default_rng(0).standard_normal(sp.n)
+    o1 = od.OracleDFT(sp)
+    L = o1.L
+    d = np.arange(L)
+    dd = np.minimum(d, L - d)
+    o2 = od.OracleDFT(sp)
+    o2.use_banded_direct_propagators()
+    for u in range(sp.MB):
+        c = np.fft.ifft(o1.wbond_filt[:, u]).real
+        c[dd > o2.bond_band] = 0.0
+        o1.wbond_filt[:, u] = np.fft.fft(c).real
+    a, b = o1.objective(x), o2.objective(x)
+    assert np.max(np.abs(a - b)) < 1e-13 * np.max(np.abs(a))
+
+
+def _brush500_state():
+    ds = configs.weak_polyelectrolyte(H=81.9149, dz=0.05, n_seg=500, pKa=5.0, pH=3.0, salt_M=0.001, grafted=0.05,
+                                      numerics=configs.SCF_NUMERICS_BRUSH)
+    sp = build_state_point(ds)
+    x = np.load(__file__.rsplit("/", 1)[0] + "/golden/brush500_dz0.05_pH3_1mM_x.npy")
+    assert x.shape == (sp.n,)
+    return sp, x
+
+
+def _grafted_segment_integrals(sp, o):
+    fam = sp.ms.species[0].state_family
+    return np.array([sum(float(np.sum(o.rho_seg[0][:, a, q] * sp.trapez[:, si])) * sp.dz for a, si in enumerate(fam[q]))
+                     for q in range(1, sp.ms.species[0].n_seg)])
+
+
+def test_fft_propagation_is_round_off_dominated_in_a_500mer_brush():
+    """At the converged 500-mer brush (tests/golden/brush500_dz0.05_pH3_1mM_x.npy) one segment's
+    propagators span 17 decades across the slit. Every free segment of a grafted chain must integrate
+    to the grafting density (sum_K rho trapez dz = 0.05). The banded direct evaluation keeps that to
+    the solve's residual, and the state is its fixed point. The FFT evaluation (the reference's) misses
+    it by percents: its ~1e-16 relative round-off floor, multiplied by the other sweep's up to ~1e18
+    larger weights in the dilute region, dominates the densities."""
+    sp, x = _brush500_state()
+    ob = od.OracleDFT(sp)
+    ob.use_banded_direct_propagators()
+    fb = ob.objective(x)
+    assert np.max(np.abs(fb - x)) < 1e-7
+    assert np.max(np.abs(_grafted_segment_integrals(sp, ob) - 0.05)) < 1e-7
+    of = od.OracleDFT(sp)
+    of.objective(x)
+    assert np.max(np.abs(_grafted_segment_integrals(sp, of) - 0.05)) > 1e-2
