@@ -175,6 +175,16 @@ int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* opts, double* x_in
 /* after sdft_solve: Omega[P], alpha[P][n_species*S_max] (NaN where undefined), p_cvt[P][2] */
 int32_t sdft_observables(sdft_ctx_t* ctx, double* omega, double* alpha, double* p_cvt);
 
+/* sdft_observables without any device allocation. No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). Same results as sdft_observables (the same kernels, in the same order), with every
+ * partial sum in a static device array of 2^20 doubles (8 MiB per GPU, held until the process ends)
+ * instead of the context's lazily allocated work buffer. Limit: the batch must need at most 2^20
+ * doubles, P * (11 + n_species * S_max) + P * sum over multistate species of N_u * S_u; larger
+ * batches return SDFT_E_ARG. Calls that overlap on one GPU (any contexts, any host threads) share
+ * that array and race. */
+int32_t sdft_observables_noalloc(sdft_ctx_t* ctx, double* omega, double* alpha, double* p_cvt);
+
 /* copy one field of one point to host (layouts above) */
 int32_t sdft_get_field(sdft_ctx_t* ctx, int32_t field_id, int32_t point, double* host_out, int64_t capacity);
 

@@ -30,7 +30,7 @@ FIELD_RHO_SEGMENTS = 16  # + species index: rho_K.segments[u], (NP, S_u, N_u)
 EXPORTED = ("sdft_abi_version", "sdft_last_error", "sdft_create", "sdft_destroy", "sdft_solution_length",
             "sdft_solution_offsets", "sdft_objective", "sdft_solve", "sdft_observables", "sdft_get_field",
             "sdft_last_timing", "sdft_last_pipeline_iters", "sdft_set_operator_band", "sdft_kernel_profile",
-            "sdft_set_launch_mode", "sdft_autotune", "sdft_set_tuning")
+            "sdft_set_launch_mode", "sdft_autotune", "sdft_set_tuning", "sdft_observables_noalloc")
 TUNE_PROPAGATOR, TUNE_CONV_FWD, TUNE_CONV_PRE, TUNE_CONV_BACK = 1, 2, 3, 4
 CONV_TALL, CONV_WIDE64, CONV_WIDE128, CONV_MID, CONV_NARROW = 0, 1, 2, 3, 4
 KP_PROP_FP32, KP_PROP_FP64, KP_CONV_FWD_FP32, KP_CONV_FWD_FP64, KP_CONV_BACK_FP32, KP_CONV_BACK_FP64, \
@@ -101,6 +101,8 @@ def load_library(path: str = LIB_PATH):
     lib.sdft_solve.argtypes = [C.c_void_p, C.POINTER(SdftSolverOpts), _f64p, _i32p, _f64p]
     lib.sdft_observables.restype = C.c_int32
     lib.sdft_observables.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
+    lib.sdft_observables_noalloc.restype = C.c_int32
+    lib.sdft_observables_noalloc.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
     lib.sdft_get_field.restype = C.c_int32
     lib.sdft_get_field.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _f64p, C.c_int64]
     lib.sdft_last_timing.restype = C.c_int32
@@ -387,6 +389,17 @@ class Batch:
         al = np.zeros(P * self.nsp * self.S_max)
         pc = np.zeros(P * 2)
         self._check(self.lib.sdft_observables(self.h, _ptr(om, C.c_double), _ptr(al, C.c_double), _ptr(pc, C.c_double)))
+        return om, al.reshape(P, self.nsp, self.S_max), pc.reshape(P, 2)
+
+    def observables_noalloc(self):
+        """observables() through sdft_observables_noalloc: no device allocation, partial sums in the
+        library's static per-GPU scratch array."""
+        P = len(self.points)
+        om = np.zeros(P)
+        al = np.zeros(P * self.nsp * self.S_max)
+        pc = np.zeros(P * 2)
+        self._check(self.lib.sdft_observables_noalloc(self.h, _ptr(om, C.c_double), _ptr(al, C.c_double),
+                                                      _ptr(pc, C.c_double)))
         return om, al.reshape(P, self.nsp, self.S_max), pc.reshape(P, 2)
 
     def get_field(self, field_id: int, point: int) -> np.ndarray:

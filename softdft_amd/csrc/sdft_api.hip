@@ -8,6 +8,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <map>
+#include <mutex>
 #include <tuple>
 
 #include "sdft_launch.h"
@@ -43,9 +44,9 @@ struct sdft_ctx {
     std::vector<double> valences;
     double lB = 0.0;
     int fwd_terms = 0, back_terms = 0;   // convolution terms per point (algorithmic flop count)
-    double* d_om_blocks = nullptr;       // per-point Omega block partials, allocated on first use
+    double* d_obs = nullptr;             // sdft_observables' work buffer (obs_layout), allocated on first use
     double* d_profile_fx = nullptr;      // sdft_kernel_profile's scratch F(x), allocated on first use
-    double* d_segsums = nullptr;         // [P][max_u N_u S_u] segment sums of sdft_observables, first use
+    double* d_scratch = nullptr;         // this device's copy of the static scratch array (sdft_create)
     // contact-value theorem table (built by sdft_create): per point the entries off[p]..off[p+1) of
     // (wall side, bead field offset j*NP + index or -1 for a constant term, factor); one gather kernel
     // sums them into d_cvt[P][2]
@@ -53,7 +54,6 @@ struct sdft_ctx {
     int32_t* d_cvt_idx = nullptr;
     int32_t* d_cvt_side = nullptr;
     double* d_cvt_fac = nullptr;
-    double* d_cvt = nullptr;
     bool cvt_any = false;
     // convolution plan inputs, kept to rebuild the plans with other tile shapes (sdft_autotune)
     std::vector<ConvJob> jobs_h;
@@ -124,6 +124,16 @@ struct EvalflagGuard {
     }
     ~EvalflagGuard() { (void)restore(); }
 };
+
+// Device scratch of sdft_observables_noalloc: a file-scope array of SDFT_SCRATCH_DOUBLES doubles
+// (8 MiB) that every GPU the library runs on holds until the process ends. Its name is a host-side
+// stand-in: host code uses only the per-device address from hipGetSymbolAddress, looked up once per
+// device by sdft_create with that device current. Calls on one GPU share the array, so calls that
+// overlap on one GPU race; each call rewrites every entry it reads.
+#define SDFT_SCRATCH_DOUBLES (1 << 20)
+static __device__ __align__(256) double scratch[SDFT_SCRATCH_DOUBLES];
+static std::mutex g_scratch_mu;
+static std::map<int, double*> g_scratch_ptr;   // device ordinal -> address of its copy of `scratch`
 
 static thread_local std::string g_last_error;
 
@@ -738,6 +748,22 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     }
     sdft_ctx* ctx = new sdft_ctx();
     (void)hipGetDevice(&ctx->device);
+    {
+        // this device's copy of the static scratch array, looked up once per device (device current)
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        auto it = g_scratch_ptr.find(ctx->device);
+        if (it == g_scratch_ptr.end()) {
+            void* ptr = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !ptr) {
+                set_error(ctx, SDFT_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+                delete ctx;
+                return SDFT_E_HIP;
+            }
+            it = g_scratch_ptr.emplace(ctx->device, static_cast<double*>(ptr)).first;
+        }
+        ctx->d_scratch = it->second;
+    }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         set_error(ctx, SDFT_E_HIP, "hipStreamCreate failed");
         delete ctx;
@@ -852,7 +878,6 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     if ((rc = upload(ctx, &ctx->d_cvt_idx, cvt_idx.data(), (int64_t)cvt_idx.size())) != SDFT_OK) return fail(rc);
     if ((rc = upload(ctx, &ctx->d_cvt_side, cvt_side.data(), (int64_t)cvt_side.size())) != SDFT_OK) return fail(rc);
     if ((rc = upload(ctx, &ctx->d_cvt_fac, cvt_fac.data(), (int64_t)cvt_fac.size())) != SDFT_OK) return fail(rc);
-    if ((rc = dalloc(ctx, &ctx->d_cvt, 2 * (int64_t)P)) != SDFT_OK) return fail(rc);
 
     // ---- species
     std::vector<DevSpecies> spec(nsp);
@@ -1511,78 +1536,102 @@ extern "C" int32_t sdft_get_field(sdft_ctx_t* ctx, int32_t field_id, int32_t poi
                       : download_field(ctx, ctx->p32.rho_bonds + P.base * 2 * c.MB, need, out);
 }
 
+// Work buffer of the observables, in doubles, in this order: Omega block partials [P][8], the segment
+// sums of every multistate species ([P][N_u][S_u] each), contact values [P][2], Omega [P], alpha
+// [P][n_species][S_max]. Returns its length and fills the offsets.
+struct ObsOffsets {
+    int64_t om, ss, cvt, omega, alpha, total;
+};
+static ObsOffsets obs_layout(const sdft_ctx* ctx, ObsLayout* lay) {
+    const Ctx& c = ctx->c;
+    const int64_t P = c.P;
+    ObsOffsets o;
+    o.om = 0;
+    o.ss = P * SDFT_OMEGA_BLOCKS;
+    int64_t ss = 0;
+    for (int u = 0; u < SDFT_MAX_SPECIES; ++u) lay->ss_base[u] = 0;
+    for (int u = 0; u < c.nsp; ++u) {
+        lay->ss_base[u] = ss;
+        if (ctx->multistate[u]) ss += P * ctx->nseg[u] * ctx->S[u];
+    }
+    o.cvt = o.ss + ss;
+    o.omega = o.cvt + 2 * P;
+    o.alpha = o.omega + P;
+    o.total = o.alpha + P * c.nsp * ctx->S_max;
+    return o;
+}
+
+// grand_potential / write_total_state_fractions / contact_value_theorem for every point, with
+// every partial in the device buffer `work` (obs_layout) and the results copied to the caller's
+// arrays (any of them may be NULL)
+static int observables_run(sdft_ctx* ctx, double* work, double* omega, double* alpha, double* p_cvt) {
+    Ctx& c = ctx->c;
+    const int P = c.P;
+    ObsLayout lay;
+    const ObsOffsets o = obs_layout(ctx, &lay);
+    double* d_om = work + o.om;
+    double* d_ss = work + o.ss;
+    ctx->p32.omega(ctx->stream, c, c.d_x, d_om);
+    ctx->p64.omega(ctx->stream, c, c.d_x, d_om);
+    for (int u = 0; u < c.nsp; ++u) {
+        if (!ctx->multistate[u]) continue;
+        const int64_t stride = (int64_t)ctx->nseg[u] * ctx->S[u];
+        ctx->p32.segment_sums(ctx->stream, c, u, d_ss + lay.ss_base[u], stride);
+        ctx->p64.segment_sums(ctx->stream, c, u, d_ss + lay.ss_base[u], stride);
+    }
+    // contact values of every point in one gather, read from the pipeline the point was evaluated in
+    if (ctx->cvt_any)
+        launch_cvt_gather(ctx->stream, c, ctx->d_cvt_off, ctx->d_cvt_idx, ctx->d_cvt_side, ctx->d_cvt_fac,
+                          ctx->p32.rho_beads, ctx->p64.rho_beads, work + o.cvt);
+    else
+        launch_zero_f64(ctx->stream, work + o.cvt, 2 * (int64_t)P);
+    launch_obs_finalize(ctx->stream, c, lay, ctx->S_max, d_om, d_ss, work + o.omega, work + o.alpha);
+    HIP_CHECK_RET(hipGetLastError());
+    if (omega)
+        HIP_CHECK_RET(hipMemcpyAsync(omega, work + o.omega, (size_t)P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    if (alpha)
+        HIP_CHECK_RET(hipMemcpyAsync(alpha, work + o.alpha, (size_t)P * c.nsp * ctx->S_max * sizeof(double),
+                                     hipMemcpyDeviceToHost, ctx->stream));
+    if (p_cvt)
+        HIP_CHECK_RET(hipMemcpyAsync(p_cvt, work + o.cvt, 2 * (size_t)P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    return SDFT_OK;
+}
+
 extern "C" int32_t sdft_observables(sdft_ctx_t* ctx, double* omega, double* alpha, double* p_cvt) {
     if (!ctx) return SDFT_E_ARG;
     if (!ctx->have_eval) {
         set_error(ctx, SDFT_E_STATE, "observables need a prior sdft_solve or sdft_objective");
         return SDFT_E_STATE;
     }
-    Ctx& c = ctx->c;
-    const int P = c.P;
-    // work buffers are allocated once per context and reused by every later call
-    if (!ctx->d_om_blocks) TRY(dalloc(ctx, &ctx->d_om_blocks, (int64_t)P * SDFT_OMEGA_BLOCKS));
-    int64_t ss_stride = 0;
-    for (int u = 0; u < c.nsp; ++u)
-        if (ctx->multistate[u]) ss_stride = std::max<int64_t>(ss_stride, (int64_t)ctx->nseg[u] * ctx->S[u]);
-    if (alpha && ss_stride > 0 && !ctx->d_segsums) TRY(dalloc(ctx, &ctx->d_segsums, (int64_t)P * ss_stride));
-    double* d_om = ctx->d_om_blocks;
-    ctx->p32.omega(ctx->stream, c, c.d_x, d_om);
-    ctx->p64.omega(ctx->stream, c, c.d_x, d_om);
-    // contact values of every point in one gather, read from the pipeline the point was evaluated in
-    if (p_cvt && ctx->cvt_any)
-        launch_cvt_gather(ctx->stream, c, ctx->d_cvt_off, ctx->d_cvt_idx, ctx->d_cvt_side, ctx->d_cvt_fac,
-                          ctx->p32.rho_beads, ctx->p64.rho_beads, ctx->d_cvt);
-    HIP_CHECK_RET(hipGetLastError());
-    std::vector<double> omp((size_t)P * SDFT_OMEGA_BLOCKS), om(P, 0.0), accs((size_t)P * 2 * SDFT_MAX_SPECIES);
-    HIP_CHECK_RET(hipMemcpyAsync(omp.data(), d_om, omp.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_CHECK_RET(hipMemcpyAsync(accs.data(), c.d_acc_species, accs.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    if (p_cvt) {
-        if (ctx->cvt_any)
-            HIP_CHECK_RET(hipMemcpyAsync(p_cvt, ctx->d_cvt, 2 * (size_t)P * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        else
-            for (int i = 0; i < 2 * P; ++i) p_cvt[i] = 0.0;
+    // the work buffer is allocated on the first call and reused by every later one
+    if (!ctx->d_obs) {
+        ObsLayout lay;
+        TRY(dalloc(ctx, &ctx->d_obs, obs_layout(ctx, &lay).total));
     }
-    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
-    for (int p = 0; p < P; ++p)
-        for (int b = 0; b < SDFT_OMEGA_BLOCKS; ++b) om[p] += omp[(size_t)p * SDFT_OMEGA_BLOCKS + b];
-    if (omega) {
-        for (int p = 0; p < P; ++p) {
-            double Nsum = 0.0;
-            for (int u = 0; u < c.nsp; ++u) Nsum += accs[(size_t)p * 2 * SDFT_MAX_SPECIES + 2 * u];
-            omega[p] = om[p] * ctx->pts[p].dz - Nsum;
-        }
+    return observables_run(ctx, ctx->d_obs, omega, alpha, p_cvt);
+}
+
+extern "C" int32_t sdft_observables_noalloc(sdft_ctx_t* ctx, double* omega, double* alpha, double* p_cvt) {
+    if (!ctx) return SDFT_E_ARG;
+    if (!ctx->have_eval) {
+        set_error(ctx, SDFT_E_STATE, "observables need a prior sdft_solve or sdft_objective");
+        return SDFT_E_STATE;
     }
-    if (alpha) {
-        const int Smax = ctx->S_max;
-        for (int64_t i = 0; i < (int64_t)P * c.nsp * Smax; ++i) alpha[i] = NAN;
-        for (int u = 0; u < c.nsp; ++u) {
-            if (!ctx->multistate[u]) continue;
-            const int N = ctx->nseg[u], S = ctx->S[u];
-            double* d_ss = ctx->d_segsums;
-            ctx->p32.segment_sums(ctx->stream, c, u, d_ss, (int64_t)N * S);
-            ctx->p64.segment_sums(ctx->stream, c, u, d_ss, (int64_t)N * S);
-            std::vector<double> ss((size_t)P * N * S);
-            HIP_CHECK_RET(hipMemcpyAsync(ss.data(), d_ss, ss.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-            HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
-            for (int p = 0; p < P; ++p) {
-                std::vector<double> acc(S, 0.0), cnt(S, 0.0);
-                for (int q = 0; q < N; ++q) {
-                    int ns = 0;
-                    for (int a = 0; a < S; ++a) if (ctx->seg_states_h[ctx->state_off[u] + q * S + a] >= 0) ++ns;
-                    if (ns == 1) continue;
-                    double tot = 0.0;
-                    for (int a = 0; a < ns; ++a) tot += ss[(size_t)p * N * S + q * S + a];
-                    for (int a = 0; a < ns; ++a) {
-                        acc[a] += tot > 0 ? ss[(size_t)p * N * S + q * S + a] / tot : 0.0;
-                        cnt[a] += 1.0;
-                    }
-                }
-                for (int a = 0; a < S; ++a)
-                    alpha[((int64_t)p * c.nsp + u) * Smax + a] = cnt[a] > 0 ? acc[a] / cnt[a] : NAN;
-            }
-        }
+    if (!ctx->d_scratch) {
+        set_error(ctx, SDFT_E_STATE, "the static scratch buffer was not resolved for this device");
+        return SDFT_E_STATE;
     }
-    return SDFT_OK;
+    ObsLayout lay;
+    const int64_t need = obs_layout(ctx, &lay).total;
+    if (need > SDFT_SCRATCH_DOUBLES) {
+        set_error(ctx, SDFT_E_ARG,
+                  "sdft_observables_noalloc: this batch needs %lld doubles of scratch, the static buffer holds %d "
+                  "(P * (11 + n_species * S_max) + P * sum over multistate species of N_u * S_u)",
+                  (long long)need, SDFT_SCRATCH_DOUBLES);
+        return SDFT_E_ARG;
+    }
+    return observables_run(ctx, ctx->d_scratch, omega, alpha, p_cvt);
 }
 
 // HIP-event time of fn() on the context's stream: one warm-up call, then reps calls; us per call

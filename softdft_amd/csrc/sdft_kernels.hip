@@ -1654,6 +1654,50 @@ __global__ void k_cvt_gather(const DevPoint* __restrict__ pts, const DevSolveSta
     out[2 * p + 1] = s[1];
 }
 
+// Observables of every point from the partials of k_omega, k_neutrality and k_segment_sums
+// (grand_potential.jl:4-28 with num_species.jl:1-38; degree_of_dissociation.jl:1-78): Omega =
+// dz * (sum of the point's block partials) - sum_u N_u; per multistate species, the state fractions
+// of each segment with more than one state, averaged over those segments (NaN where undefined).
+// One thread per point; the sums run in the same order as a host loop over blocks, species,
+// segments and states.
+__global__ void k_obs_finalize(const DevPoint* __restrict__ pts, const DevSpecies* __restrict__ spec,
+                               const int32_t* __restrict__ seg_states, ObsLayout lay, int nsp, int Smax,
+                               int P_count, const double* __restrict__ om_blocks,
+                               const double* __restrict__ acc_species, const double* __restrict__ segsums,
+                               double* __restrict__ omega, double* __restrict__ alpha) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P_count) return;
+    const double qnan = __longlong_as_double(0x7ff8000000000000LL);
+    double om = 0.0;
+    for (int b = 0; b < SDFT_OMEGA_BLOCKS; ++b) om += om_blocks[(int64_t)p * SDFT_OMEGA_BLOCKS + b];
+    double Nsum = 0.0;
+    for (int u = 0; u < nsp; ++u) Nsum += acc_species[(int64_t)p * 2 * SDFT_MAX_SPECIES + 2 * u];
+    omega[p] = om * pts[p].dz - Nsum;
+    for (int u = 0; u < nsp; ++u) {
+        double* al = alpha + ((int64_t)p * nsp + u) * Smax;
+        for (int a = 0; a < Smax; ++a) al[a] = qnan;
+        const DevSpecies sp = spec[u];
+        if (!sp.multistate) continue;
+        const int N = sp.nseg, S = sp.S;
+        const double* ss = segsums + lay.ss_base[u] + (int64_t)p * N * S;
+        const int32_t* st = seg_states + sp.state_off;
+        double acc[SDFT_MAX_S], cnt[SDFT_MAX_S];
+        for (int a = 0; a < S; ++a) acc[a] = cnt[a] = 0.0;
+        for (int q = 0; q < N; ++q) {
+            int ns = 0;
+            for (int a = 0; a < S; ++a) if (st[q * S + a] >= 0) ++ns;
+            if (ns == 1) continue;
+            double tot = 0.0;
+            for (int a = 0; a < ns; ++a) tot += ss[q * S + a];
+            for (int a = 0; a < ns; ++a) {
+                acc[a] += tot > 0 ? ss[q * S + a] / tot : 0.0;
+                cnt[a] += 1.0;
+            }
+        }
+        for (int a = 0; a < S; ++a) al[a] = cnt[a] > 0 ? acc[a] / cnt[a] : qnan;
+    }
+}
+
 __global__ void k_set_evalflag(int32_t* evalflag, int value, int P_count) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p < P_count) evalflag[p] = value;
@@ -1926,6 +1970,12 @@ void launch_cvt_gather(hipStream_t st, const Ctx& c, const int32_t* off, const i
                        const double* fac, const float* rb32, const double* rb64, double* out) {
     hipLaunchKernelGGL(k_cvt_gather, dim3((c.P + 63) / 64), dim3(64), 0, st, c.d_pts, c.d_state, off, idx, side, fac,
                        rb32, rb64, c.NB, c.P, out);
+}
+
+void launch_obs_finalize(hipStream_t st, const Ctx& c, const ObsLayout& lay, int Smax, const double* om_blocks,
+                         const double* segsums, double* omega, double* alpha) {
+    hipLaunchKernelGGL(k_obs_finalize, dim3((c.P + 63) / 64), dim3(64), 0, st, c.d_pts, c.d_spec, c.d_seg_states,
+                       lay, c.nsp, Smax, c.P, om_blocks, c.d_acc_species, segsums, omega, alpha);
 }
 
 void launch_set_evalflag(hipStream_t st, int32_t* evalflag, int value, int P) {

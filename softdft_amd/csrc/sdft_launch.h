@@ -249,6 +249,12 @@ void launch_build_kernels(hipStream_t st, const DevSystem* sys, int K, int nsets
 void launch_anderson(hipStream_t st, const Ctx& c, const SolveParams& sp);
 hipError_t prepare_kernel_attributes(const Ctx& c);
 void launch_apply_switch(hipStream_t st, const Ctx& c);
+// per-species offsets (in doubles) of the segment-sum blocks inside an observables work buffer
+struct ObsLayout {
+    int64_t ss_base[SDFT_MAX_SPECIES];
+};
+void launch_obs_finalize(hipStream_t st, const Ctx& c, const ObsLayout& lay, int Smax, const double* om_blocks,
+                         const double* segsums, double* omega, double* alpha);
 void launch_cvt_gather(hipStream_t st, const Ctx& c, const int32_t* off, const int32_t* idx, const int32_t* side,
                        const double* fac, const float* rb32, const double* rb64, double* out);
 void launch_set_evalflag(hipStream_t st, int32_t* evalflag, int value, int P);
