@@ -30,6 +30,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def copy_probe(mib):
     """A plain device-to-device copy of a known size: rocprofv3's FETCH_SIZE / WRITE_SIZE for it
     give the counter-to-bytes factor applied to the kernels' readings."""
@@ -105,9 +116,26 @@ def _oracle_eval_time(sp, x, reps=2):
 
 
 def _oracle_solve(sp):
-    """One full FP64 oracle solve (CPU baseline worker); returns the iteration count."""
+    """One full FP64 oracle solve (CPU baseline worker): iterations, Omega, bead densities and the
+    chain species' state fractions of the converged point (compared with the GPU afterwards, off the
+    clock)."""
     from oracle.dft_oracle import OracleDFT, anderson_solve
-    return anderson_solve(OracleDFT(sp).objective, sp.x0(), sp.numerics).iters
+    o = OracleDFT(sp)
+    r = anderson_solve(o.objective, sp.x0(), sp.numerics)
+    return {"iters": r.iters, "omega": o.grand_potential(r.x), "rho": o.rho_beads.copy(),
+            "alpha": o.state_fractions()}
+
+
+def b_iter(sp, w, m=5):
+    """Algorithmic HBM bytes of one SCF iteration of one state point: x in, F(x) out and the Anderson
+    history (FP64 always), one propagator sweep stored and re-read, segment densities written and
+    reduced, bead and bond densities written and read, Ext and trapez read; w = 4 (FP32 phase) or 8
+    (FP64). The propagator count follows child_propagators.jl (S^2 (N - 1) per chain species)."""
+    NP, NB, MB = sp.NP, sp.NB, sp.MB
+    chains = sum(max(len(f) for f in s.state_family) ** 2 * (len(s.sequence) - 1) for s in sp.ms.species)
+    segs = sum(max(len(f) for f in s.state_family) * len(s.sequence) for s in sp.ms.species)
+    return (8 * sp.n * (3 * m + 5) + 2 * w * NP * chains + 2 * w * NP * segs + 2 * w * NP * (NB + 2 * MB)
+            + w * NP * 2 * NB)
 
 
 def _cpu_warm(k):
@@ -119,7 +147,7 @@ def _cpu_warm(k):
 
 def cpu_points_parallel(points, cores):
     """Solve independent state points with the oracle, one spawned process per point (single-threaded
-    NumPy each); returns (total iterations, wall seconds)."""
+    NumPy each); returns (per-point results of _oracle_solve, wall seconds)."""
     import multiprocessing as mp
     from concurrent.futures import ProcessPoolExecutor
     old = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
@@ -129,7 +157,7 @@ def cpu_points_parallel(points, cores):
         with ProcessPoolExecutor(max_workers=cores, mp_context=mp.get_context("spawn")) as ex:
             list(ex.map(_cpu_warm, range(cores)))  # every worker started (and numpy imported) before the clock
             t0 = time.perf_counter()
-            its = list(ex.map(_oracle_solve, points))
+            res = list(ex.map(_oracle_solve, points))
             dt = time.perf_counter() - t0
     finally:
         for k, v in old.items():
@@ -137,7 +165,7 @@ def cpu_points_parallel(points, cores):
                 os.environ.pop(k, None)
             else:
                 os.environ[k] = v
-    return int(sum(its)), dt
+    return res, dt
 
 
 def report_single(name, ds, cpu_full, args):
@@ -391,6 +419,16 @@ def main():
     value = args.points * args.steps / total
     it32, it64 = batch.pipeline_iters()
     log(f"batch iterations with the FP32 / FP64 pipeline running: {it32} / {it64}")
+    # path-level HBM view: algorithmic bytes of every point's FP32 and FP64 iterations of one solve
+    sw = batch.switch_iters()
+    path_bytes = float(sum(int(sw[p]) * b_iter(sps[p], 4) + int(it[p] - sw[p]) * b_iter(sps[p], 8)
+                           for p in range(args.points)))
+    step_s = total / args.steps
+    log(f"path: {path_bytes / 1e9:.1f} GB algorithmic per solve, {path_bytes / step_s / 1e12:.3f} TB/s")
+    # the mixed solution of the points the CPU baseline solves too (compared with the oracle below)
+    cores = max(1, min(args.cpu_cores, os.cpu_count() or 1))
+    cpu_idx = np.linspace(0, args.points - 1, cores).astype(int)
+    rho_mixed = {int(i): batch.get_field(abi.FIELD_RHO_BEADS, int(i)) for i in cpu_idx}
     if args.solve_only:
         log(f"solve-only: {value:.1f} state points/s over {args.steps} timed solves")
         batch.close()
@@ -444,11 +482,22 @@ def main():
         log(f"kernel {name}: {ms * 1e3:.1f} us/launch, {nbytes / ms / 1e9:.2f} TB/s of {nbytes / 1e6:.0f} MB, "
             f"x {iters_run} iterations = {ms * iters_run:.1f} ms per solve")
 
-    # FP64 reference for the acceptance bound (untimed)
+    # plain FP64 on the same batch (timed, one solve each): the dense bond operator (the reference's
+    # full operator) and the banded one, so the mixed-precision gain is shown apart from batching;
+    # the dense solve is also the reference of the acceptance bound
+    t0 = time.perf_counter()
     xd, itd, errd = batch.solve(x0, precision=abi.PREC_FP64, **num)
+    t_fp64 = time.perf_counter() - t0
     om_fp64, al_fp64, _ = batch.observables()
+    t0 = time.perf_counter()
+    xb, itb, errb = batch.solve(x0, precision=abi.PREC_FP64, band_fp32=2, **num)
+    t_fp64b = time.perf_counter() - t0
+    om_fp64b, _, _ = batch.observables()
     rel = np.abs(om_mixed - om_fp64) / np.abs(om_fp64)
+    rel_b = float(np.max(np.abs(om_fp64b - om_fp64) / np.abs(om_fp64)))
     dal = float(np.nanmax(np.abs(al_mixed[:, 0, 1] - al_fp64[:, 0, 1])))  # chain species' charged fraction
+    log(f"batched FP64: dense {t_fp64 * 1e3:.1f} ms ({args.points / t_fp64:.1f} points/s, mean {itd.mean():.1f} it), "
+        f"banded {t_fp64b * 1e3:.1f} ms ({args.points / t_fp64b:.1f} points/s); banded vs dense dOmega {rel_b:.2e}")
     log(f"max rel dOmega mixed vs FP64: {rel.max():.3e}, max |d alpha|: {dal:.3e}")
     batch.close()
 
@@ -469,7 +518,7 @@ def main():
     single_rate = len(sel) / (time.perf_counter() - t0)
     log(f"FP64 one-at-a-time: {single_rate:.2f} points/s")
 
-    cpu = cpu1 = None
+    cpu = cpu1 = vs_oracle = None
     if not args.no_cpu:
         from oracle.dft_oracle import OracleDFT, anderson_solve
         sp = sps[int(np.argmin(np.abs(np.linspace(2, 12, args.points) - 7.0)))]
@@ -482,14 +531,30 @@ def main():
                           f"algorithm, {r.iters} Anderson iterations in {dt:.1f} s on one host core"}
         log(f"cpu baseline: {dt:.1f} s for one point on one core")
         # all host cores: one process per state point (points are independent), a bounded sample
-        cores = max(1, min(args.cpu_cores, os.cpu_count() or 1))
-        idx = np.linspace(0, args.points - 1, cores).astype(int)
-        n_it, dt_all = cpu_points_parallel([sps[i] for i in idx], cores)
+        idx = cpu_idx
+        res, dt_all = cpu_points_parallel([sps[i] for i in idx], cores)
+        n_it = sum(r["iters"] for r in res)
         cpu = {"value": len(idx) / dt_all, "unit": "state points/s", "cores": cores, "kind": "port",
                "sample": f"{len(idx)} of the {args.points} state points spread over pH 2..12, one process per point "
                          f"on {cores} host cores, FP64 NumPy restatement of the reference algorithm "
                          f"({n_it} Anderson iterations in total, {dt_all:.1f} s wall)"}
         log(f"cpu baseline: {len(idx)} points on {cores} cores in {dt_all:.1f} s")
+        # the GPU mixed batch against the oracle's converged FP64 solutions of those points (both solved
+        # to 1e-8): Omega, bead densities, excess adsorption, degree of ionization of the chain
+        e_om, e_rho, e_ads, e_al = [], [], [], []
+        for i, r in zip(idx, res):
+            i = int(i)
+            e_om.append(abs(om_mixed[i] - r["omega"]) / abs(r["omega"]))
+            e_rho.append(float(np.max(np.abs(rho_mixed[i] - r["rho"])) / np.max(np.abs(r["rho"]))))
+            ad_g, ad_o = _adsorption(sps[i], rho_mixed[i]), _adsorption(sps[i], r["rho"])
+            e_ads.append(float(np.max(np.abs(ad_g - ad_o)) / np.max(np.abs(ad_o))))
+            e_al.append(float(np.nanmax(np.abs(al_mixed[i, 0, :2] - r["alpha"][0][:2]))))
+        vs_oracle = {"points": len(idx), "max_rel_dOmega": max(e_om), "max_rel_drho_beads": max(e_rho),
+                     "max_rel_dadsorption": max(e_ads), "max_abs_dalpha": max(e_al),
+                     "note": "GPU mixed-precision batch vs the FP64 oracle's own solves of the same points, "
+                             "both to ||F(x)-x||_inf < 1e-8"}
+        log(f"mixed vs oracle on {len(idx)} points: dOmega {max(e_om):.2e}, drho {max(e_rho):.2e}, "
+            f"dadsorption {max(e_ads):.2e}, dalpha {max(e_al):.2e}")
 
     # roofline of the dominant kernel: the largest estimated time per mixed solve
     name, (ms, fl, prec, _) = max(prof.items(), key=lambda kv: kv[1][3])
@@ -513,13 +578,26 @@ def main():
                    "parallelism": "one GPU, batched state points"},
         "fp64_one_at_a_time": {"value": single_rate, "unit": "state points/s", "sample_points": int(len(sel))},
         "speedup_vs_fp64_one_at_a_time": value / single_rate,
+        "fp64_batched": {"dense": {"value": args.points / t_fp64, "unit": "state points/s",
+                                   "mean_iterations": float(itd.mean())},
+                         "banded": {"value": args.points / t_fp64b, "unit": "state points/s",
+                                    "mean_iterations": float(itb.mean()), "max_rel_dOmega_vs_dense": rel_b}},
+        "speedup_mixed_vs_fp64_batched": {"dense": value * t_fp64 / args.points,
+                                          "banded": value * t_fp64b / args.points},
         "max_rel_dOmega_vs_fp64": float(rel.max()),
         "max_abs_dalpha_vs_fp64": dal,
         "mean_iterations": float(np.mean([i.mean() for i in iters])),
         "roofline": {"bound": "mfma", "kernel": name, "achieved": achieved, "peak": PEAK[prec],
                      "unit": "TFLOP/s", "frac": achieved / PEAK[prec], "traffic": traffic_from_profiles(name)},
+        "roofline_hbm": {"bound": "hbm", "achieved": path_bytes / step_s / 1e9, "peak": 8000.0, "unit": "GB/s",
+                         "frac": path_bytes / step_s / 8.0e12, "bytes_per_solve": path_bytes,
+                         "note": "whole solve: algorithmic bytes of every point's FP32 and FP64 iterations "
+                                 "(B_iter per iteration and point) over the solve's wall time"},
+        "errors_vs_oracle": vs_oracle,
         "cpu_baseline": cpu,
         "cpu_baseline_one_core": cpu1,
+        "host": {"cpu_count": os.cpu_count(), "cpus_available": len(os.sched_getaffinity(0)),
+                 "cpu_model": _cpu_model()},
         "kernels_us": {k: v[0] * 1e3 for k, v in prof.items()},
         "kernels_ms_per_solve_est": {k: v[3] for k, v in prof.items()},
         "pipeline_iterations": {"fp32": it32, "fp64": it64},

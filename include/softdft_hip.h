@@ -201,6 +201,11 @@ int32_t sdft_set_operator_band(sdft_ctx_t* ctx, int32_t fp32_banded, int32_t fp6
  * (instrumentation; no counterpart in the reference) */
 int32_t sdft_last_pipeline_iters(const sdft_ctx_t* ctx, int32_t* fp32_iters, int32_t* fp64_iters);
 
+/* per point, the iterations done when the last mixed-precision sdft_solve moved it from FP32 to FP64
+ * (0: no switch, e.g. a plain FP32 or FP64 solve); with the point's total iterations this splits its
+ * work between the two precisions (instrumentation; no counterpart in the reference) */
+int32_t sdft_last_switch_iters(const sdft_ctx_t* ctx, int32_t* switch_iter);
+
 /* launch mode of sdft_solve: graph = 1 (default) runs each window of check_every iterations
  * between two host polls as one captured hipGraph (instantiated once per pair of active
  * pipelines and solve), 0 launches every kernel on the stream; concurrent = 1 (default) runs the

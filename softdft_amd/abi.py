@@ -30,7 +30,8 @@ FIELD_RHO_SEGMENTS = 16  # + species index: rho_K.segments[u], (NP, S_u, N_u)
 EXPORTED = ("sdft_abi_version", "sdft_last_error", "sdft_create", "sdft_destroy", "sdft_solution_length",
             "sdft_solution_offsets", "sdft_objective", "sdft_solve", "sdft_observables", "sdft_get_field",
             "sdft_last_timing", "sdft_last_pipeline_iters", "sdft_set_operator_band", "sdft_kernel_profile",
-            "sdft_set_launch_mode", "sdft_autotune", "sdft_set_tuning", "sdft_observables_noalloc")
+            "sdft_set_launch_mode", "sdft_autotune", "sdft_set_tuning", "sdft_observables_noalloc",
+            "sdft_last_switch_iters")
 TUNE_PROPAGATOR, TUNE_CONV_FWD, TUNE_CONV_PRE, TUNE_CONV_BACK = 1, 2, 3, 4
 CONV_TALL, CONV_WIDE64, CONV_WIDE128, CONV_MID, CONV_NARROW = 0, 1, 2, 3, 4
 KP_PROP_FP32, KP_PROP_FP64, KP_CONV_FWD_FP32, KP_CONV_FWD_FP64, KP_CONV_BACK_FP32, KP_CONV_BACK_FP64, \
@@ -101,6 +102,8 @@ def load_library(path: str = LIB_PATH):
     lib.sdft_solve.argtypes = [C.c_void_p, C.POINTER(SdftSolverOpts), _f64p, _i32p, _f64p]
     lib.sdft_observables.restype = C.c_int32
     lib.sdft_observables.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
+    lib.sdft_last_switch_iters.restype = C.c_int32
+    lib.sdft_last_switch_iters.argtypes = [C.c_void_p, _i32p]
     lib.sdft_observables_noalloc.restype = C.c_int32
     lib.sdft_observables_noalloc.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
     lib.sdft_get_field.restype = C.c_int32
@@ -367,6 +370,12 @@ class Batch:
     def set_graph_capture(self, enable: Optional[bool] = None) -> int:
         """Graph windows on / off (None: unchanged); returns the graphs built so far."""
         return self.set_launch_mode(graph=enable)
+
+    def switch_iters(self) -> np.ndarray:
+        """Per point, the iterations done at the FP32 -> FP64 switch of the last mixed solve (0: none)."""
+        out = np.zeros(len(self.points), dtype=np.int32)
+        self._check(self.lib.sdft_last_switch_iters(self.h, _ptr(out, C.c_int32)))
+        return out
 
     def pipeline_iters(self):
         """(FP32, FP64) batch iterations of the last solve during which each pipeline ran."""

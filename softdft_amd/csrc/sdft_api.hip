@@ -32,6 +32,7 @@ struct sdft_ctx {
     double last_solve_ms = 0.0;
     double last_iters_total = 0.0;
     int32_t last_iters32 = 0, last_iters64 = 0;  // batch iterations that ran each pipeline
+    std::vector<int32_t> last_switch;             // per point: iterations done at the FP32 -> FP64 switch
     int32_t obj_band32 = 1, obj_band64 = 0;       // bond operator of sdft_objective per precision
     int nsets = 0;
     int S_max = 1;
@@ -956,7 +957,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     std::vector<int64_t> set_kbase;
     std::vector<double> set_dz;
     ctx->pts.resize(P);
-    int64_t base = 0, xoff = 0, kb = 0;
+    int64_t base = 0, xoff = 0, kb = 0, gsum = 0;
     for (int p = 0; p < P; ++p) {
         DevPoint& q = ctx->pts[p];
         memset(&q, 0, sizeof(q));
@@ -966,6 +967,9 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
             return fail(SDFT_E_ARG);
         }
         q.NP = NP; q.L = 2 * NP; q.base = base; q.xoff = xoff;
+        q.gstride = (NP + 31) / 32 * 32;
+        q.gbase = gsum * c.GF;
+        gsum += q.gstride;
         q.n = (int64_t)NP * (NB + 2 * MB + 1) + 1;
         q.dz = d->dz[p];
         q.total_charge = d->total_charge ? d->total_charge[p] : 0.0;
@@ -1018,6 +1022,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         c.max_n = std::max(c.max_n, q.n);
     }
     c.sumNP = base;
+    c.sumGS = gsum;
     c.total_n = xoff;
     c.sumL_sets = kb;
     ctx->nsets = (int)set_L.size();
@@ -1172,8 +1177,8 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     auto alloc_pipe = [&](auto& pp) -> int {
         TRY(dalloc(ctx, &pp.elam, c.sumNP * NB));
         TRY(dalloc(ctx, &pp.hl, c.sumNP * 2 * MB));
-        TRY(dalloc(ctx, &pp.gC, c.sumNP * c.GF));
-        TRY(dalloc(ctx, &pp.gP, c.sumNP * c.GF));
+        TRY(dalloc(ctx, &pp.gC, c.sumGS * c.GF));
+        TRY(dalloc(ctx, &pp.gP, c.sumGS * c.GF));
         TRY(dalloc(ctx, &pp.smax, (int64_t)P * c.nsp * 2 * c.nmax_seg));
         TRY(dalloc(ctx, &pp.lsc, (int64_t)P * c.nsp * (c.nmax_seg + 1)));
         TRY(dalloc(ctx, &pp.rho_beads, c.sumNP * NB));
@@ -1421,7 +1426,9 @@ extern "C" int32_t sdft_solve(sdft_ctx_t* ctx, const sdft_solver_opts_t* o, doub
     HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
     int any_fail = 0;
     double tot = 0.0;
+    ctx->last_switch.assign(c.P, 0);
     for (int p = 0; p < c.P; ++p) {
+        ctx->last_switch[p] = st[p].switch_iter;
         if (iters) iters[p] = st[p].iter;
         if (err) err[p] = st[p].err;
         tot += st[p].iter;
@@ -1457,6 +1464,13 @@ extern "C" int32_t sdft_last_pipeline_iters(const sdft_ctx_t* ctx, int32_t* fp32
     if (!ctx) return SDFT_E_ARG;
     if (fp32_iters) *fp32_iters = ctx->last_iters32;
     if (fp64_iters) *fp64_iters = ctx->last_iters64;
+    return SDFT_OK;
+}
+
+extern "C" int32_t sdft_last_switch_iters(const sdft_ctx_t* ctx, int32_t* switch_iter) {
+    if (!ctx || !switch_iter) return SDFT_E_ARG;
+    for (int p = 0; p < ctx->c.P; ++p)
+        switch_iter[p] = p < (int)ctx->last_switch.size() ? ctx->last_switch[p] : 0;
     return SDFT_OK;
 }
 

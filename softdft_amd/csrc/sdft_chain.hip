@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
         __syncthreads();
         // ---- MFMA tiles: unit = (a, tile), summing the S input states into Q_a
         T* gout = sw == 0 ? gC : gP;
-        const int64_t gb = P.base * (int64_t)ca.GF + ca.goff;
+        const int64_t gb = P.gbase + ca.goff * P.gstride;
         const int q = lane & 15, g = lane >> 4;
         for (int unit = wave; unit < S * ntile; unit += CHAIN_WAVES) {
             const int a = unit / ntile, tile = unit - a * ntile;
@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
                         const T y = acc[v];
                         const T ay = y < T(0) ? -y : y;
                         const T val = ca.hv ? ay * HV[col * NPQ + n] : ay * hv[b][v] / (T)norm;
-                        gout[gb + (((int64_t)outseg * S + b) * S + a) * NP + n] = val;
+                        gout[gb + (((int64_t)outseg * S + b) * S + a) * P.gstride + n] = val;
                         qs[v] += val;
                     }
                 }
@@ -472,7 +472,7 @@ __global__ void __launch_bounds__(256) k_bandstep(const DevSystem* __restrict__ 
                        ? (T)(1.0 / scale_read(A.smax, scale_slot(p, A.u, sw, A.s - 1, A.nsp, A.nmax)))
                        : T(1);
     const T* gin = sw == 0 ? gC : gP;
-    const int64_t gb = P.base * (int64_t)A.GF + A.goff;
+    const int64_t gb = P.gbase + A.goff * P.gstride;
     const int64_t eb = P.base * (int64_t)A.NB, hb = P.base * (int64_t)(2 * A.MB);
     for (int e = tid; e < S * XL; e += 256) {
         const int b = e / XL, mi = e - b * XL;
@@ -486,7 +486,7 @@ __global__ void __launch_bounds__(256) k_bandstep(const DevSystem* __restrict__ 
             } else {
                 q = T(0);
                 for (int b2 = 0; b2 < S; ++b2)
-                    if (st[inseg * S + b2] >= 0) q += gin[gb + ((int64_t)(inseg * S + b2) * S + b) * NP + n];
+                    if (st[inseg * S + b2] >= 0) q += gin[gb + ((int64_t)(inseg * S + b2) * S + b) * P.gstride + n];
             }
             const int sj = colsj[b];
             v = elam[eb + (int64_t)sj * NP + n] * hl[hb + (int64_t)(2 * colbond[b] + colpj[b]) * NP + n] *
@@ -546,7 +546,7 @@ __global__ void __launch_bounds__(256) k_bandstep(const DevSystem* __restrict__ 
                     const double d = (double)(n - Kf) * P.dz;
                     if (d * d > maxd[outseg] * maxd[outseg]) val = T(0);
                 }
-                gout[gb + ((int64_t)(outseg * S + b) * S + a) * NP + n] = val;
+                gout[gb + ((int64_t)(outseg * S + b) * S + a) * P.gstride + n] = val;
                 vmax = (double)val > vmax ? (double)val : vmax;
             }
         }

@@ -3,6 +3,9 @@
 // Memory layout in HBM (all per-point data contiguous per point):
 //   grid family with F fields:   fam[base_p * F + f * NP_p + n]
 //     base_p = sum_{q<p} NP_q,  n in [0, NP_p)
+//   propagator families gC, gP (GF fields per point): g[gbase_p + f * gstride_p + n]
+//     gstride_p = NP_p rounded up to 32 (every column starts on a 128-byte line in FP32, 256 in FP64),
+//     gbase_p = GF * sum_{q<p} gstride_q
 //   convolution kernels:          kern[kbase_s * K + k * L_s + d]
 //     one set per distinct (NP, dz); L_s = 2 NP_s; K = 3 NB + MB + 1 + 2 n_sw
 //   solution vectors (FP64):      x[xoff_p + i], i in [0, n_p), reference flatten order
@@ -38,8 +41,9 @@ struct DevPoint {
     int32_t NP;
     int32_t L;
     int32_t kset;        // index of the kernel set
-    int32_t pad0;
+    int32_t gstride;     // column stride of the propagator families: NP rounded up to 32 (aligned rows)
     int64_t base;        // sum of NP over previous points
+    int64_t gbase;       // start of this point's propagator fields: GF * sum of gstride over previous points
     int64_t kbase;       // sum of L over previous kernel sets (of this point's set)
     int64_t xoff;        // offset of this point's solution vector
     int64_t n;           // solution length

@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(TB) k_prop_step(const DevSystem* __restrict__ 
     const bool in_stored = sweep == 0 ? (inseg < N - 1) : (inseg > 0);
 
     auto gidx = [&](int seg, int b, int a, int nn) -> int64_t {
-        return P.base * (int64_t)A.GF + (A.goff + ((int64_t)seg * S + b) * S + a) * NP + nn;
+        return P.gbase + (A.goff + ((int64_t)seg * S + b) * S + a) * P.gstride + nn;
     };
     // input scale: 1 / the previous step's largest output (per-step rescaling)
     const T qinv = (A.rescale && in_stored)
@@ -431,7 +431,7 @@ __global__ void __launch_bounds__(256) k_fixed_lambda(const DevSystem* __restric
     const int Kf = P.fixed_coord[A.u];
     const int q = A.fixed_seg == 0 ? 1 : 0;  // first non-fixed segment
     auto gidx = [&](int seg, int b, int a, int nn) -> int64_t {
-        return P.base * (int64_t)A.GF + (A.goff + ((int64_t)seg * S + b) * S + a) * NP + nn;
+        return P.gbase + (A.goff + ((int64_t)seg * S + b) * S + a) * P.gstride + nn;
     };
     double acc = 0.0;
     for (int n = threadIdx.x; n < NP; n += blockDim.x) {
@@ -489,8 +489,9 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                                                    int Frt, double* __restrict__ acc_part, int32_t* __restrict__ errflag,
                                                    const double* __restrict__ lsc, int nmax) {
     // A block covers ACC_POINTS grid points; its ACC_GROUPS wavefronts split the segments of every
-    // species (segment q goes to group q % ACC_GROUPS), so more independent loads are in flight per
-    // point; the groups' partial densities are added in group order at the end.
+    // species, dealt round-robin continuing from one species to the next (so single-bead species do
+    // not all fall to group 0), which keeps more independent loads in flight per point; the groups'
+    // partial densities are added in group order at the end.
     const DevSystem& sys = *psys;
     const int p = blockIdx.y;
     if (evalflag[p] != prec) return;
@@ -508,9 +509,12 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
     for (int j = 0; j < NB; ++j) rb[j * TB] = 0.0;
     for (int f = 0; f < 2 * MB; ++f) rbo[f * TB] = 0.0;
     __shared__ double red[2 * SDFT_MAX_SPECIES][TB / 64];
+    int dealt = 0;  // segments dealt to the groups by the previous species (round-robin across species)
     for (int u = 0; u < sys.n_species; ++u) {
         const DevSpecies sp = spec[u];
         const int N = sp.nseg, S = sp.S;
+        const int q0 = (grp - dealt % ACC_GROUPS + ACC_GROUPS) % ACC_GROUPS;  // this group's first segment
+        dealt += N;
         const int* st = seg_states + sp.state_off;
         const double* maxd = maxd_tab + sp.maxd_off;
         const int Kf = sp.fixed_seg >= 0 ? P.fixed_coord[u] : -1;
@@ -528,7 +532,7 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
         }
         const double pref0 = norm * exp(P.mu_species[u]);
         auto gidx = [&](int seg, int b, int a, int nn) -> int64_t {
-            return P.base * (int64_t)GF + (goffs[u] + ((int64_t)seg * S + b) * S + a) * NP + nn;
+            return P.gbase + (goffs[u] + ((int64_t)seg * S + b) * S + a) * P.gstride + nn;
         };
         double num = 0.0, chg = 0.0;
         // homogeneous species (every segment carries the same state beads): the bead of state a and
@@ -542,7 +546,7 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
 #pragma unroll
             for (int b = 0; b < SDFT_MAX_S; ++b) rc0[a][b] = rc1[a][b] = 0.0;
         }
-        for (int q = grp; q < N; q += ACC_GROUPS) {
+        for (int q = q0; q < N; q += ACC_GROUPS) {
             const double pref = sp.rescale ? pref0 * exp(seg_log_scale(sp, P, p, u, q, lsc, sys.n_species, nmax))
                                            : pref0;
             // every propagator value of this segment is loaded up front (independent loads in flight),
@@ -563,7 +567,9 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                     const int bc = b < S ? b : 0;
                     const bool okc = si >= 0 && b < S && q < N - 1 && st[(q + 1) * S + b] >= 0;
                     const bool okp = si >= 0 && b < S && q > 0 && st[(q - 1) * S + b] >= 0;
-                    const T vc = gC[gidx(q, bc, ac, nc)], vp = gP[gidx(q, bc, ac, nc)];
+                    // a single-bead species has no propagators: no loads (N is wave-uniform)
+                    const T vc = N > 1 ? gC[gidx(q, bc, ac, nc)] : T(0);
+                    const T vp = N > 1 ? gP[gidx(q, bc, ac, nc)] : T(0);
                     gcv[b][a] = okc ? vc : T(0);
                     gpv[b][a] = okp ? vp : T(0);
                 }
@@ -1550,7 +1556,7 @@ __global__ void __launch_bounds__(256) k_segment_sums(const DevSystem* __restric
     double pref = norm * exp(P.mu_species[u]);
     if (sp.rescale) pref *= exp(seg_log_scale(sp, P, p, u, q, lsc, sys.n_species, nmax));
     auto gidx = [&](int seg, int b, int a, int nn) -> int64_t {
-        return P.base * (int64_t)GF + (goffs[u] + ((int64_t)seg * S + b) * S + a) * NP + nn;
+        return P.gbase + (goffs[u] + ((int64_t)seg * S + b) * S + a) * P.gstride + nn;
     };
     __shared__ double red[4];
     for (int a = 0; a < S; ++a) {
@@ -1611,7 +1617,7 @@ __global__ void __launch_bounds__(256) k_segment_profile(const DevSystem* __rest
     double pref = norm * exp(P.mu_species[u]);
     if (sp.rescale) pref *= exp(seg_log_scale(sp, P, p, u, q, lsc, sys.n_species, nmax));
     auto gidx = [&](int seg, int b, int a) -> int64_t {
-        return P.base * (int64_t)GF + (goffs[u] + ((int64_t)seg * S + b) * S + a) * NP + n;
+        return P.gbase + (goffs[u] + ((int64_t)seg * S + b) * S + a) * P.gstride + n;
     };
     for (int a = 0; a < S; ++a) {
         const int si = st[q * S + a];

@@ -93,6 +93,7 @@ struct Ctx {
     int ncls = 1;                   // distinct bead diameters
     int maxNP = 0;
     int64_t sumNP = 0, sumL_sets = 0;
+    int64_t sumGS = 0;              // sum over points of the propagator column stride (NP rounded up to 32)
     int64_t total_n = 0, max_n = 0;
     int GF = 0;                     // propagator fields per point (sum over species of nseg*S*S)
     int Frt = 0, Fnw = 0, Fdp = 0;  // family widths

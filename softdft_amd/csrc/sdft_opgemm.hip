@@ -200,10 +200,10 @@ __global__ void __launch_bounds__(256) k_opgemm(const DevSystem* __restrict__ ps
                                 }
                             } else {
                                 q_in = T(0);
-                                const int64_t gb = P.base * (int64_t)ps.GF + ps.goff;
+                                const int64_t gb = P.gbase + ps.goff * P.gstride;
                                 for (int b2 = 0; b2 < S; ++b2)
                                     if (seg_states[ps.state_off + inseg * S + b2] >= 0)
-                                        q_in += gin[gb + (((int64_t)inseg * S + b2) * S + b) * P.NP + nn];
+                                        q_in += gin[gb + (((int64_t)inseg * S + b2) * S + b) * P.gstride + nn];
                             }
                             const int64_t eb = P.base * (int64_t)ps.NB;
                             v = elam[eb + (int64_t)sj * P.NP + nn] *
@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(256) k_opgemm(const DevSystem* __restrict__ ps
                             if (d * d > maxd[outseg] * maxd[outseg]) val = T(0);
                         }
                         T* gout = sweep == 0 ? gC : gP;
-                        gout[P.base * (int64_t)ps.GF + ps.goff + (((int64_t)outseg * S + b) * S + a) * P.NP + n] = val;
+                        gout[P.gbase + (ps.goff + ((int64_t)outseg * S + b) * S + a) * P.gstride + n] = val;
                         vmax = (double)val > vmax ? (double)val : vmax;
                     }
                     if (ps.fused) {

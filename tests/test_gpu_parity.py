@@ -288,3 +288,38 @@ def test_nonconvergence_and_nonfinite_inputs_are_reported():
         b.solve(x, max_iters=50)
     assert e.value.code == -4
     b.close()
+
+
+def _chains_not_first():
+    ds = configs.hard_sphere_chains(H=5.0, dz=0.05, n_seg=5, rho_poly=0.06, rho_solvent=0.2)
+    ds["species"] = [{"sequence": "S", "density": 0.2}, {"sequence": "AAAAA", "density": 0.06},
+                     {"sequence": "AAA", "density": 0.04}]
+    return ds
+
+
+@pytest.mark.parametrize("mapping", [0, 1, 2])
+def test_chain_species_after_other_species(mapping):
+    """Chain species whose propagator fields do not start at offset 0 (solvent listed first, two chain
+    species): every propagator mapping (per-step kernels 0, chain kernel 1 and 2), FP64 dense, FP64
+    banded (against the truncated oracle) and FP32."""
+    sp = build_state_point(_chains_not_first())
+    b = abi.Batch([sp])
+    b.set_tuning([dict(kind=abi.TUNE_PROPAGATOR, key=u, precision=0, choice=mapping) for u in range(b.nsp)])
+    rng = np.random.default_rng(3)
+    x = b.x0() + 0.03 * rng.standard_normal(b.n_total)
+    print(f"case chains after other species mapping={mapping}", flush=True)
+    ref = OracleDFT(sp).objective(x)
+    scale = max(1.0, float(np.max(np.abs(ref))))
+    fx = b.objective(x, abi.PREC_FP64)
+    assert np.max(np.abs(fx - ref)) / scale < 1e-11
+    fx32 = b.objective(x, abi.PREC_FP32)
+    assert np.max(np.abs(fx32 - ref)) / scale < 5e-5
+    b.set_operator_band(fp32=True, fp64=True)
+    o, _ = _truncated_oracle(sp)
+    refb = o.objective(x)
+    fxb = b.objective(x, abi.PREC_FP64)
+    assert np.max(np.abs(fxb - refb)) / scale < 1e-11
+    for u in (1, 2):
+        seg = b.segment_profile(u, 0)
+        assert np.max(np.abs(seg - o.rho_seg[u])) <= 1e-11 * max(float(np.max(np.abs(o.rho_seg[u]))), 1e-300)
+    b.close()
