@@ -252,7 +252,7 @@ def report_c4(args):
     n = len(res)
     # plain FP64 one width at a time (sample) and the same widths in one batched cold FP64 solve
     sel = np.linspace(0, n - 1, 4).astype(int)
-    sps = build_state_points([dss[i] for i in sel])
+    sps = build_state_points([dss[i] for i in sel], device_bulk=True)
     t0 = time.perf_counter()
     for sp in sps:
         b1 = abi.Batch([sp])
@@ -289,7 +289,7 @@ def report_c5(args):
     from softdft_amd.problem import build_state_points
     dss = configs.config5()
     t0 = time.perf_counter()
-    sps = build_state_points(dss, workers=12)
+    sps = build_state_points(dss, device_bulk=True)
     log(f"c5: {len(sps)} state points set up in {time.perf_counter() - t0:.1f} s")
     b = abi.Batch(sps)
     x0 = b.x0()
@@ -376,8 +376,10 @@ def main():
     from softdft_amd.problem import build_state_points
 
     t = time.time()
-    sps = build_state_points(configs.config3(args.points), workers=8)
-    log(f"setup: {len(sps)} state points in {time.time() - t:.1f} s")
+    # host setup with every bulk state solved on the device in one launch (sdft_bulk_solve)
+    sps = build_state_points(configs.config3(args.points), device_bulk=True)
+    setup_s = time.time() - t
+    log(f"setup: {len(sps)} state points in {setup_s:.2f} s (bulk states on the device)")
     batch = abi.Batch(sps)
     x0 = batch.x0()
     num = dict(tole=1e-8, damping=0.1, mixing_max=0.02, switch_tole=100.0, anderm=5, max_iters=3000)
@@ -596,6 +598,7 @@ def main():
         "errors_vs_oracle": vs_oracle,
         "cpu_baseline": cpu,
         "cpu_baseline_one_core": cpu1,
+        "setup_s": setup_s,
         "host": {"cpu_count": os.cpu_count(), "cpus_available": len(os.sched_getaffinity(0)),
                  "cpu_model": _cpu_model()},
         "kernels_us": {k: v[0] * 1e3 for k, v in prof.items()},

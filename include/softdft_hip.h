@@ -254,6 +254,53 @@ int32_t sdft_autotune(sdft_ctx_t* ctx, int32_t reps, sdft_tune_entry_t* entries,
  * every grid length, choice -1 restores the fixed rule.  us[] is ignored. */
 int32_t sdft_set_tuning(sdft_ctx_t* ctx, const sdft_tune_entry_t* entries, int32_t n);
 
+/*
+ * Bulk (homogeneous) state of every state point of a batch, solved on the device: the inputs
+ * sdft_create takes from the bulk (mu_species; x0 = bulk mu_ex and lng) and the bulk pressure.
+ * Replaces the iLST bulk solve, src/iLST/evaluation/objective/solver_bulkstate.jl:31-103 (with
+ * residual_bulkstate.jl:26-42, functions/balance_charge.jl:21-79, the bulk chain propagators
+ * src/iLST/species_model/) and eval_bulk_pressure (src/iLST/evaluation/analysis/eval_pressure.jl:10-25).
+ * Linear chains (freely jointed or monomerbead); one wavefront per point runs Newton on the
+ * self-consistency of (mu_ex, lng) with a forward-difference Jacobian to max |residual| < tol.
+ */
+typedef struct sdft_bulk_desc {
+    int32_t abi_version;
+    int32_t n_points, NB, MB, n_species;
+    int32_t functionals;             /* OR of SDFT_F_* (the ideal term is always present) */
+    double bjerrum;
+    const double* diameters;         /* [NB] */
+    const double* valences;          /* [NB] */
+    const int32_t* bond_types;       /* [MB][2], i <= j */
+    const int32_t* species_nseg;     /* [n_species] */
+    const int32_t* species_model;    /* [n_species] SDFT_MODEL_* */
+    const int32_t* species_nstates;  /* [n_species] */
+    const int32_t* seg_states;       /* concat over species of [nseg_u][S_u] bead types, -1 = unused */
+    const int32_t* adjustable;       /* [n_species] 1: charge compensator (balance_charge.jl) */
+    const double* delta_muH;         /* [P][NB] */
+    const double* base_density;      /* [P][n_species] chains per volume before the charge balance */
+    const double* log_activity;      /* [P][n_species] ln(c 10^-pSpecies) for pSpecies species (their
+                                        density is exp(log_activity + ln Xi)), NaN otherwise */
+    const double* species_charge;    /* [P][n_species] charge per chain used by the balance */
+    int32_t n_sw_pairs;              /* square-well pairs (SDFT_F_SW) */
+    const int32_t* sw_pairs;         /* [n_sw_pairs][2] */
+    const double* sw_energy;         /* [n_sw_pairs] */
+    const double* sw_volume;         /* [n_sw_pairs] 4 pi / 3 sigma_ij^3 (lambda^3 - 1) */
+} sdft_bulk_desc_t;
+
+typedef struct sdft_bulk_out {       /* host arrays written by sdft_bulk_solve */
+    double* mu_species;              /* [P][n_species] */
+    double* rho_species;             /* [P][n_species] */
+    double* mu_ex;                   /* [P][NB] */
+    double* rho_beads;               /* [P][NB] */
+    double* lng;                     /* [P][MB] */
+    double* rho_bonds;               /* [P][2][MB] */
+    double* pressure;                /* [P] */
+    int32_t* iterations;             /* [P] Newton iterations; -1: not converged, -2: no compensator */
+} sdft_bulk_out_t;
+
+/* returns SDFT_E_NOCONV if any point did not converge (its iterations entry says why) */
+int32_t sdft_bulk_solve(const sdft_bulk_desc_t* desc, double tol, int32_t max_newton, sdft_bulk_out_t* out);
+
 /* kernel classes for sdft_kernel_profile */
 #define SDFT_KP_PROP_FP32 0      /* one propagator step, FP32, banded bond kernel */
 #define SDFT_KP_PROP_FP64 1      /* one propagator step, FP64, bond operator as in the last solve */

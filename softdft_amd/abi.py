@@ -31,7 +31,7 @@ EXPORTED = ("sdft_abi_version", "sdft_last_error", "sdft_create", "sdft_destroy"
             "sdft_solution_offsets", "sdft_objective", "sdft_solve", "sdft_observables", "sdft_get_field",
             "sdft_last_timing", "sdft_last_pipeline_iters", "sdft_set_operator_band", "sdft_kernel_profile",
             "sdft_set_launch_mode", "sdft_autotune", "sdft_set_tuning", "sdft_observables_noalloc",
-            "sdft_last_switch_iters")
+            "sdft_last_switch_iters", "sdft_bulk_solve")
 TUNE_PROPAGATOR, TUNE_CONV_FWD, TUNE_CONV_PRE, TUNE_CONV_BACK = 1, 2, 3, 4
 CONV_TALL, CONV_WIDE64, CONV_WIDE128, CONV_MID, CONV_NARROW = 0, 1, 2, 3, 4
 KP_PROP_FP32, KP_PROP_FP64, KP_CONV_FWD_FP32, KP_CONV_FWD_FP64, KP_CONV_BACK_FP32, KP_CONV_BACK_FP64, \

@@ -1821,6 +1821,128 @@ extern "C" int32_t sdft_set_tuning(sdft_ctx_t* ctx, const sdft_tune_entry_t* ent
     return SDFT_OK;
 }
 
+extern "C" int32_t sdft_bulk_solve(const sdft_bulk_desc_t* d, double tol, int32_t max_newton, sdft_bulk_out_t* out) {
+    if (!d || !out) {
+        set_error(nullptr, SDFT_E_ARG, "null argument");
+        return SDFT_E_ARG;
+    }
+    if (d->abi_version != SDFT_ABI_VERSION) {
+        set_error(nullptr, SDFT_E_ARG, "abi_version %d != %d", d->abi_version, SDFT_ABI_VERSION);
+        return SDFT_E_ARG;
+    }
+    const int P = d->n_points, NB = d->NB, MB = d->MB, nsp = d->n_species;
+    if (P <= 0 || NB <= 0 || NB > SDFT_MAX_NB || MB < 0 || MB > SDFT_MAX_MB || nsp <= 0 || nsp > SDFT_MAX_SPECIES ||
+        max_newton < 1 || !(tol > 0.0)) {
+        set_error(nullptr, SDFT_E_ARG, "bad sizes P=%d NB=%d MB=%d n_species=%d max_newton=%d", P, NB, MB, nsp, max_newton);
+        return SDFT_E_ARG;
+    }
+    if (!d->diameters || !d->valences || (MB > 0 && !d->bond_types) || !d->species_nseg || !d->species_model ||
+        !d->species_nstates || !d->seg_states || !d->adjustable || !d->delta_muH || !d->base_density ||
+        !d->log_activity || !d->species_charge || !out->mu_species || !out->rho_species || !out->mu_ex ||
+        !out->rho_beads || !out->lng || !out->rho_bonds || !out->pressure || !out->iterations) {
+        set_error(nullptr, SDFT_E_ARG, "a required table is NULL");
+        return SDFT_E_ARG;
+    }
+    BulkSysDev s;
+    memset(&s, 0, sizeof(s));
+    s.NB = NB; s.MB = MB; s.nsp = nsp; s.F = d->functionals; s.lB = d->bjerrum;
+    for (int j = 0; j < NB; ++j) { s.sig[j] = d->diameters[j]; s.z[j] = d->valences[j]; }
+    for (int u = 0; u < MB; ++u) {
+        s.bi[u] = d->bond_types[2 * u];
+        s.bj[u] = d->bond_types[2 * u + 1];
+        if (s.bi[u] < 0 || s.bj[u] >= NB || s.bi[u] > s.bj[u]) {
+            set_error(nullptr, SDFT_E_ARG, "bond %d must satisfy 0 <= i <= j < NB", u);
+            return SDFT_E_ARG;
+        }
+    }
+    int soff = 0, ws_len = 1;
+    for (int u = 0; u < nsp; ++u) {
+        const int N = d->species_nseg[u], S = d->species_nstates[u];
+        if (N <= 0 || S <= 0 || S > SDFT_MAX_S) {
+            set_error(nullptr, SDFT_E_UNSUPPORTED, "species %d: nseg=%d states=%d (at most %d states per segment)", u, N, S,
+                      SDFT_MAX_S);
+            return SDFT_E_UNSUPPORTED;
+        }
+        s.nseg[u] = N; s.S[u] = S; s.model[u] = d->species_model[u]; s.soff[u] = soff;
+        s.adjustable[u] = d->adjustable[u];
+        for (int k = 0; k < N * S; ++k) {
+            const int b = d->seg_states[soff + k];
+            if (b >= NB || b < -1 || (k % S == 0 && b < 0)) {
+                set_error(nullptr, SDFT_E_ARG, "species %d: bad state table entry %d", u, b);
+                return SDFT_E_ARG;
+            }
+        }
+        soff += N * S;
+        ws_len = std::max(ws_len, (S + 1) * N);
+    }
+    const int n_sw = (d->functionals & SDFT_F_SW) ? d->n_sw_pairs : 0;
+    if (n_sw < 0 || n_sw > SDFT_MAX_SW || (n_sw > 0 && (!d->sw_pairs || !d->sw_energy || !d->sw_volume))) {
+        set_error(nullptr, SDFT_E_ARG, "square well: need 0..%d pairs with energies and volumes", SDFT_MAX_SW);
+        return SDFT_E_ARG;
+    }
+    s.n_sw = n_sw;
+    for (int k = 0; k < n_sw; ++k) {
+        s.swi[k] = d->sw_pairs[2 * k]; s.swj[k] = d->sw_pairs[2 * k + 1];
+        s.sweps[k] = d->sw_energy[k]; s.swV[k] = d->sw_volume[k];
+    }
+    int dev_count = 0;
+    if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) {
+        set_error(nullptr, SDFT_E_HIP, "no HIP device available");
+        return SDFT_E_HIP;
+    }
+    // one temporary context holds the device buffers and the stream (released on every exit path)
+    struct Tmp {
+        sdft_ctx* ctx = new sdft_ctx();
+        ~Tmp() { sdft_destroy(ctx); }
+    } tmp;
+    sdft_ctx* ctx = tmp.ctx;
+    (void)hipGetDevice(&ctx->device);
+    HIP_CHECK_RET(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    int32_t* d_states = nullptr;
+    double *d_dmu = nullptr, *d_base = nullptr, *d_la = nullptr, *d_q = nullptr, *d_ws = nullptr, *d_out = nullptr;
+    int32_t* d_it = nullptr;
+    TRY(upload(ctx, &d_states, d->seg_states, soff));
+    TRY(upload(ctx, &d_dmu, d->delta_muH, (int64_t)P * NB));
+    TRY(upload(ctx, &d_base, d->base_density, (int64_t)P * nsp));
+    TRY(upload(ctx, &d_la, d->log_activity, (int64_t)P * nsp));
+    TRY(upload(ctx, &d_q, d->species_charge, (int64_t)P * nsp));
+    TRY(dalloc(ctx, &d_ws, (int64_t)P * 64 * ws_len));
+    const int64_t per = 2 * nsp + 2 * NB + 3 * MB + 1;   // doubles of output per point
+    TRY(dalloc(ctx, &d_out, (int64_t)P * per));
+    TRY(dalloc(ctx, &d_it, P));
+    BulkOutDev o;
+    o.mu_species = d_out;
+    o.rho_species = o.mu_species + (int64_t)P * nsp;
+    o.mu_ex = o.rho_species + (int64_t)P * nsp;
+    o.rho_beads = o.mu_ex + (int64_t)P * NB;
+    o.lng = o.rho_beads + (int64_t)P * NB;
+    o.rho_bonds = o.lng + (int64_t)P * MB;
+    o.pressure = o.rho_bonds + (int64_t)P * 2 * MB;
+    o.iterations = d_it;
+    launch_bulk_solve(ctx->stream, P, s, d_states, d_dmu, d_base, d_la, d_q, tol, max_newton, d_ws, ws_len, o);
+    HIP_CHECK_RET(hipGetLastError());
+    auto down = [&](double* host, const double* dev, int64_t n) -> hipError_t {
+        return hipMemcpyAsync(host, dev, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+    };
+    HIP_CHECK_RET(down(out->mu_species, o.mu_species, (int64_t)P * nsp));
+    HIP_CHECK_RET(down(out->rho_species, o.rho_species, (int64_t)P * nsp));
+    HIP_CHECK_RET(down(out->mu_ex, o.mu_ex, (int64_t)P * NB));
+    HIP_CHECK_RET(down(out->rho_beads, o.rho_beads, (int64_t)P * NB));
+    HIP_CHECK_RET(down(out->lng, o.lng, (int64_t)P * MB));
+    HIP_CHECK_RET(down(out->rho_bonds, o.rho_bonds, (int64_t)P * 2 * MB));
+    HIP_CHECK_RET(down(out->pressure, o.pressure, P));
+    HIP_CHECK_RET(hipMemcpyAsync(out->iterations, d_it, (size_t)P * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    for (int p = 0; p < P; ++p)
+        if (out->iterations[p] < 0) {
+            set_error(nullptr, SDFT_E_NOCONV, "bulk state of point %d: %s", p,
+                      out->iterations[p] == -2 ? "no compensating species can absorb the net charge"
+                                               : "Newton did not reach the tolerance");
+            return SDFT_E_NOCONV;
+        }
+    return SDFT_OK;
+}
+
 extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps, double* avg_ms, double* flops) {
     if (!ctx || reps < 1) return SDFT_E_ARG;
     if (!ctx->have_eval) {

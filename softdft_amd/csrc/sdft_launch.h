@@ -250,6 +250,25 @@ void launch_build_kernels(hipStream_t st, const DevSystem* sys, int K, int nsets
 void launch_anderson(hipStream_t st, const Ctx& c, const SolveParams& sp);
 hipError_t prepare_kernel_attributes(const Ctx& c);
 void launch_apply_switch(hipStream_t st, const Ctx& c);
+// bulk state solve (sdft_bulk.hip): system layout shared by the batch, passed by value
+struct BulkSysDev {
+    int32_t NB, MB, nsp, F, n_sw;
+    double lB;
+    double sig[SDFT_MAX_NB], z[SDFT_MAX_NB];
+    int32_t bi[SDFT_MAX_MB], bj[SDFT_MAX_MB];
+    int32_t nseg[SDFT_MAX_SPECIES], S[SDFT_MAX_SPECIES], model[SDFT_MAX_SPECIES], soff[SDFT_MAX_SPECIES];
+    int32_t adjustable[SDFT_MAX_SPECIES];
+    int32_t swi[SDFT_MAX_SW], swj[SDFT_MAX_SW];
+    double sweps[SDFT_MAX_SW], swV[SDFT_MAX_SW];
+};
+struct BulkOutDev {
+    double *mu_species, *rho_species, *mu_ex, *rho_beads, *lng, *rho_bonds, *pressure;
+    int32_t* iterations;
+};
+void launch_bulk_solve(hipStream_t st, int P, const BulkSysDev& s, const int32_t* seg_states, const double* dmu,
+                       const double* base, const double* logact, const double* qch, double tol, int max_newton,
+                       double* ws, int64_t ws_len, const BulkOutDev& out);
+
 // per-species offsets (in doubles) of the segment-sum blocks inside an observables work buffer
 struct ObsLayout {
     int64_t ss_base[SDFT_MAX_SPECIES];

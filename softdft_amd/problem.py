@@ -114,10 +114,15 @@ def unflatten(x: np.ndarray, NP: int, NB: int, MB: int):
     return mu, lng, Psi, float(x[b + NP])
 
 
-def build_state_point(dataset: dict, label: str = "") -> StatePoint:
+def build_state_point(dataset: dict, label: str = "", bulk: Optional[BulkState] = None,
+                      ms: Optional[MolSys] = None) -> StatePoint:
+    """Host setup of one state point. `bulk` and `ms` (the molecular system it was solved for, built
+    from the same dataset) skip the host bulk solve, e.g. with states from bulk_device.solve_bulk_batch."""
     ds = copy.deepcopy(dataset)
-    ms = build_molsys(ds)
-    bulk = solve_bulk(ms)
+    if ms is None:
+        ms = build_molsys(ds)
+    if bulk is None:
+        bulk = solve_bulk(ms)
     geom = ds["geometry"]
     if geom.get("coordinate_system", "Cartesian") != "Cartesian" or len(geom["dimensions"]) != 1:
         raise SetupError("only 1-D Cartesian slits are supported")
@@ -251,9 +256,16 @@ def build_state_point(dataset: dict, label: str = "") -> StatePoint:
     return sp_out
 
 
-def build_state_points(datasets: Sequence[dict], workers: int = 0) -> List[StatePoint]:
+def build_state_points(datasets: Sequence[dict], workers: int = 0, device_bulk: bool = False) -> List[StatePoint]:
     """build_state_point over many datasets; with workers > 1 the host setups (bulk solves) run in
-    that many spawned processes (no GPU state is shared with them), results in input order."""
+    that many spawned processes (no GPU state is shared with them), results in input order. With
+    device_bulk the bulk states of all points are solved in one sdft_bulk_solve launch (datasets
+    sharing one molecular-system layout) and only the rest of the setup runs on the host."""
+    if device_bulk:
+        from .bulk_device import solve_bulk_batch
+        mss = [build_molsys(copy.deepcopy(d)) for d in datasets]
+        bulks = solve_bulk_batch(mss)
+        return [build_state_point(d, bulk=b, ms=m) for d, b, m in zip(datasets, bulks, mss)]
     if workers <= 1 or len(datasets) < 2:
         return [build_state_point(d) for d in datasets]
     import multiprocessing as mp

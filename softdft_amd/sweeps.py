@@ -84,11 +84,12 @@ def width_sweep(datasets: Sequence[dict], *, waves: int = 8, precision: int = ab
                 on_round: Optional[Callable[[int, float], None]] = None, workers: int = 0,
                 tune: bool = False) -> List[WidthResult]:
     """Solve the slit widths of `datasets` (increasing H, one chemistry) with neighbour warm starts;
-    `workers` > 1 runs the host setups in parallel processes; `tune` measures the launch
+    the bulk states of all widths are solved on the device in one launch (`workers` is not used:
+    the rest of the host setup takes milliseconds per point); `tune` measures the launch
     configuration on the first round's context and replays it on the others.
 
     Returns one WidthResult per width, in sweep order."""
-    sps = build_state_points(datasets, workers)
+    sps = build_state_points(datasets, workers, device_bulk=True)
     n = len(sps)
     num = dict(numerics or sps[0].numerics)
     opts = dict(tole=num["tole"], damping=num["damping"], mixing_max=num["mixing_max"],
@@ -185,7 +186,7 @@ def ph_sweep(datasets: Sequence[dict], pH_values: Sequence[float], *, waves: int
     """pH continuation over `datasets` (one per pH in `pH_values`, same grid and chemistry) with
     neighbour warm starts; the result list ends at the reference's early-stop step (`tune` as in
     width_sweep)."""
-    sps = build_state_points(datasets, workers)
+    sps = build_state_points(datasets, workers, device_bulk=True)
     n = len(sps)
     num = dict(numerics or sps[0].numerics)
     opts = dict(tole=num["tole"], damping=num["damping"], mixing_max=num["mixing_max"],

@@ -1,9 +1,8 @@
 """Every benchmark configuration at its full size, through the C ABI, against the FP64 oracle.
 
   C1  20-mers + solvent, H 16, dz 0.02 (NP 801): converged FP64 and mixed solves
-  C3  the pH sweep's chemistry on the C1 grid: pH 2, 7 and 12 of the 256-point sweep, solved as one
-      mixed-precision batch (a point's trajectory does not depend on the rest of the batch:
-      test_gpu_graphs / test_gpu_parity's batch-equals-individual checks)
+  C3  the 256-point pH sweep (bulk states on the device) solved as one mixed-precision batch; its
+      points at pH 2, 7 and 12 against oracle solves (set up on the host)
   C4  ragged slit widths at dz 0.02: FP64 objective on NP 201, 205, 1100, 2001 in one batch; the two
       narrowest widths solved, with Pi(H) = P_cvt - P_bulk
   C5  500-mer brush, dz 0.005, NP 16384: FP64 objectives at the bulk guess at pH 3 / 1 mM and pH 9 /
@@ -21,7 +20,7 @@ import numpy as np
 import pytest
 
 from softdft_amd import abi, configs
-from softdft_amd.problem import build_state_point
+from softdft_amd.problem import build_state_point, build_state_points
 from oracle.dft_oracle import OracleDFT, anderson_solve
 from parity_helpers import omega_band
 
@@ -79,20 +78,20 @@ def test_config1_full_size_solves():
     b.close()
 
 
-def test_config3_ph_2_7_12_mixed_batch():
+def test_config3_ph_2_7_12_of_the_256_point_mixed_batch():
     ds = configs.config3(256)
     pH = np.linspace(2.0, 12.0, 256)
     idx = [0, int(np.argmin(np.abs(pH - 7.0))), 255]
-    pts = [build_state_point(ds[i]) for i in idx]
+    pts = build_state_points(ds, device_bulk=True)
     b = abi.Batch(pts)
-    print(f"case C3 mixed batch pH={[round(float(pH[i]), 3) for i in idx]}", flush=True)
+    print("case C3 mixed batch of 256 points", flush=True)
     x, it, err = b.solve(b.x0(), precision=abi.PREC_MIXED, **_opts(pts[0].numerics))
     assert np.all(err < TOLE)
     om, al, pc = b.observables()
-    for p, sp in enumerate(pts):
-        print(f"case C3 oracle pH={pH[idx[p]]:.3f}", flush=True)
-        o, r, rb_ref, rbo_ref, om_ref, band = _oracle_solve(sp)
-        _check_point(b, p, o, rb_ref, rbo_ref, om_ref, band, om, al, pc)
+    for i in idx:
+        print(f"case C3 oracle pH={pH[i]:.3f}", flush=True)
+        o, r, rb_ref, rbo_ref, om_ref, band = _oracle_solve(build_state_point(ds[i]))
+        _check_point(b, i, o, rb_ref, rbo_ref, om_ref, band, om, al, pc)
     b.close()
 
 
