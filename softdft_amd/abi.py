@@ -177,7 +177,7 @@ def build_descriptor(points: List[StatePoint]):
                 raise SdftError(-5, f"species {u}: only a grafted first segment is supported")
             fseg.append(0)
             fden.append(fx.density)
-            maxd.extend(list(s.max_distance[:, 0]))
+            maxd.extend(list(s.max_distance[:, 0]) if s.max_distance is not None else [0.0] * s.n_seg)
         else:
             fseg.append(-1)
             fden.append(0.0)

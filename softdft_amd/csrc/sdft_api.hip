@@ -551,7 +551,7 @@ static int build_chain_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const st
     const SpeciesHost& sh = c.species[u];
     ch = ChainPlanHost();
     const int N = sh.nseg, S = sh.S;
-    if (N < 2 || sh.fixed_seg >= 0 || S > SDFT_MAX_S) return SDFT_OK;
+    if (N < 2 || sh.fixed_seg >= 0 || S > SDFT_MAX_S || sh.model == SDFT_MODEL_MONOMERBEAD) return SDFT_OK;
     std::vector<int> ck;
     if (!column_kernels(ctx, sys, u, canon, ck)) return SDFT_OK;
     for (int sw = 0; sw < 2; ++sw)
@@ -622,7 +622,7 @@ static int build_prop_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const std
     Ctx& c = ctx->c;
     const SpeciesHost& sh = c.species[u];
     const int S = sh.S;
-    if (sh.nseg < 2) return SDFT_OK;
+    if (sh.nseg < 2 || sh.model == SDFT_MODEL_MONOMERBEAD) return SDFT_OK;
     std::vector<int> ck;
     if (!column_kernels(ctx, sys, u, canon, ck)) return SDFT_OK;  // not homogeneous: keep the direct kernel
     struct Key { int kset, kern; bool operator<(const Key& o) const { return kset != o.kset ? kset < o.kset : kern < o.kern; } };
@@ -892,13 +892,15 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
             set_error(ctx, SDFT_E_UNSUPPORTED, "species %d: nseg=%d states=%d (at most %d states per segment)", u, N, S, SDFT_MAX_S);
             return fail(SDFT_E_UNSUPPORTED);
         }
-        if (d->species_model[u] == SDFT_MODEL_MONOMERBEAD && N > 1) {
-            set_error(ctx, SDFT_E_UNSUPPORTED, "species %d: multi-bead monomerbead species are not supported", u);
-            return fail(SDFT_E_UNSUPPORTED);
+        const bool mbead = d->species_model[u] == SDFT_MODEL_MONOMERBEAD;
+        if (d->species_model[u] != SDFT_MODEL_FREELY_JOINTED && !mbead) {
+            set_error(ctx, SDFT_E_ARG, "species %d: unknown species model %d", u, d->species_model[u]);
+            return fail(SDFT_E_ARG);
         }
         const int fseg = d->species_fixed_seg ? d->species_fixed_seg[u] : -1;
-        if (fseg > 0 || (fseg == 0 && N < 2)) {
-            set_error(ctx, SDFT_E_UNSUPPORTED, "species %d: only a grafted first segment of a chain is supported", u);
+        if (fseg > 0 || (fseg == 0 && N < 2 && !mbead)) {
+            set_error(ctx, SDFT_E_UNSUPPORTED, "species %d: only a grafted first segment is supported (of a chain, or "
+                      "of a monomerbead species)", u);
             return fail(SDFT_E_UNSUPPORTED);
         }
         DevSpecies& s = spec[u];
@@ -1009,7 +1011,9 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
                 normative *= pow(t, 1.0 / N);
             }
             q.normative[u] = normative;
-            q.chain_norm[u] = pow(normative, (double)(N - 1));
+            // free-chain normalisation normative^(N-1); a free monomerbead species is normalised by 1
+            // (monomerbead/handle_fixed.jl:39-41)
+            q.chain_norm[u] = spec[u].model == SDFT_MODEL_MONOMERBEAD ? 1.0 : pow(normative, (double)(N - 1));
         }
         for (int j = 0; j < NB; ++j) {
             q.wall_lo[j] = d->wall_lo ? d->wall_lo[(int64_t)p * NB + j] : 0;

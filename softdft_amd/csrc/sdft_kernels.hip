@@ -462,6 +462,26 @@ __global__ void __launch_bounds__(256) k_fixed_lambda(const DevSystem* __restric
     }
 }
 
+// grafted monomerbead normalisation (monomerbead/handle_fixed.jl:25-48): lambda = sum over the fixed
+// segment's states of exp(-mu_ex - Ext - delta_muH) at the grafting point (no electrostatic term),
+// from the solution vector in FP64 at either precision; norm = sigma_graft / lambda
+__global__ void k_fixed_lambda_mb(const DevPoint* __restrict__ pts, const int32_t* __restrict__ evalflag, int prec,
+                                  int P_count, int u, const int32_t* __restrict__ states, int S,
+                                  const double* __restrict__ x, const double* __restrict__ Ext, int NB,
+                                  double* __restrict__ fixed_lambda) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P_count || evalflag[p] != prec) return;
+    const DevPoint& P = pts[p];
+    const int K = P.fixed_coord[u];
+    double lam = 0.0;
+    for (int a = 0; a < S; ++a) {
+        const int si = states[a];   // the grafted segment is segment 0
+        if (si < 0) continue;
+        lam += exp(-x[P.xoff + (int64_t)si * P.NP + K] - Ext[P.base * NB + (int64_t)si * P.NP + K] - P.delta_muH[si]);
+    }
+    fixed_lambda[(int64_t)p * SDFT_MAX_SPECIES + u] = lam;
+}
+
 constexpr int ACC_POINTS = SDFT_ACC_POINTS;  // grid points per k_accumulate block (one wavefront wide)
 constexpr int ACC_GROUPS = 4;   // wavefronts per block, each taking every 4th segment
 static_assert(ACC_POINTS * ACC_GROUPS == 256, "k_accumulate runs 256-thread blocks");
@@ -539,6 +559,9 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
         // the bond of each (a, b) link are the same at every segment, so the contributions are summed
         // in registers and added to the LDS accumulators once per species
         const bool hom = sp.homog != 0;
+        // monomerbead (monomerbead/accumulate_densities.jl:46-85): every segment independent,
+        // rho = norm exp(mu_u - lambda), no propagators and no bonds
+        const bool mb = sp.model == SDFT_MODEL_MONOMERBEAD;
         double racc[SDFT_MAX_S], rc0[SDFT_MAX_S][SDFT_MAX_S], rc1[SDFT_MAX_S][SDFT_MAX_S];
 #pragma unroll
         for (int a = 0; a < SDFT_MAX_S; ++a) {
@@ -568,8 +591,8 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                     const bool okc = si >= 0 && b < S && q < N - 1 && st[(q + 1) * S + b] >= 0;
                     const bool okp = si >= 0 && b < S && q > 0 && st[(q - 1) * S + b] >= 0;
                     // a single-bead species has no propagators: no loads (N is wave-uniform)
-                    const T vc = N > 1 ? gC[gidx(q, bc, ac, nc)] : T(0);
-                    const T vp = N > 1 ? gP[gidx(q, bc, ac, nc)] : T(0);
+                    const T vc = (N > 1 && !mb) ? gC[gidx(q, bc, ac, nc)] : T(0);
+                    const T vp = (N > 1 && !mb) ? gP[gidx(q, bc, ac, nc)] : T(0);
                     gcv[b][a] = okc ? vc : T(0);
                     gpv[b][a] = okp ? vp : T(0);
                 }
@@ -579,17 +602,21 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                 const int si = a < S ? st[q * S + a] : -1;
                 if (si < 0) continue;
                 double Gp = 0.0, Qc = 0.0;
-                if (q == 0) Gp = (double)base_value<T>(0, nc, sp.fixed_seg, Kf, P.dz, maxd);
-                else
+                if (mb) {
+                    Gp = Qc = 1.0;
+                } else {
+                    if (q == 0) Gp = (double)base_value<T>(0, nc, sp.fixed_seg, Kf, P.dz, maxd);
+                    else
 #pragma unroll
-                    for (int b = 0; b < SDFT_MAX_S; ++b)
-                        if (b < S && st[(q - 1) * S + b] >= 0) Gp += (double)gpv[b][a];
-                // leaf: sum gC[K, :, 1, idx_i, leaf] is the base case 1 for every state
-                if (q == N - 1) Qc = (double)base_value<T>(q, nc, sp.fixed_seg, Kf, P.dz, maxd);
-                else
+                        for (int b = 0; b < SDFT_MAX_S; ++b)
+                            if (b < S && st[(q - 1) * S + b] >= 0) Gp += (double)gpv[b][a];
+                    // leaf: sum gC[K, :, 1, idx_i, leaf] is the base case 1 for every state
+                    if (q == N - 1) Qc = (double)base_value<T>(q, nc, sp.fixed_seg, Kf, P.dz, maxd);
+                    else
 #pragma unroll
-                    for (int b = 0; b < SDFT_MAX_S; ++b)
-                        if (b < S && st[(q + 1) * S + b] >= 0) Qc += (double)gcv[b][a];
+                        for (int b = 0; b < SDFT_MAX_S; ++b)
+                            if (b < S && st[(q + 1) * S + b] >= 0) Qc += (double)gcv[b][a];
+                }
                 const double el = (double)elv[a];
                 const double contr = pref * el * Gp;
                 const double rs = contr * Qc;
@@ -607,7 +634,7 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                         num += rs * tz * P.dz;
                         chg += rs * tz * P.dz * sys.z[si];
                     }
-                    if (q < N - 1) {
+                    if (q < N - 1 && !mb) {
 #pragma unroll
                         for (int b = 0; b < SDFT_MAX_S; ++b) {
                             const int sj = b < S ? st[(q + 1) * S + b] : -1;
@@ -623,7 +650,7 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                             rbo[(2 * bond + 0) * TB] += contr * (double)gcv[b][a];
                         }
                     }
-                    if (q > 0) {
+                    if (q > 0 && !mb) {
                         const double c1 = pref * el * Qc;
 #pragma unroll
                         for (int b = 0; b < SDFT_MAX_S; ++b) {
@@ -653,7 +680,7 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
 #pragma unroll
                 for (int b = 0; b < SDFT_MAX_S; ++b) {
                     const int sj = b < S ? st[b] : -1;
-                    if (sj < 0 || N < 2) continue;
+                    if (sj < 0 || N < 2 || mb) continue;
                     const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
                     int bond = 0;
                     for (int u2 = 0; u2 < MB; ++u2)
@@ -1565,14 +1592,17 @@ __global__ void __launch_bounds__(256) k_segment_sums(const DevSystem* __restric
         if (si >= 0) {
             for (int n = threadIdx.x; n < NP; n += blockDim.x) {
                 double Gp = 0.0, Qc = 0.0;
-                if (q == 0) Gp = (double)base_value<T>(0, n, sp.fixed_seg, Kf, P.dz, maxd);
-                else
-                    for (int b = 0; b < S; ++b)
-                        if (st[(q - 1) * S + b] >= 0) Gp += (double)gP[gidx(q, b, a, n)];
-                if (q == N - 1) Qc = (double)base_value<T>(q, n, sp.fixed_seg, Kf, P.dz, maxd);
-                else
-                    for (int b = 0; b < S; ++b)
-                        if (st[(q + 1) * S + b] >= 0) Qc += (double)gC[gidx(q, b, a, n)];
+                if (sp.model == SDFT_MODEL_MONOMERBEAD) Gp = Qc = 1.0;  // independent beads, full profiles
+                else {
+                    if (q == 0) Gp = (double)base_value<T>(0, n, sp.fixed_seg, Kf, P.dz, maxd);
+                    else
+                        for (int b = 0; b < S; ++b)
+                            if (st[(q - 1) * S + b] >= 0) Gp += (double)gP[gidx(q, b, a, n)];
+                    if (q == N - 1) Qc = (double)base_value<T>(q, n, sp.fixed_seg, Kf, P.dz, maxd);
+                    else
+                        for (int b = 0; b < S; ++b)
+                            if (st[(q + 1) * S + b] >= 0) Qc += (double)gC[gidx(q, b, a, n)];
+                }
                 acc += pref * (double)elam[fidx(P, NB, si, n)] * Gp * Qc;
             }
         }
@@ -1624,14 +1654,17 @@ __global__ void __launch_bounds__(256) k_segment_profile(const DevSystem* __rest
         double v = 0.0;
         if (si >= 0) {
             double Gp = 0.0, Qc = 0.0;
-            if (q == 0) Gp = (double)base_value<T>(0, n, sp.fixed_seg, Kf, P.dz, maxd);
-            else
-                for (int b = 0; b < S; ++b)
-                    if (st[(q - 1) * S + b] >= 0) Gp += (double)gP[gidx(q, b, a)];
-            if (q == N - 1) Qc = (double)base_value<T>(q, n, sp.fixed_seg, Kf, P.dz, maxd);
-            else
-                for (int b = 0; b < S; ++b)
-                    if (st[(q + 1) * S + b] >= 0) Qc += (double)gC[gidx(q, b, a)];
+            if (sp.model == SDFT_MODEL_MONOMERBEAD) Gp = Qc = 1.0;  // independent beads, full profiles
+            else {
+                if (q == 0) Gp = (double)base_value<T>(0, n, sp.fixed_seg, Kf, P.dz, maxd);
+                else
+                    for (int b = 0; b < S; ++b)
+                        if (st[(q - 1) * S + b] >= 0) Gp += (double)gP[gidx(q, b, a)];
+                if (q == N - 1) Qc = (double)base_value<T>(q, n, sp.fixed_seg, Kf, P.dz, maxd);
+                else
+                    for (int b = 0; b < S; ++b)
+                        if (st[(q + 1) * S + b] >= 0) Qc += (double)gC[gidx(q, b, a)];
+            }
             v = pref * (double)elam[fidx(P, NB, si, n)] * Gp * Qc;
         }
         out[n + (int64_t)NP * (a + (int64_t)S * q)] = v;
@@ -1783,6 +1816,13 @@ void Pipeline<T>::propagate(hipStream_t st, const Ctx& c, int prec, int u, bool 
     A.u = u; A.nseg = sh.nseg; A.S = sh.S; A.fixed_seg = sh.fixed_seg;
     A.state_off = sh.state_off; A.maxd_off = sh.maxd_off; A.goff = sh.goff; A.GF = c.GF;
     A.rescale = sh.rescale; A.nsp = c.nsp; A.nmax = c.nmax_seg; A.smax = smax;
+    if (sh.model == SDFT_MODEL_MONOMERBEAD) {
+        // no propagators; a grafted monomerbead species only needs its normalisation
+        if (sh.fixed_seg >= 0)
+            hipLaunchKernelGGL(k_fixed_lambda_mb, dim3((P + 63) / 64), dim3(64), 0, st, c.d_pts, c.d_evalflag, prec, P,
+                               u, c.d_seg_states + sh.state_off, sh.S, c.d_x, c.d_Ext, c.NB, c.d_fixed_lambda);
+        return;
+    }
     const bool banded = (sizeof(T) == 4) ? c.band_fp32 != 0 : c.band_fp64 != 0;
     A.band = banded ? 1 : -1;
     const bool gemm = c.use_gemm && c.prop_plan[u].ntiles > 0;

@@ -739,3 +739,43 @@ def test_fft_propagation_is_round_off_dominated_in_a_500mer_brush():
     of = od.OracleDFT(sp)
     of.objective(x)
     assert np.max(np.abs(_grafted_segment_integrals(sp, of) - 0.05)) > 1e-2
+
+
+# ----------------------------------------------------------------- monomerbead species
+def test_monomerbead_bulk_consistency_kat():
+    """A wall-free uniform slab with a free three-bead monomerbead species (acid beads with two
+    states) beside the chains: F(x0) = x0 (monomerbead/accumulate_densities.jl:46-85 with
+    handle_fixed.jl norm 1 must reproduce the bulk bead densities exactly)."""
+    ds = configs.weak_polyelectrolyte(H=3.0, dz=0.05, n_seg=4, solvent=0.2, pH=5.0)
+    ds["geometry"]["external_field"] = {}
+    ds["species"].append({"sequence": "AAA", "density": 0.03, "model": "monomerbead"})
+    sp, o = _oracle_for(ds)
+    x0 = sp.x0()
+    fx = o.objective(x0)
+    mu, lng, psi, psic = unflatten(fx - x0, sp.NP, sp.NB, sp.MB)
+    assert np.max(np.abs(mu)) < 1e-13 and np.max(np.abs(lng)) < 1e-13 and np.max(np.abs(psi)) < 1e-13
+    for j in range(sp.NB):
+        assert np.allclose(o.rho_beads[:, j], sp.bulk.rho_beads[j], rtol=1e-12, atol=1e-300)
+
+
+def test_grafted_monomerbead_bead_carries_the_grafting_density():
+    """monomerbead/handle_fixed.jl:25-48: the grafted bead's density at the grafting point is
+    sigma_g exp(-lambda) / sum_states exp(-(mu_ex + Ext + delta_muH)); for an uncharged bead type the
+    electrostatic term is absent from both, so that bead carries exactly the grafting density, and
+    update_spatial_structures! puts it into the bead density as rho / dz / trapez at that point."""
+    ds = configs.hard_sphere_chains(H=6.0, dz=0.05, n_seg=4, rho_poly=0.05, rho_solvent=0.2)
+    ds["species"].append({"sequence": "SS", "density": 0.0, "model": "monomerbead",
+                          "fixed": {"density": 0.04, "segment": [1], "position": ["left"]}})
+    sp, o = _oracle_for(ds)
+    x = sp.x0() + 0.02 * np.random.default_rng(8).standard_normal(sp.n)
+    o.objective(x)
+    u = len(sp.ms.species) - 1
+    Kf = int(sp.fixed[u].coords[0])
+    assert o.rho_seg[u][Kf, 0, 0] == pytest.approx(0.04, rel=1e-13)
+    s = sp.ms.monomers.index("S")
+    # the second (free) bead keeps its full profile; the grafted one adds a delta at Kf
+    other = sum(o.rho_seg[v][:, a, q] for v in range(len(sp.ms.species)) for q in range(sp.ms.species[v].n_seg)
+                for a, si in enumerate(sp.ms.species[v].state_family[q]) if si == s and not (v == u and q == 0))
+    delta = o.rho_beads[:, s] - other
+    assert delta[Kf] == pytest.approx(0.04 / sp.dz / sp.trapez[Kf, s], rel=1e-12)
+    assert np.max(np.abs(np.delete(delta, Kf))) < 1e-14
