@@ -57,3 +57,29 @@ def test_ph_sweep_matches_cold_solves():
     # ionization grows with pH on a weak acid
     assert all(res[k + 1].alpha > res[k].alpha for k in range(len(res) - 1))
     b.close()
+
+
+def test_width_sweep_against_oracle_solves():
+    """Pi(H) = P_cvt - P_bulk and the energy Omega + P_bulk H of a warm-started width sweep
+    (routines/adjust_distanceZ.jl:96-108) against oracle solves of the same widths, every point
+    solved to 1e-11: Pi and the energy within 1e-9 (plus the neutrality band of Omega)."""
+    from oracle.dft_oracle import OracleDFT, anderson_solve
+    from parity_helpers import omega_band
+    Hs = [4.0, 4.5, 5.0]
+    dss = [configs.weak_polyelectrolyte(H=H, dz=0.05, n_seg=4, solvent=0.2, pH=7.0) for H in Hs]
+    num = dict(dss[0]["numerical"])
+    num["tole"] = 1e-11
+    print("case width_sweep vs oracle", flush=True)
+    res = width_sweep(dss, waves=3, precision=abi.PREC_MIXED, numerics=num, max_iters=6000)
+    assert [r.warm for r in res] == [False, True, True]
+    for r, d in zip(res, dss):
+        sp = build_state_point(d)
+        o = OracleDFT(sp)
+        rr = anderson_solve(o.objective, sp.x0(), num, max_iters=20000)
+        om = o.grand_potential(rr.x)
+        band = omega_band(sp, o, rr, om, num)
+        pc = o.contact_pressure()
+        assert r.residual < 1e-11
+        assert abs(r.delta_p - (pc[0] - sp.bulk.pressure)) < 1e-9
+        assert abs(r.energy - (om + sp.bulk.pressure * sp.H)) < 1e-9 * max(1.0, abs(r.energy)) + band
+        assert r.distance == pytest.approx(sp.H)
