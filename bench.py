@@ -343,7 +343,8 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=16, help="host cores of the all-core CPU baseline")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    ap.add_argument("--kernel-only", choices=["chain32", "chain64", "fwd32", "fwd64", "back32", "back64", "acc32", "acc64"],
+    ap.add_argument("--kernel-only", choices=["chain32", "chain64", "fwd32", "fwd64", "back32", "back64", "acc32", "acc64",
+                                              "pw64"],
                     help="after one full-load evaluation, only re-launch this kernel class 21 times "
                          "(for rocprofv3 --kernel-trace / --pmc runs of that kernel)")
     ap.add_argument("--config", choices=["c1", "c2", "c4", "c5"],
@@ -396,12 +397,12 @@ def main():
         # one evaluation of every point in the class's precision with the banded operators, so that
         # every launch of the kernel in this run has all 256 points active (the full-load launch the
         # roofline figure times)
-        prec = abi.PREC_FP32 if args.kernel_only.endswith("32") else abi.PREC_FP64
+        prec = abi.PREC_FP32 if args.kernel_only.endswith("32") else abi.PREC_FP64   # pw64: FP64 fields
         batch.set_operator_band(1, 1)
         batch.objective(x0, prec)
         kc = {"chain32": abi.KP_CHAIN_FP32, "chain64": abi.KP_CHAIN_FP64, "fwd32": abi.KP_CONV_FWD_FP32,
               "fwd64": abi.KP_CONV_FWD_FP64, "back32": abi.KP_CONV_BACK_FP32, "back64": abi.KP_CONV_BACK_FP64,
-              "acc32": abi.KP_ACCUM_FP32, "acc64": abi.KP_ACCUM_FP64}
+              "acc32": abi.KP_ACCUM_FP32, "acc64": abi.KP_ACCUM_FP64, "pw64": abi.KP_POINTWISE_FP64}
         ms, fl = batch.kernel_profile(kc[args.kernel_only], reps=20)
         log(f"{args.kernel_only}: {ms * 1e3:.1f} us/launch, {fl:.4g} algorithmic flops/launch")
         batch.close()
