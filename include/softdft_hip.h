@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define SDFT_ABI_VERSION 3
+#define SDFT_ABI_VERSION 4
 
 /* error codes */
 #define SDFT_OK 0
@@ -228,8 +228,10 @@ int32_t sdft_set_launch_mode(sdft_ctx_t* ctx, int32_t graph, int32_t concurrent,
  *                         per grid-length bin (kernel sets whose NP rounds up to the same multiple of
  *                         256; key = that bound) and convolution stage: the tile shape, choice
  *                         0 = 128 x 16, 1 = 64 x 64, 2 = 128 x 64, 3 = 64 x 32, 4 = 64 x 16 rows x
- *                         columns.  Timed at FP32 and
- *                         FP64 and chosen on the sum.
+ *                         columns of the operator-GEMM, or 5 = batched FFT convolution (mirror
+ *                         lengths 2 NP that are powers of two from 4096 to 65536; elsewhere not a
+ *                         candidate, and a forced 5 falls back to the fixed rule).  Timed at FP32
+ *                         and FP64 and chosen on the sum.
  * us[k] is the measured microseconds per launch sequence of candidate k (< 0: not a candidate).
  * Without a call the library uses fixed rules (chain kernel wherever eligible; 64 x 32 tiles where a
  * grid length's operators average >= 48 columns, else 128 x 16).  The choice changes summation
@@ -244,7 +246,7 @@ typedef struct {
     int32_t key;         /* species index, or the bin's NP bound */
     int32_t precision;   /* 32 / 64 (propagators), 0 (convolutions: FP32 + FP64) */
     int32_t choice;
-    double us[5];
+    double us[6];
 } sdft_tune_entry_t;
 int32_t sdft_autotune(sdft_ctx_t* ctx, int32_t reps, sdft_tune_entry_t* entries, int32_t capacity,
                       int32_t* n_entries);

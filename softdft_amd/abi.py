@@ -18,7 +18,7 @@ from .problem import StatePoint
 LIB_NAME = "libsoftdft_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
-SDFT_ABI_VERSION = 3
+SDFT_ABI_VERSION = 4
 SDFT_OK = 0
 ERRORS = {-1: "SDFT_E_ARG", -2: "SDFT_E_HIP", -3: "SDFT_E_NORMALIZATION", -4: "SDFT_E_NOCONV",
           -5: "SDFT_E_UNSUPPORTED", -6: "SDFT_E_STATE"}
@@ -33,7 +33,7 @@ EXPORTED = ("sdft_abi_version", "sdft_last_error", "sdft_create", "sdft_destroy"
             "sdft_set_launch_mode", "sdft_autotune", "sdft_set_tuning", "sdft_observables_noalloc",
             "sdft_last_switch_iters", "sdft_bulk_solve")
 TUNE_PROPAGATOR, TUNE_CONV_FWD, TUNE_CONV_PRE, TUNE_CONV_BACK = 1, 2, 3, 4
-CONV_TALL, CONV_WIDE64, CONV_WIDE128, CONV_MID, CONV_NARROW = 0, 1, 2, 3, 4
+CONV_TALL, CONV_WIDE64, CONV_WIDE128, CONV_MID, CONV_NARROW, CONV_FFT = 0, 1, 2, 3, 4, 5
 KP_PROP_FP32, KP_PROP_FP64, KP_CONV_FWD_FP32, KP_CONV_FWD_FP64, KP_CONV_BACK_FP32, KP_CONV_BACK_FP64, \
     KP_POINTWISE_FP64, KP_CHAIN_FP32, KP_CHAIN_FP64, KP_ACCUM_FP32, KP_ACCUM_FP64 = range(11)
 
@@ -66,7 +66,7 @@ class SdftSolverOpts(C.Structure):
 
 class SdftTuneEntry(C.Structure):
     _fields_ = [("kind", C.c_int32), ("key", C.c_int32), ("precision", C.c_int32), ("choice", C.c_int32),
-                ("us", C.c_double * 5)]
+                ("us", C.c_double * 6)]
 
 
 class SdftError(RuntimeError):
@@ -356,6 +356,13 @@ class Batch:
             arr[i].precision = e.get("precision", 0)
             arr[i].choice = e["choice"]
         self._check(self.lib.sdft_set_tuning(self.h, arr, len(entries)))
+
+    def set_conv_shape(self, choice: int):
+        """Force one convolution path for every grid length and stage: an operator-GEMM tile shape
+        (CONV_TALL ... CONV_NARROW) or the batched FFT (CONV_FFT, taken where the mirror length 2 NP
+        is a power of two from 4096 to 65536; elsewhere the fixed rule applies); -1 restores the
+        fixed rule."""
+        self.set_tuning([dict(kind=k, key=0, choice=choice) for k in (TUNE_CONV_FWD, TUNE_CONV_PRE, TUNE_CONV_BACK)])
 
     def set_launch_mode(self, graph: Optional[bool] = None, concurrent: Optional[bool] = None) -> int:
         """Launch mode of solve(): graph windows (True, default) or kernel-by-kernel launches; FP32 and

@@ -363,6 +363,90 @@ static std::vector<OpTile> xcd_order(const std::vector<OpTile>& tiles) {
     return out;  // a multiple of 8 long: split launches (ID = tile + ntiles * part) keep parts on one XCD
 }
 
+// distinct input fields (infam, infield) of a job range, in first-use order
+static std::vector<FftSlot> stage_slots(const std::vector<ConvJob>& jobs, int job_base, int njobs) {
+    std::vector<FftSlot> v;
+    for (int j = 0; j < njobs; ++j) {
+        const ConvJob& J = jobs[job_base + j];
+        for (int t = 0; t < J.nterms; ++t) {
+            bool seen = false;
+            for (const FftSlot& q : v) seen = seen || (q.infam == J.t[t].infam && q.infield == J.t[t].infield);
+            if (!seen) v.push_back(FftSlot{J.t[t].infam, J.t[t].infield});
+        }
+    }
+    return v;
+}
+
+// the FFT path takes a kernel set of a stage when its mirror length is a power of two in range
+// and the stage's input spectra fit the middle pass's LDS
+static bool fft_stage_ok(int L, int nslot) {
+    if (!fft_length_ok(L) || nslot < 1) return false;
+    int L1, L2;
+    fft_split(L, &L1, &L2);
+    return fft_mid_lds_bytes(nslot, L1, sizeof(double)) <= 160 * 1024;
+}
+
+// kernel spectra and twiddles of kernel set s (once per context)
+static int fft_prepare_set(sdft_ctx* ctx, int s, int L) {
+    Ctx& c = ctx->c;
+    if ((int)c.fft_res.size() < ctx->nsets) c.fft_res.resize(ctx->nsets);
+    FftRes& r = c.fft_res[s];
+    if (r.L == L) return SDFT_OK;
+    const int64_t H1 = fft_spectrum_len(L);
+    double* W64 = nullptr;
+    float* W32 = nullptr;
+    double* ch64 = nullptr;
+    float* ch32 = nullptr;
+    double* tmp = nullptr;
+    TRY(dalloc(ctx, &W64, 2 * (int64_t)L));
+    TRY(dalloc(ctx, &W32, 2 * (int64_t)L));
+    TRY(dalloc(ctx, &ch64, 2 * H1 * c.K));
+    TRY(dalloc(ctx, &ch32, 2 * H1 * c.K));
+    TRY(dalloc(ctx, &tmp, 2 * H1 * c.K));
+    launch_fft_twiddles(ctx->stream, L, W64, W32);
+    launch_fft_kernel_spectra(ctx->stream, L, c.K, ctx->p64.kern + ctx->set_kbase_h[s] * c.K, W64, tmp, ch64, ch32);
+    HIP_CHECK_RET(hipGetLastError());
+    dfree(ctx, tmp);  // synchronises the stream first
+    r.L = L;
+    r.W64 = W64;
+    r.W32 = W32;
+    r.chat64 = ch64;
+    r.chat32 = ch32;
+    return SDFT_OK;
+}
+
+// spectra work buffers of both precisions for n_in input and n_out output spectra of length H1
+static int fft_reserve(sdft_ctx* ctx, int64_t n_in, int64_t n_out) {
+    Ctx& c = ctx->c;
+    if (n_in > c.fft_sa_n) {
+        for (int w = 0; w < 2; ++w) {
+            dfree(ctx, c.fft_sa[w]);
+            c.fft_sa[w] = nullptr;
+        }
+        float* a32 = nullptr;
+        double* a64 = nullptr;
+        TRY(dalloc(ctx, &a32, 2 * n_in));
+        TRY(dalloc(ctx, &a64, 2 * n_in));
+        c.fft_sa[0] = a32;
+        c.fft_sa[1] = a64;
+        c.fft_sa_n = n_in;
+    }
+    if (n_out > c.fft_sb_n) {
+        for (int w = 0; w < 2; ++w) {
+            dfree(ctx, c.fft_sb[w]);
+            c.fft_sb[w] = nullptr;
+        }
+        float* b32 = nullptr;
+        double* b64 = nullptr;
+        TRY(dalloc(ctx, &b32, 2 * n_out));
+        TRY(dalloc(ctx, &b64, 2 * n_out));
+        c.fft_sb[0] = b32;
+        c.fft_sb[1] = b64;
+        c.fft_sb_n = n_out;
+    }
+    return SDFT_OK;
+}
+
 // Convolution plan of one stage: columns (point, job) grouped by (kernel set, operator terms), each
 // group tiled in the shape of its kernel set (set_shape[kset]; < 0: the fixed rule, 64 x 32 tiles
 // when the set's groups average >= 48 columns, else 128 x 16). Points with point_mask[p] == 0 are
@@ -408,8 +492,11 @@ static int build_conv_plan(sdft_ctx* ctx, const std::vector<ConvJob>& jobs, int 
         e.first += kv.second.size();
         e.second += 1;
     }
+    const std::vector<FftSlot> slots = stage_slots(jobs, job_base, njobs);
     auto shape_of = [&](int kset) {
-        if (kset < (int)set_shape.size() && set_shape[kset] >= 0) return set_shape[kset];
+        if (kset < (int)set_shape.size() && set_shape[kset] >= 0 &&
+            (set_shape[kset] != CONV_FFT || fft_stage_ok(set_L[kset], (int)slots.size())))
+            return set_shape[kset];
         const auto& e = set_cols[kset];
         return e.first >= 48 * e.second ? (int)CONV_MID : (int)CONV_TALL;
     };
@@ -418,6 +505,60 @@ static int build_conv_plan(sdft_ctx* ctx, const std::vector<ConvJob>& jobs, int 
         plan = OpPlan();
         plan.shape = sh;
         plan.wide = sh != CONV_TALL;
+        if (sh == CONV_FFT) {
+            // per kernel set: its points, the stage's input slots and one output column per job
+            std::map<int, std::pair<std::vector<int32_t>, std::vector<FftJob>>> sets;
+            for (auto& kv : groups) {
+                const Key& k = kv.first;
+                if (shape_of(k.kset) != CONV_FFT) continue;
+                auto& e = sets[k.kset];
+                for (const OpCol& col : kv.second) {
+                    if (std::find(e.first.begin(), e.first.end(), col.p) == e.first.end()) e.first.push_back(col.p);
+                    bool seen = false;
+                    for (const FftJob& fj : e.second) seen = seen || fj.job == col.job;
+                    if (seen) continue;
+                    FftJob fj;
+                    memset(&fj, 0, sizeof(fj));
+                    fj.job = col.job;
+                    fj.nterms = k.nterms;
+                    const ConvJob& J = jobs[col.job];
+                    for (int t = 0; t < k.nterms; ++t) {
+                        fj.kern[t] = k.kern[t];
+                        fj.sign[t] = k.sign[t];
+                        for (size_t q = 0; q < slots.size(); ++q)
+                            if (slots[q].infam == J.t[t].infam && slots[q].infield == J.t[t].infield) fj.slot[t] = (int)q;
+                    }
+                    e.second.push_back(fj);
+                }
+            }
+            for (auto& kv : sets) {
+                FftSetPlan f;
+                f.kset = kv.first;
+                f.L = set_L[kv.first];
+                fft_split(f.L, &f.L1, &f.L2);
+                f.lg1 = 0;
+                while ((1 << f.lg1) < f.L1) ++f.lg1;
+                f.lg2 = 0;
+                while ((1 << f.lg2) < f.L2) ++f.lg2;
+                f.H = f.L2 / 2 + 1;
+                f.R = 256 / f.L1;
+                f.npts = (int)kv.second.first.size();
+                f.nslot = (int)slots.size();
+                f.njob = (int)kv.second.second.size();
+                void* dp = nullptr;
+                TRY(upload_vec(ctx, &dp, kv.second.first));
+                f.d_pts = (int32_t*)dp;
+                TRY(upload_vec(ctx, &dp, slots));
+                f.d_slots = (FftSlot*)dp;
+                TRY(upload_vec(ctx, &dp, kv.second.second));
+                f.d_jobs = (FftJob*)dp;
+                TRY(fft_prepare_set(ctx, f.kset, f.L));
+                const int64_t H1 = fft_spectrum_len(f.L);
+                TRY(fft_reserve(ctx, (int64_t)f.npts * f.nslot * H1, (int64_t)f.npts * f.njob * H1));
+                plan.fft.push_back(f);
+            }
+            continue;
+        }
         const int TM = conv_tile_rows(sh), TN = conv_tile_cols(sh);
         std::vector<OpGroup> gv;
         std::vector<OpCol> cols;
@@ -462,8 +603,13 @@ static int build_conv_plan(sdft_ctx* ctx, const std::vector<ConvJob>& jobs, int 
 // releases the device tables of a stage's plans (autotuner candidates, replaced plans)
 static void free_stage(sdft_ctx* ctx, Ctx::ConvStage& stage) {
     for (OpPlan& pl : stage.sub) {
-        void* ptrs[7] = {pl.d_tiles, pl.d_groups, pl.d_cols, pl.d_cols_act[0], pl.d_cols_act[1], pl.d_count[0],
-                         pl.d_count[1]};
+        std::vector<void*> ptrs = {pl.d_tiles, pl.d_groups, pl.d_cols, pl.d_cols_act[0], pl.d_cols_act[1],
+                                   pl.d_count[0], pl.d_count[1]};
+        for (const FftSetPlan& f : pl.fft) {
+            ptrs.push_back(f.d_pts);
+            ptrs.push_back(f.d_slots);
+            ptrs.push_back(f.d_jobs);
+        }
         for (void* q : ptrs) {
             if (!q) continue;
             auto it = std::find(ctx->allocs.begin(), ctx->allocs.end(), q);
@@ -1090,7 +1236,8 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     if ((rc = dalloc(ctx, &ctx->p64.kern, c.sumL_sets * c.K)) != SDFT_OK) return fail(rc);
     if ((rc = dalloc(ctx, &ctx->p32.kern, c.sumL_sets * c.K)) != SDFT_OK) return fail(rc);
     {
-        const hipError_t e = prepare_kernel_attributes(c);
+        hipError_t e = prepare_kernel_attributes(c);
+        if (e == hipSuccess) e = fft_prepare_attributes();
         if (e != hipSuccess) {
             set_error(ctx, SDFT_E_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
             return fail(SDFT_E_HIP);
@@ -1697,7 +1844,7 @@ static int autotune_run(sdft_ctx* ctx, int reps, hipEvent_t e0, hipEvent_t e1) {
             sdft_tune_entry_t e;
             e.kind = SDFT_TUNE_PROPAGATOR; e.key = u; e.precision = prec; e.choice = best;
             for (int mode = 0; mode < 3; ++mode) e.us[mode] = t[mode];
-            e.us[3] = e.us[4] = -1.0;
+            e.us[3] = e.us[4] = e.us[5] = -1.0;
             ctx->tuning.push_back(e);
         }
     }
@@ -1718,7 +1865,15 @@ static int autotune_run(sdft_ctx* ctx, int reps, hipEvent_t e0, hipEvent_t e1) {
         for (int k = 0; k < 3; ++k) {
             if (njob[k] == 0) continue;
             double t[CONV_NSHAPES];
+            const int nslot = (int)stage_slots(ctx->jobs_h, base[k], njob[k]).size();
+            bool fft_ok = true;  // FFT candidate: every kernel set of the bin takes the FFT path
+            for (int s = 0; s < ctx->nsets; ++s)
+                if (set_bin[s] == bin.first && !fft_stage_ok(ctx->set_L_h[s], nslot)) fft_ok = false;
             for (int sh = 0; sh < CONV_NSHAPES; ++sh) {
+                if (sh == CONV_FFT && !fft_ok) {
+                    t[sh] = -1.0;
+                    continue;
+                }
                 const std::vector<int> forced(ctx->nsets, sh);
                 Ctx::ConvStage tmp;
                 TRY(build_conv_plan(ctx, ctx->jobs_h, base[k], njob[k], ctx->canon, forced, &bin.second,
@@ -1744,7 +1899,7 @@ static int autotune_run(sdft_ctx* ctx, int reps, hipEvent_t e0, hipEvent_t e1) {
             }
             int best = 0;
             for (int sh = 1; sh < CONV_NSHAPES; ++sh)
-                if (t[sh] < t[best]) best = sh;
+                if (t[sh] >= 0.0 && t[sh] < t[best]) best = sh;
             for (int s = 0; s < ctx->nsets; ++s)
                 if (set_bin[s] == bin.first) ctx->set_shape[k][s] = best;
             sdft_tune_entry_t e;

@@ -39,7 +39,36 @@ enum ConvShape : int {
     CONV_WIDE128 = 2,  // 128 x 64 (k_convgemm; FP64 runs it as two 64-row workgroups per tile)
     CONV_MID = 3,      // 64 x 32 (k_convgemm): twice the tiles of 64 x 64, fewer registers
     CONV_NARROW = 4,   // 64 x 16 (k_convgemm)
-    CONV_NSHAPES = 5
+    CONV_FFT = 5,      // batched FFT convolution (sdft_fft.hip): power-of-two mirror lengths 4096..65536
+    CONV_NSHAPES = 6
+};
+
+// FFT convolution of one kernel set in one stage (sdft_fft.hip). The mirror length L = 2 NP is
+// split as L = L1 x L2 (L2 = 256); spectra are kept as H = L2/2 + 1 rows of L1 values (the other
+// rows follow from the Hermitian symmetry of real signals).
+struct FftSlot { int32_t infam, infield; };   // one distinct input field of the stage's jobs
+struct FftJob {                               // one output column of every point of the set
+    int32_t job;                              // index into the job table (ConvJob: output semantics)
+    int32_t nterms;
+    int32_t kern[3];                          // canonical kernel index within the set
+    int32_t sign[3];                          // sign of the Toeplitz part of each term's operator
+    int32_t slot[3];                          // input slot of each term
+};
+struct FftSetPlan {
+    int kset = 0, L = 0, L1 = 0, L2 = 0, lg1 = 0, lg2 = 0, H = 0, R = 0;
+    int npts = 0, nslot = 0, njob = 0;
+    int32_t* d_pts = nullptr;                 // [npts] points of the set
+    FftSlot* d_slots = nullptr;               // [nslot]
+    FftJob* d_jobs = nullptr;                 // [njob]
+};
+// per kernel set: the FFT of every kernel vector ([K][H][L1] complex) and the twiddles
+// W[m] = exp(-2 pi i m / L), both precisions (prepared when a plan first uses the set)
+struct FftRes {
+    int L = 0;
+    void* chat64 = nullptr;
+    void* chat32 = nullptr;
+    void* W64 = nullptr;
+    void* W32 = nullptr;
 };
 
 struct OpPlan {
@@ -55,6 +84,7 @@ struct OpPlan {
     // evaluates at that precision, compacted in order, and their number per group
     void* d_cols_act[2] = {nullptr, nullptr};
     int32_t* d_count[2] = {nullptr, nullptr};
+    std::vector<FftSetPlan> fft;   // shape CONV_FFT: one entry per kernel set (no tiles or groups)
 };
 
 // one (a, b) column of one step of the chain kernel (a: state of the output segment, b: state of the
@@ -142,6 +172,12 @@ struct Ctx {
     int32_t* d_n_active = nullptr;  // [active, active at FP32, waiting for the FP64 switch]
     int nmax_seg = 1;               // longest species (propagator scale tables)
     bool any_rescale = false;       // some species propagates with per-step rescaling
+    // FFT convolutions: per kernel set resources, and the spectra work buffers per precision
+    // (0: FP32, 1: FP64; complex elements: inputs and outputs of the largest FFT stage)
+    std::vector<FftRes> fft_res;
+    void* fft_sa[2] = {nullptr, nullptr};
+    void* fft_sb[2] = {nullptr, nullptr};
+    int64_t fft_sa_n = 0, fft_sb_n = 0;
 };
 
 // Per-step propagator scale factors (species with DevSpecies::rescale): smax holds, per
@@ -222,6 +258,18 @@ struct Pipeline {
 };
 
 void launch_compact_cols(hipStream_t st, const Ctx& c, int prec);
+// FFT convolutions (sdft_fft.hip)
+bool fft_length_ok(int L);
+size_t fft_mid_lds_bytes(int nslot, int L1, size_t elem);
+hipError_t fft_prepare_attributes();
+void fft_split(int L, int* L1, int* L2);
+int64_t fft_spectrum_len(int L);   // complex elements of one stored spectrum (H x L1)
+void launch_fft_twiddles(hipStream_t st, int L, void* W64, void* W32);
+void launch_fft_kernel_spectra(hipStream_t st, int L, int K, const double* c64, const void* W64, void* tmp,
+                               void* chat64, void* chat32);
+template <typename T>
+void launch_fft_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, const T* rhoT, const T* dphi,
+                     T* nw, double* fx);
 template <typename T>
 void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, const T* kern, const T* rhoT,
                         const T* dphi, T* nw, double* fx);

@@ -586,6 +586,10 @@ __global__ void __launch_bounds__(256, (sizeof(T) == 8 && WM == 1 && WN == 2) ? 
 template <typename T>
 void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, const T* kern, const T* rhoT,
                         const T* dphi, T* nw, double* fx) {
+    if (plan.shape == CONV_FFT) {
+        launch_fft_conv<T>(st, c, prec, plan, rhoT, dphi, nw, fx);
+        return;
+    }
     if (plan.ntiles == 0) return;
     PropStep ps;
     memset(&ps, 0, sizeof(ps));

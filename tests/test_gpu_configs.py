@@ -6,7 +6,7 @@
   C4  ragged slit widths at dz 0.02: FP64 objective on NP 201, 205, 1100, 2001 in one batch; the two
       narrowest widths solved, with Pi(H) = P_cvt - P_bulk
   C5  500-mer brush, dz 0.005, NP 16384: FP64 objectives at the bulk guess at pH 3 / 1 mM and pH 9 /
-      100 mM; the converged pH 3 / 1 mM brush is a fixed point of the oracle's map, and FP32 at that
+      100 mM, with the operator-GEMM and with the FFT convolutions; the converged pH 3 / 1 mM brush is a fixed point of the oracle's map, and FP32 at that
       state is within 5e-5
 
 Tolerances: FP64 objective 1e-11 relative to max|F|, FP32 5e-5; converged solves (both sides to
@@ -151,6 +151,14 @@ def test_config5_full_grid_objective(pH, salt):
     scale = max(1.0, float(np.max(np.abs(ref))))
     assert np.max(np.abs(f64 - ref)) / scale < 1e-11
     assert np.max(np.abs(f32 - ref)) / scale < 2e-3
+    # the same evaluation with the batched FFT convolutions (mirror length 32768)
+    b.set_conv_shape(abi.CONV_FFT)
+    print(f"case C5 FFT objective pH={pH} salt={salt}", flush=True)
+    g64 = b.objective(x0, abi.PREC_FP64)
+    g32 = b.objective(x0, abi.PREC_FP32)
+    assert np.max(np.abs(g64[:-1] - ref[:-1])) / scale < 1e-11
+    assert abs(g64[-1] - ref[-1]) < 1e-10
+    assert np.max(np.abs(g32 - ref)) / scale < 2e-3
     b.close()
 
 
