@@ -293,15 +293,24 @@ def report_c5(args):
     log(f"c5: {len(sps)} state points set up in {time.perf_counter() - t0:.1f} s")
     b = abi.Batch(sps)
     x0 = b.x0()
-    res = {}
-    for prec, label in ((abi.PREC_FP32, "fp32"), (abi.PREC_FP64, "fp64_banded")):
-        b.set_operator_band(1, 1)
-        b.objective(x0, prec)
-        t0 = time.perf_counter()
-        for _ in range(3):
+    res, conv_us = {}, {}
+    b.set_operator_band(1, 1)
+    # convolutions as dense operator-GEMMs (fixed rule) and as batched FFTs (mirror length 32768)
+    for conv, shape in (("gemm", -1), ("fft", abi.CONV_FFT)):
+        b.set_conv_shape(shape)
+        for prec, label in ((abi.PREC_FP32, "fp32"), (abi.PREC_FP64, "fp64_banded")):
             b.objective(x0, prec)
-        res[label] = (time.perf_counter() - t0) / 3
-        log(f"c5 {label}: {res[label] * 1e3:.1f} ms per batch evaluation")
+            t0 = time.perf_counter()
+            for _ in range(3):
+                b.objective(x0, prec)
+            res[f"{label}_{conv}"] = (time.perf_counter() - t0) / 3
+            log(f"c5 {label} ({conv} convolutions): {res[f'{label}_{conv}'] * 1e3:.1f} ms per batch evaluation")
+            for kc, stage in (((abi.KP_CONV_FWD_FP32, abi.KP_CONV_FWD_FP64), "forward"),
+                              ((abi.KP_CONV_BACK_FP32, abi.KP_CONV_BACK_FP64), "back")):
+                ms, _ = b.kernel_profile(kc[0] if prec == abi.PREC_FP32 else kc[1], reps=5)
+                conv_us[f"{label}_{conv}_{stage}"] = ms * 1e3
+        log(f"c5 convolution stages (us per launch sequence): {conv_us}")
+    res["fp32"], res["fp64_banded"] = res["fp32_fft"], res["fp64_banded_fft"]
     b.close()
     b1 = abi.Batch([sps[0]])
     b1.objective(b1.x0(), abi.PREC_FP64)
@@ -309,14 +318,16 @@ def report_c5(args):
     b1.objective(b1.x0(), abi.PREC_FP64)
     t_plain = time.perf_counter() - t0
     b1.close()
-    # one converged full-size solve: pH 3 at 1 mM (the first point), mixed precision
+    # one converged full-size solve: pH 3 at 1 mM (the first point), mixed precision, FFT convolutions
     bs = abi.Batch([sps[0]])
+    bs.set_conv_shape(abi.CONV_FFT)
     x, it, err, ts = _timed_solve(bs, bs.x0(), abi.PREC_MIXED, _solve_opts(sps[0].numerics, max_iters=3000))
     om, al, _ = bs.observables()
     bs.close()
     per_point_fp32 = res["fp32"] / len(sps)
     out = {"report": "config", "config": "C5", "points": len(sps), "NP": sps[0].NP, "n_seg": 500,
-           "batch_evaluation_ms": {"fp32": res["fp32"] * 1e3, "fp64_banded": res["fp64_banded"] * 1e3},
+           "batch_evaluation_ms": {k: v * 1e3 for k, v in res.items()},
+           "convolution_stage_us": conv_us,
            "batch_iterations_per_s": {"fp32": 1.0 / res["fp32"], "fp64_banded": 1.0 / res["fp64_banded"]},
            "per_point_evaluation_ms": {"fp32_in_batch": per_point_fp32 * 1e3,
                                        "fp64_banded_in_batch": res["fp64_banded"] / len(sps) * 1e3,
@@ -329,6 +340,25 @@ def report_c5(args):
                    "at pH >= 5.8 (100 mM) the residual stalls at O(1-10) for the library and the oracle alike, "
                    "so converged points/s is not quoted for the whole 128-point sweep"}
     print(json.dumps(_finite(out)), flush=True)
+
+
+def c5_fft_kernels(which):
+    """One FFT convolution stage of the C5 batch (128 brushes, NP 16384, mirror length 32768): one
+    evaluation of every point in the stage's precision, then 21 launch sequences of the stage
+    (1 warm + 20 timed with HIP events on the library's stream)."""
+    from softdft_amd import abi, configs
+    from softdft_amd.problem import build_state_points
+    sps = build_state_points(configs.config5(), device_bulk=True)
+    b = abi.Batch(sps)
+    b.set_operator_band(1, 1)
+    b.set_conv_shape(abi.CONV_FFT)
+    prec = abi.PREC_FP32 if which.endswith("32") else abi.PREC_FP64
+    b.objective(b.x0(), prec)
+    kc = {"c5fft_fwd32": abi.KP_CONV_FWD_FP32, "c5fft_fwd64": abi.KP_CONV_FWD_FP64,
+          "c5fft_back32": abi.KP_CONV_BACK_FP32, "c5fft_back64": abi.KP_CONV_BACK_FP64}[which]
+    ms, _ = b.kernel_profile(kc, reps=20)
+    log(f"{which}: {ms * 1e3:.1f} us per launch sequence (3 kernels), {len(sps)} points, NP {sps[0].NP}")
+    b.close()
 
 
 def main():
@@ -344,9 +374,10 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--kernel-only", choices=["chain32", "chain64", "fwd32", "fwd64", "back32", "back64", "acc32", "acc64",
-                                              "pw64"],
+                                              "pw64", "c5fft_fwd32", "c5fft_fwd64", "c5fft_back32", "c5fft_back64"],
                     help="after one full-load evaluation, only re-launch this kernel class 21 times "
-                         "(for rocprofv3 --kernel-trace / --pmc runs of that kernel)")
+                         "(for rocprofv3 --kernel-trace / --pmc runs of that kernel); c5fft_*: the FFT "
+                         "convolution stages of the 128-point 500-mer brush batch (NP 16384)")
     ap.add_argument("--config", choices=["c1", "c2", "c4", "c5"],
                     help="per-config report (one JSON line) instead of the benchmark line")
     ap.add_argument("--solve-only", action="store_true",
@@ -360,6 +391,9 @@ def main():
                                    "communication, but only the single-GPU figure is measured"}))
         sys.exit(2)
 
+    if args.kernel_only and args.kernel_only.startswith("c5fft_"):
+        c5_fft_kernels(args.kernel_only)
+        return
     if args.config:
         from softdft_amd import configs
         if args.config == "c1":

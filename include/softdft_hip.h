@@ -233,8 +233,9 @@ int32_t sdft_set_launch_mode(sdft_ctx_t* ctx, int32_t graph, int32_t concurrent,
  *                         candidate, and a forced 5 falls back to the fixed rule).  Timed at FP32
  *                         and FP64 and chosen on the sum.
  * us[k] is the measured microseconds per launch sequence of candidate k (< 0: not a candidate).
- * Without a call the library uses fixed rules (chain kernel wherever eligible; 64 x 32 tiles where a
- * grid length's operators average >= 48 columns, else 128 x 16).  The choice changes summation
+ * Without a call the library uses fixed rules (chain kernel wherever eligible; the FFT convolution
+ * where the mirror length allows it, else 64 x 32 tiles where a grid length's operators average >= 48
+ * columns, else 128 x 16).  The choice changes summation
  * orders (FP64 results move in the last bits) and the banded operator's padding, never the algorithm.
  * Call it before sdft_solve / sdft_objective: it overwrites the work buffers. */
 #define SDFT_TUNE_PROPAGATOR 1

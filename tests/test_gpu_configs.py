@@ -144,6 +144,7 @@ def test_config5_full_grid_objective(pH, salt):
     sp = _c5(pH, salt)
     b = abi.Batch([sp])
     x0 = b.x0()
+    b.set_conv_shape(abi.CONV_MID)  # the operator-GEMM (the fixed rule takes the FFT at this length)
     print(f"case C5 objective pH={pH} salt={salt}", flush=True)
     f64 = b.objective(x0, abi.PREC_FP64)
     f32 = b.objective(x0, abi.PREC_FP32)
