@@ -249,13 +249,16 @@ __global__ void __launch_bounds__(256) k_fft_mid(FftMidArgs<T> a, const int32_t*
     fft_rows<T, false>(xr, xi, twr, twi, a.lg1, a.nslot * a.R);
     cpx<T>* out = a.sb + (int64_t)blockIdx.y * a.njob * a.H * a.L1;
     for (int j = 0; j < a.njob; ++j) {
-        const FftJob J = a.jobs[j];
+        const FftJob& J = a.jobs[j];  // read in place: term fields at constant offsets (no private copy)
+        const int nterms = J.nterms;
         for (int e = threadIdx.x; e < RL; e += 256) {
             const int r = e >> a.lg1, k1 = e & (a.L1 - 1), k2 = k2_0 + r;
             T accr = T(0), acci = T(0);
             if (k2 < a.H) {
                 const cpx<T> w = a.W[k2 + a.L2 * k1];
-                for (int t = 0; t < J.nterms; ++t) {
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    if (t >= nterms) break;
                     const int q = J.slot[t] * RL + r * a.L1 + k1;
                     const T Xr = xr[q], Xi = xi[q];
                     const T sg = J.sign[t] < 0 ? T(-1) : T(1);
