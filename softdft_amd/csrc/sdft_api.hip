@@ -715,7 +715,7 @@ static int build_chain_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const st
     ch.S = S;
     for (int p = 0; p < c.P; ++p) {
         const int NP = ctx->pts[p].NP;
-        const int Be = (band[ctx->pts[p].kset] + 7) / 16 * 16 + 8;  // the chain kernel's half-width (8 mod 16)
+        const int Be = chain_half_width(band[ctx->pts[p].kset]);  // the chain kernel's half-width
         if (2 * Be + 1 > NP) return SDFT_OK;
         ch.npad_max = std::max(ch.npad_max, (NP + 255) / 256 * 256);
         ch.be_max = std::max(ch.be_max, Be);
@@ -1299,7 +1299,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         for (auto& v : ctx->set_shape) v.assign(ctx->nsets, -1);
         if ((rc = rebuild_conv_plans(ctx)) != SDFT_OK) return fail(rc);
         // banded propagator steps on k_bandstep: every point's applied half-width (the band rounded up
-        // to 8 mod 16) fits its grid, and the workgroup's operator windows and inputs fit 64 KB of LDS
+        // to even) fits its grid, and the workgroup's operator windows and inputs fit 64 KB of LDS
         for (int u = 0; u < nsp; ++u) {
             SpeciesHost& sh = c.species[u];
             sh.band_step = 0;
@@ -1307,7 +1307,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
             if (sh.nseg < 2 || sh.S > SDFT_MAX_S) continue;
             bool ok = true;
             for (int p = 0; p < P; ++p) {
-                const int Be = (band[ctx->pts[p].kset] + 7) / 16 * 16 + 8;
+                const int Be = chain_half_width(band[ctx->pts[p].kset]);
                 ok = ok && 2 * Be + 1 <= ctx->pts[p].NP;
                 sh.band_be = std::max(sh.band_be, Be);
             }

@@ -53,8 +53,6 @@ struct ChainArgs {
 
 __device__ __forceinline__ int chain_phys(int i) { return i + (i >> 4); }  // 1 pad word per 16: conflict-free B reads
 
-// applied half-width: the smallest Be >= B with Be = 8 (mod 16), so that K / 16 = Be / 8 + 1 is even
-__host__ __device__ __forceinline__ int chain_half_width(int B) { return (B + 7) / 16 * 16 + 8; }
 
 constexpr int CHAIN_LDS_SLACK = 64;  // elements after the carve-out: the operand prefetch may read one group past the end
 
@@ -78,7 +76,7 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
     const int NP = P.NP, L = P.L;
     const int B = band[P.kset];
     const int Be = chain_half_width(B);
-    const int KS = Be / 2 + 4;                 // K / 4 with K = 2 Be + 16: an even number of 4-step groups
+    const int KS = Be / 2 + 4;                 // k-steps: K / 4 with K = 2 Be + 16
     const int ntile = (NP + 255) >> 8;
     const int XL = ntile * 256 + 2 * Be;       // extended input length of this point
     const int S = ca.S, C = S * S, N = ca.nseg;
@@ -263,20 +261,27 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
                         bv[k4] = xp[17 * K + 4 * k4];
                     }
                 };
-                const int KG = KS / 4;  // even
+                const int KG = KS / 4;  // full 4-step groups, then KS - 4 KG single k-steps
                 fetch(0, a0, b0);
-                for (int K = 0; K < KG; K += 2) {
+                int K = 0;
+                for (; K + 1 < KG; K += 2) {
                     fetch(K + 1, a1, b1);
                     __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of the products
 #pragma unroll
                     for (int k4 = 0; k4 < 4; ++k4) acc4[k4 % NACC] = Mfma<T>::mma(a0[k4], b0[k4], acc4[k4 % NACC]);
                     __builtin_amdgcn_sched_barrier(0);
-                    fetch(K + 2, a0, b0);  // past the end on the last pass: read from the slack, unused
+                    fetch(K + 2, a0, b0);  // at most one group past the last: within the LDS carve-out, unused
                     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int k4 = 0; k4 < 4; ++k4) acc4[k4 % NACC] = Mfma<T>::mma(a1[k4], b1[k4], acc4[k4 % NACC]);
                     __builtin_amdgcn_sched_barrier(0);
                 }
+                if (K < KG) {  // odd group count: the last group, fetched above
+#pragma unroll
+                    for (int k4 = 0; k4 < 4; ++k4) acc4[k4 % NACC] = Mfma<T>::mma(a0[k4], b0[k4], acc4[k4 % NACC]);
+                }
+                for (int ks = 4 * KG; ks < KS; ++ks)
+                    acc4[0] = Mfma<T>::mma(ap[4 * ks], xp[17 * (ks >> 2) + 4 * (ks & 3)], acc4[0]);
                 typename Mfma<T>::acc_t acc = acc4[0];
 #pragma unroll
                 for (int k4 = 1; k4 < NACC; ++k4)
@@ -430,7 +435,7 @@ __global__ void __launch_bounds__(256) k_bandstep(const DevSystem* __restrict__ 
     if (si < 0) return;
     const int Be = chain_half_width(band[P.kset]);
     const int KC = 2 * Be + 32, XL = BAND_OUT + 2 * Be, XLP = XL + (XL >> 4) + 1;
-    const int KS = Be / 2 + 4;  // K / 4 with K = 2 Be + 16, an even number of 4-step groups
+    const int KS = Be / 2 + 4;  // k-steps: K / 4 with K = 2 Be + 16
     extern __shared__ unsigned char band_lds[];
     T* cw = reinterpret_cast<T*>(band_lds);  // [S][KC] operator windows, reversed
     T* X = cw + S * KC;                      // [S][XLP] extended inputs (one pad word per 16)
@@ -522,20 +527,26 @@ __global__ void __launch_bounds__(256) k_bandstep(const DevSystem* __restrict__ 
                     bv[k4] = xp[17 * K + 4 * k4];
                 }
             };
-            const int KG = KS / 4;  // even
+            const int KG = KS / 4;  // full 4-step groups, then KS - 4 KG single k-steps
             fetch(0, a0, b0);
-            for (int K = 0; K < KG; K += 2) {
+            int K = 0;
+            for (; K + 1 < KG; K += 2) {
                 fetch(K + 1, a1, b1);
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int k4 = 0; k4 < 4; ++k4) acc = Mfma<T>::mma(a0[k4], b0[k4], acc);
                 __builtin_amdgcn_sched_barrier(0);
-                fetch(K + 2, a0, b0);  // past the end on the last pass: read from the slack, unused
+                fetch(K + 2, a0, b0);  // at most one group past the last: within the carve-out and slack, unused
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int k4 = 0; k4 < 4; ++k4) acc = Mfma<T>::mma(a1[k4], b1[k4], acc);
                 __builtin_amdgcn_sched_barrier(0);
             }
+            if (K < KG) {  // odd group count: the last group, fetched above
+#pragma unroll
+                for (int k4 = 0; k4 < 4; ++k4) acc = Mfma<T>::mma(a0[k4], b0[k4], acc);
+            }
+            for (int ks = 4 * KG; ks < KS; ++ks) acc = Mfma<T>::mma(ap[4 * ks], xp[17 * (ks >> 2) + 4 * (ks & 3)], acc);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int n = n0 + 16 * (p0 + q) + Mfma<T>::row(lane, v);

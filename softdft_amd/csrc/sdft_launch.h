@@ -87,6 +87,10 @@ struct OpPlan {
     std::vector<FftSetPlan> fft;   // shape CONV_FFT: one entry per kernel set (no tiles or groups)
 };
 
+// applied half-width of the banded bond operator in the chain and banded step kernels: the band B
+// rounded up to even, so that K = 2 Be + 16 taps per 16 outputs are a whole number of 4-tap MFMA steps
+__host__ __device__ __forceinline__ int chain_half_width(int B) { return B + (B & 1); }
+
 // one (a, b) column of one step of the chain kernel (a: state of the output segment, b: state of the
 // input segment): valid pair, input bead, field of exp(lng/2) at the input and at the output end
 struct ChainCol {

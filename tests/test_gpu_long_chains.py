@@ -77,14 +77,14 @@ def test_500mer_brush_objective_matches_oracle(pH, salt):
 
 def _banded_oracle(sp):
     """Oracle whose bond operators keep |j| <= Be of the filtered bond kernel c (B: last offset with
-    |c| > 1e-9 max|c| over the bond types; Be: B rounded up to 8 mod 16, the applied half-width)."""
+    |c| > 1e-9 max|c| over the bond types; Be: B rounded up to even, the applied half-width)."""
     o = OracleDFT(sp)
     L = o.L
     d = np.arange(L)
     dd = np.minimum(d, L - d)
     cs = [np.fft.ifft(o.wbond_filt[:, u]).real for u in range(sp.MB)]
     B = max(int(dd[np.abs(c) > 1e-9 * np.abs(c).max()].max()) for c in cs)
-    Be = (B + 7) // 16 * 16 + 8
+    Be = B + (B & 1)
     for u, c in enumerate(cs):
         c = c.copy()
         c[dd > Be] = 0.0

@@ -62,12 +62,12 @@ def test_every_conv_tile_shape_matches_oracle(shape):
 @pytest.mark.parametrize("choice", [0, 1, 2])
 def test_propagator_mappings_match_banded_oracle(choice):
     # choices 1, 2: persistent chain kernel, weights in registers / re-read; choice 0: one banded
-    # step kernel launch per step; all apply the half-width Be = B rounded up to 8 mod 16
+    # step kernel launch per step; all apply the half-width Be = B rounded up to even
     sp = _points()[0]
     b = abi.Batch([sp])
     b.set_operator_band(fp32=True, fp64=True)
     b.set_tuning([dict(kind=abi.TUNE_PROPAGATOR, key=0, precision=0, choice=choice)])
-    hw = lambda B: (B + 7) // 16 * 16 + 8
+    hw = lambda B: B + (B & 1)
     o = _banded_oracle(sp, hw)
     rng = np.random.default_rng(9)
     x0 = b.x0()
