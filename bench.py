@@ -261,10 +261,30 @@ def report_c4(args):
     t1 = (time.perf_counter() - t0) / len(sps)
     b = abi.Batch(sps)
     x, it, err = b.solve(b.x0(), precision=abi.PREC_FP64, raise_on_noconv=False, **_solve_opts(sps[0].numerics))
-    om, _, pc = b.observables()
+    om, al, pc = b.observables()
+    rho_gpu = [b.get_field(abi.FIELD_RHO_BEADS, k) for k in range(len(sps))]
     b.close()
     dom = [abs(res[i].omega - om[k]) / max(1.0, abs(om[k])) for k, i in enumerate(sel)]
     dpi = [abs(res[i].delta_p - (pc[k, 0] - sps[k].bulk.pressure)) for k, i in enumerate(sel)]
+    # the FP64 GPU solves of the two narrowest sampled widths against full oracle solves (off the clock)
+    from oracle.dft_oracle import OracleDFT, anderson_solve
+    ora = []
+    for k in range(2):
+        sp = sps[k]
+        o = OracleDFT(sp)
+        r = anderson_solve(o.objective, sp.x0(), sp.numerics, max_iters=20000)
+        om_o = o.grand_potential(r.x)
+        rho_o = o.rho_beads.copy()
+        ads_g, ads_o = _adsorption(sp, rho_gpu[k]), _adsorption(sp, rho_o)
+        fr = o.state_fractions()
+        da = max((float(np.max(np.abs(al[k, u, :len(f)] - f))) for u, f in enumerate(fr) if f is not None), default=0.0)
+        ora.append({"H": float(sp.H), "NP": sp.NP, "oracle_iterations": int(r.iters),
+                    "omega_rel": abs(om[k] - om_o) / max(1.0, abs(om_o)),
+                    "rho_beads_rel": float(np.max(np.abs(rho_gpu[k] - rho_o)) / np.max(np.abs(rho_o))),
+                    "adsorption_rel": float(np.max(np.abs(ads_g - ads_o)) / max(np.max(np.abs(ads_o)), 1e-300)),
+                    "alpha_abs": da,
+                    "pi_abs": abs((pc[k, 0] - sp.bulk.pressure) - (o.contact_pressure()[0] - sp.bulk.pressure))})
+        log(f"c4 H={sp.H:.3f}: FP64 GPU vs oracle {ora[-1]}")
     teval, _ = _oracle_eval_time(sps[len(sps) // 2], x[b.offsets[len(sps) // 2]:b.offsets[len(sps) // 2 + 1]])
     mean_it = float(np.mean([r.iterations for r in res]))
     out = {"report": "config", "config": "C4", "widths": n, "NP_range": [min(s.NP for s in sps), max(s.NP for s in sps)],
@@ -275,7 +295,8 @@ def report_c4(args):
            "cpu_baseline": {"value": 1.0 / (teval * mean_it), "unit": "state points/s", "cores": 1, "kind": "port",
                             "sample": f"3 oracle objective evaluations at NP={sps[len(sps) // 2].NP} "
                                       f"({teval:.3f} s each) x the sweep's mean {mean_it:.0f} iterations (estimate)"},
-           "errors": {"omega_max_rel_sweep_vs_cold_fp64": float(max(dom)), "pi_max_abs_sweep_vs_cold_fp64": float(max(dpi))},
+           "errors": {"omega_max_rel_sweep_vs_cold_fp64": float(max(dom)), "pi_max_abs_sweep_vs_cold_fp64": float(max(dpi)),
+                      "fp64_gpu_vs_oracle": ora},
            "pi_curve": [[r.distance, r.delta_p] for r in res[::16]]}
     print(json.dumps(_finite(out)), flush=True)
 
@@ -339,7 +360,26 @@ def report_c5(args):
     bs.set_conv_shape(abi.CONV_FFT)
     x, it, err, ts = _timed_solve(bs, bs.x0(), abi.PREC_MIXED, _solve_opts(sps[0].numerics, max_iters=3000))
     om, al, _ = bs.observables()
+    rho_m = bs.get_field(abi.FIELD_RHO_BEADS, 0)
+    # the same point solved cold in plain FP64 (banded propagators, the operator the polish uses), and
+    # the oracle's map at the mixed solution (banded direct-sum propagators: the FFT evaluation of a
+    # 500-mer brush's propagators is round-off dominated); an oracle solve of this point takes hours,
+    # so the error against the oracle is the fixed-point residual of its map
+    x64, it64, err64, t64 = _timed_solve(bs, bs.x0(), abi.PREC_FP64, _solve_opts(sps[0].numerics, max_iters=4000,
+                                                                                 band_fp32=2))
+    om64, al64, _ = bs.observables()
+    rho64 = bs.get_field(abi.FIELD_RHO_BEADS, 0)
     bs.close()
+    from oracle.dft_oracle import OracleDFT
+    o5 = OracleDFT(sps[0])
+    o5.use_banded_direct_propagators()
+    f_or = o5.objective(x)
+    c5_err = {"fp64_cold_iterations": int(it64[0]), "fp64_residual": float(err64[0]), "fp64_seconds": t64,
+              "omega_rel_mixed_vs_fp64": abs(om[0] - om64[0]) / max(1.0, abs(om64[0])),
+              "rho_beads_rel_mixed_vs_fp64": float(np.max(np.abs(rho_m - rho64)) / np.max(np.abs(rho64))),
+              "alpha_abs_mixed_vs_fp64": float(np.nanmax(np.abs(al[0] - al64[0]))),
+              "oracle_map_residual_at_mixed_solution": float(np.max(np.abs(f_or - x)) / max(1.0, float(np.max(np.abs(x)))))}
+    log(f"c5 errors: {c5_err}")
     per_point_fp32 = res["fp32"] / len(sps)
     out = {"report": "config", "config": "C5", "points": len(sps), "NP": sps[0].NP, "n_seg": 500,
            "batch_evaluation_ms": {k: v * 1e3 for k, v in res.items()},
@@ -352,7 +392,7 @@ def report_c5(args):
            "batch_solve": batch_solve,
            "single_point_mixed_solve": {"pH": 3.0, "salt_M": 0.001, "iterations": int(it[0]),
                                         "residual": float(err[0]), "seconds": ts, "omega": float(om[0]),
-                                        "alpha": float(al[0, 0, 1])},
+                                        "alpha": float(al[0, 0, 1]), "errors": c5_err},
            "note": "converged_points_per_s counts the converged subset of the capped batch solve over the "
                    "batch's wall time; the charged brushes at high pH and salt stall at O(0.1-10) with the "
                    "brush numerics, and the reference's algorithm (no propagator rescaling) overflows on them "
