@@ -188,6 +188,14 @@ int32_t sdft_observables_noalloc(sdft_ctx_t* ctx, double* omega, double* alpha, 
 /* copy one field of one point to host (layouts above) */
 int32_t sdft_get_field(sdft_ctx_t* ctx, int32_t field_id, int32_t point, double* host_out, int64_t capacity);
 
+/* replace every point's chemistry: delta_muH [P][NB] and mu_species [P][n_species] (grafted species
+ * 0), with the chain normalisations that follow from them (normative, child_propagators.jl:41-50;
+ * normative^(N-1), handle_fixed.jl:105-107). This is the state a pseudo-arclength step changes when
+ * its continuation variable is the pSpecies of H+: objective_pac (src/NonlinearSolvers/continuation/
+ * objective_pac.jl:230-260) rebuilds the system at the new pH (update_cont_var + establish_dft_system)
+ * before each objective_dft. Everything else of the context (grids, walls, topology) is kept. */
+int32_t sdft_set_chemistry(sdft_ctx_t* ctx, const double* delta_muH, const double* mu_species);
+
 /* timing of the last sdft_solve: device milliseconds of the whole solve and total iterations */
 int32_t sdft_last_timing(const sdft_ctx_t* ctx, double* solve_ms, double* iterations_total);
 

@@ -120,6 +120,8 @@ def load_library(path: str = LIB_PATH):
     lib.sdft_autotune.argtypes = [C.c_void_p, C.c_int32, C.POINTER(SdftTuneEntry), C.c_int32, _i32p]
     lib.sdft_set_tuning.restype = C.c_int32
     lib.sdft_set_tuning.argtypes = [C.c_void_p, C.POINTER(SdftTuneEntry), C.c_int32]
+    lib.sdft_set_chemistry.restype = C.c_int32
+    lib.sdft_set_chemistry.argtypes = [C.c_void_p, _f64p, _f64p]
     lib.sdft_set_launch_mode.restype = C.c_int32
     lib.sdft_set_launch_mode.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _i64p]
     if lib.sdft_abi_version() != SDFT_ABI_VERSION:
@@ -326,6 +328,14 @@ class Batch:
         if rc != SDFT_OK and (raise_on_noconv or rc != -4):
             self._check(rc)
         return x, iters, err
+
+    def set_chemistry(self, delta_muH: np.ndarray, mu_species: np.ndarray):
+        """Replace every point's delta_muH [P][NB] and mu_species [P][n_species] (and the chain
+        normalisations that follow from them), e.g. after a pH step: the state of a continuation."""
+        P = len(self.points)
+        dmu = np.ascontiguousarray(delta_muH, dtype=np.float64).reshape(P, self.NB)
+        mus = np.ascontiguousarray(mu_species, dtype=np.float64).reshape(P, self.nsp)
+        self._check(self.lib.sdft_set_chemistry(self.h, _ptr(dmu, C.c_double), _ptr(mus, C.c_double)))
 
     def last_timing(self):
         ms = C.c_double()

@@ -873,6 +873,31 @@ static int validate_desc(const sdft_desc_t* d) {
     return SDFT_OK;
 }
 
+// delta_muH, mu_species and the chain normalisations that follow from them, for one point: normative
+// (child_propagators.jl:41-50) is the product over segments of (sum_states e^{-dmuH})^(1/N); a free
+// chain is normalised by normative^(N-1) (handle_fixed.jl:105-107), a free monomerbead species by 1
+// (monomerbead/handle_fixed.jl:39-41). Needs the context's species tables (nseg, S, state_off,
+// seg_states_h, model).
+static void set_point_chemistry(const sdft_ctx_t* ctx, DevPoint& q, const double* dmu, const double* mus) {
+    const int NB = ctx->c.NB, nsp = ctx->c.nsp;
+    for (int j = 0; j < NB; ++j) q.delta_muH[j] = dmu[j];
+    for (int u = 0; u < nsp; ++u) {
+        q.mu_species[u] = mus[u];
+        const int N = ctx->nseg[u], S = ctx->S[u];
+        double normative = 1.0;
+        for (int s2 = N - 1; s2 >= 0; --s2) {
+            double t = 0.0;
+            for (int a = 0; a < S; ++a) {
+                const int b = ctx->seg_states_h[ctx->state_off[u] + s2 * S + a];
+                if (b >= 0) t += exp(-q.delta_muH[b]);
+            }
+            normative *= pow(t, 1.0 / N);
+        }
+        q.normative[u] = normative;
+        q.chain_norm[u] = ctx->c.species[u].model == SDFT_MODEL_MONOMERBEAD ? 1.0 : pow(normative, (double)(N - 1));
+    }
+}
+
 extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     if (!out || !d) {
         set_error(nullptr, SDFT_E_ARG, "null argument");
@@ -1143,30 +1168,14 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
             q.kset = it->second;
         }
         q.kbase = set_kbase[q.kset];
-        for (int j = 0; j < NB; ++j) q.delta_muH[j] = d->delta_muH[(int64_t)p * NB + j];
         for (int u = 0; u < nsp; ++u) {
-            q.mu_species[u] = d->mu_species[(int64_t)p * nsp + u];
             q.fixed_coord[u] = d->fixed_coord ? d->fixed_coord[(int64_t)p * nsp + u] : -1;
             if (spec[u].fixed_seg >= 0 && (q.fixed_coord[u] < 0 || q.fixed_coord[u] >= NP)) {
                 set_error(ctx, SDFT_E_ARG, "point %d species %d: grafting coordinate %d outside [0, %d)", p, u, q.fixed_coord[u], NP);
                 return fail(SDFT_E_ARG);
             }
-            // normative (child_propagators.jl:41-50): prod over segments of (sum_states e^{-dmuH})^(1/N)
-            const int N = spec[u].nseg, S = spec[u].S;
-            double normative = 1.0;
-            for (int s2 = N - 1; s2 >= 0; --s2) {
-                double t = 0.0;
-                for (int a = 0; a < S; ++a) {
-                    const int b = d->seg_states[spec[u].state_off + s2 * S + a];
-                    if (b >= 0) t += exp(-q.delta_muH[b]);
-                }
-                normative *= pow(t, 1.0 / N);
-            }
-            q.normative[u] = normative;
-            // free-chain normalisation normative^(N-1); a free monomerbead species is normalised by 1
-            // (monomerbead/handle_fixed.jl:39-41)
-            q.chain_norm[u] = spec[u].model == SDFT_MODEL_MONOMERBEAD ? 1.0 : pow(normative, (double)(N - 1));
         }
+        set_point_chemistry(ctx, q, d->delta_muH + (int64_t)p * NB, d->mu_species + (int64_t)p * nsp);
         for (int j = 0; j < NB; ++j) {
             q.wall_lo[j] = d->wall_lo ? d->wall_lo[(int64_t)p * NB + j] : 0;
             q.wall_hi[j] = d->wall_hi ? d->wall_hi[(int64_t)p * NB + j] : 0;
@@ -1614,6 +1623,29 @@ extern "C" int32_t sdft_set_operator_band(sdft_ctx_t* ctx, int32_t fp32_banded, 
     if (!ctx) return SDFT_E_ARG;
     ctx->obj_band32 = fp32_banded ? 1 : 0;
     ctx->obj_band64 = fp64_banded ? 1 : 0;
+    return SDFT_OK;
+}
+
+extern "C" int32_t sdft_set_chemistry(sdft_ctx_t* ctx, const double* delta_muH, const double* mu_species) {
+    if (!ctx || !delta_muH || !mu_species) return SDFT_E_ARG;
+    const int P = ctx->c.P, NB = ctx->c.NB, nsp = ctx->c.nsp;
+    for (int64_t i = 0; i < (int64_t)P * NB; ++i)
+        if (!std::isfinite(delta_muH[i])) {
+            set_error(ctx, SDFT_E_ARG, "delta_muH[%lld] is not finite", (long long)i);
+            return SDFT_E_ARG;
+        }
+    for (int64_t i = 0; i < (int64_t)P * nsp; ++i)
+        if (!std::isfinite(mu_species[i])) {
+            set_error(ctx, SDFT_E_ARG, "mu_species[%lld] is not finite", (long long)i);
+            return SDFT_E_ARG;
+        }
+    for (int p = 0; p < P; ++p)
+        set_point_chemistry(ctx, ctx->pts[p], delta_muH + (int64_t)p * NB, mu_species + (int64_t)p * nsp);
+    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    HIP_CHECK_RET(hipMemcpyAsync(ctx->c.d_pts, ctx->pts.data(), sizeof(DevPoint) * P, hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    ctx->solved = false;
+    ctx->have_eval = false;
     return SDFT_OK;
 }
 

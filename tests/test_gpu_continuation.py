@@ -1,0 +1,66 @@
+"""GPU: pseudo-arclength continuation in pH (objective_pac.jl) on the library's objective.
+
+Two branches (10 and 30 mM salt) of a free weak-acid 6-mer in a slit step together. Every corrector
+solves the augmented map to 1e-11; each step's result is then checked without the driver:
+* (x*, pH*) is a fixed point of the oracle's own map at pH* (its molecular system and bulk state
+  rebuilt on the host at that pH): max|F_oracle(x*) - x*| < 1e-9;
+* the step satisfies the arclength constraint t . (X* - X_prev) = ds_init, with t re-formed from
+  the recorded history by the reference's tangent rule;
+* the reported degree of ionization equals the oracle's at x*;
+* pH and ionization increase along the branch (a weak acid above its start).
+"""
+import copy
+
+import numpy as np
+import pytest
+
+from softdft_amd import configs
+from softdft_amd.continuation import PacOptions, pac_continuation, pac_tangent
+from softdft_amd.problem import build_state_point
+
+pytestmark = pytest.mark.gpu
+
+
+def _at_pH(ds, pH):
+    d = copy.deepcopy(ds)
+    for s in d["species"]:
+        if s.get("sequence") == "Y":
+            s["pSpecies"] = float(pH)
+    return d
+
+
+def test_pac_steps_are_oracle_fixed_points_on_the_arclength_constraint():
+    from oracle.dft_oracle import OracleDFT
+    dss = [configs.weak_polyelectrolyte(H=6.0, dz=0.05, n_seg=6, solvent=0.2, pH=4.0, salt_M=c)
+           for c in (0.010, 0.030)]
+    num = dss[0]["numerical"]
+    o = PacOptions(ds_init=0.5, max_steps=3, tole=1e-11, damping=num["damping"], mixing_max=num["mixing_max"],
+                   switch_tole=num["switch_tole"], anderm=int(num["anderm"]), max_iters=20000)
+    print("case pac two branches, 3 continuation steps", flush=True)
+    res = pac_continuation(dss, o, keep_x=True,
+                           on_step=lambda k, st: print(f"branch {k} step {st.step} pH {st.pH:.6f} "
+                                                       f"alpha {st.alpha:.6f} it {st.iterations}", flush=True))
+    for k, (ds, steps) in enumerate(zip(dss, res)):
+        assert [st.step for st in steps] == [1, 2, 3, 4, 5]
+        assert steps[1].pH == pytest.approx(4.5, abs=1e-15)
+        X = [np.append(st.x, st.pH) for st in steps]
+        S = [st.arclength for st in steps]
+        hist = np.zeros((o.num_keep, X[0].size))
+        svec = np.zeros(o.num_keep)
+        for j in range(2):
+            hist[j], svec[j] = X[j], S[j]
+        for j in range(2, len(steps)):
+            step = j + 1
+            t = pac_tangent(hist, svec, step, o.num_keep)
+            assert abs(float(np.dot(t, X[j] - X[j - 1])) - o.ds_init) < 1e-8
+            assert S[j] == pytest.approx(S[j - 1] + np.linalg.norm(X[j] - X[j - 1]), rel=1e-12)
+            hist[(step - 1) % o.num_keep], svec[(step - 1) % o.num_keep] = X[j], S[j]
+        for st in steps:
+            sp = build_state_point(_at_pH(ds, st.pH))
+            orc = OracleDFT(sp)
+            fx = orc.objective(st.x)
+            assert np.max(np.abs(fx - st.x)) < 1e-9, (k, st.step)
+            alpha_o = orc.state_fractions()[0]
+            assert abs(st.alpha - float(np.atleast_1d(alpha_o)[1])) < 1e-9, (k, st.step)
+        assert all(steps[j + 1].pH > steps[j].pH for j in range(len(steps) - 1))
+        assert all(steps[j + 1].alpha > steps[j].alpha for j in range(len(steps) - 1))
