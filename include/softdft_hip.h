@@ -196,6 +196,13 @@ int32_t sdft_get_field(sdft_ctx_t* ctx, int32_t field_id, int32_t point, double*
  * before each objective_dft. Everything else of the context (grids, walls, topology) is kept. */
 int32_t sdft_set_chemistry(sdft_ctx_t* ctx, const double* delta_muH, const double* mu_species);
 
+/* replace every point's grafting densities [P][n_species] (entries of species without a grafted
+ * segment are stored and unused): the chain normalisation of a grafted species is density / lambda
+ * (handle_fixed.jl:31-43). This is the state objective_pac_graft changes each evaluation
+ * (update_cont_var_graft, src/NonlinearSolvers/continuation/objective_pac_graft.jl:286-288: the first
+ * species' fixed density = exp(continuation variable)). */
+int32_t sdft_set_graft_density(sdft_ctx_t* ctx, const double* density);
+
 /* timing of the last sdft_solve: device milliseconds of the whole solve and total iterations */
 int32_t sdft_last_timing(const sdft_ctx_t* ctx, double* solve_ms, double* iterations_total);
 

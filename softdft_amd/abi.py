@@ -122,6 +122,8 @@ def load_library(path: str = LIB_PATH):
     lib.sdft_set_tuning.argtypes = [C.c_void_p, C.POINTER(SdftTuneEntry), C.c_int32]
     lib.sdft_set_chemistry.restype = C.c_int32
     lib.sdft_set_chemistry.argtypes = [C.c_void_p, _f64p, _f64p]
+    lib.sdft_set_graft_density.restype = C.c_int32
+    lib.sdft_set_graft_density.argtypes = [C.c_void_p, _f64p]
     lib.sdft_set_launch_mode.restype = C.c_int32
     lib.sdft_set_launch_mode.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _i64p]
     if lib.sdft_abi_version() != SDFT_ABI_VERSION:
@@ -336,6 +338,12 @@ class Batch:
         dmu = np.ascontiguousarray(delta_muH, dtype=np.float64).reshape(P, self.NB)
         mus = np.ascontiguousarray(mu_species, dtype=np.float64).reshape(P, self.nsp)
         self._check(self.lib.sdft_set_chemistry(self.h, _ptr(dmu, C.c_double), _ptr(mus, C.c_double)))
+
+    def set_graft_density(self, density: np.ndarray):
+        """Replace every point's grafting density per species [P][n_species]."""
+        P = len(self.points)
+        d = np.ascontiguousarray(density, dtype=np.float64).reshape(P, self.nsp)
+        self._check(self.lib.sdft_set_graft_density(self.h, _ptr(d, C.c_double)))
 
     def last_timing(self):
         ms = C.c_double()

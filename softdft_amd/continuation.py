@@ -34,6 +34,12 @@ What differs, and why:
 * `max_steps` caps the number of predictor-corrector steps. The reference reads it (objective_pac.jl:55)
   but never uses it.
 
+The grafted variant (objective_pac_graft.jl) continues the logarithm of the first species' grafting
+density instead: update_cont_var_graft sets that density to exp(variable) (objective_pac_graft.jl:286-288),
+the loop runs while the variable is below 1 (objective_pac_graft.jl:120), its steps report the density,
+and it has no early stop. Here `variable="log_graft_density"` selects it; the grafting density is
+per point in the library (sdft_set_graft_density), so branches may differ in it.
+
 Every evaluation of the augmented map sets each branch's chemistry for its current pH:
 * delta_muH, and the bulk mu_species from the batched device bulk solve, go to the context
   through sdft_set_chemistry;
@@ -82,7 +88,8 @@ class PacOptions:
 @dataclass
 class PacStep:
     step: int                # the reference's step counter (1, 2: start solves; 3.. continuation)
-    pH: float
+    param: float             # as the reference reports it: the pH, or the grafting density
+    cont: float              # the continuation variable itself: the pH, or log(grafting density)
     alpha: float             # average state-2 fraction of the first species (NaN if it has one state)
     brush_height: float      # first grafted species, NaN without one
     omega: float
@@ -171,6 +178,63 @@ class PhChemistry:
         return dmu, mus
 
 
+class _PhVariable:
+    """pSpecies of H+ (update_cont_var, objective_pac.jl:286-293)."""
+    report = "pH"
+    early_stop = True
+
+    def __init__(self, datasets, sps, o: PacOptions):
+        self.chem = PhChemistry(datasets, [[f.any for f in sp.fixed] for sp in sps])
+        self.lo, self.hi = o.pH_min, o.pH_max
+
+    @staticmethod
+    def initial(datasets) -> np.ndarray:
+        return np.array([float(d["species"][_ph_species_index(d)]["pSpecies"]) for d in datasets])
+
+    def apply(self, batch: abi.Batch, v: np.ndarray) -> None:
+        dmu, mus = self.chem(v)
+        batch.set_chemistry(dmu, mus)
+
+    def in_range(self, v: float) -> bool:
+        return self.lo < v < self.hi
+
+    @staticmethod
+    def param(v: float) -> float:
+        return float(v)
+
+
+class _GraftVariable:
+    """log of the first species' grafting density (update_cont_var_graft, objective_pac_graft.jl:286-288);
+    the other species keep their densities."""
+    report = "graft_dens"
+    early_stop = False
+
+    def __init__(self, datasets, sps, o: PacOptions):
+        if not sps[0].fixed or not sps[0].fixed[0].any:
+            raise ValueError("grafting-density continuation needs a grafted first species")
+        self.base = np.array([[f.density for f in sp.fixed] for sp in sps], dtype=np.float64)
+
+    @staticmethod
+    def initial(datasets) -> np.ndarray:
+        return np.array([math.log(float(d["species"][0]["fixed"]["density"])) for d in datasets])
+
+    def apply(self, batch: abi.Batch, v: np.ndarray) -> None:
+        dens = self.base.copy()
+        dens[:, 0] = np.exp(v)
+        batch.set_graft_density(dens)
+
+    @staticmethod
+    def in_range(v: float) -> bool:
+        return v < 1.0
+
+    @staticmethod
+    def param(v: float) -> float:
+        return math.exp(v)
+
+
+_VARIABLES = {"pH": _PhVariable, "log_graft_density": _GraftVariable}
+
+
 def _observe(batch: abi.Batch, sps, k: int, om, al):
     segs = [batch.segment_profile(u, k) for u in range(batch.nsp)]
     hs = analysis.brush_heights(segs, sps[k])
@@ -182,17 +246,18 @@ def _observe(batch: abi.Batch, sps, k: int, om, al):
 
 
 def pac_continuation(datasets: Sequence[dict], opts: Optional[PacOptions] = None, *,
+                     variable: str = "pH",
                      on_step: Optional[Callable[[int, PacStep], None]] = None,
                      keep_x: bool = False) -> List[List[PacStep]]:
-    """Trace one pH branch per dataset (all sharing one grid and molecular layout, e.g. one per
-    salt), all branches stepping together. Returns per branch its steps; a branch stops on its own
-    when it leaves (pH_min, pH_max), reaches the degree-of-ionization bound or max_steps."""
+    """Trace one branch per dataset (all sharing one grid and molecular layout, e.g. one per salt),
+    all branches stepping together, in `variable` ("pH" or "log_graft_density"). Returns per branch
+    its steps; a branch stops on its own when it leaves the variable's range, reaches the pH
+    variant's degree-of-ionization bound or max_steps."""
     o = opts or PacOptions()
     sps = build_state_points(datasets, device_bulk=True)
     B = len(sps)
-    grafted = [[f.any for f in sp.fixed] for sp in sps]
-    chem = PhChemistry(datasets, grafted)
-    pH = np.array([float(d["species"][_ph_species_index(d)]["pSpecies"]) for d in datasets])
+    var = _VARIABLES[variable](datasets, sps, o)
+    pH = var.initial(datasets)
     batch = abi.Batch(sps)
     batch.set_operator_band(fp32=True, fp64=o.banded)
     solve_opts = dict(tole=o.tole, damping=o.damping, mixing_max=o.mixing_max, switch_tole=o.switch_tole,
@@ -202,7 +267,8 @@ def pac_continuation(datasets: Sequence[dict], opts: Optional[PacOptions] = None
 
     def record(k, step, x_aug, iters, err, s, om, al):
         alpha, height, omega = _observe(batch, sps, k, om, al)
-        st = PacStep(step=step, pH=float(x_aug[-1]), alpha=alpha, brush_height=height, omega=omega,
+        st = PacStep(step=step, param=var.param(x_aug[-1]), cont=float(x_aug[-1]), alpha=alpha,
+                     brush_height=height, omega=omega,
                      iterations=int(iters), residual=float(err), arclength=float(s),
                      x=x_aug[:-1].copy() if keep_x else None)
         out[k].append(st)
@@ -212,8 +278,7 @@ def pac_continuation(datasets: Sequence[dict], opts: Optional[PacOptions] = None
 
     def evaluate_final(xs: np.ndarray, pHs: np.ndarray):
         # the fields' densities at the final point (objective_pac.jl:194-195 evaluates before analysis)
-        dmu, mus = chem(pHs)
-        batch.set_chemistry(dmu, mus)
+        var.apply(batch, pHs)
         batch.objective(np.concatenate(list(xs)), abi.PREC_FP64)
         return batch.observables()
 
@@ -224,8 +289,7 @@ def pac_continuation(datasets: Sequence[dict], opts: Optional[PacOptions] = None
     for k in range(B):
         record(k, 1, X_prev[k], it1[k], err1[k], 0.0, om, al)
     pH2 = pH + o.ds_init
-    dmu, mus = chem(pH2)
-    batch.set_chemistry(dmu, mus)
+    var.apply(batch, pH2)
     x, it2, err2 = batch.solve(x, **solve_opts)
     X_k = np.stack([np.append(xp, p) for xp, p in zip(batch.split(x), pH2)])
     s = np.linalg.norm(X_k - X_prev, axis=1)
@@ -240,7 +304,7 @@ def pac_continuation(datasets: Sequence[dict], opts: Optional[PacOptions] = None
     S_vec[:, 1] = s
     dS = np.full(B, o.ds_init)
     ds_con = o.ds_init   # the constraint's ds, fixed at the start (objective_pac.jl:104-106)
-    active = np.array([o.pH_min < X_k[k, -1] < o.pH_max for k in range(B)])
+    active = np.array([var.in_range(X_k[k, -1]) for k in range(B)])
     step = 3
     while active.any() and step - 2 <= o.max_steps:
         T = np.stack([pac_tangent(X_hist[k], S_vec[k], step, o.num_keep) for k in range(B)])
@@ -252,8 +316,7 @@ def pac_continuation(datasets: Sequence[dict], opts: Optional[PacOptions] = None
         iters = np.zeros(B, dtype=np.int64)
         errs = np.zeros(B)
         while not done.all():
-            dmu, mus = chem(Y[:, -1])
-            batch.set_chemistry(dmu, mus)
+            var.apply(batch, Y[:, -1])
             fx = batch.split(batch.objective(np.concatenate(list(Y[:, :-1])), abi.PREC_FP64))
             for k in range(B):
                 if done[k]:
@@ -283,19 +346,25 @@ def pac_continuation(datasets: Sequence[dict], opts: Optional[PacOptions] = None
                 continue
             st = record(k, step, X_k[k], iters[k], errs[k], s[k], om, al)
             # the reference tests sign(dS), which its update (dS + ds_actual) / 2 keeps positive
-            if (st.alpha >= 0.99 and dS[k] > 0.0) or (st.alpha <= 0.01 and dS[k] < 0.0):
+            if var.early_stop and ((st.alpha >= 0.99 and dS[k] > 0.0) or (st.alpha <= 0.01 and dS[k] < 0.0)):
                 active[k] = False
-            elif not (o.pH_min < st.pH < o.pH_max):
+            elif not var.in_range(st.cont):
                 active[k] = False
         step += 1
     batch.close()
     return out
 
 
-def write_pac_summary(results: Sequence[PacStep], path: str) -> None:
-    """One line per step: step, pH, alpha, brush height (the reference's per-step summary fields,
-    objective_pac.jl:201-205)."""
+def write_pac_summary(results: Sequence[PacStep], path: str, variable: str = "pH") -> None:
+    """The reference's summary file (objective_pac.jl:383-390, objective_pac_graft.jl:373-380):
+    step, the reported parameter (pH or graft_dens), brush height and alpha, printed %.12g."""
+    def g12(v: float) -> str:
+        if math.isnan(v):
+            return "NaN"
+        if math.isinf(v):
+            return "Inf" if v > 0 else "-Inf"
+        return "%.12g" % v
     with open(path, "w") as f:
-        f.write("step,pH,alpha,brush_height\n")
+        f.write(f"step,{_VARIABLES[variable].report},brush_height,alpha\n")
         for r in results:
-            f.write(f"{r.step},{r.pH!r},{r.alpha!r},{r.brush_height!r}\n")
+            f.write(f"{r.step},{g12(r.param)},{g12(r.brush_height)},{g12(r.alpha)}\n")

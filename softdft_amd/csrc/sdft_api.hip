@@ -1084,7 +1084,6 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         memset(&s, 0, sizeof(s));
         s.nseg = N; s.S = S; s.model = d->species_model[u]; s.fixed_seg = fseg;
         s.state_off = state_off; s.maxd_off = maxd_off;
-        s.fixed_density = d->fixed_density ? d->fixed_density[u] : 0.0;
         int multi = 0;
         for (int q = 0; q < N; ++q) {
             int cnt = 0;
@@ -1176,6 +1175,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
             }
         }
         set_point_chemistry(ctx, q, d->delta_muH + (int64_t)p * NB, d->mu_species + (int64_t)p * nsp);
+        for (int u = 0; u < nsp; ++u) q.graft_density[u] = d->fixed_density ? d->fixed_density[u] : 0.0;
         for (int j = 0; j < NB; ++j) {
             q.wall_lo[j] = d->wall_lo ? d->wall_lo[(int64_t)p * NB + j] : 0;
             q.wall_hi[j] = d->wall_hi ? d->wall_hi[(int64_t)p * NB + j] : 0;
@@ -1641,6 +1641,25 @@ extern "C" int32_t sdft_set_chemistry(sdft_ctx_t* ctx, const double* delta_muH, 
         }
     for (int p = 0; p < P; ++p)
         set_point_chemistry(ctx, ctx->pts[p], delta_muH + (int64_t)p * NB, mu_species + (int64_t)p * nsp);
+    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    HIP_CHECK_RET(hipMemcpyAsync(ctx->c.d_pts, ctx->pts.data(), sizeof(DevPoint) * P, hipMemcpyHostToDevice, ctx->stream));
+    HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
+    ctx->solved = false;
+    ctx->have_eval = false;
+    return SDFT_OK;
+}
+
+extern "C" int32_t sdft_set_graft_density(sdft_ctx_t* ctx, const double* density) {
+    if (!ctx || !density) return SDFT_E_ARG;
+    const int P = ctx->c.P, nsp = ctx->c.nsp;
+    for (int64_t i = 0; i < (int64_t)P * nsp; ++i)
+        if (!(std::isfinite(density[i]) && density[i] >= 0.0)) {
+            set_error(ctx, SDFT_E_ARG, "grafting density[%lld] = %g is not a finite non-negative value",
+                      (long long)i, density[i]);
+            return SDFT_E_ARG;
+        }
+    for (int p = 0; p < P; ++p)
+        for (int u = 0; u < nsp; ++u) ctx->pts[p].graft_density[u] = density[(int64_t)p * nsp + u];
     HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));
     HIP_CHECK_RET(hipMemcpyAsync(ctx->c.d_pts, ctx->pts.data(), sizeof(DevPoint) * P, hipMemcpyHostToDevice, ctx->stream));
     HIP_CHECK_RET(hipStreamSynchronize(ctx->stream));

@@ -59,6 +59,9 @@ struct DevPoint {
     // and the free-chain normalisation normative^(N-1) (handle_fixed.jl:105-107)
     double normative[SDFT_MAX_SPECIES];
     double chain_norm[SDFT_MAX_SPECIES];
+    // grafting density of each grafted species (handle_fixed.jl:31: norm = density / lambda); per point
+    // so that a batch may hold several grafting densities (a continuation in the grafting density)
+    double graft_density[SDFT_MAX_SPECIES];
 };
 
 struct DevSpecies {
@@ -72,7 +75,6 @@ struct DevSpecies {
     int32_t maxd_off;     // offset into the max-distance table ([nseg])
     int32_t rescale;      // propagators carry per-step scale factors (long chains)
     int32_t homog;        // every segment carries the same state beads (same bonds on every link)
-    double fixed_density;
 };
 
 // Chains of at least this many segments propagate with per-step rescaling: each step's input is

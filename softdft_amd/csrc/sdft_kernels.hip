@@ -545,7 +545,7 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                 if (threadIdx.x == 0 && blockIdx.x == 0) errflag[p] = SDFT_E_NORMALIZATION;
                 norm = 0.0;
             } else {
-                norm = sp.fixed_density / lamb;
+                norm = P.graft_density[u] / lamb;
             }
         } else {
             norm = sp.rescale ? 1.0 : P.chain_norm[u];  // rescaled: normative^(N-1) enters per segment
@@ -1578,7 +1578,7 @@ __global__ void __launch_bounds__(256) k_segment_sums(const DevSystem* __restric
     const int* st = seg_states + sp.state_off;
     const double* maxd = maxd_tab + sp.maxd_off;
     const int Kf = sp.fixed_seg >= 0 ? P.fixed_coord[u] : -1;
-    double norm = sp.fixed_seg >= 0 ? sp.fixed_density / fixed_lambda[(int64_t)p * SDFT_MAX_SPECIES + u]
+    double norm = sp.fixed_seg >= 0 ? P.graft_density[u] / fixed_lambda[(int64_t)p * SDFT_MAX_SPECIES + u]
                                     : (sp.rescale ? 1.0 : P.chain_norm[u]);
     double pref = norm * exp(P.mu_species[u]);
     if (sp.rescale) pref *= exp(seg_log_scale(sp, P, p, u, q, lsc, sys.n_species, nmax));
@@ -1642,7 +1642,7 @@ __global__ void __launch_bounds__(256) k_segment_profile(const DevSystem* __rest
     const int* st = seg_states + sp.state_off;
     const double* maxd = maxd_tab + sp.maxd_off;
     const int Kf = sp.fixed_seg >= 0 ? P.fixed_coord[u] : -1;
-    const double norm = sp.fixed_seg >= 0 ? sp.fixed_density / fixed_lambda[(int64_t)p * SDFT_MAX_SPECIES + u]
+    const double norm = sp.fixed_seg >= 0 ? P.graft_density[u] / fixed_lambda[(int64_t)p * SDFT_MAX_SPECIES + u]
                                           : (sp.rescale ? 1.0 : P.chain_norm[u]);
     double pref = norm * exp(P.mu_species[u]);
     if (sp.rescale) pref *= exp(seg_log_scale(sp, P, p, u, q, lsc, sys.n_species, nmax));

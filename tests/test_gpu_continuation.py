@@ -8,6 +8,9 @@ solves the augmented map to 1e-11; each step's result is then checked without th
   the recorded history by the reference's tangent rule;
 * the reported degree of ionization equals the oracle's at x*;
 * pH and ionization increase along the branch (a weak acid above its start).
+The grafted variant (objective_pac_graft.jl) continues log(grafting density) of a brush, the
+density set per point through sdft_set_graft_density; its steps are checked the same way, with the
+oracle's grafting density set to exp(variable).
 """
 import copy
 
@@ -38,12 +41,12 @@ def test_pac_steps_are_oracle_fixed_points_on_the_arclength_constraint():
                    switch_tole=num["switch_tole"], anderm=int(num["anderm"]), max_iters=20000)
     print("case pac two branches, 3 continuation steps", flush=True)
     res = pac_continuation(dss, o, keep_x=True,
-                           on_step=lambda k, st: print(f"branch {k} step {st.step} pH {st.pH:.6f} "
+                           on_step=lambda k, st: print(f"branch {k} step {st.step} pH {st.param:.6f} "
                                                        f"alpha {st.alpha:.6f} it {st.iterations}", flush=True))
     for k, (ds, steps) in enumerate(zip(dss, res)):
         assert [st.step for st in steps] == [1, 2, 3, 4, 5]
-        assert steps[1].pH == pytest.approx(4.5, abs=1e-15)
-        X = [np.append(st.x, st.pH) for st in steps]
+        assert steps[1].param == pytest.approx(4.5, abs=1e-15)
+        X = [np.append(st.x, st.cont) for st in steps]
         S = [st.arclength for st in steps]
         hist = np.zeros((o.num_keep, X[0].size))
         svec = np.zeros(o.num_keep)
@@ -56,11 +59,54 @@ def test_pac_steps_are_oracle_fixed_points_on_the_arclength_constraint():
             assert S[j] == pytest.approx(S[j - 1] + np.linalg.norm(X[j] - X[j - 1]), rel=1e-12)
             hist[(step - 1) % o.num_keep], svec[(step - 1) % o.num_keep] = X[j], S[j]
         for st in steps:
-            sp = build_state_point(_at_pH(ds, st.pH))
+            sp = build_state_point(_at_pH(ds, st.param))
             orc = OracleDFT(sp)
             fx = orc.objective(st.x)
             assert np.max(np.abs(fx - st.x)) < 1e-9, (k, st.step)
             alpha_o = orc.state_fractions()[0]
             assert abs(st.alpha - float(np.atleast_1d(alpha_o)[1])) < 1e-9, (k, st.step)
-        assert all(steps[j + 1].pH > steps[j].pH for j in range(len(steps) - 1))
+        assert all(steps[j + 1].param > steps[j].param for j in range(len(steps) - 1))
         assert all(steps[j + 1].alpha > steps[j].alpha for j in range(len(steps) - 1))
+
+
+def _with_graft_density(ds, dens):
+    d = copy.deepcopy(ds)
+    d["species"][0]["fixed"]["density"] = float(dens)
+    return d
+
+
+def test_pac_grafting_density_steps_are_oracle_fixed_points():
+    from oracle.dft_oracle import OracleDFT
+    from softdft_amd import analysis
+    dss = [configs.weak_polyelectrolyte(H=8.0, dz=0.05, n_seg=10, pH=p, pKa=5.0, grafted=0.05) for p in (4.5, 5.5)]
+    num = dss[0]["numerical"]
+    o = PacOptions(ds_init=0.5, max_steps=2, tole=1e-11, damping=num["damping"], mixing_max=num["mixing_max"],
+                   switch_tole=num["switch_tole"], anderm=int(num["anderm"]), max_iters=40000)
+    print("case pac grafting density, two branches (pH 4.5, 5.5), 2 continuation steps", flush=True)
+    res = pac_continuation(dss, o, variable="log_graft_density", keep_x=True,
+                           on_step=lambda k, st: print(f"branch {k} step {st.step} density {st.param:.6g} "
+                                                       f"height {st.brush_height:.6f} it {st.iterations}", flush=True))
+    for k, (ds, steps) in enumerate(zip(dss, res)):
+        assert [st.step for st in steps] == [1, 2, 3, 4]
+        assert steps[0].param == pytest.approx(0.05, rel=1e-14)
+        assert steps[1].cont == pytest.approx(np.log(0.05) + 0.5, abs=1e-14)
+        X = [np.append(st.x, st.cont) for st in steps]
+        S = [st.arclength for st in steps]
+        hist = np.zeros((o.num_keep, X[0].size))
+        svec = np.zeros(o.num_keep)
+        for j in range(2):
+            hist[j], svec[j] = X[j], S[j]
+        for j in range(2, len(steps)):
+            step = j + 1
+            t = pac_tangent(hist, svec, step, o.num_keep)
+            assert abs(float(np.dot(t, X[j] - X[j - 1])) - o.ds_init) < 1e-8
+            hist[(step - 1) % o.num_keep], svec[(step - 1) % o.num_keep] = X[j], S[j]
+        for st in steps:
+            sp = build_state_point(_with_graft_density(ds, st.param))
+            orc = OracleDFT(sp)
+            fx = orc.objective(st.x)
+            assert np.max(np.abs(fx - st.x)) < 1e-9, (k, st.step)
+            (_, h), = analysis.brush_heights(orc.rho_seg, sp)
+            assert abs(st.brush_height - h) < 1e-8 * h, (k, st.step)
+        # a denser brush stretches
+        assert all(steps[j + 1].brush_height > steps[j].brush_height for j in range(len(steps) - 1))
