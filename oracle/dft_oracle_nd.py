@@ -1,0 +1,408 @@
+"""FP64 CPU oracle for two- and three-dimensional Cartesian grids -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/ may import this module. It is the checker, never the product.
+
+It restates the reference's D-dimensional path with numpy.fft.fftn on the mirrored calculation
+domain. The sizes are 2 NP_d per dimension, with even reflection in every dimension: every
+dimension is mirrored (process_cartesian.jl:14-45, compute_full_domain and mirrored_index_iterator
+in utilities/mirrored.jl, apply_mirroring.jl). The weights are radial in |k|, with
+k_d = 2 pi n_d / (N*_d h_d) for signed n_d:
+* delta_func_fft!, step_func_fft!: src/iDFT/coordinate_system/Cartesian/fourier_transforms/{delta_func,step_func}.jl;
+* vectored_delta_func_fft!: one vector weight per dimension, -4 pi (sin kR - kR cos kR) / k^3 k_v,
+  vectored_delta_func.jl:1-44;
+* coul_func_fft!: 2 pi lB / sum_v (1 - cos k_v h_v) / h_v^2, coul_func_fft.jl:1-29;
+* the propagator filter exp(-(|k| / (0.5 pi / max h))^4), apply_filter.jl:15-41.
+The FMT functional carries one vector weighted density per dimension:
+* mfmt.jl:19-62 builds its weights;
+* mfmt.jl:65-134 forms the weighted densities, and the pointwise derivative uses the vector
+  components 1..3 (src/iLST/free_energy/excluded_volume/mfmt.jl:210-244);
+* TPT1mfmt uses |nV2|^2 in xi and nV2_v in its vector derivative (TPT1mfmt.jl).
+Sums use the volume element prod h (grand_potential.jl:23, num_species.jl:13, neutrality_shift.jl).
+
+The one-dimensional oracle (dft_oracle.OracleDFT) supplies everything else unchanged: the chain
+recursion, the MSA functions, the neutrality Newton and the Anderson solver. Grid arrays are flat
+over the primary domain (first dimension fastest), as in Julia. Grafted chains are outside this
+module, as they are outside the library's multi-dimensional path.
+
+Pins (tests/test_oracle_nd_pins.py):
+* a system uniform in all but one dimension reduces to the one-dimensional oracle;
+* permuting the dimensions permutes the result;
+* transposing a square channel leaves it invariant;
+* a wall-free bulk state is a fixed point.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .dft_oracle import (PI, OracleDFT, fmt_phi_vec, free_energy_msa_vec, free_energy_tpt1msa_vec,
+                         kgrid, msa_deriv_vec, neutrality_newton, tpt1msa_deriv_vec)
+
+
+def fmt_phi_nd(n0, n1, n2, n3, nV1, nV2):
+    """free_energy_mfmt with D-component vectors nV1, nV2 [P, D] (dot products over components)."""
+    om = 1.0 - n3
+    lg = np.log(om)
+    dot12 = (nV1 * nV2).sum(1)
+    dot22 = (nV2 * nV2).sum(1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        cubic = (n2 ** 3 - 3.0 * n2 * dot22) * (n3 + om ** 2 * lg) / (36.0 * PI * n3 ** 2 * om ** 2)
+    cubic = np.where(n3 == 0.0, 0.0, cubic)
+    return -n0 * lg + (n1 * n2 - dot12) / om + cubic
+
+
+def fmt_deriv_nd(n0, n1, n2, n3, nV1, nV2):
+    """compute_mfmt_deriv! (mfmt.jl:210-244): scalar rows 1-4 and the vector rows per component."""
+    om = 1.0 - n3
+    lg = np.log(om)
+    i1 = 1.0 / om
+    i3 = 1.0 / n3
+    c = 1.0 / (36.0 * PI)
+    dot12 = (nV1 * nV2).sum(1)
+    dot22 = (nV2 * nV2).sum(1)
+    d1 = -lg
+    d2 = n2 * i1
+    d3 = n1 * i1 + (n2 ** 2 - dot22) * 3.0 * c * i3 * (lg * i3 + i1 ** 2)
+    d4 = (n0 * i1 + (n1 * n2 - dot12) * i1 ** 2
+          - (lg * i3 ** 3 * 2.0 * c + i3 ** 2 * i1 * c + (1.0 - 3.0 * n3) * i3 ** 2 * c * i1 ** 3)
+          * (n2 ** 3 - 3.0 * n2 * dot22))
+    dv1 = -nV2 * i1[:, None]
+    dv2 = -nV1 * i1[:, None] - ((lg * i3 + i1 ** 2) * n2 * i3 * 6.0 * c)[:, None] * nV2
+    return d1, d2, d3, d4, dv1, dv2
+
+
+class OracleDFTND(OracleDFT):
+    """One multi-dimensional state point (softdft_amd.geometry_nd.build_state_point_nd)."""
+
+    def __init__(self, sp):
+        if sp.grid is None:
+            raise ValueError("OracleDFTND needs a multi-dimensional state point")
+        if any(f.any for f in sp.fixed):
+            raise NotImplementedError("grafted chains on multi-dimensional grids")
+        self.sp = sp
+        ms = sp.ms
+        self.ms = ms
+        self.grid = tuple(int(v) for v in sp.grid)
+        self.bins = tuple(float(v) for v in sp.bins)
+        D = len(self.grid)
+        self.D = D
+        self.NP = int(np.prod(self.grid))
+        self.dz = float(np.prod(self.bins))       # volume element
+        self.fg = tuple(2 * n for n in self.grid)  # mirrored calculation domain
+        self.L = int(np.prod(self.fg))
+        # full-domain index tuples (first dimension fastest), their folded primary index, and the
+        # primary points' positions in the full domain
+        full = np.indices(self.fg[::-1]).reshape(D, -1)[::-1]
+        fold = np.where(full < np.array(self.grid)[:, None], full, 2 * np.array(self.grid)[:, None] - 1 - full)
+        self.src = np.ravel_multi_index(tuple(fold), self.grid, order="F")
+        prim = np.indices(self.grid[::-1]).reshape(D, -1)[::-1]
+        self.prim = np.ravel_multi_index(tuple(prim), self.fg, order="F")
+        kv = np.stack([kgrid(self.fg[d], self.bins[d])[full[d]] for d in range(D)])   # [D, L]
+        self.kv = kv
+        ka = np.sqrt((kv ** 2).sum(0))
+        self.k = ka
+        nz = ka != 0
+        sig = ms.diameters
+        NB, MB = ms.NB, ms.MB
+
+        def wdelta(b):
+            out = np.empty(self.L)
+            out[nz] = 4.0 * PI * b * np.sin(b * ka[nz]) / ka[nz]
+            out[~nz] = 4.0 * PI * b ** 2
+            return out
+
+        def wstep(b):
+            out = np.empty(self.L)
+            a = ka[nz]
+            out[nz] = 4.0 * PI * (np.sin(b * a) - b * a * np.cos(b * a)) / a ** 3
+            out[~nz] = 4.0 / 3.0 * PI * b ** 3
+            return out
+
+        def wvec(b, v):
+            out = np.zeros(self.L)
+            a = ka[nz]
+            out[nz] = -4.0 * PI * (np.sin(b * a) - b * a * np.cos(b * a)) / a ** 3 * kv[v, nz]
+            return out
+
+        self.w2 = np.stack([wdelta(s / 2.0) for s in sig], 1)               # [L, NB]
+        self.w3 = np.stack([wstep(s / 2.0) for s in sig], 1)
+        self.wVd = np.stack([np.stack([wvec(s / 2.0, v) for s in sig], 1) for v in range(D)], 2)  # [L, NB, D]
+        self.wmsa = np.stack([wdelta(s / 2.0) / (4.0 * PI * (s / 2.0) ** 2) for s in sig], 1)
+        kc = 0.5 * (PI / max(self.bins))
+        filt = np.exp(-(ka / kc) ** 4)
+        self.wbond = np.zeros((self.L, MB))
+        self.wbond_filt = np.zeros((self.L, MB))
+        for u, (i, j) in enumerate(ms.bond_types):
+            sij = (sig[i] + sig[j]) / 2.0
+            self.wbond[:, u] = wdelta(sij) / (4.0 * PI * sij ** 2)
+            self.wbond_filt[:, u] = self.wbond[:, u] * filt
+        lap = np.zeros(self.L)
+        for v in range(D):
+            lap += (1.0 - np.cos(kv[v] * self.bins[v])) / self.bins[v] ** 2
+        self.wcoul = np.zeros(self.L)
+        first = np.zeros(self.L, dtype=bool)
+        first[0] = True                      # Iterators.drop(Rsys, 1): only the all-zero mode is skipped
+        self.wcoul[~first] = 2.0 * PI * ms.bjerrum / lap[~first]
+        if ms.sw is not None:
+            raise NotImplementedError("square-well functional on multi-dimensional grids")
+        self.wsw = None
+        self.surf_hat = self.F(self.mirror(sp.surf[:self.NP]))
+        self.mu_species = sp.mu_species.copy()
+        self.fix_lambda = [1.0] * len(ms.species)
+        self.rho_seg = None
+        self.rho_bonds = None
+        self.rho_beads = None
+        self.rho_beads_hat = None
+        self.rho_bonds_hat = None
+        self.n_objective = 0
+        self.bond_band = None
+        self.bond_taps = None
+
+    # ------------------------------------------------------------ transforms
+    def mirror(self, f: np.ndarray) -> np.ndarray:
+        """apply_mirroring! in every dimension: primary-domain rows (NP, ...) -> full domain (L, ...)."""
+        return np.asarray(f)[self.src]
+
+    def F(self, a: np.ndarray) -> np.ndarray:
+        sh = a.shape[1:]
+        A = a.reshape(self.fg + sh, order="F")
+        return np.fft.fftn(A, axes=tuple(range(self.D))).reshape((self.L,) + sh, order="F")
+
+    def Fi(self, a: np.ndarray) -> np.ndarray:
+        sh = a.shape[1:]
+        A = a.reshape(self.fg + sh, order="F")
+        return np.fft.ifftn(A, axes=tuple(range(self.D))).reshape((self.L,) + sh, order="F")
+
+    def _prop_conv(self, f: np.ndarray, bond: int) -> np.ndarray:
+        g = self.Fi(self.F(self.mirror(f)) * self.wbond_filt[:, bond])
+        return np.abs(g.real[self.prim])
+
+    # ------------------------------------------------------------ functionals
+    def _wd_beads(self, w, absval=True):
+        out = self.Fi(self.rho_beads_hat * w).real
+        return np.abs(out) if absval else out
+
+    def _fmt_weighted(self):
+        n2j = self._wd_beads(self.w2)
+        n3j = self._wd_beads(self.w3)
+        nVj = np.stack([self.Fi(self.rho_beads_hat * (1j * self.wVd[:, :, v])).real for v in range(self.D)], 2)
+        return n2j, n3j, nVj
+
+    def _bond_weighted(self, w_by_bead):
+        ms = self.ms
+        out = np.zeros((self.L, 2, ms.MB))
+        for u, (i, j) in enumerate(ms.bond_types):
+            for idx, bt in ((0, i), (1, j)):
+                out[:, idx, u] = np.abs(self.Fi(self.rho_bonds_hat[:, idx, u] * w_by_bead[:, bt]).real)
+        return out
+
+    def _weighted_totals(self, n2j, n3j, nVj, tpt1):
+        sig = self.ms.diameters
+        n0i = n2j / (PI * sig ** 2)
+        n1i = n2j / (2.0 * PI * sig)
+        nV1 = (nVj / (2.0 * PI * sig)[None, :, None]).sum(1)
+        nV2 = nVj.sum(1)
+        s3 = n3j.sum(1)
+        fac = np.where(s3 > 0.95, 0.95 / np.where(s3 > 0, s3, 1.0), 1.0)
+        n0 = (n0i * fac[:, None]).sum(1)
+        n1 = (n1i * fac[:, None]).sum(1)
+        n2 = (n2j * fac[:, None]).sum(1)
+        n3 = (n3j * fac[:, None]).sum(1)
+        return n0, n1, n2, n3, nV1 * fac[:, None], nV2 * fac[:, None], s3 * (fac if tpt1 else 1.0)
+
+    def _fmt_point_nd(self, n2j, n3j, nVj, wtb=None, want_fe=False):
+        """mfmt (wtb None) or TPT1mfmt per point of the full domain; returns the free energy, or
+        (d [P, 4], dv1 [P, D], dv2 [P, D], lng [P, MB])."""
+        ms = self.ms
+        sig = ms.diameters
+        tpt1 = wtb is not None
+        n0, n1, n2, n3, nV1, nV2, s3 = self._weighted_totals(n2j, n3j, nVj, tpt1)
+        if want_fe:
+            f = fmt_phi_nd(n0, n1, n2, n3, nV1, nV2)
+            if tpt1:
+                with np.errstate(divide="ignore", invalid="ignore"):
+                    Xi = 1.0 - (nV2 * nV2).sum(1) / n2 ** 2
+                    i1 = 1.0 / (1.0 - n3)
+                    t2 = n2 * Xi / 4.0 * i1 ** 2
+                    t3 = n2 ** 2 * Xi / 72.0 * i1 ** 3
+                    for u, (i, j) in enumerate(ms.bond_types):
+                        sij = (sig[i] + sig[j]) / 2
+                        tau = sig[i] * sig[j] / sij
+                        y = i1 + t2 * tau + t3 * tau ** 2
+                        f = f - 0.5 * (wtb[:, 0, u] + wtb[:, 1, u]) * np.log(y)
+            return f
+        P = n2j.shape[0]
+        d = np.zeros((P, 4))
+        dv1 = np.zeros((P, self.D))
+        dv2 = np.zeros((P, self.D))
+        lng = np.zeros((P, ms.MB))
+        live = (s3 if tpt1 else n3) > 1e-10
+        if live.any():
+            n0l, n1l, n2l, n3l, v1, v2 = n0[live], n1[live], n2[live], n3[live], nV1[live], nV2[live]
+            a1, a2, a3, a4, b1, b2 = fmt_deriv_nd(n0l, n1l, n2l, n3l, v1, v2)
+            if tpt1:
+                Xi = np.maximum(1.0 - (v2 * v2).sum(1) / n2l ** 2, 0.0)
+                i1 = 1.0 / (1.0 - n3l)
+                t2 = n2l * Xi / 4.0 * i1 ** 2
+                t3 = n2l ** 2 * Xi / 72.0 * i1 ** 3
+                wl = wtb[live]
+                for u, (i, j) in enumerate(ms.bond_types):
+                    sij = (sig[i] + sig[j]) / 2
+                    tau = sig[i] * sig[j] / sij
+                    y = i1 + t2 * tau + t3 * tau ** 2
+                    dn2 = (tau / 4.0 * i1 ** 2 + n2l * tau ** 2 * i1 ** 3 / 36.0) / y
+                    dn3 = (i1 ** 2 + n2l * Xi / 2.0 * i1 ** 3 * tau + n2l ** 2 * Xi / 24.0 * i1 ** 4 * tau ** 2) / y
+                    rb = wl[:, 0, u] + wl[:, 1, u]
+                    a3 = a3 - 0.5 * rb * dn2
+                    a4 = a4 - 0.5 * rb * dn3
+                    dvc = -(tau / (2.0 * n2l) * i1 ** 2 + tau ** 2 / 36.0 * i1 ** 3) / y
+                    b2 = b2 - (0.5 * rb * dvc)[:, None] * v2
+                    lng[live, u] = np.log(y)
+            d[live] = np.stack([a1, a2, a3, a4], 1)
+            dv1[live] = b1
+            dv2[live] = b2
+        return d, dv1, dv2, lng
+
+    def _add_fmt_back_nd(self, d, dv1, dv2, mu_ex_hat):
+        sig = self.ms.diameters
+        for j in range(self.ms.NB):
+            s = sig[j]
+            ch0 = d[:, 0] / (PI * s ** 2) + d[:, 1] / (2.0 * PI * s) + d[:, 2]
+            mu_ex_hat[:, j] += self.F(ch0) * self.w2[:, j]
+            mu_ex_hat[:, j] += self.F(d[:, 3]) * self.w3[:, j]
+            for v in range(self.D):
+                chv = dv1[:, v] / (2.0 * PI * s) + dv2[:, v]
+                mu_ex_hat[:, j] += self.F(chv) * (-1j * self.wVd[:, j, v])
+
+    def _add_lng_back(self, lngP, w_by_bead, lng_hat):
+        for u, (i, j) in enumerate(self.ms.bond_types):
+            for idx, bt in ((0, i), (1, j)):
+                lng_hat[:, idx, u] += self.F(lngP[:, u]) * w_by_bead[:, bt]
+
+    def _functionals(self):
+        ms = self.ms
+        NB, MB, L = ms.NB, ms.MB, self.L
+        mu_ex_hat = np.zeros((L, NB), complex)
+        lng_hat = np.zeros((L, 2, MB), complex)
+        sig, Z, lB = ms.diameters, ms.valences, ms.bjerrum
+        wn2b = self.w2 / sig[None, :] ** 2 / PI
+        for fn in self.sp.functionals:
+            if fn in ("mfmt", "TPT1mfmt"):
+                n2j, n3j, nVj = self._fmt_weighted()
+                if fn == "mfmt":
+                    d, dv1, dv2, _ = self._fmt_point_nd(n2j, n3j, nVj)
+                    self._add_fmt_back_nd(d, dv1, dv2, mu_ex_hat)
+                else:
+                    wtb = self._bond_weighted(wn2b)
+                    d, dv1, dv2, lngP = self._fmt_point_nd(n2j, n3j, nVj, wtb)
+                    self._add_fmt_back_nd(d, dv1, dv2, mu_ex_hat)
+                    self._add_lng_back(lngP, wn2b, lng_hat)
+            elif fn in ("msa", "TPT1msa"):
+                rb = self._wd_beads(self.wmsa)
+                if fn == "msa":
+                    dm = msa_deriv_vec(rb, sig, Z, lB)
+                else:
+                    wtb = self._bond_weighted(self.wmsa)
+                    dm, lngP, _ = tpt1msa_deriv_vec(wtb, ms.bond_types, rb, sig, Z, lB)
+                    self._add_lng_back(lngP, self.wmsa, lng_hat)
+                for j in range(NB):
+                    mu_ex_hat[:, j] += self.F(dm[:, j]) * self.wmsa[:, j]
+        mu = self.Fi(mu_ex_hat).real[self.prim]
+        lng = self.Fi(lng_hat).real[self.prim]
+        return mu, lng
+
+    def _coulomb(self, PsiC_old):
+        ms = self.ms
+        f = self.surf_hat.copy()
+        for j in range(ms.NB):
+            f = f + self.rho_beads_hat[:, j] * ms.valences[j]
+        Psi = self.Fi(f * self.wcoul).real[self.prim]
+        Qtot = self.sp.total_charge
+        nsp = len(ms.species)
+        Num = np.zeros(nsp)
+        qs = np.zeros(nsp)
+        for u, s in enumerate(ms.species):
+            for seg in range(s.n_seg):
+                for a, si in enumerate(s.state_family[seg]):
+                    num = float(np.sum(self.rho_seg[u][:, a, seg] * self.sp.trapez[:, si] * self.dz))
+                    Num[u] += num
+                    qs[u] += num * ms.valences[si]
+            Num[u] /= s.n_seg
+            qs[u] = 0.0 if Num[u] <= 0.0 else qs[u] / Num[u]
+        for u, s in enumerate(ms.species):
+            multi = any(len(f) > 1 for f in s.state_family)
+            if multi or s.evaluation == "simulation":
+                Qtot += Num[u] * qs[u]
+                qs[u] = 0.0
+        self.neutrality_inputs = (Qtot, Num.copy(), qs.copy())
+        return Psi, PsiC_old + neutrality_newton(Qtot, Num, qs)
+
+    # ------------------------------------------------------------ objective
+    def objective(self, x: np.ndarray) -> np.ndarray:
+        sp, ms = self.sp, self.ms
+        NP, NB, MB = self.NP, ms.NB, ms.MB
+        a = NP * NB
+        b = a + NP * 2 * MB
+        mu_ex = x[:a].reshape((NP, NB), order="F")
+        lng = x[a:b].reshape((NP, 2, MB), order="F")
+        Psi = x[b:b + NP]
+        PsiC = float(x[b + NP])
+        self._muex = mu_ex
+        lam = mu_ex + (Psi + PsiC)[:, None] * ms.valences[None, :] + sp.Ext + ms.delta_muH[None, :]
+        self.lambda_K = lam
+        rho_bonds = np.zeros((NP, 2, MB))
+        rho_seg = []
+        for u, s in enumerate(ms.species):
+            S = max(len(f) for f in s.state_family)
+            rs = np.zeros((NP, S, s.n_seg))
+            if s.evaluation != "simulation":
+                self._species_density(u, lam, lng, rs, rho_bonds)
+            rho_seg.append(rs)
+        beads = np.zeros((NP, NB))
+        for u, s in enumerate(ms.species):
+            for seg in range(s.n_seg):
+                for a_, si in enumerate(s.state_family[seg]):
+                    beads[:, si] += rho_seg[u][:, a_, seg]
+        self.rho_seg, self.rho_bonds, self.rho_beads = rho_seg, rho_bonds, beads
+        self.rho_beads_hat = self.F(self.mirror(beads * sp.trapez))
+        rbh = np.zeros((self.L, 2, MB), complex)
+        for u, (i, j) in enumerate(ms.bond_types):
+            for idx, bt in ((0, i), (1, j)):
+                rbh[:, idx, u] = self.F(self.mirror(rho_bonds[:, idx, u] * sp.trapez[:, bt]))
+        self.rho_bonds_hat = rbh
+        mu_new, lng_new = self._functionals()
+        Psi_new, PsiC_new = self._coulomb(PsiC)
+        self.n_objective += 1
+        return np.concatenate([mu_new.ravel(order="F"), lng_new.ravel(order="F"), Psi_new, [PsiC_new]])
+
+    def grand_potential(self, x: np.ndarray) -> float:
+        sp, ms = self.sp, self.ms
+        NP, NB, MB = self.NP, ms.NB, ms.MB
+        a = NP * NB
+        b = a + NP * 2 * MB
+        mu_ex = x[:a].reshape((NP, NB), order="F")
+        lng = x[a:b].reshape((NP, 2, MB), order="F")
+        sig, Z, lB = ms.diameters, ms.valences, ms.bjerrum
+        wn2b = self.w2 / sig[None, :] ** 2 / PI
+        pr = self.prim
+        om = 0.0
+        for fn in sp.functionals:
+            if fn in ("mfmt", "TPT1mfmt"):
+                n2j, n3j, nVj = self._fmt_weighted()
+                wtb = self._bond_weighted(wn2b)[pr] if fn == "TPT1mfmt" else None
+                f = self._fmt_point_nd(n2j[pr], n3j[pr], nVj[pr], wtb, want_fe=True)
+                om += float(np.sum(f))
+            elif fn in ("msa", "TPT1msa"):
+                rb = self._wd_beads(self.wmsa)[pr]
+                if fn == "msa":
+                    f = free_energy_msa_vec(rb, sig, Z, lB)
+                else:
+                    wtb = self._bond_weighted(self.wmsa)[pr]
+                    f = free_energy_tpt1msa_vec(wtb, ms.bond_types, rb, sig, Z, lB)
+                om += float(np.sum(f))
+        om -= float(np.sum(self.rho_beads * mu_ex * sp.trapez))
+        om += float(np.sum(self.rho_bonds * lng)) / 2.0
+        om *= self.dz
+        om -= float(np.sum(self.num_species()))
+        return om

@@ -79,6 +79,14 @@ class StatePoint:
     # wall (-1: no well for that bead) and the contact-theorem factor exp(energy) - 1
     sw_wall_idx: Optional[np.ndarray] = None   # [NB, 2] int
     sw_wall_fac: Optional[np.ndarray] = None   # [NB, 2]
+    # multi-dimensional grids (geometry_nd): points and bin width per dimension, first fastest; then
+    # NP = prod(grid), dz = prod(bins) (the volume element), Ext / trapez / surf flattened over the grid
+    grid: Optional[tuple] = None
+    bins: Optional[tuple] = None
+
+    @property
+    def ndim(self) -> int:
+        return 1 if self.grid is None else len(self.grid)
 
     @property
     def NB(self) -> int:
