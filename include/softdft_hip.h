@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define SDFT_ABI_VERSION 4
+#define SDFT_ABI_VERSION 5
 
 /* error codes */
 #define SDFT_OK 0
@@ -137,6 +137,18 @@ typedef struct sdft_desc {
      * wall (-1: none) and exp(energy) - 1; both NULL without square-well walls */
     const int32_t* sw_wall_idx;    /* [P][NB][2] */
     const double* sw_wall_fac;     /* [P][NB][2] */
+
+    /* multi-dimensional Cartesian grids (create_cartesian.jl:25-68; every dimension mirrored,
+     * process_cartesian.jl:14-45): ndim 0 or 1 is the one-dimensional path above. With ndim 2 or 3
+     * every point has grid[p][d] points and bin width bins[p][d] per dimension (first dimension
+     * fastest in every grid array), the same for all points of the context; NP[p] is their product
+     * (2 <= grid <= 256 per dimension) and dz[p] their volume element prod(bins). Grid arrays (Ext,
+     * trapez, surf) are flattened over the grid. Not supported there: grafted segments, the square-well
+     * functional, and the contact-value tables (hard_walls must be 0: the host forms the per-dimension
+     * contact values from the bead densities). grid, bins: [P][3] (entries past ndim ignored). */
+    int32_t ndim;
+    const int32_t* grid;
+    const double* bins;
 } sdft_desc_t;
 
 typedef struct sdft_solver_opts {

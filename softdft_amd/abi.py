@@ -18,7 +18,7 @@ from .problem import StatePoint
 LIB_NAME = "libsoftdft_hip.so"
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
-SDFT_ABI_VERSION = 4
+SDFT_ABI_VERSION = 5
 SDFT_OK = 0
 ERRORS = {-1: "SDFT_E_ARG", -2: "SDFT_E_HIP", -3: "SDFT_E_NORMALIZATION", -4: "SDFT_E_NOCONV",
           -5: "SDFT_E_UNSUPPORTED", -6: "SDFT_E_STATE"}
@@ -31,7 +31,7 @@ EXPORTED = ("sdft_abi_version", "sdft_last_error", "sdft_create", "sdft_destroy"
             "sdft_solution_offsets", "sdft_objective", "sdft_solve", "sdft_observables", "sdft_get_field",
             "sdft_last_timing", "sdft_last_pipeline_iters", "sdft_set_operator_band", "sdft_kernel_profile",
             "sdft_set_launch_mode", "sdft_autotune", "sdft_set_tuning", "sdft_observables_noalloc",
-            "sdft_last_switch_iters", "sdft_bulk_solve")
+            "sdft_last_switch_iters", "sdft_bulk_solve", "sdft_set_chemistry", "sdft_set_graft_density")
 TUNE_PROPAGATOR, TUNE_CONV_FWD, TUNE_CONV_PRE, TUNE_CONV_BACK = 1, 2, 3, 4
 CONV_TALL, CONV_WIDE64, CONV_WIDE128, CONV_MID, CONV_NARROW, CONV_FFT = 0, 1, 2, 3, 4, 5
 KP_PROP_FP32, KP_PROP_FP64, KP_CONV_FWD_FP32, KP_CONV_FWD_FP64, KP_CONV_BACK_FP32, KP_CONV_BACK_FP64, \
@@ -54,7 +54,8 @@ class SdftDesc(C.Structure):
                 ("Ext", _f64p), ("trapez", _f64p), ("surf", _f64p),
                 ("hard_walls", C.c_int32), ("wall_lo", _i32p), ("wall_hi", _i32p), ("wall_charge", _f64p),
                 ("n_sw_pairs", C.c_int32), ("sw_pairs", _i32p), ("sw_energy", _f64p), ("sw_lambda", _f64p),
-                ("sw_wall_idx", _i32p), ("sw_wall_fac", _f64p)]
+                ("sw_wall_idx", _i32p), ("sw_wall_fac", _f64p),
+                ("ndim", C.c_int32), ("grid", _i32p), ("bins", _f64p)]
 
 
 class SdftSolverOpts(C.Structure):
@@ -255,6 +256,19 @@ def build_descriptor(points: List[StatePoint]):
         d.sw_pairs = _ptr(keep["sw_pairs"], C.c_int32)
         d.sw_energy = _ptr(keep["sw_eps"], C.c_double)
         d.sw_lambda = _ptr(keep["sw_lam"], C.c_double)
+    nds = {sp.ndim for sp in points}
+    if len(nds) != 1:
+        raise ValueError("a batch mixes one- and multi-dimensional grids")
+    D = nds.pop()
+    if D > 1:
+        # multi-dimensional grids: per-point grid and bins; the contact values come from the host
+        keep["grid"] = np.ascontiguousarray(np.array([list(sp.grid) + [1] * (3 - D) for sp in points], dtype=np.int32))
+        keep["bins"] = np.ascontiguousarray(np.array([list(sp.bins) + [1.0] * (3 - D) for sp in points], dtype=np.float64))
+        d.ndim = D
+        d.grid = _ptr(keep["grid"], C.c_int32)
+        d.bins = _ptr(keep["bins"], C.c_double)
+        d.hard_walls = 0
+        d.wall_charge = None
     return d, keep, nst, nseg
 
 

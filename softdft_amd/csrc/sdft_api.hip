@@ -847,6 +847,45 @@ static int validate_desc(const sdft_desc_t* d) {
         set_error(nullptr, SDFT_E_ARG, "hard_walls = 1 needs both contact-index tables wall_lo and wall_hi");
         return SDFT_E_ARG;
     }
+    if (d->ndim > 1) {
+        if (d->ndim > 3 || !d->grid || !d->bins) {
+            set_error(nullptr, SDFT_E_ARG, "ndim = %d needs 2 or 3 dimensions and the grid and bins tables", d->ndim);
+            return SDFT_E_ARG;
+        }
+        for (int p = 0; p < P; ++p) {
+            int64_t prod = 1;
+            double vol = 1.0;
+            for (int k = 0; k < d->ndim; ++k) {
+                const int n = d->grid[(size_t)p * 3 + k];
+                const double h = d->bins[(size_t)p * 3 + k];
+                if (n < 2 || n > 256 || !(h > 0.0)) {
+                    set_error(nullptr, SDFT_E_ARG, "point %d dimension %d: %d points of width %g (2..256 points)", p, k, n, h);
+                    return SDFT_E_ARG;
+                }
+                if (n != d->grid[k] || h != d->bins[k]) {
+                    set_error(nullptr, SDFT_E_ARG, "point %d: every point of a multi-dimensional context needs the same grid", p);
+                    return SDFT_E_ARG;
+                }
+                prod *= n;
+                vol *= h;
+            }
+            if (prod != d->NP[p] || fabs(d->dz[p] - vol) > 1e-12 * vol) {
+                set_error(nullptr, SDFT_E_ARG, "point %d: NP = %d and dz = %g must be the grid's %lld points and volume element %g",
+                          p, d->NP[p], d->dz[p], (long long)prod, vol);
+                return SDFT_E_ARG;
+            }
+        }
+        if (d->hard_walls || d->wall_charge || d->sw_wall_idx || (d->functionals & SDFT_F_SW)) {
+            set_error(nullptr, SDFT_E_UNSUPPORTED, "multi-dimensional grids: no contact-value tables, wall charges table or "
+                                                   "square-well terms (the wall charge enters through surf)");
+            return SDFT_E_UNSUPPORTED;
+        }
+        for (int u = 0; u < nsp; ++u)
+            if (d->species_fixed_seg && d->species_fixed_seg[u] >= 0) {
+                set_error(nullptr, SDFT_E_UNSUPPORTED, "multi-dimensional grids: species %d has a grafted segment", u);
+                return SDFT_E_UNSUPPORTED;
+            }
+    }
     if ((d->sw_wall_idx != nullptr) != (d->sw_wall_fac != nullptr)) {
         set_error(nullptr, SDFT_E_ARG, "square-well walls need both sw_wall_idx and sw_wall_fac");
         return SDFT_E_ARG;
@@ -918,7 +957,10 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         set_error(nullptr, SDFT_E_UNSUPPORTED, "list at most one FMT and one MSA functional");
         return SDFT_E_UNSUPPORTED;
     }
-    if (validate_desc(d) != SDFT_OK) return SDFT_E_ARG;
+    {
+        const int vr = validate_desc(d);
+        if (vr != SDFT_OK) return vr;
+    }
     int dev_count = 0;
     if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) {
         set_error(nullptr, SDFT_E_HIP, "no HIP device available");
@@ -968,6 +1010,15 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         return fail(SDFT_E_ARG);
     }
     c.P = P; c.NB = NB; c.MB = MB; c.nsp = nsp; c.K = 3 * NB + MB + 1 + 2 * n_sw;
+    if (d->ndim > 1) {
+        c.nd = d->ndim;
+        c.ndp.D = d->ndim;
+        c.ndp.NPt = d->NP[0];
+        for (int k = 0; k < d->ndim; ++k) {
+            c.ndp.n[k] = d->grid[k];
+            c.ndp.h[k] = d->bins[k];
+        }
+    }
     ctx->lB = d->bjerrum;
     ctx->valences.assign(d->valences, d->valences + NB);
 
@@ -1128,6 +1179,10 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     c.Frt = NB + 2 * MB + sys.ncls;  // bead and bond-end densities x trapez, then one class sum per diameter
     c.Fnw = 3 * NB + 2 * MB + sys.n_sw_beads;  // ... then one square-well field per participating bead type
     c.Fdp = 3 * NB + 2 * MB + 1;
+    if (c.nd > 1) {   // one vector block of NB fields per dimension
+        c.Fnw = (2 + c.nd) * NB + 2 * MB;
+        c.Fdp = (2 + c.nd) * NB + 2 * MB + 1;
+    }
 
     // ---- points and kernel sets (one set per distinct (NP, dz))
     std::map<std::pair<int, uint64_t>, int> setmap;
@@ -1149,7 +1204,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         q.gbase = gsum * c.GF;
         gsum += q.gstride;
         q.n = (int64_t)NP * (NB + 2 * MB + 1) + 1;
-        q.dz = d->dz[p];
+        q.dz = d->dz[p];   // multi-dimensional grids: the volume element prod(bins) (validated)
         q.total_charge = d->total_charge ? d->total_charge[p] : 0.0;
         uint64_t bits;
         memcpy(&bits, &d->dz[p], 8);
@@ -1239,6 +1294,25 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     if ((rc = dalloc(ctx, &ctx->p32.trapez, c.sumNP * NB)) != SDFT_OK) return fail(rc);
     launch_convert_f32(ctx->stream, d_trap64, ctx->p32.trapez, c.sumNP * NB);
 
+    if (c.nd > 1) {
+        std::string nerr;
+        const int nrc = nd_setup(c, sys, fwd, pre, back, ctx->allocs, nerr);
+        if (nrc != SDFT_OK) {
+            set_error(ctx, nrc, "%s", nerr.c_str());
+            return fail(nrc);
+        }
+        hipError_t e = prepare_kernel_attributes(c);
+        if (e == hipSuccess) e = nd_prepare_attributes(c);
+        if (e != hipSuccess) {
+            set_error(ctx, SDFT_E_HIP, "hipFuncSetAttribute: %s", hipGetErrorString(e));
+            return fail(SDFT_E_HIP);
+        }
+        c.use_gemm = false;
+        c.merge_back = false;
+        ctx->jobs_h = jobs;
+        c.prop_plan.assign(nsp, OpPlan());
+        c.chain_plan.assign(nsp, ChainPlanHost());
+    } else {
     // kernel vectors
     int32_t* d_setL = nullptr;
     int64_t* d_setkb = nullptr;
@@ -1338,6 +1412,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
             }
         }
     }
+    }   // one-dimensional kernel vectors and plans
 
     // ---- work buffers
     auto alloc_pipe = [&](auto& pp) -> int {
@@ -1352,6 +1427,10 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         TRY(dalloc(ctx, &pp.rhoT, c.sumNP * c.Frt));
         TRY(dalloc(ctx, &pp.nw, c.sumNP * c.Fnw));
         TRY(dalloc(ctx, &pp.dphi, c.sumNP * c.Fdp));
+        if (c.nd > 1) {
+            TRY(dalloc(ctx, &pp.nd_a, (int64_t)P * c.ndp.maxF * c.ndp.NPt));
+            TRY(dalloc(ctx, &pp.nd_b, (int64_t)P * c.ndp.maxF * c.ndp.NPt));
+        }
         return SDFT_OK;
     };
     {
@@ -1972,6 +2051,10 @@ extern "C" int32_t sdft_autotune(sdft_ctx_t* ctx, int32_t reps, sdft_tune_entry_
                                  int32_t* n_entries) {
     if (!ctx || reps < 1 || (capacity > 0 && !entries)) return SDFT_E_ARG;
     Ctx& c = ctx->c;
+    if (ctx->c.nd > 1) {
+        set_error(ctx, SDFT_E_UNSUPPORTED, "multi-dimensional grids have no tunable launch decisions or kernel classes");
+        return SDFT_E_UNSUPPORTED;
+    }
     ctx->tuning.clear();
     EvalflagGuard flags(ctx->stream, c.d_evalflag, c.P);
     HIP_CHECK_RET(flags.save());
@@ -1996,6 +2079,10 @@ extern "C" int32_t sdft_autotune(sdft_ctx_t* ctx, int32_t reps, sdft_tune_entry_
 extern "C" int32_t sdft_set_tuning(sdft_ctx_t* ctx, const sdft_tune_entry_t* entries, int32_t n) {
     if (!ctx || n < 0 || (n > 0 && !entries)) return SDFT_E_ARG;
     Ctx& c = ctx->c;
+    if (ctx->c.nd > 1) {
+        set_error(ctx, SDFT_E_UNSUPPORTED, "multi-dimensional grids have no tunable launch decisions or kernel classes");
+        return SDFT_E_UNSUPPORTED;
+    }
     bool conv = false;
     for (int i = 0; i < n; ++i) {
         const sdft_tune_entry_t& e = entries[i];
@@ -2161,6 +2248,10 @@ extern "C" int32_t sdft_bulk_solve(const sdft_bulk_desc_t* d, double tol, int32_
 
 extern "C" int32_t sdft_kernel_profile(sdft_ctx_t* ctx, int32_t kc, int32_t reps, double* avg_ms, double* flops) {
     if (!ctx || reps < 1) return SDFT_E_ARG;
+    if (ctx->c.nd > 1) {
+        set_error(ctx, SDFT_E_UNSUPPORTED, "multi-dimensional grids have no tunable launch decisions or kernel classes");
+        return SDFT_E_UNSUPPORTED;
+    }
     if (!ctx->have_eval) {
         set_error(ctx, SDFT_E_STATE, "kernel profiling needs a prior sdft_solve or sdft_objective");
         return SDFT_E_STATE;

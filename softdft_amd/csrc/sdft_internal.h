@@ -22,7 +22,7 @@
 #define SDFT_SW_TERMS 3         // square-well partners per bead type (terms of one convolution job)
 #define SDFT_MAX_M 8
 #define SDFT_ACC_POINTS 64          // grid points per density-accumulation block
-#define SDFT_MAX_TILES 256          // accumulation blocks per grid (NP <= 256 * 64 = 16384)
+#define SDFT_MAX_TILES 2048         // accumulation blocks per grid (NP <= 2048 * 64 = 131072)
 #define SDFT_GRAM_BLOCKS 64         // most blocks per point for the Anderson reductions
 #define SDFT_GRAM_CHUNK 2048        // solution entries per reduction block
 #define SDFT_GRAM_LEN 44            // SDFT_MAX_M*(SDFT_MAX_M+1)/2 + SDFT_MAX_M

@@ -181,19 +181,6 @@ __global__ void k_prepare(const DevSystem* __restrict__ psys, const DevPoint* __
 // max_distance zeroing of child_propagators.jl:147-157.
 // --------------------------------------------------------------------------
 
-struct PropArgs {
-    int u;            // species
-    int s;            // step 1..N-1
-    int nseg, S;
-    int fixed_seg;
-    int state_off, maxd_off;
-    int64_t goff;     // field offset of this species in the gC / gP families
-    int GF;           // fields per point of the gC / gP families
-    int band;         // >= 0: banded Toeplitz window (FP32 phase), -1: dense
-    int rescale;      // per-step rescaling (scale_slot tables)
-    int nsp, nmax;
-    unsigned long long* smax;
-};
 
 
 template <typename T, int TB>
@@ -792,15 +779,29 @@ __global__ void __launch_bounds__(TB) k_conv(const DevSystem* __restrict__ psys,
 //   dphi[3NB + 2MB]  = surf + sum_j z_j rho_j trapez_j                                     -> Coulomb
 // --------------------------------------------------------------------------
 
+// weighted densities of the mixture at one point; D vector components (one per grid dimension)
+template <int D>
 struct FmtIn {
-    double n0, n1, n2, n3, nV1, nV2;
+    double n0, n1, n2, n3, nV1[D], nV2[D];
     double s3;   // sum of capped n3_i (TPT1) or capped n3
 };
 
-// weighted densities of the mixture at point n, read from the nw family
-template <typename T>
-__device__ FmtIn fmt_load(const DevSystem& sys, const DevPoint& P, int n, const T* __restrict__ nw, int Fnw) {
-    FmtIn f{0, 0, 0, 0, 0, 0, 0};
+template <int D>
+__device__ __forceinline__ double dotv(const double* a, const double* b) {
+    double r = a[0] * b[0];
+#pragma unroll
+    for (int v = 1; v < D; ++v) r += a[v] * b[v];
+    return r;
+}
+
+// weighted densities of the mixture at point n, read from the nw family: n2_j at j, n3_j at NB + j,
+// the vector component v of diameter class c at (2 + v) NB + c
+template <int D, typename T>
+__device__ FmtIn<D> fmt_load(const DevSystem& sys, const DevPoint& P, int n, const T* __restrict__ nw, int Fnw) {
+    FmtIn<D> f;
+    f.n0 = f.n1 = f.n2 = f.n3 = f.s3 = 0.0;
+#pragma unroll
+    for (int v = 0; v < D; ++v) f.nV1[v] = f.nV2[v] = 0.0;
     const int NB = sys.NB;
     for (int j = 0; j < NB; ++j) {
         const double s = sys.sigma[j];
@@ -813,41 +814,52 @@ __device__ FmtIn fmt_load(const DevSystem& sys, const DevPoint& P, int n, const 
     // the vector weighted densities are linear in rho: one per diameter class (beads share w_V)
     for (int c = 0; c < sys.ncls; ++c) {
         const double s = sys.cls_sigma[c];
-        const double nv = (double)nw[fidx(P, Fnw, 2 * NB + c, n)];
-        f.nV1 += nv / (2.0 * M_PI * s);
-        f.nV2 += nv;
+#pragma unroll
+        for (int v = 0; v < D; ++v) {
+            const double nv = (double)nw[fidx(P, Fnw, (2 + v) * NB + c, n)];
+            f.nV1[v] += nv / (2.0 * M_PI * s);
+            f.nV2[v] += nv;
+        }
     }
     if (f.n3 > 0.95) {  // cap (mfmt.jl:126-131 / TPT1mfmt.jl:154-159: uniform scaling)
         const double fac = 0.95 / f.n3;
-        f.n0 *= fac; f.n1 *= fac; f.n2 *= fac; f.n3 *= fac; f.nV1 *= fac; f.nV2 *= fac;
+        f.n0 *= fac; f.n1 *= fac; f.n2 *= fac; f.n3 *= fac;
+#pragma unroll
+        for (int v = 0; v < D; ++v) { f.nV1[v] *= fac; f.nV2[v] *= fac; }
     }
     f.s3 = f.n3;
     return f;
 }
 
-// compute_mfmt_deriv!: rows 1..4, 5 (nV1_z), 8 (nV2_z)
-__device__ void fmt_deriv(const FmtIn& f, double* d) {
+// compute_mfmt_deriv! (mfmt.jl:210-244): d[0..3] the scalar rows, dv1[v] / dv2[v] the vector rows
+// 4 + v and 7 + v
+template <int D>
+__device__ void fmt_deriv(const FmtIn<D>& f, double* d, double* dv1, double* dv2) {
     const double om = 1.0 - f.n3;
     const double lg = log(om);
     const double i1 = 1.0 / om;
     const double i3 = 1.0 / f.n3;
     const double c = 1.0 / (36.0 * M_PI);
-    const double dot12 = f.nV1 * f.nV2, dot22 = f.nV2 * f.nV2;
+    const double dot12 = dotv<D>(f.nV1, f.nV2), dot22 = dotv<D>(f.nV2, f.nV2);
     d[0] = -lg;
     d[1] = f.n2 * i1;
     d[2] = f.n1 * i1 + (f.n2 * f.n2 - dot22) * 3.0 * c * i3 * (lg * i3 + i1 * i1);
     d[3] = f.n0 * i1 + (f.n1 * f.n2 - dot12) * i1 * i1 -
            (lg * i3 * i3 * i3 * 2.0 * c + i3 * i3 * i1 * c + (1.0 - 3.0 * f.n3) * i3 * i3 * c * i1 * i1 * i1) *
                (f.n2 * f.n2 * f.n2 - 3.0 * f.n2 * dot22);
-    d[4] = -f.nV2 * i1;
-    d[5] = -f.nV1 * i1 - (lg * i3 + i1 * i1) * f.n2 * f.nV2 * i3 * 6.0 * c;
+#pragma unroll
+    for (int v = 0; v < D; ++v) {
+        dv1[v] = -f.nV2[v] * i1;
+        dv2[v] = -f.nV1[v] * i1 - (lg * i3 + i1 * i1) * f.n2 * f.nV2[v] * i3 * 6.0 * c;
+    }
 }
 
 // free_energy_mfmt (inhomogeneous); cubic term takes its n3 -> 0 limit (0) at n3 == 0
-__device__ double fmt_phi(const FmtIn& f) {
+template <int D>
+__device__ double fmt_phi(const FmtIn<D>& f) {
     const double om = 1.0 - f.n3;
     const double lg = log(om);
-    const double dot12 = f.nV1 * f.nV2, dot22 = f.nV2 * f.nV2;
+    const double dot12 = dotv<D>(f.nV1, f.nV2), dot22 = dotv<D>(f.nV2, f.nV2);
     double cubic = 0.0;
     if (f.n3 != 0.0)
         cubic = (f.n2 * f.n2 * f.n2 - 3.0 * f.n2 * dot22) * (f.n3 + om * om * lg) / (36.0 * M_PI * f.n3 * f.n3 * om * om);
@@ -958,9 +970,17 @@ static void cls_dispatch(int ncls, F&& f) {
     else f(std::integral_constant<int, SDFT_MAX_NB>{});
 }
 
+// calls f(std::integral_constant<int, D>) with the grid dimension (1, 2 or 3)
+template <class F>
+static void nd_dispatch(int nd, F&& f) {
+    if (nd == 3) f(std::integral_constant<int, 3>{});
+    else if (nd == 2) f(std::integral_constant<int, 2>{});
+    else f(std::integral_constant<int, 1>{});
+}
+
 // pointwise functional derivatives, FP64 arithmetic, no per-bead arrays:
 // the weighted densities are re-read from nw where a second pass needs them
-template <typename T, int NCL>
+template <typename T, int NCL, int D>
 __global__ void __launch_bounds__(256) k_pointwise(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
                                                   const int32_t* __restrict__ evalflag, int prec,
                                                   const T* __restrict__ nw, int Fnw, const T* __restrict__ rhoT, int Frt,
@@ -978,13 +998,17 @@ __global__ void __launch_bounds__(256) k_pointwise(const DevSystem* __restrict__
     const bool tpt1msa = (F & SDFT_F_TPT1MSA) != 0;
 
     // ---- FMT (mfmt.jl:106-199, TPT1mfmt.jl:135-241)
-    FmtIn f{0, 0, 0, 0, 0, 0, 0};
+    constexpr int VB = 2 + D;   // bond fields start at VB * NB in nw and dphi
+    FmtIn<D> f;
+    f.n0 = f.n1 = f.n2 = f.n3 = f.s3 = 0.0;
     bool fmt_on = false;
-    double d[6] = {0, 0, 0, 0, 0, 0};
+    double d[4] = {0, 0, 0, 0}, dv1[D], dv2[D];
+#pragma unroll
+    for (int v = 0; v < D; ++v) { f.nV1[v] = f.nV2[v] = 0.0; dv1[v] = dv2[v] = 0.0; }
     if (has_fmt) {
-        f = fmt_load(sys, P, n, nw, Fnw);
+        f = fmt_load<D>(sys, P, n, nw, Fnw);
         fmt_on = f.s3 > 1e-10;
-        if (fmt_on) fmt_deriv(f, d);
+        if (fmt_on) fmt_deriv<D>(f, d, dv1, dv2);
     }
 
     // ---- MSA (msa.jl:160-230, TPT1msa.jl:210-282)
@@ -1025,7 +1049,7 @@ __global__ void __launch_bounds__(256) k_pointwise(const DevSystem* __restrict__
     const bool tpt1fmt_on = fmt_on && (F & SDFT_F_TPT1MFMT);
     double Xif = 0.0, i1 = 0.0, t2 = 0.0, t3 = 0.0;
     if (tpt1fmt_on) {
-        Xif = fmax(1.0 - f.nV2 * f.nV2 / (f.n2 * f.n2), 0.0);
+        Xif = fmax(1.0 - dotv<D>(f.nV2, f.nV2) / (f.n2 * f.n2), 0.0);
         i1 = 1.0 / (1.0 - f.n3);
         t2 = f.n2 * Xif / 4.0 * i1 * i1;
         t3 = f.n2 * f.n2 * Xif / 72.0 * i1 * i1 * i1;
@@ -1036,17 +1060,20 @@ __global__ void __launch_bounds__(256) k_pointwise(const DevSystem* __restrict__
         const double sij = (si + sj) / 2.0;
         double lng = 0.0;
         if (tpt1fmt_on || tpt1msa) {
-            const double rb = (double)nw[fidx(P, Fnw, 3 * NB + 2 * u, n)] + (double)nw[fidx(P, Fnw, 3 * NB + 2 * u + 1, n)];
+            const double rb = (double)nw[fidx(P, Fnw, VB * NB + 2 * u, n)] + (double)nw[fidx(P, Fnw, VB * NB + 2 * u + 1, n)];
             if (tpt1fmt_on) {
                 const double tau = si * sj / sij;
                 const double y = i1 + t2 * tau + t3 * tau * tau;
                 const double dn2 = (tau / 4.0 * i1 * i1 + f.n2 * tau * tau * i1 * i1 * i1 / 36.0) / y;
                 const double dn3 = (i1 * i1 + f.n2 * Xif / 2.0 * i1 * i1 * i1 * tau +
                                     f.n2 * f.n2 * Xif / 24.0 * i1 * i1 * i1 * i1 * tau * tau) / y;
-                const double dv = -f.nV2 * (tau / (2.0 * f.n2) * i1 * i1 + tau * tau / 36.0 * i1 * i1 * i1) / y;
                 d[2] -= 0.5 * rb * dn2;
                 d[3] -= 0.5 * rb * dn3;
-                d[5] -= 0.5 * rb * dv;
+#pragma unroll
+                for (int v = 0; v < D; ++v) {
+                    const double dv = -f.nV2[v] * (tau / (2.0 * f.n2) * i1 * i1 + tau * tau / 36.0 * i1 * i1 * i1) / y;
+                    dv2[v] -= 0.5 * rb * dv;
+                }
                 lng += log(y);
             }
             if (tpt1msa) {
@@ -1066,19 +1093,22 @@ __global__ void __launch_bounds__(256) k_pointwise(const DevSystem* __restrict__
                 }
             }
         }
-        dphi[fidx(P, Fdp, 3 * NB + 2 * u, n)] = (T)(lng / (M_PI * si * si));
-        dphi[fidx(P, Fdp, 3 * NB + 2 * u + 1, n)] = (T)(lng / (M_PI * sj * sj));
+        dphi[fidx(P, Fdp, VB * NB + 2 * u, n)] = (T)(lng / (M_PI * si * si));
+        dphi[fidx(P, Fdp, VB * NB + 2 * u + 1, n)] = (T)(lng / (M_PI * sj * sj));
     }
 
     // ---- bead outputs
     double q = surf[P.base + n];
     for (int j = 0; j < NB; ++j) {
         const double s = sys.sigma[j], z = sys.z[j];
-        double ch0 = 0.0, ch1 = 0.0, chv = 0.0;
+        double ch0 = 0.0, ch1 = 0.0, chv[D];
+#pragma unroll
+        for (int v = 0; v < D; ++v) chv[v] = 0.0;
         if (fmt_on) {
             ch0 = d[0] / (M_PI * s * s) + d[1] / (2.0 * M_PI * s) + d[2];
             ch1 = d[3];
-            chv = d[4] / (2.0 * M_PI * s) + d[5];
+#pragma unroll
+            for (int v = 0; v < D; ++v) chv[v] = dv1[v] / (2.0 * M_PI * s) + dv2[v];
         }
         if (has_msa) {
             const double ig = 1.0 / (1.0 + m.G * s);
@@ -1097,10 +1127,11 @@ __global__ void __launch_bounds__(256) k_pointwise(const DevSystem* __restrict__
         }
         dphi[fidx(P, Fdp, j, n)] = (T)ch0;
         dphi[fidx(P, Fdp, NB + j, n)] = (T)ch1;
-        dphi[fidx(P, Fdp, 2 * NB + j, n)] = (T)chv;
+#pragma unroll
+        for (int v = 0; v < D; ++v) dphi[fidx(P, Fdp, (2 + v) * NB + j, n)] = (T)chv[v];
         q += z * (double)rhoT[fidx(P, Frt, j, n)];
     }
-    dphi[fidx(P, Fdp, 3 * NB + 2 * MB, n)] = (T)q;
+    dphi[fidx(P, Fdp, VB * NB + 2 * MB, n)] = (T)q;
 }
 
 // --------------------------------------------------------------------------
@@ -1481,7 +1512,7 @@ __global__ void __launch_bounds__(256) k_and_update(const DevPoint* __restrict__
 // --------------------------------------------------------------------------
 // grand potential integrand on the primary points (grand_potential.jl:4-28):
 //   f_ex(K) - sum_j rho_j mu_ex_j trapez_j + 1/2 sum rho_bond lng
-template <typename T, int NCL>
+template <typename T, int NCL, int D>
 __global__ void __launch_bounds__(256) k_omega(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
                                               const DevSolveState* __restrict__ stt,
                                               const T* __restrict__ nw, int Fnw, const T* __restrict__ rhoT, int Frt,
@@ -1503,10 +1534,10 @@ __global__ void __launch_bounds__(256) k_omega(const DevSystem* __restrict__ psy
                      (double)nw[fidx(P, Fnw, sys.nw_sw_base + s, n)];
         }
         if (F & (SDFT_F_MFMT | SDFT_F_TPT1MFMT)) {
-            const FmtIn fi = fmt_load(sys, P, n, nw, Fnw);
-            f += fmt_phi(fi);
+            const FmtIn<D> fi = fmt_load<D>(sys, P, n, nw, Fnw);
+            f += fmt_phi<D>(fi);
             if (F & SDFT_F_TPT1MFMT) {
-                const double Xi = 1.0 - fi.nV2 * fi.nV2 / (fi.n2 * fi.n2);  // no max(., 0) in the free energy
+                const double Xi = 1.0 - dotv<D>(fi.nV2, fi.nV2) / (fi.n2 * fi.n2);  // no max(., 0) in the free energy
                 const double i1 = 1.0 / (1.0 - fi.n3);
                 const double t2 = fi.n2 * Xi / 4.0 * i1 * i1;
                 const double t3 = fi.n2 * fi.n2 * Xi / 72.0 * i1 * i1 * i1;
@@ -1514,8 +1545,8 @@ __global__ void __launch_bounds__(256) k_omega(const DevSystem* __restrict__ psy
                     const double si = sys.sigma[sys.bond_i[u]], sj = sys.sigma[sys.bond_j[u]];
                     const double sij = (si + sj) / 2.0, tau = si * sj / sij;
                     const double y = i1 + t2 * tau + t3 * tau * tau;
-                    const double rb = (double)nw[fidx(P, Fnw, 3 * NB + 2 * u, n)] +
-                                      (double)nw[fidx(P, Fnw, 3 * NB + 2 * u + 1, n)];
+                    const double rb = (double)nw[fidx(P, Fnw, (2 + D) * NB + 2 * u, n)] +
+                                      (double)nw[fidx(P, Fnw, (2 + D) * NB + 2 * u + 1, n)];
                     f -= 0.5 * rb * log(y);
                 }
             }
@@ -1536,8 +1567,8 @@ __global__ void __launch_bounds__(256) k_omega(const DevSystem* __restrict__ psy
                     const double sij = (si + sj) / 2.0;
                     const double Xi = (sys.z[i] - m.eta * si * si) / (1.0 + m.G * si);
                     const double Xj = (sys.z[j] - m.eta * sj * sj) / (1.0 + m.G * sj);
-                    const double rb = (double)nw[fidx(P, Fnw, 3 * NB + 2 * u, n)] +
-                                      (double)nw[fidx(P, Fnw, 3 * NB + 2 * u + 1, n)];
+                    const double rb = (double)nw[fidx(P, Fnw, (2 + D) * NB + 2 * u, n)] +
+                                      (double)nw[fidx(P, Fnw, (2 + D) * NB + 2 * u + 1, n)];
                     fe -= 0.5 * rb * (-sys.lB / sij * Xi * Xj);
                 }
             }
@@ -1872,6 +1903,12 @@ void Pipeline<T>::propagate(hipStream_t st, const Ctx& c, int prec, int u, bool 
     }
 }
 
+void launch_scale_prefix(hipStream_t st, const Ctx& c, int prec, int u, unsigned long long* smax, double* lsc) {
+    const SpeciesHost& sh = c.species[u];
+    hipLaunchKernelGGL(k_scale_prefix, dim3((c.P + 63) / 64), dim3(64), 0, st, c.d_evalflag, prec, c.P, u, sh.nseg,
+                       sh.fixed_seg, smax, lsc, c.nsp, c.nmax_seg);
+}
+
 template <typename T>
 void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double* x, double* fx) {
     const int P = c.P;
@@ -1883,11 +1920,17 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
         hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st,
                            reinterpret_cast<double*>(smax), ns);
     }
-    for (int u = 0; u < c.nsp; ++u) propagate(st, c, prec, u, true);
+    const bool nd = c.nd > 1;
+    for (int u = 0; u < c.nsp; ++u) {
+        if (nd) propagate_nd(st, c, prec, u);
+        else propagate(st, c, prec, u, true);
+    }
     launch_accumulate(st, c, prec);
-    launch_conv(st, c, prec, false, fx);
+    if (nd) conv_nd(st, c, prec, false, fx);
+    else launch_conv(st, c, prec, false, fx);
     launch_pointwise(st, c, prec);
-    launch_conv(st, c, prec, true, fx);
+    if (nd) conv_nd(st, c, prec, true, fx);
+    else launch_conv(st, c, prec, true, fx);
     hipLaunchKernelGGL(k_neutrality, dim3(P), dim3(64), 0, st, c.d_sys, c.d_pts, c.d_spec, c.d_evalflag, prec,
                        c.d_acc_part, c.d_acc_species, x, fx);
 }
@@ -1950,8 +1993,11 @@ void Pipeline<T>::launch_pointwise(hipStream_t st, const Ctx& c, int prec) {
     T* dphi_ = dphi;
     cls_dispatch(c.ncls, [&](auto k) {
         constexpr int NCL = decltype(k)::value;
-        hipLaunchKernelGGL((k_pointwise<T, NCL>), grid2(c.maxNP, 256, c.P), dim3(256), 0, st, c.d_sys, c.d_pts,
-                           c.d_evalflag, prec, nw_, c.Fnw, rhoT_, c.Frt, c.d_surf, dphi_, c.Fdp);
+        nd_dispatch(c.nd, [&](auto dd) {
+            constexpr int D = decltype(dd)::value;
+            hipLaunchKernelGGL((k_pointwise<T, NCL, D>), grid2(c.maxNP, 256, c.P), dim3(256), 0, st, c.d_sys, c.d_pts,
+                               c.d_evalflag, prec, nw_, c.Fnw, rhoT_, c.Frt, c.d_surf, dphi_, c.Fdp);
+        });
     });
 }
 
@@ -1960,8 +2006,11 @@ void Pipeline<T>::omega(hipStream_t st, const Ctx& c, const double* x, double* o
     const T *nw_ = nw, *rhoT_ = rhoT, *rb_ = rho_bonds, *rbe_ = rho_beads;
     cls_dispatch(c.ncls, [&](auto k) {
         constexpr int NCL = decltype(k)::value;
-        hipLaunchKernelGGL((k_omega<T, NCL>), dim3(SDFT_OMEGA_BLOCKS, c.P), dim3(256), 0, st, c.d_sys, c.d_pts,
-                           c.d_state, nw_, c.Fnw, rhoT_, c.Frt, rb_, rbe_, x, omega_acc);
+        nd_dispatch(c.nd, [&](auto dd) {
+            constexpr int D = decltype(dd)::value;
+            hipLaunchKernelGGL((k_omega<T, NCL, D>), dim3(SDFT_OMEGA_BLOCKS, c.P), dim3(256), 0, st, c.d_sys, c.d_pts,
+                               c.d_state, nw_, c.Fnw, rhoT_, c.Frt, rb_, rbe_, x, omega_acc);
+        });
     });
 }
 

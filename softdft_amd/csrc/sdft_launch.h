@@ -113,6 +113,35 @@ struct ChainPlanHost {
                                                    // exp(lng/2)/normative tabulated in LDS
 };
 
+// multi-dimensional grids (sdft_nd.hip): one convolution column of the spectral path
+struct NdTerm { int32_t kern, infield; };   // spectral kernel index, input field of the stage's family
+struct NdJob {
+    int32_t nterms;
+    NdTerm t[4];
+    int32_t odd_axis;   // the output is odd along this axis (vector weighted density), -1: even
+    ConvJob out;        // output semantics (destination, |.|, scale, shared partials)
+};
+// every point of a multi-dimensional context shares one grid NP_1 x .. x NP_D (first fastest)
+struct NdPlan {
+    int D = 1;
+    int n[3] = {1, 1, 1};
+    double h[3] = {1.0, 1.0, 1.0};
+    int64_t NPt = 0;
+    int K = 0;                     // spectral kernels: (2 + D) per bead (w2, w3, w_V per axis), bonds, Coulomb
+    int kbond0 = 0;                // first bond kernel
+    double* kspec64 = nullptr;     // [K][NPt] weights on the mode grid
+    float* kspec32 = nullptr;
+    double* mat64[3][4] = {};      // per axis: forward even / odd, inverse even / odd (transposed n x n)
+    float* mat32[3][4] = {};
+    NdJob* d_fwd = nullptr;        // forward jobs (vector jobs expanded per axis)
+    NdJob* d_back = nullptr;       // shared partials, lng and Psi columns, then the projections
+    int nfwd = 0, npre = 0, nback = 0;
+    int8_t* d_par_fwdout = nullptr;  // per forward job: odd axis of its output
+    int8_t* d_par_dphi = nullptr;    // per dphi field: odd axis (vector block of axis v), -1 even
+    int8_t* d_par_even = nullptr;    // all -1
+    int maxF = 0;                  // most fields transformed at once per point
+};
+
 struct SpeciesHost {
     int nseg, S, model, fixed_seg, multistate, rescale;
     int state_off, maxd_off;
@@ -178,6 +207,8 @@ struct Ctx {
     bool any_rescale = false;       // some species propagates with per-step rescaling
     // FFT convolutions: per kernel set resources, and the spectra work buffers per precision
     // (0: FP32, 1: FP64; complex elements: inputs and outputs of the largest FFT stage)
+    int nd = 1;                     // grid dimensions (2 or 3: the spectral path of sdft_nd.hip)
+    NdPlan ndp;
     std::vector<FftRes> fft_res;
     void* fft_sa[2] = {nullptr, nullptr};
     void* fft_sb[2] = {nullptr, nullptr};
@@ -203,6 +234,21 @@ __device__ __forceinline__ void scale_note(unsigned long long* smax, int64_t i, 
 __device__ __forceinline__ const double* scale_row(const double* lsc, int p, int u, int nsp, int nmax) {
     return lsc + ((int64_t)p * nsp + u) * (nmax + 1);
 }
+
+// arguments of one propagator step launch (species, step, layout of its propagator fields)
+struct PropArgs {
+    int u;            // species
+    int s;            // step 1..N-1
+    int nseg, S;
+    int fixed_seg;
+    int state_off, maxd_off;
+    int64_t goff;     // field offset of this species in the gC / gP families
+    int GF;           // fields per point of the gC / gP families
+    int band;         // >= 0: banded Toeplitz window (FP32 phase), -1: dense
+    int rescale;      // per-step rescaling (scale_slot tables)
+    int nsp, nmax;
+    unsigned long long* smax;
+};
 
 // propagator input at an end of the recursion: 1 at a free chain end; for a grafted chain 1 within
 // reach (max_distance) of the grafting point and 0 beyond, and at the grafted segment itself only at
@@ -249,6 +295,8 @@ struct Pipeline {
     T* fbuf = nullptr;      // chained propagator inputs, two step buffers of (2 sweeps x S x S) fields
     unsigned long long* smax = nullptr;  // per-step scale factors (see scale_slot)
     double* lsc = nullptr;               // summed log scales per segment (see scale_row)
+    T* nd_a = nullptr;                   // multi-dimensional grids: transform work buffers,
+    T* nd_b = nullptr;                   // [P][maxF][NPt] each
     void objective(hipStream_t st, const Ctx& c, int prec, const double* x, double* fx);
     void omega(hipStream_t st, const Ctx& c, const double* x, double* omega_acc);
     void segment_sums(hipStream_t st, const Ctx& c, int u, double* out, int64_t stride);
@@ -259,7 +307,18 @@ struct Pipeline {
     void launch_conv(hipStream_t st, const Ctx& c, int prec, bool back, double* fx);
     void launch_pointwise(hipStream_t st, const Ctx& c, int prec);
     void launch_accumulate(hipStream_t st, const Ctx& c, int prec);
+    // multi-dimensional grids (sdft_nd.hip)
+    void propagate_nd(hipStream_t st, const Ctx& c, int prec, int u);
+    void conv_nd(hipStream_t st, const Ctx& c, int prec, bool back, double* fx);
 };
+
+// multi-dimensional setup (sdft_nd.hip): spectral weights, axis matrices, job tables from the
+// one-dimensional ones; device buffers are recorded in allocs (freed with the context)
+int nd_setup(Ctx& c, const DevSystem& sys, const std::vector<ConvJob>& fwd1, const std::vector<ConvJob>& pre1,
+             const std::vector<ConvJob>& back1, std::vector<void*>& allocs, std::string& err);
+hipError_t nd_prepare_attributes(const Ctx& c);
+size_t nd_axis_lds_bytes(int n, size_t elem);
+void launch_scale_prefix(hipStream_t st, const Ctx& c, int prec, int u, unsigned long long* smax, double* lsc);
 
 void launch_compact_cols(hipStream_t st, const Ctx& c, int prec);
 // FFT convolutions (sdft_fft.hip)

@@ -22,7 +22,7 @@ def omega_band(sp, o, r, om_ref, num):
     band = 2e-10 / fp
     x1 = r.x.copy()
     x1[-1] += 2.0 * band
-    o1 = OracleDFT(sp)
+    o1 = type(o)(sp)
     r1 = anderson_solve(o1.objective, x1, num, max_iters=20000)
     dpsi = float(r1.x[-1] - r.x[-1])
     if abs(dpsi) < 1e-15:
