@@ -419,6 +419,77 @@ def c5_fft_kernels(which):
     b.close()
 
 
+def report_nd(args):
+    """Three-dimensional pore: weak-acid 20-mers with solvent and 10 mM salt (the pH-sweep chemistry)
+    in a square pore of 4 x 4 sigma (hard walls along x and y), 2 sigma along the open axis,
+    h = sigma/8 (33 x 33 x 17 = 18513 points), 16 pH values 3..9 as one batch: mixed and plain FP64
+    batch solves on the spectral multi-dimensional path, errors mixed vs FP64, the oracle's map at
+    the FP64 solution of one point, and the CPU oracle's objective time (estimate: x iterations)."""
+    import copy
+    import numpy as np
+    from softdft_amd import abi, configs
+    from softdft_amd.geometry_nd import build_state_points_nd
+    from oracle.dft_oracle_nd import OracleDFTND
+    pHs = np.linspace(3.0, 9.0, 16)
+    dss = []
+    for pH in pHs:
+        d = configs.weak_polyelectrolyte(H=4.0, dz=0.125, n_seg=20, rho_poly=0.0954930, solvent=0.2864789, pH=float(pH))
+        d["geometry"]["dimensions"] = [4.0, 4.0, 2.0]
+        d["geometry"]["bin_width"] = [0.125, 0.125, 0.125]
+        d["geometry"]["external_field"] = {"hard_walls": {"dims": [1, 2]}}
+        dss.append(d)
+    t0 = time.perf_counter()
+    sps = build_state_points_nd(dss, device_bulk=True)
+    setup = time.perf_counter() - t0
+    sp0 = sps[0]
+    log(f"ND: {len(sps)} points on a {sp0.grid} grid ({sp0.NP} points), setup {setup:.2f} s")
+    b = abi.Batch(sps)
+    opts = _solve_opts(sp0.numerics, max_iters=6000)
+    xm, itm, em, tm = _timed_solve(b, b.x0(), abi.PREC_MIXED, opts)
+    omm, alm, _ = b.observables()
+    rhom = [b.get_field(abi.FIELD_RHO_BEADS, k) for k in range(len(sps))]
+    x64, it64, e64, t64 = _timed_solve(b, b.x0(), abi.PREC_FP64, opts)
+    om64, al64, _ = b.observables()
+    rho64 = [b.get_field(abi.FIELD_RHO_BEADS, k) for k in range(len(sps))]
+    # time of one batched FP64 objective evaluation (all points)
+    b.objective(x64, abi.PREC_FP64)
+    t1 = time.perf_counter()
+    for _ in range(3):
+        b.objective(x64, abi.PREC_FP64)
+    tobj = (time.perf_counter() - t1) / 3
+    b.close()
+    log(f"ND: mixed {tm:.2f} s (mean {itm.mean():.0f} it, max {itm.max()}), FP64 {t64:.2f} s (mean {it64.mean():.0f} it), "
+        f"FP64 objective {tobj * 1e3:.1f} ms per batch evaluation")
+    k = len(sps) // 2
+    x_k = x64[int(sum(sp.n for sp in sps[:k])):int(sum(sp.n for sp in sps[:k + 1]))]
+    o = OracleDFTND(sps[k])
+    t2 = time.perf_counter()
+    fo = o.objective(x_k)
+    tor = time.perf_counter() - t2
+    om_or = o.grand_potential(x_k)
+    out = {
+        "report": "config", "config": "ND3 pore", "grid": list(sp0.grid), "NP": sp0.NP, "points": len(sps),
+        "mixed_gpu": {"value": len(sps) / tm, "unit": "state points/s", "mean_iterations": float(itm.mean()),
+                      "max_iterations": int(itm.max()), "max_residual": float(em.max())},
+        "fp64_gpu": {"value": len(sps) / t64, "unit": "state points/s", "mean_iterations": float(it64.mean()),
+                     "max_residual": float(e64.max())},
+        "fp64_objective_ms_per_batch": tobj * 1e3,
+        "fp64_objective_ms_per_point": tobj * 1e3 / len(sps),
+        "cpu_baseline": {"value": 1.0 / (tor * float(it64[k])), "unit": "state points/s", "cores": 1, "kind": "port",
+                         "sample": f"one oracle objective evaluation ({tor:.2f} s, fftn on the mirrored grid) x the "
+                                   f"{int(it64[k])} iterations of the point's FP64 solve (estimate)"},
+        "errors": {
+            "omega_max_rel_mixed_vs_fp64": float(np.max(np.abs(omm - om64) / np.abs(om64))),
+            "density_max_rel_mixed_vs_fp64": float(max(np.max(np.abs(a - c)) / np.max(np.abs(c)) for a, c in zip(rhom, rho64))),
+            "ionization_max_abs_mixed_vs_fp64": float(np.max(np.abs(alm[:, 0, 1] - al64[:, 0, 1]))),
+            "oracle_residual_at_gpu_fp64_solution": float(np.max(np.abs(fo - x_k))),
+            "omega_fp64_rel_vs_oracle_at_same_x": float(abs(om64[k] - om_or) / abs(om_or)),
+            "density_max_rel_fp64_vs_oracle_at_same_x": float(np.max(np.abs(rho64[k] - o.rho_beads)) / np.max(np.abs(o.rho_beads))),
+        },
+    }
+    print(json.dumps(_finite(out)), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -436,7 +507,7 @@ def main():
                     help="after one full-load evaluation, only re-launch this kernel class 21 times "
                          "(for rocprofv3 --kernel-trace / --pmc runs of that kernel); c5fft_*: the FFT "
                          "convolution stages of the 128-point 500-mer brush batch (NP 16384)")
-    ap.add_argument("--config", choices=["c1", "c2", "c4", "c5"],
+    ap.add_argument("--config", choices=["c1", "c2", "c4", "c5", "nd3"],
                     help="per-config report (one JSON line) instead of the benchmark line")
     ap.add_argument("--c5-iters", type=int, default=1500, help="iteration cap of the C5 batch solve")
     ap.add_argument("--solve-only", action="store_true",
@@ -461,6 +532,8 @@ def main():
             report_single("C2", configs.config2(), cpu_full=False, args=args)
         elif args.config == "c4":
             report_c4(args)
+        elif args.config == "nd3":
+            report_nd(args)
         else:
             report_c5(args)
         return
