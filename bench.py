@@ -419,6 +419,35 @@ def c5_fft_kernels(which):
     b.close()
 
 
+def nd3_datasets():
+    """The 3-D pore workload of report_nd: 16 pH values, 33 x 33 x 17 grid."""
+    import numpy as np
+    from softdft_amd import configs
+    out = []
+    for pH in np.linspace(3.0, 9.0, 16):
+        d = configs.weak_polyelectrolyte(H=4.0, dz=0.125, n_seg=20, rho_poly=0.0954930, solvent=0.2864789, pH=float(pH))
+        d["geometry"]["dimensions"] = [4.0, 4.0, 2.0]
+        d["geometry"]["bin_width"] = [0.125, 0.125, 0.125]
+        d["geometry"]["external_field"] = {"hard_walls": {"dims": [1, 2]}}
+        out.append(d)
+    return out
+
+
+def nd3_kernels(which):
+    """--kernel-only nd3_64 / nd3_32: 21 objective evaluations of the 3-D pore batch at the bulk guess
+    (for rocprofv3 runs of the multi-dimensional kernels)."""
+    from softdft_amd import abi
+    from softdft_amd.geometry_nd import build_state_points_nd
+    b = abi.Batch(build_state_points_nd(nd3_datasets(), device_bulk=True))
+    prec = abi.PREC_FP64 if which.endswith("64") else abi.PREC_FP32
+    x0 = b.x0()
+    t0 = time.perf_counter()
+    for _ in range(21):
+        b.objective(x0, prec)
+    log(f"{which}: 21 batch objective evaluations in {time.perf_counter() - t0:.3f} s")
+    b.close()
+
+
 def report_nd(args):
     """Three-dimensional pore: weak-acid 20-mers with solvent and 10 mM salt (the pH-sweep chemistry)
     in a square pore of 4 x 4 sigma (hard walls along x and y), 2 sigma along the open axis,
@@ -430,14 +459,7 @@ def report_nd(args):
     from softdft_amd import abi, configs
     from softdft_amd.geometry_nd import build_state_points_nd
     from oracle.dft_oracle_nd import OracleDFTND
-    pHs = np.linspace(3.0, 9.0, 16)
-    dss = []
-    for pH in pHs:
-        d = configs.weak_polyelectrolyte(H=4.0, dz=0.125, n_seg=20, rho_poly=0.0954930, solvent=0.2864789, pH=float(pH))
-        d["geometry"]["dimensions"] = [4.0, 4.0, 2.0]
-        d["geometry"]["bin_width"] = [0.125, 0.125, 0.125]
-        d["geometry"]["external_field"] = {"hard_walls": {"dims": [1, 2]}}
-        dss.append(d)
+    dss = nd3_datasets()
     t0 = time.perf_counter()
     sps = build_state_points_nd(dss, device_bulk=True)
     setup = time.perf_counter() - t0
@@ -503,7 +525,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--kernel-only", choices=["chain32", "chain64", "fwd32", "fwd64", "back32", "back64", "acc32", "acc64",
-                                              "pw64", "c5fft_fwd32", "c5fft_fwd64", "c5fft_back32", "c5fft_back64"],
+                                              "pw64", "c5fft_fwd32", "c5fft_fwd64", "c5fft_back32", "c5fft_back64",
+                                              "nd3_64", "nd3_32"],
                     help="after one full-load evaluation, only re-launch this kernel class 21 times "
                          "(for rocprofv3 --kernel-trace / --pmc runs of that kernel); c5fft_*: the FFT "
                          "convolution stages of the 128-point 500-mer brush batch (NP 16384)")
@@ -521,6 +544,9 @@ def main():
                                    "communication, but only the single-GPU figure is measured"}))
         sys.exit(2)
 
+    if args.kernel_only and args.kernel_only.startswith("nd3_"):
+        nd3_kernels(args.kernel_only)
+        return
     if args.kernel_only and args.kernel_only.startswith("c5fft_"):
         c5_fft_kernels(args.kernel_only)
         return

@@ -142,7 +142,7 @@ typedef struct sdft_desc {
      * process_cartesian.jl:14-45): ndim 0 or 1 is the one-dimensional path above. With ndim 2 or 3
      * every point has grid[p][d] points and bin width bins[p][d] per dimension (first dimension
      * fastest in every grid array), the same for all points of the context; NP[p] is their product
-     * (2 <= grid <= 256 per dimension) and dz[p] their volume element prod(bins). Grid arrays (Ext,
+     * (2 <= grid <= 128 per dimension) and dz[p] their volume element prod(bins). Grid arrays (Ext,
      * trapez, surf) are flattened over the grid. Not supported there: grafted segments, the square-well
      * functional, and the contact-value tables (hard_walls must be 0: the host forms the per-dimension
      * contact values from the bead densities). grid, bins: [P][3] (entries past ndim ignored). */

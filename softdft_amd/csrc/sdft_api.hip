@@ -858,8 +858,8 @@ static int validate_desc(const sdft_desc_t* d) {
             for (int k = 0; k < d->ndim; ++k) {
                 const int n = d->grid[(size_t)p * 3 + k];
                 const double h = d->bins[(size_t)p * 3 + k];
-                if (n < 2 || n > 256 || !(h > 0.0)) {
-                    set_error(nullptr, SDFT_E_ARG, "point %d dimension %d: %d points of width %g (2..256 points)", p, k, n, h);
+                if (n < 2 || n > 128 || !(h > 0.0)) {
+                    set_error(nullptr, SDFT_E_ARG, "point %d dimension %d: %d points of width %g (2..128 points)", p, k, n, h);
                     return SDFT_E_ARG;
                 }
                 if (n != d->grid[k] || h != d->bins[k]) {

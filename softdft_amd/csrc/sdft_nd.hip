@@ -44,12 +44,19 @@ __device__ __forceinline__ double nd_warp_max(double v) {   // NaN-propagating
 }
 }  // namespace
 
-size_t nd_axis_lds_bytes(int n, size_t elem) { return (size_t)n * (ND_LINES + 1) * elem; }
+static constexpr int ND_R = 4;        // outputs per thread and line (register blocking)
+
+// LDS of one k_nd_axis workgroup: the input tile [n][ND_LINES + 1] and, for contiguous lines, the
+// output tile of the same shape
+size_t nd_axis_lds_bytes(int n, size_t elem) { return (size_t)2 * n * (ND_LINES + 1) * elem; }
 
 // out[o] = sum_i A[o][i] in[i] along one axis of every field of every point evaluated at `prec`.
 // The grid of one field is [outer][n][inner] (inner = product of the faster dimensions); a line is
 // one (outer, inner) pair. AT holds A transposed: AT[i * n + o]. Fields whose parity entry names
-// this axis use AT_odd, the others AT_even.
+// this axis use AT_odd, the others AT_even. A workgroup stages 64 lines in LDS; each lane owns one
+// line and ND_R consecutive outputs of it, so one LDS read of the input feeds ND_R FMAs against
+// matrix entries that are wave-uniform (scalar loads). Contiguous lines (inner == 1) go out through
+// an LDS tile so that the stores run along the line.
 template <typename T>
 __global__ void __launch_bounds__(256) k_nd_axis(const T* __restrict__ in, T* __restrict__ out, int F, int64_t NPt,
                                                  int n, int inner, int nlines, const T* __restrict__ AT_even,
@@ -62,7 +69,8 @@ __global__ void __launch_bounds__(256) k_nd_axis(const T* __restrict__ in, T* __
     const T* src = in + (int64_t)b * NPt;
     T* dst = out + (int64_t)b * NPt;
     extern __shared__ char nd_lds[];
-    T* tile = reinterpret_cast<T*>(nd_lds);   // [n][ND_LINES + 1]
+    T* tile = reinterpret_cast<T*>(nd_lds);           // [n][ND_LINES + 1]
+    T* otile = tile + n * (ND_LINES + 1);             // [n][ND_LINES + 1] (contiguous lines only)
     const int line0 = blockIdx.x * ND_LINES;
     const int total = n * ND_LINES;
     for (int e = threadIdx.x; e < total; e += blockDim.x) {
@@ -78,16 +86,47 @@ __global__ void __launch_bounds__(256) k_nd_axis(const T* __restrict__ in, T* __
         tile[j * (ND_LINES + 1) + l] = v;
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        int m, l;
-        if (inner == 1) { m = e % n; l = e / n; }
-        else { l = e % ND_LINES; m = e / ND_LINES; }
-        const int line = line0 + l;
-        if (line >= nlines) continue;
-        T acc = T(0);
-        for (int j = 0; j < n; ++j) acc += AT[(int64_t)j * n + m] * tile[j * (ND_LINES + 1) + l];
-        const int o = line / inner, i = line - o * inner;
-        dst[((int64_t)o * n + m) * inner + i] = acc;
+    const int l = threadIdx.x % ND_LINES;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / ND_LINES);
+    const int nw = (int)(blockDim.x / ND_LINES);
+    const int line = line0 + l;
+    const int o = line / inner, i = line - o * inner;
+    for (int m0 = w * ND_R; m0 < n; m0 += nw * ND_R) {
+        T acc[ND_R];
+#pragma unroll
+        for (int r = 0; r < ND_R; ++r) acc[r] = T(0);
+        if (m0 + ND_R <= n) {
+            for (int j = 0; j < n; ++j) {
+                const T a = tile[j * (ND_LINES + 1) + l];
+                const T* arow = AT + (int64_t)j * n + m0;
+#pragma unroll
+                for (int r = 0; r < ND_R; ++r) acc[r] += arow[r] * a;
+            }
+        } else {   // last block of outputs: only the entries below n exist
+            for (int j = 0; j < n; ++j) {
+                const T a = tile[j * (ND_LINES + 1) + l];
+                const T* arow = AT + (int64_t)j * n + m0;
+#pragma unroll
+                for (int r = 0; r < ND_R; ++r)
+                    if (m0 + r < n) acc[r] += arow[r] * a;
+            }
+        }
+        if (inner == 1) {
+#pragma unroll
+            for (int r = 0; r < ND_R; ++r)
+                if (m0 + r < n) otile[(m0 + r) * (ND_LINES + 1) + l] = acc[r];
+        } else if (line < nlines) {
+#pragma unroll
+            for (int r = 0; r < ND_R; ++r)
+                if (m0 + r < n) dst[((int64_t)o * n + m0 + r) * inner + i] = acc[r];
+        }
+    }
+    if (inner == 1) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < total; e += blockDim.x) {
+            const int m = e % n, ll = e / n;
+            if (line0 + ll < nlines) dst[(int64_t)(line0 + ll) * n + m] = otile[m * (ND_LINES + 1) + ll];
+        }
     }
 }
 
