@@ -143,7 +143,7 @@ typedef struct sdft_desc {
      * every point has grid[p][d] points and bin width bins[p][d] per dimension (first dimension
      * fastest in every grid array), the same for all points of the context; NP[p] is their product
      * (2 <= grid <= 128 per dimension) and dz[p] their volume element prod(bins). Grid arrays (Ext,
-     * trapez, surf) are flattened over the grid. Not supported there: grafted segments, the square-well
+     * trapez, surf) are flattened over the grid. Not supported there: grafted segments on 2-D grids, the square-well
      * functional, and the contact-value tables (hard_walls must be 0: the host forms the per-dimension
      * contact values from the bead densities). grid, bins: [P][3] (entries past ndim ignored). */
     int32_t ndim;

@@ -367,6 +367,14 @@ class OracleDFT:
         return np.abs(np.convolve(ext, self.bond_taps[bond], mode="valid"))
 
     # -------------------------------------------------------- densities
+    def _dist2(self, K1, K):
+        """compute_distance_squared (child_propagators.jl:165-182) between grid indices."""
+        return ((K1 - K) * self.dz) ** 2
+
+    def _renorm(self, Dij: float) -> float:
+        """get_renorm (child_propagators.jl:218-226): 2 D_ij on a one-dimensional grid."""
+        return 2.0 * Dij
+
     def _species_density(self, u, lam, lng, rho_seg_u, rho_bonds):
         ms, sp = self.ms, self.ms.species[u]
         NP, dz = self.NP, self.dz
@@ -411,8 +419,7 @@ class OracleDFT:
         def coord(seg):
             return int(fx.coords[seg])
 
-        def dist2(K1, K):
-            return ((K1 - K) * dz) ** 2
+        dist2 = self._dist2
 
         # ---- child propagators
         gC = np.zeros((NP, S, C, S, N))
@@ -427,7 +434,7 @@ class OracleDFT:
                         for b, sj in enumerate(fam[c]):
                             bond, pi, pj = ms.bond_index(si, sj)
                             Dij = (sig[si] + sig[sj]) / 2.0
-                            renorm = 2.0 * Dij
+                            renorm = self._renorm(Dij)
                             Dij2 = Dij ** 2
                             f = np.exp(-lam[:, sj] + lng[:, pj, bond] / 2.0)
                             for gci, _ in enumerate(ch[c]):
@@ -477,7 +484,7 @@ class OracleDFT:
                             bond, pi, pj = ms.bond_index(si, sj)
                             Dij = (sig[si] + sig[sj]) / 2.0
                             Dij2 = Dij ** 2
-                            renorm = 2.0 * Dij
+                            renorm = self._renorm(Dij)
                             f = np.exp(-lam[:, sj] + lng[:, pj, bond] / 2.0) * gP[:, :, b, p].sum(1)
                             for oci, oc in enumerate(ch[p]):
                                 if oc != c:

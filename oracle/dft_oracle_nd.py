@@ -21,8 +21,10 @@ Sums use the volume element prod h (grand_potential.jl:23, num_species.jl:13, ne
 
 The one-dimensional oracle (dft_oracle.OracleDFT) supplies everything else unchanged: the chain
 recursion, the MSA functions, the neutrality Newton and the Anderson solver. Grid arrays are flat
-over the primary domain (first dimension fastest), as in Julia. Grafted chains are outside this
-module, as they are outside the library's multi-dimensional path.
+over the primary domain (first dimension fastest), as in Julia. Grafted chains use the
+reference's D-dimensional distances (compute_distance_squared, child_propagators.jl:165-182), the
+3-D bond renormalisation 4 pi D_ij^2 (get_renorm, :218-226) and the volume element in the direct
+sums (child_propagators.jl:74-117, handle_fixed.jl:72).
 
 Pins (tests/test_oracle_nd_pins.py):
 * a system uniform in all but one dimension reduces to the one-dimensional oracle;
@@ -78,8 +80,6 @@ class OracleDFTND(OracleDFT):
     def __init__(self, sp):
         if sp.grid is None:
             raise ValueError("OracleDFTND needs a multi-dimensional state point")
-        if any(f.any for f in sp.fixed):
-            raise NotImplementedError("grafted chains on multi-dimensional grids")
         self.sp = sp
         ms = sp.ms
         self.ms = ms
@@ -173,6 +173,20 @@ class OracleDFTND(OracleDFT):
         sh = a.shape[1:]
         A = a.reshape(self.fg + sh, order="F")
         return np.fft.ifftn(A, axes=tuple(range(self.D))).reshape((self.L,) + sh, order="F")
+
+    def _dist2(self, K1, K):
+        """compute_distance_squared (child_propagators.jl:165-182) between flat grid indices."""
+        a = np.array(np.unravel_index(np.asarray(K1), self.grid, order="F"), dtype=float)
+        b = np.array(np.unravel_index(np.asarray(K), self.grid, order="F"), dtype=float)
+        h = np.array(self.bins).reshape((-1,) + (1,) * (a.ndim - 1))
+        return (((a - b.reshape(b.shape + (1,) * (a.ndim - b.ndim))) * h) ** 2).sum(0)
+
+    def _renorm(self, Dij: float) -> float:
+        """get_renorm (child_propagators.jl:218-226): 4 pi D_ij^2 on a three-dimensional grid; the
+        reference has no two-dimensional form."""
+        if self.D != 3:
+            raise NotImplementedError("Not added yet.")
+        return 4.0 * PI * Dij * Dij
 
     def _prop_conv(self, f: np.ndarray, bond: int) -> np.ndarray:
         g = self.Fi(self.F(self.mirror(f)) * self.wbond_filt[:, bond])
@@ -323,16 +337,20 @@ class OracleDFTND(OracleDFT):
         Num = np.zeros(nsp)
         qs = np.zeros(nsp)
         for u, s in enumerate(ms.species):
+            fx = self.sp.fixed[u]
             for seg in range(s.n_seg):
                 for a, si in enumerate(s.state_family[seg]):
-                    num = float(np.sum(self.rho_seg[u][:, a, seg] * self.sp.trapez[:, si] * self.dz))
+                    if fx.segments[seg]:
+                        num = self.rho_seg[u][int(fx.coords[0]), a, seg]
+                    else:
+                        num = float(np.sum(self.rho_seg[u][:, a, seg] * self.sp.trapez[:, si] * self.dz))
                     Num[u] += num
                     qs[u] += num * ms.valences[si]
             Num[u] /= s.n_seg
             qs[u] = 0.0 if Num[u] <= 0.0 else qs[u] / Num[u]
         for u, s in enumerate(ms.species):
             multi = any(len(f) > 1 for f in s.state_family)
-            if multi or s.evaluation == "simulation":
+            if multi or self.sp.fixed[u].any or s.evaluation == "simulation":
                 Qtot += Num[u] * qs[u]
                 qs[u] = 0.0
         self.neutrality_inputs = (Qtot, Num.copy(), qs.copy())
@@ -361,9 +379,15 @@ class OracleDFTND(OracleDFT):
             rho_seg.append(rs)
         beads = np.zeros((NP, NB))
         for u, s in enumerate(ms.species):
+            fx = sp.fixed[u]
             for seg in range(s.n_seg):
                 for a_, si in enumerate(s.state_family[seg]):
-                    beads[:, si] += rho_seg[u][:, a_, seg]
+                    if fx.segments[seg]:
+                        # update_spatial_structures.jl:23-25: rho / prod(h) / trapez at the graft point
+                        K = int(fx.coords[0])
+                        beads[K, si] += rho_seg[u][K, a_, seg] / self.dz / sp.trapez[K, si]
+                    else:
+                        beads[:, si] += rho_seg[u][:, a_, seg]
         self.rho_seg, self.rho_bonds, self.rho_beads = rho_seg, rho_bonds, beads
         self.rho_beads_hat = self.F(self.mirror(beads * sp.trapez))
         rbh = np.zeros((self.L, 2, MB), complex)

@@ -881,8 +881,9 @@ static int validate_desc(const sdft_desc_t* d) {
             return SDFT_E_UNSUPPORTED;
         }
         for (int u = 0; u < nsp; ++u)
-            if (d->species_fixed_seg && d->species_fixed_seg[u] >= 0) {
-                set_error(nullptr, SDFT_E_UNSUPPORTED, "multi-dimensional grids: species %d has a grafted segment", u);
+            if (d->ndim != 3 && d->species_fixed_seg && d->species_fixed_seg[u] >= 0) {
+                set_error(nullptr, SDFT_E_UNSUPPORTED, "two-dimensional grids: species %d has a grafted segment "
+                                                       "(the reference places grafted chains in 3-D only)", u);
                 return SDFT_E_UNSUPPORTED;
             }
     }
@@ -1205,6 +1206,16 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         gsum += q.gstride;
         q.n = (int64_t)NP * (NB + 2 * MB + 1) + 1;
         q.dz = d->dz[p];   // multi-dimensional grids: the volume element prod(bins) (validated)
+        q.nd = 1;
+        q.gn[0] = NP;
+        q.gh[0] = d->dz[p];
+        if (d->ndim > 1) {
+            q.nd = d->ndim;
+            for (int k = 0; k < d->ndim; ++k) {
+                q.gn[k] = d->grid[(size_t)p * 3 + k];
+                q.gh[k] = d->bins[(size_t)p * 3 + k];
+            }
+        }
         q.total_charge = d->total_charge ? d->total_charge[p] : 0.0;
         uint64_t bits;
         memcpy(&bits, &d->dz[p], 8);

@@ -487,7 +487,7 @@ __global__ void __launch_bounds__(256) k_bandstep(const DevSystem* __restrict__ 
         if (colv[b] && n >= 0 && n < NP) {
             T q;
             if (!stored) {
-                q = base_value<T>(inseg, n, A.fixed_seg, Kf, P.dz, maxd);
+                q = base_value<T>(inseg, n, A.fixed_seg, Kf, P, maxd);
             } else {
                 q = T(0);
                 for (int b2 = 0; b2 < S; ++b2)

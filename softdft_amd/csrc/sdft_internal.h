@@ -59,6 +59,11 @@ struct DevPoint {
     // and the free-chain normalisation normative^(N-1) (handle_fixed.jl:105-107)
     double normative[SDFT_MAX_SPECIES];
     double chain_norm[SDFT_MAX_SPECIES];
+    // multi-dimensional grids: dimensions, points and bin width per dimension (first fastest); nd <= 1
+    // is the one-dimensional grid of NP points and width dz
+    int32_t nd;
+    int32_t gn[3];
+    double gh[3];
     // grafting density of each grafted species (handle_fixed.jl:31: norm = density / lambda); per point
     // so that a batch may hold several grafting densities (a continuation in the grafting density)
     double graft_density[SDFT_MAX_SPECIES];

@@ -223,7 +223,7 @@ __global__ void __launch_bounds__(TB) k_prop_step(const DevSystem* __restrict__ 
                        : T(1);
     // Q(n') = sum_b' g_in[inseg][b'][b](n')  or base
     auto qin = [&](int b, int nn) -> T {
-        if (!in_stored) return base_value<T>(inseg, nn, A.fixed_seg, Kf, dz, maxd);
+        if (!in_stored) return base_value<T>(inseg, nn, A.fixed_seg, Kf, P, maxd);
         T q = 0;
         for (int b2 = 0; b2 < S; ++b2)
             if (st[inseg * S + b2] >= 0) q += gin[gidx(inseg, b2, b, nn)];
@@ -426,12 +426,12 @@ __global__ void __launch_bounds__(256) k_fixed_lambda(const DevSystem* __restric
             const int si = st[q * S + a];
             if (si < 0) continue;
             double Gp = 0.0;
-            if (q == 0) Gp = (double)base_value<T>(0, n, A.fixed_seg, Kf, P.dz, maxd);
+            if (q == 0) Gp = (double)base_value<T>(0, n, A.fixed_seg, Kf, P, maxd);
             else
                 for (int b = 0; b < S; ++b)
                     if (st[(q - 1) * S + b] >= 0) Gp += (double)gP[gidx(q, b, a, n)];
             double Qc = 0.0;
-            if (q == N - 1) Qc = (a == 0) ? (double)base_value<T>(q, n, A.fixed_seg, Kf, P.dz, maxd) : 0.0;
+            if (q == N - 1) Qc = (a == 0) ? (double)base_value<T>(q, n, A.fixed_seg, Kf, P, maxd) : 0.0;
             else
                 for (int b = 0; b < S; ++b)
                     if (st[(q + 1) * S + b] >= 0) Qc += (double)gC[gidx(q, b, a, n)];
@@ -592,13 +592,13 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                 if (mb) {
                     Gp = Qc = 1.0;
                 } else {
-                    if (q == 0) Gp = (double)base_value<T>(0, nc, sp.fixed_seg, Kf, P.dz, maxd);
+                    if (q == 0) Gp = (double)base_value<T>(0, nc, sp.fixed_seg, Kf, P, maxd);
                     else
 #pragma unroll
                         for (int b = 0; b < SDFT_MAX_S; ++b)
                             if (b < S && st[(q - 1) * S + b] >= 0) Gp += (double)gpv[b][a];
                     // leaf: sum gC[K, :, 1, idx_i, leaf] is the base case 1 for every state
-                    if (q == N - 1) Qc = (double)base_value<T>(q, nc, sp.fixed_seg, Kf, P.dz, maxd);
+                    if (q == N - 1) Qc = (double)base_value<T>(q, nc, sp.fixed_seg, Kf, P, maxd);
                     else
 #pragma unroll
                         for (int b = 0; b < SDFT_MAX_S; ++b)
@@ -1625,11 +1625,11 @@ __global__ void __launch_bounds__(256) k_segment_sums(const DevSystem* __restric
                 double Gp = 0.0, Qc = 0.0;
                 if (sp.model == SDFT_MODEL_MONOMERBEAD) Gp = Qc = 1.0;  // independent beads, full profiles
                 else {
-                    if (q == 0) Gp = (double)base_value<T>(0, n, sp.fixed_seg, Kf, P.dz, maxd);
+                    if (q == 0) Gp = (double)base_value<T>(0, n, sp.fixed_seg, Kf, P, maxd);
                     else
                         for (int b = 0; b < S; ++b)
                             if (st[(q - 1) * S + b] >= 0) Gp += (double)gP[gidx(q, b, a, n)];
-                    if (q == N - 1) Qc = (double)base_value<T>(q, n, sp.fixed_seg, Kf, P.dz, maxd);
+                    if (q == N - 1) Qc = (double)base_value<T>(q, n, sp.fixed_seg, Kf, P, maxd);
                     else
                         for (int b = 0; b < S; ++b)
                             if (st[(q + 1) * S + b] >= 0) Qc += (double)gC[gidx(q, b, a, n)];
@@ -1687,11 +1687,11 @@ __global__ void __launch_bounds__(256) k_segment_profile(const DevSystem* __rest
             double Gp = 0.0, Qc = 0.0;
             if (sp.model == SDFT_MODEL_MONOMERBEAD) Gp = Qc = 1.0;  // independent beads, full profiles
             else {
-                if (q == 0) Gp = (double)base_value<T>(0, n, sp.fixed_seg, Kf, P.dz, maxd);
+                if (q == 0) Gp = (double)base_value<T>(0, n, sp.fixed_seg, Kf, P, maxd);
                 else
                     for (int b = 0; b < S; ++b)
                         if (st[(q - 1) * S + b] >= 0) Gp += (double)gP[gidx(q, b, a)];
-                if (q == N - 1) Qc = (double)base_value<T>(q, n, sp.fixed_seg, Kf, P.dz, maxd);
+                if (q == N - 1) Qc = (double)base_value<T>(q, n, sp.fixed_seg, Kf, P, maxd);
                 else
                     for (int b = 0; b < S; ++b)
                         if (st[(q + 1) * S + b] >= 0) Qc += (double)gC[gidx(q, b, a)];
@@ -1902,6 +1902,17 @@ void Pipeline<T>::propagate(hipStream_t st, const Ctx& c, int prec, int u, bool 
                            c.d_seg_states, c.d_maxd, elam, trapez, gC, gP, c.d_fixed_lambda);
     }
 }
+
+template <typename T>
+void launch_fixed_lambda(hipStream_t st, const Ctx& c, int prec, const PropArgs& A, const T* elam, const T* trapez,
+                         const T* gC, const T* gP) {
+    hipLaunchKernelGGL(k_fixed_lambda<T>, dim3(c.P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag, prec, A,
+                       c.d_seg_states, c.d_maxd, elam, trapez, gC, gP, c.d_fixed_lambda);
+}
+template void launch_fixed_lambda<float>(hipStream_t, const Ctx&, int, const PropArgs&, const float*, const float*,
+                                         const float*, const float*);
+template void launch_fixed_lambda<double>(hipStream_t, const Ctx&, int, const PropArgs&, const double*, const double*,
+                                          const double*, const double*);
 
 void launch_scale_prefix(hipStream_t st, const Ctx& c, int prec, int u, unsigned long long* smax, double* lsc) {
     const SpeciesHost& sh = c.species[u];

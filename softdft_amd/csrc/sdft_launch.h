@@ -250,15 +250,34 @@ struct PropArgs {
     unsigned long long* smax;
 };
 
+// squared distance between grid points n and Kf (compute_distance_squared, child_propagators.jl:165-182):
+// one-dimensional grids (n - Kf) dz; multi-dimensional grids from the flat indices (first dimension
+// fastest) with the bin width of each dimension
+__device__ __forceinline__ double graft_dist2(const DevPoint& P, int n, int Kf) {
+    if (P.nd <= 1) {
+        const double d = (double)(n - Kf) * P.dz;
+        return d * d;
+    }
+    double s = 0.0;
+    int a = n, b = Kf;
+    for (int v = 0; v < P.nd; ++v) {
+        const int ia = a % P.gn[v], ib = b % P.gn[v];
+        a /= P.gn[v];
+        b /= P.gn[v];
+        const double d = (double)(ia - ib) * P.gh[v];
+        s += d * d;
+    }
+    return s;
+}
+
 // propagator input at an end of the recursion: 1 at a free chain end; for a grafted chain 1 within
 // reach (max_distance) of the grafting point and 0 beyond, and at the grafted segment itself only at
 // the grafting point (handle_fixed.jl)
 template <typename T>
-__device__ __forceinline__ T base_value(int seg, int n, int fixed_seg, int Kf, double dz, const double* maxd) {
+__device__ __forceinline__ T base_value(int seg, int n, int fixed_seg, int Kf, const DevPoint& P, const double* maxd) {
     if (fixed_seg < 0) return T(1);
     if (seg == fixed_seg) return n == Kf ? T(1) : T(0);
-    const double d = (double)(n - Kf) * dz;
-    return (d * d > maxd[seg] * maxd[seg]) ? T(0) : T(1);
+    return (graft_dist2(P, n, Kf) > maxd[seg] * maxd[seg]) ? T(0) : T(1);
 }
 
 // stores one convolution column's value y at grid point n of point P (ConvJob semantics)
@@ -319,6 +338,10 @@ int nd_setup(Ctx& c, const DevSystem& sys, const std::vector<ConvJob>& fwd1, con
 hipError_t nd_prepare_attributes(const Ctx& c);
 size_t nd_axis_lds_bytes(int n, size_t elem);
 void launch_scale_prefix(hipStream_t st, const Ctx& c, int prec, int u, unsigned long long* smax, double* lsc);
+// grafted chain normalisation lambda (handle_fixed.jl:41-119) of species A.u into d_fixed_lambda
+template <typename T>
+void launch_fixed_lambda(hipStream_t st, const Ctx& c, int prec, const PropArgs& A, const T* elam, const T* trapez,
+                         const T* gC, const T* gP);
 
 void launch_compact_cols(hipStream_t st, const Ctx& c, int prec);
 // FFT convolutions (sdft_fft.hip)

@@ -192,7 +192,8 @@ __device__ __forceinline__ void nd_pairs(const DevSystem& sys, const int* st, in
 template <typename T>
 __global__ void __launch_bounds__(256) k_nd_prop_in(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
                                                     const int32_t* __restrict__ evalflag, int prec, PropArgs A,
-                                                    const int32_t* __restrict__ seg_states, const T* __restrict__ elam,
+                                                    const int32_t* __restrict__ seg_states,
+                                                    const double* __restrict__ maxd_tab, const T* __restrict__ elam,
                                                     const T* __restrict__ hl, const T* __restrict__ trapez,
                                                     const T* __restrict__ gC, const T* __restrict__ gP, T* __restrict__ W) {
     const DevSystem& sys = *psys;
@@ -221,7 +222,8 @@ __global__ void __launch_bounds__(256) k_nd_prop_in(const DevSystem* __restrict_
         const int sj = st[inseg * S + b];
         T qv;
         if (!in_stored) {
-            qv = T(1);
+            qv = base_value<T>(inseg, n, A.fixed_seg, A.fixed_seg >= 0 ? P.fixed_coord[A.u] : -1, P,
+                               maxd_tab + A.maxd_off);
         } else {
             qv = T(0);
             for (int b2 = 0; b2 < S; ++b2)
@@ -265,7 +267,8 @@ __global__ void __launch_bounds__(256) k_nd_prop_mul(const DevSystem* __restrict
 template <typename T>
 __global__ void __launch_bounds__(256) k_nd_prop_out(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
                                                      const int32_t* __restrict__ evalflag, int prec, PropArgs A,
-                                                     const int32_t* __restrict__ seg_states, const T* __restrict__ hl,
+                                                     const int32_t* __restrict__ seg_states,
+                                                     const double* __restrict__ maxd_tab, const T* __restrict__ hl,
                                                      const T* __restrict__ R, T* __restrict__ gC, T* __restrict__ gP) {
     const DevSystem& sys = *psys;
     const int p = blockIdx.z;
@@ -285,10 +288,17 @@ __global__ void __launch_bounds__(256) k_nd_prop_out(const DevSystem* __restrict
         const int S2 = S * S;
         const T* r = R + ((int64_t)p * 2 * S2 + (int64_t)sweep * S2) * P.NP + n;
         const double normative = P.normative[A.u];
+        // beyond the segment's reach of the grafting point the propagator is 0 (child_propagators.jl:147-157)
+        bool far = false;
+        if (A.fixed_seg >= 0) {
+            const double md = maxd_tab[A.maxd_off + outseg];
+            far = graft_dist2(P, n, P.fixed_coord[A.u]) > md * md;
+        }
         for (int k = 0; k < q.n; ++k) {
             const int a = q.a[k], b = q.b[k];
-            const T v = nd_abs(r[(int64_t)(a * S + b) * P.NP]) * hl[nd_fidx(P, 2 * MB, 2 * q.bond[k] + q.pi[k], n)] /
-                        (T)normative;
+            T v = nd_abs(r[(int64_t)(a * S + b) * P.NP]) * hl[nd_fidx(P, 2 * MB, 2 * q.bond[k] + q.pi[k], n)] /
+                  (T)normative;
+            if (far) v = T(0);
             gout[P.gbase + (A.goff + ((int64_t)outseg * S + b) * S + a) * P.gstride + n] = v;
             vmax = (double)v > vmax ? (double)v : vmax;
         }
@@ -297,6 +307,99 @@ __global__ void __launch_bounds__(256) k_nd_prop_out(const DevSystem* __restrict
         vmax = nd_warp_max(vmax);
         if ((threadIdx.x & 63) == 0) scale_note(A.smax, scale_slot(p, A.u, sweep, A.s, A.nsp, A.nmax), vmax);
     }
+}
+
+// A step whose parent is the grafted segment (handle_fixed.jl / child_propagators.jl:74-117,
+// parent_propagators.jl:80-100): direct sums over the ball |r - r_graft| <= D_ij instead of a
+// convolution, with the 3-D bond renormalisation 4 pi D_ij^2 (get_renorm) and half weight on the ball's
+// surface (|d^2 - D_ij^2| <= 1e-6, isapprox). Child sweep: only the grafting point gets a value,
+//     gC(Kf) = sum_K1 |f(K1)| e^{lng_i(Kf)/2} dV trapez_j(K1) / renorm / normative;
+// parent sweep: gP(n) = |f(Kf)| e^{lng_i(n)/2} / renorm / normative inside the ball, 0 outside.
+template <typename T>
+__global__ void __launch_bounds__(256) k_nd_prop_graft(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
+                                                       const int32_t* __restrict__ evalflag, int prec, PropArgs A, int sweep,
+                                                       const int32_t* __restrict__ seg_states,
+                                                       const double* __restrict__ maxd_tab, const T* __restrict__ elam,
+                                                       const T* __restrict__ hl, const T* __restrict__ trapez,
+                                                       T* __restrict__ gC, T* __restrict__ gP) {
+    const DevSystem& sys = *psys;
+    const int p = blockIdx.z;
+    if (evalflag[p] != prec) return;
+    const DevPoint& P = pts[p];
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= P.NP) return;
+    const int N = A.nseg, S = A.S, NB = sys.NB, MB = sys.MB;
+    const int outseg = sweep == 0 ? N - 1 - A.s : A.s;
+    const int inseg = sweep == 0 ? outseg + 1 : outseg - 1;
+    const int* st = seg_states + A.state_off;
+    const double* maxd = maxd_tab + A.maxd_off;
+    const int Kf = P.fixed_coord[A.u];
+    const T* gin = sweep == 0 ? gC : gP;
+    T* gout = sweep == 0 ? gC : gP;
+    const bool in_stored = sweep == 0 ? (inseg < N - 1) : (inseg > 0);
+    const T qinv = (A.rescale && in_stored)
+                       ? (T)(1.0 / scale_read(A.smax, scale_slot(p, A.u, sweep, A.s - 1, A.nsp, A.nmax)))
+                       : T(1);
+    auto qin = [&](int b, int nn) -> double {
+        if (!in_stored) return (double)base_value<T>(inseg, nn, A.fixed_seg, Kf, P, maxd);
+        T qv = T(0);
+        for (int b2 = 0; b2 < S; ++b2)
+            if (st[inseg * S + b2] >= 0) qv += gin[P.gbase + (A.goff + ((int64_t)inseg * S + b2) * S + b) * P.gstride + nn];
+        return (double)(qv * qinv);
+    };
+    NdPairs q;
+    nd_pairs(sys, st, S, outseg, inseg, q);
+    const double normative = P.normative[A.u];
+    const double md = maxd[outseg];
+    const bool far = graft_dist2(P, n, Kf) > md * md;
+    double vmax = 0.0;
+    for (int k = 0; k < q.n; ++k) {
+        const int a = q.a[k], b = q.b[k], bond = q.bond[k];
+        const int si = st[outseg * S + a], sj = st[inseg * S + b];
+        const double Dij = (sys.sigma[si] + sys.sigma[sj]) / 2.0;
+        const double Dij2 = Dij * Dij, renorm = 4.0 * M_PI * Dij2;
+        double val = 0.0;
+        if (sweep == 0) {
+            if (n == Kf) {
+                // the ball's bounding box on the grid
+                int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}, c[3] = {0, 0, 0};
+                int r = Kf;
+                for (int v = 0; v < P.nd; ++v) {
+                    c[v] = r % P.gn[v];
+                    r /= P.gn[v];
+                    const int w = (int)ceil(Dij / P.gh[v]) + 1;
+                    lo[v] = max(0, c[v] - w);
+                    hi[v] = min(P.gn[v] - 1, c[v] + w);
+                }
+                const double hlo = (double)hl[nd_fidx(P, 2 * MB, 2 * bond + q.pi[k], Kf)];
+                for (int i2 = lo[2]; i2 <= hi[2]; ++i2)
+                    for (int i1 = lo[1]; i1 <= hi[1]; ++i1)
+                        for (int i0 = lo[0]; i0 <= hi[0]; ++i0) {
+                            const int K1 = i0 + P.gn[0] * (i1 + P.gn[1] * i2);
+                            const double d2 = graft_dist2(P, K1, Kf);
+                            if (d2 > Dij2) continue;
+                            const double f = (double)elam[nd_fidx(P, NB, sj, K1)] *
+                                             (double)hl[nd_fidx(P, 2 * MB, 2 * bond + q.pj[k], K1)] * qin(b, K1);
+                            double t = fabs(f) * hlo * P.dz * (double)trapez[nd_fidx(P, NB, sj, K1)] / renorm;
+                            if (fabs(d2 - Dij2) <= 1e-6) t /= 2.0;
+                            val += t / normative;
+                        }
+            }
+        } else {
+            const double d2 = graft_dist2(P, n, Kf);
+            if (d2 <= Dij2) {
+                const double f = (double)elam[nd_fidx(P, NB, sj, Kf)] *
+                                 (double)hl[nd_fidx(P, 2 * MB, 2 * bond + q.pj[k], Kf)] * qin(b, Kf);
+                val = fabs(f) * (double)hl[nd_fidx(P, 2 * MB, 2 * bond + q.pi[k], n)] / renorm / normative;
+                if (fabs(d2 - Dij2) <= 1e-6) val /= 2.0;
+            }
+        }
+        if (far) val = 0.0;
+        gout[P.gbase + (A.goff + ((int64_t)outseg * S + b) * S + a) * P.gstride + n] = (T)val;
+        vmax = (double)(T)val > vmax ? (double)(T)val : vmax;
+    }
+    // per-lane atomic maximum (no wave-wide reduction: lanes past the grid returned above)
+    if (A.rescale) scale_note(A.smax, scale_slot(p, A.u, sweep, A.s, A.nsp, A.nmax), vmax);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -353,16 +456,27 @@ void Pipeline<T>::propagate_nd(hipStream_t st, const Ctx& c, int prec, int u) {
     const unsigned gx = (unsigned)((g.NPt + 255) / 256);
     for (int s = 1; s < sh.nseg; ++s) {
         A.s = s;
+        // sweeps whose parent is the grafted segment 0 take the direct sums; they are computed after
+        // the spectral path, which also writes (and then has overwritten) their outputs
+        const bool fix0 = sh.fixed_seg == 0 && s == sh.nseg - 1;   // child sweep, parent = segment 0
+        const bool fix1 = sh.fixed_seg == 0 && s == 1;             // parent sweep, parent = segment 0
         hipLaunchKernelGGL(k_nd_prop_in<T>, dim3(gx, 2, c.P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag, prec, A,
-                           c.d_seg_states, elam, hl, trapez, gC, gP, nd_a);
+                           c.d_seg_states, c.d_maxd, elam, hl, trapez, gC, gP, nd_a);
         T* S = nd_transform<T>(st, c, prec, nd_a, 2 * S2, g.d_par_even, false, nd_a, nd_b);
         hipLaunchKernelGGL(k_nd_prop_mul<T>, dim3(gx, 2 * S2, c.P), dim3(256), 0, st, c.d_sys, c.d_evalflag, prec, A,
                            c.d_seg_states, kspec, g.kbond0, g.NPt, S);
         T* R = nd_transform<T>(st, c, prec, S, 2 * S2, g.d_par_even, true, nd_a, nd_b);
         hipLaunchKernelGGL(k_nd_prop_out<T>, dim3(gx, 2, c.P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag, prec,
-                           A, c.d_seg_states, hl, R, gC, gP);
+                           A, c.d_seg_states, c.d_maxd, hl, R, gC, gP);
+        if (fix0)
+            hipLaunchKernelGGL(k_nd_prop_graft<T>, dim3(gx, 1, c.P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag,
+                               prec, A, 0, c.d_seg_states, c.d_maxd, elam, hl, trapez, gC, gP);
+        if (fix1)
+            hipLaunchKernelGGL(k_nd_prop_graft<T>, dim3(gx, 1, c.P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag,
+                               prec, A, 1, c.d_seg_states, c.d_maxd, elam, hl, trapez, gC, gP);
     }
     if (sh.rescale) launch_scale_prefix(st, c, prec, u, smax, lsc);
+    if (sh.fixed_seg >= 0) launch_fixed_lambda<T>(st, c, prec, A, elam, trapez, gC, gP);
 }
 
 template <typename T>

@@ -22,9 +22,9 @@ returns one (left, right) pair per dimension. The reference reads the hard-wall 
 dimension v at the grid point (NP_1, ..., NP_{D-1}, contact index along v), whatever v is, and that
 is kept here.
 
-Grafted (fixed) segments in more than one dimension are rejected:
-* the reference places at most one grafted chain per wall in 3-D (determine_fixed.jl:31-86);
-* this build does not propagate grafted chains on multi-dimensional grids.
+Grafted (fixed) segments are placed as the reference's 3-D method does (determine_fixed.jl:31-86): one
+chain per wall, grafted at the centre of the wall, with the grafting density reset to 1 / (L_x L_y).
+Two-dimensional grids have no such method and are rejected.
 Chains of more than one segment need three dimensions: the reference's bond renormalisation has no
 two-dimensional form (get_renorm, child_propagators.jl:218-226).
 """
@@ -69,8 +69,8 @@ def build_state_point_nd(dataset: dict, label: str = "", bulk: Optional[BulkStat
     NPt = int(np.prod(grid))
     NB = ms.NB
     for u, s in enumerate(ms.species):
-        if s.fixed:
-            raise SetupError(f"species {u}: grafted segments are not supported on {D}-D grids")
+        if s.fixed and D != 3:
+            raise SetupError(f"species {u}: grafted segments need a 3-D grid (determine_fixed has no 2-D method)")
         if D == 2 and s.n_seg > 1 and s.model != "monomerbead":
             raise SetupError("Not added yet.")   # get_renorm has no 2-D form (child_propagators.jl:218-226)
     idx = grid_index_tuples(grid) + 1        # 1-based, as in the reference
@@ -124,8 +124,13 @@ def build_state_point_nd(dataset: dict, label: str = "", bulk: Optional[BulkStat
             Ext[out_hi, j] += 1.0e20
             trapez[out_hi, j] = 1.0
             trapez[on_hi, j] = 0.5
-    fixed = [FixedSpec(segments=np.zeros(s.n_seg, dtype=bool), coords=np.zeros(s.n_seg, dtype=np.int64))
-             for s in ms.species]
+    fixed = determine_fixed_xyz(ms, grid, bins, dims) if any(s.fixed for s in ms.species) else \
+        [FixedSpec(segments=np.zeros(s.n_seg, dtype=bool), coords=np.zeros(s.n_seg, dtype=np.int64))
+         for s in ms.species]
+    mu_sp = bulk.mu_species.copy()
+    for u, fs in enumerate(fixed):
+        if fs.any:
+            mu_sp[u] = 0.0   # handle_fixed.jl:43
     funcs = [m for m in ms.fe_model if m in FUNCTIONAL_KEYS]
     if "sw" in funcs:
         raise SetupError("the square-well functional is outside this build's multi-dimensional scope")
@@ -133,12 +138,58 @@ def build_state_point_nd(dataset: dict, label: str = "", bulk: Optional[BulkStat
     sp = StatePoint(ms=ms, bulk=bulk, H=float(dims[-1]), dz=float(np.prod(bins)), NP=NPt, Ext=Ext,
                     trapez=trapez, surf=surf, total_charge=total_charge, hard_walls=False,
                     wall_lo=np.zeros(NB, dtype=np.int64), wall_hi=np.zeros(NB, dtype=np.int64),
-                    wall_charges=None, fixed=fixed, mu_species=bulk.mu_species.copy(), functionals=funcs,
+                    wall_charges=None, fixed=fixed, mu_species=mu_sp, functionals=funcs,
                     numerics=num, label=label, grid=grid, bins=tuple(bins))
     sp.nd_hard_dims = hard_dims
     sp.nd_wall_q = wall_q
     sp.nd_dims = tuple(dims)
     return sp
+
+
+def determine_fixed_xyz(ms: MolSys, grid, bins, dims) -> List[FixedSpec]:
+    """determine_fixed for CartesianXYZ (determine_fixed.jl:31-86): one grafted segment per chain, at
+    (round(NP_x / 2) + 1, round(NP_y / 2) + 1) in x and y and round(d / h_z / 2) grid steps from the
+    wall along z (1-based), the grafting density set to 1 / (L_x L_y); x and y must have the same
+    length and each wall carries at most one brush. Coordinates are returned as flat 0-based indices."""
+    out = []
+    left = right = 0
+    for u, s in enumerate(ms.species):
+        fs = FixedSpec(segments=np.zeros(s.n_seg, dtype=bool), coords=np.zeros(s.n_seg, dtype=np.int64))
+        fd = s.fixed
+        if fd:
+            if s.input_density > 0.0:
+                raise SetupError(f"Species {u} defines a fixed segment but also has nonzero input density")
+            if "density" not in fd:
+                raise SetupError(f"Species {u} contains 'fixed' but does not specify a density.")
+            segs, poss = fd["segment"], fd["position"]
+            if len(segs) != len(poss):
+                raise SetupError("Mismatch in number of fixed segments and positions")
+            if len(segs) > 1:
+                raise SetupError(f"Only one fixed segment per chain is allowed in 3D (species {u}).")
+            if dims[0] != dims[1]:
+                raise SetupError("For 3D brushes, wall dimensions in x and y must match.")
+            fs.density = 1.0 / (dims[0] * dims[1])
+            idx_lw = julia_round(0.0 / bins[-1]) + 1
+            idx_rw = julia_round(dims[-1] / bins[-1]) + 1
+            for L, pos in zip(segs, poss):
+                seg = int(L) - 1
+                fs.segments[seg] = True
+                dzi = julia_round(ms.diameters[s.sequence[seg]] / bins[-1] / 2.0)
+                xc = julia_round(grid[0] / 2) + 1
+                yc = julia_round(grid[1] / 2) + 1
+                if pos == "left":
+                    right += 1          # the reference's counters, as written
+                    zc = idx_lw + dzi
+                elif pos == "right":
+                    left += 1
+                    zc = idx_rw - dzi
+                else:
+                    raise SetupError(f"Invalid grafting position tag for species {u}, segment {L}: '{pos}'")
+                fs.coords[seg] = (xc - 1) + grid[0] * ((yc - 1) + grid[1] * (zc - 1))
+        out.append(fs)
+    if left > 1 or right > 1:
+        raise SetupError("Only one brush may be attached to each wall in 3D.")
+    return out
 
 
 def build_state_points_nd(datasets: Sequence[dict], device_bulk: bool = False) -> List[StatePoint]:

@@ -119,3 +119,57 @@ def test_nd_solve_matches_oracle_solve(precision):
     rb = b.get_field(abi.FIELD_RHO_BEADS, 0).reshape(o.rho_beads.shape, order="F")
     assert np.max(np.abs(rb - o.rho_beads)) < 1e-8 * np.max(np.abs(o.rho_beads))
     b.close()
+
+
+def _brush3d():
+    # grafting density 1 / (2 x 2) chains per sigma^2 (the reference's 3-D rule); at 1 / 1.5^2 the
+    # reference's map no longer converges with these numerics (residual 12 after 6000 iterations)
+    d = configs.weak_polyelectrolyte(H=3.0, dz=0.125, n_seg=4, pH=3.0, pKa=5.0, grafted=0.05)
+    return _nd(d, [2.0, 2.0, 3.0], [0.125, 0.125, 0.125], [3])
+
+
+def test_nd_grafted_brush_objective_and_solve_match_oracle():
+    """A chain grafted at the centre of the left wall of a 3-D box (determine_fixed.jl:31-86):
+    direct ball sums with 4 pi D^2 around the grafting point, grafted normalisation, max-distance
+    cut, graft delta density. FP64 objective 1e-11 (or 20x the oracle's response to a 1e-15 relative
+    change of x where that is larger), FP32 5e-5; the converged FP64 solve (1e-10) is a
+    fixed point of the oracle's map (1e-9), with Omega and bead densities at that point matching
+    the oracle's (an oracle solve of a 3-D brush takes minutes on the host)."""
+    from oracle.dft_oracle_nd import OracleDFTND
+    sp = build_state_point_nd(_brush3d())
+    print(f"case nd grafted brush grid {sp.grid}: objectives", flush=True)
+    o = OracleDFTND(sp)
+    b = abi.Batch([sp])
+    rng = np.random.default_rng(0)
+    sens = 0.0
+    for x in (sp.x0(), _perturbed(o, sp.x0(), n=2)):
+        ref = o.objective(x)
+        # the map's own round-off response: the grafted bead's point density fills the FMT cap next to
+        # the grafting point, where the derivative carries (1 - n3)^-3; a 1e-15 relative change of x
+        # moves the oracle's F by ~1e-9 here, so the two evaluation orders are compared at that scale
+        sens = max(sens, float(np.max(np.abs(o.objective(x * (1 + 1e-15 * rng.standard_normal(x.size))) - ref))))
+        ref = o.objective(x)
+        scale = max(1.0, float(np.max(np.abs(ref))))
+        d64 = float(np.max(np.abs(b.objective(x, abi.PREC_FP64) - ref)))
+        print(f"  FP64 vs oracle {d64:.2e}, oracle self-sensitivity {sens:.2e}", flush=True)
+        assert d64 < max(1e-11 * scale, 20.0 * sens)
+        assert np.max(np.abs(b.objective(x, abi.PREC_FP32) - ref)) < 5e-5 * scale
+    num = dict(sp.numerics)
+    print("case nd grafted brush: FP64 solve", flush=True)
+    x, it, err = b.solve(sp.x0(), tole=1e-10, damping=num["damping"], mixing_max=num["mixing_max"],
+                         switch_tole=num["switch_tole"], anderm=int(num["anderm"]), max_iters=20000,
+                         precision=abi.PREC_FP64)
+    assert err[0] < 1e-10
+    print(f"case nd grafted brush: {it[0]} iterations; oracle map at the solution", flush=True)
+    fo = o.objective(x)
+    assert np.max(np.abs(fo - x)) < max(1e-9, 20.0 * sens)
+    b.objective(x, abi.PREC_FP64)
+    om, _, _ = b.observables()
+    # the reference's TPT1 free energy takes xi = 1 - |nV2|^2 / n2^2 without the clamp its derivative
+    # uses; next to the walls of this box |nV2| exceeds n2 (xi down to -4e3), y_HS turns negative at a
+    # few points and Omega is NaN, in the oracle as in the library
+    om_o = o.grand_potential(x)
+    assert (np.isnan(om[0]) and np.isnan(om_o)) or abs(om[0] - om_o) < 1e-10 * abs(om_o)
+    rb = b.get_field(abi.FIELD_RHO_BEADS, 0)
+    assert np.max(np.abs(rb - o.rho_beads)) < 1e-9 * np.max(np.abs(o.rho_beads))
+    b.close()

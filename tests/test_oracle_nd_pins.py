@@ -130,6 +130,27 @@ def test_nd_setup_rejects_what_the_path_does_not_build():
     ds1 = configs.weak_polyelectrolyte(H=2.0, dz=0.1, n_seg=3, solvent=0.2, pH=5.0)
     with pytest.raises(SetupError):
         build_state_point_nd(_nd(ds1, [0.4, 2.0], [0.1, 0.1], [2]))   # chains need 3-D (get_renorm)
-    g = configs.weak_polyelectrolyte(H=2.0, dz=0.1, n_seg=3, pH=5.0, grafted=0.05)
+    g = configs.weak_polyelectrolyte(H=2.0, dz=0.1, n_seg=1, pH=5.0, grafted=0.05)
     with pytest.raises(SetupError):
-        build_state_point_nd(_nd(g, [0.4, 0.4, 2.0], [0.1] * 3, [3]))
+        build_state_point_nd(_nd(g, [0.4, 2.0], [0.1] * 2, [2]))     # grafting needs 3-D (determine_fixed)
+
+
+def test_grafted_chain_placement_and_distances_in_3d():
+    """determine_fixed (CartesianXYZ, determine_fixed.jl:31-86): centre of the wall in x and y,
+    round(d / h_z / 2) steps from the left wall, density 1 / (L_x L_y); the oracle's distances are the
+    Euclidean ones of compute_distance_squared, and its bond renormalisation is 4 pi D^2."""
+    d = configs.weak_polyelectrolyte(H=3.0, dz=0.125, n_seg=4, pH=3.0, pKa=5.0, grafted=0.05)
+    sp = build_state_point_nd(_nd(d, [2.0, 2.0, 3.0], [0.125, 0.125, 0.125], [3]))
+    g = sp.grid
+    fx = sp.fixed[0]
+    assert fx.any and fx.segments[0] and not fx.segments[1:].any()
+    i, j, k = np.unravel_index(int(fx.coords[0]), g, order="F")
+    assert (i, j, k) == (round(g[0] / 2), round(g[1] / 2), round(1.0 / 0.125 / 2))   # 0-based
+    assert fx.density == pytest.approx(1.0 / (2.0 * 2.0))
+    assert sp.mu_species[0] == 0.0
+    o = OracleDFTND(sp)
+    K = np.arange(sp.NP)
+    a = np.array(np.unravel_index(K, g, order="F")).T * 0.125
+    c = np.array([i, j, k]) * 0.125
+    assert np.allclose(o._dist2(K, int(fx.coords[0])), ((a - c) ** 2).sum(1), rtol=0, atol=1e-14)
+    assert o._renorm(1.0) == pytest.approx(4.0 * np.pi)
