@@ -116,6 +116,15 @@ def weak_polyelectrolyte(H: float = 30.0, dz: float = 0.01, n_seg: int = 100, rh
             "numerical": dict(numerics or SCF_NUMERICS_CHARGED), "model": models + ["coul"]}
 
 
+def diprotic_polyelectrolyte(pK2: float = 6.5, **kw) -> dict:
+    """weak_polyelectrolyte with a second dissociation step B -> C + Y (pK2), C of valence -2: every
+    chain segment carries the three-state family {A, B, C} (state_families.jl)."""
+    d = weak_polyelectrolyte(**kw)
+    d["monomers"]["C"] = {"diameter": 1.0, "valence": -2.0}
+    d["reactions"].insert(1, {"species": ["B", "Y", "C"], "coeffs": [-1, 1, 1], "pK": pK2})
+    return d
+
+
 # ---------------------------------------------------------------- configs
 
 def config1() -> dict:

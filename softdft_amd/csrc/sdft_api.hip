@@ -703,11 +703,11 @@ static int build_chain_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const st
     const SpeciesHost& sh = c.species[u];
     ch = ChainPlanHost();
     const int N = sh.nseg, S = sh.S;
-    if (N < 2 || sh.fixed_seg >= 0 || S > SDFT_MAX_S || sh.model == SDFT_MODEL_MONOMERBEAD) return SDFT_OK;
+    if (N < 2 || sh.fixed_seg >= 0 || S > SDFT_FAST_S || sh.model == SDFT_MODEL_MONOMERBEAD) return SDFT_OK;
     std::vector<int> ck;
     if (!column_kernels(ctx, sys, u, canon, ck)) return SDFT_OK;
     for (int sw = 0; sw < 2; ++sw)
-        for (int k = 0; k < SDFT_MAX_S * SDFT_MAX_S; ++k) ch.ck[sw][k] = k < S * S ? ck[sw * S * S + k] : -1;
+        for (int k = 0; k < SDFT_FAST_S * SDFT_FAST_S; ++k) ch.ck[sw][k] = k < S * S ? ck[sw * S * S + k] : -1;
     const int* st = ctx->seg_states_h.data() + sh.state_off;
     ch.homog = true;
     for (int q = 1; q < N; ++q)
@@ -774,7 +774,8 @@ static int build_prop_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const std
     Ctx& c = ctx->c;
     const SpeciesHost& sh = c.species[u];
     const int S = sh.S;
-    if (sh.nseg < 2 || sh.model == SDFT_MODEL_MONOMERBEAD) return SDFT_OK;
+    // more than two states: the direct step kernel (its general-state instantiation)
+    if (sh.nseg < 2 || sh.model == SDFT_MODEL_MONOMERBEAD || S > SDFT_FAST_S) return SDFT_OK;
     std::vector<int> ck;
     if (!column_kernels(ctx, sys, u, canon, ck)) return SDFT_OK;  // not homogeneous: keep the direct kernel
     struct Key { int kset, kern; bool operator<(const Key& o) const { return kset != o.kset ? kset < o.kset : kern < o.kern; } };
@@ -1171,6 +1172,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
         ctx->state_off.push_back(state_off);
         ctx->multistate.push_back(multi);
         ctx->S_max = std::max(ctx->S_max, S);
+        c.states_max = std::max(c.states_max, S);
         goff += (int64_t)N * S * S;
         state_off += N * S;
         maxd_off += N;
@@ -1398,7 +1400,7 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
             SpeciesHost& sh = c.species[u];
             sh.band_step = 0;
             sh.band_be = 0;
-            if (sh.nseg < 2 || sh.S > SDFT_MAX_S) continue;
+            if (sh.nseg < 2 || sh.S > SDFT_FAST_S) continue;
             bool ok = true;
             for (int p = 0; p < P; ++p) {
                 const int Be = chain_half_width(band[ctx->pts[p].kset]);

@@ -44,7 +44,7 @@ struct ChainArgs {
     int32_t hv;                    // output factors exp(lng/2)/normative tabulated in LDS (homog only)
     int32_t pad2;
     int64_t goff;
-    int32_t ck[2][SDFT_MAX_S * SDFT_MAX_S];  // canonical bond kernel of (sweep, a * S + b)
+    int32_t ck[2][SDFT_FAST_S * SDFT_FAST_S];  // canonical bond kernel of (sweep, a * S + b)
     int32_t npad_max, be_max;                // LDS carve-out (largest point of the launch)
     int32_t rescale, nsp, nmax, pad;         // per-step rescaling (scale_slot tables)
     unsigned long long* smax;
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
     T* X = cw + C * KC;                        // [C][XLP]
     T* Q = X + C * XLP;                        // [S][NPQ]
     T* HV = Q + S * NPQ;                       // [C][NPQ] exp(lng/2) of the output end / normative (ca.hv)
-    __shared__ ChainCol info[2][SDFT_MAX_S * SDFT_MAX_S];
+    __shared__ ChainCol info[2][SDFT_FAST_S * SDFT_FAST_S];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double norm = P.normative[ca.u];
@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
 
     // input weights W_ab of the extended element mi of each column; with EPC > 0 each thread keeps
     // the weights of its elements (mi = tid + k * CHAIN_THREADS) in registers
-    T wreg[SDFT_MAX_S * SDFT_MAX_S][EPC > 0 ? EPC : 1];
+    T wreg[SDFT_FAST_S * SDFT_FAST_S][EPC > 0 ? EPC : 1];
     const int64_t eb = P.base * (int64_t)ca.NB;
     const int64_t hb = P.base * (int64_t)(2 * ca.MB);
     auto ext_index = [&](int mi) -> int {  // point index of extended element mi, -1 beyond the mirror
@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
             qscale = (T)(1.0 / m);
         }
 #pragma unroll
-        for (int col = 0; col < SDFT_MAX_S * SDFT_MAX_S; ++col) {
+        for (int col = 0; col < SDFT_FAST_S * SDFT_FAST_S; ++col) {
             if (col < C) {
                 const ChainCol cc = ci[col];
                 const T* el = elam + eb + (int64_t)cc.sj * NP;
@@ -222,10 +222,10 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
             const int a = unit / ntile, tile = unit - a * ntile;
             const int p0 = tile * 16;
             // epilogue factors exp(lng/2) of the output end, fetched before the products
-            T hv[SDFT_MAX_S][4];
+            T hv[SDFT_FAST_S][4];
             if (!ca.hv) {
 #pragma unroll
-                for (int b = 0; b < SDFT_MAX_S; ++b) {
+                for (int b = 0; b < SDFT_FAST_S; ++b) {
                     const ChainCol cc = ci[(a * S + b) % C];
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
             }
             T qs[4] = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-            for (int b = 0; b < SDFT_MAX_S; ++b) {
+            for (int b = 0; b < SDFT_FAST_S; ++b) {
                 if (b >= S) break;
                 const int col = a * S + b;
                 const ChainCol cc = ci[col];
@@ -354,7 +354,7 @@ void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, 
     ca.GF = c.GF; ca.MB = c.MB; ca.NB = c.NB; ca.homog = ch.homog ? 1 : 0;
     ca.goff = sh.goff;
     for (int sw = 0; sw < 2; ++sw)
-        for (int k = 0; k < SDFT_MAX_S * SDFT_MAX_S; ++k) ca.ck[sw][k] = ch.ck[sw][k];
+        for (int k = 0; k < SDFT_FAST_S * SDFT_FAST_S; ++k) ca.ck[sw][k] = ch.ck[sw][k];
     ca.npad_max = ch.npad_max; ca.be_max = ch.be_max;
     ca.rescale = sh.rescale; ca.nsp = c.nsp; ca.nmax = c.nmax_seg; ca.smax = smax;
     ca.info = ch.d_info;
@@ -439,7 +439,7 @@ __global__ void __launch_bounds__(256) k_bandstep(const DevSystem* __restrict__ 
     extern __shared__ unsigned char band_lds[];
     T* cw = reinterpret_cast<T*>(band_lds);  // [S][KC] operator windows, reversed
     T* X = cw + S * KC;                      // [S][XLP] extended inputs (one pad word per 16)
-    __shared__ int32_t colv[SDFT_MAX_S], colsj[SDFT_MAX_S], colbond[SDFT_MAX_S], colpi[SDFT_MAX_S], colpj[SDFT_MAX_S];
+    __shared__ int32_t colv[SDFT_FAST_S], colsj[SDFT_FAST_S], colbond[SDFT_FAST_S], colpi[SDFT_FAST_S], colpj[SDFT_FAST_S];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid < S) {
         const int sj = st[inseg * S + tid];

@@ -16,7 +16,9 @@
 
 #define SDFT_MAX_NB 16
 #define SDFT_MAX_MB 16
-#define SDFT_MAX_S 2
+#define SDFT_MAX_S 4            // states per segment (state families, state_families.jl)
+#define SDFT_FAST_S 2           // the MFMA chain, banded-step and operator-GEMM propagators and the
+                                // two-state density accumulation cover species up to this many states
 #define SDFT_MAX_SPECIES 16
 #define SDFT_MAX_SW 16          // square-well pairs
 #define SDFT_SW_TERMS 3         // square-well partners per bead type (terms of one convolution job)

@@ -183,7 +183,7 @@ __global__ void k_prepare(const DevSystem* __restrict__ psys, const DevPoint* __
 
 
 
-template <typename T, int TB>
+template <typename T, int TB, int SM>
 __global__ void __launch_bounds__(TB) k_prop_step(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
                                                   const int32_t* __restrict__ evalflag, int prec, PropArgs A,
                                                   const int32_t* __restrict__ seg_states, const double* __restrict__ maxd_tab,
@@ -232,8 +232,8 @@ __global__ void __launch_bounds__(TB) k_prop_step(const DevSystem* __restrict__ 
     const int64_t oslot = scale_slot(p, A.u, sweep, A.s, A.nsp, A.nmax);
 
     int npair = 0;
-    int pa[SDFT_MAX_S * SDFT_MAX_S], pb[SDFT_MAX_S * SDFT_MAX_S], pbond[SDFT_MAX_S * SDFT_MAX_S];
-    int ppi[SDFT_MAX_S * SDFT_MAX_S], ppj[SDFT_MAX_S * SDFT_MAX_S];
+    int pa[SM * SM], pb[SM * SM], pbond[SM * SM];
+    int ppi[SM * SM], ppj[SM * SM];
     for (int a = 0; a < S; ++a) {
         const int si = st[outseg * S + a];
         if (si < 0) continue;
@@ -298,11 +298,11 @@ __global__ void __launch_bounds__(TB) k_prop_step(const DevSystem* __restrict__ 
     }
 
     // convolution path
-    __shared__ T fs[SDFT_MAX_S * SDFT_MAX_S][TB];
-    __shared__ T cm[SDFT_MAX_S * SDFT_MAX_S][2 * TB];
-    __shared__ T cp[SDFT_MAX_S * SDFT_MAX_S][2 * TB];
-    T acc[SDFT_MAX_S * SDFT_MAX_S];
-    for (int q = 0; q < SDFT_MAX_S * SDFT_MAX_S; ++q) acc[q] = T(0);
+    __shared__ T fs[SM * SM][TB];
+    __shared__ T cm[SM * SM][2 * TB];
+    __shared__ T cp[SM * SM][2 * TB];
+    T acc[SM * SM];
+    for (int q = 0; q < SM * SM; ++q) acc[q] = T(0);
     const T* cset = kern + P.kbase * (int64_t)K;
     const int band = A.band >= 0 ? band_of_set[P.kset] : -1;
     int k_lo = 0, k_hi = NP;
@@ -484,7 +484,7 @@ static_assert(ACC_POINTS * ACC_GROUPS == 256, "k_accumulate runs 256-thread bloc
 // The FP64 variant is held to 4 waves per SIMD (128 VGPRs, a 12-byte spill): the kernel is bound by
 // load latency, and the extra wave made the full-load launch 156 -> 133 us. The FP32 variant reaches
 // 4 waves without a bound (held to 5 it spills 84 bytes per lane).
-template <typename T, int TB>
+template <typename T, int TB, int SM>
 __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts,
                                                    const int32_t* __restrict__ evalflag, int prec,
                                                    const DevSpecies* __restrict__ spec, const int64_t* __restrict__ goffs, int GF,
@@ -549,12 +549,12 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
         // monomerbead (monomerbead/accumulate_densities.jl:46-85): every segment independent,
         // rho = norm exp(mu_u - lambda), no propagators and no bonds
         const bool mb = sp.model == SDFT_MODEL_MONOMERBEAD;
-        double racc[SDFT_MAX_S], rc0[SDFT_MAX_S][SDFT_MAX_S], rc1[SDFT_MAX_S][SDFT_MAX_S];
+        double racc[SM], rc0[SM][SM], rc1[SM][SM];
 #pragma unroll
-        for (int a = 0; a < SDFT_MAX_S; ++a) {
+        for (int a = 0; a < SM; ++a) {
             racc[a] = 0.0;
 #pragma unroll
-            for (int b = 0; b < SDFT_MAX_S; ++b) rc0[a][b] = rc1[a][b] = 0.0;
+            for (int b = 0; b < SM; ++b) rc0[a][b] = rc1[a][b] = 0.0;
         }
         for (int q = q0; q < N; q += ACC_GROUPS) {
             const double pref = sp.rescale ? pref0 * exp(seg_log_scale(sp, P, p, u, q, lsc, sys.n_species, nmax))
@@ -563,9 +563,9 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
             // gcv[b][a] = gC[q][b][a] (child side, b: state of q+1), gpv[b][a] = gP[q][b][a] (parent side)
             // The loads are unconditional, from clamped (always in-bounds) indices, and masked after
             // the fact, so that they issue back to back instead of one conditional load at a time.
-            T gcv[SDFT_MAX_S][SDFT_MAX_S], gpv[SDFT_MAX_S][SDFT_MAX_S], elv[SDFT_MAX_S], tzv[SDFT_MAX_S];
+            T gcv[SM][SM], gpv[SM][SM], elv[SM], tzv[SM];
 #pragma unroll
-            for (int a = 0; a < SDFT_MAX_S; ++a) {
+            for (int a = 0; a < SM; ++a) {
                 const int ac = a < S ? a : 0;
                 const int si = a < S ? st[q * S + a] : -1;
                 const int sic = si >= 0 ? si : 0;
@@ -573,7 +573,7 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                 elv[a] = si >= 0 ? e : T(0);
                 tzv[a] = si >= 0 ? t : T(0);
 #pragma unroll
-                for (int b = 0; b < SDFT_MAX_S; ++b) {
+                for (int b = 0; b < SM; ++b) {
                     const int bc = b < S ? b : 0;
                     const bool okc = si >= 0 && b < S && q < N - 1 && st[(q + 1) * S + b] >= 0;
                     const bool okp = si >= 0 && b < S && q > 0 && st[(q - 1) * S + b] >= 0;
@@ -585,7 +585,7 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                 }
             }
 #pragma unroll
-            for (int a = 0; a < SDFT_MAX_S; ++a) {
+            for (int a = 0; a < SM; ++a) {
                 const int si = a < S ? st[q * S + a] : -1;
                 if (si < 0) continue;
                 double Gp = 0.0, Qc = 0.0;
@@ -595,13 +595,13 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                     if (q == 0) Gp = (double)base_value<T>(0, nc, sp.fixed_seg, Kf, P, maxd);
                     else
 #pragma unroll
-                        for (int b = 0; b < SDFT_MAX_S; ++b)
+                        for (int b = 0; b < SM; ++b)
                             if (b < S && st[(q - 1) * S + b] >= 0) Gp += (double)gpv[b][a];
                     // leaf: sum gC[K, :, 1, idx_i, leaf] is the base case 1 for every state
                     if (q == N - 1) Qc = (double)base_value<T>(q, nc, sp.fixed_seg, Kf, P, maxd);
                     else
 #pragma unroll
-                        for (int b = 0; b < SDFT_MAX_S; ++b)
+                        for (int b = 0; b < SM; ++b)
                             if (b < S && st[(q + 1) * S + b] >= 0) Qc += (double)gcv[b][a];
                 }
                 const double el = (double)elv[a];
@@ -623,7 +623,7 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                     }
                     if (q < N - 1 && !mb) {
 #pragma unroll
-                        for (int b = 0; b < SDFT_MAX_S; ++b) {
+                        for (int b = 0; b < SM; ++b) {
                             const int sj = b < S ? st[(q + 1) * S + b] : -1;
                             if (sj < 0) continue;
                             if (hom) {
@@ -640,7 +640,7 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                     if (q > 0 && !mb) {
                         const double c1 = pref * el * Qc;
 #pragma unroll
-                        for (int b = 0; b < SDFT_MAX_S; ++b) {
+                        for (int b = 0; b < SM; ++b) {
                             const int sj = b < S ? st[(q - 1) * S + b] : -1;
                             if (sj < 0) continue;
                             if (hom) {
@@ -660,12 +660,12 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
         if (hom && valid) {
             // segment 0's states and their links stand for every segment (homogeneous species)
 #pragma unroll
-            for (int a = 0; a < SDFT_MAX_S; ++a) {
+            for (int a = 0; a < SM; ++a) {
                 const int si = a < S ? st[a] : -1;
                 if (si < 0) continue;
                 rb[si * TB] += racc[a];
 #pragma unroll
-                for (int b = 0; b < SDFT_MAX_S; ++b) {
+                for (int b = 0; b < SM; ++b) {
                     const int sj = b < S ? st[b] : -1;
                     if (sj < 0 || N < 2 || mb) continue;
                     const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
@@ -1805,11 +1805,15 @@ static size_t accumulate_lds_bytes(const Ctx& c) { return (size_t)(c.NB + 2 * c.
 hipError_t prepare_kernel_attributes(const Ctx& c) {
     const size_t lds = accumulate_lds_bytes(c);
     if (lds <= 65536) return hipSuccess;
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_accumulate<float, 256>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_accumulate<double, 256>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const void* fns[4] = {reinterpret_cast<const void*>(&k_accumulate<float, 256, SDFT_FAST_S>),
+                          reinterpret_cast<const void*>(&k_accumulate<double, 256, SDFT_FAST_S>),
+                          reinterpret_cast<const void*>(&k_accumulate<float, 256, SDFT_MAX_S>),
+                          reinterpret_cast<const void*>(&k_accumulate<double, 256, SDFT_MAX_S>)};
+    for (const void* f : fns) {
+        const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 // Merged back stage: the back projections were stored without their shared partials (computed in
@@ -1834,6 +1838,22 @@ __global__ void __launch_bounds__(256) k_add_partials(const DevPoint* __restrict
         *f = y;
         if (J.outfield2 >= 0) fx[P.xoff + (int64_t)J.outfield2 * P.NP + n] = y;
     }
+}
+
+// the direct propagator step kernel: 256-point blocks for up to two states per segment, 64-point
+// blocks (LDS for 16 state pairs) beyond
+template <typename T>
+static void launch_prop_step_direct(hipStream_t st, const Ctx& c, int prec, const PropArgs& A, int nsweeps,
+                                    int sweep_base, const T* kern, const T* elam, const T* hl, const T* trapez, T* gC,
+                                    T* gP) {
+    if (A.S <= SDFT_FAST_S)
+        hipLaunchKernelGGL((k_prop_step<T, 256, SDFT_FAST_S>), grid2(c.maxNP, 256, c.P, nsweeps), dim3(256), 0, st,
+                           c.d_sys, c.d_pts, c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl, trapez,
+                           gC, gP, c.d_band, sweep_base);
+    else
+        hipLaunchKernelGGL((k_prop_step<T, 64, SDFT_MAX_S>), grid2(c.maxNP, 64, c.P, nsweeps), dim3(64), 0, st,
+                           c.d_sys, c.d_pts, c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl, trapez,
+                           gC, gP, c.d_band, sweep_base);
 }
 
 // propagators of species u (child_propagators.jl / parent_propagators.jl / handle_fixed.jl): the
@@ -1874,15 +1894,11 @@ void Pipeline<T>::propagate(hipStream_t st, const Ctx& c, int prec, int u, bool 
             const int skip = (fix0 && fix1) ? 2 : fix0 ? 0 : fix1 ? 1 : -1;
             if (skip != 2) launch_bandstep<T>(st, c, prec, u, s, skip, kern, elam, hl, trapez, gC, gP, smax);
             if (skip >= 0)
-                hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(maxNP, 256, P, skip == 2 ? 2 : 1), dim3(256), 0, st,
-                                   c.d_sys, c.d_pts, c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl,
-                                   trapez, gC, gP, c.d_band, skip == 2 ? 0 : skip);
+                launch_prop_step_direct<T>(st, c, prec, A, skip == 2 ? 2 : 1, skip == 2 ? 0 : skip, kern, elam, hl, trapez, gC, gP);
             continue;
         }
         if (!gemm) {
-            hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(maxNP, 256, P, 2), dim3(256), 0, st, c.d_sys,
-                               c.d_pts, c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl, trapez,
-                               gC, gP, c.d_band, 0);
+            launch_prop_step_direct<T>(st, c, prec, A, 2, 0, kern, elam, hl, trapez, gC, gP);
             continue;
         }
         const int skip = (fix0 && fix1) ? 2 : fix0 ? 0 : fix1 ? 1 : -1;
@@ -1890,9 +1906,7 @@ void Pipeline<T>::propagate(hipStream_t st, const Ctx& c, int prec, int u, bool 
             launch_opgemm_prop<T>(st, c, prec, c.prop_plan[u], u, s, skip, banded, kern, elam, hl, trapez, gC, gP,
                                   fbuf, smax);
         if (skip >= 0)
-            hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(maxNP, 256, P, skip == 2 ? 2 : 1), dim3(256), 0, st,
-                               c.d_sys, c.d_pts, c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl,
-                               trapez, gC, gP, c.d_band, skip == 2 ? 0 : skip);
+            launch_prop_step_direct<T>(st, c, prec, A, skip == 2 ? 2 : 1, skip == 2 ? 0 : skip, kern, elam, hl, trapez, gC, gP);
     }
     if (sh.rescale)
         hipLaunchKernelGGL(k_scale_prefix, dim3((P + 63) / 64), dim3(64), 0, st, c.d_evalflag, prec, P, u, sh.nseg,
@@ -1948,10 +1962,18 @@ void Pipeline<T>::objective(hipStream_t st, const Ctx& c, int prec, const double
 
 template <typename T>
 void Pipeline<T>::launch_accumulate(hipStream_t st, const Ctx& c, int prec) {
-    hipLaunchKernelGGL((k_accumulate<T, 256>), grid2(c.maxNP, ACC_POINTS, c.P), dim3(256), accumulate_lds_bytes(c), st,
-                       c.d_sys, c.d_pts, c.d_evalflag, prec, c.d_spec, c.d_goffs, c.GF, c.d_seg_states, c.d_maxd, elam,
-                       trapez, gC, gP, c.d_fixed_lambda, rho_beads, rho_bonds, rhoT, c.Frt, c.d_acc_part, c.d_errflag,
-                       lsc, c.nmax_seg);
+    // species of up to two states per segment (every benchmark configuration) take the two-state
+    // instantiation, whose per-state arrays stay in registers
+    if (c.states_max <= SDFT_FAST_S)
+        hipLaunchKernelGGL((k_accumulate<T, 256, SDFT_FAST_S>), grid2(c.maxNP, ACC_POINTS, c.P), dim3(256),
+                           accumulate_lds_bytes(c), st, c.d_sys, c.d_pts, c.d_evalflag, prec, c.d_spec, c.d_goffs, c.GF,
+                           c.d_seg_states, c.d_maxd, elam, trapez, gC, gP, c.d_fixed_lambda, rho_beads, rho_bonds, rhoT,
+                           c.Frt, c.d_acc_part, c.d_errflag, lsc, c.nmax_seg);
+    else
+        hipLaunchKernelGGL((k_accumulate<T, 256, SDFT_MAX_S>), grid2(c.maxNP, ACC_POINTS, c.P), dim3(256),
+                           accumulate_lds_bytes(c), st, c.d_sys, c.d_pts, c.d_evalflag, prec, c.d_spec, c.d_goffs, c.GF,
+                           c.d_seg_states, c.d_maxd, elam, trapez, gC, gP, c.d_fixed_lambda, rho_beads, rho_bonds, rhoT,
+                           c.Frt, c.d_acc_part, c.d_errflag, lsc, c.nmax_seg);
 }
 
 template <typename T>
@@ -1967,8 +1989,7 @@ void Pipeline<T>::launch_prop_step(hipStream_t st, const Ctx& c, int prec, int u
         launch_opgemm_prop<T>(st, c, prec, c.prop_plan[u], u, s, -1, banded, kern, elam, hl, trapez, gC, gP,
                               fbuf, smax);
     else
-        hipLaunchKernelGGL((k_prop_step<T, 256>), grid2(c.maxNP, 256, c.P, 2), dim3(256), 0, st, c.d_sys, c.d_pts,
-                           c.d_evalflag, prec, A, c.d_seg_states, c.d_maxd, kern, elam, hl, trapez, gC, gP, c.d_band, 0);
+        launch_prop_step_direct<T>(st, c, prec, A, 2, 0, kern, elam, hl, trapez, gC, gP);
 }
 
 // forward: weighted densities; back: shared partials first, then the projections that add them

@@ -106,7 +106,7 @@ struct ChainPlanHost {
                                                    // held in registers (fewer registers per lane)
     bool homog = false;                            // every segment carries the same state beads
     int S = 1;
-    int ck[2][SDFT_MAX_S * SDFT_MAX_S];            // canonical bond kernel of (sweep, a * S + b), -1 absent
+    int ck[2][SDFT_FAST_S * SDFT_FAST_S];          // canonical bond kernel of (sweep, a * S + b), -1 absent
     int npad_max = 0, be_max = 0;                  // largest padded NP and even band half-width
     ChainCol* d_info = nullptr;                    // [2 sweeps][N-1 steps][S*S] column table (device)
     bool hv32 = false, hv64 = false;               // homogeneous chain: the output factors
@@ -204,6 +204,7 @@ struct Ctx {
     DevSolveState* d_state = nullptr;
     int32_t* d_n_active = nullptr;  // [active, active at FP32, waiting for the FP64 switch]
     int nmax_seg = 1;               // longest species (propagator scale tables)
+    int states_max = 1;             // most states per segment over the species
     bool any_rescale = false;       // some species propagates with per-step rescaling
     // FFT convolutions: per kernel set resources, and the spectra work buffers per precision
     // (0: FP32, 1: FP64; complex elements: inputs and outputs of the largest FFT stage)

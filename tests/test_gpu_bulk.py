@@ -30,6 +30,8 @@ def _cases():
         "square_well": [configs.square_well_chains(H=6.0, dz=0.05, n_seg=5)],
         "mfmt_msa": [dict(configs.weak_polyelectrolyte(H=5.0, dz=0.05, n_seg=4, solvent=0.2, pH=4.0, msa="msa"),
                           model=["ideal", "mfmt", "msa", "coul"])],
+        "diprotic": [configs.diprotic_polyelectrolyte(H=5.0, dz=0.05, n_seg=4, solvent=0.2, pH=pH)
+                     for pH in (3.0, 5.5, 8.0)],
     }
 
 

@@ -306,6 +306,19 @@ def test_ideal_chain_without_walls_reproduces_bulk_densities():
         assert np.allclose(o.rho_beads[:, j], sp.bulk.rho_beads[j], rtol=1e-12, atol=1e-300)
 
 
+def test_diprotic_state_fractions_without_interactions():
+    """Three states per segment (A -> B + Y, pKa; B -> C + Y, pK2): without interactions the state
+    fractions are the closed form (1, 10^(pH - pKa), 10^(2 pH - pKa - pK2)) / Z for every segment."""
+    for pH in (3.5, 5.5, 7.5):
+        ds = _ideal(configs.diprotic_polyelectrolyte(H=3.0, dz=0.05, n_seg=4, pH=pH, pKa=4.5, pK2=6.5))
+        ds["geometry"]["external_field"] = {}
+        sp, o = _oracle_for(ds)
+        assert [len(f) for f in sp.ms.species[0].state_family] == [3] * 4
+        o.objective(sp.x0())
+        w = np.array([1.0, 10 ** (pH - 4.5), 10 ** (2 * pH - 4.5 - 6.5)])
+        assert np.allclose(o.state_fractions()[0], w / w.sum(), rtol=1e-12)
+
+
 def test_henderson_hasselbalch_without_interactions():
     for pH in (3.5, 4.5, 6.0):
         ds = _ideal(configs.weak_polyelectrolyte(H=3.0, dz=0.05, n_seg=4, pH=pH, pKa=4.5))
