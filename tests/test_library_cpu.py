@@ -87,3 +87,43 @@ def test_hard_walls_need_contact_tables_and_indices_inside_the_grid():
     if not os.path.exists("/dev/kfd"):
         # a valid descriptor passes validation and reaches the device check
         assert lib.sdft_create(C.byref(d), C.byref(h)) == -2
+
+
+def test_multi_dimensional_descriptor_validation():
+    """2-D / 3-D descriptors are checked before any device call: NP must be the grid's point count
+    and dz its volume element, every point must share the grid, 2..128 points per dimension, no
+    contact-value tables, and grafted segments only in 3-D (determine_fixed has no 2-D method)."""
+    import copy
+    from softdft_amd.geometry_nd import build_state_point_nd
+    lib = abi.load_library()
+    d1 = configs.weak_polyelectrolyte(H=2.0, dz=0.1, n_seg=3, solvent=0.2, pH=5.0)
+    d3 = copy.deepcopy(d1)
+    d3["geometry"]["dimensions"] = [0.5, 0.4, 2.0]
+    d3["geometry"]["bin_width"] = [0.1, 0.1, 0.1]
+    d3["geometry"]["external_field"] = {"hard_walls": {"dims": [3]}}
+    sp = build_state_point_nd(d3)
+    d, keep, _, _ = abi.build_descriptor([sp])
+    assert d.ndim == 3 and d.hard_walls == 0
+    h = C.c_void_p()
+    keep["NP"][0] += 1
+    assert lib.sdft_create(C.byref(d), C.byref(h)) == -1
+    assert b"NP" in lib.sdft_last_error(None)
+    keep["NP"][0] -= 1
+    keep["grid"][0, 0] = 200
+    assert lib.sdft_create(C.byref(d), C.byref(h)) == -1
+    keep["grid"][0, 0] = sp.grid[0]
+    d.hard_walls = 1
+    assert lib.sdft_create(C.byref(d), C.byref(h)) == -5
+    d.hard_walls = 0
+    if not os.path.exists("/dev/kfd"):
+        assert lib.sdft_create(C.byref(d), C.byref(h)) == -2
+    # a grafted species on a 2-D grid
+    fseg = keep["fseg"].copy()
+    keep["fseg"][0] = 0
+    d.ndim = 2
+    keep["grid"][0, :] = [sp.grid[0], sp.grid[1] * sp.grid[2], 1]
+    keep["bins"][0, :] = [0.1, 0.1, 1.0]
+    keep["dz"][0] = 0.01
+    assert lib.sdft_create(C.byref(d), C.byref(h)) == -5
+    assert b"grafted" in lib.sdft_last_error(None)
+    keep["fseg"][:] = fseg
