@@ -733,6 +733,10 @@ static int build_chain_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const st
     };
     ch.hv32 = hv_fits(sizeof(float), ch.ok32);
     ch.hv64 = hv_fits(sizeof(double), ch.ok64);
+    // homogeneous chains whose double-buffered inputs and weight tables fit: one barrier per step
+    // (FP32 keeps two workgroups per CU, so within 80 KB)
+    ch.fused32 = ch.ok32 && ch.homog && chain_fused_lds_bytes(ch, sizeof(float)) <= 80 * 1024;
+    ch.fused64 = ch.ok64 && ch.homog && chain_fused_lds_bytes(ch, sizeof(double)) <= 160 * 1024 - 256;
     ch.ok = ch.ok32 || ch.ok64;
     ch.elig32 = ch.ok32;
     ch.elig64 = ch.ok64;

@@ -308,6 +308,244 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
     }
 }
 
+// --------------------------------------------------------------------------
+// Homogeneous chains (every segment carries the same state beads, so the input weights
+// W_ab = exp(-lambda) exp(lng/2) trapez and the output factors are the same at every step):
+// one barrier per step. The next step's extended inputs X_{s+1}[(a', a)] = W_(a', a) Q_a are written
+// by the epilogue that produces Q_a, into the second of two X buffers, so the separate staging pass
+// and its barrier go away. The per-step rescaling is applied at the output instead of the input,
+// g_{s+1} = |A (W Q_s)| hv / max Q_s (the operator is linear, so the propagators are the same up to
+// rounding), and the maximum is reduced per wave in the epilogue.
+// The MFMA row r of a tile holds output phase phi(r): for FP64 (D[4 v + l/16][l % 16]) phi permutes
+// the rows so that the four accumulators of a lane are four consecutive outputs, as they are for
+// FP32 (D[4 (l/16) + v][l % 16]); the epilogue then reads its factors and stores its propagators
+// as 4-element vectors.
+// --------------------------------------------------------------------------
+__host__ __device__ __forceinline__ int round4(int n) { return (n + 3) & ~3; }
+
+size_t chain_fused_lds_bytes(const ChainPlanHost& ch, size_t elem) {
+    const int S = ch.S, C = S * S;
+    const int KC = 2 * ch.be_max + 32;
+    const int XLm = ch.npad_max + 2 * ch.be_max;
+    const int XLP = XLm + (XLm >> 4) + 1;
+    return (size_t)(round4(C * KC) + round4(2 * C * XLP) + 2 * C * ch.npad_max + CHAIN_LDS_SLACK) * elem;
+}
+
+template <typename T> struct Vec4;
+template <> struct Vec4<float> {
+    static __device__ __forceinline__ void load(const float* p, float (&v)[4]) {
+        const float4 x = *reinterpret_cast<const float4*>(p);
+        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+    }
+    static __device__ __forceinline__ void store(float* p, const float (&v)[4]) {
+        *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    }
+};
+template <> struct Vec4<double> {
+    static __device__ __forceinline__ void load(const double* p, double (&v)[4]) {
+        const double2 x = reinterpret_cast<const double2*>(p)[0], y = reinterpret_cast<const double2*>(p)[1];
+        v[0] = x.x; v[1] = x.y; v[2] = y.x; v[3] = y.y;
+    }
+    static __device__ __forceinline__ void store(double* p, const double (&v)[4]) {
+        reinterpret_cast<double2*>(p)[0] = make_double2(v[0], v[1]);
+        reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
+    }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain_fused(
+        const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts, const int32_t* __restrict__ evalflag,
+        int prec, ChainArgs ca, const int32_t* __restrict__ band, const T* __restrict__ kern,
+        const T* __restrict__ elam, const T* __restrict__ hl, const T* __restrict__ trapez, T* __restrict__ gC,
+        T* __restrict__ gP) {
+    const int p = blockIdx.x, sw = blockIdx.y;
+    if (evalflag[p] != prec) return;
+    const DevSystem& sys = *psys;
+    const DevPoint& P = pts[p];
+    const int NP = P.NP, L = P.L;
+    const int Be = chain_half_width(band[P.kset]);
+    const int KS = Be / 2 + 4;                 // k-steps: K / 4 with K = 2 Be + 16
+    const int ntile = (NP + 255) >> 8;
+    const int XL = ntile * 256 + 2 * Be;       // extended input length of this point
+    const int S = ca.S, C = S * S, N = ca.nseg;
+    const int KC = 2 * ca.be_max + 32;
+    const int XLm = ca.npad_max + 2 * ca.be_max;
+    const int XLP = XLm + (XLm >> 4) + 1;
+    const int NPQ = ca.npad_max;
+
+    extern __shared__ __attribute__((aligned(16))) unsigned char chain_fused_lds[];
+    T* cw = reinterpret_cast<T*>(chain_fused_lds);  // [C][KC] operator windows (reversed)
+    T* XB = cw + round4(C * KC);                    // [2][C][XLP] extended inputs, by step parity
+    T* WT = XB + round4(2 * C * XLP);               // [C][NPQ] input weights W
+    T* HV = WT + C * NPQ;                           // [C][NPQ] exp(lng/2) of the output end / normative
+    __shared__ ChainCol cinfo[SDFT_FAST_S * SDFT_FAST_S];
+    __shared__ double redm[2][CHAIN_WAVES];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const double norm = P.normative[ca.u];
+    {
+        const T* cset = kern + (int64_t)P.kbase * sys.K;
+        for (int e = tid; e < C * KC; e += CHAIN_THREADS) {
+            const int col = e / KC, ri = e - col * KC;
+            const int k = ca.ck[sw][col];
+            const int j = (KC - 1 - ri) - 15 - Be;
+            T v = T(0);
+            if (k >= 0 && j >= -Be && j <= Be) {
+                int jj = j % L;
+                if (jj < 0) jj += L;
+                v = cset[(int64_t)k * L + jj];
+            }
+            cw[e] = v;
+        }
+    }
+    // the columns of the first step hold for every step of a homogeneous chain
+    if (tid < C) cinfo[tid] = ca.info[(int64_t)sw * (N - 1) * C + tid];
+    __syncthreads();
+    const int64_t eb = P.base * (int64_t)ca.NB;
+    const int64_t hb = P.base * (int64_t)(2 * ca.MB);
+    for (int e = tid; e < C * NP; e += CHAIN_THREADS) {
+        const int col = e / NP, n = e - col * NP;
+        const ChainCol cc = cinfo[col];
+        T w = T(0), h = T(0);
+        if (cc.valid) {
+            w = elam[eb + (int64_t)cc.sj * NP + n] * hl[hb + (int64_t)cc.hl_in * NP + n] *
+                trapez[eb + (int64_t)cc.sj * NP + n];
+            h = hl[hb + (int64_t)cc.hl_out * NP + n] / (T)norm;
+        }
+        WT[col * NPQ + n] = w;
+        HV[col * NPQ + n] = h;
+    }
+    __syncthreads();
+    // inputs of step 1 (Q = 1 at the free chain end) into buffer 1; positions past the mirror stay
+    // zero in both buffers (the epilogues write every other position)
+    for (int e = tid; e < C * XL; e += CHAIN_THREADS) {
+        const int col = e / XL, mi = e - col * XL;
+        const int m = mi - Be;
+        const int n = m < 0 ? -1 - m : (m < NP ? m : 2 * NP - 1 - m);
+        XB[(C + col) * XLP + chain_phys(mi)] = n >= 0 ? WT[col * NPQ + n] : T(0);
+        if (n < 0) XB[col * XLP + chain_phys(mi)] = T(0);
+    }
+
+    T* gout = sw == 0 ? gC : gP;
+    const int64_t gb = P.gbase + ca.goff * P.gstride;
+    const int q = lane & 15, g = lane >> 4;
+    const int phq = sizeof(T) == 8 ? 4 * (q & 3) + (q >> 2) : q;   // output phase of MFMA row q
+    for (int s = 1; s < N; ++s) {
+        const int outseg = sw == 0 ? N - 1 - s : s;
+        __syncthreads();  // inputs of step s written; the wave maxima of step s - 1 complete
+        T scale = T(1);
+        if (ca.rescale && s > 1) {
+            double m = 0.0;
+#pragma unroll
+            for (int w = 0; w < CHAIN_WAVES; ++w) m = redm[(s - 1) & 1][w] > m ? redm[(s - 1) & 1][w] : m;
+            if (tid == 0) {
+                const int64_t slot = scale_slot(p, ca.u, sw, s - 1, ca.nsp, ca.nmax);
+                ca.smax[slot] = (unsigned long long)__double_as_longlong(m);
+            }
+            m = (m > 0.0 && m <= 1.7976931348623157e308) ? m : 1.0;  // scale_read's rule
+            scale = (T)(1.0 / m);
+        }
+        const T* Xc = XB + (s & 1) * C * XLP;
+        T* Xn = XB + ((s + 1) & 1) * C * XLP;
+        const bool last = s == N - 1;
+        double wmax = 0.0;
+        for (int unit = wave; unit < S * ntile; unit += CHAIN_WAVES) {
+            const int a = unit / ntile, tile = unit - a * ntile;
+            const int p0 = tile * 16;
+            const int n0 = 16 * (p0 + q) + 4 * g;   // the lane's four consecutive outputs n0 .. n0 + 3
+            T qs[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int b = 0; b < SDFT_FAST_S; ++b) {
+                if (b >= S) break;
+                const int col = a * S + b;
+                if (!cinfo[col].valid) continue;
+                const T* ap = cw + col * KC + (KC - 16 - 2 * Be) - phq + g;
+                const T* xp = Xc + col * XLP + 17 * (p0 + q) + g;
+                typename Mfma<T>::acc_t acc;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[v] = T(0);
+                T a0[4], b0[4], a1[4], b1[4];
+                auto fetch = [&](int K, T (&av)[4], T (&bv)[4]) {
+#pragma unroll
+                    for (int k4 = 0; k4 < 4; ++k4) {
+                        av[k4] = ap[16 * K + 4 * k4];
+                        bv[k4] = xp[17 * K + 4 * k4];
+                    }
+                };
+                const int KG = KS / 4;
+                fetch(0, a0, b0);
+                int K = 0;
+                for (; K + 1 < KG; K += 2) {
+                    fetch(K + 1, a1, b1);
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int k4 = 0; k4 < 4; ++k4) acc = Mfma<T>::mma(a0[k4], b0[k4], acc);
+                    __builtin_amdgcn_sched_barrier(0);
+                    fetch(K + 2, a0, b0);  // at most one group past the last: within the LDS carve-out, unused
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int k4 = 0; k4 < 4; ++k4) acc = Mfma<T>::mma(a1[k4], b1[k4], acc);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if (K < KG) {
+#pragma unroll
+                    for (int k4 = 0; k4 < 4; ++k4) acc = Mfma<T>::mma(a0[k4], b0[k4], acc);
+                }
+                for (int ks = 4 * KG; ks < KS; ++ks)
+                    acc = Mfma<T>::mma(ap[4 * ks], xp[17 * (ks >> 2) + 4 * (ks & 3)], acc);
+                if (n0 < NP) {
+                    T hv[4], val[4];
+                    Vec4<T>::load(HV + col * NPQ + n0, hv);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const T y = acc[v];
+                        const T ay = y < T(0) ? -y : y;
+                        val[v] = n0 + v < NP ? ay * hv[v] * scale : T(0);
+                        qs[v] += val[v];
+                    }
+                    // the padding of the propagator rows past NP receives zeros
+                    Vec4<T>::store(gout + gb + (((int64_t)outseg * S + b) * S + a) * P.gstride + n0, val);
+                }
+            }
+            if (!last && n0 < NP) {
+                // inputs of step s + 1 whose input state is a: X[(a2, a)] = W Q_a, at the point and its mirrors
+#pragma unroll
+                for (int a2 = 0; a2 < SDFT_FAST_S; ++a2) {
+                    if (a2 >= S) break;
+                    const int col2 = a2 * S + a;
+                    T w[4];
+                    Vec4<T>::load(WT + col2 * NPQ + n0, w);
+                    T* xn = Xn + col2 * XLP;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int n = n0 + v;
+                        if (n < NP) {
+                            const T x = w[v] * qs[v];
+                            xn[chain_phys(n + Be)] = x;
+                            if (n < Be) xn[chain_phys(Be - 1 - n)] = x;
+                            const int mt = 2 * NP - 1 - n + Be;
+                            if (mt < XL) xn[chain_phys(mt)] = x;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const double qv = (double)qs[v];
+                    wmax = qv > wmax ? qv : wmax;   // qs is zero past NP
+                }
+            }
+        }
+        if (ca.rescale && !last) {
+            for (int o = 32; o > 0; o >>= 1) {
+                const double w = __shfl_xor(wmax, o, 64);
+                wmax = w > wmax ? w : wmax;
+            }
+            if (lane == 0) redm[s & 1][wave] = wmax;
+        }
+        // the next step's barrier orders these writes before their reads
+    }
+}
+
 size_t chain_lds_bytes(const ChainPlanHost& ch, size_t elem, bool hv_table) {
     const int S = ch.S, C = S * S;
     const int KC = 2 * ch.be_max + 32;
@@ -324,8 +562,18 @@ static hipError_t chain_set_lds(size_t bytes) {
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+template <typename T>
+static hipError_t chain_fused_set_lds(size_t bytes) {
+    if (bytes <= 65536) return hipSuccess;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chain_fused<T>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
 hipError_t chain_prepare(const ChainPlanHost& ch) {
     hipError_t e = hipSuccess;
+    if (ch.fused32 && (e = chain_fused_set_lds<float>(chain_fused_lds_bytes(ch, sizeof(float)))) != hipSuccess) return e;
+    if (ch.fused64 && (e = chain_fused_set_lds<double>(chain_fused_lds_bytes(ch, sizeof(double)))) != hipSuccess)
+        return e;
     if (ch.ok32) {
         const size_t b = chain_lds_bytes(ch, sizeof(float), ch.hv32);
         if ((e = chain_set_lds<float, 0>(b)) != hipSuccess) return e;
@@ -358,6 +606,11 @@ void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, 
     ca.npad_max = ch.npad_max; ca.be_max = ch.be_max;
     ca.rescale = sh.rescale; ca.nsp = c.nsp; ca.nmax = c.nmax_seg; ca.smax = smax;
     ca.info = ch.d_info;
+    if (sizeof(T) == 4 ? ch.fused32 : ch.fused64) {
+        hipLaunchKernelGGL(k_chain_fused<T>, dim3(c.P, 2), dim3(CHAIN_THREADS), chain_fused_lds_bytes(ch, sizeof(T)), st,
+                           c.d_sys, c.d_pts, c.d_evalflag, prec, ca, c.d_band, kern, elam, hl, trapez, gC, gP);
+        return;
+    }
     const bool hvt = sizeof(T) == 4 ? ch.hv32 : ch.hv64;
     ca.hv = hvt ? 1 : 0;
     const size_t lds = chain_lds_bytes(ch, sizeof(T), hvt);

@@ -111,6 +111,8 @@ struct ChainPlanHost {
     ChainCol* d_info = nullptr;                    // [2 sweeps][N-1 steps][S*S] column table (device)
     bool hv32 = false, hv64 = false;               // homogeneous chain: the output factors
                                                    // exp(lng/2)/normative tabulated in LDS
+    bool fused32 = false, fused64 = false;         // homogeneous chain on k_chain_fused (one barrier
+                                                   // per step, double-buffered inputs)
 };
 
 // multi-dimensional grids (sdft_nd.hip): one convolution column of the spectral path
@@ -361,6 +363,7 @@ template <typename T>
 void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, const T* kern, const T* rhoT,
                         const T* dphi, T* nw, double* fx);
 size_t chain_lds_bytes(const ChainPlanHost& ch, size_t elem, bool hv_table);
+size_t chain_fused_lds_bytes(const ChainPlanHost& ch, size_t elem);
 hipError_t chain_prepare(const ChainPlanHost& ch);
 template <typename T>
 void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, const T* elam, const T* hl,
