@@ -719,6 +719,7 @@ static int build_chain_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const st
         if (2 * Be + 1 > NP) return SDFT_OK;
         ch.npad_max = std::max(ch.npad_max, (NP + 255) / 256 * 256);
         ch.be_max = std::max(ch.be_max, Be);
+        ch.xl_need = std::max(ch.xl_need, NP + 2 * Be);
     }
     ch.ok32 = chain_lds_bytes(ch, sizeof(float), false) <= 160 * 1024;
     ch.ok64 = chain_lds_bytes(ch, sizeof(double), false) <= 160 * 1024;
@@ -733,10 +734,17 @@ static int build_chain_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const st
     };
     ch.hv32 = hv_fits(sizeof(float), ch.ok32);
     ch.hv64 = hv_fits(sizeof(double), ch.ok64);
-    // homogeneous chains whose double-buffered inputs and weight tables fit: one barrier per step
-    // (FP32 keeps two workgroups per CU, so within 80 KB)
-    ch.fused32 = ch.ok32 && ch.homog && chain_fused_lds_bytes(ch, sizeof(float)) <= 80 * 1024;
-    ch.fused64 = ch.ok64 && ch.homog && chain_fused_lds_bytes(ch, sizeof(double)) <= 160 * 1024 - 256;
+    // homogeneous chains on the one-barrier kernel. FP64: with at most one (a, tile) unit per wave
+    // the tables sit in registers and two workgroups share a CU (within 80 KB of LDS each), else the
+    // tables sit in LDS and one workgroup runs per CU. FP32 keeps its tables in LDS with two
+    // workgroups per CU (measured at NP 801: 144 us against 148 us with register tables).
+    {
+        const int units = S * (ch.npad_max / 256);
+        const size_t half = 80 * 1024 - 256, full = 160 * 1024 - 256;
+        ch.fused32 = ch.ok32 && ch.homog && chain_fused_lds_bytes(ch, sizeof(float), false) <= half;
+        ch.fused_reg64 = ch.ok64 && ch.homog && units <= 8 && chain_fused_lds_bytes(ch, sizeof(double), true) <= half;
+        ch.fused64 = ch.fused_reg64 || (ch.ok64 && ch.homog && chain_fused_lds_bytes(ch, sizeof(double), false) <= full);
+    }
     ch.ok = ch.ok32 || ch.ok64;
     ch.elig32 = ch.ok32;
     ch.elig64 = ch.ok64;

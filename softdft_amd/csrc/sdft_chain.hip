@@ -46,6 +46,7 @@ struct ChainArgs {
     int64_t goff;
     int32_t ck[2][SDFT_FAST_S * SDFT_FAST_S];  // canonical bond kernel of (sweep, a * S + b)
     int32_t npad_max, be_max;                // LDS carve-out (largest point of the launch)
+    int32_t xl_need, pad3;                   // largest NP + 2 Be
     int32_t rescale, nsp, nmax, pad;         // per-step rescaling (scale_slot tables)
     unsigned long long* smax;
     const ChainCol* info;                    // [2][N-1][S*S] column table of every step (host-built)
@@ -318,17 +319,31 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
 // rounding), and the maximum is reduced per wave in the epilogue.
 // The MFMA row r of a tile holds output phase phi(r): for FP64 (D[4 v + l/16][l % 16]) phi permutes
 // the rows so that the four accumulators of a lane are four consecutive outputs, as they are for
-// FP32 (D[4 (l/16) + v][l % 16]); the epilogue then reads its factors and stores its propagators
-// as 4-element vectors.
+// FP32 (D[4 (l/16) + v][l % 16]); the epilogue then stores its propagators as 4-element vectors.
+// X holds only the extended positions that valid outputs read (m < NP + 2 Be) and 16 zeros after
+// them: a tile column reads 2 Be + 16 positions for all 16 rows, where the operator window is zero,
+// so those positions must be finite (0 x NaN would reach a valid output). Reads past that feed only
+// tile columns whose outputs all lie beyond NP, which the epilogue discards, and stay inside the
+// carve-out.
+// REG: every wave owns at most one (a, tile) unit and keeps that unit's weights W and output factors
+// in registers; the LDS then holds only the operator windows and the two X buffers, which lets two
+// workgroups share a CU (the FP64 variant included) so that one's epilogue and barrier overlap the
+// other's matrix products. Otherwise W and the output factors are LDS tables.
 // --------------------------------------------------------------------------
 __host__ __device__ __forceinline__ int round4(int n) { return (n + 3) & ~3; }
+__host__ __device__ __forceinline__ int chain_fused_xlp(int xl_need) {
+    return round4(xl_need + 16 + ((xl_need + 16) >> 4) + 1);
+}
 
-size_t chain_fused_lds_bytes(const ChainPlanHost& ch, size_t elem) {
+size_t chain_fused_lds_bytes(const ChainPlanHost& ch, size_t elem, bool reg) {
     const int S = ch.S, C = S * S;
     const int KC = 2 * ch.be_max + 32;
-    const int XLm = ch.npad_max + 2 * ch.be_max;
-    const int XLP = XLm + (XLm >> 4) + 1;
-    return (size_t)(round4(C * KC) + round4(2 * C * XLP) + 2 * C * ch.npad_max + CHAIN_LDS_SLACK) * elem;
+    const int tables = reg ? 0 : 2 * C * ch.npad_max;
+    // the MFMA tiles read up to npad_max + 2 be_max + 16 extended positions of the last column
+    const int full = ch.npad_max + 2 * ch.be_max + 16;
+    const int over = full + (full >> 4) + 1 - chain_fused_xlp(ch.xl_need);
+    return (size_t)(round4(C * KC) + 2 * C * chain_fused_xlp(ch.xl_need) + tables + (over > 0 ? over : 0) +
+                    CHAIN_LDS_SLACK) * elem;
 }
 
 template <typename T> struct Vec4;
@@ -352,8 +367,8 @@ template <> struct Vec4<double> {
     }
 };
 
-template <typename T>
-__global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain_fused(
+template <typename T, bool REG>
+__global__ void __launch_bounds__(CHAIN_THREADS, (REG || sizeof(T) == 4) ? 4 : 1) k_chain_fused(
         const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts, const int32_t* __restrict__ evalflag,
         int prec, ChainArgs ca, const int32_t* __restrict__ band, const T* __restrict__ kern,
         const T* __restrict__ elam, const T* __restrict__ hl, const T* __restrict__ trapez, T* __restrict__ gC,
@@ -366,18 +381,17 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
     const int Be = chain_half_width(band[P.kset]);
     const int KS = Be / 2 + 4;                 // k-steps: K / 4 with K = 2 Be + 16
     const int ntile = (NP + 255) >> 8;
-    const int XL = ntile * 256 + 2 * Be;       // extended input length of this point
+    const int XN = NP + 2 * Be;                // extended positions the valid outputs read
     const int S = ca.S, C = S * S, N = ca.nseg;
     const int KC = 2 * ca.be_max + 32;
-    const int XLm = ca.npad_max + 2 * ca.be_max;
-    const int XLP = XLm + (XLm >> 4) + 1;
+    const int XLR = chain_fused_xlp(ca.xl_need);
     const int NPQ = ca.npad_max;
 
     extern __shared__ __attribute__((aligned(16))) unsigned char chain_fused_lds[];
     T* cw = reinterpret_cast<T*>(chain_fused_lds);  // [C][KC] operator windows (reversed)
-    T* XB = cw + round4(C * KC);                    // [2][C][XLP] extended inputs, by step parity
-    T* WT = XB + round4(2 * C * XLP);               // [C][NPQ] input weights W
-    T* HV = WT + C * NPQ;                           // [C][NPQ] exp(lng/2) of the output end / normative
+    T* XB = cw + round4(C * KC);                    // [2][C][XLR] extended inputs, by step parity
+    T* WT = XB + 2 * C * XLR;                       // !REG: [C][NPQ] input weights W
+    T* HV = WT + C * NPQ;                           // !REG: [C][NPQ] exp(lng/2) of the output end / normative
     __shared__ ChainCol cinfo[SDFT_FAST_S * SDFT_FAST_S];
     __shared__ double redm[2][CHAIN_WAVES];
 
@@ -403,33 +417,55 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
     __syncthreads();
     const int64_t eb = P.base * (int64_t)ca.NB;
     const int64_t hb = P.base * (int64_t)(2 * ca.MB);
-    for (int e = tid; e < C * NP; e += CHAIN_THREADS) {
-        const int col = e / NP, n = e - col * NP;
-        const ChainCol cc = cinfo[col];
-        T w = T(0), h = T(0);
-        if (cc.valid) {
-            w = elam[eb + (int64_t)cc.sj * NP + n] * hl[hb + (int64_t)cc.hl_in * NP + n] *
-                trapez[eb + (int64_t)cc.sj * NP + n];
-            h = hl[hb + (int64_t)cc.hl_out * NP + n] / (T)norm;
+    auto weight = [&](const ChainCol& cc, int n) -> T {  // W of column cc at point n
+        return cc.valid ? elam[eb + (int64_t)cc.sj * NP + n] * hl[hb + (int64_t)cc.hl_in * NP + n] *
+                              trapez[eb + (int64_t)cc.sj * NP + n]
+                        : T(0);
+    };
+    auto outfac = [&](const ChainCol& cc, int n) -> T {
+        return cc.valid ? hl[hb + (int64_t)cc.hl_out * NP + n] / (T)norm : T(0);
+    };
+    if (!REG) {
+        for (int e = tid; e < C * NP; e += CHAIN_THREADS) {
+            const int col = e / NP, n = e - col * NP;
+            const ChainCol cc = cinfo[col];
+            WT[col * NPQ + n] = weight(cc, n);
+            HV[col * NPQ + n] = outfac(cc, n);
         }
-        WT[col * NPQ + n] = w;
-        HV[col * NPQ + n] = h;
+        __syncthreads();
     }
-    __syncthreads();
-    // inputs of step 1 (Q = 1 at the free chain end) into buffer 1; positions past the mirror stay
-    // zero in both buffers (the epilogues write every other position)
-    for (int e = tid; e < C * XL; e += CHAIN_THREADS) {
-        const int col = e / XL, mi = e - col * XL;
-        const int m = mi - Be;
-        const int n = m < 0 ? -1 - m : (m < NP ? m : 2 * NP - 1 - m);
-        XB[(C + col) * XLP + chain_phys(mi)] = n >= 0 ? WT[col * NPQ + n] : T(0);
-        if (n < 0) XB[col * XLP + chain_phys(mi)] = T(0);
+    // inputs of step 1 (Q = 1 at the free chain end) into buffer 1 (the epilogues write the others),
+    // and the 16 zeros past XN in both buffers
+    for (int e = tid; e < C * (XN + 16); e += CHAIN_THREADS) {
+        const int col = e / (XN + 16), mi = e - col * (XN + 16);
+        if (mi < XN) {
+            const int m = mi - Be;
+            const int n = m < 0 ? -1 - m : (m < NP ? m : 2 * NP - 1 - m);
+            XB[(C + col) * XLR + chain_phys(mi)] = REG ? weight(cinfo[col], n) : WT[col * NPQ + n];
+        } else {
+            XB[col * XLR + chain_phys(mi)] = T(0);
+            XB[(C + col) * XLR + chain_phys(mi)] = T(0);
+        }
     }
 
     T* gout = sw == 0 ? gC : gP;
     const int64_t gb = P.gbase + ca.goff * P.gstride;
     const int q = lane & 15, g = lane >> 4;
     const int phq = sizeof(T) == 8 ? 4 * (q & 3) + (q >> 2) : q;   // output phase of MFMA row q
+    // REG: the unit's output factors hr[b][v] and next-step weights wr[a2][v], v = the lane's outputs
+    T hr[SDFT_FAST_S][4], wr[SDFT_FAST_S][4];
+    if (REG && wave < S * ntile) {
+        const int a = wave / ntile, tile = wave - a * ntile;
+        const int n0 = 16 * (16 * tile + q) + 4 * g;
+#pragma unroll
+        for (int b = 0; b < SDFT_FAST_S; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const bool in = b < S && n0 + v < NP;
+                hr[b][v] = in ? outfac(cinfo[(a * S + b) % C], n0 + v) : T(0);
+                wr[b][v] = in ? weight(cinfo[(b * S + a) % C], n0 + v) : T(0);
+            }
+    }
     for (int s = 1; s < N; ++s) {
         const int outseg = sw == 0 ? N - 1 - s : s;
         __syncthreads();  // inputs of step s written; the wave maxima of step s - 1 complete
@@ -445,8 +481,8 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
             m = (m > 0.0 && m <= 1.7976931348623157e308) ? m : 1.0;  // scale_read's rule
             scale = (T)(1.0 / m);
         }
-        const T* Xc = XB + (s & 1) * C * XLP;
-        T* Xn = XB + ((s + 1) & 1) * C * XLP;
+        const T* Xc = XB + (s & 1) * C * XLR;
+        T* Xn = XB + ((s + 1) & 1) * C * XLR;
         const bool last = s == N - 1;
         double wmax = 0.0;
         for (int unit = wave; unit < S * ntile; unit += CHAIN_WAVES) {
@@ -459,8 +495,10 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
                 if (b >= S) break;
                 const int col = a * S + b;
                 if (!cinfo[col].valid) continue;
+                // A[q][t] = c[phi(q) - t + Be] (reversed window); B[g][q] = x_ext[16 (p0 + q) + t - Be],
+                // t = 4 ks + g, ks = 4 K + k4: X[phys(16 (p0 + q) + g + 4 ks)] = X[17 (p0 + q) + g + 17 K + 4 k4]
                 const T* ap = cw + col * KC + (KC - 16 - 2 * Be) - phq + g;
-                const T* xp = Xc + col * XLP + 17 * (p0 + q) + g;
+                const T* xp = Xc + col * XLR + 17 * (p0 + q) + g;
                 typename Mfma<T>::acc_t acc;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) acc[v] = T(0);
@@ -495,7 +533,12 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
                     acc = Mfma<T>::mma(ap[4 * ks], xp[17 * (ks >> 2) + 4 * (ks & 3)], acc);
                 if (n0 < NP) {
                     T hv[4], val[4];
-                    Vec4<T>::load(HV + col * NPQ + n0, hv);
+                    if (REG) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) hv[v] = hr[b][v];
+                    } else {
+                        Vec4<T>::load(HV + col * NPQ + n0, hv);
+                    }
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const T y = acc[v];
@@ -514,8 +557,13 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
                     if (a2 >= S) break;
                     const int col2 = a2 * S + a;
                     T w[4];
-                    Vec4<T>::load(WT + col2 * NPQ + n0, w);
-                    T* xn = Xn + col2 * XLP;
+                    if (REG) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) w[v] = wr[a2][v];
+                    } else {
+                        Vec4<T>::load(WT + col2 * NPQ + n0, w);
+                    }
+                    T* xn = Xn + col2 * XLR;
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int n = n0 + v;
@@ -524,7 +572,7 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
                             xn[chain_phys(n + Be)] = x;
                             if (n < Be) xn[chain_phys(Be - 1 - n)] = x;
                             const int mt = 2 * NP - 1 - n + Be;
-                            if (mt < XL) xn[chain_phys(mt)] = x;
+                            if (mt < XN) xn[chain_phys(mt)] = x;
                         }
                     }
                 }
@@ -562,18 +610,22 @@ static hipError_t chain_set_lds(size_t bytes) {
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <typename T>
+template <typename T, bool REG>
 static hipError_t chain_fused_set_lds(size_t bytes) {
     if (bytes <= 65536) return hipSuccess;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chain_fused<T>),
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chain_fused<T, REG>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 hipError_t chain_prepare(const ChainPlanHost& ch) {
     hipError_t e = hipSuccess;
-    if (ch.fused32 && (e = chain_fused_set_lds<float>(chain_fused_lds_bytes(ch, sizeof(float)))) != hipSuccess) return e;
-    if (ch.fused64 && (e = chain_fused_set_lds<double>(chain_fused_lds_bytes(ch, sizeof(double)))) != hipSuccess)
+    if (ch.fused32 && (e = chain_fused_set_lds<float, false>(chain_fused_lds_bytes(ch, sizeof(float), false))) != hipSuccess)
         return e;
+    if (ch.fused64) {
+        e = ch.fused_reg64 ? chain_fused_set_lds<double, true>(chain_fused_lds_bytes(ch, sizeof(double), true))
+                           : chain_fused_set_lds<double, false>(chain_fused_lds_bytes(ch, sizeof(double), false));
+        if (e != hipSuccess) return e;
+    }
     if (ch.ok32) {
         const size_t b = chain_lds_bytes(ch, sizeof(float), ch.hv32);
         if ((e = chain_set_lds<float, 0>(b)) != hipSuccess) return e;
@@ -603,12 +655,18 @@ void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, 
     ca.goff = sh.goff;
     for (int sw = 0; sw < 2; ++sw)
         for (int k = 0; k < SDFT_FAST_S * SDFT_FAST_S; ++k) ca.ck[sw][k] = ch.ck[sw][k];
-    ca.npad_max = ch.npad_max; ca.be_max = ch.be_max;
+    ca.npad_max = ch.npad_max; ca.be_max = ch.be_max; ca.xl_need = ch.xl_need;
     ca.rescale = sh.rescale; ca.nsp = c.nsp; ca.nmax = c.nmax_seg; ca.smax = smax;
     ca.info = ch.d_info;
     if (sizeof(T) == 4 ? ch.fused32 : ch.fused64) {
-        hipLaunchKernelGGL(k_chain_fused<T>, dim3(c.P, 2), dim3(CHAIN_THREADS), chain_fused_lds_bytes(ch, sizeof(T)), st,
-                           c.d_sys, c.d_pts, c.d_evalflag, prec, ca, c.d_band, kern, elam, hl, trapez, gC, gP);
+        const bool reg = sizeof(T) == 8 && ch.fused_reg64;
+        const size_t lds = chain_fused_lds_bytes(ch, sizeof(T), reg);
+        if (reg)
+            hipLaunchKernelGGL((k_chain_fused<T, sizeof(T) == 8>), dim3(c.P, 2), dim3(CHAIN_THREADS), lds, st, c.d_sys, c.d_pts,
+                               c.d_evalflag, prec, ca, c.d_band, kern, elam, hl, trapez, gC, gP);
+        else
+            hipLaunchKernelGGL((k_chain_fused<T, false>), dim3(c.P, 2), dim3(CHAIN_THREADS), lds, st, c.d_sys, c.d_pts,
+                               c.d_evalflag, prec, ca, c.d_band, kern, elam, hl, trapez, gC, gP);
         return;
     }
     const bool hvt = sizeof(T) == 4 ? ch.hv32 : ch.hv64;

@@ -108,11 +108,13 @@ struct ChainPlanHost {
     int S = 1;
     int ck[2][SDFT_FAST_S * SDFT_FAST_S];          // canonical bond kernel of (sweep, a * S + b), -1 absent
     int npad_max = 0, be_max = 0;                  // largest padded NP and even band half-width
+    int xl_need = 0;                               // largest NP + 2 Be (extended positions read)
     ChainCol* d_info = nullptr;                    // [2 sweeps][N-1 steps][S*S] column table (device)
     bool hv32 = false, hv64 = false;               // homogeneous chain: the output factors
                                                    // exp(lng/2)/normative tabulated in LDS
     bool fused32 = false, fused64 = false;         // homogeneous chain on k_chain_fused (one barrier
                                                    // per step, double-buffered inputs)
+    bool fused_reg64 = false;                      // ... FP64 with one unit per wave, tables in registers
 };
 
 // multi-dimensional grids (sdft_nd.hip): one convolution column of the spectral path
@@ -363,7 +365,7 @@ template <typename T>
 void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, const T* kern, const T* rhoT,
                         const T* dphi, T* nw, double* fx);
 size_t chain_lds_bytes(const ChainPlanHost& ch, size_t elem, bool hv_table);
-size_t chain_fused_lds_bytes(const ChainPlanHost& ch, size_t elem);
+size_t chain_fused_lds_bytes(const ChainPlanHost& ch, size_t elem, bool reg);
 hipError_t chain_prepare(const ChainPlanHost& ch);
 template <typename T>
 void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, const T* elam, const T* hl,
