@@ -744,6 +744,26 @@ static int build_chain_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const st
         ch.fused32 = ch.ok32 && ch.homog && chain_fused_lds_bytes(ch, sizeof(float), false) <= half;
         ch.fused_reg64 = ch.ok64 && ch.homog && units <= 8 && chain_fused_lds_bytes(ch, sizeof(double), true) <= half;
         ch.fused64 = ch.fused_reg64 || (ch.ok64 && ch.homog && chain_fused_lds_bytes(ch, sizeof(double), false) <= full);
+        // both sweeps of a point in one workgroup when more points are active than half the CUs
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 2)
+            ncu = 256;
+        ch.pp_min = ncu / 2 + 1;
+        const bool pp_ok = ch.homog && units <= 8;
+        // one bond operator for every valid column: k_chain_pp runs both input states of a unit as
+        // two accumulation chains over one operator window
+        ch.shared_a = S == 2;
+        int k0 = -1;
+        for (int sw = 0; sw < 2; ++sw)
+            for (int col = 0; col < S * S; ++col) {
+                const int k = ch.ck[sw][col];
+                if (k < 0) continue;
+                if (k0 < 0) k0 = k;
+                ch.shared_a = ch.shared_a && k == k0;
+            }
+        ch.pp32 = pp_ok && ch.ok32 && chain_pp_lds_bytes(ch, sizeof(float)) <= full;
+        ch.pp64 = pp_ok && ch.ok64 && chain_pp_lds_bytes(ch, sizeof(double)) <= full;
     }
     ch.ok = ch.ok32 || ch.ok64;
     ch.elig32 = ch.ok32;

@@ -42,7 +42,7 @@ struct ChainArgs {
     int32_t u, nseg, S, state_off;
     int32_t GF, MB, NB, homog;     // homog: every segment has the same state beads
     int32_t hv;                    // output factors exp(lng/2)/normative tabulated in LDS (homog only)
-    int32_t pad2;
+    int32_t shared_a;              // every (a, b) column of both sweeps applies the same bond operator
     int64_t goff;
     int32_t ck[2][SDFT_FAST_S * SDFT_FAST_S];  // canonical bond kernel of (sweep, a * S + b)
     int32_t npad_max, be_max;                // LDS carve-out (largest point of the launch)
@@ -393,9 +393,13 @@ __global__ void __launch_bounds__(CHAIN_THREADS, (REG || sizeof(T) == 4) ? 4 : 1
     T* WT = XB + 2 * C * XLR;                       // !REG: [C][NPQ] input weights W
     T* HV = WT + C * NPQ;                           // !REG: [C][NPQ] exp(lng/2) of the output end / normative
     __shared__ ChainCol cinfo[SDFT_FAST_S * SDFT_FAST_S];
-    __shared__ double redm[2][CHAIN_WAVES];
+    // largest Q of step s in slot s mod 3 (bit patterns of non-negative doubles order as integers):
+    // written by LDS atomics in the epilogues of step s, read after the barrier of step s + 1,
+    // cleared during step s - 1, after the reads of step s - 3's maximum
+    __shared__ unsigned long long qmax[3];
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: scalar branches
     const double norm = P.normative[ca.u];
     {
         const T* cset = kern + (int64_t)P.kbase * sys.K;
@@ -414,7 +418,11 @@ __global__ void __launch_bounds__(CHAIN_THREADS, (REG || sizeof(T) == 4) ? 4 : 1
     }
     // the columns of the first step hold for every step of a homogeneous chain
     if (tid < C) cinfo[tid] = ca.info[(int64_t)sw * (N - 1) * C + tid];
+    if (tid < 3) qmax[tid] = 0ull;
     __syncthreads();
+    int validmask = 0;   // bit col: column col carries a bond (the same at every step)
+    for (int col = 0; col < C; ++col) validmask |= cinfo[col].valid ? 1 << col : 0;
+    validmask = __builtin_amdgcn_readfirstlane(validmask);
     const int64_t eb = P.base * (int64_t)ca.NB;
     const int64_t hb = P.base * (int64_t)(2 * ca.MB);
     auto weight = [&](const ChainCol& cc, int n) -> T {  // W of column cc at point n
@@ -449,7 +457,8 @@ __global__ void __launch_bounds__(CHAIN_THREADS, (REG || sizeof(T) == 4) ? 4 : 1
     }
 
     T* gout = sw == 0 ? gC : gP;
-    const int64_t gb = P.gbase + ca.goff * P.gstride;
+    const int64_t gstride = P.gstride;
+    const int64_t gb = P.gbase + ca.goff * gstride;
     const int q = lane & 15, g = lane >> 4;
     const int phq = sizeof(T) == 8 ? 4 * (q & 3) + (q >> 2) : q;   // output phase of MFMA row q
     // REG: the unit's output factors hr[b][v] and next-step weights wr[a2][v], v = the lane's outputs
@@ -468,12 +477,11 @@ __global__ void __launch_bounds__(CHAIN_THREADS, (REG || sizeof(T) == 4) ? 4 : 1
     }
     for (int s = 1; s < N; ++s) {
         const int outseg = sw == 0 ? N - 1 - s : s;
-        __syncthreads();  // inputs of step s written; the wave maxima of step s - 1 complete
+        __syncthreads();  // inputs of step s written; the maximum of step s - 1 complete
+        if (tid == 0) qmax[(s + 1) % 3] = 0ull;
         T scale = T(1);
         if (ca.rescale && s > 1) {
-            double m = 0.0;
-#pragma unroll
-            for (int w = 0; w < CHAIN_WAVES; ++w) m = redm[(s - 1) & 1][w] > m ? redm[(s - 1) & 1][w] : m;
+            double m = __longlong_as_double((long long)qmax[(s - 1) % 3]);
             if (tid == 0) {
                 const int64_t slot = scale_slot(p, ca.u, sw, s - 1, ca.nsp, ca.nmax);
                 ca.smax[slot] = (unsigned long long)__double_as_longlong(m);
@@ -484,7 +492,6 @@ __global__ void __launch_bounds__(CHAIN_THREADS, (REG || sizeof(T) == 4) ? 4 : 1
         const T* Xc = XB + (s & 1) * C * XLR;
         T* Xn = XB + ((s + 1) & 1) * C * XLR;
         const bool last = s == N - 1;
-        double wmax = 0.0;
         for (int unit = wave; unit < S * ntile; unit += CHAIN_WAVES) {
             const int a = unit / ntile, tile = unit - a * ntile;
             const int p0 = tile * 16;
@@ -494,7 +501,7 @@ __global__ void __launch_bounds__(CHAIN_THREADS, (REG || sizeof(T) == 4) ? 4 : 1
             for (int b = 0; b < SDFT_FAST_S; ++b) {
                 if (b >= S) break;
                 const int col = a * S + b;
-                if (!cinfo[col].valid) continue;
+                if (!((validmask >> col) & 1)) continue;
                 // A[q][t] = c[phi(q) - t + Be] (reversed window); B[g][q] = x_ext[16 (p0 + q) + t - Be],
                 // t = 4 ks + g, ks = 4 K + k4: X[phys(16 (p0 + q) + g + 4 ks)] = X[17 (p0 + q) + g + 17 K + 4 k4]
                 const T* ap = cw + col * KC + (KC - 16 - 2 * Be) - phq + g;
@@ -547,7 +554,7 @@ __global__ void __launch_bounds__(CHAIN_THREADS, (REG || sizeof(T) == 4) ? 4 : 1
                         qs[v] += val[v];
                     }
                     // the padding of the propagator rows past NP receives zeros
-                    Vec4<T>::store(gout + gb + (((int64_t)outseg * S + b) * S + a) * P.gstride + n0, val);
+                    Vec4<T>::store(gout + gb + (((int64_t)outseg * S + b) * S + a) * gstride + n0, val);
                 }
             }
             if (!last && n0 < NP) {
@@ -576,19 +583,16 @@ __global__ void __launch_bounds__(CHAIN_THREADS, (REG || sizeof(T) == 4) ? 4 : 1
                         }
                     }
                 }
+                if (ca.rescale) {
+                    double mx = 0.0;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const double qv = (double)qs[v];
-                    wmax = qv > wmax ? qv : wmax;   // qs is zero past NP
+                    for (int v = 0; v < 4; ++v) {
+                        const double qv = (double)qs[v];
+                        mx = qv > mx ? qv : mx;   // qs is zero past NP
+                    }
+                    atomicMax(&qmax[s % 3], (unsigned long long)__double_as_longlong(mx));
                 }
             }
-        }
-        if (ca.rescale && !last) {
-            for (int o = 32; o > 0; o >>= 1) {
-                const double w = __shfl_xor(wmax, o, 64);
-                wmax = w > wmax ? w : wmax;
-            }
-            if (lane == 0) redm[s & 1][wave] = wmax;
         }
         // the next step's barrier orders these writes before their reads
     }
@@ -610,6 +614,332 @@ static hipError_t chain_set_lds(size_t bytes) {
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
+// --------------------------------------------------------------------------
+// Both sweeps of a point in one 16-wave workgroup, offset by half a step (ping-pong): waves 0-7 run
+// the child sweep and waves 8-15 the parent sweep, and in every phase between two barriers one half
+// runs the matrix products of its step while the other half finishes its previous step (output
+// factors, propagator stores, next inputs, maximum). The matrix cores then always have a half with
+// products to issue, and each half needs only one X buffer: a half writes its next inputs in the
+// phase after its products have read the current ones. Phase ph: half h runs the products of step
+// (ph - h) / 2 + 1 when ph - h is even and the finish of step (ph - h + 1) / 2 when it is odd.
+// The two halves share a CU, so a point's chain takes 2 N - 1 phases instead of N - 1 steps on two
+// CUs: at low load (at most half as many active points as CUs) each workgroup instead runs one
+// sweep on waves 0-7 with a double-buffered X (the one-barrier schedule above), and the second
+// workgroup of the point runs the other sweep. Every wave runs the same barrier sequence.
+// One (a, tile) unit per wave, its weights and output factors in registers.
+// --------------------------------------------------------------------------
+size_t chain_pp_lds_bytes(const ChainPlanHost& ch, size_t elem) {
+    const int S = ch.S, C = S * S;
+    const int KC = 2 * ch.be_max + 32;
+    const int full = ch.npad_max + 2 * ch.be_max + 16;
+    const int over = full + (full >> 4) + 1 - chain_fused_xlp(ch.xl_need);
+    return (size_t)(2 * round4(C * KC) + 2 * C * chain_fused_xlp(ch.xl_need) + (over > 0 ? over : 0) +
+                    CHAIN_LDS_SLACK) * elem;
+}
+
+constexpr int PP_THREADS = 1024;
+
+template <typename T, bool SHA>
+__global__ void __launch_bounds__(PP_THREADS) k_chain_pp(
+        const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts, const int32_t* __restrict__ evalflag,
+        int prec, int npts, int pp_min, ChainArgs ca, const int32_t* __restrict__ band, const T* __restrict__ kern,
+        const T* __restrict__ elam, const T* __restrict__ hl, const T* __restrict__ trapez, T* __restrict__ gC,
+        T* __restrict__ gP) {
+    const int p = blockIdx.x, wg = blockIdx.y;
+    if (evalflag[p] != prec) return;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: scalar branches
+    const int half = wave >> 3, hw = wave & 7;
+    // regime: the number of points evaluated at this precision
+    __shared__ int nact_s;
+    if (tid == 0) nact_s = 0;
+    __syncthreads();
+    {
+        int c = 0;
+        for (int i = tid; i < npts; i += PP_THREADS) c += evalflag[i] == prec ? 1 : 0;
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0 && c) atomicAdd(&nact_s, c);
+    }
+    __syncthreads();
+    const bool pp = nact_s >= pp_min;
+    if (pp && wg == 1) return;                       // the first workgroup runs both sweeps
+    const DevSystem& sys = *psys;
+    const DevPoint& P = pts[p];
+    const int NP = P.NP, L = P.L;
+    const int Be = chain_half_width(band[P.kset]);
+    const int KS = Be / 2 + 4;
+    const int ntile = (NP + 255) >> 8;
+    const int XN = NP + 2 * Be;
+    const int S = ca.S, C = S * S, N = ca.nseg;
+    const int KC = 2 * ca.be_max + 32;
+    const int XLR = chain_fused_xlp(ca.xl_need);
+    const int nslot = pp ? 2 : 1;                    // sweeps in this workgroup
+    const bool active = half < nslot;                // this wave's half has a sweep
+    const int sw = pp ? half : wg;                   // its sweep
+
+    extern __shared__ __attribute__((aligned(16))) unsigned char chain_pp_lds[];
+    T* cwb = reinterpret_cast<T*>(chain_pp_lds);    // [2][C][KC] operator windows of each half
+    T* XB = cwb + 2 * round4(C * KC);               // [2][C][XLR]: one X per half (pp), or the two
+                                                    // parities of the single sweep
+    __shared__ ChainCol cinfo[2][SDFT_FAST_S * SDFT_FAST_S];
+    __shared__ unsigned long long qmax[2][3];       // largest Q of step s in [half][s mod 3]
+
+    const double norm = P.normative[ca.u];
+    {
+        const T* cset = kern + (int64_t)P.kbase * sys.K;
+        for (int e = tid; e < nslot * C * KC; e += PP_THREADS) {
+            const int j = e / (C * KC), e2 = e - j * C * KC;
+            const int col = e2 / KC, ri = e2 - col * KC;
+            const int k = ca.ck[pp ? j : wg][col];
+            const int jj0 = (KC - 1 - ri) - 15 - Be;
+            T v = T(0);
+            if (k >= 0 && jj0 >= -Be && jj0 <= Be) {
+                int jj = jj0 % L;
+                if (jj < 0) jj += L;
+                v = cset[(int64_t)k * L + jj];
+            }
+            cwb[j * round4(C * KC) + e2] = v;
+        }
+    }
+    if (tid < nslot * C) {
+        const int j = tid / C, col = tid - j * C;
+        cinfo[j][col] = ca.info[(int64_t)(pp ? j : wg) * (N - 1) * C + col];
+    }
+    if (tid < 6) qmax[tid / 3][tid % 3] = 0ull;
+    __syncthreads();
+    const int64_t eb = P.base * (int64_t)ca.NB;
+    const int64_t hb = P.base * (int64_t)(2 * ca.MB);
+    auto weight = [&](const ChainCol& cc, int n) -> T {
+        return cc.valid ? elam[eb + (int64_t)cc.sj * NP + n] * hl[hb + (int64_t)cc.hl_in * NP + n] *
+                              trapez[eb + (int64_t)cc.sj * NP + n]
+                        : T(0);
+    };
+    // step-1 inputs: X of half j (pp) or the parity-1 buffer (one sweep), and the 16 zeros past XN in
+    // both buffers
+    for (int e = tid; e < 2 * C * (XN + 16); e += PP_THREADS) {
+        const int j = e / (C * (XN + 16)), e2 = e - j * C * (XN + 16);
+        const int col = e2 / (XN + 16), mi = e2 - col * (XN + 16);
+        T* xb = XB + (j * C + col) * XLR;
+        if (mi >= XN) {
+            xb[chain_phys(mi)] = T(0);
+        } else if (pp || j == 1) {
+            const int m = mi - Be;
+            const int n = m < 0 ? -1 - m : (m < NP ? m : 2 * NP - 1 - m);
+            xb[chain_phys(mi)] = weight(cinfo[pp ? j : 0][col], n);
+        }
+    }
+    const int slot = pp ? half : 0;
+    const ChainCol* ci = cinfo[slot];
+    int validmask = 0;
+    for (int col = 0; col < C; ++col) validmask |= ci[col].valid ? 1 << col : 0;
+    validmask = __builtin_amdgcn_readfirstlane(validmask);
+    const T* cw = cwb + slot * round4(C * KC);
+    unsigned long long* qm = qmax[slot];
+    T* gout = sw == 0 ? gC : gP;
+    const int64_t gstride = P.gstride;
+    const int64_t gb = P.gbase + ca.goff * gstride;
+    const int q = lane & 15, g = lane >> 4;
+    const int phq = sizeof(T) == 8 ? 4 * (q & 3) + (q >> 2) : q;
+    const bool has_unit = active && hw < S * ntile;
+    const int a = has_unit ? hw / ntile : 0, tile = has_unit ? hw - a * ntile : 0;
+    const int p0 = tile * 16;
+    const int n0 = 16 * (p0 + q) + 4 * g;
+    T hr[SDFT_FAST_S][4], wr[SDFT_FAST_S][4];
+#pragma unroll
+    for (int b = 0; b < SDFT_FAST_S; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const bool in = has_unit && b < S && n0 + v < NP;
+            hr[b][v] = in && ci[(a * S + b) % C].valid ? hl[hb + (int64_t)ci[(a * S + b) % C].hl_out * NP + n0 + v] / (T)norm
+                          : T(0);
+            wr[b][v] = in ? weight(ci[(b * S + a) % C], n0 + v) : T(0);
+        }
+    typename Mfma<T>::acc_t acc[SDFT_FAST_S];
+    const int nph = pp ? 2 * N - 1 : N - 1;
+    for (int ph = 0; ph < nph; ++ph) {
+        __syncthreads();
+        int sm = 0, se = 0;   // step whose products / finish this half runs in this phase (0: none)
+        if (active) {
+            if (pp) {
+                const int r = ph - half;
+                if (r >= 0) {
+                    if ((r & 1) == 0) sm = r / 2 + 1 <= N - 1 ? r / 2 + 1 : 0;
+                    else se = (r + 1) / 2 <= N - 1 ? (r + 1) / 2 : 0;
+                }
+            } else {
+                sm = se = ph + 1;
+            }
+        }
+        if (sm) {
+            if (hw == 0 && lane == 0) qm[(sm + 1) % 3] = 0ull;
+            const T* Xc = XB + (pp ? half : (sm & 1)) * C * XLR;
+            if (SHA && has_unit) {
+                // one operator window for both input states b: two independent accumulation chains
+                // share its reads (k-steps taken two at a time, the next pair prefetched)
+#pragma unroll
+                for (int b = 0; b < SDFT_FAST_S; ++b)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[b][v] = T(0);
+                const T* ap = cw + a * S * KC + (KC - 16 - 2 * Be) - phq + g;
+                const T* xp0 = Xc + (a * S) * XLR + 17 * (p0 + q) + g;
+                const T* xp1 = xp0 + XLR;
+                const bool v0 = (validmask >> (a * S)) & 1, v1 = (validmask >> (a * S + 1)) & 1;
+                T a0[2], b0[2], c0[2], a1[2], b1[2], c1[2];
+                // k-step ks = 2 H + k: A at ap[4 ks], B at xp[17 (ks / 4) + 4 (ks % 4)]
+                auto fetch2 = [&](int H, T (&av)[2], T (&bv)[2], T (&cv)[2]) {
+                    const int xo = 17 * (H >> 1) + 8 * (H & 1);
+#pragma unroll
+                    for (int kk = 0; kk < 2; ++kk) {
+                        av[kk] = ap[8 * H + 4 * kk];
+                        bv[kk] = xp0[xo + 4 * kk];
+                        cv[kk] = xp1[xo + 4 * kk];
+                    }
+                };
+                auto mma2 = [&](const T (&av)[2], const T (&bv)[2], const T (&cv)[2]) {
+#pragma unroll
+                    for (int kk = 0; kk < 2; ++kk) {
+                        acc[0] = Mfma<T>::mma(av[kk], bv[kk], acc[0]);
+                        acc[1] = Mfma<T>::mma(av[kk], cv[kk], acc[1]);
+                    }
+                };
+                const int NH = KS >> 1;
+                fetch2(0, a0, b0, c0);
+                int H = 0;
+                for (; H + 1 < NH; H += 2) {
+                    fetch2(H + 1, a1, b1, c1);
+                    __builtin_amdgcn_sched_barrier(0);
+                    mma2(a0, b0, c0);
+                    __builtin_amdgcn_sched_barrier(0);
+                    fetch2(H + 2, a0, b0, c0);  // at most one pair past the last: within the LDS carve-out, unused
+                    __builtin_amdgcn_sched_barrier(0);
+                    mma2(a1, b1, c1);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                if (H < NH) mma2(a0, b0, c0);
+                if (KS & 1) {
+                    const int ks = KS - 1;
+                    const T av = ap[4 * ks];
+                    const int xo = 17 * (ks >> 2) + 4 * (ks & 3);
+                    acc[0] = Mfma<T>::mma(av, xp0[xo], acc[0]);
+                    acc[1] = Mfma<T>::mma(av, xp1[xo], acc[1]);
+                }
+                // a column without a bond contributes nothing (its window and inputs are zero; keep
+                // the finish's rule of skipping it)
+                if (!v0)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[0][v] = T(0);
+                if (!v1)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[1][v] = T(0);
+            } else if (!SHA && has_unit) {
+#pragma unroll
+                for (int b = 0; b < SDFT_FAST_S; ++b) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[b][v] = T(0);
+                    if (b >= S) break;
+                    const int col = a * S + b;
+                    if (!((validmask >> col) & 1)) continue;
+                    const T* ap = cw + col * KC + (KC - 16 - 2 * Be) - phq + g;
+                    const T* xp = Xc + col * XLR + 17 * (p0 + q) + g;
+                    T a0[4], b0[4], a1[4], b1[4];
+                    auto fetch = [&](int K, T (&av)[4], T (&bv)[4]) {
+#pragma unroll
+                        for (int k4 = 0; k4 < 4; ++k4) {
+                            av[k4] = ap[16 * K + 4 * k4];
+                            bv[k4] = xp[17 * K + 4 * k4];
+                        }
+                    };
+                    const int KG = KS / 4;
+                    fetch(0, a0, b0);
+                    int K = 0;
+                    for (; K + 1 < KG; K += 2) {
+                        fetch(K + 1, a1, b1);
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int k4 = 0; k4 < 4; ++k4) acc[b] = Mfma<T>::mma(a0[k4], b0[k4], acc[b]);
+                        __builtin_amdgcn_sched_barrier(0);
+                        fetch(K + 2, a0, b0);  // at most one group past the last: within the LDS carve-out, unused
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int k4 = 0; k4 < 4; ++k4) acc[b] = Mfma<T>::mma(a1[k4], b1[k4], acc[b]);
+                        __builtin_amdgcn_sched_barrier(0);
+                    }
+                    if (K < KG) {
+#pragma unroll
+                        for (int k4 = 0; k4 < 4; ++k4) acc[b] = Mfma<T>::mma(a0[k4], b0[k4], acc[b]);
+                    }
+                    for (int ks = 4 * KG; ks < KS; ++ks)
+                        acc[b] = Mfma<T>::mma(ap[4 * ks], xp[17 * (ks >> 2) + 4 * (ks & 3)], acc[b]);
+                }
+            }
+        }
+        if (se) {
+            const int s = se;
+            __builtin_amdgcn_s_setprio(1);   // the finish is latency-bound: ahead of the other half's products
+            T scale = T(1);
+            if (ca.rescale && s > 1) {
+                double m = __longlong_as_double((long long)qm[(s - 1) % 3]);
+                if (hw == 0 && lane == 0) {
+                    const int64_t sl = scale_slot(p, ca.u, sw, s - 1, ca.nsp, ca.nmax);
+                    ca.smax[sl] = (unsigned long long)__double_as_longlong(m);
+                }
+                m = (m > 0.0 && m <= 1.7976931348623157e308) ? m : 1.0;  // scale_read's rule
+                scale = (T)(1.0 / m);
+            }
+            if (has_unit && n0 < NP) {
+                const int outseg = sw == 0 ? N - 1 - s : s;
+                T qs[4] = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+                for (int b = 0; b < SDFT_FAST_S; ++b) {
+                    if (b >= S) break;
+                    const int col = a * S + b;
+                    if (!((validmask >> col) & 1)) continue;
+                    T val[4];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const T y = acc[b][v];
+                        const T ay = y < T(0) ? -y : y;
+                        val[v] = n0 + v < NP ? ay * hr[b][v] * scale : T(0);
+                        qs[v] += val[v];
+                    }
+                    // the padding of the propagator rows past NP receives zeros
+                    Vec4<T>::store(gout + gb + (((int64_t)outseg * S + b) * S + a) * gstride + n0, val);
+                }
+                if (s < N - 1) {
+                    T* Xn = XB + (pp ? half : ((s + 1) & 1)) * C * XLR;
+#pragma unroll
+                    for (int a2 = 0; a2 < SDFT_FAST_S; ++a2) {
+                        if (a2 >= S) break;
+                        T* xn = Xn + (a2 * S + a) * XLR;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int n = n0 + v;
+                            if (n < NP) {
+                                const T x = wr[a2][v] * qs[v];
+                                xn[chain_phys(n + Be)] = x;
+                                if (n < Be) xn[chain_phys(Be - 1 - n)] = x;
+                                const int mt = 2 * NP - 1 - n + Be;
+                                if (mt < XN) xn[chain_phys(mt)] = x;
+                            }
+                        }
+                    }
+                    if (ca.rescale) {
+                        double mx = 0.0;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const double qv = (double)qs[v];
+                            mx = qv > mx ? qv : mx;   // qs is zero past NP
+                        }
+                        atomicMax(&qm[s % 3], (unsigned long long)__double_as_longlong(mx));
+                    }
+                }
+            }
+            __builtin_amdgcn_s_setprio(0);
+        }
+        // the next phase's barrier orders these writes before their reads
+    }
+}
+
 template <typename T, bool REG>
 static hipError_t chain_fused_set_lds(size_t bytes) {
     if (bytes <= 65536) return hipSuccess;
@@ -619,6 +949,19 @@ static hipError_t chain_fused_set_lds(size_t bytes) {
 
 hipError_t chain_prepare(const ChainPlanHost& ch) {
     hipError_t e = hipSuccess;
+    auto pp_attr = [&](const void* f, size_t bytes) {
+        return hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    };
+    if (ch.pp32 && (e = ch.shared_a ? pp_attr(reinterpret_cast<const void*>(&k_chain_pp<float, true>),
+                                              chain_pp_lds_bytes(ch, sizeof(float)))
+                                    : pp_attr(reinterpret_cast<const void*>(&k_chain_pp<float, false>),
+                                              chain_pp_lds_bytes(ch, sizeof(float)))) != hipSuccess)
+        return e;
+    if (ch.pp64 && (e = ch.shared_a ? pp_attr(reinterpret_cast<const void*>(&k_chain_pp<double, true>),
+                                              chain_pp_lds_bytes(ch, sizeof(double)))
+                                    : pp_attr(reinterpret_cast<const void*>(&k_chain_pp<double, false>),
+                                              chain_pp_lds_bytes(ch, sizeof(double)))) != hipSuccess)
+        return e;
     if (ch.fused32 && (e = chain_fused_set_lds<float, false>(chain_fused_lds_bytes(ch, sizeof(float), false))) != hipSuccess)
         return e;
     if (ch.fused64) {
@@ -656,8 +999,19 @@ void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, 
     for (int sw = 0; sw < 2; ++sw)
         for (int k = 0; k < SDFT_FAST_S * SDFT_FAST_S; ++k) ca.ck[sw][k] = ch.ck[sw][k];
     ca.npad_max = ch.npad_max; ca.be_max = ch.be_max; ca.xl_need = ch.xl_need;
+    ca.shared_a = ch.shared_a ? 1 : 0;
     ca.rescale = sh.rescale; ca.nsp = c.nsp; ca.nmax = c.nmax_seg; ca.smax = smax;
     ca.info = ch.d_info;
+    if (sizeof(T) == 4 ? ch.pp32 : ch.pp64) {
+        const size_t lds = chain_pp_lds_bytes(ch, sizeof(T));
+        if (ch.shared_a)
+            hipLaunchKernelGGL((k_chain_pp<T, true>), dim3(c.P, 2), dim3(PP_THREADS), lds, st, c.d_sys, c.d_pts,
+                               c.d_evalflag, prec, c.P, ch.pp_min, ca, c.d_band, kern, elam, hl, trapez, gC, gP);
+        else
+            hipLaunchKernelGGL((k_chain_pp<T, false>), dim3(c.P, 2), dim3(PP_THREADS), lds, st, c.d_sys, c.d_pts,
+                               c.d_evalflag, prec, c.P, ch.pp_min, ca, c.d_band, kern, elam, hl, trapez, gC, gP);
+        return;
+    }
     if (sizeof(T) == 4 ? ch.fused32 : ch.fused64) {
         const bool reg = sizeof(T) == 8 && ch.fused_reg64;
         const size_t lds = chain_fused_lds_bytes(ch, sizeof(T), reg);
