@@ -639,6 +639,36 @@ size_t chain_pp_lds_bytes(const ChainPlanHost& ch, size_t elem) {
 
 constexpr int PP_THREADS = 1024;
 
+// Largest 64-bit pattern over the wave (every lane active): row prefix maxima by DPP row shifts,
+// then the four row maxima read as scalars. Non-negative doubles order as their bit patterns.
+template <int CTRL>
+__device__ __forceinline__ void dpp_max_step(unsigned& lo, unsigned& hi) {
+    const unsigned lo2 = __builtin_amdgcn_update_dpp(0u, lo, CTRL, 0xf, 0xf, false);
+    const unsigned hi2 = __builtin_amdgcn_update_dpp(0u, hi, CTRL, 0xf, 0xf, false);
+    const unsigned long long v = ((unsigned long long)hi2 << 32) | lo2;
+    const unsigned long long w = ((unsigned long long)hi << 32) | lo;
+    if (v > w) {
+        lo = lo2;
+        hi = hi2;
+    }
+}
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long u) {
+    unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+    dpp_max_step<0x111>(lo, hi);   // row_shr:1
+    dpp_max_step<0x112>(lo, hi);   // row_shr:2
+    dpp_max_step<0x114>(lo, hi);   // row_shr:4
+    dpp_max_step<0x118>(lo, hi);   // row_shr:8: lane 15 of each row holds the row maximum
+    unsigned long long m = 0ull;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const unsigned long long v =
+            ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)hi, 16 * r + 15) << 32) |
+            (unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)lo, 16 * r + 15);
+        m = v > m ? v : m;
+    }
+    return m;
+}
+
 template <typename T, bool SHA>
 __global__ void __launch_bounds__(PP_THREADS) k_chain_pp(
         const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts, const int32_t* __restrict__ evalflag,
@@ -886,6 +916,7 @@ __global__ void __launch_bounds__(PP_THREADS) k_chain_pp(
                 m = (m > 0.0 && m <= 1.7976931348623157e308) ? m : 1.0;  // scale_read's rule
                 scale = (T)(1.0 / m);
             }
+            double mx = 0.0;   // largest Q of this lane's outputs (step s < N - 1)
             if (has_unit && n0 < NP) {
                 const int outseg = sw == 0 ? N - 1 - s : s;
                 T qs[4] = {T(0), T(0), T(0), T(0)};
@@ -923,16 +954,16 @@ __global__ void __launch_bounds__(PP_THREADS) k_chain_pp(
                             }
                         }
                     }
-                    if (ca.rescale) {
-                        double mx = 0.0;
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) {
-                            const double qv = (double)qs[v];
-                            mx = qv > mx ? qv : mx;   // qs is zero past NP
-                        }
-                        atomicMax(&qm[s % 3], (unsigned long long)__double_as_longlong(mx));
+                    for (int v = 0; v < 4; ++v) {
+                        const double qv = (double)qs[v];
+                        mx = qv > mx ? qv : mx;   // qs is zero past NP
                     }
                 }
+            }
+            if (has_unit && ca.rescale && s < N - 1) {   // wave-uniform: every lane takes part
+                const unsigned long long wm = wave_max_u64((unsigned long long)__double_as_longlong(mx));
+                if (lane == 0) atomicMax(&qm[s % 3], wm);
             }
             __builtin_amdgcn_s_setprio(0);
         }
