@@ -734,16 +734,10 @@ static int build_chain_plan(sdft_ctx* ctx, const DevSystem& sys, int u, const st
     };
     ch.hv32 = hv_fits(sizeof(float), ch.ok32);
     ch.hv64 = hv_fits(sizeof(double), ch.ok64);
-    // homogeneous chains on the one-barrier kernel. FP64: with at most one (a, tile) unit per wave
-    // the tables sit in registers and two workgroups share a CU (within 80 KB of LDS each), else the
-    // tables sit in LDS and one workgroup runs per CU. FP32 keeps its tables in LDS with two
-    // workgroups per CU (measured at NP 801: 144 us against 148 us with register tables).
+    // homogeneous chains with at most one (a, tile) unit per wave run on k_chain_pp
     {
         const int units = S * (ch.npad_max / 256);
-        const size_t half = 80 * 1024 - 256, full = 160 * 1024 - 256;
-        ch.fused32 = ch.ok32 && ch.homog && chain_fused_lds_bytes(ch, sizeof(float), false) <= half;
-        ch.fused_reg64 = ch.ok64 && ch.homog && units <= 8 && chain_fused_lds_bytes(ch, sizeof(double), true) <= half;
-        ch.fused64 = ch.fused_reg64 || (ch.ok64 && ch.homog && chain_fused_lds_bytes(ch, sizeof(double), false) <= full);
+        const size_t full = 160 * 1024 - 256;
         // both sweeps of a point in one workgroup when more points are active than half the CUs
         int dev = 0, ncu = 0;
         if (hipGetDevice(&dev) != hipSuccess ||

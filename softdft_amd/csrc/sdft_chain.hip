@@ -311,12 +311,12 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
 
 // --------------------------------------------------------------------------
 // Homogeneous chains (every segment carries the same state beads, so the input weights
-// W_ab = exp(-lambda) exp(lng/2) trapez and the output factors are the same at every step):
-// one barrier per step. The next step's extended inputs X_{s+1}[(a', a)] = W_(a', a) Q_a are written
-// by the epilogue that produces Q_a, into the second of two X buffers, so the separate staging pass
-// and its barrier go away. The per-step rescaling is applied at the output instead of the input,
-// g_{s+1} = |A (W Q_s)| hv / max Q_s (the operator is linear, so the propagators are the same up to
-// rounding), and the maximum is reduced per wave in the epilogue.
+// W_ab = exp(-lambda) exp(lng/2) trapez and the output factors are the same at every step): the
+// next step's extended inputs X_{s+1}[(a', a)] = W_(a', a) Q_a are written by the epilogue that
+// produces Q_a, so no separate staging pass runs between the steps. The per-step rescaling is
+// applied at the output instead of the input, g_{s+1} = |A (W Q_s)| hv / max Q_s (the operator is
+// linear, so the propagators are the same up to rounding), and the maximum is reduced per wave in
+// the epilogue.
 // The MFMA row r of a tile holds output phase phi(r): for FP64 (D[4 v + l/16][l % 16]) phi permutes
 // the rows so that the four accumulators of a lane are four consecutive outputs, as they are for
 // FP32 (D[4 (l/16) + v][l % 16]); the epilogue then stores its propagators as 4-element vectors.
@@ -325,25 +325,10 @@ __global__ void __launch_bounds__(CHAIN_THREADS, sizeof(T) == 4 ? 4 : 1) k_chain
 // so those positions must be finite (0 x NaN would reach a valid output). Reads past that feed only
 // tile columns whose outputs all lie beyond NP, which the epilogue discards, and stay inside the
 // carve-out.
-// REG: every wave owns at most one (a, tile) unit and keeps that unit's weights W and output factors
-// in registers; the LDS then holds only the operator windows and the two X buffers, which lets two
-// workgroups share a CU (the FP64 variant included) so that one's epilogue and barrier overlap the
-// other's matrix products. Otherwise W and the output factors are LDS tables.
 // --------------------------------------------------------------------------
 __host__ __device__ __forceinline__ int round4(int n) { return (n + 3) & ~3; }
-__host__ __device__ __forceinline__ int chain_fused_xlp(int xl_need) {
+__host__ __device__ __forceinline__ int chain_xlp(int xl_need) {
     return round4(xl_need + 16 + ((xl_need + 16) >> 4) + 1);
-}
-
-size_t chain_fused_lds_bytes(const ChainPlanHost& ch, size_t elem, bool reg) {
-    const int S = ch.S, C = S * S;
-    const int KC = 2 * ch.be_max + 32;
-    const int tables = reg ? 0 : 2 * C * ch.npad_max;
-    // the MFMA tiles read up to npad_max + 2 be_max + 16 extended positions of the last column
-    const int full = ch.npad_max + 2 * ch.be_max + 16;
-    const int over = full + (full >> 4) + 1 - chain_fused_xlp(ch.xl_need);
-    return (size_t)(round4(C * KC) + 2 * C * chain_fused_xlp(ch.xl_need) + tables + (over > 0 ? over : 0) +
-                    CHAIN_LDS_SLACK) * elem;
 }
 
 template <typename T> struct Vec4;
@@ -367,253 +352,6 @@ template <> struct Vec4<double> {
     }
 };
 
-template <typename T, bool REG>
-__global__ void __launch_bounds__(CHAIN_THREADS, (REG || sizeof(T) == 4) ? 4 : 1) k_chain_fused(
-        const DevSystem* __restrict__ psys, const DevPoint* __restrict__ pts, const int32_t* __restrict__ evalflag,
-        int prec, ChainArgs ca, const int32_t* __restrict__ band, const T* __restrict__ kern,
-        const T* __restrict__ elam, const T* __restrict__ hl, const T* __restrict__ trapez, T* __restrict__ gC,
-        T* __restrict__ gP) {
-    const int p = blockIdx.x, sw = blockIdx.y;
-    if (evalflag[p] != prec) return;
-    const DevSystem& sys = *psys;
-    const DevPoint& P = pts[p];
-    const int NP = P.NP, L = P.L;
-    const int Be = chain_half_width(band[P.kset]);
-    const int KS = Be / 2 + 4;                 // k-steps: K / 4 with K = 2 Be + 16
-    const int ntile = (NP + 255) >> 8;
-    const int XN = NP + 2 * Be;                // extended positions the valid outputs read
-    const int S = ca.S, C = S * S, N = ca.nseg;
-    const int KC = 2 * ca.be_max + 32;
-    const int XLR = chain_fused_xlp(ca.xl_need);
-    const int NPQ = ca.npad_max;
-
-    extern __shared__ __attribute__((aligned(16))) unsigned char chain_fused_lds[];
-    T* cw = reinterpret_cast<T*>(chain_fused_lds);  // [C][KC] operator windows (reversed)
-    T* XB = cw + round4(C * KC);                    // [2][C][XLR] extended inputs, by step parity
-    T* WT = XB + 2 * C * XLR;                       // !REG: [C][NPQ] input weights W
-    T* HV = WT + C * NPQ;                           // !REG: [C][NPQ] exp(lng/2) of the output end / normative
-    __shared__ ChainCol cinfo[SDFT_FAST_S * SDFT_FAST_S];
-    // largest Q of step s in slot s mod 3 (bit patterns of non-negative doubles order as integers):
-    // written by LDS atomics in the epilogues of step s, read after the barrier of step s + 1,
-    // cleared during step s - 1, after the reads of step s - 3's maximum
-    __shared__ unsigned long long qmax[3];
-
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform: scalar branches
-    const double norm = P.normative[ca.u];
-    {
-        const T* cset = kern + (int64_t)P.kbase * sys.K;
-        for (int e = tid; e < C * KC; e += CHAIN_THREADS) {
-            const int col = e / KC, ri = e - col * KC;
-            const int k = ca.ck[sw][col];
-            const int j = (KC - 1 - ri) - 15 - Be;
-            T v = T(0);
-            if (k >= 0 && j >= -Be && j <= Be) {
-                int jj = j % L;
-                if (jj < 0) jj += L;
-                v = cset[(int64_t)k * L + jj];
-            }
-            cw[e] = v;
-        }
-    }
-    // the columns of the first step hold for every step of a homogeneous chain
-    if (tid < C) cinfo[tid] = ca.info[(int64_t)sw * (N - 1) * C + tid];
-    if (tid < 3) qmax[tid] = 0ull;
-    __syncthreads();
-    int validmask = 0;   // bit col: column col carries a bond (the same at every step)
-    for (int col = 0; col < C; ++col) validmask |= cinfo[col].valid ? 1 << col : 0;
-    validmask = __builtin_amdgcn_readfirstlane(validmask);
-    const int64_t eb = P.base * (int64_t)ca.NB;
-    const int64_t hb = P.base * (int64_t)(2 * ca.MB);
-    auto weight = [&](const ChainCol& cc, int n) -> T {  // W of column cc at point n
-        return cc.valid ? elam[eb + (int64_t)cc.sj * NP + n] * hl[hb + (int64_t)cc.hl_in * NP + n] *
-                              trapez[eb + (int64_t)cc.sj * NP + n]
-                        : T(0);
-    };
-    auto outfac = [&](const ChainCol& cc, int n) -> T {
-        return cc.valid ? hl[hb + (int64_t)cc.hl_out * NP + n] / (T)norm : T(0);
-    };
-    if (!REG) {
-        for (int e = tid; e < C * NP; e += CHAIN_THREADS) {
-            const int col = e / NP, n = e - col * NP;
-            const ChainCol cc = cinfo[col];
-            WT[col * NPQ + n] = weight(cc, n);
-            HV[col * NPQ + n] = outfac(cc, n);
-        }
-        __syncthreads();
-    }
-    // inputs of step 1 (Q = 1 at the free chain end) into buffer 1 (the epilogues write the others),
-    // and the 16 zeros past XN in both buffers
-    for (int e = tid; e < C * (XN + 16); e += CHAIN_THREADS) {
-        const int col = e / (XN + 16), mi = e - col * (XN + 16);
-        if (mi < XN) {
-            const int m = mi - Be;
-            const int n = m < 0 ? -1 - m : (m < NP ? m : 2 * NP - 1 - m);
-            XB[(C + col) * XLR + chain_phys(mi)] = REG ? weight(cinfo[col], n) : WT[col * NPQ + n];
-        } else {
-            XB[col * XLR + chain_phys(mi)] = T(0);
-            XB[(C + col) * XLR + chain_phys(mi)] = T(0);
-        }
-    }
-
-    T* gout = sw == 0 ? gC : gP;
-    const int64_t gstride = P.gstride;
-    const int64_t gb = P.gbase + ca.goff * gstride;
-    const int q = lane & 15, g = lane >> 4;
-    const int phq = sizeof(T) == 8 ? 4 * (q & 3) + (q >> 2) : q;   // output phase of MFMA row q
-    // REG: the unit's output factors hr[b][v] and next-step weights wr[a2][v], v = the lane's outputs
-    T hr[SDFT_FAST_S][4], wr[SDFT_FAST_S][4];
-    if (REG && wave < S * ntile) {
-        const int a = wave / ntile, tile = wave - a * ntile;
-        const int n0 = 16 * (16 * tile + q) + 4 * g;
-#pragma unroll
-        for (int b = 0; b < SDFT_FAST_S; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const bool in = b < S && n0 + v < NP;
-                hr[b][v] = in ? outfac(cinfo[(a * S + b) % C], n0 + v) : T(0);
-                wr[b][v] = in ? weight(cinfo[(b * S + a) % C], n0 + v) : T(0);
-            }
-    }
-    for (int s = 1; s < N; ++s) {
-        const int outseg = sw == 0 ? N - 1 - s : s;
-        __syncthreads();  // inputs of step s written; the maximum of step s - 1 complete
-        if (tid == 0) qmax[(s + 1) % 3] = 0ull;
-        T scale = T(1);
-        if (ca.rescale && s > 1) {
-            double m = __longlong_as_double((long long)qmax[(s - 1) % 3]);
-            if (tid == 0) {
-                const int64_t slot = scale_slot(p, ca.u, sw, s - 1, ca.nsp, ca.nmax);
-                ca.smax[slot] = (unsigned long long)__double_as_longlong(m);
-            }
-            m = (m > 0.0 && m <= 1.7976931348623157e308) ? m : 1.0;  // scale_read's rule
-            scale = (T)(1.0 / m);
-        }
-        const T* Xc = XB + (s & 1) * C * XLR;
-        T* Xn = XB + ((s + 1) & 1) * C * XLR;
-        const bool last = s == N - 1;
-        for (int unit = wave; unit < S * ntile; unit += CHAIN_WAVES) {
-            const int a = unit / ntile, tile = unit - a * ntile;
-            const int p0 = tile * 16;
-            const int n0 = 16 * (p0 + q) + 4 * g;   // the lane's four consecutive outputs n0 .. n0 + 3
-            T qs[4] = {T(0), T(0), T(0), T(0)};
-#pragma unroll
-            for (int b = 0; b < SDFT_FAST_S; ++b) {
-                if (b >= S) break;
-                const int col = a * S + b;
-                if (!((validmask >> col) & 1)) continue;
-                // A[q][t] = c[phi(q) - t + Be] (reversed window); B[g][q] = x_ext[16 (p0 + q) + t - Be],
-                // t = 4 ks + g, ks = 4 K + k4: X[phys(16 (p0 + q) + g + 4 ks)] = X[17 (p0 + q) + g + 17 K + 4 k4]
-                const T* ap = cw + col * KC + (KC - 16 - 2 * Be) - phq + g;
-                const T* xp = Xc + col * XLR + 17 * (p0 + q) + g;
-                typename Mfma<T>::acc_t acc;
-#pragma unroll
-                for (int v = 0; v < 4; ++v) acc[v] = T(0);
-                T a0[4], b0[4], a1[4], b1[4];
-                auto fetch = [&](int K, T (&av)[4], T (&bv)[4]) {
-#pragma unroll
-                    for (int k4 = 0; k4 < 4; ++k4) {
-                        av[k4] = ap[16 * K + 4 * k4];
-                        bv[k4] = xp[17 * K + 4 * k4];
-                    }
-                };
-                const int KG = KS / 4;
-                fetch(0, a0, b0);
-                int K = 0;
-                for (; K + 1 < KG; K += 2) {
-                    fetch(K + 1, a1, b1);
-                    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                    for (int k4 = 0; k4 < 4; ++k4) acc = Mfma<T>::mma(a0[k4], b0[k4], acc);
-                    __builtin_amdgcn_sched_barrier(0);
-                    fetch(K + 2, a0, b0);  // at most one group past the last: within the LDS carve-out, unused
-                    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                    for (int k4 = 0; k4 < 4; ++k4) acc = Mfma<T>::mma(a1[k4], b1[k4], acc);
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-                if (K < KG) {
-#pragma unroll
-                    for (int k4 = 0; k4 < 4; ++k4) acc = Mfma<T>::mma(a0[k4], b0[k4], acc);
-                }
-                for (int ks = 4 * KG; ks < KS; ++ks)
-                    acc = Mfma<T>::mma(ap[4 * ks], xp[17 * (ks >> 2) + 4 * (ks & 3)], acc);
-                if (n0 < NP) {
-                    T hv[4], val[4];
-                    if (REG) {
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) hv[v] = hr[b][v];
-                    } else {
-                        Vec4<T>::load(HV + col * NPQ + n0, hv);
-                    }
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const T y = acc[v];
-                        const T ay = y < T(0) ? -y : y;
-                        val[v] = n0 + v < NP ? ay * hv[v] * scale : T(0);
-                        qs[v] += val[v];
-                    }
-                    // the padding of the propagator rows past NP receives zeros
-                    Vec4<T>::store(gout + gb + (((int64_t)outseg * S + b) * S + a) * gstride + n0, val);
-                }
-            }
-            if (!last && n0 < NP) {
-                // inputs of step s + 1 whose input state is a: X[(a2, a)] = W Q_a, at the point and its mirrors
-#pragma unroll
-                for (int a2 = 0; a2 < SDFT_FAST_S; ++a2) {
-                    if (a2 >= S) break;
-                    const int col2 = a2 * S + a;
-                    T w[4];
-                    if (REG) {
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) w[v] = wr[a2][v];
-                    } else {
-                        Vec4<T>::load(WT + col2 * NPQ + n0, w);
-                    }
-                    T* xn = Xn + col2 * XLR;
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int n = n0 + v;
-                        if (n < NP) {
-                            const T x = w[v] * qs[v];
-                            xn[chain_phys(n + Be)] = x;
-                            if (n < Be) xn[chain_phys(Be - 1 - n)] = x;
-                            const int mt = 2 * NP - 1 - n + Be;
-                            if (mt < XN) xn[chain_phys(mt)] = x;
-                        }
-                    }
-                }
-                if (ca.rescale) {
-                    double mx = 0.0;
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const double qv = (double)qs[v];
-                        mx = qv > mx ? qv : mx;   // qs is zero past NP
-                    }
-                    atomicMax(&qmax[s % 3], (unsigned long long)__double_as_longlong(mx));
-                }
-            }
-        }
-        // the next step's barrier orders these writes before their reads
-    }
-}
-
-size_t chain_lds_bytes(const ChainPlanHost& ch, size_t elem, bool hv_table) {
-    const int S = ch.S, C = S * S;
-    const int KC = 2 * ch.be_max + 32;
-    const int XLm = ch.npad_max + 2 * ch.be_max;
-    const int XLP = XLm + (XLm >> 4) + 1;
-    const int HV = hv_table ? C * ch.npad_max : 0;
-    return (size_t)(C * KC + C * XLP + S * ch.npad_max + HV + CHAIN_LDS_SLACK) * elem;
-}
-
-template <typename T, int EPC>
-static hipError_t chain_set_lds(size_t bytes) {
-    if (bytes <= 65536) return hipSuccess;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chain<T, EPC>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-}
-
 // --------------------------------------------------------------------------
 // Both sweeps of a point in one 16-wave workgroup, offset by half a step (ping-pong): waves 0-7 run
 // the child sweep and waves 8-15 the parent sweep, and in every phase between two barriers one half
@@ -624,16 +362,16 @@ static hipError_t chain_set_lds(size_t bytes) {
 // (ph - h) / 2 + 1 when ph - h is even and the finish of step (ph - h + 1) / 2 when it is odd.
 // The two halves share a CU, so a point's chain takes 2 N - 1 phases instead of N - 1 steps on two
 // CUs: at low load (at most half as many active points as CUs) each workgroup instead runs one
-// sweep on waves 0-7 with a double-buffered X (the one-barrier schedule above), and the second
-// workgroup of the point runs the other sweep. Every wave runs the same barrier sequence.
+// sweep on waves 0-7 with a double-buffered X and one barrier per step, and the second workgroup
+// of the point runs the other sweep. Every wave runs the same barrier sequence.
 // One (a, tile) unit per wave, its weights and output factors in registers.
 // --------------------------------------------------------------------------
 size_t chain_pp_lds_bytes(const ChainPlanHost& ch, size_t elem) {
     const int S = ch.S, C = S * S;
     const int KC = 2 * ch.be_max + 32;
     const int full = ch.npad_max + 2 * ch.be_max + 16;
-    const int over = full + (full >> 4) + 1 - chain_fused_xlp(ch.xl_need);
-    return (size_t)(2 * round4(C * KC) + 2 * C * chain_fused_xlp(ch.xl_need) + (over > 0 ? over : 0) +
+    const int over = full + (full >> 4) + 1 - chain_xlp(ch.xl_need);
+    return (size_t)(2 * round4(C * KC) + 2 * C * chain_xlp(ch.xl_need) + (over > 0 ? over : 0) +
                     CHAIN_LDS_SLACK) * elem;
 }
 
@@ -702,7 +440,7 @@ __global__ void __launch_bounds__(PP_THREADS) k_chain_pp(
     const int XN = NP + 2 * Be;
     const int S = ca.S, C = S * S, N = ca.nseg;
     const int KC = 2 * ca.be_max + 32;
-    const int XLR = chain_fused_xlp(ca.xl_need);
+    const int XLR = chain_xlp(ca.xl_need);
     const int nslot = pp ? 2 : 1;                    // sweeps in this workgroup
     const bool active = half < nslot;                // this wave's half has a sweep
     const int sw = pp ? half : wg;                   // its sweep
@@ -971,10 +709,19 @@ __global__ void __launch_bounds__(PP_THREADS) k_chain_pp(
     }
 }
 
-template <typename T, bool REG>
-static hipError_t chain_fused_set_lds(size_t bytes) {
+size_t chain_lds_bytes(const ChainPlanHost& ch, size_t elem, bool hv_table) {
+    const int S = ch.S, C = S * S;
+    const int KC = 2 * ch.be_max + 32;
+    const int XLm = ch.npad_max + 2 * ch.be_max;
+    const int XLP = XLm + (XLm >> 4) + 1;
+    const int HV = hv_table ? C * ch.npad_max : 0;
+    return (size_t)(C * KC + C * XLP + S * ch.npad_max + HV + CHAIN_LDS_SLACK) * elem;
+}
+
+template <typename T, int EPC>
+static hipError_t chain_set_lds(size_t bytes) {
     if (bytes <= 65536) return hipSuccess;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chain_fused<T, REG>),
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_chain<T, EPC>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
@@ -993,13 +740,6 @@ hipError_t chain_prepare(const ChainPlanHost& ch) {
                                     : pp_attr(reinterpret_cast<const void*>(&k_chain_pp<double, false>),
                                               chain_pp_lds_bytes(ch, sizeof(double)))) != hipSuccess)
         return e;
-    if (ch.fused32 && (e = chain_fused_set_lds<float, false>(chain_fused_lds_bytes(ch, sizeof(float), false))) != hipSuccess)
-        return e;
-    if (ch.fused64) {
-        e = ch.fused_reg64 ? chain_fused_set_lds<double, true>(chain_fused_lds_bytes(ch, sizeof(double), true))
-                           : chain_fused_set_lds<double, false>(chain_fused_lds_bytes(ch, sizeof(double), false));
-        if (e != hipSuccess) return e;
-    }
     if (ch.ok32) {
         const size_t b = chain_lds_bytes(ch, sizeof(float), ch.hv32);
         if ((e = chain_set_lds<float, 0>(b)) != hipSuccess) return e;
@@ -1041,17 +781,6 @@ void launch_chain(hipStream_t st, const Ctx& c, int prec, int u, const T* kern, 
         else
             hipLaunchKernelGGL((k_chain_pp<T, false>), dim3(c.P, 2), dim3(PP_THREADS), lds, st, c.d_sys, c.d_pts,
                                c.d_evalflag, prec, c.P, ch.pp_min, ca, c.d_band, kern, elam, hl, trapez, gC, gP);
-        return;
-    }
-    if (sizeof(T) == 4 ? ch.fused32 : ch.fused64) {
-        const bool reg = sizeof(T) == 8 && ch.fused_reg64;
-        const size_t lds = chain_fused_lds_bytes(ch, sizeof(T), reg);
-        if (reg)
-            hipLaunchKernelGGL((k_chain_fused<T, sizeof(T) == 8>), dim3(c.P, 2), dim3(CHAIN_THREADS), lds, st, c.d_sys, c.d_pts,
-                               c.d_evalflag, prec, ca, c.d_band, kern, elam, hl, trapez, gC, gP);
-        else
-            hipLaunchKernelGGL((k_chain_fused<T, false>), dim3(c.P, 2), dim3(CHAIN_THREADS), lds, st, c.d_sys, c.d_pts,
-                               c.d_evalflag, prec, ca, c.d_band, kern, elam, hl, trapez, gC, gP);
         return;
     }
     const bool hvt = sizeof(T) == 4 ? ch.hv32 : ch.hv64;

@@ -112,9 +112,6 @@ struct ChainPlanHost {
     ChainCol* d_info = nullptr;                    // [2 sweeps][N-1 steps][S*S] column table (device)
     bool hv32 = false, hv64 = false;               // homogeneous chain: the output factors
                                                    // exp(lng/2)/normative tabulated in LDS
-    bool fused32 = false, fused64 = false;         // homogeneous chain on k_chain_fused (one barrier
-                                                   // per step, double-buffered inputs)
-    bool fused_reg64 = false;                      // ... FP64 with one unit per wave, tables in registers
     bool pp32 = false, pp64 = false;               // both sweeps in one 16-wave workgroup (k_chain_pp)
     int pp_min = 0;                                // active points from which k_chain_pp pairs the sweeps
     bool shared_a = false;                         // every valid column of both sweeps: one bond operator
@@ -368,7 +365,6 @@ template <typename T>
 void launch_opgemm_conv(hipStream_t st, const Ctx& c, int prec, const OpPlan& plan, const T* kern, const T* rhoT,
                         const T* dphi, T* nw, double* fx);
 size_t chain_lds_bytes(const ChainPlanHost& ch, size_t elem, bool hv_table);
-size_t chain_fused_lds_bytes(const ChainPlanHost& ch, size_t elem, bool reg);
 size_t chain_pp_lds_bytes(const ChainPlanHost& ch, size_t elem);
 hipError_t chain_prepare(const ChainPlanHost& ch);
 template <typename T>

@@ -38,3 +38,21 @@ def hip_free_memory() -> int:
     assert hip.hipDeviceSynchronize() == 0
     assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
     return free.value
+
+
+def truncated_oracle(sp):
+    """Oracle whose bond operators keep only |j| <= Be of the filtered bond kernel c, where B is the
+    last offset with |c| > 1e-9 max|c| over the bond types and Be is B rounded up to even: the
+    operator the banded chain kernel applies."""
+    o = OracleDFT(sp)
+    L = o.L
+    d = np.arange(L)
+    dd = np.minimum(d, L - d)
+    cs = [np.fft.ifft(o.wbond_filt[:, u]).real for u in range(sp.MB)]
+    B = max(int(dd[np.abs(c) > 1e-9 * np.abs(c).max()].max()) for c in cs)
+    Be = B + (B & 1)
+    for u, c in enumerate(cs):
+        c = c.copy()
+        c[dd > Be] = 0.0
+        o.wbond_filt[:, u] = np.fft.fft(c).real
+    return o, Be

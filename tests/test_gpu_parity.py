@@ -17,7 +17,7 @@ import pytest
 from softdft_amd import abi, configs
 from softdft_amd.problem import build_state_point
 from oracle.dft_oracle import OracleDFT, anderson_solve
-from parity_helpers import hip_free_memory, omega_band
+from parity_helpers import hip_free_memory, omega_band, truncated_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -164,28 +164,10 @@ def test_observables_reuse_their_buffers():
     b.close()
 
 
-def _truncated_oracle(sp):
-    """Oracle whose bond operators keep only |j| <= Be of the filtered bond kernel c, where B is the
-    last offset with |c| > 1e-9 max|c| over the bond types and Be is B rounded up to even: the
-    operator the banded chain kernel applies."""
-    o = OracleDFT(sp)
-    L = o.L
-    d = np.arange(L)
-    dd = np.minimum(d, L - d)
-    cs = [np.fft.ifft(o.wbond_filt[:, u]).real for u in range(sp.MB)]
-    B = max(int(dd[np.abs(c) > 1e-9 * np.abs(c).max()].max()) for c in cs)
-    Be = B + (B & 1)
-    for u, c in enumerate(cs):
-        c = c.copy()
-        c[dd > Be] = 0.0
-        o.wbond_filt[:, u] = np.fft.fft(c).real
-    return o, Be
-
-
 @pytest.mark.parametrize("name", ["chains", "weak_pe", "weak_pe_charged_walls", "mfmt_msa"])
 def test_banded_fp64_objective_matches_truncated_oracle(name):
     sp = build_state_point(SYSTEMS[name])
-    o, Be = _truncated_oracle(sp)
+    o, Be = truncated_oracle(sp)
     assert 2 * Be + 1 <= sp.NP  # mirror-extended band fits: the persistent chain kernel runs
     b = abi.Batch([sp])
     b.set_operator_band(fp32=True, fp64=True)
@@ -213,7 +195,7 @@ def test_banded_fp64_ragged_batch():
     x0 = b.x0()
     fx = b.objective(x0, abi.PREC_FP64)
     for p, sp in enumerate(sps):
-        o, Be = _truncated_oracle(sp)
+        o, Be = truncated_oracle(sp)
         assert 2 * Be + 1 <= sp.NP
         ref = o.objective(sp.x0())
         assert np.max(np.abs(b.split(fx)[p] - ref)) / max(1.0, np.max(np.abs(ref))) < 1e-11
@@ -315,7 +297,7 @@ def test_chain_species_after_other_species(mapping):
     fx32 = b.objective(x, abi.PREC_FP32)
     assert np.max(np.abs(fx32 - ref)) / scale < 5e-5
     b.set_operator_band(fp32=True, fp64=True)
-    o, _ = _truncated_oracle(sp)
+    o, _ = truncated_oracle(sp)
     refb = o.objective(x)
     fxb = b.objective(x, abi.PREC_FP64)
     assert np.max(np.abs(fxb - refb)) / scale < 1e-11
