@@ -546,6 +546,11 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
         // the bond of each (a, b) link are the same at every segment, so the contributions are summed
         // in registers and added to the LDS accumulators once per species
         const bool hom = sp.homog != 0;
+        // states of segment q (bead index, -1 absent); a homogeneous species reads segment 0's once
+        int st0[SM];
+#pragma unroll
+        for (int a = 0; a < SM; ++a) st0[a] = a < S ? st[a] : -1;
+        auto stq = [&](int qq, int a) -> int { return hom ? st0[a] : st[qq * S + a]; };
         // monomerbead (monomerbead/accumulate_densities.jl:46-85): every segment independent,
         // rho = norm exp(mu_u - lambda), no propagators and no bonds
         const bool mb = sp.model == SDFT_MODEL_MONOMERBEAD;
@@ -567,7 +572,7 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
 #pragma unroll
             for (int a = 0; a < SM; ++a) {
                 const int ac = a < S ? a : 0;
-                const int si = a < S ? st[q * S + a] : -1;
+                const int si = a < S ? stq(q, a) : -1;
                 const int sic = si >= 0 ? si : 0;
                 const T e = elam[fidx(P, NB, sic, nc)], t = trapez[fidx(P, NB, sic, nc)];
                 elv[a] = si >= 0 ? e : T(0);
@@ -575,8 +580,8 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
 #pragma unroll
                 for (int b = 0; b < SM; ++b) {
                     const int bc = b < S ? b : 0;
-                    const bool okc = si >= 0 && b < S && q < N - 1 && st[(q + 1) * S + b] >= 0;
-                    const bool okp = si >= 0 && b < S && q > 0 && st[(q - 1) * S + b] >= 0;
+                    const bool okc = si >= 0 && b < S && q < N - 1 && stq(q + 1, b) >= 0;
+                    const bool okp = si >= 0 && b < S && q > 0 && stq(q - 1, b) >= 0;
                     // a single-bead species has no propagators: no loads (N is wave-uniform)
                     const T vc = (N > 1 && !mb) ? gC[gidx(q, bc, ac, nc)] : T(0);
                     const T vp = (N > 1 && !mb) ? gP[gidx(q, bc, ac, nc)] : T(0);
@@ -586,7 +591,7 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
             }
 #pragma unroll
             for (int a = 0; a < SM; ++a) {
-                const int si = a < S ? st[q * S + a] : -1;
+                const int si = a < S ? stq(q, a) : -1;
                 if (si < 0) continue;
                 double Gp = 0.0, Qc = 0.0;
                 if (mb) {
@@ -596,13 +601,13 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                     else
 #pragma unroll
                         for (int b = 0; b < SM; ++b)
-                            if (b < S && st[(q - 1) * S + b] >= 0) Gp += (double)gpv[b][a];
+                            if (b < S && stq(q - 1, b) >= 0) Gp += (double)gpv[b][a];
                     // leaf: sum gC[K, :, 1, idx_i, leaf] is the base case 1 for every state
                     if (q == N - 1) Qc = (double)base_value<T>(q, nc, sp.fixed_seg, Kf, P, maxd);
                     else
 #pragma unroll
                         for (int b = 0; b < SM; ++b)
-                            if (b < S && st[(q + 1) * S + b] >= 0) Qc += (double)gcv[b][a];
+                            if (b < S && stq(q + 1, b) >= 0) Qc += (double)gcv[b][a];
                 }
                 const double el = (double)elv[a];
                 const double contr = pref * el * Gp;
@@ -624,7 +629,7 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                     if (q < N - 1 && !mb) {
 #pragma unroll
                         for (int b = 0; b < SM; ++b) {
-                            const int sj = b < S ? st[(q + 1) * S + b] : -1;
+                            const int sj = b < S ? stq(q + 1, b) : -1;
                             if (sj < 0) continue;
                             if (hom) {
                                 rc0[a][b] += contr * (double)gcv[b][a];
@@ -641,7 +646,7 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                         const double c1 = pref * el * Qc;
 #pragma unroll
                         for (int b = 0; b < SM; ++b) {
-                            const int sj = b < S ? st[(q - 1) * S + b] : -1;
+                            const int sj = b < S ? stq(q - 1, b) : -1;
                             if (sj < 0) continue;
                             if (hom) {
                                 rc1[a][b] += c1 * (double)gpv[b][a];
