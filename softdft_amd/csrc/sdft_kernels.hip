@@ -499,10 +499,32 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
     // species, dealt round-robin continuing from one species to the next (so single-bead species do
     // not all fall to group 0), which keeps more independent loads in flight per point; the groups'
     // partial densities are added in group order at the end.
-    const DevSystem& sys = *psys;
+    // The point's, the system's and the species' descriptors are copied to LDS first: read through
+    // the scalar cache, their many fields (a different point in every block) cost a chain of
+    // dependent global round trips per species and segment.
     const int p = blockIdx.y;
     if (evalflag[p] != prec) return;
-    const DevPoint& P = pts[p];
+    __shared__ DevPoint Ps;
+    __shared__ DevSystem Ss;
+    __shared__ DevSpecies Us[SDFT_MAX_SPECIES];
+    {
+        static_assert(sizeof(DevPoint) % 4 == 0 && sizeof(DevSystem) % 4 == 0 && sizeof(DevSpecies) % 4 == 0,
+                      "descriptor copies move 4-byte words");
+        const int32_t* sp_src = reinterpret_cast<const int32_t*>(pts + p);
+        int32_t* sp_dst = reinterpret_cast<int32_t*>(&Ps);
+        for (int i = threadIdx.x; i < (int)(sizeof(DevPoint) / 4); i += TB) sp_dst[i] = sp_src[i];
+        const int32_t* ss_src = reinterpret_cast<const int32_t*>(psys);
+        int32_t* ss_dst = reinterpret_cast<int32_t*>(&Ss);
+        for (int i = threadIdx.x; i < (int)(sizeof(DevSystem) / 4); i += TB) ss_dst[i] = ss_src[i];
+        __syncthreads();
+        const int nsp_words = Ss.n_species * (int)(sizeof(DevSpecies) / 4);
+        const int32_t* su_src = reinterpret_cast<const int32_t*>(spec);
+        int32_t* su_dst = reinterpret_cast<int32_t*>(Us);
+        for (int i = threadIdx.x; i < nsp_words; i += TB) su_dst[i] = su_src[i];
+        __syncthreads();
+    }
+    const DevSystem& sys = Ss;
+    const DevPoint& P = Ps;
     const int NP = P.NP, NB = sys.NB, MB = sys.MB;
     const int nl = threadIdx.x % ACC_POINTS;
     const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x / ACC_POINTS);  // wave-uniform: scalar segment loop
@@ -518,7 +540,7 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
     __shared__ double red[2 * SDFT_MAX_SPECIES][TB / 64];
     int dealt = 0;  // segments dealt to the groups by the previous species (round-robin across species)
     for (int u = 0; u < sys.n_species; ++u) {
-        const DevSpecies sp = spec[u];
+        const DevSpecies sp = Us[u];
         const int N = sp.nseg, S = sp.S;
         const int q0 = (grp - dealt % ACC_GROUPS + ACC_GROUPS) % ACC_GROUPS;  // this group's first segment
         dealt += N;
