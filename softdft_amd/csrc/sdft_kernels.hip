@@ -559,154 +559,159 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
         } else {
             norm = sp.rescale ? 1.0 : P.chain_norm[u];  // rescaled: normative^(N-1) enters per segment
         }
-        const double pref0 = norm * exp(P.mu_species[u]);
-        auto gidx = [&](int seg, int b, int a, int nn) -> int64_t {
-            return P.gbase + (goffs[u] + ((int64_t)seg * S + b) * S + a) * P.gstride + nn;
-        };
         double num = 0.0, chg = 0.0;
-        // homogeneous species (every segment carries the same state beads): the bead of state a and
-        // the bond of each (a, b) link are the same at every segment, so the contributions are summed
-        // in registers and added to the LDS accumulators once per species
-        const bool hom = sp.homog != 0;
-        // states of segment q (bead index, -1 absent); a homogeneous species reads segment 0's once
-        int st0[SM];
+        // a group with no segment of this species adds nothing (its accumulators stay +0)
+        if (q0 < N) {
+            const double pref0 = norm * exp(P.mu_species[u]);
+            auto gidx = [&](int seg, int b, int a, int nn) -> int64_t {
+                return P.gbase + (goffs[u] + ((int64_t)seg * S + b) * S + a) * P.gstride + nn;
+            };
+            // homogeneous species (every segment carries the same state beads): the bead of state a and
+            // the bond of each (a, b) link are the same at every segment, so the contributions are summed
+            // in registers and added to the LDS accumulators once per species
+            const bool hom = sp.homog != 0;
+            // states of segment q (bead index, -1 absent); a homogeneous species reads segment 0's once
+            int st0[SM];
 #pragma unroll
-        for (int a = 0; a < SM; ++a) st0[a] = a < S ? st[a] : -1;
-        auto stq = [&](int qq, int a) -> int { return hom ? st0[a] : st[qq * S + a]; };
-        // monomerbead (monomerbead/accumulate_densities.jl:46-85): every segment independent,
-        // rho = norm exp(mu_u - lambda), no propagators and no bonds
-        const bool mb = sp.model == SDFT_MODEL_MONOMERBEAD;
-        double racc[SM], rc0[SM][SM], rc1[SM][SM];
-#pragma unroll
-        for (int a = 0; a < SM; ++a) {
-            racc[a] = 0.0;
-#pragma unroll
-            for (int b = 0; b < SM; ++b) rc0[a][b] = rc1[a][b] = 0.0;
-        }
-        for (int q = q0; q < N; q += ACC_GROUPS) {
-            const double pref = sp.rescale ? pref0 * exp(seg_log_scale(sp, P, p, u, q, lsc, sys.n_species, nmax))
-                                           : pref0;
-            // every propagator value of this segment is loaded up front (independent loads in flight),
-            // gcv[b][a] = gC[q][b][a] (child side, b: state of q+1), gpv[b][a] = gP[q][b][a] (parent side)
-            // The loads are unconditional, from clamped (always in-bounds) indices, and masked after
-            // the fact, so that they issue back to back instead of one conditional load at a time.
-            T gcv[SM][SM], gpv[SM][SM], elv[SM], tzv[SM];
+            for (int a = 0; a < SM; ++a) st0[a] = a < S ? st[a] : -1;
+            auto stq = [&](int qq, int a) -> int { return hom ? st0[a] : st[qq * S + a]; };
+            // monomerbead (monomerbead/accumulate_densities.jl:46-85): every segment independent,
+            // rho = norm exp(mu_u - lambda), no propagators and no bonds
+            const bool mb = sp.model == SDFT_MODEL_MONOMERBEAD;
+            double racc[SM], rc0[SM][SM], rc1[SM][SM];
 #pragma unroll
             for (int a = 0; a < SM; ++a) {
-                const int ac = a < S ? a : 0;
-                const int si = a < S ? stq(q, a) : -1;
-                const int sic = si >= 0 ? si : 0;
-                const T e = elam[fidx(P, NB, sic, nc)], t = trapez[fidx(P, NB, sic, nc)];
-                elv[a] = si >= 0 ? e : T(0);
-                tzv[a] = si >= 0 ? t : T(0);
+                racc[a] = 0.0;
 #pragma unroll
-                for (int b = 0; b < SM; ++b) {
-                    const int bc = b < S ? b : 0;
-                    const bool okc = si >= 0 && b < S && q < N - 1 && stq(q + 1, b) >= 0;
-                    const bool okp = si >= 0 && b < S && q > 0 && stq(q - 1, b) >= 0;
-                    // a single-bead species has no propagators: no loads (N is wave-uniform)
-                    const T vc = (N > 1 && !mb) ? gC[gidx(q, bc, ac, nc)] : T(0);
-                    const T vp = (N > 1 && !mb) ? gP[gidx(q, bc, ac, nc)] : T(0);
-                    gcv[b][a] = okc ? vc : T(0);
-                    gpv[b][a] = okp ? vp : T(0);
-                }
+                for (int b = 0; b < SM; ++b) rc0[a][b] = rc1[a][b] = 0.0;
             }
+            for (int q = q0; q < N; q += ACC_GROUPS) {
+                const double pref = sp.rescale ? pref0 * exp(seg_log_scale(sp, P, p, u, q, lsc, sys.n_species, nmax))
+                                               : pref0;
+                // every propagator value of this segment is loaded up front (independent loads in flight),
+                // gcv[b][a] = gC[q][b][a] (child side, b: state of q+1), gpv[b][a] = gP[q][b][a] (parent side)
+                // The loads are unconditional, from clamped (always in-bounds) indices, and masked after
+                // the fact, so that they issue back to back instead of one conditional load at a time.
+                T gcv[SM][SM], gpv[SM][SM], elv[SM], tzv[SM];
 #pragma unroll
-            for (int a = 0; a < SM; ++a) {
-                const int si = a < S ? stq(q, a) : -1;
-                if (si < 0) continue;
-                double Gp = 0.0, Qc = 0.0;
-                if (mb) {
-                    Gp = Qc = 1.0;
-                } else {
-                    if (q == 0) Gp = (double)base_value<T>(0, nc, sp.fixed_seg, Kf, P, maxd);
-                    else
+                for (int a = 0; a < SM; ++a) {
+                    const int ac = a < S ? a : 0;
+                    const int si = a < S ? stq(q, a) : -1;
+                    const int sic = si >= 0 ? si : 0;
+                    const T e = elam[fidx(P, NB, sic, nc)], t = trapez[fidx(P, NB, sic, nc)];
+                    elv[a] = si >= 0 ? e : T(0);
+                    tzv[a] = si >= 0 ? t : T(0);
 #pragma unroll
-                        for (int b = 0; b < SM; ++b)
-                            if (b < S && stq(q - 1, b) >= 0) Gp += (double)gpv[b][a];
-                    // leaf: sum gC[K, :, 1, idx_i, leaf] is the base case 1 for every state
-                    if (q == N - 1) Qc = (double)base_value<T>(q, nc, sp.fixed_seg, Kf, P, maxd);
-                    else
-#pragma unroll
-                        for (int b = 0; b < SM; ++b)
-                            if (b < S && stq(q + 1, b) >= 0) Qc += (double)gcv[b][a];
+                    for (int b = 0; b < SM; ++b) {
+                        const int bc = b < S ? b : 0;
+                        const bool okc = si >= 0 && b < S && q < N - 1 && stq(q + 1, b) >= 0;
+                        const bool okp = si >= 0 && b < S && q > 0 && stq(q - 1, b) >= 0;
+                        // a single-bead species has no propagators: no loads (N is wave-uniform)
+                        const T vc = (N > 1 && !mb) ? gC[gidx(q, bc, ac, nc)] : T(0);
+                        const T vp = (N > 1 && !mb) ? gP[gidx(q, bc, ac, nc)] : T(0);
+                        gcv[b][a] = okc ? vc : T(0);
+                        gpv[b][a] = okp ? vp : T(0);
+                    }
                 }
-                const double el = (double)elv[a];
-                const double contr = pref * el * Gp;
-                const double rs = contr * Qc;
-                const double tz = (double)tzv[a];
-                if (valid) {
-                    if (q == sp.fixed_seg) {
-                        if (nc == Kf) {
-                            rb[si * TB] += rs / P.dz / tz;
-                            num += rs;
-                            chg += rs * sys.z[si];
-                        }
+#pragma unroll
+                for (int a = 0; a < SM; ++a) {
+                    const int si = a < S ? stq(q, a) : -1;
+                    if (si < 0) continue;
+                    double Gp = 0.0, Qc = 0.0;
+                    if (mb) {
+                        Gp = Qc = 1.0;
                     } else {
-                        if (hom) racc[a] += rs;
-                        else rb[si * TB] += rs;
-                        num += rs * tz * P.dz;
-                        chg += rs * tz * P.dz * sys.z[si];
-                    }
-                    if (q < N - 1 && !mb) {
+                        if (q == 0) Gp = (double)base_value<T>(0, nc, sp.fixed_seg, Kf, P, maxd);
+                        else
 #pragma unroll
-                        for (int b = 0; b < SM; ++b) {
-                            const int sj = b < S ? stq(q + 1, b) : -1;
-                            if (sj < 0) continue;
-                            if (hom) {
-                                rc0[a][b] += contr * (double)gcv[b][a];
-                                continue;
+                            for (int b = 0; b < SM; ++b)
+                                if (b < S && stq(q - 1, b) >= 0) Gp += (double)gpv[b][a];
+                        // leaf: sum gC[K, :, 1, idx_i, leaf] is the base case 1 for every state
+                        if (q == N - 1) Qc = (double)base_value<T>(q, nc, sp.fixed_seg, Kf, P, maxd);
+                        else
+#pragma unroll
+                            for (int b = 0; b < SM; ++b)
+                                if (b < S && stq(q + 1, b) >= 0) Qc += (double)gcv[b][a];
+                    }
+                    const double el = (double)elv[a];
+                    const double contr = pref * el * Gp;
+                    const double rs = contr * Qc;
+                    const double tz = (double)tzv[a];
+                    if (valid) {
+                        if (q == sp.fixed_seg) {
+                            if (nc == Kf) {
+                                rb[si * TB] += rs / P.dz / tz;
+                                num += rs;
+                                chg += rs * sys.z[si];
                             }
-                            const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
-                            int bond = 0;
-                            for (int u2 = 0; u2 < MB; ++u2)
-                                if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
-                            rbo[(2 * bond + 0) * TB] += contr * (double)gcv[b][a];
+                        } else {
+                            if (hom) racc[a] += rs;
+                            else rb[si * TB] += rs;
+                            num += rs * tz * P.dz;
+                            chg += rs * tz * P.dz * sys.z[si];
                         }
-                    }
-                    if (q > 0 && !mb) {
-                        const double c1 = pref * el * Qc;
+                        if (q < N - 1 && !mb) {
 #pragma unroll
-                        for (int b = 0; b < SM; ++b) {
-                            const int sj = b < S ? stq(q - 1, b) : -1;
-                            if (sj < 0) continue;
-                            if (hom) {
-                                rc1[a][b] += c1 * (double)gpv[b][a];
-                                continue;
+                            for (int b = 0; b < SM; ++b) {
+                                const int sj = b < S ? stq(q + 1, b) : -1;
+                                if (sj < 0) continue;
+                                if (hom) {
+                                    rc0[a][b] += contr * (double)gcv[b][a];
+                                    continue;
+                                }
+                                const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
+                                int bond = 0;
+                                for (int u2 = 0; u2 < MB; ++u2)
+                                    if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
+                                rbo[(2 * bond + 0) * TB] += contr * (double)gcv[b][a];
                             }
-                            const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
-                            int bond = 0;
-                            for (int u2 = 0; u2 < MB; ++u2)
-                                if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
-                            rbo[(2 * bond + 1) * TB] += c1 * (double)gpv[b][a];
+                        }
+                        if (q > 0 && !mb) {
+                            const double c1 = pref * el * Qc;
+#pragma unroll
+                            for (int b = 0; b < SM; ++b) {
+                                const int sj = b < S ? stq(q - 1, b) : -1;
+                                if (sj < 0) continue;
+                                if (hom) {
+                                    rc1[a][b] += c1 * (double)gpv[b][a];
+                                    continue;
+                                }
+                                const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
+                                int bond = 0;
+                                for (int u2 = 0; u2 < MB; ++u2)
+                                    if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
+                                rbo[(2 * bond + 1) * TB] += c1 * (double)gpv[b][a];
+                            }
                         }
                     }
                 }
             }
-        }
-        if (hom && valid) {
-            // segment 0's states and their links stand for every segment (homogeneous species)
+            if (hom && valid) {
+                // segment 0's states and their links stand for every segment (homogeneous species)
 #pragma unroll
-            for (int a = 0; a < SM; ++a) {
-                const int si = a < S ? st[a] : -1;
-                if (si < 0) continue;
-                rb[si * TB] += racc[a];
+                for (int a = 0; a < SM; ++a) {
+                    const int si = a < S ? st[a] : -1;
+                    if (si < 0) continue;
+                    rb[si * TB] += racc[a];
 #pragma unroll
-                for (int b = 0; b < SM; ++b) {
-                    const int sj = b < S ? st[b] : -1;
-                    if (sj < 0 || N < 2 || mb) continue;
-                    const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
-                    int bond = 0;
-                    for (int u2 = 0; u2 < MB; ++u2)
-                        if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
-                    rbo[(2 * bond + 0) * TB] += rc0[a][b];
-                    rbo[(2 * bond + 1) * TB] += rc1[a][b];
+                    for (int b = 0; b < SM; ++b) {
+                        const int sj = b < S ? st[b] : -1;
+                        if (sj < 0 || N < 2 || mb) continue;
+                        const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
+                        int bond = 0;
+                        for (int u2 = 0; u2 < MB; ++u2)
+                            if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
+                        rbo[(2 * bond + 0) * TB] += rc0[a][b];
+                        rbo[(2 * bond + 1) * TB] += rc1[a][b];
+                    }
                 }
             }
         }
         // Num_u = sum num / N_u (chains); charge = sum num * z (not divided): neutrality_shift.jl:34-45
-        num = warp_sum(num / N);
-        chg = warp_sum(chg);
+        // a wave whose lanes all hold +0 (no items of this species, or no charge) skips the shuffle
+        // rounds: the butterfly's sum of zeros is +0 as well
+        num = __ballot(num != 0.0) ? warp_sum(num / N) : 0.0;
+        chg = __ballot(chg != 0.0) ? warp_sum(chg) : 0.0;
         if ((threadIdx.x & 63) == 0) {
             red[2 * u][threadIdx.x >> 6] = num;
             red[2 * u + 1][threadIdx.x >> 6] = chg;
