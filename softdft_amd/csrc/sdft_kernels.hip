@@ -724,28 +724,28 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
         for (int w = 0; w < TB / 64; ++w) s += red[threadIdx.x][w];
         acc_part[((int64_t)p * SDFT_MAX_TILES + blockIdx.x) * 2 * SDFT_MAX_SPECIES + threadIdx.x] = s;
     }
-    if (!valid || grp != 0) return;
-    // densities of this grid point: the segment groups' partials in group order
+    if (!valid) return;
+    // densities of this grid point: the segment groups' partials in group order; the four groups
+    // share the output columns (group g writes the beads, bond ends and classes congruent to g)
     auto total = [&](int f) {
         double v = 0.0;
         for (int g = 0; g < ACC_GROUPS; ++g) v += acc_lds[f * TB + g * ACC_POINTS + nl];
         return v;
     };
-    for (int j = 0; j < NB; ++j) {
+    for (int j = grp; j < NB; j += ACC_GROUPS) {
         const double v = total(j);
         rho_beads[fidx(P, NB, j, n)] = (T)v;
         rhoT[fidx(P, Frt, j, n)] = (T)(v * (double)trapez[fidx(P, NB, j, n)]);
     }
-    for (int u = 0; u < MB; ++u) {
-        for (int e = 0; e < 2; ++e) {
-            const int bt = e == 0 ? sys.bond_i[u] : sys.bond_j[u];
-            const double v = total(NB + 2 * u + e);
-            rho_bonds[fidx(P, 2 * MB, 2 * u + e, n)] = (T)v;
-            rhoT[fidx(P, Frt, NB + 2 * u + e, n)] = (T)(v * (double)trapez[fidx(P, NB, bt, n)]);
-        }
+    for (int f = grp; f < 2 * MB; f += ACC_GROUPS) {
+        const int u = f >> 1, e = f & 1;
+        const int bt = e == 0 ? sys.bond_i[u] : sys.bond_j[u];
+        const double v = total(NB + f);
+        rho_bonds[fidx(P, 2 * MB, f, n)] = (T)v;
+        rhoT[fidx(P, Frt, NB + f, n)] = (T)(v * (double)trapez[fidx(P, NB, bt, n)]);
     }
     // per diameter class: sum of rho_j trapez_j over its beads (input of the class-shared vector weight)
-    for (int c = 0; c < sys.ncls; ++c) {
+    for (int c = grp; c < sys.ncls; c += ACC_GROUPS) {
         double t = 0.0;
         for (int j = 0; j < NB; ++j)
             if (sys.cls[j] == c) t += total(j) * (double)trapez[fidx(P, NB, j, n)];
