@@ -482,20 +482,6 @@ __global__ void __launch_bounds__(PP_THREADS) k_chain_pp(
                               trapez[eb + (int64_t)cc.sj * NP + n]
                         : T(0);
     };
-    // step-1 inputs: X of half j (pp) or the parity-1 buffer (one sweep), and the 16 zeros past XN in
-    // both buffers
-    for (int e = tid; e < 2 * C * (XN + 16); e += PP_THREADS) {
-        const int j = e / (C * (XN + 16)), e2 = e - j * C * (XN + 16);
-        const int col = e2 / (XN + 16), mi = e2 - col * (XN + 16);
-        T* xb = XB + (j * C + col) * XLR;
-        if (mi >= XN) {
-            xb[chain_phys(mi)] = T(0);
-        } else if (pp || j == 1) {
-            const int m = mi - Be;
-            const int n = m < 0 ? -1 - m : (m < NP ? m : 2 * NP - 1 - m);
-            xb[chain_phys(mi)] = weight(cinfo[pp ? j : 0][col], n);
-        }
-    }
     const int slot = pp ? half : 0;
     const ChainCol* ci = cinfo[slot];
     int validmask = 0;
@@ -512,16 +498,59 @@ __global__ void __launch_bounds__(PP_THREADS) k_chain_pp(
     const int a = has_unit ? hw / ntile : 0, tile = has_unit ? hw - a * ntile : 0;
     const int p0 = tile * 16;
     const int n0 = 16 * (p0 + q) + 4 * g;
+    // Each lane's output factors and next-input weights are four consecutive positions of a row;
+    // read per lane from global memory, a wave's reads would touch every cache line of its 256
+    // positions four times. The rows are instead read once, coalesced, into the X buffers: first
+    // the output factor rows e^{lng_i/2} of every column (unpadded, [j][col][n]), then, after the
+    // lanes have taken theirs, the step-1 inputs, which are the weights themselves.
+    for (int e = tid; e < nslot * C * NP; e += PP_THREADS) {
+        const int j = e / (C * NP), e2 = e - j * C * NP;
+        const int col = e2 / NP, n = e2 - col * NP;
+        XB[(j * C + col) * XLR + n] = hl[hb + (int64_t)cinfo[j][col].hl_out * NP + n];
+    }
+    __syncthreads();
     T hr[SDFT_FAST_S][4], wr[SDFT_FAST_S][4];
 #pragma unroll
-    for (int b = 0; b < SDFT_FAST_S; ++b)
+    for (int b = 0; b < SDFT_FAST_S; ++b) {
+        const int col = (a * S + b) % C;
+        const bool vo = ci[col].valid;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const bool in = has_unit && b < S && n0 + v < NP;
-            hr[b][v] = in && ci[(a * S + b) % C].valid ? hl[hb + (int64_t)ci[(a * S + b) % C].hl_out * NP + n0 + v] / (T)norm
-                          : T(0);
-            wr[b][v] = in ? weight(ci[(b * S + a) % C], n0 + v) : T(0);
+            const T h = XB[(slot * C + col) * XLR + (n0 + v < NP ? n0 + v : NP - 1)];
+            hr[b][v] = in && vo ? h / (T)norm : T(0);
         }
+    }
+    __syncthreads();
+    // step-1 inputs: X of half j (pp) or the parity-1 buffer (one sweep), and the 16 zeros past XN in
+    // both buffers
+    for (int e = tid; e < 2 * C * (XN + 16); e += PP_THREADS) {
+        const int j = e / (C * (XN + 16)), e2 = e - j * C * (XN + 16);
+        const int col = e2 / (XN + 16), mi = e2 - col * (XN + 16);
+        T* xb = XB + (j * C + col) * XLR;
+        if (mi >= XN) {
+            xb[chain_phys(mi)] = T(0);
+        } else if (pp || j == 1) {
+            const int m = mi - Be;
+            const int n = m < 0 ? -1 - m : (m < NP ? m : 2 * NP - 1 - m);
+            xb[chain_phys(mi)] = weight(cinfo[pp ? j : 0][col], n);
+        }
+    }
+    __syncthreads();
+    {
+        // the weight of column (b, a) at position n is the step-1 input at extended position n + Be
+        const T* x1 = XB + (pp ? half : 1) * C * XLR;
+#pragma unroll
+        for (int b = 0; b < SDFT_FAST_S; ++b) {
+            const int col = (b * S + a) % C;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const bool in = has_unit && b < S && n0 + v < NP;
+                const T w = x1[col * XLR + chain_phys((n0 + v < NP ? n0 + v : NP - 1) + Be)];
+                wr[b][v] = in ? w : T(0);
+            }
+        }
+    }
     typename Mfma<T>::acc_t acc[SDFT_FAST_S];
     const int nph = pp ? 2 * N - 1 : N - 1;
     for (int ph = 0; ph < nph; ++ph) {
