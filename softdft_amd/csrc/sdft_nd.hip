@@ -19,6 +19,7 @@
 // The per-dimension matrix products are what this file launches; each is memory- and LDS-bound at
 // these sizes (a few to a few hundred points per dimension).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -127,6 +128,101 @@ __global__ void __launch_bounds__(256) k_nd_axis(const T* __restrict__ in, T* __
             const int m = e % n, ll = e / n;
             if (line0 + ll < nlines) dst[(int64_t)(line0 + ll) * n + m] = otile[m * (ND_LINES + 1) + ll];
         }
+    }
+}
+
+// Axes of at most ND_SMAX points: the same transform with the matrix staged in LDS as well. The
+// per-lane reads of wave-uniform matrix entries from global memory wait a cache round trip every
+// few FMAs; from LDS they are broadcast reads. Each wave keeps all its output blocks in registers
+// (ND_SMAX / (4 ND_R) of them), so contiguous lines reuse the input tile as the output tile. Every
+// output sums over the input positions in the same order as k_nd_axis: the results are identical.
+static constexpr int ND_SMAX = 64;
+static constexpr int ND_SB = ND_SMAX / (4 * ND_R);   // output blocks per wave (4 waves)
+
+static __host__ __device__ __forceinline__ int nd_round4(int v) { return (v + 3) & ~3; }
+
+// input (and output) tile [n][ND_LINES + 1], then the matrix AT [n][round4(n)]
+size_t nd_axis_s_lds_bytes(int n, size_t elem) {
+    return (size_t)(nd_round4(n * (ND_LINES + 1)) + n * nd_round4(n)) * elem;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_nd_axis_s(const T* __restrict__ in, T* __restrict__ out, int F, int64_t NPt,
+                                                   int n, int inner, int nlines, const T* __restrict__ AT_even,
+                                                   const T* __restrict__ AT_odd, const int8_t* __restrict__ fpar,
+                                                   int axis, const int32_t* __restrict__ evalflag, int prec) {
+    const int b = blockIdx.y;
+    const int p = b / F, f = b - p * F;
+    if (evalflag[p] != prec) return;
+    const T* __restrict__ AT = fpar[f] == axis ? AT_odd : AT_even;
+    const T* src = in + (int64_t)b * NPt;
+    T* dst = out + (int64_t)b * NPt;
+    extern __shared__ __attribute__((aligned(16))) char nd_lds_s[];
+    T* tile = reinterpret_cast<T*>(nd_lds_s);                      // [n][ND_LINES + 1]
+    const int n4 = nd_round4(n);
+    T* Ms = tile + nd_round4(n * (ND_LINES + 1));                   // [n][n4], zero past n
+    const int line0 = blockIdx.x * ND_LINES;
+    const int total = n * ND_LINES;
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        int j, l;
+        if (inner == 1) { j = e % n; l = e / n; }        // contiguous lines: read along the line
+        else { l = e % ND_LINES; j = e / ND_LINES; }     // strided lines: read across lines
+        const int line = line0 + l;
+        T v = T(0);
+        if (line < nlines) {
+            const int o = line / inner, i = line - o * inner;
+            v = src[((int64_t)o * n + j) * inner + i];
+        }
+        tile[j * (ND_LINES + 1) + l] = v;
+    }
+    for (int e = threadIdx.x; e < n * n4; e += blockDim.x) {
+        const int j = e / n4, m = e - j * n4;
+        Ms[e] = m < n ? AT[(int64_t)j * n + m] : T(0);
+    }
+    __syncthreads();
+    const int l = threadIdx.x % ND_LINES;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / ND_LINES);
+    const int line = line0 + l;
+    const int o = line / inner, i = line - o * inner;
+    T acc[ND_SB][ND_R];
+#pragma unroll
+    for (int k = 0; k < ND_SB; ++k)
+#pragma unroll
+        for (int r = 0; r < ND_R; ++r) acc[k][r] = T(0);
+    for (int j = 0; j < n; ++j) {
+        const T a = tile[j * (ND_LINES + 1) + l];
+        const T* arow = Ms + j * n4;
+#pragma unroll
+        for (int k = 0; k < ND_SB; ++k) {
+            const int m0 = (w + 4 * k) * ND_R;   // wave w: output blocks w, w + 4, ... (as k_nd_axis)
+            if (m0 < n) {
+#pragma unroll
+                for (int r = 0; r < ND_R; ++r) acc[k][r] += arow[m0 + r] * a;
+            }
+        }
+    }
+    if (inner == 1) {
+        __syncthreads();   // every wave has read the input tile: it becomes the output tile
+#pragma unroll
+        for (int k = 0; k < ND_SB; ++k)
+#pragma unroll
+            for (int r = 0; r < ND_R; ++r) {
+                const int m = (w + 4 * k) * ND_R + r;
+                if (m < n) tile[m * (ND_LINES + 1) + l] = acc[k][r];
+            }
+        __syncthreads();
+        for (int e = threadIdx.x; e < total; e += blockDim.x) {
+            const int m = e % n, ll = e / n;
+            if (line0 + ll < nlines) dst[(int64_t)(line0 + ll) * n + m] = tile[m * (ND_LINES + 1) + ll];
+        }
+    } else if (line < nlines) {
+#pragma unroll
+        for (int k = 0; k < ND_SB; ++k)
+#pragma unroll
+            for (int r = 0; r < ND_R; ++r) {
+                const int m = (w + 4 * k) * ND_R + r;
+                if (m < n) dst[((int64_t)o * n + m) * inner + i] = acc[k][r];
+            }
     }
 }
 
@@ -411,9 +507,17 @@ hipError_t nd_prepare_attributes(const Ctx& c) {
     for (int d = 0; d < c.ndp.D; ++d) nmax = std::max(nmax, c.ndp.n[d]);
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_nd_axis<double>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)nd_axis_lds_bytes(nmax, 8));
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_nd_axis<float>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)nd_axis_lds_bytes(nmax, 4));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_nd_axis<float>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)nd_axis_lds_bytes(nmax, 4));
+    const int smax = std::min(nmax, ND_SMAX);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_nd_axis_s<double>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)nd_axis_s_lds_bytes(smax, 8));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_nd_axis_s<float>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)nd_axis_s_lds_bytes(smax, 4));
+    return e;
 }
 
 // transforms the F fields per point of `in` along every axis (forward or inverse), ping-ponging
@@ -432,9 +536,13 @@ static T* nd_transform(hipStream_t st, const Ctx& c, int prec, const T* in, int 
                                                                                : (const void* const*)g.mat32[d]);
         const T* ev = mats[inverse ? 2 : 0];
         const T* od = mats[inverse ? 3 : 1];
-        hipLaunchKernelGGL(k_nd_axis<T>, dim3((nlines + ND_LINES - 1) / ND_LINES, c.P * F), dim3(256),
-                           nd_axis_lds_bytes(n, sizeof(T)), st, src, dst, F, g.NPt, n, inner, nlines, ev, od, fpar, d,
-                           c.d_evalflag, prec);
+        const dim3 grid((nlines + ND_LINES - 1) / ND_LINES, c.P * F);
+        if (n <= ND_SMAX)
+            hipLaunchKernelGGL(k_nd_axis_s<T>, grid, dim3(256), nd_axis_s_lds_bytes(n, sizeof(T)), st, src, dst, F,
+                               g.NPt, n, inner, nlines, ev, od, fpar, d, c.d_evalflag, prec);
+        else
+            hipLaunchKernelGGL(k_nd_axis<T>, grid, dim3(256), nd_axis_lds_bytes(n, sizeof(T)), st, src, dst, F, g.NPt,
+                               n, inner, nlines, ev, od, fpar, d, c.d_evalflag, prec);
         inner *= n;
         src = dst;
         dst = (dst == buf_a) ? buf_b : buf_a;
