@@ -1319,6 +1319,9 @@ extern "C" int32_t sdft_create(const sdft_desc_t* d, sdft_ctx_t** out) {
     if ((rc = upload(ctx, &c.d_spec, spec.data(), nsp)) != SDFT_OK) return fail(rc);
     if ((rc = upload(ctx, &c.d_goffs, goffs.data(), nsp)) != SDFT_OK) return fail(rc);
     if ((rc = upload(ctx, &c.d_seg_states, d->seg_states, state_off)) != SDFT_OK) return fail(rc);
+    c.h_seg_states.assign(d->seg_states, d->seg_states + state_off);
+    c.h_bond_i.assign(sys.bond_i, sys.bond_i + c.MB);
+    c.h_bond_j.assign(sys.bond_j, sys.bond_j + c.MB);
     std::vector<double> maxd(maxd_off, 0.0);
     if (d->max_distance) maxd.assign(d->max_distance, d->max_distance + maxd_off);
     if ((rc = upload(ctx, &c.d_maxd, maxd.data(), maxd_off)) != SDFT_OK) return fail(rc);

@@ -187,6 +187,8 @@ struct Ctx {
     DevSpecies* d_spec = nullptr;
     int64_t* d_goffs = nullptr;
     int32_t* d_seg_states = nullptr;
+    std::vector<int32_t> h_seg_states;            // host copies of the state and bond tables
+    std::vector<int32_t> h_bond_i, h_bond_j;
     double* d_maxd = nullptr;
     ConvJob* d_jobs = nullptr;
     double* d_wsum = nullptr;       // (sumNP * Fws) shared back-projection partials

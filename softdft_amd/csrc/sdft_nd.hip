@@ -283,6 +283,50 @@ __device__ __forceinline__ void nd_pairs(const DevSystem& sys, const int* st, in
     }
 }
 
+// The pairs of both sweeps of one step, built on the host from the same tables (nd_step_pairs) and
+// passed by value: the elementwise step kernels then read them from the kernel arguments instead of
+// searching the state and bond tables in global memory in every thread.
+struct NdStep {
+    NdPairs q[2];
+    int32_t si[2][SDFT_MAX_S * SDFT_MAX_S], sj[2][SDFT_MAX_S * SDFT_MAX_S];   // bead types of pair k
+    int32_t slot_bond[2][SDFT_MAX_S * SDFT_MAX_S];   // bond of slot a S + b, -1: no pair
+    int32_t in_valid[2];                             // bit b2: input state b2 exists
+};
+
+static void nd_step_pairs(const Ctx& c, const SpeciesHost& sh, int s, NdStep& ns) {
+    const int N = sh.nseg, S = sh.S;
+    const int* st = c.h_seg_states.data() + sh.state_off;
+    for (int sweep = 0; sweep < 2; ++sweep) {
+        const int outseg = sweep == 0 ? N - 1 - s : s;
+        const int inseg = sweep == 0 ? outseg + 1 : outseg - 1;
+        NdPairs& q = ns.q[sweep];
+        q.n = 0;
+        ns.in_valid[sweep] = 0;
+        for (int k = 0; k < SDFT_MAX_S * SDFT_MAX_S; ++k) ns.slot_bond[sweep][k] = -1;
+        for (int b = 0; b < S; ++b)
+            if (st[inseg * S + b] >= 0) ns.in_valid[sweep] |= 1 << b;
+        for (int a = 0; a < S; ++a) {
+            const int si = st[outseg * S + a];
+            if (si < 0) continue;
+            for (int b = 0; b < S; ++b) {
+                const int sj = st[inseg * S + b];
+                if (sj < 0) continue;
+                const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
+                int bond = 0;
+                for (int u2 = 0; u2 < c.MB; ++u2)
+                    if (c.h_bond_i[u2] == lo && c.h_bond_j[u2] == hi) bond = u2;   // get_bond_info
+                q.a[q.n] = a; q.b[q.n] = b; q.bond[q.n] = bond;
+                q.pi[q.n] = si <= sj ? 0 : 1;
+                q.pj[q.n] = si <= sj ? 1 : 0;
+                ns.si[sweep][q.n] = si;
+                ns.sj[sweep][q.n] = sj;
+                ns.slot_bond[sweep][a * S + b] = bond;
+                ++q.n;
+            }
+        }
+    }
+}
+
 // propagator step input f = e^{-lambda_j} e^{lng_j / 2} Q_in trapez_j for every (sweep, a, b) slot
 // (slot = a * S + b; unused slots 0): calc_f_hat_child! / calc_f_hat_parent! before the FFT
 template <typename T>
@@ -291,7 +335,8 @@ __global__ void __launch_bounds__(256) k_nd_prop_in(const DevSystem* __restrict_
                                                     const int32_t* __restrict__ seg_states,
                                                     const double* __restrict__ maxd_tab, const T* __restrict__ elam,
                                                     const T* __restrict__ hl, const T* __restrict__ trapez,
-                                                    const T* __restrict__ gC, const T* __restrict__ gP, T* __restrict__ W) {
+                                                    const T* __restrict__ gC, const T* __restrict__ gP, T* __restrict__ W,
+                                                    NdStep ns) {
     const DevSystem& sys = *psys;
     const int p = blockIdx.z;
     if (evalflag[p] != prec) return;
@@ -302,20 +347,18 @@ __global__ void __launch_bounds__(256) k_nd_prop_in(const DevSystem* __restrict_
     const int N = A.nseg, S = A.S, NB = sys.NB, MB = sys.MB;
     const int outseg = sweep == 0 ? N - 1 - A.s : A.s;
     const int inseg = sweep == 0 ? outseg + 1 : outseg - 1;
-    const int* st = seg_states + A.state_off;
     const T* gin = sweep == 0 ? gC : gP;
     const bool in_stored = sweep == 0 ? (inseg < N - 1) : (inseg > 0);
     const T qinv = (A.rescale && in_stored)
                        ? (T)(1.0 / scale_read(A.smax, scale_slot(p, A.u, sweep, A.s - 1, A.nsp, A.nmax)))
                        : T(1);
-    NdPairs q;
-    nd_pairs(sys, st, S, outseg, inseg, q);
+    const NdPairs& q = ns.q[sweep];
     const int S2 = S * S;
     T* w = W + ((int64_t)p * 2 * S2 + (int64_t)sweep * S2) * P.NP + n;
     for (int k = 0; k < S2; ++k) w[(int64_t)k * P.NP] = T(0);
     for (int k = 0; k < q.n; ++k) {
         const int b = q.b[k];
-        const int sj = st[inseg * S + b];
+        const int sj = ns.sj[sweep][k];
         T qv;
         if (!in_stored) {
             qv = base_value<T>(inseg, n, A.fixed_seg, A.fixed_seg >= 0 ? P.fixed_coord[A.u] : -1, P,
@@ -323,7 +366,7 @@ __global__ void __launch_bounds__(256) k_nd_prop_in(const DevSystem* __restrict_
         } else {
             qv = T(0);
             for (int b2 = 0; b2 < S; ++b2)
-                if (st[inseg * S + b2] >= 0)
+                if ((ns.in_valid[sweep] >> b2) & 1)
                     qv += gin[P.gbase + (A.goff + ((int64_t)inseg * S + b2) * S + b) * P.gstride + n];
             qv *= qinv;
         }
@@ -336,26 +379,17 @@ __global__ void __launch_bounds__(256) k_nd_prop_in(const DevSystem* __restrict_
 template <typename T>
 __global__ void __launch_bounds__(256) k_nd_prop_mul(const DevSystem* __restrict__ psys, const int32_t* __restrict__ evalflag,
                                                      int prec, PropArgs A, const int32_t* __restrict__ seg_states,
-                                                     const T* __restrict__ kspec, int kbond0, int64_t NPt, T* __restrict__ W) {
-    const DevSystem& sys = *psys;
+                                                     const T* __restrict__ kspec, int kbond0, int64_t NPt, T* __restrict__ W,
+                                                     NdStep ns) {
     const int p = blockIdx.z;
     if (evalflag[p] != prec) return;
     const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= NPt) return;
     const int S = A.S, S2 = S * S;
     const int sweep = blockIdx.y / S2, slot = blockIdx.y - sweep * S2;
-    const int a = slot / S, b = slot - a * S;
-    const int N = A.nseg;
-    const int outseg = sweep == 0 ? N - 1 - A.s : A.s;
-    const int inseg = sweep == 0 ? outseg + 1 : outseg - 1;
-    const int* st = seg_states + A.state_off;
-    const int si = st[outseg * S + a], sj = st[inseg * S + b];
     T* w = W + ((int64_t)p * 2 * S2 + blockIdx.y) * NPt + m;
-    if (si < 0 || sj < 0) { *w = T(0); return; }
-    const int lo = si <= sj ? si : sj, hi = si <= sj ? sj : si;
-    int bond = 0;
-    for (int u2 = 0; u2 < sys.MB; ++u2)
-        if (sys.bond_i[u2] == lo && sys.bond_j[u2] == hi) bond = u2;
+    const int bond = ns.slot_bond[sweep][slot];
+    if (bond < 0) { *w = T(0); return; }
     *w *= kspec[(int64_t)(kbond0 + bond) * NPt + m];
 }
 
@@ -365,7 +399,8 @@ __global__ void __launch_bounds__(256) k_nd_prop_out(const DevSystem* __restrict
                                                      const int32_t* __restrict__ evalflag, int prec, PropArgs A,
                                                      const int32_t* __restrict__ seg_states,
                                                      const double* __restrict__ maxd_tab, const T* __restrict__ hl,
-                                                     const T* __restrict__ R, T* __restrict__ gC, T* __restrict__ gP) {
+                                                     const T* __restrict__ R, T* __restrict__ gC, T* __restrict__ gP,
+                                                     NdStep ns) {
     const DevSystem& sys = *psys;
     const int p = blockIdx.z;
     if (evalflag[p] != prec) return;
@@ -375,12 +410,10 @@ __global__ void __launch_bounds__(256) k_nd_prop_out(const DevSystem* __restrict
     const int N = A.nseg, S = A.S, MB = sys.MB;
     const int outseg = sweep == 0 ? N - 1 - A.s : A.s;
     const int inseg = sweep == 0 ? outseg + 1 : outseg - 1;
-    const int* st = seg_states + A.state_off;
     T* gout = sweep == 0 ? gC : gP;
     double vmax = 0.0;
     if (n < P.NP) {
-        NdPairs q;
-        nd_pairs(sys, st, S, outseg, inseg, q);
+        const NdPairs& q = ns.q[sweep];
         const int S2 = S * S;
         const T* r = R + ((int64_t)p * 2 * S2 + (int64_t)sweep * S2) * P.NP + n;
         const double normative = P.normative[A.u];
@@ -562,20 +595,22 @@ void Pipeline<T>::propagate_nd(hipStream_t st, const Ctx& c, int prec, int u) {
     const int S2 = sh.S * sh.S;
     const T* kspec = reinterpret_cast<const T*>(sizeof(T) == 8 ? (const void*)g.kspec64 : (const void*)g.kspec32);
     const unsigned gx = (unsigned)((g.NPt + 255) / 256);
+    NdStep ns;
     for (int s = 1; s < sh.nseg; ++s) {
         A.s = s;
+        nd_step_pairs(c, sh, s, ns);
         // sweeps whose parent is the grafted segment 0 take the direct sums; they are computed after
         // the spectral path, which also writes (and then has overwritten) their outputs
         const bool fix0 = sh.fixed_seg == 0 && s == sh.nseg - 1;   // child sweep, parent = segment 0
         const bool fix1 = sh.fixed_seg == 0 && s == 1;             // parent sweep, parent = segment 0
         hipLaunchKernelGGL(k_nd_prop_in<T>, dim3(gx, 2, c.P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag, prec, A,
-                           c.d_seg_states, c.d_maxd, elam, hl, trapez, gC, gP, nd_a);
+                           c.d_seg_states, c.d_maxd, elam, hl, trapez, gC, gP, nd_a, ns);
         T* S = nd_transform<T>(st, c, prec, nd_a, 2 * S2, g.d_par_even, false, nd_a, nd_b);
         hipLaunchKernelGGL(k_nd_prop_mul<T>, dim3(gx, 2 * S2, c.P), dim3(256), 0, st, c.d_sys, c.d_evalflag, prec, A,
-                           c.d_seg_states, kspec, g.kbond0, g.NPt, S);
+                           c.d_seg_states, kspec, g.kbond0, g.NPt, S, ns);
         T* R = nd_transform<T>(st, c, prec, S, 2 * S2, g.d_par_even, true, nd_a, nd_b);
         hipLaunchKernelGGL(k_nd_prop_out<T>, dim3(gx, 2, c.P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag, prec,
-                           A, c.d_seg_states, c.d_maxd, hl, R, gC, gP);
+                           A, c.d_seg_states, c.d_maxd, hl, R, gC, gP, ns);
         if (fix0)
             hipLaunchKernelGGL(k_nd_prop_graft<T>, dim3(gx, 1, c.P), dim3(256), 0, st, c.d_sys, c.d_pts, c.d_evalflag,
                                prec, A, 0, c.d_seg_states, c.d_maxd, elam, hl, trapez, gC, gP);
