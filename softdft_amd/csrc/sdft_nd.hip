@@ -146,6 +146,31 @@ size_t nd_axis_s_lds_bytes(int n, size_t elem) {
     return (size_t)(nd_round4(n * (ND_LINES + 1)) + n * nd_round4(n)) * elem;
 }
 
+// The sums of one wave over its NBK output blocks (wave-uniform count, so the loop body has no
+// branches and the compiler can overlap the next position's LDS reads with this one's FMAs).
+template <typename T, int NBK>
+__device__ __forceinline__ void nd_axis_sums(const T* tile, const T* Ms, int n, int n4, int l, int w,
+                                             T (&acc)[ND_SB][ND_R]) {
+#pragma unroll 2
+    for (int j = 0; j < n; ++j) {
+        const T a = tile[j * (ND_LINES + 1) + l];
+        const T* arow = Ms + j * n4;
+#pragma unroll
+        for (int k = 0; k < NBK; ++k) {
+            const int m0 = (w + 4 * k) * ND_R;   // wave w: output blocks w, w + 4, ... (as k_nd_axis)
+#pragma unroll
+            for (int r = 0; r < ND_R; ++r) acc[k][r] += arow[m0 + r] * a;
+        }
+    }
+}
+
+// e / d for 0 <= e < 4096 and 1 <= d <= ND_SMAX: multiply by ceil(2^20 / d) and shift (exact over
+// that whole range, checked exhaustively), instead of a 32-bit integer division
+__device__ __forceinline__ int nd_div(int e, uint32_t mul) { return (int)(((uint32_t)e * mul) >> 20); }
+__host__ __device__ __forceinline__ uint32_t nd_divmul(int d) { return (1u << 20) / (uint32_t)d + 1u; }
+
+static constexpr int ND_LB = 8;   // global loads a thread issues before it writes them to LDS
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_nd_axis_s(const T* __restrict__ in, T* __restrict__ out, int F, int64_t NPt,
                                                    int n, int inner, int nlines, const T* __restrict__ AT_even,
@@ -153,8 +178,10 @@ __global__ void __launch_bounds__(256) k_nd_axis_s(const T* __restrict__ in, T* 
                                                    int axis, const int32_t* __restrict__ evalflag, int prec) {
     const int b = blockIdx.y;
     const int p = b / F, f = b - p * F;
-    if (evalflag[p] != prec) return;
-    const T* __restrict__ AT = fpar[f] == axis ? AT_odd : AT_even;
+    const int ef = evalflag[p];
+    const int par = fpar[f];
+    if (ef != prec) return;
+    const T* __restrict__ AT = par == axis ? AT_odd : AT_even;
     const T* src = in + (int64_t)b * NPt;
     T* dst = out + (int64_t)b * NPt;
     extern __shared__ __attribute__((aligned(16))) char nd_lds_s[];
@@ -163,21 +190,64 @@ __global__ void __launch_bounds__(256) k_nd_axis_s(const T* __restrict__ in, T* 
     T* Ms = tile + nd_round4(n * (ND_LINES + 1));                   // [n][n4], zero past n
     const int line0 = blockIdx.x * ND_LINES;
     const int total = n * ND_LINES;
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        int j, l;
-        if (inner == 1) { j = e % n; l = e / n; }        // contiguous lines: read along the line
-        else { l = e % ND_LINES; j = e / ND_LINES; }     // strided lines: read across lines
-        const int line = line0 + l;
-        T v = T(0);
-        if (line < nlines) {
-            const int o = line / inner, i = line - o * inner;
-            v = src[((int64_t)o * n + j) * inner + i];
+    // input tile and matrix: ND_LB loads in flight per thread (addresses clamped into the field,
+    // values selected after), then the LDS writes
+    const uint32_t mul_n = nd_divmul(n), mul_n4 = nd_divmul(n4);
+    if (inner == 1) {   // contiguous lines: the tile is one run of memory
+        const int64_t base = (int64_t)line0 * n, lim = (int64_t)nlines * n;
+        for (int e0 = threadIdx.x; e0 < total; e0 += ND_LB * 256) {
+            T v[ND_LB];
+#pragma unroll
+            for (int u = 0; u < ND_LB; ++u) {
+                const int64_t g = base + e0 + u * 256;
+                const T x = src[g < lim ? g : lim - 1];
+                v[u] = g < lim ? x : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < ND_LB; ++u) {
+                const int e = e0 + u * 256;
+                if (e < total) {
+                    const int l = nd_div(e, mul_n), jj = e - l * n;
+                    tile[jj * (ND_LINES + 1) + l] = v[u];
+                }
+            }
         }
-        tile[j * (ND_LINES + 1) + l] = v;
+    } else {   // strided lines: each thread keeps its line, positions 4 apart
+        const int l = threadIdx.x % ND_LINES, j0 = threadIdx.x / ND_LINES;
+        const int line = line0 + l;
+        const int lc = line < nlines ? line : nlines - 1;
+        const int o = lc / inner, i = lc - o * inner;
+        const T* sl = src + (int64_t)o * n * inner + i;
+        for (int jb = j0; jb < n; jb += ND_LB * 4) {
+            T v[ND_LB];
+#pragma unroll
+            for (int u = 0; u < ND_LB; ++u) {
+                const int jj = jb + 4 * u;
+                const T x = sl[(int64_t)(jj < n ? jj : n - 1) * inner];
+                v[u] = line < nlines ? x : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < ND_LB; ++u) {
+                const int jj = jb + 4 * u;
+                if (jj < n) tile[jj * (ND_LINES + 1) + l] = v[u];
+            }
+        }
     }
-    for (int e = threadIdx.x; e < n * n4; e += blockDim.x) {
-        const int j = e / n4, m = e - j * n4;
-        Ms[e] = m < n ? AT[(int64_t)j * n + m] : T(0);
+    for (int e0 = threadIdx.x; e0 < n * n4; e0 += ND_LB * 256) {
+        T v[ND_LB];
+#pragma unroll
+        for (int u = 0; u < ND_LB; ++u) {
+            const int e = e0 + u * 256;
+            const int ec = e < n * n4 ? e : n * n4 - 1;
+            const int jj = nd_div(ec, mul_n4), m = ec - jj * n4;
+            const T x = AT[jj * n + (m < n ? m : n - 1)];
+            v[u] = m < n ? x : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < ND_LB; ++u) {
+            const int e = e0 + u * 256;
+            if (e < n * n4) Ms[e] = v[u];
+        }
     }
     __syncthreads();
     const int l = threadIdx.x % ND_LINES;
@@ -189,17 +259,15 @@ __global__ void __launch_bounds__(256) k_nd_axis_s(const T* __restrict__ in, T* 
     for (int k = 0; k < ND_SB; ++k)
 #pragma unroll
         for (int r = 0; r < ND_R; ++r) acc[k][r] = T(0);
-    for (int j = 0; j < n; ++j) {
-        const T a = tile[j * (ND_LINES + 1) + l];
-        const T* arow = Ms + j * n4;
+    int nbk = 0;   // this wave's output blocks that start below n
 #pragma unroll
-        for (int k = 0; k < ND_SB; ++k) {
-            const int m0 = (w + 4 * k) * ND_R;   // wave w: output blocks w, w + 4, ... (as k_nd_axis)
-            if (m0 < n) {
-#pragma unroll
-                for (int r = 0; r < ND_R; ++r) acc[k][r] += arow[m0 + r] * a;
-            }
-        }
+    for (int k = 0; k < ND_SB; ++k) nbk = (w + 4 * k) * ND_R < n ? k + 1 : nbk;
+    switch (nbk) {
+        case 1: nd_axis_sums<T, 1>(tile, Ms, n, n4, l, w, acc); break;
+        case 2: nd_axis_sums<T, 2>(tile, Ms, n, n4, l, w, acc); break;
+        case 3: nd_axis_sums<T, 3>(tile, Ms, n, n4, l, w, acc); break;
+        case 4: nd_axis_sums<T, 4>(tile, Ms, n, n4, l, w, acc); break;
+        default: break;
     }
     if (inner == 1) {
         __syncthreads();   // every wave has read the input tile: it becomes the output tile
@@ -211,9 +279,21 @@ __global__ void __launch_bounds__(256) k_nd_axis_s(const T* __restrict__ in, T* 
                 if (m < n) tile[m * (ND_LINES + 1) + l] = acc[k][r];
             }
         __syncthreads();
-        for (int e = threadIdx.x; e < total; e += blockDim.x) {
-            const int m = e % n, ll = e / n;
-            if (line0 + ll < nlines) dst[(int64_t)(line0 + ll) * n + m] = tile[m * (ND_LINES + 1) + ll];
+        const int64_t base = (int64_t)line0 * n, lim = (int64_t)nlines * n;
+        for (int e0 = threadIdx.x; e0 < total; e0 += ND_LB * 256) {
+            T v[ND_LB];
+#pragma unroll
+            for (int u = 0; u < ND_LB; ++u) {
+                const int e = e0 + u * 256;
+                const int ec = e < total ? e : total - 1;
+                const int ll = nd_div(ec, mul_n), m = ec - ll * n;
+                v[u] = tile[m * (ND_LINES + 1) + ll];
+            }
+#pragma unroll
+            for (int u = 0; u < ND_LB; ++u) {
+                const int64_t g = base + e0 + u * 256;
+                if (e0 + u * 256 < total && g < lim) dst[g] = v[u];
+            }
         }
     } else if (line < nlines) {
 #pragma unroll
