@@ -494,33 +494,51 @@ __global__ void __launch_bounds__(TB, sizeof(T) == 4 ? 1 : 4) k_accumulate(const
                                                    const double* __restrict__ fixed_lambda,
                                                    T* __restrict__ rho_beads, T* __restrict__ rho_bonds, T* __restrict__ rhoT,
                                                    int Frt, double* __restrict__ acc_part, int32_t* __restrict__ errflag,
-                                                   const double* __restrict__ lsc, int nmax) {
+                                                   const double* __restrict__ lsc, int nmax, int nsp) {
     // A block covers ACC_POINTS grid points; its ACC_GROUPS wavefronts split the segments of every
     // species, dealt round-robin continuing from one species to the next (so single-bead species do
     // not all fall to group 0), which keeps more independent loads in flight per point; the groups'
     // partial densities are added in group order at the end.
-    // The point's, the system's and the species' descriptors are copied to LDS first: read through
-    // the scalar cache, their many fields (a different point in every block) cost a chain of
-    // dependent global round trips per species and segment.
+    // The point's, the system's and the species' descriptors (nsp of them) are copied to LDS first:
+    // read through the scalar cache, their many fields (a different point in every block) cost a
+    // chain of dependent global round trips per species and segment. The words are read in one
+    // pass, issued together with the point's evaluation flag, and written to LDS after its test.
     const int p = blockIdx.y;
-    if (evalflag[p] != prec) return;
+    const int ef = evalflag[p];
     __shared__ DevPoint Ps;
     __shared__ DevSystem Ss;
     __shared__ DevSpecies Us[SDFT_MAX_SPECIES];
     {
         static_assert(sizeof(DevPoint) % 4 == 0 && sizeof(DevSystem) % 4 == 0 && sizeof(DevSpecies) % 4 == 0,
                       "descriptor copies move 4-byte words");
+        constexpr int WP = (int)(sizeof(DevPoint) / 4), WS = (int)(sizeof(DevSystem) / 4);
+        constexpr int WU = (int)(sizeof(DevSpecies) / 4), WR = 3;
+        static_assert(WP + WS + SDFT_MAX_SPECIES * WU <= WR * TB, "descriptor words per thread");
+        const int wtot = WP + WS + nsp * WU;
         const int32_t* sp_src = reinterpret_cast<const int32_t*>(pts + p);
-        int32_t* sp_dst = reinterpret_cast<int32_t*>(&Ps);
-        for (int i = threadIdx.x; i < (int)(sizeof(DevPoint) / 4); i += TB) sp_dst[i] = sp_src[i];
         const int32_t* ss_src = reinterpret_cast<const int32_t*>(psys);
-        int32_t* ss_dst = reinterpret_cast<int32_t*>(&Ss);
-        for (int i = threadIdx.x; i < (int)(sizeof(DevSystem) / 4); i += TB) ss_dst[i] = ss_src[i];
-        __syncthreads();
-        const int nsp_words = Ss.n_species * (int)(sizeof(DevSpecies) / 4);
         const int32_t* su_src = reinterpret_cast<const int32_t*>(spec);
+        int32_t wv[WR];
+#pragma unroll
+        for (int r = 0; r < WR; ++r) {
+            const int i = threadIdx.x + r * TB;
+            int32_t v = 0;
+            if (i < WP) v = sp_src[i];
+            else if (i < WP + WS) v = ss_src[i - WP];
+            else if (i < wtot) v = su_src[i - WP - WS];
+            wv[r] = v;
+        }
+        if (ef != prec) return;
+        int32_t* sp_dst = reinterpret_cast<int32_t*>(&Ps);
+        int32_t* ss_dst = reinterpret_cast<int32_t*>(&Ss);
         int32_t* su_dst = reinterpret_cast<int32_t*>(Us);
-        for (int i = threadIdx.x; i < nsp_words; i += TB) su_dst[i] = su_src[i];
+#pragma unroll
+        for (int r = 0; r < WR; ++r) {
+            const int i = threadIdx.x + r * TB;
+            if (i < WP) sp_dst[i] = wv[r];
+            else if (i < WP + WS) ss_dst[i - WP] = wv[r];
+            else if (i < wtot) su_dst[i - WP - WS] = wv[r];
+        }
         __syncthreads();
     }
     const DevSystem& sys = Ss;
@@ -2000,12 +2018,12 @@ void Pipeline<T>::launch_accumulate(hipStream_t st, const Ctx& c, int prec) {
         hipLaunchKernelGGL((k_accumulate<T, 256, SDFT_FAST_S>), grid2(c.maxNP, ACC_POINTS, c.P), dim3(256),
                            accumulate_lds_bytes(c), st, c.d_sys, c.d_pts, c.d_evalflag, prec, c.d_spec, c.d_goffs, c.GF,
                            c.d_seg_states, c.d_maxd, elam, trapez, gC, gP, c.d_fixed_lambda, rho_beads, rho_bonds, rhoT,
-                           c.Frt, c.d_acc_part, c.d_errflag, lsc, c.nmax_seg);
+                           c.Frt, c.d_acc_part, c.d_errflag, lsc, c.nmax_seg, c.nsp);
     else
         hipLaunchKernelGGL((k_accumulate<T, 256, SDFT_MAX_S>), grid2(c.maxNP, ACC_POINTS, c.P), dim3(256),
                            accumulate_lds_bytes(c), st, c.d_sys, c.d_pts, c.d_evalflag, prec, c.d_spec, c.d_goffs, c.GF,
                            c.d_seg_states, c.d_maxd, elam, trapez, gC, gP, c.d_fixed_lambda, rho_beads, rho_bonds, rhoT,
-                           c.Frt, c.d_acc_part, c.d_errflag, lsc, c.nmax_seg);
+                           c.Frt, c.d_acc_part, c.d_errflag, lsc, c.nmax_seg, c.nsp);
 }
 
 template <typename T>
